@@ -1,0 +1,487 @@
+// gfx950 kernels for imageproc's filter module:
+//   separable_filter / gaussian_blur_f32   (filter/mod.rs:147-195, 228-429)
+//   filter_clamped                         (filter/mod.rs:24-69, 197-226)
+//   box_filter                             (filter/box_filter.rs:19-70)
+//
+// Parity rules kept by every kernel here:
+//   * the sum for one output is one sequential chain `acc = acc + p * w`
+//     over taps in ascending order, each step a separately rounded f32 mul
+//     and add (__fmul_rn/__fadd_rn; the file is also built with
+//     -ffp-contract=off), or wrapping i32 arithmetic for integer kernels;
+//   * out-of-image taps read the nearest edge pixel (clamp-to-edge);
+//   * between the horizontal and the vertical pass the intermediate is
+//     stored in the pixel type, i.e. truncated and clamped to u8 for
+//     GrayImage, exactly as the reference's Image<P> intermediate.
+#include "iph_internal.hpp"
+
+#include <algorithm>
+#include <vector>
+
+namespace iph {
+namespace {
+
+// ---- element conversions and Clamp<K> casts (definitions.rs:114-190) ----
+
+__device__ __forceinline__ float to_acc(uint8_t v, float) { return (float)v; }
+__device__ __forceinline__ float to_acc(float v, float) { return v; }
+__device__ __forceinline__ int32_t to_acc(uint8_t v, int32_t) { return (int32_t)v; }
+
+// accumulate, filter/mod.rs:407-416: acc + c * w, no FMA contraction.
+__device__ __forceinline__ float mac(float acc, float c, float w) {
+    return __fadd_rn(acc, __fmul_rn(c, w));
+}
+// Integer kernels: i32 arithmetic that wraps like a release build.
+__device__ __forceinline__ int32_t mac(int32_t acc, int32_t c, int32_t w) {
+    return (int32_t)((uint32_t)acc + (uint32_t)c * (uint32_t)w);
+}
+
+template <typename S> struct ClampTo;
+template <> struct ClampTo<uint8_t> {
+    // x < 255 ? (x > 0 ? x as u8 : 0) : 255; NaN -> 255.
+    __device__ static uint8_t from(float x) {
+        return x < 255.0f ? (x > 0.0f ? (uint8_t)(uint32_t)x : (uint8_t)0) : (uint8_t)255;
+    }
+    __device__ static uint8_t from(int32_t x) {
+        return x < 255 ? (x > 0 ? (uint8_t)x : (uint8_t)0) : (uint8_t)255;
+    }
+};
+template <> struct ClampTo<uint16_t> {
+    __device__ static uint16_t from(float x) {
+        return x < 65535.0f ? (x > 0.0f ? (uint16_t)(uint32_t)x : (uint16_t)0) : (uint16_t)65535;
+    }
+    __device__ static uint16_t from(int32_t x) {
+        return x < 65535 ? (x > 0 ? (uint16_t)x : (uint16_t)0) : (uint16_t)65535;
+    }
+};
+template <> struct ClampTo<int16_t> {
+    __device__ static int16_t from(int32_t x) {
+        return x < 32767 ? (x > -32768 ? (int16_t)x : (int16_t)-32768) : (int16_t)32767;
+    }
+};
+template <> struct ClampTo<int32_t> {
+    __device__ static int32_t from(int32_t x) { return x; }
+};
+template <> struct ClampTo<float> {
+    __device__ static float from(float x) { return x; }
+};
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+
+// ---- fused separable filter ------------------------------------------------
+//
+// One workgroup produces a TW x TH output tile of one image:
+//   1. load the (TH+L-1) x (TW+L-1) input window, clamped to the image, into LDS;
+//   2. horizontal pass over every window row -> P-typed intermediate in LDS;
+//   3. vertical pass over the intermediate -> output.
+// Window row j holds image row clamp(y0 - L/2 + j), so the horizontal pass of
+// a replicated edge row equals the replicated horizontal result, and the
+// vertical taps clamp exactly like vertical_filter's.
+
+constexpr int kFusedMaxTaps = 48;
+constexpr int kSepTW = 64;
+constexpr int kSepTH = 32;
+
+template <typename K> struct FusedTaps {
+    K h[kFusedMaxTaps];
+    K v[kFusedMaxTaps];
+};
+
+template <typename P, typename K>
+__global__ __launch_bounds__(256) void sep_fused_kernel(const P *__restrict__ src,
+                                                        P *__restrict__ dst, int W, int H,
+                                                        size_t stride, int L,
+                                                        FusedTaps<K> taps) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int R = kSepTH + L - 1;
+    const int C = kSepTW + L - 1;
+    P *in_s = reinterpret_cast<P *>(smem_raw);
+    P *mid_s = in_s + R * C;
+
+    const int half = L / 2;
+    const int x0 = blockIdx.x * kSepTW;
+    const int y0 = blockIdx.y * kSepTH;
+    const P *img = src + (size_t)blockIdx.z * stride;
+    P *out = dst + (size_t)blockIdx.z * stride;
+
+    for (int i = threadIdx.x; i < R * C; i += blockDim.x) {
+        int r = i / C, c = i - r * C;
+        int gy = clampi(y0 - half + r, 0, H - 1);
+        int gx = clampi(x0 - half + c, 0, W - 1);
+        in_s[i] = img[(size_t)gy * W + gx];
+    }
+    __syncthreads();
+
+    for (int i = threadIdx.x; i < R * kSepTW; i += blockDim.x) {
+        int r = i / kSepTW, c = i - r * kSepTW;
+        const P *row = in_s + r * C + c;
+        K acc = K(0);
+        for (int t = 0; t < L; t++) acc = mac(acc, to_acc(row[t], K(0)), taps.h[t]);
+        mid_s[i] = ClampTo<P>::from(acc);
+    }
+    __syncthreads();
+
+    for (int i = threadIdx.x; i < kSepTH * kSepTW; i += blockDim.x) {
+        int r = i / kSepTW, c = i - r * kSepTW;
+        int gy = y0 + r, gx = x0 + c;
+        if (gy >= H || gx >= W) continue;
+        const P *col = mid_s + r * kSepTW + c;
+        K acc = K(0);
+        for (int t = 0; t < L; t++) acc = mac(acc, to_acc(col[t * kSepTW], K(0)), taps.v[t]);
+        out[(size_t)gy * W + gx] = ClampTo<P>::from(acc);
+    }
+}
+
+// ---- one 1-D pass, any tap count (weights in device memory) ----------------
+
+template <typename P, typename K, bool VERTICAL>
+__global__ __launch_bounds__(256) void filter1d_kernel(const P *__restrict__ src,
+                                                       P *__restrict__ dst, int W, int H,
+                                                       size_t stride, const K *__restrict__ k,
+                                                       int L) {
+    int x = blockIdx.x * 64 + threadIdx.x;
+    int y = blockIdx.y * 4 + threadIdx.y;
+    if (x >= W || y >= H) return;
+    const P *img = src + (size_t)blockIdx.z * stride;
+    const int half = L / 2;
+    K acc = K(0);
+    if (VERTICAL) {
+        for (int t = 0; t < L; t++) {
+            int yy = clampi(y + t - half, 0, H - 1);
+            acc = mac(acc, to_acc(img[(size_t)yy * W + x], K(0)), k[t]);
+        }
+    } else {
+        const P *row = img + (size_t)y * W;
+        for (int t = 0; t < L; t++) {
+            int xx = clampi(x + t - half, 0, W - 1);
+            acc = mac(acc, to_acc(row[xx], K(0)), k[t]);
+        }
+    }
+    dst[(size_t)blockIdx.z * stride + (size_t)y * W + x] = ClampTo<P>::from(acc);
+}
+
+template <typename P, typename K>
+__global__ void fill_kernel(P *dst, size_t count, K value) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t step = (size_t)gridDim.x * blockDim.x;
+    P v = ClampTo<P>::from(value);
+    for (; i < count; i += step) dst[i] = v;
+}
+
+// ---- 2-D correlation (filter, filter/mod.rs:24-69) --------------------------
+
+constexpr int kF2TW = 64;
+constexpr int kF2TH = 16;
+
+template <typename P, typename K, typename S>
+__global__ __launch_bounds__(256) void filter2d_tiled_kernel(const P *__restrict__ src,
+                                                             S *__restrict__ dst, int W, int H,
+                                                             int kw, int kh, Taps<K> k) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    P *in_s = reinterpret_cast<P *>(smem_raw);
+    const int R = kF2TH + kh - 1, C = kF2TW + kw - 1;
+    const int x0 = blockIdx.x * kF2TW, y0 = blockIdx.y * kF2TH;
+    // Window origin: kernel centre (kw/2, kh/2), filter/mod.rs:50,53.
+    const int ox = x0 - kw / 2, oy = y0 - kh / 2;
+    for (int i = threadIdx.x; i < R * C; i += blockDim.x) {
+        int r = i / C, c = i - r * C;
+        in_s[i] = src[(size_t)clampi(oy + r, 0, H - 1) * W + clampi(ox + c, 0, W - 1)];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kF2TH * kF2TW; i += blockDim.x) {
+        int r = i / kF2TW, c = i - r * kF2TW;
+        int gy = y0 + r, gx = x0 + c;
+        if (gy >= H || gx >= W) continue;
+        K acc = K(0);
+        int t = 0;
+        for (int ky = 0; ky < kh; ky++) {  // ky outer, kx inner: filter/mod.rs:49-62
+            const P *row = in_s + (r + ky) * C + c;
+            for (int kx = 0; kx < kw; kx++, t++) acc = mac(acc, to_acc(row[kx], K(0)), k.w[t]);
+        }
+        dst[(size_t)gy * W + gx] = ClampTo<S>::from(acc);
+    }
+}
+
+template <typename P, typename K, typename S>
+__global__ __launch_bounds__(256) void filter2d_naive_kernel(const P *__restrict__ src,
+                                                             S *__restrict__ dst, int W, int H,
+                                                             int kw, int kh,
+                                                             const K *__restrict__ k) {
+    int x = blockIdx.x * 64 + threadIdx.x;
+    int y = blockIdx.y * 4 + threadIdx.y;
+    if (x >= W || y >= H) return;
+    K acc = K(0);
+    for (int ky = 0; ky < kh; ky++) {
+        int yp = clampi(y + ky - kh / 2, 0, H - 1);
+        for (int kx = 0; kx < kw; kx++) {
+            int xp = clampi(x + kx - kw / 2, 0, W - 1);
+            acc = mac(acc, to_acc(src[(size_t)yp * W + xp], K(0)), k[(size_t)ky * kw + kx]);
+        }
+    }
+    dst[(size_t)y * W + x] = ClampTo<S>::from(acc);
+}
+
+// ---- box filter (exact u32 window sums) ---------------------------------------
+//
+// out = floor(sum over the clamped (2r+1)-window / (2r+1)), first along rows
+// into a u8 intermediate, then along columns of it: box_filter.rs:30-67.
+// Each thread slides a window over kBoxSeg outputs. Its first window sum is
+// (taps left of the image) * first + (taps right of it) * last + the in-image
+// run, all in wrapping u32, which equals the reference's running-sum
+// difference buf[u] - buf[l] (integral_image.rs:348-463) modulo 2^32.
+
+constexpr int kBoxSeg = 64;
+
+__device__ __forceinline__ uint32_t window_sum_first(const uint8_t *p, size_t pitch, int n,
+                                                     int c, int r) {
+    long long lo = (long long)c - r, hi = (long long)c + r;
+    uint32_t s = 0;
+    long long nl = min(hi, -1LL) - lo + 1;
+    if (nl > 0) s += (uint32_t)nl * (uint32_t)p[0];
+    long long nr = hi - max(lo, (long long)n) + 1;
+    if (nr > 0) s += (uint32_t)nr * (uint32_t)p[(size_t)(n - 1) * pitch];
+    long long a = max(lo, 0LL), b = min(hi, (long long)n - 1);
+    for (long long j = a; j <= b; j++) s += p[(size_t)j * pitch];
+    return s;
+}
+
+// Slides along one line of n pixels (pitch apart), writing kBoxSeg outputs.
+__device__ __forceinline__ void box_line(const uint8_t *in, uint8_t *out, size_t pitch, int n,
+                                         int c0, int r, uint32_t kdiv) {
+    uint32_t s = window_sum_first(in, pitch, n, c0, r);
+    int c1 = min(c0 + kBoxSeg, n);
+    for (int c = c0; c < c1; c++) {
+        out[(size_t)c * pitch] = (uint8_t)(s / kdiv);
+        int add = min(c + 1 + r, n - 1);
+        int sub = max(c - r, 0);
+        s += (uint32_t)in[(size_t)add * pitch];
+        s -= (uint32_t)in[(size_t)sub * pitch];
+    }
+}
+
+__global__ __launch_bounds__(256) void box_rows_kernel(const uint8_t *__restrict__ src,
+                                                       uint8_t *__restrict__ mid, int W, int H,
+                                                       int rx, uint32_t kw) {
+    int nseg = (W + kBoxSeg - 1) / kBoxSeg;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nseg * H) return;
+    int y = (int)(t / nseg), seg = (int)(t - (long long)y * nseg);
+    box_line(src + (size_t)y * W, mid + (size_t)y * W, 1, W, seg * kBoxSeg, rx, kw);
+}
+
+__global__ __launch_bounds__(256) void box_cols_kernel(const uint8_t *__restrict__ mid,
+                                                       uint8_t *__restrict__ dst, int W, int H,
+                                                       int ry, uint32_t kh) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int seg = blockIdx.y;
+    if (x >= W) return;
+    box_line(mid + x, dst + x, (size_t)W, H, seg * kBoxSeg, ry, kh);
+}
+
+// ---- host-side dispatch helpers -----------------------------------------------
+
+template <typename K> void copy_taps(K *dst, const void *src, uint32_t len) {
+    const K *s = static_cast<const K *>(src);
+    for (uint32_t i = 0; i < len; i++) dst[i] = s[i];
+}
+
+template <typename P, typename K>
+iph_status separable_typed(const void *src, void *dst, uint32_t w, uint32_t h, uint32_t n,
+                           size_t stride, const void *hk, const void *vk, uint32_t len,
+                           void *ws, size_t ws_bytes, hipStream_t s) {
+    const P *in = static_cast<const P *>(src);
+    P *out = static_cast<P *>(dst);
+    if (len == 0) {
+        // Both passes sum nothing: every pixel is Clamp(0) (filter/mod.rs:418-429).
+        for (uint32_t i = 0; i < n; i++) {
+            fill_kernel<P, K><<<1024, 256, 0, s>>>(out + (size_t)i * stride, (size_t)w * h, K(0));
+        }
+        return launch_status("fill_kernel");
+    }
+    if (len <= (uint32_t)kFusedMaxTaps) {
+        FusedTaps<K> taps{};
+        copy_taps(taps.h, hk, len);
+        copy_taps(taps.v, vk, len);
+        int L = (int)len;
+        size_t lds = sizeof(P) * ((size_t)(kSepTH + L - 1) * (kSepTW + L - 1) +
+                                  (size_t)(kSepTH + L - 1) * kSepTW);
+        dim3 grid((w + kSepTW - 1) / kSepTW, (h + kSepTH - 1) / kSepTH, n);
+        sep_fused_kernel<P, K><<<grid, 256, lds, s>>>(in, out, (int)w, (int)h, stride, L, taps);
+        return launch_status("sep_fused_kernel");
+    }
+    // Long kernels: two passes through a P-typed intermediate in the workspace.
+    Workspace wsp(ws, ws_bytes);
+    K *dk = wsp.take<K>(2 * (size_t)len);
+    P *mid = wsp.take<P>((size_t)w * h * n);
+    if (!dk || !mid) {
+        set_error("workspace too small for a %u-tap separable filter", len);
+        return IPH_EINVAL;
+    }
+    IPH_TRY_HIP(hipMemcpyAsync(dk, hk, sizeof(K) * len, hipMemcpyHostToDevice, s));
+    IPH_TRY_HIP(hipMemcpyAsync(dk + len, vk, sizeof(K) * len, hipMemcpyHostToDevice, s));
+    dim3 grid((w + 63) / 64, (h + 3) / 4, n), block(64, 4);
+    for (uint32_t i = 0; i < n; i++) {
+        filter1d_kernel<P, K, false><<<dim3(grid.x, grid.y, 1), block, 0, s>>>(
+            in + (size_t)i * stride, mid + (size_t)i * w * h, (int)w, (int)h, 0, dk, (int)len);
+        filter1d_kernel<P, K, true><<<dim3(grid.x, grid.y, 1), block, 0, s>>>(
+            mid + (size_t)i * w * h, out + (size_t)i * stride, (int)w, (int)h, 0, dk + len,
+            (int)len);
+    }
+    return launch_status("filter1d_kernel");
+}
+
+template <typename P, typename K>
+iph_status filter1d_typed(const void *src, void *dst, uint32_t w, uint32_t h, const void *k,
+                          uint32_t len, bool vertical, void *ws, size_t ws_bytes,
+                          hipStream_t s) {
+    if (len == 0) {
+        fill_kernel<P, K><<<1024, 256, 0, s>>>(static_cast<P *>(dst), (size_t)w * h, K(0));
+        return launch_status("fill_kernel");
+    }
+    Workspace wsp(ws, ws_bytes);
+    K *dk = wsp.take<K>(len);
+    if (!dk) {
+        set_error("workspace too small for a %u-tap filter", len);
+        return IPH_EINVAL;
+    }
+    IPH_TRY_HIP(hipMemcpyAsync(dk, k, sizeof(K) * len, hipMemcpyHostToDevice, s));
+    dim3 grid((w + 63) / 64, (h + 3) / 4, 1), block(64, 4);
+    if (vertical)
+        filter1d_kernel<P, K, true><<<grid, block, 0, s>>>(
+            static_cast<const P *>(src), static_cast<P *>(dst), (int)w, (int)h, 0, dk, (int)len);
+    else
+        filter1d_kernel<P, K, false><<<grid, block, 0, s>>>(
+            static_cast<const P *>(src), static_cast<P *>(dst), (int)w, (int)h, 0, dk, (int)len);
+    return launch_status("filter1d_kernel");
+}
+
+constexpr size_t kMaxTiledLds = 64 * 1024;
+
+template <typename P, typename K, typename S>
+iph_status filter2d_typed(const void *src, uint32_t w, uint32_t h, const void *kernel,
+                          uint32_t kw, uint32_t kh, void *dst, void *ws, size_t ws_bytes,
+                          hipStream_t s) {
+    size_t taps = (size_t)kw * kh;
+    size_t lds = sizeof(P) * ((size_t)kF2TH + kh - 1) * ((size_t)kF2TW + kw - 1);
+    if (taps <= (size_t)kMaxArgTaps && lds <= kMaxTiledLds) {
+        Taps<K> k{};
+        copy_taps(k.w, kernel, (uint32_t)taps);
+        dim3 grid((w + kF2TW - 1) / kF2TW, (h + kF2TH - 1) / kF2TH, 1);
+        filter2d_tiled_kernel<P, K, S><<<grid, 256, lds, s>>>(
+            static_cast<const P *>(src), static_cast<S *>(dst), (int)w, (int)h, (int)kw,
+            (int)kh, k);
+        return launch_status("filter2d_tiled_kernel");
+    }
+    Workspace wsp(ws, ws_bytes);
+    K *dk = wsp.take<K>(taps);
+    if (!dk) {
+        set_error("workspace too small for a %ux%u kernel", kw, kh);
+        return IPH_EINVAL;
+    }
+    IPH_TRY_HIP(hipMemcpyAsync(dk, kernel, sizeof(K) * taps, hipMemcpyHostToDevice, s));
+    dim3 grid((w + 63) / 64, (h + 3) / 4, 1), block(64, 4);
+    filter2d_naive_kernel<P, K, S><<<grid, block, 0, s>>>(static_cast<const P *>(src),
+                                                          static_cast<S *>(dst), (int)w, (int)h,
+                                                          (int)kw, (int)kh, dk);
+    return launch_status("filter2d_naive_kernel");
+}
+
+}  // namespace
+
+// ---- launchers ----------------------------------------------------------------
+
+size_t separable_workspace(uint32_t w, uint32_t h, uint32_t n, int32_t p_type, uint32_t len,
+                           int32_t k_type) {
+    if (len <= (uint32_t)kFusedMaxTaps) return 0;
+    WorkspacePlan p;
+    p.take<int32_t>(2 * (size_t)len);
+    p.used = align_up(p.used, 256) + align_up((size_t)w * h * n * elem_size(p_type), 256);
+    return p.used;
+}
+
+iph_status launch_separable(const void *src, void *dst, uint32_t w, uint32_t h, uint32_t n,
+                            size_t stride, int32_t p_type, const void *hk, const void *vk,
+                            uint32_t len, int32_t k_type, void *ws, size_t ws_bytes,
+                            hipStream_t s) {
+    if (w == 0 || h == 0 || n == 0) return IPH_OK;
+    if (p_type == IPH_U8 && k_type == IPH_F32)
+        return separable_typed<uint8_t, float>(src, dst, w, h, n, stride, hk, vk, len, ws,
+                                               ws_bytes, s);
+    if (p_type == IPH_U8 && k_type == IPH_I32)
+        return separable_typed<uint8_t, int32_t>(src, dst, w, h, n, stride, hk, vk, len, ws,
+                                                 ws_bytes, s);
+    if (p_type == IPH_F32 && k_type == IPH_F32)
+        return separable_typed<float, float>(src, dst, w, h, n, stride, hk, vk, len, ws,
+                                             ws_bytes, s);
+    set_error("separable filter: pixel type %d with kernel type %d is not supported", p_type,
+              k_type);
+    return IPH_EUNSUPPORTED;
+}
+
+iph_status launch_filter1d(const void *src, void *dst, uint32_t w, uint32_t h, int32_t p_type,
+                           const void *k, uint32_t len, int32_t k_type, bool vertical,
+                           void *ws, size_t ws_bytes, hipStream_t s) {
+    if (w == 0 || h == 0) return IPH_OK;
+    if (p_type == IPH_U8 && k_type == IPH_F32)
+        return filter1d_typed<uint8_t, float>(src, dst, w, h, k, len, vertical, ws, ws_bytes, s);
+    if (p_type == IPH_U8 && k_type == IPH_I32)
+        return filter1d_typed<uint8_t, int32_t>(src, dst, w, h, k, len, vertical, ws, ws_bytes,
+                                                s);
+    if (p_type == IPH_F32 && k_type == IPH_F32)
+        return filter1d_typed<float, float>(src, dst, w, h, k, len, vertical, ws, ws_bytes, s);
+    set_error("1-D filter: pixel type %d with kernel type %d is not supported", p_type, k_type);
+    return IPH_EUNSUPPORTED;
+}
+
+size_t filter_clamped_workspace(int32_t k_type, uint32_t kw, uint32_t kh) {
+    return align_up((size_t)kw * kh * 4, 256);
+}
+
+iph_status launch_filter_clamped(const void *src, int32_t p, uint32_t w, uint32_t h,
+                                 const void *kernel, int32_t k, uint32_t kw, uint32_t kh,
+                                 void *dst, int32_t st, void *ws, size_t ws_bytes,
+                                 hipStream_t s) {
+    if (w == 0 || h == 0) return IPH_OK;
+#define F2(PT, KT, ST)                                                                       \
+    return filter2d_typed<PT, KT, ST>(src, w, h, kernel, kw, kh, dst, ws, ws_bytes, s)
+    if (k == IPH_F32 && p == IPH_U8) {
+        if (st == IPH_U8) F2(uint8_t, float, uint8_t);
+        if (st == IPH_U16) F2(uint8_t, float, uint16_t);
+        if (st == IPH_F32) F2(uint8_t, float, float);
+    } else if (k == IPH_F32 && p == IPH_F32) {
+        if (st == IPH_U8) F2(float, float, uint8_t);
+        if (st == IPH_U16) F2(float, float, uint16_t);
+        if (st == IPH_F32) F2(float, float, float);
+    } else if (k == IPH_I32 && p == IPH_U8) {
+        if (st == IPH_U8) F2(uint8_t, int32_t, uint8_t);
+        if (st == IPH_U16) F2(uint8_t, int32_t, uint16_t);
+        if (st == IPH_I16) F2(uint8_t, int32_t, int16_t);
+        if (st == IPH_I32) F2(uint8_t, int32_t, int32_t);
+    }
+#undef F2
+    set_error("filter_clamped: (pixel %d, kernel %d, output %d) is not a combination the "
+              "reference accepts",
+              p, k, st);
+    return IPH_EUNSUPPORTED;
+}
+
+iph_status launch_box_filter(const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h,
+                             uint32_t rx, uint32_t ry, void *ws, size_t ws_bytes,
+                             hipStream_t s) {
+    if (w == 0 || h == 0) return IPH_OK;  // box_filter.rs:22-24
+    Workspace wsp(ws, ws_bytes);
+    uint8_t *mid = wsp.take<uint8_t>((size_t)w * h);
+    if (!mid) {
+        set_error("workspace too small for box_filter");
+        return IPH_EINVAL;
+    }
+    uint32_t kw = 2u * rx + 1u, kh = 2u * ry + 1u;  // box_filter.rs:26-27 (u32)
+    long long nrow = (long long)((w + kBoxSeg - 1) / kBoxSeg) * h;
+    box_rows_kernel<<<(unsigned)((nrow + 255) / 256), 256, 0, s>>>(src, mid, (int)w, (int)h,
+                                                                   (int)rx, kw);
+    IPH_TRY(launch_status("box_rows_kernel"));
+    dim3 grid((w + 255) / 256, (h + kBoxSeg - 1) / kBoxSeg);
+    box_cols_kernel<<<grid, 256, 0, s>>>(mid, dst, (int)w, (int)h, (int)ry, kh);
+    return launch_status("box_cols_kernel");
+}
+
+}  // namespace iph

@@ -1,0 +1,113 @@
+// Shared definitions for the gfx950 kernels and the C ABI host code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+
+#include "imageproc_hip.h"
+
+namespace iph {
+
+// Records the message returned by iph_last_error() for this thread.
+void set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+void clear_error();
+
+#define IPH_TRY_HIP(expr)                                                                    \
+    do {                                                                                     \
+        hipError_t iph_e_ = (expr);                                                          \
+        if (iph_e_ != hipSuccess) {                                                          \
+            ::iph::set_error("%s: %s", #expr, hipGetErrorString(iph_e_));                    \
+            return IPH_EDEVICE;                                                              \
+        }                                                                                    \
+    } while (0)
+
+#define IPH_TRY(expr)                                                                        \
+    do {                                                                                     \
+        iph_status iph_s_ = (expr);                                                          \
+        if (iph_s_ != IPH_OK) return iph_s_;                                                 \
+    } while (0)
+
+// Checks the launch status of the kernel just enqueued.
+inline iph_status launch_status(const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("launch of %s failed: %s", what, hipGetErrorString(e));
+        return IPH_EDEVICE;
+    }
+    return IPH_OK;
+}
+
+inline size_t elem_size(int32_t t) {
+    switch (t) {
+    case IPH_U8: return 1;
+    case IPH_U16:
+    case IPH_I16: return 2;
+    default: return 4;
+    }
+}
+
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Bump allocator over a caller-provided device workspace.
+struct Workspace {
+    char *base;
+    size_t cap, used;
+    Workspace(void *p, size_t n) : base(static_cast<char *>(p)), cap(n), used(0) {}
+    template <typename T> T *take(size_t count) {
+        size_t off = align_up(used, 256);
+        size_t bytes = align_up(count * sizeof(T), 256);
+        if (off + bytes > cap || base == nullptr) return nullptr;
+        used = off + bytes;
+        return reinterpret_cast<T *>(base + off);
+    }
+};
+
+// Grows a workspace size the same way Workspace::take consumes it.
+struct WorkspacePlan {
+    size_t used = 0;
+    template <typename T> void take(size_t count) {
+        used = align_up(used, 256) + align_up(count * sizeof(T), 256);
+    }
+};
+
+// Largest tap count passed by value in a kernel argument block.
+constexpr int kMaxArgTaps = 256;
+template <typename K> struct Taps {
+    K w[kMaxArgTaps];
+};
+
+// ---- launchers implemented in filter.hip --------------------------------
+
+iph_status launch_separable(const void *src, void *dst, uint32_t w, uint32_t h, uint32_t n,
+                            size_t stride, int32_t p_type, const void *hk, const void *vk,
+                            uint32_t len, int32_t k_type, void *ws, size_t ws_bytes,
+                            hipStream_t s);
+size_t separable_workspace(uint32_t w, uint32_t h, uint32_t n, int32_t p_type, uint32_t len,
+                           int32_t k_type);
+iph_status launch_filter1d(const void *src, void *dst, uint32_t w, uint32_t h, int32_t p_type,
+                           const void *k, uint32_t len, int32_t k_type, bool vertical,
+                           void *ws, size_t ws_bytes, hipStream_t s);
+iph_status launch_filter_clamped(const void *src, int32_t p_type, uint32_t w, uint32_t h,
+                                 const void *kernel, int32_t k_type, uint32_t kw, uint32_t kh,
+                                 void *dst, int32_t s_type, void *ws, size_t ws_bytes,
+                                 hipStream_t s);
+size_t filter_clamped_workspace(int32_t k_type, uint32_t kw, uint32_t kh);
+iph_status launch_box_filter(const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h,
+                             uint32_t rx, uint32_t ry, void *ws, size_t ws_bytes,
+                             hipStream_t s);
+
+// ---- launchers implemented in match.hip ---------------------------------
+
+size_t match_workspace(int32_t p_type, uint32_t w, uint32_t h, uint32_t tw, uint32_t th,
+                       int32_t method, int32_t path);
+iph_status launch_match(const void *img, int32_t p_type, uint32_t w, uint32_t h,
+                        const void *tpl, uint32_t tw, uint32_t th, int32_t method,
+                        int32_t path, float *out, void *ws, size_t ws_bytes, hipStream_t s);
+size_t extremes_workspace();
+iph_status launch_extremes(const void *img, int32_t p_type, uint32_t w, uint32_t h,
+                           iph_extremes *out, void *ws, size_t ws_bytes, hipStream_t s);
+
+}  // namespace iph

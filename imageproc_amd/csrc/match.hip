@@ -1,0 +1,247 @@
+// gfx950 kernels for imageproc's template matching:
+//   match_template (template_matching.rs:66-110, 165-293, 378-425)
+//   find_extremes  (template_matching.rs:463-511)
+//
+// Two correlation engines:
+//   * direct: one thread per output window, walking dy-outer/dx-inner in f32
+//     with separately rounded multiplies and adds, i.e. the reference's own
+//     operation sequence (slide_window_at, template_matching.rs:405-417);
+//     bit-identical to it.
+//   * MFMA (match_mfma.hip): exact integer correlation on int8 MFMA for u8
+//     images, BF16x3 on MFMA for f32 images.
+#include "iph_internal.hpp"
+
+#include "match_common.hpp"
+
+namespace iph {
+namespace {
+
+__device__ __forceinline__ float fmul(float a, float b) { return __fmul_rn(a, b); }
+__device__ __forceinline__ float fadd(float a, float b) { return __fadd_rn(a, b); }
+__device__ __forceinline__ float fsub(float a, float b) { return __fsub_rn(a, b); }
+
+// square_sum, template_matching.rs:378-380: sequential f32 sum of t*t.
+template <typename P>
+__global__ void square_sum_kernel(const P *__restrict__ tpl, size_t n, float *__restrict__ out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    float s = -0.0f;
+    for (size_t i = 0; i < n; i++) {
+        float t = (float)tpl[i];
+        s = fadd(s, fmul(t, t));
+    }
+    *out = s;
+}
+
+// One output per thread, reference operation order. The template element
+// index is uniform across the wave, so template reads are scalar loads.
+template <typename P, int METHOD>
+__global__ __launch_bounds__(256) void match_direct_kernel(const P *__restrict__ img, int W,
+                                                           const P *__restrict__ tpl, int tw,
+                                                           int th, const float *__restrict__ tss,
+                                                           float *__restrict__ out, int ow,
+                                                           int oh) {
+    int x = blockIdx.x * 64 + threadIdx.x;
+    int y = blockIdx.y * 4 + threadIdx.y;
+    if (x >= ow || y >= oh) return;
+    float score = 0.0f, ii = 0.0f;
+    for (int dy = 0; dy < th; dy++) {
+        const P *irow = img + (size_t)(y + dy) * W + x;
+        const P *trow = tpl + (size_t)dy * tw;
+        for (int dx = 0; dx < tw; dx++) {
+            float i = (float)irow[dx];
+            float t = (float)trow[dx];
+            if (METHOD == IPH_SSE || METHOD == IPH_SSE_NORMED) {
+                float d = fsub(t, i);                      // (t - i).powi(2)
+                score = fadd(score, fmul(d, d));
+            } else {
+                score = fadd(score, fmul(i, t));           // i * t
+            }
+            if (METHOD == IPH_SSE_NORMED || METHOD == IPH_CCORR_NORMED)
+                ii = fadd(ii, fmul(i, i));
+        }
+    }
+    if (METHOD == IPH_SSE_NORMED || METHOD == IPH_CCORR_NORMED) {
+        // template_matching.rs:249-250 / 289-291. sqrtf and '/' are correctly
+        // rounded (built with -fhip-fp32-correctly-rounded-divide-sqrt).
+        float norm = sqrtf(fmul(ii, *tss));
+        score = norm > 0.0f ? score / norm : score;
+    }
+    out[(size_t)y * ow + x] = score;
+}
+
+template <typename P>
+iph_status direct_typed(const P *img, uint32_t w, const P *tpl, uint32_t tw, uint32_t th,
+                        int32_t method, float *out, uint32_t ow, uint32_t oh, float *tss,
+                        hipStream_t s) {
+    square_sum_kernel<P><<<1, 64, 0, s>>>(tpl, (size_t)tw * th, tss);
+    IPH_TRY(launch_status("square_sum_kernel"));
+    dim3 grid((ow + 63) / 64, (oh + 3) / 4), block(64, 4);
+#define LAUNCH(M)                                                                            \
+    match_direct_kernel<P, M><<<grid, block, 0, s>>>(img, (int)w, tpl, (int)tw, (int)th, tss, \
+                                                     out, (int)ow, (int)oh)
+    switch (method) {
+    case IPH_SSE: LAUNCH(IPH_SSE); break;
+    case IPH_SSE_NORMED: LAUNCH(IPH_SSE_NORMED); break;
+    case IPH_CCORR: LAUNCH(IPH_CCORR); break;
+    default: LAUNCH(IPH_CCORR_NORMED); break;
+    }
+#undef LAUNCH
+    return launch_status("match_direct_kernel");
+}
+
+// ---- find_extremes ------------------------------------------------------------
+//
+// Sequential semantics (template_matching.rs:488-503): start from (0,0), a
+// later pixel replaces the current extreme only if strictly smaller/larger,
+// so the result is the first pixel in row-major order holding the extreme
+// value, NaNs never replace, and a NaN at (0,0) is never replaced.
+// Parallel form: reduce 64-bit keys (ordered value << 32 | index) with
+// atomicMin/atomicMax, skipping NaNs; -0.0 and +0.0 map to the same key
+// because the sequential compare treats them as equal. The finalize step
+// reads the value stored at the winning index and applies the NaN-at-(0,0)
+// rule.
+
+__device__ __forceinline__ uint32_t order_key(float v) {
+    uint32_t u = __float_as_uint(v);
+    if ((u & 0x7fffffffu) == 0) u = 0;  // -0.0 == +0.0
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ uint32_t order_key(uint8_t v) { return v; }
+__device__ __forceinline__ bool is_nan(float v) { return v != v; }
+__device__ __forceinline__ bool is_nan(uint8_t) { return false; }
+
+__global__ void extremes_init_kernel(unsigned long long *keys) {
+    if (threadIdx.x == 0) {
+        keys[0] = ~0ull;  // min key
+        keys[1] = 0ull;   // max key
+    }
+}
+
+__device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v, int m) {
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+    lo = __shfl_xor(lo, m);
+    hi = __shfl_xor(hi, m);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+template <typename P>
+__global__ __launch_bounds__(256) void extremes_reduce_kernel(const P *__restrict__ img,
+                                                              size_t n,
+                                                              unsigned long long *keys) {
+    unsigned long long kmin = ~0ull, kmax = 0ull;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t step = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += step) {
+        P v = img[i];
+        if (is_nan(v)) continue;
+        unsigned long long k = (unsigned long long)order_key(v) << 32;
+        unsigned long long a = k | (uint32_t)i;
+        unsigned long long b = k | (0xffffffffu - (uint32_t)i);
+        kmin = a < kmin ? a : kmin;
+        kmax = b > kmax ? b : kmax;
+    }
+    for (int m = 32; m >= 1; m >>= 1) {
+        unsigned long long o = shfl_xor_u64(kmin, m);
+        kmin = o < kmin ? o : kmin;
+        o = shfl_xor_u64(kmax, m);
+        kmax = o > kmax ? o : kmax;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (kmin != ~0ull) atomicMin(&keys[0], kmin);
+        if (kmax != 0ull) atomicMax(&keys[1], kmax);
+    }
+}
+
+template <typename P>
+__global__ void extremes_finalize_kernel(const P *__restrict__ img, uint32_t w,
+                                         const unsigned long long *keys, iph_extremes *out) {
+    if (threadIdx.x != 0) return;
+    iph_extremes r;
+    r.max_value.raw = 0;
+    r.min_value.raw = 0;
+    P v0 = img[0];
+    uint32_t imin = 0, imax = 0;
+    if (!is_nan(v0)) {
+        imin = (uint32_t)(keys[0] & 0xffffffffu);
+        imax = 0xffffffffu - (uint32_t)(keys[1] & 0xffffffffu);
+    }
+    P vmin = img[imin], vmax = img[imax];
+    if constexpr (sizeof(P) == 1) {
+        r.min_value.u8 = vmin;
+        r.max_value.u8 = vmax;
+    } else {
+        r.min_value.f32 = vmin;
+        r.max_value.f32 = vmax;
+    }
+    r.min_x = imin % w;
+    r.min_y = imin / w;
+    r.max_x = imax % w;
+    r.max_y = imax / w;
+    *out = r;
+}
+
+}  // namespace
+
+size_t match_workspace(int32_t p_type, uint32_t w, uint32_t h, uint32_t tw, uint32_t th,
+                       int32_t method, int32_t path) {
+    if (path == IPH_PATH_DIRECT) return 256;
+    return mfma_match_workspace(p_type, w, h, tw, th, method);
+}
+
+iph_status launch_match(const void *img, int32_t p_type, uint32_t w, uint32_t h,
+                        const void *tpl, uint32_t tw, uint32_t th, int32_t method,
+                        int32_t path, float *out, void *ws, size_t ws_bytes, hipStream_t s) {
+    if (method < IPH_SSE || method > IPH_CCORR_NORMED) {
+        set_error("unknown MatchTemplateMethod %d", method);
+        return IPH_EINVAL;
+    }
+    uint32_t ow = w - tw + 1, oh = h - th + 1;
+    if (ow == 0 || oh == 0) return IPH_OK;
+    if (path == IPH_PATH_DIRECT) {
+        Workspace wsp(ws, ws_bytes);
+        float *tss = wsp.take<float>(1);
+        if (!tss) {
+            set_error("workspace too small for match_template");
+            return IPH_EINVAL;
+        }
+        if (p_type == IPH_U8)
+            return direct_typed(static_cast<const uint8_t *>(img), w,
+                                static_cast<const uint8_t *>(tpl), tw, th, method, out, ow, oh,
+                                tss, s);
+        return direct_typed(static_cast<const float *>(img), w, static_cast<const float *>(tpl),
+                            tw, th, method, out, ow, oh, tss, s);
+    }
+    return launch_mfma_match(img, p_type, w, h, tpl, tw, th, method, out, ws, ws_bytes, s);
+}
+
+size_t extremes_workspace() { return 256; }
+
+iph_status launch_extremes(const void *img, int32_t p_type, uint32_t w, uint32_t h,
+                           iph_extremes *out, void *ws, size_t ws_bytes, hipStream_t s) {
+    Workspace wsp(ws, ws_bytes);
+    unsigned long long *keys = wsp.take<unsigned long long>(2);
+    if (!keys) {
+        set_error("workspace too small for find_extremes");
+        return IPH_EINVAL;
+    }
+    size_t n = (size_t)w * h;
+    extremes_init_kernel<<<1, 64, 0, s>>>(keys);
+    IPH_TRY(launch_status("extremes_init_kernel"));
+    unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 2048);
+    if (p_type == IPH_U8) {
+        extremes_reduce_kernel<uint8_t><<<blocks, 256, 0, s>>>(static_cast<const uint8_t *>(img),
+                                                              n, keys);
+        IPH_TRY(launch_status("extremes_reduce_kernel"));
+        extremes_finalize_kernel<uint8_t><<<1, 64, 0, s>>>(static_cast<const uint8_t *>(img), w,
+                                                           keys, out);
+    } else {
+        extremes_reduce_kernel<float><<<blocks, 256, 0, s>>>(static_cast<const float *>(img), n,
+                                                            keys);
+        IPH_TRY(launch_status("extremes_reduce_kernel"));
+        extremes_finalize_kernel<float><<<1, 64, 0, s>>>(static_cast<const float *>(img), w,
+                                                         keys, out);
+    }
+    return launch_status("extremes_finalize_kernel");
+}
+
+}  // namespace iph

@@ -1,0 +1,123 @@
+"""Device-resident entry points: the iph_dev_* functions over torch tensors.
+
+torch is used only to own device memory and to name the current HIP stream;
+every computation is a kernel of libimageproc_hip.so. Workspaces are cached
+per (function, size) so repeated calls do not allocate.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from ._lib import F32, I16, I32, U8, U16, Extremes, check, lib
+from .template_matching import MatchTemplateMethod, _PATHS
+
+_TORCH_CODE = {torch.uint8: U8, torch.int16: I16, torch.int32: I32, torch.float32: F32,
+               torch.uint16: U16}
+_ws_cache: dict = {}
+
+
+def _stream() -> C.c_void_p:
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _workspace(key, nbytes: int, device) -> torch.Tensor:
+    t = _ws_cache.get((key, device))
+    if t is None or t.numel() < nbytes:
+        t = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=device)
+        _ws_cache[(key, device)] = t
+    return t
+
+
+def _code(t: torch.Tensor) -> int:
+    if not t.is_cuda or not t.is_contiguous():
+        raise ValueError("expected a contiguous device tensor")
+    return _TORCH_CODE[t.dtype]
+
+
+def gaussian_blur_f32(src: torch.Tensor, sigma: float, out: torch.Tensor | None = None):
+    """Batched gaussian_blur_f32 over src of shape (n, h, w) or (h, w)."""
+    p = _code(src)
+    n, h, w = (1, *src.shape) if src.dim() == 2 else src.shape
+    out = torch.empty_like(src) if out is None else out
+    wsb = lib().iph_dev_gaussian_blur_workspace(w, h, n, p, float(sigma))
+    ws = _workspace("blur", wsb, src.device)
+    check(lib().iph_dev_gaussian_blur_f32(src.data_ptr(), out.data_ptr(), w, h, n, h * w, p,
+                                          float(sigma), ws.data_ptr(), ws.numel(), _stream()))
+    return out
+
+
+def separable_filter(src: torch.Tensor, h_kernel, v_kernel, out: torch.Tensor | None = None):
+    """Batched separable_filter; kernels are host sequences (f32 or i32)."""
+    import numpy as np
+
+    p = _code(src)
+    n, h, w = (1, *src.shape) if src.dim() == 2 else src.shape
+    hk = np.asarray(h_kernel)
+    kt = F32 if hk.dtype.kind == "f" else I32
+    dt = np.float32 if kt == F32 else np.int32
+    hk = np.ascontiguousarray(hk, dtype=dt)
+    vk = np.ascontiguousarray(np.asarray(v_kernel), dtype=dt)
+    out = torch.empty_like(src) if out is None else out
+    wsb = lib().iph_dev_separable_filter_workspace(w, h, n, p, hk.size, kt)
+    ws = _workspace("sep", wsb, src.device)
+    check(lib().iph_dev_separable_filter(src.data_ptr(), out.data_ptr(), w, h, n, h * w, p,
+                                         hk.ctypes.data, hk.size, vk.ctypes.data, vk.size, kt,
+                                         ws.data_ptr(), ws.numel(), _stream()))
+    return out
+
+
+def filter_clamped(src: torch.Tensor, kernel, out_dtype=None, out: torch.Tensor | None = None):
+    p = _code(src)
+    h, w = src.shape
+    kt = F32 if kernel.data.dtype.kind == "f" else I32
+    if out is None:
+        out = torch.empty_like(src, dtype=out_dtype or src.dtype)
+    wsb = lib().iph_dev_filter_clamped_workspace(kt, kernel.width, kernel.height)
+    ws = _workspace("f2d", wsb, src.device)
+    check(lib().iph_dev_filter_clamped(src.data_ptr(), p, w, h, kernel.data.ctypes.data, kt,
+                                       kernel.width, kernel.height, out.data_ptr(), _code(out),
+                                       ws.data_ptr(), ws.numel(), _stream()))
+    return out
+
+
+def box_filter(src: torch.Tensor, x_radius: int, y_radius: int, out=None):
+    _code(src)
+    h, w = src.shape
+    out = torch.empty_like(src) if out is None else out
+    wsb = lib().iph_dev_box_filter_workspace(w, h)
+    ws = _workspace("box", wsb, src.device)
+    check(lib().iph_dev_box_filter(src.data_ptr(), out.data_ptr(), w, h, x_radius, y_radius,
+                                   ws.data_ptr(), ws.numel(), _stream()))
+    return out
+
+
+def match_template(image: torch.Tensor, template: torch.Tensor, method, path: str = "auto",
+                   out: torch.Tensor | None = None):
+    p = _code(image)
+    h, w = image.shape
+    th, tw = template.shape
+    m = int(MatchTemplateMethod(method))
+    if out is None:
+        out = torch.empty((max(h - th + 1, 0), max(w - tw + 1, 0)), dtype=torch.float32,
+                          device=image.device)
+    wsb = lib().iph_dev_match_template_workspace(p, w, h, tw, th, m, _PATHS[path])
+    ws = _workspace(("match", path), wsb, image.device)
+    check(lib().iph_dev_match_template(image.data_ptr(), p, w, h, template.data_ptr(), tw, th,
+                                       m, _PATHS[path], out.data_ptr(), ws.data_ptr(),
+                                       ws.numel(), _stream()))
+    return out
+
+
+def find_extremes(image: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+    """Writes the 24-byte iph_extremes record into `out` (a device uint8 tensor)."""
+    p = _code(image)
+    h, w = image.shape
+    if out is None:
+        out = torch.empty(C.sizeof(Extremes), dtype=torch.uint8, device=image.device)
+    wsb = lib().iph_dev_find_extremes_workspace()
+    ws = _workspace("ext", wsb, image.device)
+    check(lib().iph_dev_find_extremes(image.data_ptr(), p, w, h, out.data_ptr(), ws.data_ptr(),
+                                      ws.numel(), _stream()))
+    return out

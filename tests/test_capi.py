@@ -1,0 +1,94 @@
+"""CPU tests of the C ABI and of the host-side mirror: the library loads,
+exports every symbol include/imageproc_hip.h declares, computes the Gaussian
+taps on the host bit-identically to the oracle, and rejects every argument
+the reference panics on before any GPU work is attempted."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import ROOT
+
+import imageproc_amd as ip
+from imageproc_amd import _lib
+from imageproc_amd.filter import (box_filter, gaussian_blur_f32, gaussian_kernel_f32,
+                                  separable_filter)
+from imageproc_amd.template_matching import MatchTemplateMethod, find_extremes, match_template
+
+
+def _declared_symbols():
+    src = open(os.path.join(ROOT, "include", "imageproc_hip.h")).read()
+    return sorted(set(re.findall(r"IPH_API\s+[\w\s\*]+?\b(iph_\w+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    declared = _declared_symbols()
+    assert len(declared) >= 20
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert sorted(_lib.EXPORTED) == declared
+
+
+def test_library_holds_gfx950_code_only():
+    data = open(_lib.LIB_PATH, "rb").read()
+    assert b"gfx950" in data
+    for other in (b"gfx942", b"gfx90a", b"gfx1100"):
+        assert other not in data
+
+
+@pytest.mark.parametrize("sigma", [0.5, 0.7, 1.0, 1.4, 2.0, 3.0, 4.0, 7.25, 8.0, 10.0, 63.0])
+def test_host_gaussian_taps_are_bit_identical_to_the_oracle(sigma):
+    a = gaussian_kernel_f32(sigma)
+    b = O.gaussian_kernel_f32(sigma)
+    assert a.dtype == np.float32 and np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_reference_panics_are_raised_before_gpu_work():
+    img = np.zeros((4, 4), np.uint8)
+    with pytest.raises(ip.Panic, match="sigma must be > 0.0"):
+        gaussian_blur_f32(img, 0.0)
+    with pytest.raises(ip.Panic, match="sigma must be > 0.0"):
+        gaussian_blur_f32(img, -0.5)
+    with pytest.raises(ip.Panic, match="sigma must be > 0.0"):
+        gaussian_blur_f32(img, float("nan"))
+    with pytest.raises(ip.Panic, match="the two 1D kernels must be the same length"):
+        separable_filter(img, np.ones(3, np.float32), np.ones(5, np.float32))
+    with pytest.raises(ip.Panic, match="image width must be greater than or equal to template width"):
+        match_template(np.zeros((5, 5), np.uint8), np.zeros((5, 6), np.uint8),
+                       MatchTemplateMethod.SumOfSquaredErrors)
+    with pytest.raises(ip.Panic, match="image height must be greater than or equal to template height"):
+        match_template(np.zeros((5, 5), np.uint8), np.zeros((6, 5), np.uint8),
+                       MatchTemplateMethod.SumOfSquaredErrors)
+    with pytest.raises(ip.Panic, match="image must be non-empty"):
+        find_extremes(np.zeros((0, 3), np.float32))
+
+
+def test_kernel_new_panics():
+    # kernel.rs:24-33, filter/mod.rs:755-760, kernel.rs:95-106
+    with pytest.raises(ip.Panic, match="width and height must be non-zero"):
+        ip.Kernel([], 0, 0)
+    with pytest.raises(ip.Panic):
+        ip.Kernel([1, 2, 3], 2, 2)
+    with pytest.raises(ip.Panic):
+        ip.Kernel(np.zeros(2, np.float32), 2, 2_147_483_649)
+
+
+def test_type_combinations_the_reference_rejects():
+    img = np.zeros((3, 3), np.float32)
+    with pytest.raises(TypeError):
+        separable_filter(img, np.ones(3, np.int32), np.ones(3, np.int32))
+    with pytest.raises(TypeError):
+        ip.filter.filter_clamped(np.zeros((3, 3), np.uint8), ip.Kernel([1.0], 1, 1), np.int16)
+
+
+def test_empty_images_need_no_device():
+    # box_filter.rs:77-82 and filter/mod.rs:779-789: empty in, empty out
+    for shape in [(0, 0), (1, 0), (0, 1)]:
+        assert box_filter(np.zeros(shape, np.uint8), 3, 3).shape == shape
+        assert gaussian_blur_f32(np.zeros(shape, np.uint8), 1.0).shape == shape
+        assert ip.filter.filter_clamped(np.zeros(shape, np.uint8),
+                                        ip.Kernel([2], 1, 1)).shape == shape

@@ -1,0 +1,109 @@
+"""GPU parity of match_template and find_extremes.
+
+* direct path: the reference's per-window f32 loop, bit-identical to the oracle;
+* MFMA path, u8 images: exact integer sums, so CrossCorrelation and
+  SumOfSquaredErrors equal the exact value rounded once to f32, and the
+  normalised methods are within a few f32 ulps of the FP64 oracle;
+* MFMA path, f32 images (BF16x3): within the stated tolerance of FP64.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import gray_image
+
+from imageproc_amd.template_matching import MatchTemplateMethod as M
+from imageproc_amd.template_matching import find_extremes, match_template
+
+pytestmark = pytest.mark.gpu
+F = np.float32
+
+IMG = gray_image([[1, 4, 2], [2, 1, 3], [3, 3, 4]])
+TPL = gray_image([[1, 2], [3, 4]])
+PATHS = ["direct", "mfma"]
+
+
+def _bits_equal(a, b):
+    assert a.shape == b.shape and a.dtype == b.dtype == np.float32
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_match_template_kats(path):
+    # template_matching.rs:562-657
+    tss = F(30.0)
+    nz = lambda s, ii: F(s) / np.sqrt(F(ii) * tss)
+    exp = {
+        M.SumOfSquaredErrors: [[14, 14], [3, 1]],
+        M.SumOfSquaredErrorsNormalized: [[nz(14, 22), nz(14, 30)], [nz(3, 23), nz(1, 35)]],
+        M.CrossCorrelation: [[19, 23], [25, 32]],
+        M.CrossCorrelationNormalized: [[nz(19, 22), nz(23, 30)], [nz(25, 23), nz(32, 35)]],
+    }
+    for m, e in exp.items():
+        _bits_equal(match_template(IMG, TPL, m, path), np.array(e, F))
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_match_template_edge_cases(path):
+    z = np.zeros((5, 5), np.uint8)
+    _bits_equal(match_template(z, z, M.SumOfSquaredErrors, path), np.array([[0.0]], F))
+    one = np.zeros((1, 1), np.uint8)
+    _bits_equal(match_template(one, one, M.SumOfSquaredErrorsNormalized, path),
+                np.array([[0.0]], F))
+    # an all-zero image gives zero norms everywhere: the raw sum is returned
+    img = np.zeros((9, 12), np.uint8)
+    tpl = np.arange(12, dtype=np.uint8).reshape(3, 4)
+    for m in M:
+        _bits_equal(match_template(img, tpl, m, path), O.match_template(img, tpl, int(m)))
+
+
+@pytest.mark.parametrize("shape,tshape", [((23, 31), (3, 4)), ((64, 70), (8, 8)),
+                                          ((90, 80), (16, 17)), ((150, 140), (32, 32)),
+                                          ((75, 300), (40, 9)), ((128, 128), (1, 1))])
+def test_direct_path_is_bit_identical(rng, shape, tshape):
+    img = rng.integers(0, 256, shape, dtype=np.uint8)
+    tpl = rng.integers(0, 256, tshape, dtype=np.uint8)
+    for m in M:
+        _bits_equal(match_template(img, tpl, m, "direct"), O.match_template(img, tpl, int(m)))
+
+
+def test_direct_path_f32_images_bit_identical(rng):
+    img = rng.standard_normal((60, 70)).astype(F)
+    tpl = rng.standard_normal((9, 13)).astype(F)
+    for m in M:
+        _bits_equal(match_template(img, tpl, m, "direct"), O.match_template(img, tpl, int(m)))
+
+
+# ---- find_extremes -----------------------------------------------------------------------
+
+def test_find_extremes_kat():
+    img = gray_image([[10, 7, 8, 1], [9, 15, 4, 2]])
+    e = find_extremes(img)
+    assert (e.max_value, e.min_value, e.max_value_location, e.min_value_location) == \
+        (15, 1, (1, 1), (3, 0))
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.float32])
+def test_find_extremes_ties_and_sizes(rng, dtype):
+    for h, w in [(1, 1), (1, 1000), (1000, 1), (37, 29), (1024, 1023), (2017, 2017)]:
+        img = rng.integers(0, 7, (h, w)).astype(dtype)
+        mx, mn, mxl, mnl = O.find_extremes(img)
+        e = find_extremes(img)
+        assert (e.max_value, e.min_value) == (mx, mn)
+        assert (e.max_value_location, e.min_value_location) == (mxl, mnl), (h, w)
+
+
+def test_find_extremes_float_specials():
+    img = np.array([[1.0, -0.0, 0.0, 5.0], [np.nan, -3.0, 5.0, -3.0]], F)
+    e = find_extremes(img)
+    assert (e.max_value, e.max_value_location) == (5.0, (3, 0))
+    assert (e.min_value, e.min_value_location) == (-3.0, (1, 1))
+    img2 = np.array([[0.0, -0.0], [-0.0, 0.0]], F)
+    e2 = find_extremes(img2)
+    r = O.find_extremes(img2)
+    assert e2.min_value_location == r[3] and e2.max_value_location == r[2]
+    assert np.signbit(e2.min_value) == np.signbit(r[1])
+    nan0 = np.array([[np.nan, 1.0], [2.0, -1.0]], F)
+    e3 = find_extremes(nan0)
+    assert np.isnan(e3.max_value) and np.isnan(e3.min_value)
+    assert e3.max_value_location == (0, 0) and e3.min_value_location == (0, 0)
