@@ -1,16 +1,593 @@
-// MFMA correlation engine for match_template (work in progress).
+// MFMA correlation engine for match_template.
+//
+// u8 images: exact integer correlation on v_mfma_i32_32x32x32_i8.
+//   C[y][x] = sum_{dy,dx} I[y+dy][x+dx] T[dy][dx]
+// is computed per 32x32 output tile as
+//   C_tile = sum_dy sum_kb A(dy,kb) x B(dy,kb)
+// where A(dy,kb) is a 32x32 block of the image (rows y0+dy.., columns
+// x0+32kb..), read straight from an LDS copy of the image block, and B(dy,kb)
+// is the Toeplitz matrix of template row dy: B[k][n] = T[dy][32kb + k - n]
+// (zero outside the row). Both operands are offset by -128 to fit i8:
+//   sum I T = sum I'T' + 128 sum_window I + 128 sum T - 128^2 N,
+// with I' = I - 128, T' = T - 128, every term an exact integer. The window
+// sums sum I and sum I^2 come from a separate exact integer pass, so
+//   SumOfSquaredErrors  = sum T^2 - 2 C + sum I^2      (exact)
+//   CrossCorrelation    = C                             (exact)
+// each rounded once to f32, and the normalised methods divide by
+// sqrt(sum I^2 * sum T^2): in f32 like the reference for windows under 64x64
+// pixels, in f64 for windows of at least 64x64.
+//
+// f32 images: BF16x3 on v_mfma_f32_32x32x16_bf16 with FP32 accumulation.
+// Each operand is split x = hi + lo (hi = bf16(x), lo = bf16(x - hi)) and
+//   sum I T ~= sum (I_hi T_hi + I_hi T_lo + I_lo T_hi),
+// with the same image-block x Toeplitz decomposition at K = 16. Template
+// rows are processed in chunks of 32, restaging the image rows of each chunk
+// in LDS, so any template height fits. The window energy sum I^2 is summed
+// in f64.
 #include "match_common.hpp"
 
 namespace iph {
+namespace {
 
-size_t mfma_match_workspace(int32_t, uint32_t, uint32_t, uint32_t, uint32_t, int32_t) {
-    return 256;
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+constexpr int kXT = 4;         // 32-column output tiles per wave
+constexpr int kMaxSide = 128;  // largest template side on the int8 path
+constexpr int kWinRows = 16;   // output rows per window-sum workgroup
+constexpr int kWinCols = 64;   // output columns per window-sum workgroup
+
+__device__ __forceinline__ int align_up_dev(int v) { return (v + 15) & ~15; }
+
+// sum T and sum T^2 of the template (exact, 64-bit).
+__global__ __launch_bounds__(256) void template_sums_kernel(const uint8_t *__restrict__ tpl,
+                                                            int n, long long *__restrict__ out) {
+    __shared__ long long s1[256], s2[256];
+    long long a = 0, b = 0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        long long t = tpl[i];
+        a += t;
+        b += t * t;
+    }
+    s1[threadIdx.x] = a;
+    s2[threadIdx.x] = b;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            s1[threadIdx.x] += s1[threadIdx.x + s];
+            s2[threadIdx.x] += s2[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = s1[0];
+        out[1] = s2[0];
+    }
 }
 
-iph_status launch_mfma_match(const void *, int32_t, uint32_t, uint32_t, const void *, uint32_t,
-                             uint32_t, int32_t, float *, void *, size_t, hipStream_t) {
-    set_error("the MFMA correlation path is not built yet");
-    return IPH_EUNSUPPORTED;
+// Toeplitz operands in MFMA lane order: entry [dy][kb][lane] holds the 16
+// bytes lane `lane` feeds as B: column n = lane % 32, rows k = 16*(lane/32)+j.
+__global__ __launch_bounds__(64) void toeplitz_i8_kernel(const uint8_t *__restrict__ tpl, int tw,
+                                                         int KB, int8_t *__restrict__ bsw) {
+    const int dy = blockIdx.y, kb = blockIdx.x, lane = threadIdx.x;
+    const int n = lane & 31, g = lane >> 5;
+    int8_t *dst = bsw + ((size_t)(dy * KB + kb) * 64 + lane) * 16;
+    for (int j = 0; j < 16; j++) {
+        int dx = 32 * kb + 16 * g + j - n;
+        dst[j] = (dx >= 0 && dx < tw) ? (int8_t)((int)tpl[dy * tw + dx] - 128) : (int8_t)0;
+    }
+}
+
+// Exact window sums (sum I, sum I^2) for every output position: vertical
+// sliding sums over th rows into LDS, then horizontal sliding sums over tw.
+__global__ __launch_bounds__(256) void window_sums_u8_kernel(const uint8_t *__restrict__ img,
+                                                             int W, int tw, int th, int ow,
+                                                             int oh, uint2 *__restrict__ ws) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int x0 = blockIdx.x * kWinCols, y0 = blockIdx.y * kWinRows;
+    const int cols = kWinCols + tw - 1, rows = kWinRows + th - 1;
+    uint8_t *blk = smem;                                             // rows x cols
+    uint2 *v = reinterpret_cast<uint2 *>(smem + align_up_dev(rows * cols));  // kWinRows x cols
+    const int H = oh + th - 1;
+    for (int i = threadIdx.x; i < rows * cols; i += 256) {
+        int r = i / cols, c = i - r * cols;
+        int gy = y0 + r, gx = x0 + c;
+        blk[i] = (gy < H && gx < W) ? img[(size_t)gy * W + gx] : 0;
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < cols; c += 256) {
+        uint32_t a = 0, b = 0;
+        for (int r = 0; r < th; r++) {
+            uint32_t p = blk[r * cols + c];
+            a += p;
+            b += p * p;
+        }
+        v[c] = make_uint2(a, b);
+        for (int y = 1; y < kWinRows; y++) {
+            uint32_t pin = blk[(y + th - 1) * cols + c], pout = blk[(y - 1) * cols + c];
+            a += pin - pout;
+            b += pin * pin - pout * pout;
+            v[y * cols + c] = make_uint2(a, b);
+        }
+    }
+    __syncthreads();
+    const int y = threadIdx.x >> 4, seg = threadIdx.x & 15;  // 16 rows x 16 segments of 4
+    const int gy = y0 + y;
+    if (gy >= oh) return;
+    const uint2 *vr = v + y * cols;
+    int c0 = seg * 4;
+    uint32_t a = 0, b = 0;
+    for (int dx = 0; dx < tw; dx++) {
+        a += vr[c0 + dx].x;
+        b += vr[c0 + dx].y;
+    }
+    for (int k = 0; k < 4; k++) {
+        int gx = x0 + c0 + k;
+        if (gx < ow) ws[(size_t)gy * ow + gx] = make_uint2(a, b);
+        if (k < 3) {
+            a += vr[c0 + k + tw].x - vr[c0 + k].x;
+            b += vr[c0 + k + tw].y - vr[c0 + k].y;
+        }
+    }
+}
+
+template <int METHOD>
+__device__ __forceinline__ float finish_score(long long c, long long sii, long long stt,
+                                              long long n) {
+    const bool sse = (METHOD == IPH_SSE || METHOD == IPH_SSE_NORMED);
+    long long num = sse ? (stt - 2 * c + sii) : c;
+    if (METHOD == IPH_SSE || METHOD == IPH_CCORR) return (float)num;
+    if (n < 64 * 64) {
+        // the reference's normalisation in f32 (template_matching.rs:249-250)
+        float norm = sqrtf((float)sii * (float)stt);
+        float s = (float)num;
+        return norm > 0.0f ? s / norm : s;
+    }
+    double norm = sqrt((double)sii * (double)stt);
+    return norm > 0.0 ? (float)((double)num / norm) : (float)num;
+}
+
+// One wave computes a 32 x (32*kXT) output strip; RT waves stack vertically.
+template <int KB, int METHOD>
+__global__ __launch_bounds__(256) void match_i8_mfma_kernel(
+    const uint8_t *__restrict__ img, int W, int H, const v4i *__restrict__ bsw,
+    const uint2 *__restrict__ wsum, const long long *__restrict__ tsums,
+    float *__restrict__ out, int ow, int oh, int tw, int th, int pitch) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    int8_t *blk = reinterpret_cast<int8_t *>(smem);
+    constexpr int BW = 32 * (kXT + KB - 1);  // bytes per staged row
+    const int RT = blockDim.x >> 6;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int x0 = blockIdx.x * (32 * kXT), y0 = blockIdx.y * 32 * RT;
+    const int rows = 32 * RT + th - 1;
+
+    // Stage image rows y0.. and columns x0.. as i8 = u8 - 128 (0 outside).
+    for (int i = threadIdx.x; i < rows * (BW / 4); i += blockDim.x) {
+        int r = i / (BW / 4), c = (i - r * (BW / 4)) * 4;
+        int gy = y0 + r, gx = x0 + c;
+        uint32_t v = 0x80808080u;
+        if (gy < H) {
+            const uint8_t *p = img + (size_t)gy * W + gx;
+            if (gx + 3 < W && (reinterpret_cast<uintptr_t>(p) & 3) == 0) {
+                v = *reinterpret_cast<const uint32_t *>(p);
+            } else {
+                uint32_t b0 = gx < W ? p[0] : 0x80u;
+                uint32_t b1 = gx + 1 < W ? p[1] : 0x80u;
+                uint32_t b2 = gx + 2 < W ? p[2] : 0x80u;
+                uint32_t b3 = gx + 3 < W ? p[3] : 0x80u;
+                v = b0 | (b1 << 8) | (b2 << 16) | (b3 << 24);
+            }
+        }
+        *reinterpret_cast<uint32_t *>(blk + r * pitch + c) = v ^ 0x80808080u;
+    }
+    __syncthreads();
+
+    const int n = lane & 31, g = lane >> 5;
+    v16i acc[kXT];
+#pragma unroll
+    for (int t = 0; t < kXT; t++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) acc[t][r] = 0;
+
+    const int8_t *arow = blk + (wave * 32 + n) * pitch + 16 * g;
+    const v4i *bp = bsw + lane;
+    for (int dy = 0; dy < th; dy++) {
+        v4i b[KB];
+#pragma unroll
+        for (int kb = 0; kb < KB; kb++) b[kb] = bp[(dy * KB + kb) * 64];
+#pragma unroll
+        for (int s = 0; s < kXT + KB - 1; s++) {
+            v4i a = *reinterpret_cast<const v4i *>(arow + 32 * s);
+#pragma unroll
+            for (int t = 0; t < kXT; t++) {
+                if (s - t >= 0 && s - t < KB)
+                    acc[t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b[s - t], acc[t], 0, 0, 0);
+            }
+        }
+        arow += pitch;
+    }
+
+    const long long st = tsums[0], stt = tsums[1];
+    const long long nn = (long long)tw * th;
+    const long long base = 128LL * st - 16384LL * nn;
+#pragma unroll
+    for (int t = 0; t < kXT; t++) {
+        const int x = x0 + 32 * t + n;
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            // 32x32 accumulator layout: register r of lane l holds row
+            // 8*(r/4) + 4*(l/32) + r%4, column l%32.
+            const int y = y0 + wave * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
+            if (x < ow && y < oh) {
+                uint2 s = wsum[(size_t)y * ow + x];
+                long long c = (long long)acc[t][r] + 128LL * s.x + base;
+                out[(size_t)y * ow + x] = finish_score<METHOD>(c, (long long)s.y, stt, nn);
+            }
+        }
+    }
+}
+
+
+// ---- BF16x3 path for f32 images --------------------------------------------------
+
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+constexpr int kXT16 = 2;     // 32-column output tiles per wave (f32 images)
+constexpr int kDyChunk = 32; // template rows per LDS staging round
+
+// Round-to-nearest-even f32 -> bf16 bits (finite inputs).
+__device__ __forceinline__ uint16_t bf16_bits(float x) {
+    uint32_t u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_value(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// hi/lo split of x: hi = bf16(x), lo = bf16(x - hi).
+__device__ __forceinline__ void split_bf16(float x, uint16_t &hi, uint16_t &lo) {
+    hi = bf16_bits(x);
+    lo = bf16_bits(x - bf16_value(hi));
+}
+
+// sum T^2 of the template in f64 (single block tree reduction).
+__global__ __launch_bounds__(256) void template_sq_f32_kernel(const float *__restrict__ tpl, int n,
+                                                              double *__restrict__ out) {
+    __shared__ double s[256];
+    double a = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) a += (double)tpl[i] * (double)tpl[i];
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) s[threadIdx.x] += s[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = s[0];
+}
+
+// Toeplitz operands at K = 16 in MFMA lane order, hi and lo parts:
+// entry [dy][kb][part][lane] holds 8 bf16: column n = lane % 32,
+// rows k = 8*(lane/32) + j, value T[dy][16kb + k - n].
+__global__ __launch_bounds__(64) void toeplitz_bf16_kernel(const float *__restrict__ tpl, int tw,
+                                                           int KB, uint16_t *__restrict__ bsw) {
+    const int dy = blockIdx.y, kb = blockIdx.x, lane = threadIdx.x;
+    const int n = lane & 31, g = lane >> 5;
+    uint16_t *hi = bsw + (((size_t)(dy * KB + kb) * 2 + 0) * 64 + lane) * 8;
+    uint16_t *lo = bsw + (((size_t)(dy * KB + kb) * 2 + 1) * 64 + lane) * 8;
+    for (int j = 0; j < 8; j++) {
+        int dx = 16 * kb + 8 * g + j - n;
+        uint16_t h = 0, l = 0;
+        if (dx >= 0 && dx < tw) split_bf16(tpl[dy * tw + dx], h, l);
+        hi[j] = h;
+        lo[j] = l;
+    }
+}
+
+// Window energy sum I^2 in f64 for every output: vertical sums over th rows,
+// then horizontal sums over tw, each evaluated directly (no running
+// differences, so no cancellation).
+__global__ __launch_bounds__(256) void window_sq_f32_kernel(const float *__restrict__ img, int W,
+                                                            int tw, int th, int ow, int oh,
+                                                            double *__restrict__ ws) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double *v = reinterpret_cast<double *>(smem);  // kWinRows x cols
+    const int x0 = blockIdx.x * kWinCols, y0 = blockIdx.y * kWinRows;
+    const int cols = kWinCols + tw - 1;
+    const int H = oh + th - 1;
+    for (int i = threadIdx.x; i < kWinRows * cols; i += 256) {
+        int y = i / cols, c = i - y * cols;
+        int gx = x0 + c;
+        double a = 0.0;
+        if (gx < W) {
+            for (int r = 0; r < th; r++) {
+                int gy = y0 + y + r;
+                if (gy < H) {
+                    double p = img[(size_t)gy * W + gx];
+                    a += p * p;
+                }
+            }
+        }
+        v[i] = a;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kWinRows * kWinCols; i += 256) {
+        int y = i / kWinCols, c = i - y * kWinCols;
+        int gy = y0 + y, gx = x0 + c;
+        if (gy >= oh || gx >= ow) continue;
+        const double *vr = v + y * cols + c;
+        double a = 0.0;
+        for (int dx = 0; dx < tw; dx++) a += vr[dx];
+        ws[(size_t)gy * ow + gx] = a;
+    }
+}
+
+template <int METHOD>
+__device__ __forceinline__ float finish_score_f32(float c, double sii, double stt, long long n) {
+    const bool sse = (METHOD == IPH_SSE || METHOD == IPH_SSE_NORMED);
+    if (METHOD == IPH_CCORR) return c;
+    double num = sse ? (stt - 2.0 * (double)c + sii) : (double)c;
+    if (METHOD == IPH_SSE) return (float)num;
+    if (n < 64 * 64) {
+        float norm = sqrtf((float)sii * (float)stt);
+        float s = (float)num;
+        return norm > 0.0f ? s / norm : s;
+    }
+    double norm = sqrt(sii * stt);
+    return norm > 0.0 ? (float)(num / norm) : (float)num;
+}
+
+// One wave computes a 32 x (32*kXT16) output strip; RT waves stack vertically.
+// LDS holds the (32*RT + kDyChunk - 1) image rows of one template-row chunk,
+// split into bf16 hi and lo planes.
+template <int KB, int METHOD>
+__global__ __launch_bounds__(128) void match_bf16x3_mfma_kernel(
+    const float *__restrict__ img, int W, int H, const v8bf *__restrict__ bsw,
+    const double *__restrict__ wsq, const double *__restrict__ tsq, float *__restrict__ out,
+    int ow, int oh, int tw, int th, int pitch) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int BW = 16 * (2 * kXT16 + KB - 1);  // staged columns
+    const int RT = blockDim.x >> 6;
+    const int rows = 32 * RT + kDyChunk - 1;
+    uint16_t *hi_s = reinterpret_cast<uint16_t *>(smem);
+    uint16_t *lo_s = hi_s + (size_t)rows * pitch;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int n = lane & 31, g = lane >> 5;
+    const int x0 = blockIdx.x * (32 * kXT16), y0 = blockIdx.y * 32 * RT;
+
+    v16f acc[kXT16];
+#pragma unroll
+    for (int t = 0; t < kXT16; t++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) acc[t][r] = 0.0f;
+
+    const v8bf *bp = bsw + lane;
+    for (int c0 = 0; c0 < th; c0 += kDyChunk) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < rows * BW; i += blockDim.x) {
+            int r = i / BW, c = i - r * BW;
+            int gy = y0 + c0 + r, gx = x0 + c;
+            float v = (gy < H && gx < W) ? img[(size_t)gy * W + gx] : 0.0f;
+            uint16_t h, l;
+            split_bf16(v, h, l);
+            hi_s[r * pitch + c] = h;
+            lo_s[r * pitch + c] = l;
+        }
+        __syncthreads();
+        const int dy_end = min(kDyChunk, th - c0);
+        const uint16_t *ah = hi_s + (wave * 32 + n) * pitch + 8 * g;
+        const uint16_t *al = lo_s + (wave * 32 + n) * pitch + 8 * g;
+        for (int d = 0; d < dy_end; d++) {
+            const int dy = c0 + d;
+            v8bf bh[KB], bl[KB];
+#pragma unroll
+            for (int kb = 0; kb < KB; kb++) {
+                bh[kb] = bp[((size_t)(dy * KB + kb) * 2 + 0) * 64];
+                bl[kb] = bp[((size_t)(dy * KB + kb) * 2 + 1) * 64];
+            }
+#pragma unroll
+            for (int s = 0; s < 2 * kXT16 + KB - 1; s++) {
+                v8bf a_h = *reinterpret_cast<const v8bf *>(ah + d * pitch + 16 * s);
+                v8bf a_l = *reinterpret_cast<const v8bf *>(al + d * pitch + 16 * s);
+#pragma unroll
+                for (int t = 0; t < kXT16; t++) {
+                    const int kb = s - 2 * t;
+                    if (kb >= 0 && kb < KB) {
+                        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_l, bh[kb], acc[t], 0, 0, 0);
+                        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_h, bl[kb], acc[t], 0, 0, 0);
+                        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_h, bh[kb], acc[t], 0, 0, 0);
+                    }
+                }
+            }
+        }
+    }
+
+    const double stt = tsq[0];
+    const long long nn = (long long)tw * th;
+#pragma unroll
+    for (int t = 0; t < kXT16; t++) {
+        const int x = x0 + 32 * t + n;
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int y = y0 + wave * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
+            if (x < ow && y < oh) {
+                double sii = (METHOD == IPH_CCORR) ? 0.0 : wsq[(size_t)y * ow + x];
+                out[(size_t)y * ow + x] = finish_score_f32<METHOD>(acc[t][r], sii, stt, nn);
+            }
+        }
+    }
+}
+
+template <int KB>
+iph_status launch_bf16_kb(const float *img, int W, int H, const uint16_t *bsw, const double *wsq,
+                          const double *tsq, float *out, int ow, int oh, int tw, int th,
+                          int method, hipStream_t s) {
+    constexpr int BW = 16 * (2 * kXT16 + KB - 1);
+    int pitch = ((BW / 8) % 2 == 0) ? BW + 8 : BW;  // odd multiple of 16 bytes
+    int RT = 2;
+    while (RT > 1 && (size_t)(32 * RT + kDyChunk - 1) * pitch * 4 > 64 * 1024) RT /= 2;
+    size_t lds = (size_t)(32 * RT + kDyChunk - 1) * pitch * 4;
+    dim3 grid((ow + 32 * kXT16 - 1) / (32 * kXT16), (oh + 32 * RT - 1) / (32 * RT));
+    const v8bf *b = reinterpret_cast<const v8bf *>(bsw);
+#define L(M)                                                                                 \
+    match_bf16x3_mfma_kernel<KB, M><<<grid, 64 * RT, lds, s>>>(img, W, H, b, wsq, tsq, out,   \
+                                                               ow, oh, tw, th, pitch)
+    switch (method) {
+    case IPH_SSE: L(IPH_SSE); break;
+    case IPH_SSE_NORMED: L(IPH_SSE_NORMED); break;
+    case IPH_CCORR: L(IPH_CCORR); break;
+    default: L(IPH_CCORR_NORMED); break;
+    }
+#undef L
+    return launch_status("match_bf16x3_mfma_kernel");
+}
+
+constexpr int kMaxSideF32 = 128;
+
+int blocks16_for(int tw) { return (31 + tw + 15) / 16; }
+
+int blocks_for(int tw) { return (31 + tw + 31) / 32; }
+
+size_t win_lds(int tw, int th) {
+    int cols = kWinCols + tw - 1, rows = kWinRows + th - 1;
+    return align_up((size_t)rows * cols, 16) + (size_t)kWinRows * cols * sizeof(uint2);
+}
+
+template <int KB>
+iph_status launch_i8_kb(const uint8_t *img, int W, int H, const int8_t *bsw, const uint2 *wsum,
+                        const long long *tsums, float *out, int ow, int oh, int tw, int th,
+                        int method, hipStream_t s) {
+    constexpr int BW = 32 * (kXT + KB - 1);
+    int pitch = ((BW / 16) % 2 == 0) ? BW + 16 : BW;  // odd multiple of 16: no bank conflicts
+    int RT = 4;
+    while (RT > 1 && (size_t)(32 * RT + th - 1) * pitch > 64 * 1024) RT /= 2;
+    size_t lds = (size_t)(32 * RT + th - 1) * pitch;
+    dim3 grid((ow + 32 * kXT - 1) / (32 * kXT), (oh + 32 * RT - 1) / (32 * RT));
+    const v4i *b = reinterpret_cast<const v4i *>(bsw);
+#define L(M)                                                                                 \
+    match_i8_mfma_kernel<KB, M><<<grid, 64 * RT, lds, s>>>(img, W, H, b, wsum, tsums, out, ow, \
+                                                           oh, tw, th, pitch)
+    switch (method) {
+    case IPH_SSE: L(IPH_SSE); break;
+    case IPH_SSE_NORMED: L(IPH_SSE_NORMED); break;
+    case IPH_CCORR: L(IPH_CCORR); break;
+    default: L(IPH_CCORR_NORMED); break;
+    }
+#undef L
+    return launch_status("match_i8_mfma_kernel");
+}
+
+struct U8Plan {
+    int KB;
+    size_t bsw_bytes, win_bytes;
+};
+
+U8Plan plan_u8(uint32_t w, uint32_t h, uint32_t tw, uint32_t th) {
+    U8Plan p;
+    p.KB = blocks_for((int)tw);
+    p.bsw_bytes = (size_t)th * p.KB * 1024;
+    p.win_bytes = (size_t)(w - tw + 1) * (h - th + 1) * sizeof(uint2);
+    return p;
+}
+
+}  // namespace
+
+iph_status launch_bf16x3_match(const float *img, uint32_t w, uint32_t h, const float *tpl,
+                               uint32_t tw, uint32_t th, int32_t method, float *out, void *ws,
+                               size_t ws_bytes, hipStream_t s) {
+    if (tw > (uint32_t)kMaxSideF32 || th > (uint32_t)kMaxSideF32) {
+        set_error("the BF16x3 MFMA path takes templates up to %dx%d; use the direct path",
+                  kMaxSideF32, kMaxSideF32);
+        return IPH_EUNSUPPORTED;
+    }
+    const int KB = blocks16_for((int)tw);
+    Workspace wsp(ws, ws_bytes);
+    double *tsq = wsp.take<double>(1);
+    uint16_t *bsw = wsp.take<uint16_t>((size_t)th * KB * 2 * 64 * 8);
+    double *wsq = wsp.take<double>((size_t)(w - tw + 1) * (h - th + 1));
+    if (!tsq || !bsw || !wsq) {
+        set_error("workspace too small for the BF16x3 MFMA match_template");
+        return IPH_EINVAL;
+    }
+    int ow = (int)(w - tw + 1), oh = (int)(h - th + 1);
+    template_sq_f32_kernel<<<1, 256, 0, s>>>(tpl, (int)(tw * th), tsq);
+    IPH_TRY(launch_status("template_sq_f32_kernel"));
+    toeplitz_bf16_kernel<<<dim3(KB, th), 64, 0, s>>>(tpl, (int)tw, KB, bsw);
+    IPH_TRY(launch_status("toeplitz_bf16_kernel"));
+    if (method != IPH_CCORR) {
+        dim3 wg((ow + kWinCols - 1) / kWinCols, (oh + kWinRows - 1) / kWinRows);
+        size_t lds = (size_t)kWinRows * (kWinCols + tw - 1) * sizeof(double);
+        window_sq_f32_kernel<<<wg, 256, lds, s>>>(img, (int)w, (int)tw, (int)th, ow, oh, wsq);
+        IPH_TRY(launch_status("window_sq_f32_kernel"));
+    }
+    switch (KB) {
+#define C(K)                                                                                 \
+    case K:                                                                                  \
+        return launch_bf16_kb<K>(img, w, h, bsw, wsq, tsq, out, ow, oh, tw, th, method, s);
+        C(2) C(3) C(4) C(5) C(6) C(7) C(8) C(9) C(10)
+#undef C
+    default:
+        set_error("unsupported template width %u", tw);
+        return IPH_EUNSUPPORTED;
+    }
+}
+
+size_t mfma_match_workspace(int32_t p_type, uint32_t w, uint32_t h, uint32_t tw, uint32_t th,
+                            int32_t) {
+    if (tw == 0 || th == 0 || w < tw || h < th) return 256;
+    if (p_type == IPH_F32) {
+        WorkspacePlan wp;
+        wp.take<double>(1);
+        wp.take<uint16_t>((size_t)th * blocks16_for((int)tw) * 2 * 64 * 8);
+        wp.take<double>((size_t)(w - tw + 1) * (h - th + 1));
+        return wp.used;
+    }
+    U8Plan p = plan_u8(w, h, tw, th);
+    WorkspacePlan wp;
+    wp.take<long long>(2);
+    wp.take<int8_t>(p.bsw_bytes);
+    wp.take<uint2>((size_t)(w - tw + 1) * (h - th + 1));
+    return wp.used;
+}
+
+iph_status launch_mfma_match(const void *img, int32_t p_type, uint32_t w, uint32_t h,
+                             const void *tpl, uint32_t tw, uint32_t th, int32_t method,
+                             float *out, void *ws, size_t ws_bytes, hipStream_t s) {
+    if (p_type == IPH_F32)
+        return launch_bf16x3_match(static_cast<const float *>(img), w, h,
+                                   static_cast<const float *>(tpl), tw, th, method, out, ws,
+                                   ws_bytes, s);
+    if (tw > (uint32_t)kMaxSide || th > (uint32_t)kMaxSide) {
+        set_error("the int8 MFMA path takes templates up to %dx%d; use the direct path",
+                  kMaxSide, kMaxSide);
+        return IPH_EUNSUPPORTED;
+    }
+    U8Plan p = plan_u8(w, h, tw, th);
+    Workspace wsp(ws, ws_bytes);
+    long long *tsums = wsp.take<long long>(2);
+    int8_t *bsw = wsp.take<int8_t>(p.bsw_bytes);
+    uint2 *wsum = wsp.take<uint2>((size_t)(w - tw + 1) * (h - th + 1));
+    if (!tsums || !bsw || !wsum) {
+        set_error("workspace too small for the int8 MFMA match_template");
+        return IPH_EINVAL;
+    }
+    const uint8_t *im = static_cast<const uint8_t *>(img);
+    const uint8_t *tp = static_cast<const uint8_t *>(tpl);
+    int ow = (int)(w - tw + 1), oh = (int)(h - th + 1);
+    template_sums_kernel<<<1, 256, 0, s>>>(tp, (int)(tw * th), tsums);
+    IPH_TRY(launch_status("template_sums_kernel"));
+    toeplitz_i8_kernel<<<dim3(p.KB, th), 64, 0, s>>>(tp, (int)tw, p.KB, bsw);
+    IPH_TRY(launch_status("toeplitz_i8_kernel"));
+    dim3 wg((ow + kWinCols - 1) / kWinCols, (oh + kWinRows - 1) / kWinRows);
+    window_sums_u8_kernel<<<wg, 256, win_lds((int)tw, (int)th), s>>>(im, (int)w, (int)tw,
+                                                                    (int)th, ow, oh, wsum);
+    IPH_TRY(launch_status("window_sums_u8_kernel"));
+    switch (p.KB) {
+    case 1: return launch_i8_kb<1>(im, w, h, bsw, wsum, tsums, out, ow, oh, tw, th, method, s);
+    case 2: return launch_i8_kb<2>(im, w, h, bsw, wsum, tsums, out, ow, oh, tw, th, method, s);
+    case 3: return launch_i8_kb<3>(im, w, h, bsw, wsum, tsums, out, ow, oh, tw, th, method, s);
+    case 4: return launch_i8_kb<4>(im, w, h, bsw, wsum, tsums, out, ow, oh, tw, th, method, s);
+    default: return launch_i8_kb<5>(im, w, h, bsw, wsum, tsums, out, ow, oh, tw, th, method, s);
+    }
 }
 
 }  // namespace iph

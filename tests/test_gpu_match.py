@@ -107,3 +107,106 @@ def test_find_extremes_float_specials():
     e3 = find_extremes(nan0)
     assert np.isnan(e3.max_value) and np.isnan(e3.min_value)
     assert e3.max_value_location == (0, 0) and e3.min_value_location == (0, 0)
+
+
+# ---- int8 MFMA path (u8 images) ------------------------------------------------------
+
+def _check_mfma_u8(img, tpl):
+    n = tpl.size
+    for m in M:
+        got = match_template(img, tpl, m, "mfma")
+        exact = O.match_template_f64(img, tpl, int(m))
+        if m in (M.SumOfSquaredErrors, M.CrossCorrelation):
+            # exact integer sums rounded once to f32
+            _bits_equal(got, exact.astype(F))
+        else:
+            rel = np.abs(got - exact) / np.maximum(np.abs(exact), 1e-30)
+            # f32 normalisation below 64x64 windows (~4 ulp), f64 at or above (<= 0.5 ulp)
+            assert rel.max() <= (4.0 if n < 4096 else 0.5) * 2.0 ** -23 + 1e-30, (m, rel.max())
+        if n * 65025 < 2 ** 24:
+            # the reference's f32 sums are exact here: bit-identical to it
+            _bits_equal(got, O.match_template(img, tpl, int(m)))
+
+
+@pytest.mark.parametrize("shape,tshape", [((40, 40), (1, 1)), ((50, 61), (3, 5)),
+                                          ((100, 130), (16, 16)), ((161, 200), (32, 32)),
+                                          ((97, 97), (33, 31)), ((150, 300), (8, 96)),
+                                          ((260, 190), (96, 64)), ((300, 310), (128, 128)),
+                                          ((70, 70), (64, 70))])
+def test_mfma_u8_random(rng, shape, tshape):
+    img = rng.integers(0, 256, shape, dtype=np.uint8)
+    tpl = rng.integers(0, 256, tshape, dtype=np.uint8)
+    _check_mfma_u8(img, tpl)
+
+
+def test_mfma_u8_saturated_and_constant():
+    # extreme operands for the -128 offset algebra: all 255 / all 0 / mixed
+    for iv, tv in [(255, 255), (0, 255), (255, 0), (0, 0), (128, 127)]:
+        img = np.full((70, 90), iv, np.uint8)
+        tpl = np.full((32, 32), tv, np.uint8)
+        _check_mfma_u8(img, tpl)
+
+
+def test_mfma_u8_unaligned_widths(rng):
+    for w in (33, 129, 130, 131):
+        img = rng.integers(0, 256, (45, w), dtype=np.uint8)
+        tpl = rng.integers(0, 256, (7, 9), dtype=np.uint8)
+        _check_mfma_u8(img, tpl)
+
+
+def test_mfma_u8_config_2048_argmax_matches_direct():
+    import bench
+    img, tpl, loc = bench.match_workload()
+    for m in M:
+        a = match_template(img, tpl, m, "mfma")
+        d = match_template(img, tpl, m, "direct")
+        pick = np.argmin if m in (M.SumOfSquaredErrors, M.SumOfSquaredErrorsNormalized) else np.argmax
+        assert pick(a) == pick(d), m
+    cc = match_template(img, tpl, M.CrossCorrelationNormalized, "mfma")
+    assert tuple(np.unravel_index(np.argmax(cc), cc.shape)[::-1]) == loc
+
+
+# ---- BF16x3 MFMA path (f32 images) -----------------------------------------------------
+
+def _check_bf16x3(img, tpl, ccn_tol=1e-4):
+    """BF16x3 keeps ~16 significant bits per product: |C - C_exact| is bounded
+    by 2^-14 * sum|I||T|, so the normalised correlation is within ~1e-4 of
+    FP64 (Cauchy-Schwarz), and SSE within twice the correlation bound."""
+    from numpy.lib.stride_tricks import sliding_window_view
+    win = sliding_window_view(np.abs(img.astype(np.float64)), tpl.shape)
+    bound = np.einsum("yxij,ij->yx", win, np.abs(tpl.astype(np.float64))) * 2.0 ** -14 + 1e-6
+    for m in M:
+        got = match_template(img, tpl, m, "mfma").astype(np.float64)
+        exact = O.match_template_f64(img, tpl, int(m))
+        err = np.abs(got - exact)
+        if m == M.CrossCorrelation:
+            assert np.all(err <= bound), (m, (err / bound).max())
+        elif m == M.SumOfSquaredErrors:
+            assert np.all(err <= 2 * bound + 1e-6 * np.abs(exact)), (m, (err / bound).max())
+        elif m == M.CrossCorrelationNormalized:
+            assert err.max() <= ccn_tol, (m, err.max())
+        else:
+            sse_bound = 2 * bound + 1e-6 * np.abs(O.match_template_f64(img, tpl, 0))
+            norm = np.abs(O.match_template_f64(img, tpl, 0)) / np.maximum(np.abs(exact), 1e-30)
+            assert np.all(err <= sse_bound / np.maximum(norm, 1e-30) + 1e-6), m
+
+
+@pytest.mark.parametrize("shape,tshape", [((40, 50), (3, 3)), ((90, 100), (16, 16)),
+                                          ((130, 120), (33, 65)), ((200, 210), (64, 64)),
+                                          ((300, 290), (128, 128)), ((100, 180), (20, 100))])
+def test_bf16x3_f32_random(rng, shape, tshape):
+    img = rng.standard_normal(shape).astype(F)
+    tpl = rng.standard_normal(tshape).astype(F)
+    _check_bf16x3(img, tpl)
+
+
+def test_bf16x3_argmax_matches_direct_on_smooth_field(rng):
+    # smoothed N(0,1) field, 64x64 template cropped and perturbed by N(0, 0.05)
+    from imageproc_amd import filter as flt
+    img = flt.gaussian_blur_f32(rng.standard_normal((512, 512)).astype(F), 3.0)
+    y, x = 200, 137
+    tpl = (img[y:y + 64, x:x + 64] + 0.05 * rng.standard_normal((64, 64))).astype(F)
+    a = match_template(img, tpl, M.CrossCorrelationNormalized, "mfma")
+    d = match_template(img, tpl, M.CrossCorrelationNormalized, "direct")
+    assert np.argmax(a) == np.argmax(d)
+    assert np.unravel_index(np.argmax(a), a.shape) == (y, x)
