@@ -117,6 +117,13 @@ def lib() -> C.CDLL:
             raise ImportError(
                 f"{LIB_PATH} is not built; run `python -m imageproc_amd.build` "
                 "(imageproc_amd has no CPU implementation)")
+        # torch ships its own libamdhip64.so.7. Loading it first makes this
+        # library bind to the same HIP runtime (same soname), so one process
+        # never holds two runtimes that would contend for the device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         handle = C.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             fn = getattr(handle, name)

@@ -1,0 +1,504 @@
+// Separable filter for GrayImage (u8 pixels, f32 taps) on f16 MFMA, with the
+// reference's exact result (filter/mod.rs:228-429).
+//
+// The reference computes, per output, the sequential non-fused f32 chain
+//   R = (((0 + p0 w0) + p1 w1) + ...),  out = R < 255 ? (R > 0 ? trunc(R) : 0) : 255
+// once along rows into a u8 intermediate and once along columns of it. Only
+// trunc(R) matters, so this kernel
+//   1. estimates every sum E on v_mfma_f32_32x32x16_f16: pixels (0..255) are
+//      exact in f16, each tap is split w*2^S = hi + lo in f16 (22 bits), and
+//      the 1-D correlation of a 32x32 block is A(32x(16*NKB)) x B, B the
+//      Toeplitz matrix B[k][n] = w[k - n];
+//   2. proves trunc(E) = trunc(R) whenever E is farther than eps from an
+//      integer, eps >= |E - R| (bound derived in blur_eps below);
+//   3. recomputes the few outputs within eps of an integer with the
+//      reference's sequential chain (separately rounded __fmul_rn /
+//      __fadd_rn) from the same LDS data, and overwrites them.
+// Both passes run in one workgroup: the horizontal pass writes the u8
+// intermediate (as exact f16 integers, transposed) to LDS, the vertical pass
+// reads it back as its A operand.
+#include "iph_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace iph {
+namespace {
+
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+typedef _Float16 v2h __attribute__((ext_vector_type(2)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+constexpr int kTW = 128;       // output columns per workgroup (4 column blocks)
+constexpr int kTH = 96;        // output rows per workgroup (3 row blocks)
+constexpr int kMaxTaps = 48;   // NKB = ceil((31 + L) / 16) <= 5
+
+struct BlurArgs {
+    float h[kMaxTaps];
+    float v[kMaxTaps];
+    float scale;      // 2^S applied to the taps fed to MFMA
+    float inv_scale;  // 2^-S
+    float eps_h, eps_v;
+    int L;
+    int m;  // (x0 - L/2) mod 4: the window's staged columns start 4-aligned
+};
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+
+constexpr int kTapPad = 40;  // zeros on both sides of the taps in LDS
+
+// Toeplitz B operand for this lane: column n = lane % 32, rows
+// k = 16*kb + 8*(lane/32) + j, value w[k - n - shift] * 2^S split into hi + lo
+// (shift: offset of the first tap's input column inside the staged window).
+// `taps_s` holds the scaled taps at [kTapPad, kTapPad + L) and zeros around.
+template <int NKB>
+__device__ __forceinline__ void build_b(const float *taps_s, int lane, int shift,
+                                        v8h (&bhi)[NKB], v8h (&blo)[NKB]) {
+    const int n = lane & 31, g = lane >> 5;
+#pragma unroll
+    for (int kb = 0; kb < NKB; kb++) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            float w = taps_s[kTapPad + 16 * kb + 8 * g + j - n - shift];
+            _Float16 hi = (_Float16)w;
+            _Float16 lo = (_Float16)(w - (float)hi);
+            bhi[kb][j] = hi;
+            blo[kb][j] = lo;
+        }
+    }
+}
+
+// 8 u8 pixels -> 8 exact f16: (0x6400 | p) is 1024 + p in f16.
+__device__ __forceinline__ v8h bytes_to_half8(uint2 b) {
+    const uint32_t magic = 0x64646464u;
+    uint32_t q0 = __builtin_amdgcn_perm(magic, b.x, 0x04010400u);
+    uint32_t q1 = __builtin_amdgcn_perm(magic, b.x, 0x04030402u);
+    uint32_t q2 = __builtin_amdgcn_perm(magic, b.y, 0x04010400u);
+    uint32_t q3 = __builtin_amdgcn_perm(magic, b.y, 0x04030402u);
+    const v2h off = {(_Float16)-1024.0f, (_Float16)-1024.0f};
+    v2h h0 = __builtin_bit_cast(v2h, q0) + off;
+    v2h h1 = __builtin_bit_cast(v2h, q1) + off;
+    v2h h2 = __builtin_bit_cast(v2h, q2) + off;
+    v2h h3 = __builtin_bit_cast(v2h, q3) + off;
+    v8h r;
+    r[0] = h0[0]; r[1] = h0[1]; r[2] = h1[0]; r[3] = h1[1];
+    r[4] = h2[0]; r[5] = h2[1]; r[6] = h3[0]; r[7] = h3[1];
+    return r;
+}
+
+// trunc/clamp of the reference (Clamp<f32> for u8) applied to an estimate
+// known to be more than eps away from any integer: floor == trunc for t > 0.
+__device__ __forceinline__ float clamp_floor(float t) {
+    return __builtin_amdgcn_fmed3f(floorf(t), 0.0f, 255.0f);
+}
+
+__device__ __forceinline__ bool near_integer(float t, float eps) {
+    float d = t - floorf(t);
+    return fabsf(d - 0.5f) > 0.5f - eps;
+}
+
+__device__ __forceinline__ uint8_t clamp_u8_exact(float x) {
+    return x < 255.0f ? (x > 0.0f ? (uint8_t)(uint32_t)x : (uint8_t)0) : (uint8_t)255;
+}
+
+
+// Flagged-output queue entries per wave; drained whenever more than
+// kQDrain entries wait (one group of 4 accumulator slots adds at most 256).
+constexpr int kQCap = 512;
+constexpr int kQDrain = kQCap - 256;
+
+// Appends this lane's entry to the wave's queue when `flag` is set. `qn` is
+// the wave-uniform queue length. Costs one compare and a scalar branch when
+// no lane is flagged.
+__device__ __forceinline__ void enqueue(bool flag, uint32_t entry, uint32_t *q, int &qn) {
+    const unsigned long long m = __ballot(flag);
+    if (m) {
+        const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        if (flag && qn + below < kQCap) q[qn + below] = entry;
+        qn += __popcll(m);
+    }
+}
+
+// The reference's sequential chain over taps w*2^S (exact: power-of-two
+// scaling commutes with every f32 rounding here), returned unscaled.
+// `stride` steps through the pixels; all loads are issued before the chain.
+template <int LMAX, typename T>
+__device__ __forceinline__ float exact_chain(const T *p, const float *taps_s, int L,
+                                             float inv_scale) {
+    float v[LMAX];
+#pragma unroll
+    for (int q = 0; q < LMAX; q++) v[q] = (float)p[q];
+    float s = 0.0f;
+#pragma unroll
+    for (int q = 0; q < LMAX; q++)
+        if (q < L) s = __fadd_rn(s, __fmul_rn(v[q], taps_s[kTapPad + q]));
+    return s * inv_scale;
+}
+
+// Staged rows per tile: max(kTH + L - 1, 32*(kTH/32 - 1) + 16*NKB) <= 128
+// for L <= 33, rounded up to whole 32-row blocks.
+constexpr int kRows = 128;
+
+template <int NKB> struct BlurShape {
+    static constexpr int CIN = kTW - 32 + 16 * NKB;  // staged input columns
+    static constexpr int C4 = CIN / 4;                 // dwords per staged row
+    static constexpr int NREG = kRows * C4 / 256;      // staged dwords per thread
+    // staged input row pitch: an odd multiple of 2 dwords, so the 32 rows one
+    // 8-byte A-operand read touches fall in distinct LDS banks
+    static constexpr int PIN = ((CIN / 4) % 2 == 0) ? CIN + 8 : CIN + 4;
+    static constexpr int PITCH_T = kRows + 8;          // f16 per intermediate column
+    static constexpr int LMAX = 16 * NKB - 31;         // largest tap count for NKB
+    static_assert(kRows * C4 % 256 == 0, "staging must divide evenly");
+};
+
+// Issues this thread's share of one tile's input-window loads into
+// registers: rows clamp(ys + r), columns xs + c, xs = x0 - L/2 - m a multiple
+// of 4. ALIGNED (rows start 4-byte aligned): every staged dword lies wholly
+// inside or wholly outside its row and is one dword load from the clamped
+// column; the edge byte is replicated later, in store_tile, once the data has
+// arrived (so no load waits for its predecessor). Otherwise each byte is
+// loaded from its clamped column.
+template <int NKB, bool ALIGNED>
+__device__ __forceinline__ void load_tile(const uint8_t *__restrict__ img, int W, int H, int xs,
+                                          int ys, uint32_t (&regs)[BlurShape<NKB>::NREG]) {
+    using S = BlurShape<NKB>;
+#pragma unroll
+    for (int k = 0; k < S::NREG; k++) {
+        const int i = (int)threadIdx.x + 256 * k;
+        const int r = i / S::C4, c = (i - r * S::C4) * 4;
+        const uint8_t *row = img + (size_t)clampi(ys + r, 0, H - 1) * W;
+        const int gx = xs + c;
+        if (ALIGNED) {
+            regs[k] = *reinterpret_cast<const uint32_t *>(row + clampi(gx, 0, W - 4));
+        } else {
+            regs[k] = (uint32_t)row[clampi(gx, 0, W - 1)] |
+                      ((uint32_t)row[clampi(gx + 1, 0, W - 1)] << 8) |
+                      ((uint32_t)row[clampi(gx + 2, 0, W - 1)] << 16) |
+                      ((uint32_t)row[clampi(gx + 3, 0, W - 1)] << 24);
+        }
+    }
+}
+
+// Writes the loaded window to LDS; for ALIGNED loads, dwords left of the row
+// become its first byte replicated and dwords right of it its last byte.
+template <int NKB, bool ALIGNED>
+__device__ __forceinline__ void store_tile(uint8_t *in_s, int W, int xs,
+                                           const uint32_t (&regs)[BlurShape<NKB>::NREG]) {
+    using S = BlurShape<NKB>;
+#pragma unroll
+    for (int k = 0; k < S::NREG; k++) {
+        const int i = (int)threadIdx.x + 256 * k;
+        const int r = i / S::C4, c = (i - r * S::C4) * 4;
+        uint32_t v = regs[k];
+        if (ALIGNED) {
+            const int gx = xs + c;
+            const uint32_t left = (v & 0xffu) * 0x01010101u;
+            const uint32_t right = (v >> 24) * 0x01010101u;
+            v = gx < 0 ? left : v;
+            v = gx >= W ? right : v;
+        }
+        *reinterpret_cast<uint32_t *>(in_s + r * S::PIN + c) = v;
+    }
+}
+
+// Persistent kernel: each workgroup walks a strided list of 128x96 output
+// tiles. Tiles are split into 8 contiguous ranges, one per XCD (workgroups
+// are dispatched round-robin over XCDs), so neighbouring tiles, which share
+// halo rows and columns, meet in the same L2. The next tile's input is
+// loaded into registers while the current tile is computed; the output tile
+// is assembled in LDS and written with 16-byte row-contiguous stores.
+// LDS holds two workgroups per CU (2 waves per SIMD), so the register budget
+// of 2 waves per SIMD is free: B operands and the prefetch stay resident.
+template <int NKB, bool SAME_TAPS, bool ALIGNED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void blur_u8_mfma_kernel(const uint8_t *__restrict__ src,
+                                                           uint8_t *__restrict__ dst, int W,
+                                                           int H, size_t stride, int n_img,
+                                                           int tiles_x, int tiles_y,
+                                                           BlurArgs a) {
+    using S = BlurShape<NKB>;
+    constexpr int PIN = S::PIN, PITCH_T = S::PITCH_T, LMAX = S::LMAX;
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint8_t *in_s = smem;                                                    // kRows x PIN
+    _Float16 *ht = reinterpret_cast<_Float16 *>(smem + (size_t)kRows * PIN);  // kTW x PITCH_T
+    uint8_t *out_s = reinterpret_cast<uint8_t *>(ht + (size_t)kTW * PITCH_T);  // kTH x kTW
+    float *taps_h = reinterpret_cast<float *>(out_s + (size_t)kTH * kTW);
+    float *taps_v = taps_h + (2 * kTapPad + kMaxTaps);
+    uint32_t *queue = reinterpret_cast<uint32_t *>(taps_v + (2 * kTapPad + kMaxTaps)) +
+                      (threadIdx.x >> 6) * kQCap;
+    const int L = a.L;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = lane & 31, g = lane >> 5;
+    const int half = L / 2;
+
+    // scaled taps, zero padded, for branch-free Toeplitz construction
+    if (threadIdx.x < 2 * kTapPad + kMaxTaps) {
+        int t = (int)threadIdx.x - kTapPad;
+        bool in = t >= 0 && t < L;
+        taps_h[threadIdx.x] = in ? a.h[t] * a.scale : 0.0f;
+        taps_v[threadIdx.x] = in ? a.v[t] * a.scale : 0.0f;
+    }
+    __syncthreads();
+    v8h bh_hi[NKB], bh_lo[NKB], bv_hi[NKB], bv_lo[NKB];
+    const int m = a.m;
+    build_b<NKB>(taps_h, lane, m, bh_hi, bh_lo);
+    build_b<NKB>(taps_v, lane, 0, bv_hi, bv_lo);
+    const float *vts = SAME_TAPS ? taps_h : taps_v;
+
+    const int per_img = tiles_x * tiles_y;
+    const int ntiles = n_img * per_img;
+    const int per_xcd = (int)gridDim.x >> 3;
+    const int chunk = (ntiles + 7) >> 3;
+    const int t_begin = (int)(blockIdx.x & 7) * chunk;
+    const int t_end = min(ntiles, t_begin + chunk);
+    int t = t_begin + (int)(blockIdx.x >> 3);
+
+    uint32_t regs[S::NREG];
+    if (t < t_end) {
+        int im = t / per_img, rem = t - im * per_img, ty = rem / tiles_x;
+        load_tile<NKB, ALIGNED>(src + (size_t)im * stride, W, H,
+                                (rem - ty * tiles_x) * kTW - half - m, ty * kTH - half, regs);
+    }
+    for (; t < t_end; t += per_xcd) {
+        const int im = t / per_img, rem = t - im * per_img, ty = rem / tiles_x;
+        const int x0 = (rem - ty * tiles_x) * kTW, y0 = ty * kTH;
+        uint8_t *out = dst + (size_t)im * stride;
+
+        __syncthreads();  // the previous tile is done with in_s, ht and out_s
+        store_tile<NKB, ALIGNED>(in_s, W, x0 - half - m, regs);
+        const int tn = t + per_xcd;
+        if (tn < t_end) {
+            int im2 = tn / per_img, rem2 = tn - im2 * per_img, ty2 = rem2 / tiles_x;
+            load_tile<NKB, ALIGNED>(src + (size_t)im2 * stride, W, H,
+                                    (rem2 - ty2 * tiles_x) * kTW - half - m, ty2 * kTH - half, regs);
+        }
+        __syncthreads();
+
+        // ---- horizontal pass: (kRows/32) x 4 blocks of 32x32 -> ht (transposed f16)
+        int qn = 0;
+        for (int blk = wave; blk < (kRows / 32) * (kTW / 32); blk += 4) {
+            const int rb = blk >> 2, cb = blk & 3;
+            v16f acc;
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[r] = 0.0f;
+            const uint8_t *arow = in_s + (32 * rb + n) * PIN + 32 * cb + 8 * g;
+#pragma unroll
+            for (int kb = 0; kb < NKB; kb++) {
+                v8h av = bytes_to_half8(*reinterpret_cast<const uint2 *>(arow + 16 * kb));
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bh_lo[kb], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bh_hi[kb], acc, 0, 0, 0);
+            }
+            // lane holds column n of the block, rows 8*(r/4) + 4*g + r%4
+            _Float16 *col = ht + (size_t)(32 * cb + n) * PITCH_T + 32 * rb + 4 * g;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                v2h p0, p1;
+                float tv[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    tv[i] = acc[4 * j + i] * a.inv_scale;
+                    const int row = 32 * rb + 8 * j + 4 * g + i;
+                    enqueue(near_integer(tv[i], a.eps_h), (uint32_t)(row << 8) | (32 * cb + n),
+                            queue, qn);
+                }
+                p0[0] = (_Float16)clamp_floor(tv[0]);
+                p0[1] = (_Float16)clamp_floor(tv[1]);
+                p1[0] = (_Float16)clamp_floor(tv[2]);
+                p1[1] = (_Float16)clamp_floor(tv[3]);
+                uint2 pk = make_uint2(__builtin_bit_cast(uint32_t, p0),
+                                      __builtin_bit_cast(uint32_t, p1));
+                *reinterpret_cast<uint2 *>(col + 8 * j) = pk;
+                if (qn > kQDrain) {
+                    // exact recomputation of the outputs an integer boundary could flip
+                    for (int e = lane; e < qn; e += 64) {
+                        const uint32_t ent = queue[e];
+                        const int row = (int)(ent >> 8), cl = (int)(ent & 255);
+                        float r = exact_chain<LMAX>(in_s + row * PIN + m + cl, taps_h, L,
+                                                    a.inv_scale);
+                        ht[(size_t)cl * PITCH_T + row] = (_Float16)(float)clamp_u8_exact(r);
+                    }
+                    qn = 0;
+                }
+            }
+        }
+        for (int e = lane; e < qn; e += 64) {
+            const uint32_t ent = queue[e];
+            const int row = (int)(ent >> 8), cl = (int)(ent & 255);
+            float r = exact_chain<LMAX>(in_s + row * PIN + m + cl, taps_h, L, a.inv_scale);
+            ht[(size_t)cl * PITCH_T + row] = (_Float16)(float)clamp_u8_exact(r);
+        }
+        __syncthreads();
+
+        // ---- vertical pass: D[m = x][n = y] = sum_k H^T[x][k] B[k][y] -> out_s
+        qn = 0;
+        for (int blk = wave; blk < (kTH / 32) * (kTW / 32); blk += 4) {
+            const int rb = blk >> 2, cb = blk & 3;
+            v16f acc;
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[r] = 0.0f;
+            const _Float16 *acol = ht + (size_t)(32 * cb + n) * PITCH_T + 32 * rb + 8 * g;
+#pragma unroll
+            for (int kb = 0; kb < NKB; kb++) {
+                v8h av = *reinterpret_cast<const v8h *>(acol + 16 * kb);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv_lo[kb], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv_hi[kb], acc, 0, 0, 0);
+            }
+            // lane holds output row y = 32*rb + n, columns x = 32*cb + 8*(r/4) + 4*g + r%4
+            const int yl = 32 * rb + n;
+            uint8_t *orow = out_s + yl * kTW;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                uint32_t word = 0;
+                const int xl = 32 * cb + 8 * j + 4 * g;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    float tv = acc[4 * j + i] * a.inv_scale;
+                    enqueue(near_integer(tv, a.eps_v), (uint32_t)(yl << 8) | (uint32_t)(xl + i),
+                            queue, qn);
+                    word |= (uint32_t)clamp_floor(tv) << (8 * i);
+                }
+                *reinterpret_cast<uint32_t *>(orow + xl) = word;
+                if (qn > kQDrain) {
+                    for (int e = lane; e < qn; e += 64) {
+                        const uint32_t ent = queue[e];
+                        const int y2 = (int)(ent >> 8), x2 = (int)(ent & 255);
+                        float r = exact_chain<LMAX>(ht + (size_t)x2 * PITCH_T + y2, vts, L,
+                                                    a.inv_scale);
+                        out_s[y2 * kTW + x2] = clamp_u8_exact(r);
+                    }
+                    qn = 0;
+                }
+            }
+        }
+        for (int e = lane; e < qn; e += 64) {
+            const uint32_t ent = queue[e];
+            const int y2 = (int)(ent >> 8), x2 = (int)(ent & 255);
+            float r = exact_chain<LMAX>(ht + (size_t)x2 * PITCH_T + y2, vts, L, a.inv_scale);
+            out_s[y2 * kTW + x2] = clamp_u8_exact(r);
+        }
+        __syncthreads();
+
+        // ---- copy the output tile out: 8 lanes x 16 bytes per row
+        const bool vec_ok = ((W & 15) == 0) && ((reinterpret_cast<uintptr_t>(out) & 15) == 0);
+        for (int i = threadIdx.x; i < kTH * (kTW / 16); i += 256) {
+            const int yl = i >> 3, xl = (i & 7) * 16;
+            const int gy = y0 + yl, gx = x0 + xl;
+            if (gy >= H || gx >= W) continue;
+            const uint4 v = *reinterpret_cast<const uint4 *>(out_s + yl * kTW + xl);
+            uint8_t *o = out + (size_t)gy * W + gx;
+            if (vec_ok && gx + 15 < W) {
+                *reinterpret_cast<uint4 *>(o) = v;
+            } else {
+                const uint8_t *b = out_s + yl * kTW + xl;
+                for (int q = 0; q < 16 && gx + q < W; q++) o[q] = b[q];
+            }
+        }
+    }
+}
+
+// |E - R| bound for one pass, in output units. E: MFMA estimate, at most
+// 2L non-zero products plus 2*NKB accumulations, each rounded (toward zero
+// in the worst case, 2^-23 relative) at a magnitude <= 255 * sum|w|, plus
+// the 2^-22 relative residual of the hi/lo tap split. R: the reference's
+// chain, L products and L additions rounded to nearest (2^-24 relative).
+float blur_eps(const float *w, int L, int nkb) {
+    double sabs = 0.0;
+    for (int i = 0; i < L; i++) sabs += std::fabs((double)w[i]);
+    double mag = 255.0 * sabs;
+    double e = mag * (std::ldexp(1.0, -23) * (2.0 * L + 2.0 * nkb) + std::ldexp(1.0, -22) +
+                      std::ldexp(1.0, -24) * (2.0 * L));
+    return (float)(e * 1.25 + 1e-7);
+}
+
+}  // namespace
+
+// Taps the MFMA path handles: finite, at most 33 (sigma <= 8 for a Gaussian;
+// the LDS footprint stays under 64 KiB), not all zero.
+bool blur_mfma_supported(const float *hk, const float *vk, uint32_t len) {
+    if (len == 0 || len > 33u) return false;
+    float m = 0.0f;
+    for (uint32_t i = 0; i < len; i++) {
+        if (!std::isfinite(hk[i]) || !std::isfinite(vk[i])) return false;
+        m = std::fmax(m, std::fmax(std::fabs(hk[i]), std::fabs(vk[i])));
+    }
+    if (!(m > 0.0f && m < 1.0e30f)) return false;
+    // nonzero taps stay far from the subnormal range, so products scaled by
+    // a power of two round exactly like the unscaled ones
+    for (uint32_t i = 0; i < len; i++) {
+        float a = std::fabs(hk[i]), b = std::fabs(vk[i]);
+        if ((a != 0.0f && a < 0x1p-90f) || (b != 0.0f && b < 0x1p-90f)) return false;
+    }
+    return true;
+}
+
+iph_status launch_blur_u8_mfma(const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h,
+                               uint32_t n, size_t stride, const float *hk, const float *vk,
+                               uint32_t len, hipStream_t s) {
+    BlurArgs a{};
+    const int L = (int)len;
+    const int half = L / 2;
+    const int m = (4 - half % 4) % 4;            // x0 is a multiple of 128
+    const int nkb = (31 + L + m + 15) / 16;
+    float mx = 0.0f;
+    for (int i = 0; i < L; i++) {
+        a.h[i] = hk[i];
+        a.v[i] = vk[i];
+        mx = std::fmax(mx, std::fmax(std::fabs(hk[i]), std::fabs(vk[i])));
+    }
+    // 2^S puts the largest tap in [512, 1024): hi and lo parts stay normal f16.
+    int ex = 0;
+    std::frexp(mx, &ex);  // mx = f * 2^ex, f in [0.5, 1)
+    int S = 10 - ex;
+    a.scale = std::ldexp(1.0f, S);
+    a.inv_scale = std::ldexp(1.0f, -S);
+    a.eps_h = blur_eps(hk, L, nkb);
+    a.eps_v = blur_eps(vk, L, nkb);
+    a.L = L;
+    a.m = m;
+    const int tiles_x = (int)((w + kTW - 1) / kTW), tiles_y = (int)((h + kTH - 1) / kTH);
+    const long long ntiles = (long long)tiles_x * tiles_y * n;
+    if (ntiles > (1LL << 30)) {
+        set_error("batch of %u images of %ux%u is too large for one launch", n, w, h);
+        return IPH_EINVAL;
+    }
+    bool same = true;
+    for (int i = 0; i < L; i++) same = same && (hk[i] == vk[i]);
+    const bool aligned = w >= 4 && (w % 4) == 0 && (stride % 4) == 0 &&
+                         (reinterpret_cast<uintptr_t>(src) & 3) == 0;
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+#define LB(K, SAME, AL)                                                                      \
+    do {                                                                                     \
+        using Sh = BlurShape<K>;                                                             \
+        size_t lds = (size_t)kRows * Sh::PIN + (size_t)kTW * Sh::PITCH_T * sizeof(_Float16) +  \
+                     (size_t)kTH * kTW + 2 * (2 * kTapPad + kMaxTaps) * sizeof(float) +      \
+                     4 * kQCap * sizeof(uint32_t);                                            \
+        int per_cu = 0;                                                                      \
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(                                    \
+                &per_cu, blur_u8_mfma_kernel<K, SAME, AL>, 256, lds) != hipSuccess ||        \
+            per_cu < 1)                                                                      \
+            per_cu = 1;                                                                      \
+        long long grid = (long long)cus * per_cu;                                            \
+        grid = std::min<long long>(grid, (ntiles + 7) / 8 * 8);                              \
+        grid = std::max<long long>(8, grid / 8 * 8);                                         \
+        blur_u8_mfma_kernel<K, SAME, AL><<<(unsigned)grid, 256, lds, s>>>(                  \
+            src, dst, (int)w, (int)h, stride, (int)n, tiles_x, tiles_y, a);                  \
+    } while (0)
+#define LA(K, SAME)                                                                          \
+    if (aligned) LB(K, SAME, true); else LB(K, SAME, false)
+#define LK(K)                                                                                \
+    if (same) { LA(K, true); } else { LA(K, false); }                                        \
+    break
+    switch (nkb) {
+    case 2: LK(2);
+    case 3: LK(3);
+    default: LK(4);
+    }
+#undef LK
+#undef LA
+#undef LB
+    return launch_status("blur_u8_mfma_kernel");
+}
+
+}  // namespace iph

@@ -15,6 +15,7 @@
 #include "iph_internal.hpp"
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 namespace iph {
@@ -296,6 +297,17 @@ iph_status separable_typed(const void *src, void *dst, uint32_t w, uint32_t h, u
             fill_kernel<P, K><<<1024, 256, 0, s>>>(out + (size_t)i * stride, (size_t)w * h, K(0));
         }
         return launch_status("fill_kernel");
+    }
+    if constexpr (std::is_same<P, uint8_t>::value && std::is_same<K, float>::value) {
+        const float *hf = static_cast<const float *>(hk), *vf = static_cast<const float *>(vk);
+        if (blur_mfma_supported(hf, vf, len)) {
+            for (uint32_t i0 = 0; i0 < n; i0 += 65535) {
+                uint32_t cnt = std::min<uint32_t>(65535, n - i0);
+                IPH_TRY(launch_blur_u8_mfma(in + (size_t)i0 * stride, out + (size_t)i0 * stride, w,
+                                            h, cnt, stride, hf, vf, len, s));
+            }
+            return IPH_OK;
+        }
     }
     if (len <= (uint32_t)kFusedMaxTaps) {
         FusedTaps<K> taps{};

@@ -95,6 +95,10 @@ iph_status launch_filter_clamped(const void *src, int32_t p_type, uint32_t w, ui
                                  void *dst, int32_t s_type, void *ws, size_t ws_bytes,
                                  hipStream_t s);
 size_t filter_clamped_workspace(int32_t k_type, uint32_t kw, uint32_t kh);
+bool blur_mfma_supported(const float *hk, const float *vk, uint32_t len);
+iph_status launch_blur_u8_mfma(const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h,
+                               uint32_t n, size_t stride, const float *hk, const float *vk,
+                               uint32_t len, hipStream_t s);
 iph_status launch_box_filter(const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h,
                              uint32_t rx, uint32_t ry, void *ws, size_t ws_bytes,
                              hipStream_t s);

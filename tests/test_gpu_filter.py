@@ -194,3 +194,46 @@ def test_box_filter_random(rng, h, w, rx, ry):
 def test_box_filter_exact_on_saturated_windows():
     img = np.full((64, 70), 255, np.uint8)
     _same(flt.box_filter(img, 30, 31), img)
+
+
+# ---- MFMA-estimated u8 blur: exactness stress ------------------------------------------
+
+def test_mfma_blur_many_images_bit_exact():
+    rng = np.random.default_rng(99)
+    for i in range(24):
+        h, w = int(rng.integers(1, 400)), int(rng.integers(1, 400))
+        sigma = float(rng.uniform(0.3, 8.0))
+        img = rng.integers(0, 256, (h, w), dtype=np.uint8)
+        _same(flt.gaussian_blur_f32(img, sigma), O.gaussian_blur_f32(img, sigma))
+
+
+def test_mfma_blur_flat_and_structured_images():
+    # flat regions put every sum next to an integer: all outputs take the exact path
+    for v in (0, 1, 37, 128, 254, 255):
+        img = np.full((130, 260), v, np.uint8)
+        _same(flt.gaussian_blur_f32(img, 2.0), O.gaussian_blur_f32(img, 2.0))
+    y, x = np.mgrid[0:300, 0:500]
+    img = (((x // 7) + (y // 5)) % 2 * 255).astype(np.uint8)
+    for s in (1.0, 2.0, 5.5):
+        _same(flt.gaussian_blur_f32(img, s), O.gaussian_blur_f32(img, s))
+
+
+def test_mfma_separable_signed_and_unnormalised_taps():
+    rng = np.random.default_rng(5)
+    img = rng.integers(0, 256, (257, 333), dtype=np.uint8)
+    for L in (1, 2, 3, 9, 16, 33):
+        hk = rng.uniform(-0.5, 1.0, L).astype(F)
+        vk = rng.uniform(-0.5, 1.0, L).astype(F)
+        _same(flt.separable_filter(img, hk, vk), O.separable_filter(img, hk, vk))
+        big = (hk * 37.0).astype(F)
+        _same(flt.separable_filter(img, big, big), O.separable_filter(img, big, big))
+
+
+def test_mfma_blur_batch_device_api():
+    import torch
+    from imageproc_amd import device as dev
+    rng = np.random.default_rng(3)
+    imgs = rng.integers(0, 256, (5, 200, 333), dtype=np.uint8)
+    out = dev.gaussian_blur_f32(torch.from_numpy(imgs).cuda(), 2.0).cpu().numpy()
+    for i in range(5):
+        _same(out[i], O.gaussian_blur_f32(imgs[i], 2.0))
