@@ -68,6 +68,30 @@ template <> struct ClampTo<float> {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
+// Copies a clamp-to-edge window of an image into LDS: element i of the
+// R x C window is image[clamp(oy + i / C)][clamp(ox + i % C)]. Loads are
+// issued in batches of 8 before their LDS stores, so global latency is paid
+// once per batch rather than once per element.
+template <typename P>
+__device__ __forceinline__ void stage_window(const P *__restrict__ img, int W, int H, int oy,
+                                             int ox, int R, int C, P *dst) {
+    const int total = R * C;
+    for (int base = threadIdx.x; base < total; base += 8 * (int)blockDim.x) {
+        P v[8];
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const int i = min(base + b * (int)blockDim.x, total - 1);
+            const int r = i / C, c = i - r * C;
+            v[b] = img[(size_t)clampi(oy + r, 0, H - 1) * W + clampi(ox + c, 0, W - 1)];
+        }
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const int i = base + b * (int)blockDim.x;
+            if (i < total) dst[i] = v[b];
+        }
+    }
+}
+
 // ---- fused separable filter ------------------------------------------------
 //
 // One workgroup produces a TW x TH output tile of one image:
@@ -104,12 +128,7 @@ __global__ __launch_bounds__(256) void sep_fused_kernel(const P *__restrict__ sr
     const P *img = src + (size_t)blockIdx.z * stride;
     P *out = dst + (size_t)blockIdx.z * stride;
 
-    for (int i = threadIdx.x; i < R * C; i += blockDim.x) {
-        int r = i / C, c = i - r * C;
-        int gy = clampi(y0 - half + r, 0, H - 1);
-        int gx = clampi(x0 - half + c, 0, W - 1);
-        in_s[i] = img[(size_t)gy * W + gx];
-    }
+    stage_window(img, W, H, y0 - half, x0 - half, R, C, in_s);
     __syncthreads();
 
     for (int i = threadIdx.x; i < R * kSepTW; i += blockDim.x) {
@@ -183,10 +202,7 @@ __global__ __launch_bounds__(256) void filter2d_tiled_kernel(const P *__restrict
     const int x0 = blockIdx.x * kF2TW, y0 = blockIdx.y * kF2TH;
     // Window origin: kernel centre (kw/2, kh/2), filter/mod.rs:50,53.
     const int ox = x0 - kw / 2, oy = y0 - kh / 2;
-    for (int i = threadIdx.x; i < R * C; i += blockDim.x) {
-        int r = i / C, c = i - r * C;
-        in_s[i] = src[(size_t)clampi(oy + r, 0, H - 1) * W + clampi(ox + c, 0, W - 1)];
-    }
+    stage_window(src, W, H, oy, ox, R, C, in_s);
     __syncthreads();
     for (int i = threadIdx.x; i < kF2TH * kF2TW; i += blockDim.x) {
         int r = i / kF2TW, c = i - r * kF2TW;

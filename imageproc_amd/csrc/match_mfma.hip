@@ -34,7 +34,7 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 
 constexpr int kXT = 4;         // 32-column output tiles per wave
 constexpr int kMaxSide = 128;  // largest template side on the int8 path
-constexpr int kWinRows = 16;   // output rows per window-sum workgroup
+constexpr int kWinRows = 32;   // output rows per window-sum workgroup
 constexpr int kWinCols = 64;   // output columns per window-sum workgroup
 
 __device__ __forceinline__ int align_up_dev(int v) { return (v + 15) & ~15; }
@@ -78,56 +78,70 @@ __global__ __launch_bounds__(64) void toeplitz_i8_kernel(const uint8_t *__restri
     }
 }
 
-// Exact window sums (sum I, sum I^2) for every output position: vertical
-// sliding sums over th rows into LDS, then horizontal sliding sums over tw.
+// Exact window sums (sum I, sum I^2) for every output position of a
+// kWinCols x kWinRows output tile: each staged image row gets its horizontal
+// tw-window sums by a sliding pass (one thread per row), then each output
+// column its vertical th-window sums over those (one thread per column and
+// 16-row segment), written coalesced. All arithmetic is exact u32.
 __global__ __launch_bounds__(256) void window_sums_u8_kernel(const uint8_t *__restrict__ img,
                                                              int W, int tw, int th, int ow,
                                                              int oh, uint2 *__restrict__ ws) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int x0 = blockIdx.x * kWinCols, y0 = blockIdx.y * kWinRows;
     const int cols = kWinCols + tw - 1, rows = kWinRows + th - 1;
-    uint8_t *blk = smem;                                             // rows x cols
-    uint2 *v = reinterpret_cast<uint2 *>(smem + align_up_dev(rows * cols));  // kWinRows x cols
     const int H = oh + th - 1;
-    for (int i = threadIdx.x; i < rows * cols; i += 256) {
-        int r = i / cols, c = i - r * cols;
-        int gy = y0 + r, gx = x0 + c;
-        blk[i] = (gy < H && gx < W) ? img[(size_t)gy * W + gx] : 0;
+    uint8_t *blk = smem;                                                         // rows x cols
+    uint2 *hs = reinterpret_cast<uint2 *>(smem + align_up_dev(rows * cols));     // rows x kWinCols
+    for (int base = threadIdx.x; base < rows * cols; base += 16 * 256) {
+        uint8_t v[16];
+#pragma unroll
+        for (int b = 0; b < 16; b++) {
+            const int i = base + 256 * b;
+            const int r = i / cols, c = i - r * cols;
+            const int gy = min(y0 + r, H - 1), gx = min(x0 + c, W - 1);
+            v[b] = img[(size_t)gy * W + gx];
+        }
+#pragma unroll
+        for (int b = 0; b < 16; b++) {
+            const int i = base + 256 * b;
+            if (i < rows * cols) blk[i] = v[b];
+        }
     }
     __syncthreads();
-    for (int c = threadIdx.x; c < cols; c += 256) {
+    for (int r = threadIdx.x; r < rows; r += 256) {
+        const uint8_t *p = blk + r * cols;
         uint32_t a = 0, b = 0;
-        for (int r = 0; r < th; r++) {
-            uint32_t p = blk[r * cols + c];
-            a += p;
-            b += p * p;
+        for (int dx = 0; dx < tw; dx++) {
+            uint32_t v = p[dx];
+            a += v;
+            b += v * v;
         }
-        v[c] = make_uint2(a, b);
-        for (int y = 1; y < kWinRows; y++) {
-            uint32_t pin = blk[(y + th - 1) * cols + c], pout = blk[(y - 1) * cols + c];
-            a += pin - pout;
-            b += pin * pin - pout * pout;
-            v[y * cols + c] = make_uint2(a, b);
+        uint2 *o = hs + r * kWinCols;
+        o[0] = make_uint2(a, b);
+        for (int x = 1; x < kWinCols; x++) {
+            uint32_t vin = p[x + tw - 1], vout = p[x - 1];
+            a += vin - vout;
+            b += vin * vin - vout * vout;
+            o[x] = make_uint2(a, b);
         }
     }
     __syncthreads();
-    const int y = threadIdx.x >> 4, seg = threadIdx.x & 15;  // 16 rows x 16 segments of 4
-    const int gy = y0 + y;
-    if (gy >= oh) return;
-    const uint2 *vr = v + y * cols;
-    int c0 = seg * 4;
+    const int x = threadIdx.x & (kWinCols - 1), seg = threadIdx.x / kWinCols;
+    constexpr int kSeg = kWinRows / (256 / kWinCols);
+    const int ya = seg * kSeg;
     uint32_t a = 0, b = 0;
-    for (int dx = 0; dx < tw; dx++) {
-        a += vr[c0 + dx].x;
-        b += vr[c0 + dx].y;
+    for (int dy = 0; dy < th; dy++) {
+        uint2 v = hs[(ya + dy) * kWinCols + x];
+        a += v.x;
+        b += v.y;
     }
-    for (int k = 0; k < 4; k++) {
-        int gx = x0 + c0 + k;
-        if (gx < ow) ws[(size_t)gy * ow + gx] = make_uint2(a, b);
-        if (k < 3) {
-            a += vr[c0 + k + tw].x - vr[c0 + k].x;
-            b += vr[c0 + k + tw].y - vr[c0 + k].y;
-        }
+    for (int y = ya; y < ya + kSeg; y++) {
+        const int gy = y0 + y, gx = x0 + x;
+        if (gy < oh && gx < ow) ws[(size_t)gy * ow + gx] = make_uint2(a, b);
+        uint2 vin = hs[(y + th) * kWinCols + x < rows * kWinCols ? (y + th) * kWinCols + x : 0];
+        uint2 vout = hs[y * kWinCols + x];
+        a += vin.x - vout.x;
+        b += vin.y - vout.y;
     }
 }
 
@@ -161,24 +175,36 @@ __global__ __launch_bounds__(256) void match_i8_mfma_kernel(
     const int x0 = blockIdx.x * (32 * kXT), y0 = blockIdx.y * 32 * RT;
     const int rows = 32 * RT + th - 1;
 
-    // Stage image rows y0.. and columns x0.. as i8 = u8 - 128 (0 outside).
-    for (int i = threadIdx.x; i < rows * (BW / 4); i += blockDim.x) {
-        int r = i / (BW / 4), c = (i - r * (BW / 4)) * 4;
-        int gy = y0 + r, gx = x0 + c;
-        uint32_t v = 0x80808080u;
-        if (gy < H) {
-            const uint8_t *p = img + (size_t)gy * W + gx;
-            if (gx + 3 < W && (reinterpret_cast<uintptr_t>(p) & 3) == 0) {
-                v = *reinterpret_cast<const uint32_t *>(p);
+    // Stage image rows y0.. and columns x0.. as i8 = u8 - 128. Bytes outside
+    // the image only meet zero Toeplitz entries or produce discarded outputs.
+    const bool aligned = (W & 3) == 0 && (reinterpret_cast<uintptr_t>(img) & 3) == 0 && W >= 4;
+    const int total = rows * (BW / 4);
+    // loads are issued in batches of 8 before their LDS stores, so a
+    // workgroup waits for global latency once per batch, not per dword
+    for (int base = threadIdx.x; base < total; base += 8 * (int)blockDim.x) {
+        uint32_t v[8];
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const int i = min(base + b * (int)blockDim.x, total - 1);
+            const int r = i / (BW / 4), c = (i - r * (BW / 4)) * 4;
+            const uint8_t *row = img + (size_t)min(y0 + r, H - 1) * W;
+            const int gx = x0 + c;
+            if (aligned) {
+                v[b] = *reinterpret_cast<const uint32_t *>(row + min(gx, W - 4));
             } else {
-                uint32_t b0 = gx < W ? p[0] : 0x80u;
-                uint32_t b1 = gx + 1 < W ? p[1] : 0x80u;
-                uint32_t b2 = gx + 2 < W ? p[2] : 0x80u;
-                uint32_t b3 = gx + 3 < W ? p[3] : 0x80u;
-                v = b0 | (b1 << 8) | (b2 << 16) | (b3 << 24);
+                v[b] = (uint32_t)row[min(gx, W - 1)] | ((uint32_t)row[min(gx + 1, W - 1)] << 8) |
+                       ((uint32_t)row[min(gx + 2, W - 1)] << 16) |
+                       ((uint32_t)row[min(gx + 3, W - 1)] << 24);
             }
         }
-        *reinterpret_cast<uint32_t *>(blk + r * pitch + c) = v ^ 0x80808080u;
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const int i = base + b * (int)blockDim.x;
+            if (i < total) {
+                const int r = i / (BW / 4), c = (i - r * (BW / 4)) * 4;
+                *reinterpret_cast<uint32_t *>(blk + r * pitch + c) = v[b] ^ 0x80808080u;
+            }
+        }
     }
     __syncthreads();
 
@@ -191,10 +217,14 @@ __global__ __launch_bounds__(256) void match_i8_mfma_kernel(
 
     const int8_t *arow = blk + (wave * 32 + n) * pitch + 16 * g;
     const v4i *bp = bsw + lane;
-    for (int dy = 0; dy < th; dy++) {
-        v4i b[KB];
+    // Toeplitz operands of the next template row are loaded one step ahead.
+    v4i b[KB], bn[KB];
 #pragma unroll
-        for (int kb = 0; kb < KB; kb++) b[kb] = bp[(dy * KB + kb) * 64];
+    for (int kb = 0; kb < KB; kb++) b[kb] = bp[kb * 64];
+    for (int dy = 0; dy < th; dy++) {
+        const int dn = dy + 1 < th ? dy + 1 : dy;
+#pragma unroll
+        for (int kb = 0; kb < KB; kb++) bn[kb] = bp[(dn * KB + kb) * 64];
 #pragma unroll
         for (int s = 0; s < kXT + KB - 1; s++) {
             v4i a = *reinterpret_cast<const v4i *>(arow + 32 * s);
@@ -204,6 +234,8 @@ __global__ __launch_bounds__(256) void match_i8_mfma_kernel(
                     acc[t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b[s - t], acc[t], 0, 0, 0);
             }
         }
+#pragma unroll
+        for (int kb = 0; kb < KB; kb++) b[kb] = bn[kb];
         arow += pitch;
     }
 
@@ -213,16 +245,22 @@ __global__ __launch_bounds__(256) void match_i8_mfma_kernel(
 #pragma unroll
     for (int t = 0; t < kXT; t++) {
         const int x = x0 + 32 * t + n;
+        // all 16 window-sum loads of this column are issued before any use
+        // (clamped addresses, so none is conditional)
+        uint2 sums[16];
 #pragma unroll
         for (int r = 0; r < 16; r++) {
             // 32x32 accumulator layout: register r of lane l holds row
             // 8*(r/4) + 4*(l/32) + r%4, column l%32.
             const int y = y0 + wave * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
-            if (x < ow && y < oh) {
-                uint2 s = wsum[(size_t)y * ow + x];
-                long long c = (long long)acc[t][r] + 128LL * s.x + base;
-                out[(size_t)y * ow + x] = finish_score<METHOD>(c, (long long)s.y, stt, nn);
-            }
+            sums[r] = wsum[(size_t)min(y, oh - 1) * ow + min(x, ow - 1)];
+        }
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int y = y0 + wave * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
+            long long c = (long long)acc[t][r] + 128LL * sums[r].x + base;
+            float v = finish_score<METHOD>(c, (long long)sums[r].y, stt, nn);
+            if (x < ow && y < oh) out[(size_t)y * ow + x] = v;
         }
     }
 }
@@ -363,14 +401,26 @@ __global__ __launch_bounds__(128) void match_bf16x3_mfma_kernel(
     const v8bf *bp = bsw + lane;
     for (int c0 = 0; c0 < th; c0 += kDyChunk) {
         __syncthreads();
-        for (int i = threadIdx.x; i < rows * BW; i += blockDim.x) {
-            int r = i / BW, c = i - r * BW;
-            int gy = y0 + c0 + r, gx = x0 + c;
-            float v = (gy < H && gx < W) ? img[(size_t)gy * W + gx] : 0.0f;
-            uint16_t h, l;
-            split_bf16(v, h, l);
-            hi_s[r * pitch + c] = h;
-            lo_s[r * pitch + c] = l;
+        for (int base = threadIdx.x; base < rows * BW; base += 8 * (int)blockDim.x) {
+            float v[8];
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                const int i = base + b * (int)blockDim.x;
+                const int r = i / BW, c = i - r * BW;
+                const int gy = min(y0 + c0 + r, H - 1), gx = min(x0 + c, W - 1);
+                v[b] = img[(size_t)gy * W + gx];
+            }
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                const int i = base + b * (int)blockDim.x;
+                if (i < rows * BW) {
+                    const int r = i / BW, c = i - r * BW;
+                    uint16_t h, l;
+                    split_bf16(v[b], h, l);
+                    hi_s[r * pitch + c] = h;
+                    lo_s[r * pitch + c] = l;
+                }
+            }
         }
         __syncthreads();
         const int dy_end = min(kDyChunk, th - c0);
@@ -406,13 +456,17 @@ __global__ __launch_bounds__(128) void match_bf16x3_mfma_kernel(
 #pragma unroll
     for (int t = 0; t < kXT16; t++) {
         const int x = x0 + 32 * t + n;
+        double sq[16];
 #pragma unroll
         for (int r = 0; r < 16; r++) {
             const int y = y0 + wave * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
-            if (x < ow && y < oh) {
-                double sii = (METHOD == IPH_CCORR) ? 0.0 : wsq[(size_t)y * ow + x];
-                out[(size_t)y * ow + x] = finish_score_f32<METHOD>(acc[t][r], sii, stt, nn);
-            }
+            sq[r] = (METHOD == IPH_CCORR) ? 0.0 : wsq[(size_t)min(y, oh - 1) * ow + min(x, ow - 1)];
+        }
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int y = y0 + wave * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
+            float v = finish_score_f32<METHOD>(acc[t][r], sq[r], stt, nn);
+            if (x < ow && y < oh) out[(size_t)y * ow + x] = v;
         }
     }
 }
@@ -449,7 +503,7 @@ int blocks_for(int tw) { return (31 + tw + 31) / 32; }
 
 size_t win_lds(int tw, int th) {
     int cols = kWinCols + tw - 1, rows = kWinRows + th - 1;
-    return align_up((size_t)rows * cols, 16) + (size_t)kWinRows * cols * sizeof(uint2);
+    return align_up((size_t)rows * cols, 16) + (size_t)rows * kWinCols * sizeof(uint2);
 }
 
 template <int KB>
