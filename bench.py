@@ -8,7 +8,9 @@ the 256 MB Infinity Cache, so the bytes really cross HBM). `value` is the
 batch's Gpix/s. The same JSON line carries:
   * roofline: the blur kernel's algorithmic bytes (read + write, 1 572 864 B
     per image) over its average duration, measured with HIP events on the
-    stream the kernel runs on, against the 8 TB/s HBM peak;
+    stream the kernel runs on, against the 8 TB/s HBM peak; `traffic` is the
+    HBM bytes per launch from the FETCH_SIZE/WRITE_SIZE counter runs recorded
+    in profiles/blur_u8_sigma2_512x1024x768.md (copy-calibrated);
   * match_template: ms per 2048x2048 u8 image with a 32x32 template, for each
     of the four methods on the MFMA path and on the direct FP32 path, with
     argmax/argmin agreement between them;
@@ -113,6 +115,22 @@ def timed(fn, steps: int, warmup: int):
     return [s.elapsed_time(e) for s, e in zip(starts, ends)], wall
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "blur_u8_sigma2_512x1024x768.md")
+
+
+def measured_traffic():
+    """HBM bytes per blur launch from the FETCH_SIZE/WRITE_SIZE counter runs
+    (calibrated against a plain copy) recorded in the profile summary, or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_blur_baseline(budget_s: float = 12.0):
     """Oracle blur on one host core over a bounded sample of the workload."""
     import oracle as O
@@ -158,6 +176,18 @@ def bench_blur(steps: int, warmup: int):
     return ms, wall, bytes_per_step, ok
 
 
+def bench_copy(steps: int, warmup: int):
+    """A plain elementwise copy of the blur batch's size (403 MB read, 403 MB
+    written): the known-size kernel that scales HBM counters."""
+    import torch
+
+    src = torch.ones((BLUR_N, BLUR_H, BLUR_W), dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    times, _ = timed(lambda: torch.mul(src, 1, out=dst), steps, warmup)
+    ms = float(np.mean(times))
+    return ms, 2.0 * src.numel() / (ms * 1e-3) / 1e9
+
+
 def bench_match(steps: int):
     import torch
 
@@ -198,6 +228,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--leg", choices=("all", "blur", "copy"), default="all",
+                    help="blur: only the blur batch (for counter runs); copy: only a plain "
+                         "copy of the same size (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X and has no "
@@ -209,7 +242,16 @@ def main():
     import imageproc_amd as ip
 
     print(f"[bench] device: {ip.device_info()}", flush=True)
+    if args.leg == "copy":
+        ms, gbs = bench_copy(args.steps, args.warmup)
+        print(json.dumps({"leg": "copy", "bytes": 2 * BLUR_N * BLUR_W * BLUR_H,
+                          "ms": round(ms, 4), "GB/s": round(gbs, 1)}), flush=True)
+        return
     ms, wall, nbytes, ok = bench_blur(args.steps, args.warmup)
+    if args.leg == "blur":
+        print(json.dumps({"leg": "blur", "ms": round(ms, 4), "parity_spot_check": bool(ok),
+                          "GB/s": round(nbytes / (ms * 1e-3) / 1e9, 1)}), flush=True)
+        return
     gpix = BLUR_N * BLUR_W * BLUR_H / (ms * 1e-3) / 1e9
     achieved = nbytes / (ms * 1e-3) / 1e9
     match = bench_match(max(5, args.steps // 2))
@@ -232,7 +274,8 @@ def main():
                    "images": BLUR_N, "width": BLUR_W, "height": BLUR_H, "sigma": BLUR_SIGMA,
                    "parity_spot_check": bool(ok), "wall_s": round(wall, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic()},
         "match_template": match,
         "cpu_baseline": cpu,
     }
