@@ -140,16 +140,16 @@ __device__ __forceinline__ float exact_chain(const T *p, const float *taps_s, in
 // for L <= 33, rounded up to whole 32-row blocks.
 constexpr int kRows = 128;
 
-template <int NKB> struct BlurShape {
+template <int NKB, int ROWS = kRows> struct BlurShape {
     static constexpr int CIN = kTW - 32 + 16 * NKB;  // staged input columns
     static constexpr int C4 = CIN / 4;                 // dwords per staged row
-    static constexpr int NREG = kRows * C4 / 256;      // staged dwords per thread
+    static constexpr int NTOT = ROWS * C4;             // staged dwords per tile
+    static constexpr int NREG = (NTOT + 255) / 256;    // staged dwords per thread
     // staged input row pitch: an odd multiple of 2 dwords, so the 32 rows one
     // 8-byte A-operand read touches fall in distinct LDS banks
     static constexpr int PIN = ((CIN / 4) % 2 == 0) ? CIN + 8 : CIN + 4;
-    static constexpr int PITCH_T = kRows + 8;          // f16 per intermediate column
+    static constexpr int PITCH_T = ROWS + 8;           // f16 per intermediate column
     static constexpr int LMAX = 16 * NKB - 31;         // largest tap count for NKB
-    static_assert(kRows * C4 % 256 == 0, "staging must divide evenly");
 };
 
 // Issues this thread's share of one tile's input-window loads into
@@ -159,13 +159,12 @@ template <int NKB> struct BlurShape {
 // column; the edge byte is replicated later, in store_tile, once the data has
 // arrived (so no load waits for its predecessor). Otherwise each byte is
 // loaded from its clamped column.
-template <int NKB, bool ALIGNED>
+template <typename S, bool ALIGNED>
 __device__ __forceinline__ void load_tile(const uint8_t *__restrict__ img, int W, int H, int xs,
-                                          int ys, uint32_t (&regs)[BlurShape<NKB>::NREG]) {
-    using S = BlurShape<NKB>;
+                                          int ys, uint32_t (&regs)[S::NREG]) {
 #pragma unroll
     for (int k = 0; k < S::NREG; k++) {
-        const int i = (int)threadIdx.x + 256 * k;
+        const int i = min((int)threadIdx.x + 256 * k, S::NTOT - 1);
         const int r = i / S::C4, c = (i - r * S::C4) * 4;
         const uint8_t *row = img + (size_t)clampi(ys + r, 0, H - 1) * W;
         const int gx = xs + c;
@@ -182,13 +181,13 @@ __device__ __forceinline__ void load_tile(const uint8_t *__restrict__ img, int W
 
 // Writes the loaded window to LDS; for ALIGNED loads, dwords left of the row
 // become its first byte replicated and dwords right of it its last byte.
-template <int NKB, bool ALIGNED>
+template <typename S, bool ALIGNED>
 __device__ __forceinline__ void store_tile(uint8_t *in_s, int W, int xs,
-                                           const uint32_t (&regs)[BlurShape<NKB>::NREG]) {
-    using S = BlurShape<NKB>;
+                                           const uint32_t (&regs)[S::NREG]) {
 #pragma unroll
     for (int k = 0; k < S::NREG; k++) {
         const int i = (int)threadIdx.x + 256 * k;
+        if (S::NTOT % 256 != 0 && i >= S::NTOT) break;
         const int r = i / S::C4, c = (i - r * S::C4) * 4;
         uint32_t v = regs[k];
         if (ALIGNED) {
@@ -256,7 +255,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     uint32_t regs[S::NREG];
     if (t < t_end) {
         int im = t / per_img, rem = t - im * per_img, ty = rem / tiles_x;
-        load_tile<NKB, ALIGNED>(src + (size_t)im * stride, W, H,
+        load_tile<S, ALIGNED>(src + (size_t)im * stride, W, H,
                                 (rem - ty * tiles_x) * kTW - half - m, ty * kTH - half, regs);
     }
     for (; t < t_end; t += per_xcd) {
@@ -265,11 +264,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         uint8_t *out = dst + (size_t)im * stride;
 
         __syncthreads();  // the previous tile is done with in_s, ht and out_s
-        store_tile<NKB, ALIGNED>(in_s, W, x0 - half - m, regs);
+        store_tile<S, ALIGNED>(in_s, W, x0 - half - m, regs);
         const int tn = t + per_xcd;
         if (tn < t_end) {
             int im2 = tn / per_img, rem2 = tn - im2 * per_img, ty2 = rem2 / tiles_x;
-            load_tile<NKB, ALIGNED>(src + (size_t)im2 * stride, W, H,
+            load_tile<S, ALIGNED>(src + (size_t)im2 * stride, W, H,
                                     (rem2 - ty2 * tiles_x) * kTW - half - m, ty2 * kTH - half, regs);
         }
         __syncthreads();
@@ -396,6 +395,260 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     }
 }
 
+// ---- fixed-point readout: taps >= 0 with 255 * sum(w) < 256 ------------------
+//
+// With taps scaled by 2^15 and the accumulator started at 2^23 + c, every
+// partial sum of the (non-negative) products stays in [2^23, 2^24), where the
+// f32 ulp is 1. The final value is 2^23 + c + E, E = the estimate in units of
+// 2^-15, so its bit pattern u carries floor((E + c) / 2^15) in bits 15..22 and
+// (E + c) mod 2^15 in bits 0..14:
+//   * (u & 0x7fff) >= 2c  proves the estimate is at least c units (>= eps) away
+//     from an integer, so bits 15..22 are the reference's truncated result (the
+//     reference's sum lies in [0, 256), where its clamp is the identity);
+//   * otherwise the output is recomputed with the reference's chain.
+// Readout is one AND, one compare and one byte insert per output.
+
+constexpr int kTH2 = 192;           // output rows per tile (6 row blocks)
+constexpr int kRows2 = 224;         // staged rows: kTH2 + 32 covers L <= 33
+constexpr int kPitchB = kRows2 + 8; // bytes per intermediate column (58 dwords)
+constexpr int kOPitch = kTW + 16;   // bytes per output-tile row
+
+struct FxArgs {
+    float h[kMaxTaps];
+    float v[kMaxTaps];
+    int L;
+    int m;
+    int ch, cv;  // readout margins c (units of 2^-15) of the two passes
+};
+
+// Toeplitz B operand with taps * 2^15 split hi + lo (see build_b).
+template <int NKB>
+__device__ __forceinline__ void build_b_fx(const float *taps, int lane, int shift,
+                                           v8h (&bhi)[NKB], v8h (&blo)[NKB]) {
+    const int n = lane & 31, g = lane >> 5;
+#pragma unroll
+    for (int kb = 0; kb < NKB; kb++) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            float w = taps[kTapPad + 16 * kb + 8 * g + j - n - shift] * 32768.0f;
+            _Float16 hi = (_Float16)w;
+            _Float16 lo = (_Float16)(w - (float)hi);
+            bhi[kb][j] = hi;
+            blo[kb][j] = lo;
+        }
+    }
+}
+
+typedef unsigned short v2u16 __attribute__((ext_vector_type(2)));
+
+// Bits 15..22 of four accumulator bit patterns, packed into one dword: bytes
+// 1..2 of two values form two 16-bit halves (bits 8..23), a packed shift by 7
+// leaves bits 15..23 in each half (bit 23, the lowest exponent bit of a value
+// in [2^23, 2^24), is zero), and a last byte permute gathers the four bytes.
+__device__ __forceinline__ uint32_t pack_bits15(uint32_t u0, uint32_t u1, uint32_t u2,
+                                                uint32_t u3) {
+    const uint32_t p01 = __builtin_amdgcn_perm(u1, u0, 0x06050201u);
+    const uint32_t p23 = __builtin_amdgcn_perm(u3, u2, 0x06050201u);
+    const v2u16 seven = {7, 7};
+    const v2u16 h01 = __builtin_bit_cast(v2u16, p01) >> seven;
+    const v2u16 h23 = __builtin_bit_cast(v2u16, p23) >> seven;
+    return __builtin_amdgcn_perm(__builtin_bit_cast(uint32_t, h23),
+                                 __builtin_bit_cast(uint32_t, h01), 0x06040200u);
+}
+
+template <int NKB, bool SAME_TAPS, bool ALIGNED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void blur_u8_fx_kernel(
+    const uint8_t *__restrict__ src, uint8_t *__restrict__ dst, int W, int H, size_t stride,
+    int n_img, int tiles_x, int tiles_y, FxArgs a) {
+    using S = BlurShape<NKB, kRows2>;
+    constexpr int PIN = S::PIN, LMAX = S::LMAX;
+    constexpr int IN_BYTES = (kRows2 * PIN + 15) / 16 * 16;
+    static_assert(kTH2 * kOPitch <= IN_BYTES, "output tile reuses the input window");
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint8_t *in_s = smem;                     // kRows2 x PIN, then the output tile
+    uint8_t *out_s = smem;                    // kTH2 x kOPitch (after the horizontal pass)
+    uint8_t *ht = smem + IN_BYTES;            // kTW x kPitchB, column-major u8
+    float *taps_h = reinterpret_cast<float *>(ht + kTW * kPitchB);
+    float *taps_v = taps_h + (2 * kTapPad + kMaxTaps);
+    uint32_t *queue = reinterpret_cast<uint32_t *>(taps_v + (2 * kTapPad + kMaxTaps)) +
+                      (threadIdx.x >> 6) * kQCap;
+    const int L = a.L;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = lane & 31, g = lane >> 5;
+    const int half = L / 2;
+
+    if (threadIdx.x < 2 * kTapPad + kMaxTaps) {
+        int t = (int)threadIdx.x - kTapPad;
+        bool in = t >= 0 && t < L;
+        taps_h[threadIdx.x] = in ? a.h[t] : 0.0f;
+        taps_v[threadIdx.x] = in ? a.v[t] : 0.0f;
+    }
+    __syncthreads();
+    v8h bh_hi[NKB], bh_lo[NKB], bv_hi[NKB], bv_lo[NKB];
+    const int m = a.m;
+    build_b_fx<NKB>(taps_h, lane, m, bh_hi, bh_lo);
+    build_b_fx<NKB>(taps_v, lane, 0, bv_hi, bv_lo);
+    const float *vts = SAME_TAPS ? taps_h : taps_v;
+    const float init_h = 8388608.0f + (float)a.ch, init_v = 8388608.0f + (float)a.cv;
+    const uint32_t thr_h = 2u * (uint32_t)a.ch, thr_v = 2u * (uint32_t)a.cv;
+
+    const int per_img = tiles_x * tiles_y;
+    const int ntiles = n_img * per_img;
+    const int per_xcd = (int)gridDim.x >> 3;
+    const int chunk = (ntiles + 7) >> 3;
+    const int t_begin = (int)(blockIdx.x & 7) * chunk;
+    const int t_end = min(ntiles, t_begin + chunk);
+    int t = t_begin + (int)(blockIdx.x >> 3);
+
+    uint32_t regs[S::NREG];
+    if (t < t_end) {
+        int im = t / per_img, rem = t - im * per_img, ty = rem / tiles_x;
+        load_tile<S, ALIGNED>(src + (size_t)im * stride, W, H,
+                              (rem - ty * tiles_x) * kTW - half - m, ty * kTH2 - half, regs);
+    }
+    for (; t < t_end; t += per_xcd) {
+        const int im = t / per_img, rem = t - im * per_img, ty = rem / tiles_x;
+        const int x0 = (rem - ty * tiles_x) * kTW, y0 = ty * kTH2;
+        uint8_t *out = dst + (size_t)im * stride;
+
+        __syncthreads();  // the previous tile is done with in_s / out_s and ht
+        store_tile<S, ALIGNED>(in_s, W, x0 - half - m, regs);
+        const int tn = t + per_xcd;
+        if (tn < t_end) {
+            int im2 = tn / per_img, rem2 = tn - im2 * per_img, ty2 = rem2 / tiles_x;
+            load_tile<S, ALIGNED>(src + (size_t)im2 * stride, W, H,
+                                  (rem2 - ty2 * tiles_x) * kTW - half - m, ty2 * kTH2 - half,
+                                  regs);
+        }
+        __syncthreads();
+
+        // ---- horizontal pass: 7 x 4 blocks of 32x32 -> ht (column-major u8)
+        int qn = 0;
+        for (int blk = wave; blk < (kRows2 / 32) * (kTW / 32); blk += 4) {
+            const int rb = blk >> 2, cb = blk & 3;
+            v16f acc;
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[r] = init_h;
+            const uint8_t *arow = in_s + (32 * rb + n) * PIN + 32 * cb + 8 * g;
+#pragma unroll
+            for (int kb = 0; kb < NKB; kb++) {
+                v8h av = bytes_to_half8(*reinterpret_cast<const uint2 *>(arow + 16 * kb));
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bh_lo[kb], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bh_hi[kb], acc, 0, 0, 0);
+            }
+            // lane holds column 32*cb + n, rows 32*rb + 8*(r/4) + 4*g + r%4
+            uint8_t *col = ht + (32 * cb + n) * kPitchB + 32 * rb + 4 * g;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                uint32_t u[4];
+                bool f[4];
+                unsigned long long any = 0;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    u[i] = __float_as_uint(acc[4 * j + i]);  // (a bit_cast of a vector element reads element 0 with this compiler)
+                    f[i] = (u[i] & 0x7fffu) < thr_h;
+                    any |= __ballot(f[i]);
+                }
+                *reinterpret_cast<uint32_t *>(col + 8 * j) = pack_bits15(u[0], u[1], u[2], u[3]);
+                if (any) {
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+                        enqueue(f[i], (uint32_t)((32 * rb + 8 * j + 4 * g + i) << 8) | (32 * cb + n),
+                                queue, qn);
+                    if (qn > kQDrain) {
+                        for (int e = lane; e < qn; e += 64) {
+                            const uint32_t ent = queue[e];
+                            const int row = (int)(ent >> 8), cl = (int)(ent & 255);
+                            float r = exact_chain<LMAX>(in_s + row * PIN + m + cl, taps_h, L, 1.0f);
+                            ht[cl * kPitchB + row] = clamp_u8_exact(r);
+                        }
+                        qn = 0;
+                    }
+                }
+            }
+        }
+        for (int e = lane; e < qn; e += 64) {
+            const uint32_t ent = queue[e];
+            const int row = (int)(ent >> 8), cl = (int)(ent & 255);
+            float r = exact_chain<LMAX>(in_s + row * PIN + m + cl, taps_h, L, 1.0f);
+            ht[cl * kPitchB + row] = clamp_u8_exact(r);
+        }
+        __syncthreads();
+
+        // ---- vertical pass: D[m = x][n = y] = sum_k H^T[x][k] B[k][y] -> out_s
+        qn = 0;
+        for (int blk = wave; blk < (kTH2 / 32) * (kTW / 32); blk += 4) {
+            const int rb = blk >> 2, cb = blk & 3;
+            v16f acc;
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[r] = init_v;
+            const uint8_t *acol = ht + (32 * cb + n) * kPitchB + 32 * rb + 8 * g;
+#pragma unroll
+            for (int kb = 0; kb < NKB; kb++) {
+                v8h av = bytes_to_half8(*reinterpret_cast<const uint2 *>(acol + 16 * kb));
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv_lo[kb], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv_hi[kb], acc, 0, 0, 0);
+            }
+            // lane holds output row 32*rb + n, columns 32*cb + 8*(r/4) + 4*g + r%4
+            const int yl = 32 * rb + n;
+            uint8_t *orow = out_s + yl * kOPitch;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int xl = 32 * cb + 8 * j + 4 * g;
+                uint32_t u[4];
+                bool f[4];
+                unsigned long long any = 0;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    u[i] = __float_as_uint(acc[4 * j + i]);  // (a bit_cast of a vector element reads element 0 with this compiler)
+                    f[i] = (u[i] & 0x7fffu) < thr_v;
+                    any |= __ballot(f[i]);
+                }
+                *reinterpret_cast<uint32_t *>(orow + xl) = pack_bits15(u[0], u[1], u[2], u[3]);
+                if (any) {
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+                        enqueue(f[i], (uint32_t)(yl << 8) | (uint32_t)(xl + i), queue, qn);
+                    if (qn > kQDrain) {
+                        for (int e = lane; e < qn; e += 64) {
+                            const uint32_t ent = queue[e];
+                            const int y2 = (int)(ent >> 8), x2 = (int)(ent & 255);
+                            float r = exact_chain<LMAX>(ht + x2 * kPitchB + y2, vts, L, 1.0f);
+                            out_s[y2 * kOPitch + x2] = clamp_u8_exact(r);
+                        }
+                        qn = 0;
+                    }
+                }
+            }
+        }
+        for (int e = lane; e < qn; e += 64) {
+            const uint32_t ent = queue[e];
+            const int y2 = (int)(ent >> 8), x2 = (int)(ent & 255);
+            float r = exact_chain<LMAX>(ht + x2 * kPitchB + y2, vts, L, 1.0f);
+            out_s[y2 * kOPitch + x2] = clamp_u8_exact(r);
+        }
+        __syncthreads();
+
+        // ---- copy the output tile out: 8 lanes x 16 bytes per row
+        const bool vec_ok = ((W & 15) == 0) && ((reinterpret_cast<uintptr_t>(out) & 15) == 0) &&
+                            ((stride & 15) == 0);
+#pragma unroll 2
+        for (int i = threadIdx.x; i < kTH2 * (kTW / 16); i += 256) {
+            const int yl = i >> 3, xl = (i & 7) * 16;
+            const int gy = y0 + yl, gx = x0 + xl;
+            if (gy >= H || gx >= W) continue;
+            const uint4 v = *reinterpret_cast<const uint4 *>(out_s + yl * kOPitch + xl);
+            uint8_t *o = out + (size_t)gy * W + gx;
+            if (vec_ok && gx + 15 < W) {
+                *reinterpret_cast<uint4 *>(o) = v;
+            } else {
+                const uint8_t *b = out_s + yl * kOPitch + xl;
+                for (int q = 0; q < 16 && gx + q < W; q++) o[q] = b[q];
+            }
+        }
+    }
+}
+
 // |E - R| bound for one pass, in output units. E: MFMA estimate, at most
 // 2L non-zero products plus 2*NKB accumulations, each rounded (toward zero
 // in the worst case, 2^-23 relative) at a magnitude <= 255 * sum|w|, plus
@@ -408,6 +661,32 @@ float blur_eps(const float *w, int L, int nkb) {
     double e = mag * (std::ldexp(1.0, -23) * (2.0 * L + 2.0 * nkb) + std::ldexp(1.0, -22) +
                       std::ldexp(1.0, -24) * (2.0 * L));
     return (float)(e * 1.25 + 1e-7);
+}
+
+// Readout margin c of the fixed-point kernel, in units of 2^-15, or -1 when
+// the taps do not qualify. Bound on |E - R| (output units): the MFMA sum rounds
+// at most 2L + 2*NKB times (every non-zero product and every accumulator
+// update; each partial sum is below 2^24 in magnitude, so one rounding costs at
+// most 1 unit = 2^-15), the hi/lo split leaves 2^-22 relative per tap (plus
+// 2^-28 absolute should an f16 part be flushed), and the reference's chain of
+// L products and L sums is off by at most 2L * 2^-24 relative (1 % slack for
+// second-order terms). c exceeds the bound by 25 % plus one unit.
+int blur_fx_margin(const float *w, int L, int nkb) {
+    double sum = 0.0;
+    for (int i = 0; i < L; i++) {
+        if (!(w[i] >= 0.0f) || !std::isfinite(w[i])) return -1;
+        sum += (double)w[i];
+    }
+    const double u = std::ldexp(1.0, -15);
+    double e = (2.0 * L + 2.0 * nkb) * u +
+               255.0 * sum * (std::ldexp(1.0, -22) + 2.02 * L * std::ldexp(1.0, -24)) +
+               255.0 * L * std::ldexp(1.0, -28);
+    double c = std::ceil((1.25 * e + u) / u);
+    // every partial sum and the final value stay below 2^24: the largest
+    // estimate, its rounding and the margin fit under 256
+    if (255.0 * sum * (1.0 + std::ldexp(1.0, -10)) + 2.0 * c * u >= 255.99) return -1;
+    if (c >= 4096.0) return -1;
+    return (int)c;
 }
 
 }  // namespace
@@ -490,6 +769,51 @@ iph_status launch_blur_u8_mfma(const uint8_t *src, uint8_t *dst, uint32_t w, uin
 #define LK(K)                                                                                \
     if (same) { LA(K, true); } else { LA(K, false); }                                        \
     break
+    const int ch = blur_fx_margin(hk, L, nkb), cv = blur_fx_margin(vk, L, nkb);
+    if (ch > 0 && cv > 0) {
+        FxArgs f{};
+        for (int i = 0; i < L; i++) {
+            f.h[i] = hk[i];
+            f.v[i] = vk[i];
+        }
+        f.L = L;
+        f.m = m;
+        f.ch = ch;
+        f.cv = cv;
+        const int ftx = (int)((w + kTW - 1) / kTW), fty = (int)((h + kTH2 - 1) / kTH2);
+        const long long fnt = (long long)ftx * fty * n;
+#define FB(K, SAME, AL)                                                                      \
+    do {                                                                                     \
+        using Sh = BlurShape<K, kRows2>;                                                     \
+        size_t lds = (size_t)((kRows2 * Sh::PIN + 15) / 16 * 16) + (size_t)kTW * kPitchB +   \
+                     2 * (2 * kTapPad + kMaxTaps) * sizeof(float) +                          \
+                     4 * kQCap * sizeof(uint32_t);                                            \
+        int per_cu = 0;                                                                      \
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(                                    \
+                &per_cu, blur_u8_fx_kernel<K, SAME, AL>, 256, lds) != hipSuccess ||          \
+            per_cu < 1)                                                                      \
+            per_cu = 1;                                                                      \
+        long long grid = (long long)cus * per_cu;                                            \
+        grid = std::min<long long>(grid, (fnt + 7) / 8 * 8);                                 \
+        grid = std::max<long long>(8, grid / 8 * 8);                                         \
+        blur_u8_fx_kernel<K, SAME, AL><<<(unsigned)grid, 256, lds, s>>>(                    \
+            src, dst, (int)w, (int)h, stride, (int)n, ftx, fty, f);                          \
+    } while (0)
+#define FA(K, SAME)                                                                          \
+    if (aligned) FB(K, SAME, true); else FB(K, SAME, false)
+#define FK(K)                                                                                \
+    if (same) { FA(K, true); } else { FA(K, false); }                                        \
+    break
+        switch (nkb) {
+        case 2: FK(2);
+        case 3: FK(3);
+        default: FK(4);
+        }
+#undef FK
+#undef FA
+#undef FB
+        return launch_status("blur_u8_fx_kernel");
+    }
     switch (nkb) {
     case 2: LK(2);
     case 3: LK(3);
