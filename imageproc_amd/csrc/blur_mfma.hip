@@ -120,6 +120,18 @@ __device__ __forceinline__ void enqueue(bool flag, uint32_t entry, uint32_t *q, 
     }
 }
 
+// Appends this lane's 16-bit entry when `flag` is set; `m` is the flag's
+// wave-wide ballot. The caller guarantees room (qn + popcount(m) <= capacity).
+__device__ __forceinline__ void enqueue_ballot(bool flag, unsigned long long m, uint32_t entry,
+                                               uint16_t *q, int &qn) {
+    if (m) {
+        const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        if (flag) q[qn + below] = (uint16_t)entry;
+        qn += __popcll(m);
+    }
+}
+
 // The reference's sequential chain over taps w*2^S (exact: power-of-two
 // scaling commutes with every f32 rounding here), returned unscaled.
 // `stride` steps through the pixels; all loads are issued before the chain.
@@ -134,6 +146,24 @@ __device__ __forceinline__ float exact_chain(const T *p, const float *taps_s, in
     for (int q = 0; q < LMAX; q++)
         if (q < L) s = __fadd_rn(s, __fmul_rn(v[q], taps_s[kTapPad + q]));
     return s * inv_scale;
+}
+
+// exact_chain with its loads issued eight at a time (fewer live registers).
+template <int LMAX, typename T>
+__device__ __forceinline__ float exact_chain8(const T *p, const float *taps, int L) {
+    float s = 0.0f;
+#pragma unroll
+    for (int q0 = 0; q0 < LMAX; q0 += 8) {
+        float v[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            if (q0 + q < LMAX) v[q] = (float)p[q0 + q];
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            if (q0 + q < LMAX && q0 + q < L)
+                s = __fadd_rn(s, __fmul_rn(v[q], taps[kTapPad + q0 + q]));
+    }
+    return s;
 }
 
 // Staged rows per tile: max(kTH + L - 1, 32*(kTH/32 - 1) + 16*NKB) <= 128
@@ -397,20 +427,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 
 // ---- fixed-point readout: taps >= 0 with 255 * sum(w) < 256 ------------------
 //
-// With taps scaled by 2^15 and the accumulator started at 2^23 + c, every
-// partial sum of the (non-negative) products stays in [2^23, 2^24), where the
-// f32 ulp is 1. The final value is 2^23 + c + E, E = the estimate in units of
-// 2^-15, so its bit pattern u carries floor((E + c) / 2^15) in bits 15..22 and
-// (E + c) mod 2^15 in bits 0..14:
+// Taps are scaled by 2^15 and split w * 2^15 = hi + lo with hi = rint(w * 2^15)
+// (an integer, exact in f16) and lo in f16. The hi products p * hi are
+// integers, accumulated from 2^23 + c: every partial sum is an integer below
+// 2^24, so that accumulator is exact whatever order the MFMA sums in. The lo
+// products go to a second accumulator started at 0, whose partial sums stay
+// small enough that its roundings are negligible. Their f32 sum is
+// 2^23 + c + E, E the estimate in units of 2^-15 (one rounding, at most half a
+// unit), and its bit pattern u carries floor((E + c) / 2^15) in bits 15..22
+// and (E + c) mod 2^15 in bits 0..14:
 //   * (u & 0x7fff) >= 2c  proves the estimate is at least c units (>= eps) away
 //     from an integer, so bits 15..22 are the reference's truncated result (the
 //     reference's sum lies in [0, 256), where its clamp is the identity);
 //   * otherwise the output is recomputed with the reference's chain.
-// Readout is one AND, one compare and one byte insert per output.
+// Readout is an add, an AND, a compare and 1.25 byte-packing ops per output.
 
-constexpr int kTH2 = 192;           // output rows per tile (6 row blocks)
-constexpr int kRows2 = 224;         // staged rows: kTH2 + 32 covers L <= 33
-constexpr int kPitchB = kRows2 + 8; // bytes per intermediate column (58 dwords)
+// Flagged-output queue of the fixed-point kernel: 16-bit entries (row << 8 |
+// column), room for half a block (512 outputs) above the drain threshold, so
+// the queue is drained at most twice per block.
+constexpr int kFxQDrain = 64;
+constexpr int kFxQCap = kFxQDrain + 512;
+
+constexpr int kTH2 = 128;           // output rows per tile (4 row blocks)
+constexpr int kRows2 = 160;         // staged rows: kTH2 + 32 covers L <= 33
+constexpr int kPitchB = kRows2 + 8; // bytes per intermediate column (42 dwords)
 constexpr int kOPitch = kTW + 16;   // bytes per output-tile row
 
 struct FxArgs {
@@ -421,7 +461,8 @@ struct FxArgs {
     int ch, cv;  // readout margins c (units of 2^-15) of the two passes
 };
 
-// Toeplitz B operand with taps * 2^15 split hi + lo (see build_b).
+// Toeplitz B operand (see build_b) with taps * 2^15 split into an integer hi
+// and an f16 lo.
 template <int NKB>
 __device__ __forceinline__ void build_b_fx(const float *taps, int lane, int shift,
                                            v8h (&bhi)[NKB], v8h (&blo)[NKB]) {
@@ -431,7 +472,7 @@ __device__ __forceinline__ void build_b_fx(const float *taps, int lane, int shif
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             float w = taps[kTapPad + 16 * kb + 8 * g + j - n - shift] * 32768.0f;
-            _Float16 hi = (_Float16)w;
+            _Float16 hi = (_Float16)rintf(w);
             _Float16 lo = (_Float16)(w - (float)hi);
             bhi[kb][j] = hi;
             blo[kb][j] = lo;
@@ -456,11 +497,101 @@ __device__ __forceinline__ uint32_t pack_bits15(uint32_t u0, uint32_t u1, uint32
                                  __builtin_bit_cast(uint32_t, h01), 0x06040200u);
 }
 
-template <int NKB, bool SAME_TAPS, bool ALIGNED>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void blur_u8_fx_kernel(
+// Row-wise staging of the fixed-point kernel's input window: each wave loads
+// RPW whole staged rows per step, CH lanes per row, 16 bytes per lane. Only
+// the row index depends on the step, so a load costs a clamp and a multiply.
+template <int NKB, int ROWS> struct RowStage {
+    static constexpr int CIN = kTW - 32 + 16 * NKB;  // staged columns (multiple of 16)
+    static constexpr int CH = CIN / 16;              // 16-byte chunks per row
+    static constexpr int RPW = 64 / CH;              // rows per wave per step
+    static constexpr int RPK = 4 * RPW;              // rows per workgroup step
+    static constexpr int NLD = (ROWS + RPK - 1) / RPK;
+    static constexpr int PIN = ((CIN / 4) % 2 == 0) ? CIN + 8 : CIN + 4;
+    static constexpr int LMAX = 16 * NKB - 31;
+};
+
+// Issues this thread's loads of one tile's window (rows clamp(ys + r), columns
+// xs + c; xs is a multiple of 4). INTERIOR tiles (the window lies inside the
+// row) load 16 bytes at once; at the left and right image edges ALIGNED rows
+// load whole dwords from the clamped column and store_rows replicates the edge
+// byte; unaligned rows load each byte from its clamped column.
+template <typename RS, bool ALIGNED>
+__device__ __forceinline__ void load_rows(const uint8_t *__restrict__ img, int W, int H, int xs,
+                                          int ys, bool interior, uint4 (&regs)[RS::NLD]) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int rr = lane / RS::CH, c0 = (lane - rr * RS::CH) * 16;
+    if (rr >= RS::RPW) return;
+    if (ALIGNED && interior) {
+#pragma unroll
+        for (int k = 0; k < RS::NLD; k++) {
+            const int r = k * RS::RPK + wave * RS::RPW + rr;
+            const uint32_t row = (uint32_t)clampi(ys + min(r, kRows2 - 1), 0, H - 1);
+            regs[k] = *reinterpret_cast<const uint4 *>(img + (row * (uint32_t)W + (uint32_t)(xs + c0)));
+        }
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < RS::NLD; k++) {
+        const int r = k * RS::RPK + wave * RS::RPW + rr;
+        const uint8_t *row = img + (size_t)clampi(ys + min(r, kRows2 - 1), 0, H - 1) * W;
+        uint32_t d[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int gx = xs + c0 + 4 * q;
+            if (ALIGNED) {
+                d[q] = *reinterpret_cast<const uint32_t *>(row + clampi(gx, 0, W - 4));
+            } else {
+                d[q] = (uint32_t)row[clampi(gx, 0, W - 1)] |
+                       ((uint32_t)row[clampi(gx + 1, 0, W - 1)] << 8) |
+                       ((uint32_t)row[clampi(gx + 2, 0, W - 1)] << 16) |
+                       ((uint32_t)row[clampi(gx + 3, 0, W - 1)] << 24);
+            }
+        }
+        regs[k] = make_uint4(d[0], d[1], d[2], d[3]);
+    }
+}
+
+template <typename RS, bool ALIGNED>
+__device__ __forceinline__ void store_rows(uint8_t *in_s, int W, int xs, bool interior,
+                                           const uint4 (&regs)[RS::NLD]) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int rr = lane / RS::CH, c0 = (lane - rr * RS::CH) * 16;
+    if (rr >= RS::RPW) return;
+    uint8_t *base = in_s + (wave * RS::RPW + rr) * RS::PIN + c0;
+#pragma unroll
+    for (int k = 0; k < RS::NLD; k++) {
+        const int r = k * RS::RPK + wave * RS::RPW + rr;
+        if (k == RS::NLD - 1 && r >= kRows2) break;
+        uint32_t d[4] = {regs[k].x, regs[k].y, regs[k].z, regs[k].w};
+        if (ALIGNED && !interior) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int gx = xs + c0 + 4 * q;
+                const uint32_t left = (d[q] & 0xffu) * 0x01010101u;
+                const uint32_t right = (d[q] >> 24) * 0x01010101u;
+                d[q] = gx < 0 ? left : d[q];
+                d[q] = gx >= W ? right : d[q];
+            }
+        }
+        uint8_t *dst = base + k * RS::RPK * RS::PIN;
+        *reinterpret_cast<uint2 *>(dst) = make_uint2(d[0], d[1]);
+        *reinterpret_cast<uint2 *>(dst + 8) = make_uint2(d[2], d[3]);
+    }
+}
+
+// Waves per SIMD the fixed-point kernel is compiled for: 3 (three workgroups
+// per CU) where its registers fit in 168 without spilling, else 2.
+constexpr int fx_waves(int nkb, int taps, bool aligned) {
+    return (taps == 2 && nkb <= 3) ? 3 : 2;
+}
+
+// TAPS: 0 = distinct horizontal and vertical taps; 1 = equal taps; 2 = equal
+// taps and no window shift (m = 0), so both passes use the same B operands.
+template <int NKB, int TAPS, bool ALIGNED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fx_waves(NKB, TAPS, ALIGNED), fx_waves(NKB, TAPS, ALIGNED)))) void blur_u8_fx_kernel(
     const uint8_t *__restrict__ src, uint8_t *__restrict__ dst, int W, int H, size_t stride,
     int n_img, int tiles_x, int tiles_y, FxArgs a) {
-    using S = BlurShape<NKB, kRows2>;
+    using S = RowStage<NKB, kRows2>;
     constexpr int PIN = S::PIN, LMAX = S::LMAX;
     constexpr int IN_BYTES = (kRows2 * PIN + 15) / 16 * 16;
     static_assert(kTH2 * kOPitch <= IN_BYTES, "output tile reuses the input window");
@@ -470,8 +601,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     uint8_t *ht = smem + IN_BYTES;            // kTW x kPitchB, column-major u8
     float *taps_h = reinterpret_cast<float *>(ht + kTW * kPitchB);
     float *taps_v = taps_h + (2 * kTapPad + kMaxTaps);
-    uint32_t *queue = reinterpret_cast<uint32_t *>(taps_v + (2 * kTapPad + kMaxTaps)) +
-                      (threadIdx.x >> 6) * kQCap;
+    uint16_t *queue = reinterpret_cast<uint16_t *>(taps_v + (2 * kTapPad + kMaxTaps)) +
+                      (threadIdx.x >> 6) * kFxQCap;
     const int L = a.L;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int n = lane & 31, g = lane >> 5;
@@ -484,11 +615,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         taps_v[threadIdx.x] = in ? a.v[t] : 0.0f;
     }
     __syncthreads();
-    v8h bh_hi[NKB], bh_lo[NKB], bv_hi[NKB], bv_lo[NKB];
-    const int m = a.m;
+    constexpr bool SHARE_B = TAPS == 2;
+    v8h bh_hi[NKB], bh_lo[NKB], bv_hi_[SHARE_B ? 1 : NKB], bv_lo_[SHARE_B ? 1 : NKB];
+    const int m = SHARE_B ? 0 : a.m;
     build_b_fx<NKB>(taps_h, lane, m, bh_hi, bh_lo);
-    build_b_fx<NKB>(taps_v, lane, 0, bv_hi, bv_lo);
-    const float *vts = SAME_TAPS ? taps_h : taps_v;
+    if constexpr (!SHARE_B) {
+        build_b_fx<NKB>(taps_v, lane, 0, bv_hi_, bv_lo_);
+    }
+    auto &bv_hi = *reinterpret_cast<v8h(*)[NKB]>(SHARE_B ? (void *)bh_hi : (void *)bv_hi_);
+    auto &bv_lo = *reinterpret_cast<v8h(*)[NKB]>(SHARE_B ? (void *)bh_lo : (void *)bv_lo_);
+    const float *vts = TAPS != 0 ? taps_h : taps_v;
     const float init_h = 8388608.0f + (float)a.ch, init_v = 8388608.0f + (float)a.cv;
     const uint32_t thr_h = 2u * (uint32_t)a.ch, thr_v = 2u * (uint32_t)a.cv;
 
@@ -500,11 +636,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const int t_end = min(ntiles, t_begin + chunk);
     int t = t_begin + (int)(blockIdx.x >> 3);
 
-    uint32_t regs[S::NREG];
+    // a window lies inside its rows when xs >= 0 and xs + CIN <= W
+    uint4 regs[S::NLD];
+    bool interior = false;
     if (t < t_end) {
         int im = t / per_img, rem = t - im * per_img, ty = rem / tiles_x;
-        load_tile<S, ALIGNED>(src + (size_t)im * stride, W, H,
-                              (rem - ty * tiles_x) * kTW - half - m, ty * kTH2 - half, regs);
+        const int xs = (rem - ty * tiles_x) * kTW - half - m;
+        interior = xs >= 0 && xs + S::CIN <= W;
+        load_rows<S, ALIGNED>(src + (size_t)im * stride, W, H, xs, ty * kTH2 - half, interior,
+                              regs);
     }
     for (; t < t_end; t += per_xcd) {
         const int im = t / per_img, rem = t - im * per_img, ty = rem / tiles_x;
@@ -512,84 +652,120 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         uint8_t *out = dst + (size_t)im * stride;
 
         __syncthreads();  // the previous tile is done with in_s / out_s and ht
-        store_tile<S, ALIGNED>(in_s, W, x0 - half - m, regs);
+        store_rows<S, ALIGNED>(in_s, W, x0 - half - m, interior, regs);
         const int tn = t + per_xcd;
         if (tn < t_end) {
             int im2 = tn / per_img, rem2 = tn - im2 * per_img, ty2 = rem2 / tiles_x;
-            load_tile<S, ALIGNED>(src + (size_t)im2 * stride, W, H,
-                                  (rem2 - ty2 * tiles_x) * kTW - half - m, ty2 * kTH2 - half,
-                                  regs);
+            const int xs2 = (rem2 - ty2 * tiles_x) * kTW - half - m;
+            interior = xs2 >= 0 && xs2 + S::CIN <= W;
+            load_rows<S, ALIGNED>(src + (size_t)im2 * stride, W, H, xs2, ty2 * kTH2 - half,
+                                  interior, regs);
         }
         __syncthreads();
 
-        // ---- horizontal pass: 7 x 4 blocks of 32x32 -> ht (column-major u8)
+        // ---- horizontal pass: (kRows2 / 32) x 4 blocks of 32x32 -> ht (column-major u8);
+        // the A operand of a wave's next block is read while the current one is read out.
         int qn = 0;
-        for (int blk = wave; blk < (kRows2 / 32) * (kTW / 32); blk += 4) {
+        auto h_load = [&](int blk, uint2 (&raw)[NKB]) {
             const int rb = blk >> 2, cb = blk & 3;
-            v16f acc;
-#pragma unroll
-            for (int r = 0; r < 16; r++) acc[r] = init_h;
             const uint8_t *arow = in_s + (32 * rb + n) * PIN + 32 * cb + 8 * g;
 #pragma unroll
+            for (int kb = 0; kb < NKB; kb++) raw[kb] = *reinterpret_cast<const uint2 *>(arow + 16 * kb);
+        };
+        auto h_mma = [&](const uint2 (&raw)[NKB], v16f &acc, v16f &acl) {
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[r] = init_h;
+#pragma unroll
+            for (int r = 0; r < 16; r++) acl[r] = 0.0f;
+#pragma unroll
             for (int kb = 0; kb < NKB; kb++) {
-                v8h av = bytes_to_half8(*reinterpret_cast<const uint2 *>(arow + 16 * kb));
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bh_lo[kb], acc, 0, 0, 0);
+                v8h av = bytes_to_half8(raw[kb]);
                 acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bh_hi[kb], acc, 0, 0, 0);
+                acl = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bh_lo[kb], acl, 0, 0, 0);
             }
-            // lane holds column 32*cb + n, rows 32*rb + 8*(r/4) + 4*g + r%4
+        };
+        auto h_drain = [&]() {
+            for (int e = lane; e < qn; e += 64) {
+                const uint32_t ent = queue[e];
+                const int row = (int)(ent >> 8), cl = (int)(ent & 255);
+                float r = exact_chain8<LMAX>(in_s + row * PIN + m + cl, taps_h, L);
+                ht[cl * kPitchB + row] = clamp_u8_exact(r);
+            }
+            qn = 0;
+        };
+        // lane holds column 32*cb + n, rows 32*rb + 8*(r/4) + 4*g + r%4
+        auto h_out = [&](int blk, const v16f &acc, const v16f &acl) {
+            const int rb = blk >> 2, cb = blk & 3;
             uint8_t *col = ht + (32 * cb + n) * kPitchB + 32 * rb + 4 * g;
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 uint32_t u[4];
                 bool f[4];
-                unsigned long long any = 0;
+                unsigned long long bal[4], any = 0;
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
-                    u[i] = __float_as_uint(acc[4 * j + i]);  // (a bit_cast of a vector element reads element 0 with this compiler)
+                    // (a bit_cast of a vector element reads element 0 with this compiler)
+                    u[i] = __float_as_uint(acc[4 * j + i] + acl[4 * j + i]);
                     f[i] = (u[i] & 0x7fffu) < thr_h;
-                    any |= __ballot(f[i]);
+                    bal[i] = __ballot(f[i]);
+                    any |= bal[i];
                 }
                 *reinterpret_cast<uint32_t *>(col + 8 * j) = pack_bits15(u[0], u[1], u[2], u[3]);
                 if (any) {
+                    const uint32_t e0 = (uint32_t)((32 * rb + 8 * j + 4 * g) << 8) | (32 * cb + n);
 #pragma unroll
-                    for (int i = 0; i < 4; i++)
-                        enqueue(f[i], (uint32_t)((32 * rb + 8 * j + 4 * g + i) << 8) | (32 * cb + n),
-                                queue, qn);
-                    if (qn > kQDrain) {
-                        for (int e = lane; e < qn; e += 64) {
-                            const uint32_t ent = queue[e];
-                            const int row = (int)(ent >> 8), cl = (int)(ent & 255);
-                            float r = exact_chain<LMAX>(in_s + row * PIN + m + cl, taps_h, L, 1.0f);
-                            ht[cl * kPitchB + row] = clamp_u8_exact(r);
-                        }
-                        qn = 0;
-                    }
+                    for (int i = 0; i < 4; i++) enqueue_ballot(f[i], bal[i], e0 + (i << 8), queue, qn);
                 }
+                if ((j & 1) && qn > kFxQDrain) h_drain();
+            }
+        };
+        {
+            constexpr int NB = (kRows2 / 32) * (kTW / 32) / 4;  // blocks per wave
+            uint2 raw[NKB];
+            h_load(wave, raw);
+#pragma unroll 1
+            for (int i = 0; i < NB; i++) {
+                v16f acc, acl;
+                h_mma(raw, acc, acl);
+                if (i + 1 < NB) h_load(wave + 4 * (i + 1), raw);
+                h_out(wave + 4 * i, acc, acl);
             }
         }
-        for (int e = lane; e < qn; e += 64) {
-            const uint32_t ent = queue[e];
-            const int row = (int)(ent >> 8), cl = (int)(ent & 255);
-            float r = exact_chain<LMAX>(in_s + row * PIN + m + cl, taps_h, L, 1.0f);
-            ht[cl * kPitchB + row] = clamp_u8_exact(r);
-        }
+        h_drain();
         __syncthreads();
 
         // ---- vertical pass: D[m = x][n = y] = sum_k H^T[x][k] B[k][y] -> out_s
         qn = 0;
-        for (int blk = wave; blk < (kTH2 / 32) * (kTW / 32); blk += 4) {
+        auto v_load = [&](int blk, uint2 (&raw)[NKB]) {
             const int rb = blk >> 2, cb = blk & 3;
-            v16f acc;
-#pragma unroll
-            for (int r = 0; r < 16; r++) acc[r] = init_v;
             const uint8_t *acol = ht + (32 * cb + n) * kPitchB + 32 * rb + 8 * g;
 #pragma unroll
+            for (int kb = 0; kb < NKB; kb++) raw[kb] = *reinterpret_cast<const uint2 *>(acol + 16 * kb);
+        };
+        auto v_mma = [&](const uint2 (&raw)[NKB], v16f &acc, v16f &acl) {
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[r] = init_v;
+#pragma unroll
+            for (int r = 0; r < 16; r++) acl[r] = 0.0f;
+#pragma unroll
             for (int kb = 0; kb < NKB; kb++) {
-                v8h av = bytes_to_half8(*reinterpret_cast<const uint2 *>(acol + 16 * kb));
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv_lo[kb], acc, 0, 0, 0);
+                v8h av = bytes_to_half8(raw[kb]);
                 acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv_hi[kb], acc, 0, 0, 0);
+                acl = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv_lo[kb], acl, 0, 0, 0);
             }
-            // lane holds output row 32*rb + n, columns 32*cb + 8*(r/4) + 4*g + r%4
+        };
+        auto v_drain = [&]() {
+            for (int e = lane; e < qn; e += 64) {
+                const uint32_t ent = queue[e];
+                const int y2 = (int)(ent >> 8), x2 = (int)(ent & 255);
+                float r = exact_chain8<LMAX>(ht + x2 * kPitchB + y2, vts, L);
+                out_s[y2 * kOPitch + x2] = clamp_u8_exact(r);
+            }
+            qn = 0;
+        };
+        // lane holds output row 32*rb + n, columns 32*cb + 8*(r/4) + 4*g + r%4
+        auto v_out = [&](int blk, const v16f &acc, const v16f &acl) {
+            const int rb = blk >> 2, cb = blk & 3;
             const int yl = 32 * rb + n;
             uint8_t *orow = out_s + yl * kOPitch;
 #pragma unroll
@@ -597,36 +773,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 const int xl = 32 * cb + 8 * j + 4 * g;
                 uint32_t u[4];
                 bool f[4];
-                unsigned long long any = 0;
+                unsigned long long bal[4], any = 0;
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
-                    u[i] = __float_as_uint(acc[4 * j + i]);  // (a bit_cast of a vector element reads element 0 with this compiler)
+                    u[i] = __float_as_uint(acc[4 * j + i] + acl[4 * j + i]);
                     f[i] = (u[i] & 0x7fffu) < thr_v;
-                    any |= __ballot(f[i]);
+                    bal[i] = __ballot(f[i]);
+                    any |= bal[i];
                 }
                 *reinterpret_cast<uint32_t *>(orow + xl) = pack_bits15(u[0], u[1], u[2], u[3]);
                 if (any) {
+                    const uint32_t e0 = (uint32_t)(yl << 8) | (uint32_t)xl;
 #pragma unroll
-                    for (int i = 0; i < 4; i++)
-                        enqueue(f[i], (uint32_t)(yl << 8) | (uint32_t)(xl + i), queue, qn);
-                    if (qn > kQDrain) {
-                        for (int e = lane; e < qn; e += 64) {
-                            const uint32_t ent = queue[e];
-                            const int y2 = (int)(ent >> 8), x2 = (int)(ent & 255);
-                            float r = exact_chain<LMAX>(ht + x2 * kPitchB + y2, vts, L, 1.0f);
-                            out_s[y2 * kOPitch + x2] = clamp_u8_exact(r);
-                        }
-                        qn = 0;
-                    }
+                    for (int i = 0; i < 4; i++) enqueue_ballot(f[i], bal[i], e0 + i, queue, qn);
                 }
+                if ((j & 1) && qn > kFxQDrain) v_drain();
+            }
+        };
+        {
+            constexpr int NB = (kTH2 / 32) * (kTW / 32) / 4;  // blocks per wave
+            uint2 raw[NKB];
+            v_load(wave, raw);
+#pragma unroll 1
+            for (int i = 0; i < NB; i++) {
+                v16f acc, acl;
+                v_mma(raw, acc, acl);
+                if (i + 1 < NB) v_load(wave + 4 * (i + 1), raw);
+                v_out(wave + 4 * i, acc, acl);
             }
         }
-        for (int e = lane; e < qn; e += 64) {
-            const uint32_t ent = queue[e];
-            const int y2 = (int)(ent >> 8), x2 = (int)(ent & 255);
-            float r = exact_chain<LMAX>(ht + x2 * kPitchB + y2, vts, L, 1.0f);
-            out_s[y2 * kOPitch + x2] = clamp_u8_exact(r);
-        }
+        v_drain();
         __syncthreads();
 
         // ---- copy the output tile out: 8 lanes x 16 bytes per row
@@ -664,27 +840,42 @@ float blur_eps(const float *w, int L, int nkb) {
 }
 
 // Readout margin c of the fixed-point kernel, in units of 2^-15, or -1 when
-// the taps do not qualify. Bound on |E - R| (output units): the MFMA sum rounds
-// at most 2L + 2*NKB times (every non-zero product and every accumulator
-// update; each partial sum is below 2^24 in magnitude, so one rounding costs at
-// most 1 unit = 2^-15), the hi/lo split leaves 2^-22 relative per tap (plus
-// 2^-28 absolute should an f16 part be flushed), and the reference's chain of
-// L products and L sums is off by at most 2L * 2^-24 relative (1 % slack for
-// second-order terms). c exceeds the bound by 25 % plus one unit.
+// the taps do not qualify. It bounds |E - R| in units, E the kernel's
+// estimate and R the reference's f32 result:
+//   * the hi accumulator is exact; the lo accumulator's partial sums stay
+//     below B_lo = 255 * sum|lo|, so each of its at most 2L + 2*NKB roundings
+//     costs at most B_lo * 2^-23; the final f32 add rounds once (half a unit);
+//   * the split leaves |w * 2^15 - hi - lo| per tap, times a pixel <= 255
+//     (an f16 lo below 2^-14 counts as lost entirely, should it be flushed);
+//   * the reference's chain: product i is rounded once and then passes
+//     through L - max(i, 1) rounded additions (the first addition, 0 + x0,
+//     is exact), each off by at most 2^-24 relative, all terms >= 0 and
+//     pixels <= 255 (1 % slack for second-order terms).
+// c exceeds the bound by 25 % plus one unit.
 int blur_fx_margin(const float *w, int L, int nkb) {
-    double sum = 0.0;
+    double sum = 0.0, sum_hi = 0.0, lo_abs = 0.0, split = 0.0, chain = 0.0;
     for (int i = 0; i < L; i++) {
         if (!(w[i] >= 0.0f) || !std::isfinite(w[i])) return -1;
+        const float ws = w[i] * 32768.0f;
+        const _Float16 hi = (_Float16)rintf(ws);
+        const _Float16 lo = (_Float16)(ws - (float)hi);
         sum += (double)w[i];
+        sum_hi += (double)(float)hi;
+        lo_abs += std::fabs((double)(float)lo);
+        double res = std::fabs((double)ws - (double)(float)hi - (double)(float)lo);
+        if (std::fabs((double)(float)lo) < std::ldexp(1.0, -14)) res += std::fabs((double)(float)lo);
+        split += res;
+        chain += (double)w[i] * (1.0 + (double)(L - std::max(i, 1)));
     }
-    const double u = std::ldexp(1.0, -15);
-    double e = (2.0 * L + 2.0 * nkb) * u +
-               255.0 * sum * (std::ldexp(1.0, -22) + 2.02 * L * std::ldexp(1.0, -24)) +
-               255.0 * L * std::ldexp(1.0, -28);
-    double c = std::ceil((1.25 * e + u) / u);
-    // every partial sum and the final value stay below 2^24: the largest
-    // estimate, its rounding and the margin fit under 256
-    if (255.0 * sum * (1.0 + std::ldexp(1.0, -10)) + 2.0 * c * u >= 255.99) return -1;
+    const double b_lo = 255.0 * lo_abs;
+    double e = 0.5 + (2.0 * L + 2.0 * nkb) * b_lo * std::ldexp(1.0, -23) + 255.0 * split +
+               32768.0 * 255.0 * chain * std::ldexp(1.0, -24) * 1.01;
+    double c = std::ceil(1.25 * e + 1.0);
+    // every partial sum of the hi accumulator, and the final value, stay
+    // below 2^24: the largest estimate plus the margin fits under 256
+    if (255.0 * std::max(sum_hi + lo_abs, sum * 32768.0 * (1.0 + 1e-6)) + 2.0 * c >=
+        255.99 * 32768.0)
+        return -1;
     if (c >= 4096.0) return -1;
     return (int)c;
 }
@@ -784,10 +975,10 @@ iph_status launch_blur_u8_mfma(const uint8_t *src, uint8_t *dst, uint32_t w, uin
         const long long fnt = (long long)ftx * fty * n;
 #define FB(K, SAME, AL)                                                                      \
     do {                                                                                     \
-        using Sh = BlurShape<K, kRows2>;                                                     \
+        using Sh = RowStage<K, kRows2>;                                                     \
         size_t lds = (size_t)((kRows2 * Sh::PIN + 15) / 16 * 16) + (size_t)kTW * kPitchB +   \
                      2 * (2 * kTapPad + kMaxTaps) * sizeof(float) +                          \
-                     4 * kQCap * sizeof(uint32_t);                                            \
+                     4 * kFxQCap * sizeof(uint16_t);                                          \
         int per_cu = 0;                                                                      \
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(                                    \
                 &per_cu, blur_u8_fx_kernel<K, SAME, AL>, 256, lds) != hipSuccess ||          \
@@ -802,7 +993,7 @@ iph_status launch_blur_u8_mfma(const uint8_t *src, uint8_t *dst, uint32_t w, uin
 #define FA(K, SAME)                                                                          \
     if (aligned) FB(K, SAME, true); else FB(K, SAME, false)
 #define FK(K)                                                                                \
-    if (same) { FA(K, true); } else { FA(K, false); }                                        \
+    if (same && m == 0) { FA(K, 2); } else if (same) { FA(K, 1); } else { FA(K, 0); }       \
     break
         switch (nkb) {
         case 2: FK(2);
