@@ -453,16 +453,43 @@ constexpr int kRows2 = 160;         // staged rows: kTH2 + 32 covers L <= 33
 constexpr int kPitchB = kRows2 + 8; // bytes per intermediate column (42 dwords)
 constexpr int kOPitch = kTW + 16;   // bytes per output-tile row
 
+// The A operand is fed as 1024 + p (the byte-to-f16 permute without the
+// subtraction); taps are scaled by 2^13 so that every partial sum of the
+// integer hi products (1024 + p) * hi stays below 2^24 in magnitude, and the
+// 1024 * sum(hi) offset is folded into the accumulator's start value.
+#ifndef IPH_FX_BIASED_A
+#define IPH_FX_BIASED_A 0
+#endif
+constexpr bool kFxBiasedA = IPH_FX_BIASED_A;
+constexpr int kFxShift = kFxBiasedA ? 13 : 15;  // estimates in units of 2^-kFxShift
+constexpr uint32_t kFxFracMask = (1u << kFxShift) - 1u;
+
 struct FxArgs {
     float h[kMaxTaps];
     float v[kMaxTaps];
     int L;
     int m;
-    int ch, cv;  // readout margins c (units of 2^-15) of the two passes
+    int ch, cv;                  // readout margins c (units of 2^-kFxShift) of the two passes
+    float init_h, init_v;        // hi accumulator start values (integers) of the two passes
 };
 
-// Toeplitz B operand (see build_b) with taps * 2^15 split into an integer hi
-// and an f16 lo.
+// 8 u8 pixels -> 8 f16 values 1024 + p (exact).
+__device__ __forceinline__ v8h bytes_to_half8_biased(uint2 b) {
+    const uint32_t magic = 0x64646464u;
+    uint32_t q[4] = {__builtin_amdgcn_perm(magic, b.x, 0x04010400u),
+                     __builtin_amdgcn_perm(magic, b.x, 0x04030402u),
+                     __builtin_amdgcn_perm(magic, b.y, 0x04010400u),
+                     __builtin_amdgcn_perm(magic, b.y, 0x04030402u)};
+    return __builtin_bit_cast(v8h, q);
+}
+
+__device__ __forceinline__ v8h fx_a_operand(uint2 b) {
+    if constexpr (kFxBiasedA) return bytes_to_half8_biased(b);
+    else return bytes_to_half8(b);
+}
+
+// Toeplitz B operand (see build_b) with taps * 2^kFxShift split into an
+// integer hi and an f16 lo.
 template <int NKB>
 __device__ __forceinline__ void build_b_fx(const float *taps, int lane, int shift,
                                            v8h (&bhi)[NKB], v8h (&blo)[NKB]) {
@@ -471,7 +498,7 @@ __device__ __forceinline__ void build_b_fx(const float *taps, int lane, int shif
     for (int kb = 0; kb < NKB; kb++) {
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-            float w = taps[kTapPad + 16 * kb + 8 * g + j - n - shift] * 32768.0f;
+            float w = taps[kTapPad + 16 * kb + 8 * g + j - n - shift] * (float)(1 << kFxShift);
             _Float16 hi = (_Float16)rintf(w);
             _Float16 lo = (_Float16)(w - (float)hi);
             bhi[kb][j] = hi;
@@ -482,17 +509,19 @@ __device__ __forceinline__ void build_b_fx(const float *taps, int lane, int shif
 
 typedef unsigned short v2u16 __attribute__((ext_vector_type(2)));
 
-// Bits 15..22 of four accumulator bit patterns, packed into one dword: bytes
-// 1..2 of two values form two 16-bit halves (bits 8..23), a packed shift by 7
-// leaves bits 15..23 in each half (bit 23, the lowest exponent bit of a value
-// in [2^23, 2^24), is zero), and a last byte permute gathers the four bytes.
+// Bits kFxShift..kFxShift+7 of four accumulator bit patterns, packed into one
+// dword: bytes 1..2 of two values form two 16-bit halves (bits 8..23), a
+// packed shift by kFxShift - 8 leaves the byte alone in each half (the bits
+// above it up to bit 22 are zero, and bit 23, the lowest exponent bit of a
+// value in [2^23, 2^24), is zero too), and a last byte permute gathers the
+// four bytes.
 __device__ __forceinline__ uint32_t pack_bits15(uint32_t u0, uint32_t u1, uint32_t u2,
                                                 uint32_t u3) {
     const uint32_t p01 = __builtin_amdgcn_perm(u1, u0, 0x06050201u);
     const uint32_t p23 = __builtin_amdgcn_perm(u3, u2, 0x06050201u);
-    const v2u16 seven = {7, 7};
-    const v2u16 h01 = __builtin_bit_cast(v2u16, p01) >> seven;
-    const v2u16 h23 = __builtin_bit_cast(v2u16, p23) >> seven;
+    const v2u16 sh = {kFxShift - 8, kFxShift - 8};
+    const v2u16 h01 = __builtin_bit_cast(v2u16, p01) >> sh;
+    const v2u16 h23 = __builtin_bit_cast(v2u16, p23) >> sh;
     return __builtin_amdgcn_perm(__builtin_bit_cast(uint32_t, h23),
                                  __builtin_bit_cast(uint32_t, h01), 0x06040200u);
 }
@@ -582,7 +611,7 @@ __device__ __forceinline__ void store_rows(uint8_t *in_s, int W, int xs, bool in
 // Waves per SIMD the fixed-point kernel is compiled for: 3 (three workgroups
 // per CU) where its registers fit in 168 without spilling, else 2.
 constexpr int fx_waves(int nkb, int taps, bool aligned) {
-    return (taps == 2 && nkb <= 3) ? 3 : 2;
+    return (taps == 2 && nkb <= 3 && aligned) ? 3 : 2;
 }
 
 // TAPS: 0 = distinct horizontal and vertical taps; 1 = equal taps; 2 = equal
@@ -625,8 +654,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fx_waves(NK
     auto &bv_hi = *reinterpret_cast<v8h(*)[NKB]>(SHARE_B ? (void *)bh_hi : (void *)bv_hi_);
     auto &bv_lo = *reinterpret_cast<v8h(*)[NKB]>(SHARE_B ? (void *)bh_lo : (void *)bv_lo_);
     const float *vts = TAPS != 0 ? taps_h : taps_v;
-    const float init_h = 8388608.0f + (float)a.ch, init_v = 8388608.0f + (float)a.cv;
     const uint32_t thr_h = 2u * (uint32_t)a.ch, thr_v = 2u * (uint32_t)a.cv;
+    // hi accumulator start vectors, held in VGPRs for the whole kernel (the
+    // empty asm keeps the compiler from re-splatting them before every block)
+    v16f init_hv, init_vv_;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        float th = a.init_h, tv = a.init_v;
+        asm volatile("" : "+v"(th));
+        init_hv[r] = th;
+        if constexpr (TAPS == 0) {
+            asm volatile("" : "+v"(tv));
+            init_vv_[r] = tv;
+        }
+    }
+    const v16f &init_vv = TAPS == 0 ? init_vv_ : init_hv;  // equal taps: equal start values
 
     const int per_img = tiles_x * tiles_y;
     const int ntiles = n_img * per_img;
@@ -673,15 +715,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fx_waves(NK
             for (int kb = 0; kb < NKB; kb++) raw[kb] = *reinterpret_cast<const uint2 *>(arow + 16 * kb);
         };
         auto h_mma = [&](const uint2 (&raw)[NKB], v16f &acc, v16f &acl) {
-#pragma unroll
-            for (int r = 0; r < 16; r++) acc[r] = init_h;
-#pragma unroll
-            for (int r = 0; r < 16; r++) acl[r] = 0.0f;
+            const v16f zero = {};
 #pragma unroll
             for (int kb = 0; kb < NKB; kb++) {
-                v8h av = bytes_to_half8(raw[kb]);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bh_hi[kb], acc, 0, 0, 0);
-                acl = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bh_lo[kb], acl, 0, 0, 0);
+                v8h av = fx_a_operand(raw[kb]);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bh_hi[kb], kb == 0 ? init_hv : acc, 0, 0, 0);
+                acl = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bh_lo[kb], kb == 0 ? zero : acl, 0, 0, 0);
             }
         };
         auto h_drain = [&]() {
@@ -706,7 +745,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fx_waves(NK
                 for (int i = 0; i < 4; i++) {
                     // (a bit_cast of a vector element reads element 0 with this compiler)
                     u[i] = __float_as_uint(acc[4 * j + i] + acl[4 * j + i]);
-                    f[i] = (u[i] & 0x7fffu) < thr_h;
+                    f[i] = (u[i] & kFxFracMask) < thr_h;
                     bal[i] = __ballot(f[i]);
                     any |= bal[i];
                 }
@@ -743,15 +782,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fx_waves(NK
             for (int kb = 0; kb < NKB; kb++) raw[kb] = *reinterpret_cast<const uint2 *>(acol + 16 * kb);
         };
         auto v_mma = [&](const uint2 (&raw)[NKB], v16f &acc, v16f &acl) {
-#pragma unroll
-            for (int r = 0; r < 16; r++) acc[r] = init_v;
-#pragma unroll
-            for (int r = 0; r < 16; r++) acl[r] = 0.0f;
+            const v16f zero = {};
 #pragma unroll
             for (int kb = 0; kb < NKB; kb++) {
-                v8h av = bytes_to_half8(raw[kb]);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv_hi[kb], acc, 0, 0, 0);
-                acl = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv_lo[kb], acl, 0, 0, 0);
+                v8h av = fx_a_operand(raw[kb]);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv_hi[kb], kb == 0 ? init_vv : acc, 0, 0, 0);
+                acl = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv_lo[kb], kb == 0 ? zero : acl, 0, 0, 0);
             }
         };
         auto v_drain = [&]() {
@@ -777,7 +813,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fx_waves(NK
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
                     u[i] = __float_as_uint(acc[4 * j + i] + acl[4 * j + i]);
-                    f[i] = (u[i] & 0x7fffu) < thr_v;
+                    f[i] = (u[i] & kFxFracMask) < thr_v;
                     bal[i] = __ballot(f[i]);
                     any |= bal[i];
                 }
@@ -839,45 +875,64 @@ float blur_eps(const float *w, int L, int nkb) {
     return (float)(e * 1.25 + 1e-7);
 }
 
-// Readout margin c of the fixed-point kernel, in units of 2^-15, or -1 when
-// the taps do not qualify. It bounds |E - R| in units, E the kernel's
-// estimate and R the reference's f32 result:
-//   * the hi accumulator is exact; the lo accumulator's partial sums stay
-//     below B_lo = 255 * sum|lo|, so each of its at most 2L + 2*NKB roundings
-//     costs at most B_lo * 2^-23; the final f32 add rounds once (half a unit);
-//   * the split leaves |w * 2^15 - hi - lo| per tap, times a pixel <= 255
-//     (an f16 lo below 2^-14 counts as lost entirely, should it be flushed);
+// Readout margin c of the fixed-point kernel (units of 2^-kFxShift) and its
+// accumulator start values, or c = -1 when the taps do not qualify. c bounds
+// |E - R| in units, E the kernel's estimate and R the reference's f32 result:
+//   * the hi accumulator is exact; it starts at 2^23 + c - 1024 * sum(hi)
+//     + rint(-1024 * sum(lo)) (the A operand's offset, folded in as an
+//     integer: at most half a unit off); every partial sum of the lo
+//     accumulator (started at 0) stays below B_lo = max(A) * sum|lo|, so each
+//     of its at most 2L + 2*NKB roundings costs at most B_lo * 2^-23; the
+//     final f32 add rounds once (half a unit);
+//   * the split leaves |w * 2^kFxShift - hi - lo| per tap, times a pixel
+//     <= 255 (an f16 lo below 2^-14 counts as lost entirely, should it be
+//     flushed);
 //   * the reference's chain: product i is rounded once and then passes
 //     through L - max(i, 1) rounded additions (the first addition, 0 + x0,
 //     is exact), each off by at most 2^-24 relative, all terms >= 0 and
 //     pixels <= 255 (1 % slack for second-order terms).
 // c exceeds the bound by 25 % plus one unit.
-int blur_fx_margin(const float *w, int L, int nkb) {
-    double sum = 0.0, sum_hi = 0.0, lo_abs = 0.0, split = 0.0, chain = 0.0;
+struct FxPlan {
+    int c;
+    float init;
+};
+
+FxPlan blur_fx_plan(const float *w, int L, int nkb) {
+    const double unit = std::ldexp(1.0, kFxShift);
+    const double amax = kFxBiasedA ? 1279.0 : 255.0;  // largest A operand
+    double sum = 0.0, sum_hi = 0.0, sum_lo = 0.0, lo_abs = 0.0, split = 0.0, chain = 0.0;
     for (int i = 0; i < L; i++) {
-        if (!(w[i] >= 0.0f) || !std::isfinite(w[i])) return -1;
-        const float ws = w[i] * 32768.0f;
+        if (!(w[i] >= 0.0f) || !std::isfinite(w[i])) return {-1, 0.0f};
+        const float ws = w[i] * (float)unit;
         const _Float16 hi = (_Float16)rintf(ws);
         const _Float16 lo = (_Float16)(ws - (float)hi);
         sum += (double)w[i];
         sum_hi += (double)(float)hi;
+        sum_lo += (double)(float)lo;
         lo_abs += std::fabs((double)(float)lo);
         double res = std::fabs((double)ws - (double)(float)hi - (double)(float)lo);
         if (std::fabs((double)(float)lo) < std::ldexp(1.0, -14)) res += std::fabs((double)(float)lo);
         split += res;
         chain += (double)w[i] * (1.0 + (double)(L - std::max(i, 1)));
     }
-    const double b_lo = 255.0 * lo_abs;
-    double e = 0.5 + (2.0 * L + 2.0 * nkb) * b_lo * std::ldexp(1.0, -23) + 255.0 * split +
-               32768.0 * 255.0 * chain * std::ldexp(1.0, -24) * 1.01;
-    double c = std::ceil(1.25 * e + 1.0);
-    // every partial sum of the hi accumulator, and the final value, stay
-    // below 2^24: the largest estimate plus the margin fits under 256
-    if (255.0 * std::max(sum_hi + lo_abs, sum * 32768.0 * (1.0 + 1e-6)) + 2.0 * c >=
-        255.99 * 32768.0)
-        return -1;
-    if (c >= 4096.0) return -1;
-    return (int)c;
+    // offset of the biased A operand: 1024 * (sum(hi) + sum(lo)), the lo part
+    // rounded to an integer (error at most half a unit)
+    const double off_hi = kFxBiasedA ? 1024.0 * sum_hi : 0.0;
+    const double off_lo = kFxBiasedA ? std::rint(1024.0 * sum_lo) : 0.0;
+    const double b_lo = amax * lo_abs;
+    double e = 0.5 + (2.0 * L + 2.0 * nkb) * b_lo * std::ldexp(1.0, -23) +
+               (kFxBiasedA ? std::fabs(off_lo - 1024.0 * sum_lo) : 0.0) + 255.0 * split +
+               unit * 255.0 * chain * std::ldexp(1.0, -24) * 1.01;
+    const double c = std::ceil(1.25 * e + 1.0);
+    const double init_hi = 8388608.0 + c - off_hi - off_lo;
+    // the hi accumulator is exact: every partial sum is an integer of
+    // magnitude below 2^24; the final value stays in [2^23, 2^24) with the
+    // estimate's integer part in 8 bits
+    if (std::fabs(init_hi) + amax * sum_hi >= 16777216.0) return {-1, 0.0f};
+    if (255.0 * std::max(sum_hi + lo_abs, sum * unit * (1.0 + 1e-6)) + 2.0 * c >= 255.99 * unit)
+        return {-1, 0.0f};
+    if (c >= unit / 8.0) return {-1, 0.0f};
+    return {(int)c, (float)init_hi};
 }
 
 }  // namespace
@@ -960,8 +1015,8 @@ iph_status launch_blur_u8_mfma(const uint8_t *src, uint8_t *dst, uint32_t w, uin
 #define LK(K)                                                                                \
     if (same) { LA(K, true); } else { LA(K, false); }                                        \
     break
-    const int ch = blur_fx_margin(hk, L, nkb), cv = blur_fx_margin(vk, L, nkb);
-    if (ch > 0 && cv > 0) {
+    const FxPlan ph = blur_fx_plan(hk, L, nkb), pv = blur_fx_plan(vk, L, nkb);
+    if (ph.c > 0 && pv.c > 0) {
         FxArgs f{};
         for (int i = 0; i < L; i++) {
             f.h[i] = hk[i];
@@ -969,8 +1024,10 @@ iph_status launch_blur_u8_mfma(const uint8_t *src, uint8_t *dst, uint32_t w, uin
         }
         f.L = L;
         f.m = m;
-        f.ch = ch;
-        f.cv = cv;
+        f.ch = ph.c;
+        f.cv = pv.c;
+        f.init_h = ph.init;
+        f.init_v = pv.init;
         const int ftx = (int)((w + kTW - 1) / kTW), fty = (int)((h + kTH2 - 1) / kTH2);
         const long long fnt = (long long)ftx * fty * n;
 #define FB(K, SAME, AL)                                                                      \
