@@ -14,6 +14,10 @@ batch's Gpix/s. The same JSON line carries:
   * match_template: ms per 2048x2048 u8 image with a 32x32 template, for each
     of the four methods on the MFMA path and on the direct FP32 path, with
     argmax/argmin agreement between them;
+  * precision_f32: CrossCorrelationNormalized on a 4096x4096 Luma<f32> field
+    with 64x64 and 128x128 templates, BF16x3 MFMA vs FP32 direct, each with
+    the max abs / rel error over every output against a float64 reference
+    (FFT correlation, integral-image energies) and argmax agreement;
   * cpu_baseline: the oracle's blur (the reference's loop restated in C,
     single-threaded like the reference's gaussian_blur_f32) on host cores.
 Inputs are synthetic and seeded (SplitMix64).
@@ -36,6 +40,8 @@ HBM_PEAK_GBS = 8000.0
 INT8_PEAK_TOPS = 5000.0
 BLUR_W, BLUR_H, BLUR_N, BLUR_SIGMA = 1024, 768, 512, 2.0
 MATCH_SIDE, MATCH_T = 2048, 32
+PREC_SIDE, PREC_SIGMA, PREC_TEMPLATES = 4096, 3.0, (64, 128)
+BF16_PEAK_TFLOPS = 2500.0
 
 
 # ---- seeded synthetic inputs -----------------------------------------------------
@@ -91,6 +97,43 @@ def match_workload(seed: int = 2):
     crop = img[y:y + MATCH_T, x:x + MATCH_T].astype(np.float64)
     tpl = np.clip(np.rint(crop + 8.0 * normal(seed + 200, crop.shape)), 0, 255).astype(np.uint8)
     return img, tpl, (x, y)
+
+
+def precision_workload(seed: int = 3):
+    """4096x4096 Luma<f32>: N(0,1) field smoothed with this library's
+    gaussian_blur_f32 (sigma 3) on the GPU; 64x64 and 128x128 templates cropped
+    at seeded locations and perturbed by N(0, 0.05). Returns device tensors."""
+    import torch
+
+    from imageproc_amd import device as dev
+
+    field = torch.from_numpy(normal(seed, (PREC_SIDE, PREC_SIDE)).astype(np.float32)).cuda()
+    img = dev.gaussian_blur_f32(field, PREC_SIGMA)
+    templates = {}
+    for k, t in enumerate(PREC_TEMPLATES):
+        loc = splitmix64(seed + 100 + k, 2) % np.uint64(PREC_SIDE - t + 1)
+        x, y = int(loc[0]), int(loc[1])
+        noise = torch.from_numpy((0.05 * normal(seed + 200 + k, (t, t))).astype(np.float32)).cuda()
+        templates[t] = ((img[y:y + t, x:x + t] + noise).contiguous(), (x, y))
+    return img, templates
+
+
+def ccorr_normed_f64(img, tpl):
+    """Reference CrossCorrelationNormalized in float64 on the GPU: the
+    correlation by FFT, the window energies from a float64 integral image."""
+    import torch
+
+    I = img.double()
+    T = tpl.double()
+    H, W = I.shape
+    th, tw = T.shape
+    s = (H + th, W + tw)
+    num = torch.fft.irfft2(torch.fft.rfft2(I, s) * torch.conj(torch.fft.rfft2(T, s)), s)
+    num = num[:H - th + 1, :W - tw + 1]
+    sq = torch.zeros((H + 1, W + 1), dtype=torch.float64, device=I.device)
+    sq[1:, 1:] = (I * I).cumsum(0).cumsum(1)
+    ii = sq[th:, tw:] - sq[:-th, tw:] - sq[th:, :-tw] + sq[:-th, :-tw]
+    return num / torch.sqrt(ii * (T * T).sum())
 
 
 # ---- measurement helpers -------------------------------------------------------------
@@ -222,6 +265,50 @@ def bench_match(steps: int):
     return res
 
 
+def bench_precision(steps: int):
+    """CrossCorrelationNormalized on the 4096x4096 f32 field: BF16x3 MFMA vs
+    FP32 direct, each with max abs / rel error over every output against the
+    float64 reference, and argmax agreement."""
+    import torch
+
+    from imageproc_amd import device as dev
+    from imageproc_amd.template_matching import MatchTemplateMethod as M
+
+    print("[bench] precision: generating 4096x4096 f32 field", flush=True)
+    img, templates = precision_workload()
+    res = {"workload": f"{PREC_SIDE}x{PREC_SIDE} Luma<f32> N(0,1) smoothed sigma {PREC_SIGMA}, "
+                       "CrossCorrelationNormalized"}
+    for t, (tpl, loc) in templates.items():
+        ref = ccorr_normed_f64(img, tpl)
+        ref_best = int(torch.argmax(ref))
+        ow = PREC_SIDE - t + 1
+        entry = {"planted_location": list(loc),
+                 "reference_best": [ref_best % ow, ref_best // ow]}
+        for path in ("mfma", "direct"):
+            out = dev.match_template(img, tpl, M.CrossCorrelationNormalized, path)
+            times, _ = timed(lambda: dev.match_template(img, tpl, M.CrossCorrelationNormalized, path,
+                                                        out=out), steps, 1)
+            ms = float(np.median(times))
+            d = (out.double() - ref).abs()
+            best = int(torch.argmax(out))
+            big = ref.abs() >= 1e-2
+            entry[path] = {"ms_per_image": round(ms, 3),
+                           "max_abs_err": float(d.max()),
+                           "max_rel_err_where_abs_score_ge_0.01": float((d / ref.abs())[big].max()),
+                           "normwise_rel_err": float(d.max() / ref.abs().max()),
+                           "best_location": [best % ow, best // ow],
+                           "argmax_agrees_with_f64": best == ref_best}
+            if path == "mfma":
+                flops = 2.0 * ow * ow * t * t
+                entry[path]["roofline"] = {
+                    "bound": "mfma", "achieved": round(flops / (ms * 1e-3) / 1e12, 2),
+                    "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(flops / (ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4)}
+        res[f"template_{t}"] = entry
+        del ref
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -255,6 +342,7 @@ def main():
     gpix = BLUR_N * BLUR_W * BLUR_H / (ms * 1e-3) / 1e9
     achieved = nbytes / (ms * 1e-3) / 1e9
     match = bench_match(max(5, args.steps // 2))
+    precision = bench_precision(3)
     cpu = None if args.no_cpu else cpu_blur_baseline()
     line = {
         "metric": "blur Gpix/s & %HBM peak; match_template ms/image at equal argmax vs plain GPU",
@@ -277,6 +365,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic()},
         "match_template": match,
+        "precision_f32": precision,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

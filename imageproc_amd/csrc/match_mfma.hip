@@ -324,38 +324,61 @@ __global__ __launch_bounds__(64) void toeplitz_bf16_kernel(const float *__restri
 // Window energy sum I^2 in f64 for every output: vertical sums over th rows,
 // then horizontal sums over tw, each evaluated directly (no running
 // differences, so no cancellation).
-__global__ __launch_bounds__(256) void window_sq_f32_kernel(const float *__restrict__ img, int W,
-                                                            int tw, int th, int ow, int oh,
-                                                            double *__restrict__ ws) {
+// Window energies sum(i^2) of an f32 image in f64, as two sliding-window
+// passes. Horizontal: one workgroup per (row, 1024 outputs); p^2 of the
+// needed columns is staged in LDS, each thread sums its first window and
+// slides over 3 more (4 consecutive outputs).
+constexpr int kHsOut = 1024;  // outputs per horizontal workgroup
+__global__ __launch_bounds__(256) void window_sq_rows_f32_kernel(const float *__restrict__ img,
+                                                                 int W, int tw, int ow,
+                                                                 double *__restrict__ hs) {
     extern __shared__ __align__(16) unsigned char smem[];
-    double *v = reinterpret_cast<double *>(smem);  // kWinRows x cols
-    const int x0 = blockIdx.x * kWinCols, y0 = blockIdx.y * kWinRows;
-    const int cols = kWinCols + tw - 1;
-    const int H = oh + th - 1;
-    for (int i = threadIdx.x; i < kWinRows * cols; i += 256) {
-        int y = i / cols, c = i - y * cols;
-        int gx = x0 + c;
-        double a = 0.0;
-        if (gx < W) {
-            for (int r = 0; r < th; r++) {
-                int gy = y0 + y + r;
-                if (gy < H) {
-                    double p = img[(size_t)gy * W + gx];
-                    a += p * p;
-                }
-            }
-        }
-        v[i] = a;
+    double *v = reinterpret_cast<double *>(smem);  // kHsOut + tw - 1
+    const int y = blockIdx.y, x0 = blockIdx.x * kHsOut;
+    const int ncols = min(kHsOut + tw - 1, W - x0);
+    const float *row = img + (size_t)y * W + x0;
+    for (int c = threadIdx.x; c < ncols; c += 256) {
+        const double p = row[c];
+        v[c] = p * p;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < kWinRows * kWinCols; i += 256) {
-        int y = i / kWinCols, c = i - y * kWinCols;
-        int gy = y0 + y, gx = x0 + c;
-        if (gy >= oh || gx >= ow) continue;
-        const double *vr = v + y * cols + c;
-        double a = 0.0;
-        for (int dx = 0; dx < tw; dx++) a += vr[dx];
-        ws[(size_t)gy * ow + gx] = a;
+    const int xl = threadIdx.x * 4;
+    if (x0 + xl >= ow) return;
+    double a = 0.0;
+    for (int d = 0; d < tw; d++) a += v[xl + d];
+    double *o = hs + (size_t)y * ow + x0 + xl;
+    o[0] = a;
+#pragma unroll
+    for (int q = 1; q < 4; q++) {
+        if (x0 + xl + q >= ow) break;
+        a += v[xl + q - 1 + tw] - v[xl + q - 1];
+        o[q] = a;
+    }
+}
+
+// Vertical: one workgroup per 32 columns x 128 output rows; each thread sums
+// its first window of th horizontal sums and slides over 15 more rows.
+constexpr int kVsCols = 32, kVsRows = 128;
+__global__ __launch_bounds__(256) void window_sq_cols_f32_kernel(const double *__restrict__ hs,
+                                                                 int th, int ow, int oh,
+                                                                 double *__restrict__ ws) {
+    const int x = blockIdx.x * kVsCols + (threadIdx.x & (kVsCols - 1));
+    const int y0 = blockIdx.y * kVsRows + (threadIdx.x / kVsCols) * (kVsRows / 8);
+    if (x >= ow || y0 >= oh) return;
+    const double *col = hs + x;
+    double a = 0.0;
+    for (int r0 = 0; r0 < th; r0 += 8) {
+        double t[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) t[q] = (r0 + q < th) ? col[(size_t)(y0 + r0 + q) * ow] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) a += t[q];
+    }
+    ws[(size_t)y0 * ow + x] = a;
+    const int yend = min(y0 + kVsRows / 8, oh);
+    for (int y = y0 + 1; y < yend; y++) {
+        a += col[(size_t)(y - 1 + th) * ow] - col[(size_t)(y - 1) * ow];
+        ws[(size_t)y * ow + x] = a;
     }
 }
 
@@ -558,7 +581,8 @@ iph_status launch_bf16x3_match(const float *img, uint32_t w, uint32_t h, const f
     double *tsq = wsp.take<double>(1);
     uint16_t *bsw = wsp.take<uint16_t>((size_t)th * KB * 2 * 64 * 8);
     double *wsq = wsp.take<double>((size_t)(w - tw + 1) * (h - th + 1));
-    if (!tsq || !bsw || !wsq) {
+    double *hsq = method == IPH_CCORR ? nullptr : wsp.take<double>((size_t)(w - tw + 1) * h);
+    if (!tsq || !bsw || !wsq || (method != IPH_CCORR && !hsq)) {
         set_error("workspace too small for the BF16x3 MFMA match_template");
         return IPH_EINVAL;
     }
@@ -568,10 +592,13 @@ iph_status launch_bf16x3_match(const float *img, uint32_t w, uint32_t h, const f
     toeplitz_bf16_kernel<<<dim3(KB, th), 64, 0, s>>>(tpl, (int)tw, KB, bsw);
     IPH_TRY(launch_status("toeplitz_bf16_kernel"));
     if (method != IPH_CCORR) {
-        dim3 wg((ow + kWinCols - 1) / kWinCols, (oh + kWinRows - 1) / kWinRows);
-        size_t lds = (size_t)kWinRows * (kWinCols + tw - 1) * sizeof(double);
-        window_sq_f32_kernel<<<wg, 256, lds, s>>>(img, (int)w, (int)tw, (int)th, ow, oh, wsq);
-        IPH_TRY(launch_status("window_sq_f32_kernel"));
+        dim3 hg((ow + kHsOut - 1) / kHsOut, h);
+        size_t lds = (size_t)(kHsOut + tw - 1) * sizeof(double);
+        window_sq_rows_f32_kernel<<<hg, 256, lds, s>>>(img, (int)w, (int)tw, ow, hsq);
+        IPH_TRY(launch_status("window_sq_rows_f32_kernel"));
+        dim3 vg((ow + kVsCols - 1) / kVsCols, (oh + kVsRows - 1) / kVsRows);
+        window_sq_cols_f32_kernel<<<vg, 256, 0, s>>>(hsq, (int)th, ow, oh, wsq);
+        IPH_TRY(launch_status("window_sq_cols_f32_kernel"));
     }
     switch (KB) {
 #define C(K)                                                                                 \
@@ -586,13 +613,14 @@ iph_status launch_bf16x3_match(const float *img, uint32_t w, uint32_t h, const f
 }
 
 size_t mfma_match_workspace(int32_t p_type, uint32_t w, uint32_t h, uint32_t tw, uint32_t th,
-                            int32_t) {
+                            int32_t method) {
     if (tw == 0 || th == 0 || w < tw || h < th) return 256;
     if (p_type == IPH_F32) {
         WorkspacePlan wp;
         wp.take<double>(1);
         wp.take<uint16_t>((size_t)th * blocks16_for((int)tw) * 2 * 64 * 8);
         wp.take<double>((size_t)(w - tw + 1) * (h - th + 1));
+        if (method != IPH_CCORR) wp.take<double>((size_t)(w - tw + 1) * h);
         return wp.used;
     }
     U8Plan p = plan_u8(w, h, tw, th);
