@@ -464,13 +464,33 @@ constexpr bool kFxBiasedA = IPH_FX_BIASED_A;
 constexpr int kFxShift = kFxBiasedA ? 13 : 15;  // estimates in units of 2^-kFxShift
 constexpr uint32_t kFxFracMask = (1u << kFxShift) - 1u;
 
-struct FxArgs {
-    float h[kMaxTaps];
-    float v[kMaxTaps];
-    int L;
-    int m;
+// One blur job of the fixed-point kernel: n_img images of W x H (rows W bytes
+// apart, images `stride` bytes apart) and their taps. A launch walks the
+// global tile list [0, ntiles); a job owns tiles [tile0, tile0 + n_img *
+// tiles_x * tiles_y).
+constexpr int kFxMaxL = 33;
+struct FxJob {
+    const uint8_t *src;
+    uint8_t *dst;
+    long long stride;
+    int W, H, n_img, tiles_x, tiles_y, tile0;
+    int L, m;                    // taps, window shift (the staged columns start 4-aligned)
     int ch, cv;                  // readout margins c (units of 2^-kFxShift) of the two passes
     float init_h, init_v;        // hi accumulator start values (integers) of the two passes
+    float h[kFxMaxL], v[kFxMaxL];
+};
+
+// Job accessors: one job passed by value, or a device table.
+struct FxOneJob {
+    const FxJob &j;  // the kernel argument itself (no private copy)
+    __device__ __forceinline__ int count() const { return 1; }
+    __device__ __forceinline__ const FxJob &operator[](int) const { return j; }
+};
+struct FxJobTable {
+    const FxJob *p;
+    int n;
+    __device__ __forceinline__ int count() const { return n; }
+    __device__ __forceinline__ const FxJob &operator[](int i) const { return p[i]; }
 };
 
 // 8 u8 pixels -> 8 f16 values 1024 + p (exact).
@@ -616,10 +636,8 @@ constexpr int fx_waves(int nkb, int taps, bool aligned) {
 
 // TAPS: 0 = distinct horizontal and vertical taps; 1 = equal taps; 2 = equal
 // taps and no window shift (m = 0), so both passes use the same B operands.
-template <int NKB, int TAPS, bool ALIGNED>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fx_waves(NKB, TAPS, ALIGNED), fx_waves(NKB, TAPS, ALIGNED)))) void blur_u8_fx_kernel(
-    const uint8_t *__restrict__ src, uint8_t *__restrict__ dst, int W, int H, size_t stride,
-    int n_img, int tiles_x, int tiles_y, FxArgs a) {
+template <int NKB, int TAPS, bool ALIGNED, typename Jobs>
+__device__ __forceinline__ void blur_u8_fx_body(const Jobs &jobs, int ntiles) {
     using S = RowStage<NKB, kRows2>;
     constexpr int PIN = S::PIN, LMAX = S::LMAX;
     constexpr int IN_BYTES = (kRows2 * PIN + 15) / 16 * 16;
@@ -632,46 +650,58 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fx_waves(NK
     float *taps_v = taps_h + (2 * kTapPad + kMaxTaps);
     uint16_t *queue = reinterpret_cast<uint16_t *>(taps_v + (2 * kTapPad + kMaxTaps)) +
                       (threadIdx.x >> 6) * kFxQCap;
-    const int L = a.L;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int n = lane & 31, g = lane >> 5;
-    const int half = L / 2;
-
-    if (threadIdx.x < 2 * kTapPad + kMaxTaps) {
-        int t = (int)threadIdx.x - kTapPad;
-        bool in = t >= 0 && t < L;
-        taps_h[threadIdx.x] = in ? a.h[t] : 0.0f;
-        taps_v[threadIdx.x] = in ? a.v[t] : 0.0f;
-    }
-    __syncthreads();
     constexpr bool SHARE_B = TAPS == 2;
+
+    // per-job state, rebuilt when a workgroup's tile belongs to another job
+    int L = 0, half = 0, m = 0;
+    uint32_t thr_h = 0, thr_v = 0;
     v8h bh_hi[NKB], bh_lo[NKB], bv_hi_[SHARE_B ? 1 : NKB], bv_lo_[SHARE_B ? 1 : NKB];
-    const int m = SHARE_B ? 0 : a.m;
-    build_b_fx<NKB>(taps_h, lane, m, bh_hi, bh_lo);
-    if constexpr (!SHARE_B) {
-        build_b_fx<NKB>(taps_v, lane, 0, bv_hi_, bv_lo_);
-    }
+    v16f init_hv, init_vv_;
     auto &bv_hi = *reinterpret_cast<v8h(*)[NKB]>(SHARE_B ? (void *)bh_hi : (void *)bv_hi_);
     auto &bv_lo = *reinterpret_cast<v8h(*)[NKB]>(SHARE_B ? (void *)bh_lo : (void *)bv_lo_);
     const float *vts = TAPS != 0 ? taps_h : taps_v;
-    const uint32_t thr_h = 2u * (uint32_t)a.ch, thr_v = 2u * (uint32_t)a.cv;
-    // hi accumulator start vectors, held in VGPRs for the whole kernel (the
-    // empty asm keeps the compiler from re-splatting them before every block)
-    v16f init_hv, init_vv_;
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-        float th = a.init_h, tv = a.init_v;
-        asm volatile("" : "+v"(th));
-        init_hv[r] = th;
-        if constexpr (TAPS == 0) {
-            asm volatile("" : "+v"(tv));
-            init_vv_[r] = tv;
-        }
-    }
     const v16f &init_vv = TAPS == 0 ? init_vv_ : init_hv;  // equal taps: equal start values
+    // every thread: the job's zero-padded taps into LDS (a barrier follows)
+    auto stage_taps = [&](const FxJob &jb) {
+        if (threadIdx.x < 2 * kTapPad + kMaxTaps) {
+            int t = (int)threadIdx.x - kTapPad;
+            bool in = t >= 0 && t < jb.L;
+            taps_h[threadIdx.x] = in ? jb.h[t] : 0.0f;
+            taps_v[threadIdx.x] = in ? jb.v[t] : 0.0f;
+        }
+    };
+    // after the barrier: this lane's B operands and start vectors
+    auto setup_job = [&](const FxJob &jb) {
+        L = jb.L;
+        half = L / 2;
+        m = SHARE_B ? 0 : jb.m;
+        thr_h = 2u * (uint32_t)jb.ch;
+        thr_v = 2u * (uint32_t)jb.cv;
+        build_b_fx<NKB>(taps_h, lane, m, bh_hi, bh_lo);
+        if constexpr (!SHARE_B) {
+            build_b_fx<NKB>(taps_v, lane, 0, bv_hi_, bv_lo_);
+        }
+        // hi accumulator start vectors, held in VGPRs (the empty asm keeps
+        // the compiler from re-splatting them before every block)
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            float th = jb.init_h, tv = jb.init_v;
+            asm volatile("" : "+v"(th));
+            init_hv[r] = th;
+            if constexpr (TAPS == 0) {
+                asm volatile("" : "+v"(tv));
+                init_vv_[r] = tv;
+            }
+        }
+    };
+    // the job owning tile t, scanning forward from job j
+    auto job_of = [&](int t, int j) {
+        while (j + 1 < jobs.count() && t >= jobs[j + 1].tile0) j++;
+        return j;
+    };
 
-    const int per_img = tiles_x * tiles_y;
-    const int ntiles = n_img * per_img;
     const int per_xcd = (int)gridDim.x >> 3;
     const int chunk = (ntiles + 7) >> 3;
     const int t_begin = (int)(blockIdx.x & 7) * chunk;
@@ -681,29 +711,46 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fx_waves(NK
     // a window lies inside its rows when xs >= 0 and xs + CIN <= W
     uint4 regs[S::NLD];
     bool interior = false;
+    int jc = 0, jl = -1;  // job of the current tile; job whose state is loaded
+    auto issue_loads = [&](int tt, int j) {
+        const FxJob &jb = jobs[j];
+        const int per_img = jb.tiles_x * jb.tiles_y;
+        const int lt = tt - jb.tile0;
+        const int im = lt / per_img, rem = lt - im * per_img, ty = rem / jb.tiles_x;
+        const int hf = jb.L / 2, mm = SHARE_B ? 0 : jb.m;
+        const int xs = (rem - ty * jb.tiles_x) * kTW - hf - mm;
+        interior = xs >= 0 && xs + S::CIN <= jb.W;
+        load_rows<S, ALIGNED>(jb.src + (size_t)im * jb.stride, jb.W, jb.H, xs, ty * kTH2 - hf,
+                              interior, regs);
+    };
     if (t < t_end) {
-        int im = t / per_img, rem = t - im * per_img, ty = rem / tiles_x;
-        const int xs = (rem - ty * tiles_x) * kTW - half - m;
-        interior = xs >= 0 && xs + S::CIN <= W;
-        load_rows<S, ALIGNED>(src + (size_t)im * stride, W, H, xs, ty * kTH2 - half, interior,
-                              regs);
+        jc = job_of(t, 0);
+        issue_loads(t, jc);
     }
     for (; t < t_end; t += per_xcd) {
-        const int im = t / per_img, rem = t - im * per_img, ty = rem / tiles_x;
-        const int x0 = (rem - ty * tiles_x) * kTW, y0 = ty * kTH2;
-        uint8_t *out = dst + (size_t)im * stride;
+        const FxJob &jb = jobs[jc];
+        const int per_img = jb.tiles_x * jb.tiles_y;
+        const int lt = t - jb.tile0;
+        const int im = lt / per_img, rem = lt - im * per_img, ty = rem / jb.tiles_x;
+        const int x0 = (rem - ty * jb.tiles_x) * kTW, y0 = ty * kTH2;
+        const int W = jb.W, H = jb.H;
+        uint8_t *out = jb.dst + (size_t)im * jb.stride;
+        const bool new_job = jc != jl;
 
-        __syncthreads();  // the previous tile is done with in_s / out_s and ht
-        store_rows<S, ALIGNED>(in_s, W, x0 - half - m, interior, regs);
+        __syncthreads();  // the previous tile is done with in_s / out_s, ht and the taps
+        if (new_job) stage_taps(jb);
+        store_rows<S, ALIGNED>(in_s, W, x0 - jb.L / 2 - (SHARE_B ? 0 : jb.m), interior, regs);
         const int tn = t + per_xcd;
+        int jn = jc;
         if (tn < t_end) {
-            int im2 = tn / per_img, rem2 = tn - im2 * per_img, ty2 = rem2 / tiles_x;
-            const int xs2 = (rem2 - ty2 * tiles_x) * kTW - half - m;
-            interior = xs2 >= 0 && xs2 + S::CIN <= W;
-            load_rows<S, ALIGNED>(src + (size_t)im2 * stride, W, H, xs2, ty2 * kTH2 - half,
-                                  interior, regs);
+            jn = job_of(tn, jc);
+            issue_loads(tn, jn);
         }
         __syncthreads();
+        if (new_job) {
+            setup_job(jb);
+            jl = jc;
+        }
 
         // ---- horizontal pass: (kRows2 / 32) x 4 blocks of 32x32 -> ht (column-major u8);
         // the A operand of a wave's next block is read while the current one is read out.
@@ -842,8 +889,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fx_waves(NK
         __syncthreads();
 
         // ---- copy the output tile out: 8 lanes x 16 bytes per row
-        const bool vec_ok = ((W & 15) == 0) && ((reinterpret_cast<uintptr_t>(out) & 15) == 0) &&
-                            ((stride & 15) == 0);
+        const bool vec_ok = ((W & 15) == 0) && ((reinterpret_cast<uintptr_t>(out) & 15) == 0);
 #pragma unroll 2
         for (int i = threadIdx.x; i < kTH2 * (kTW / 16); i += 256) {
             const int yl = i >> 3, xl = (i & 7) * 16;
@@ -858,7 +904,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fx_waves(NK
                 for (int q = 0; q < 16 && gx + q < W; q++) o[q] = b[q];
             }
         }
+        jc = jn;
     }
+}
+
+template <int NKB, int TAPS, bool ALIGNED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fx_waves(NKB, TAPS, ALIGNED), fx_waves(NKB, TAPS, ALIGNED)))) void blur_u8_fx_kernel(
+    FxJob job) {
+    FxOneJob one{job};
+    blur_u8_fx_body<NKB, TAPS, ALIGNED>(one, job.n_img * job.tiles_x * job.tiles_y);
+}
+
+// The same kernel over a device table of jobs: images of different sizes and
+// taps that share NKB, the tap class and row alignment.
+template <int NKB, int TAPS, bool ALIGNED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fx_waves(NKB, TAPS, ALIGNED), fx_waves(NKB, TAPS, ALIGNED)))) void blur_u8_fx_jobs_kernel(
+    const FxJob *__restrict__ jobs, int njobs, int ntiles) {
+    FxJobTable tab{jobs, njobs};
+    blur_u8_fx_body<NKB, TAPS, ALIGNED>(tab, ntiles);
 }
 
 // |E - R| bound for one pass, in output units. E: MFMA estimate, at most
@@ -1017,7 +1080,16 @@ iph_status launch_blur_u8_mfma(const uint8_t *src, uint8_t *dst, uint32_t w, uin
     break
     const FxPlan ph = blur_fx_plan(hk, L, nkb), pv = blur_fx_plan(vk, L, nkb);
     if (ph.c > 0 && pv.c > 0) {
-        FxArgs f{};
+        FxJob f{};
+        f.src = src;
+        f.dst = dst;
+        f.stride = (long long)stride;
+        f.W = (int)w;
+        f.H = (int)h;
+        f.n_img = (int)n;
+        f.tiles_x = (int)((w + kTW - 1) / kTW);
+        f.tiles_y = (int)((h + kTH2 - 1) / kTH2);
+        f.tile0 = 0;
         for (int i = 0; i < L; i++) {
             f.h[i] = hk[i];
             f.v[i] = vk[i];
@@ -1028,8 +1100,7 @@ iph_status launch_blur_u8_mfma(const uint8_t *src, uint8_t *dst, uint32_t w, uin
         f.cv = pv.c;
         f.init_h = ph.init;
         f.init_v = pv.init;
-        const int ftx = (int)((w + kTW - 1) / kTW), fty = (int)((h + kTH2 - 1) / kTH2);
-        const long long fnt = (long long)ftx * fty * n;
+        const long long fnt = (long long)f.tiles_x * f.tiles_y * n;
 #define FB(K, SAME, AL)                                                                      \
     do {                                                                                     \
         using Sh = RowStage<K, kRows2>;                                                     \
@@ -1044,8 +1115,7 @@ iph_status launch_blur_u8_mfma(const uint8_t *src, uint8_t *dst, uint32_t w, uin
         long long grid = (long long)cus * per_cu;                                            \
         grid = std::min<long long>(grid, (fnt + 7) / 8 * 8);                                 \
         grid = std::max<long long>(8, grid / 8 * 8);                                         \
-        blur_u8_fx_kernel<K, SAME, AL><<<(unsigned)grid, 256, lds, s>>>(                    \
-            src, dst, (int)w, (int)h, stride, (int)n, ftx, fty, f);                          \
+        blur_u8_fx_kernel<K, SAME, AL><<<(unsigned)grid, 256, lds, s>>>(f);                 \
     } while (0)
 #define FA(K, SAME)                                                                          \
     if (aligned) FB(K, SAME, true); else FB(K, SAME, false)
