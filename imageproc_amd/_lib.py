@@ -99,6 +99,19 @@ _SIGS = {
                                            C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32,
                                            C.c_int32, C.c_void_p, C.c_void_p, C.c_size_t,
                                            C.c_void_p]),
+    "iph_batch_create": (C.c_int32, [C.POINTER(C.c_void_p)]),
+    "iph_batch_destroy": (None, [C.c_void_p]),
+    "iph_batch_add_gaussian_blur": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                                C.c_uint32, C.c_float]),
+    "iph_batch_add_filter_clamped": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                                 C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32]),
+    "iph_batch_add_match_template": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
+                                                 C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32,
+                                                 C.c_void_p]),
+    "iph_batch_plan": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p]),
+    "iph_batch_run": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "iph_batch_run_eager": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "iph_batch_tuning": (C.c_int32, [C.c_void_p, C.c_char_p, C.c_size_t]),
     "iph_dev_find_extremes_workspace": (C.c_size_t, []),
     "iph_dev_find_extremes": (C.c_int32, [C.c_void_p, C.c_int32, C.c_uint32, C.c_uint32,
                                           C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),

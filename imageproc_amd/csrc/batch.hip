@@ -1,0 +1,443 @@
+// Batches of uneven images: many gaussian_blur_f32, filter_clamped (u8 image,
+// f32 kernel, u8 output) and match_template (GrayImage) jobs planned together
+// and replayed as one hipGraph. The reference has no batch API (a batch there
+// is a loop of per-image calls); each job computes exactly what the
+// corresponding single call computes.
+//
+// Plan: the blur jobs become device job tables of the persistent fixed-point
+// blur kernel (one launch per kernel variant, each walking the tiles of all
+// its images), the filter jobs one table of the ragged 2-D filter kernel (one
+// launch), the match jobs one int8-MFMA pipeline each with a workspace slice.
+// The three kinds run on three streams forked from the caller's stream and
+// joined back, captured once into a graph. The autotuner times the candidate
+// launch shapes of each kind on the real jobs (filter tile shape, rows of
+// 32 outputs per match workgroup, persistent blur workgroups per CU) and
+// keeps the fastest before capture.
+#include "iph_internal.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+namespace {
+
+struct MatchSpec {
+    const uint8_t *img;
+    uint32_t w, h;
+    const uint8_t *tpl;
+    uint32_t tw, th;
+    int32_t method;
+    float *out;
+    size_t ws_off, ws_bytes;
+};
+
+struct FallbackBlur {
+    int spec;
+    size_t ws_off, ws_bytes;
+};
+
+}  // namespace
+
+struct iph_batch {
+    std::vector<iph::BlurSpec> blur;
+    std::vector<iph::FilterSpec> filt;
+    std::vector<MatchSpec> match;
+
+    bool planned = false;
+    int device = -1;
+    char *mem = nullptr;
+    size_t mem_bytes = 0;
+    size_t blur_table_off = 0, filt_table_off = 0;
+    std::vector<iph::BlurGroup> blur_groups;
+    std::vector<FallbackBlur> blur_fallback;
+    int filt_tiles = 0;
+    uint32_t filt_max_kw = 1, filt_max_kh = 1;
+    int blur_grid_pct = 100, filt_variant = 0, match_rows = 0;
+    hipStream_t side[2] = {nullptr, nullptr};
+    hipStream_t capture = nullptr;  // the graph is captured here (the caller's may be the null stream)
+    hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::string tuning;
+};
+
+namespace iph {
+namespace {
+
+void unplan(iph_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (int i = 0; i < 2; i++) {
+        if (b->side[i]) (void)hipStreamDestroy(b->side[i]);
+        if (b->join[i]) (void)hipEventDestroy(b->join[i]);
+        b->side[i] = nullptr;
+        b->join[i] = nullptr;
+    }
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->capture) (void)hipStreamDestroy(b->capture);
+    b->capture = nullptr;
+    if (b->mem) (void)hipFree(b->mem);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->fork = nullptr;
+    b->mem = nullptr;
+    b->mem_bytes = 0;
+    b->planned = false;
+}
+
+iph_status enqueue_blur(iph_batch *b, hipStream_t s) {
+    IPH_TRY(launch_blur_jobs(b->mem + b->blur_table_off, b->blur_groups, b->blur_grid_pct, s));
+    for (const FallbackBlur &f : b->blur_fallback) {
+        const BlurSpec &sp = b->blur[f.spec];
+        IPH_TRY(launch_separable(sp.src, sp.dst, sp.w, sp.h, 1, (size_t)sp.w * sp.h, IPH_U8,
+                                 sp.taps, sp.taps, sp.len, IPH_F32, b->mem + f.ws_off,
+                                 f.ws_bytes, s));
+    }
+    return IPH_OK;
+}
+
+iph_status enqueue_filters(iph_batch *b, hipStream_t s) {
+    return launch_filter_jobs(b->mem + b->filt_table_off, (int)b->filt.size(), b->filt_tiles,
+                              b->filt_variant, b->filt_max_kw, b->filt_max_kh, s);
+}
+
+iph_status enqueue_matches(iph_batch *b, hipStream_t s) {
+    set_match_rows_hint(b->match_rows);
+    for (const MatchSpec &m : b->match) {
+        iph_status st = launch_match(m.img, IPH_U8, m.w, m.h, m.tpl, m.tw, m.th, m.method,
+                                     resolve_path(IPH_PATH_AUTO, m.tw, m.th), m.out,
+                                     b->mem + m.ws_off, m.ws_bytes, s);
+        if (st != IPH_OK) {
+            set_match_rows_hint(0);
+            return st;
+        }
+    }
+    set_match_rows_hint(0);
+    return IPH_OK;
+}
+
+// All jobs: blurs on `s`, filters and matches on the two side streams forked
+// from `s` and joined back into it.
+iph_status enqueue_all(iph_batch *b, hipStream_t s) {
+    IPH_TRY_HIP(hipEventRecord(b->fork, s));
+    for (int i = 0; i < 2; i++) IPH_TRY_HIP(hipStreamWaitEvent(b->side[i], b->fork, 0));
+    IPH_TRY(enqueue_blur(b, s));
+    IPH_TRY(enqueue_filters(b, b->side[0]));
+    IPH_TRY(enqueue_matches(b, b->side[1]));
+    for (int i = 0; i < 2; i++) {
+        IPH_TRY_HIP(hipEventRecord(b->join[i], b->side[i]));
+        IPH_TRY_HIP(hipStreamWaitEvent(s, b->join[i], 0));
+    }
+    return IPH_OK;
+}
+
+iph_status upload_filter_table(iph_batch *b, int variant) {
+    if (b->filt.empty()) return IPH_OK;
+    std::vector<unsigned char> host(filter_jobs_table_bytes((int)b->filt.size()));
+    IPH_TRY(filter_jobs_table(b->filt.data(), (int)b->filt.size(), variant, host.data(),
+                              &b->filt_tiles));
+    IPH_TRY_HIP(hipMemcpy(b->mem + b->filt_table_off, host.data(), host.size(),
+                          hipMemcpyHostToDevice));
+    b->filt_variant = variant;
+    return IPH_OK;
+}
+
+// Milliseconds of the fastest of `reps` runs of fn on stream s.
+template <typename F> iph_status time_min(hipStream_t s, int reps, F fn, float *best) {
+    hipEvent_t e0, e1;
+    IPH_TRY_HIP(hipEventCreate(&e0));
+    IPH_TRY_HIP(hipEventCreate(&e1));
+    *best = 1e30f;
+    iph_status st = IPH_OK;
+    for (int r = 0; r < reps && st == IPH_OK; r++) {
+        (void)hipEventRecord(e0, s);
+        st = fn();
+        (void)hipEventRecord(e1, s);
+        if (hipEventSynchronize(e1) != hipSuccess) break;
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess) *best = std::min(*best, ms);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return st;
+}
+
+iph_status autotune(iph_batch *b, hipStream_t s) {
+    char line[160];
+    b->tuning.clear();
+    if (!b->filt.empty()) {
+        int best_v = 0;
+        float best_ms = 1e30f;
+        for (int v = 0; v < filter_jobs_variants(); v++) {
+            IPH_TRY(upload_filter_table(b, v));
+            float ms;
+            IPH_TRY(time_min(s, 3, [&] { return enqueue_filters(b, s); }, &ms));
+            std::snprintf(line, sizeof line, "filter tile %s: %.3f ms\n",
+                          filter_jobs_variant_name(v), ms);
+            b->tuning += line;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_v = v;
+            }
+        }
+        IPH_TRY(upload_filter_table(b, best_v));
+    }
+    if (!b->match.empty()) {
+        int best_r = 0;
+        float best_ms = 1e30f;
+        for (int rows : {4, 2, 1}) {
+            b->match_rows = rows;
+            float ms;
+            IPH_TRY(time_min(s, 3, [&] { return enqueue_matches(b, s); }, &ms));
+            std::snprintf(line, sizeof line, "match rows/workgroup %d: %.3f ms\n", 32 * rows, ms);
+            b->tuning += line;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_r = rows;
+            }
+        }
+        b->match_rows = best_r;
+    }
+    if (!b->blur_groups.empty()) {
+        int best_p = 100;
+        float best_ms = 1e30f;
+        for (int pct : {100, 67, 34}) {
+            b->blur_grid_pct = pct;
+            float ms;
+            IPH_TRY(time_min(s, 3, [&] { return enqueue_blur(b, s); }, &ms));
+            std::snprintf(line, sizeof line, "blur persistent workgroups %d%%: %.3f ms\n", pct, ms);
+            b->tuning += line;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_p = pct;
+            }
+        }
+        b->blur_grid_pct = best_p;
+    }
+    std::snprintf(line, sizeof line, "chosen: filter %s, match %d rows, blur %d%%\n",
+                  filter_jobs_variant_name(b->filt_variant), 32 * std::max(b->match_rows, 1),
+                  b->blur_grid_pct);
+    b->tuning += line;
+    return IPH_OK;
+}
+
+iph_status plan(iph_batch *b, int32_t tune, hipStream_t s) {
+    unplan(b);
+    IPH_TRY(require_device());
+    IPH_TRY_HIP(hipGetDevice(&b->device));
+    // layout of the one device allocation
+    WorkspacePlan wp;
+    std::vector<unsigned char> blur_host(blur_jobs_table_bytes((int)b->blur.size()));
+    std::vector<int> fallback;
+    IPH_TRY(blur_jobs_table(b->blur.data(), (int)b->blur.size(), blur_host.data(),
+                            b->blur_groups, fallback));
+    b->blur_table_off = 0;
+    wp.take<unsigned char>(blur_host.size());
+    b->blur_fallback.clear();
+    for (int i : fallback) {
+        const BlurSpec &sp = b->blur[i];
+        size_t n = separable_workspace(sp.w, sp.h, 1, IPH_U8, sp.len, IPH_F32);
+        b->blur_fallback.push_back({i, align_up(wp.used, 256), n});
+        wp.take<unsigned char>(std::max<size_t>(n, 1));
+    }
+    b->filt_table_off = align_up(wp.used, 256);
+    wp.take<unsigned char>(filter_jobs_table_bytes((int)b->filt.size()));
+    b->filt_max_kw = b->filt_max_kh = 1;
+    for (const FilterSpec &f : b->filt) {
+        b->filt_max_kw = std::max(b->filt_max_kw, f.kw);
+        b->filt_max_kh = std::max(b->filt_max_kh, f.kh);
+    }
+    for (MatchSpec &m : b->match) {
+        m.ws_bytes = match_workspace(IPH_U8, m.w, m.h, m.tw, m.th, m.method,
+                                     resolve_path(IPH_PATH_AUTO, m.tw, m.th));
+        m.ws_off = align_up(wp.used, 256);
+        wp.take<unsigned char>(std::max<size_t>(m.ws_bytes, 1));
+    }
+    b->mem_bytes = std::max<size_t>(wp.used, 256);
+    if (hipMalloc(reinterpret_cast<void **>(&b->mem), b->mem_bytes) != hipSuccess) {
+        b->mem = nullptr;
+        set_error("cannot allocate %zu bytes of batch memory", b->mem_bytes);
+        return IPH_ENOMEM;
+    }
+    IPH_TRY_HIP(hipMemcpy(b->mem + b->blur_table_off, blur_host.data(), blur_host.size(),
+                          hipMemcpyHostToDevice));
+    IPH_TRY(upload_filter_table(b, 0));
+    for (int i = 0; i < 2; i++) {
+        IPH_TRY_HIP(hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking));
+        IPH_TRY_HIP(hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming));
+    }
+    IPH_TRY_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    IPH_TRY_HIP(hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking));
+    b->blur_grid_pct = 100;
+    b->match_rows = 0;
+    b->tuning = "not tuned\n";
+    if (tune) IPH_TRY(autotune(b, s));
+    IPH_TRY_HIP(hipStreamSynchronize(s));
+    // one graph for the whole batch, captured on the batch's own stream
+    IPH_TRY_HIP(hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal));
+    iph_status st = enqueue_all(b, b->capture);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->capture, &g);
+    if (st != IPH_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) {
+        set_error("graph capture of the batch failed: %s", hipGetErrorString(e));
+        return IPH_EDEVICE;
+    }
+    b->graph = g;
+    IPH_TRY_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->planned = true;
+    return IPH_OK;
+}
+
+iph_status check_batch(iph_batch *b) {
+    if (!b) {
+        set_error("null batch");
+        return IPH_EINVAL;
+    }
+    return IPH_OK;
+}
+
+}  // namespace
+}  // namespace iph
+
+using namespace iph;
+
+extern "C" {
+
+IPH_API iph_status iph_batch_create(iph_batch **out) {
+    clear_error();
+    if (!out) {
+        set_error("null output pointer");
+        return IPH_EINVAL;
+    }
+    *out = new (std::nothrow) iph_batch();
+    if (!*out) {
+        set_error("out of host memory");
+        return IPH_ENOMEM;
+    }
+    return IPH_OK;
+}
+
+IPH_API void iph_batch_destroy(iph_batch *b) {
+    if (!b) return;
+    unplan(b);
+    delete b;
+}
+
+IPH_API iph_status iph_batch_add_gaussian_blur(iph_batch *b, const uint8_t *src, uint8_t *dst,
+                                               uint32_t w, uint32_t h, float sigma) {
+    clear_error();
+    IPH_TRY(check_batch(b));
+    // gaussian_blur_f32, filter/mod.rs:156-164
+    if (!(sigma > 0.0f)) {
+        set_error("sigma must be > 0.0");
+        return IPH_EINVAL;
+    }
+    IPH_TRY(check_dims(w, h));
+    std::vector<float> k;
+    const int32_t len = gaussian_kernel(sigma, k);
+    if (len < 0 || len > kBlurSpecMaxTaps) {
+        set_error("batched gaussian_blur_f32 takes sigma up to 8 (at most %d taps)",
+                  kBlurSpecMaxTaps);
+        return IPH_EUNSUPPORTED;
+    }
+    if (w == 0 || h == 0) return IPH_OK;
+    BlurSpec sp{};
+    sp.src = src;
+    sp.dst = dst;
+    sp.w = w;
+    sp.h = h;
+    sp.len = (uint32_t)len;
+    for (int32_t i = 0; i < len; i++) sp.taps[i] = k[i];
+    unplan(b);
+    b->blur.push_back(sp);
+    return IPH_OK;
+}
+
+IPH_API iph_status iph_batch_add_filter_clamped(iph_batch *b, const uint8_t *src, uint8_t *dst,
+                                                uint32_t w, uint32_t h, const float *kernel,
+                                                uint32_t kw, uint32_t kh) {
+    clear_error();
+    IPH_TRY(check_batch(b));
+    // Kernel::new, kernel.rs:24-35
+    if (kw == 0 || kh == 0) {
+        set_error("width and height must be non-zero");
+        return IPH_EINVAL;
+    }
+    if (kw * kh > (uint32_t)kF2MaxJobTaps) {
+        set_error("batched filter_clamped takes kernels of at most %d taps", kF2MaxJobTaps);
+        return IPH_EUNSUPPORTED;
+    }
+    IPH_TRY(check_dims(w, h));
+    if (w == 0 || h == 0) return IPH_OK;
+    FilterSpec f{};
+    f.src = src;
+    f.dst = dst;
+    f.w = w;
+    f.h = h;
+    f.kw = kw;
+    f.kh = kh;
+    std::memcpy(f.k, kernel, sizeof(float) * kw * kh);
+    unplan(b);
+    b->filt.push_back(f);
+    return IPH_OK;
+}
+
+IPH_API iph_status iph_batch_add_match_template(iph_batch *b, const uint8_t *image, uint32_t w,
+                                                uint32_t h, const uint8_t *tpl, uint32_t tw,
+                                                uint32_t th, int32_t method, float *out) {
+    clear_error();
+    IPH_TRY(check_batch(b));
+    IPH_TRY(check_match(IPH_U8, w, h, tw, th, method, IPH_PATH_AUTO));
+    if (tw > 128 || th > 128) {
+        set_error("batched match_template takes templates up to 128x128");
+        return IPH_EUNSUPPORTED;
+    }
+    unplan(b);
+    b->match.push_back({image, w, h, tpl, tw, th, method, out, 0, 0});
+    return IPH_OK;
+}
+
+IPH_API iph_status iph_batch_plan(iph_batch *b, int32_t autotune, void *stream) {
+    clear_error();
+    IPH_TRY(check_batch(b));
+    iph_status st = plan(b, autotune, static_cast<hipStream_t>(stream));
+    if (st != IPH_OK) unplan(b);
+    return st;
+}
+
+IPH_API iph_status iph_batch_run(iph_batch *b, void *stream) {
+    clear_error();
+    IPH_TRY(check_batch(b));
+    if (!b->planned) {
+        set_error("batch is not planned (iph_batch_plan)");
+        return IPH_EINVAL;
+    }
+    IPH_TRY_HIP(hipGraphLaunch(b->exec, static_cast<hipStream_t>(stream)));
+    return IPH_OK;
+}
+
+IPH_API iph_status iph_batch_run_eager(iph_batch *b, void *stream) {
+    clear_error();
+    IPH_TRY(check_batch(b));
+    if (!b->planned) {
+        set_error("batch is not planned (iph_batch_plan)");
+        return IPH_EINVAL;
+    }
+    return enqueue_all(b, static_cast<hipStream_t>(stream));
+}
+
+IPH_API iph_status iph_batch_tuning(iph_batch *b, char *buf, size_t cap) {
+    clear_error();
+    IPH_TRY(check_batch(b));
+    if (!buf || cap == 0) return IPH_OK;
+    std::snprintf(buf, cap, "%s", b->tuning.c_str());
+    return IPH_OK;
+}
+
+}  // extern "C"

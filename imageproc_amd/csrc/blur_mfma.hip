@@ -1143,4 +1143,117 @@ iph_status launch_blur_u8_mfma(const uint8_t *src, uint8_t *dst, uint32_t w, uin
     return launch_status("blur_u8_mfma_kernel");
 }
 
+// ---- batches: fixed-point jobs grouped by kernel variant -------------------------
+
+size_t blur_jobs_table_bytes(int n) { return (size_t)std::max(n, 1) * sizeof(FxJob); }
+
+iph_status blur_jobs_table(const BlurSpec *specs, int n, void *host_table,
+                           std::vector<BlurGroup> &groups, std::vector<int> &fallback) {
+    struct Entry {
+        int key, idx;
+        FxJob job;
+    };
+    std::vector<Entry> entries;
+    groups.clear();
+    fallback.clear();
+    for (int i = 0; i < n; i++) {
+        const BlurSpec &b = specs[i];
+        const int L = (int)b.len;
+        bool same = true;
+        if (L < 1 || L > kFxMaxL || !blur_mfma_supported(b.taps, b.taps, b.len)) {
+            fallback.push_back(i);
+            continue;
+        }
+        const int half = L / 2, m = (4 - half % 4) % 4, nkb = (31 + L + m + 15) / 16;
+        const FxPlan pl = blur_fx_plan(b.taps, L, nkb);
+        if (pl.c <= 0 || nkb < 2 || nkb > 4) {
+            fallback.push_back(i);
+            continue;
+        }
+        const bool aligned = b.w >= 4 && (b.w % 4) == 0 &&
+                             (reinterpret_cast<uintptr_t>(b.src) & 3) == 0;
+        const int cls = (same && m == 0) ? 2 : 1;
+        FxJob f{};
+        f.src = b.src;
+        f.dst = b.dst;
+        f.stride = (long long)b.w * b.h;
+        f.W = (int)b.w;
+        f.H = (int)b.h;
+        f.n_img = 1;
+        f.tiles_x = (int)((b.w + kTW - 1) / kTW);
+        f.tiles_y = (int)((b.h + kTH2 - 1) / kTH2);
+        for (int q = 0; q < L; q++) f.h[q] = f.v[q] = b.taps[q];
+        f.L = L;
+        f.m = m;
+        f.ch = f.cv = pl.c;
+        f.init_h = f.init_v = pl.init;
+        entries.push_back({nkb * 8 + cls * 2 + (aligned ? 1 : 0), i, f});
+    }
+    std::stable_sort(entries.begin(), entries.end(),
+                     [](const Entry &a, const Entry &b) { return a.key < b.key; });
+    FxJob *table = static_cast<FxJob *>(host_table);
+    for (size_t e = 0; e < entries.size(); e++) {
+        const int key = entries[e].key;
+        if (groups.empty() || groups.back().nkb * 8 + groups.back().taps_class * 2 +
+                                      (groups.back().aligned ? 1 : 0) != key) {
+            groups.push_back({key / 8, (key % 8) / 2, (key & 1) != 0, (int)e, 0, 0});
+        }
+        BlurGroup &g = groups.back();
+        FxJob f = entries[e].job;
+        f.tile0 = g.ntiles;
+        const long long nt = (long long)g.ntiles + (long long)f.tiles_x * f.tiles_y;
+        if (nt > (1LL << 30)) {
+            set_error("batch too large for one launch");
+            return IPH_EINVAL;
+        }
+        g.ntiles = (int)nt;
+        g.count++;
+        table[e] = f;
+    }
+    return IPH_OK;
+}
+
+iph_status launch_blur_jobs(const void *table_dev, const std::vector<BlurGroup> &groups,
+                            int grid_pct, hipStream_t s) {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const FxJob *table = static_cast<const FxJob *>(table_dev);
+    for (const BlurGroup &g : groups) {
+        if (g.ntiles == 0) continue;
+#define JB(K, CLS, AL)                                                                       \
+    do {                                                                                     \
+        using Sh = RowStage<K, kRows2>;                                                     \
+        size_t lds = (size_t)((kRows2 * Sh::PIN + 15) / 16 * 16) + (size_t)kTW * kPitchB +   \
+                     2 * (2 * kTapPad + kMaxTaps) * sizeof(float) +                          \
+                     4 * kFxQCap * sizeof(uint16_t);                                          \
+        int per_cu = 0;                                                                      \
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(                                    \
+                &per_cu, blur_u8_fx_jobs_kernel<K, CLS, AL>, 256, lds) != hipSuccess ||      \
+            per_cu < 1)                                                                      \
+            per_cu = 1;                                                                      \
+        long long grid = (long long)cus * per_cu * std::max(grid_pct, 1) / 100;              \
+        grid = std::min<long long>(grid, ((long long)g.ntiles + 7) / 8 * 8);                 \
+        grid = std::max<long long>(8, grid / 8 * 8);                                         \
+        blur_u8_fx_jobs_kernel<K, CLS, AL><<<(unsigned)grid, 256, lds, s>>>(                \
+            table + g.first, g.count, g.ntiles);                                             \
+    } while (0)
+#define JA(K, CLS)                                                                           \
+    if (g.aligned) JB(K, CLS, true); else JB(K, CLS, false)
+#define JK(K)                                                                                \
+    if (g.taps_class == 2) { JA(K, 2); } else { JA(K, 1); }                                  \
+    break
+        switch (g.nkb) {
+        case 2: JK(2);
+        case 3: JK(3);
+        default: JK(4);
+        }
+#undef JK
+#undef JA
+#undef JB
+        IPH_TRY(launch_status("blur_u8_fx_jobs_kernel"));
+    }
+    return IPH_OK;
+}
+
 }  // namespace iph

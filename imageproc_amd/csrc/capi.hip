@@ -24,7 +24,6 @@ void set_error(const char *fmt, ...) {
 
 void clear_error() { g_last_error.clear(); }
 
-namespace {
 
 // Fails unless a gfx950 device is current: the library ships gfx950 code only.
 iph_status require_device() {
@@ -181,7 +180,6 @@ int32_t resolve_path(int32_t path, uint32_t tw, uint32_t th) {
     return path == IPH_PATH_AUTO ? IPH_PATH_MFMA : path;
 }
 
-}  // namespace
 }  // namespace iph
 
 using namespace iph;

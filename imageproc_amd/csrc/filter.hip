@@ -237,6 +237,52 @@ __global__ __launch_bounds__(256) void filter2d_naive_kernel(const P *__restrict
     dst[(size_t)y * W + x] = ClampTo<S>::from(acc);
 }
 
+// ---- ragged 2-D filter for batches ------------------------------------------
+//
+// filter_clamped<u8, f32, u8> over many images of different sizes in one
+// launch: the grid walks a global list of TW x TH output tiles; job j owns
+// tiles [tile0, tile0 + tiles_x * tiles_y) of its image. The arithmetic is
+// filter2d_tiled_kernel's (ky outer, kx inner, f32 mul-then-add).
+struct F2Job {
+    const uint8_t *src;
+    uint8_t *dst;
+    int W, H, kw, kh, tiles_x, tile0;
+    float w[kF2MaxJobTaps];
+};
+
+template <int TW, int TH>
+__global__ __launch_bounds__(256) void filter2d_jobs_kernel(const F2Job *__restrict__ jobs,
+                                                            int njobs) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    uint8_t *in_s = smem_raw;
+    // the job owning this tile: binary search over tile0 (uniform per workgroup)
+    const int t = (int)blockIdx.x;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].tile0 <= t) lo = mid; else hi = mid - 1;
+    }
+    const F2Job &jb = jobs[lo];
+    const int lt = t - jb.tile0, ty = lt / jb.tiles_x, tx = lt - ty * jb.tiles_x;
+    const int W = jb.W, H = jb.H, kw = jb.kw, kh = jb.kh;
+    const int R = TH + kh - 1, C = TW + kw - 1;
+    const int x0 = tx * TW, y0 = ty * TH;
+    stage_window(jb.src, W, H, y0 - kh / 2, x0 - kw / 2, R, C, in_s);
+    __syncthreads();
+    for (int i = threadIdx.x; i < TH * TW; i += blockDim.x) {
+        int r = i / TW, c = i - r * TW;
+        int gy = y0 + r, gx = x0 + c;
+        if (gy >= H || gx >= W) continue;
+        float acc = 0.0f;
+        int q = 0;
+        for (int ky = 0; ky < kh; ky++) {
+            const uint8_t *row = in_s + (r + ky) * C + c;
+            for (int kx = 0; kx < kw; kx++, q++) acc = mac(acc, to_acc(row[kx], 0.0f), jb.w[q]);
+        }
+        jb.dst[(size_t)gy * W + gx] = ClampTo<uint8_t>::from(acc);
+    }
+}
+
 // ---- box filter (exact u32 window sums) ---------------------------------------
 //
 // out = floor(sum over the clamped (2r+1)-window / (2r+1)), first along rows
@@ -510,6 +556,75 @@ iph_status launch_box_filter(const uint8_t *src, uint8_t *dst, uint32_t w, uint3
     dim3 grid((w + 255) / 256, (h + kBoxSeg - 1) / kBoxSeg);
     box_cols_kernel<<<grid, 256, 0, s>>>(mid, dst, (int)w, (int)h, (int)ry, kh);
     return launch_status("box_cols_kernel");
+}
+
+// ---- ragged 2-D filter batches --------------------------------------------------
+
+namespace {
+constexpr int kF2Variants[3][2] = {{64, 16}, {64, 32}, {128, 16}};
+
+template <int TW, int TH> int f2_tiles(const FilterSpec &f) {
+    return (int)(((f.w + TW - 1) / TW) * ((f.h + TH - 1) / TH));
+}
+}  // namespace
+
+int filter_jobs_variants() { return 3; }
+
+const char *filter_jobs_variant_name(int v) {
+    static const char *names[3] = {"64x16", "64x32", "128x16"};
+    return names[std::max(0, std::min(2, v))];
+}
+
+size_t filter_jobs_table_bytes(int n) { return (size_t)std::max(n, 1) * sizeof(F2Job); }
+
+iph_status filter_jobs_table(const FilterSpec *specs, int n, int variant, void *host_table,
+                             int *total_tiles) {
+    const int TW = kF2Variants[variant][0], TH = kF2Variants[variant][1];
+    F2Job *jobs = static_cast<F2Job *>(host_table);
+    long long tile = 0;
+    for (int i = 0; i < n; i++) {
+        const FilterSpec &f = specs[i];
+        if (f.kw * f.kh > (uint32_t)kF2MaxJobTaps || f.kw == 0 || f.kh == 0) {
+            set_error("batched filter_clamped takes kernels of 1 to %d taps", kF2MaxJobTaps);
+            return IPH_EUNSUPPORTED;
+        }
+        F2Job j{};
+        j.src = f.src;
+        j.dst = f.dst;
+        j.W = (int)f.w;
+        j.H = (int)f.h;
+        j.kw = (int)f.kw;
+        j.kh = (int)f.kh;
+        j.tiles_x = (int)((f.w + TW - 1) / TW);
+        j.tile0 = (int)tile;
+        for (uint32_t q = 0; q < f.kw * f.kh; q++) j.w[q] = f.k[q];
+        jobs[i] = j;
+        tile += (long long)j.tiles_x * ((f.h + TH - 1) / TH);
+        if (tile > (1LL << 30)) {
+            set_error("batch too large for one launch");
+            return IPH_EINVAL;
+        }
+    }
+    *total_tiles = (int)tile;
+    return IPH_OK;
+}
+
+iph_status launch_filter_jobs(const void *table_dev, int n, int total_tiles, int variant,
+                              uint32_t max_kw, uint32_t max_kh, hipStream_t s) {
+    if (n == 0 || total_tiles == 0) return IPH_OK;
+    const F2Job *jobs = static_cast<const F2Job *>(table_dev);
+#define FV(TW, TH)                                                                           \
+    do {                                                                                     \
+        size_t lds = (size_t)(TH + max_kh - 1) * (TW + max_kw - 1);                          \
+        filter2d_jobs_kernel<TW, TH><<<total_tiles, 256, lds, s>>>(jobs, n);                 \
+    } while (0)
+    switch (variant) {
+    case 0: FV(64, 16); break;
+    case 1: FV(64, 32); break;
+    default: FV(128, 16); break;
+    }
+#undef FV
+    return launch_status("filter2d_jobs_kernel");
 }
 
 }  // namespace iph

@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <vector>
 
 #include "imageproc_hip.h"
 
@@ -102,6 +103,62 @@ iph_status launch_blur_u8_mfma(const uint8_t *src, uint8_t *dst, uint32_t w, uin
 iph_status launch_box_filter(const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h,
                              uint32_t rx, uint32_t ry, void *ws, size_t ws_bytes,
                              hipStream_t s);
+
+// ---- argument checks and host helpers shared by the C ABI (capi.hip) -------
+
+// Fails unless a gfx950 device is current.
+iph_status require_device();
+// gaussian_kernel_f32's taps (host, glibc expf); returns the count or -1.
+int32_t gaussian_kernel(float sigma, std::vector<float> &k);
+iph_status check_dims(uint32_t w, uint32_t h);
+iph_status check_match(int32_t p, uint32_t w, uint32_t h, uint32_t tw, uint32_t th,
+                       int32_t method, int32_t path);
+int32_t resolve_path(int32_t path, uint32_t tw, uint32_t th);
+
+// ---- batches of uneven images (filter.hip, blur_mfma.hip, batch.hip) ------
+
+// filter_clamped<u8, f32, u8> job of a batch (device pointers).
+constexpr int kF2MaxJobTaps = 49;
+struct FilterSpec {
+    const uint8_t *src;
+    uint8_t *dst;
+    uint32_t w, h, kw, kh;
+    float k[kF2MaxJobTaps];
+};
+int filter_jobs_variants();
+const char *filter_jobs_variant_name(int variant);
+size_t filter_jobs_table_bytes(int n);
+iph_status filter_jobs_table(const FilterSpec *specs, int n, int variant, void *host_table,
+                             int *total_tiles);
+iph_status launch_filter_jobs(const void *table_dev, int n, int total_tiles, int variant,
+                              uint32_t max_kw, uint32_t max_kh, hipStream_t s);
+
+// gaussian_blur_f32 job of a batch (device pointers, host taps).
+constexpr int kBlurSpecMaxTaps = 33;
+struct BlurSpec {
+    const uint8_t *src;
+    uint8_t *dst;
+    uint32_t w, h, len;
+    float taps[kBlurSpecMaxTaps];
+};
+// Jobs of one fixed-point kernel variant, consecutive in the device table.
+struct BlurGroup {
+    int nkb, taps_class;
+    bool aligned;
+    int first, count, ntiles;
+};
+size_t blur_jobs_table_bytes(int n);
+// Fills the host table (grouped by kernel variant) and lists the specs the
+// fixed-point kernel does not take in `fallback`.
+iph_status blur_jobs_table(const BlurSpec *specs, int n, void *host_table,
+                           std::vector<BlurGroup> &groups, std::vector<int> &fallback);
+// grid_pct: persistent workgroups as a percentage of the resident maximum.
+iph_status launch_blur_jobs(const void *table_dev, const std::vector<BlurGroup> &groups,
+                            int grid_pct, hipStream_t s);
+
+// Rows of 32 outputs per workgroup of the int8 MFMA match kernel (0 = the
+// largest that fits in LDS); set by the batch autotuner for its launches.
+void set_match_rows_hint(int rows);
 
 // ---- launchers implemented in match.hip ---------------------------------
 

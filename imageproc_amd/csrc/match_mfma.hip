@@ -26,6 +26,8 @@
 // in f64.
 #include "match_common.hpp"
 
+#include <algorithm>
+
 namespace iph {
 namespace {
 
@@ -529,13 +531,15 @@ size_t win_lds(int tw, int th) {
     return align_up((size_t)rows * cols, 16) + (size_t)rows * kWinCols * sizeof(uint2);
 }
 
+thread_local int g_match_rows_hint = 0;
+
 template <int KB>
 iph_status launch_i8_kb(const uint8_t *img, int W, int H, const int8_t *bsw, const uint2 *wsum,
                         const long long *tsums, float *out, int ow, int oh, int tw, int th,
                         int method, hipStream_t s) {
     constexpr int BW = 32 * (kXT + KB - 1);
     int pitch = ((BW / 16) % 2 == 0) ? BW + 16 : BW;  // odd multiple of 16: no bank conflicts
-    int RT = 4;
+    int RT = g_match_rows_hint > 0 ? std::min(g_match_rows_hint, 4) : 4;
     while (RT > 1 && (size_t)(32 * RT + th - 1) * pitch > 64 * 1024) RT /= 2;
     size_t lds = (size_t)(32 * RT + th - 1) * pitch;
     dim3 grid((ow + 32 * kXT - 1) / (32 * kXT), (oh + 32 * RT - 1) / (32 * RT));
@@ -671,5 +675,7 @@ iph_status launch_mfma_match(const void *img, int32_t p_type, uint32_t w, uint32
     default: return launch_i8_kb<5>(im, w, h, bsw, wsum, tsums, out, ow, oh, tw, th, method, s);
     }
 }
+
+void set_match_rows_hint(int rows) { g_match_rows_hint = rows; }
 
 }  // namespace iph

@@ -184,6 +184,44 @@ IPH_API iph_status iph_dev_find_extremes(const void *image, int32_t p_type, uint
                                          uint32_t height, iph_extremes *out, void *workspace,
                                          size_t workspace_bytes, void *stream);
 
+/*
+ * Batches of uneven GrayImages. The reference has no batch API: a batch
+ * replaces a loop of per-image calls of gaussian_blur_f32
+ * (filter/mod.rs:156-164), filter_clamped::<Luma<u8>, f32, u8>
+ * (filter/mod.rs:202-210) and match_template (template_matching.rs:78-92),
+ * and each job's output is bit-identical to that call's. All pointers are
+ * device memory that stays valid while the batch is used; outputs must not
+ * overlap inputs of other jobs.
+ *
+ * iph_batch_plan allocates the batch's device memory, autotunes the launch
+ * shapes when `autotune` is non-zero (it runs the jobs a few times to time
+ * them), and captures every job into one hipGraph. iph_batch_run replays that
+ * graph on `stream`; iph_batch_run_eager enqueues the same launches without
+ * the graph. Adding a job after iph_batch_plan discards the plan. A batch is
+ * not thread-safe, and runs of one batch must not overlap.
+ */
+typedef struct iph_batch iph_batch;
+IPH_API iph_status iph_batch_create(iph_batch **out);
+IPH_API void iph_batch_destroy(iph_batch *batch);
+/* sigma in (0, 8] (at most 33 taps). */
+IPH_API iph_status iph_batch_add_gaussian_blur(iph_batch *batch, const uint8_t *src,
+                                               uint8_t *dst, uint32_t width, uint32_t height,
+                                               float sigma);
+/* kernel: kw * kh f32 weights, row-major, host memory; at most 49 taps. */
+IPH_API iph_status iph_batch_add_filter_clamped(iph_batch *batch, const uint8_t *src,
+                                                uint8_t *dst, uint32_t width, uint32_t height,
+                                                const float *kernel, uint32_t kw, uint32_t kh);
+/* templates up to 128x128; out: (width - tw + 1) x (height - th + 1) f32. */
+IPH_API iph_status iph_batch_add_match_template(iph_batch *batch, const uint8_t *image,
+                                                uint32_t width, uint32_t height,
+                                                const uint8_t *tmpl, uint32_t tw, uint32_t th,
+                                                int32_t method, float *out);
+IPH_API iph_status iph_batch_plan(iph_batch *batch, int32_t autotune, void *stream);
+IPH_API iph_status iph_batch_run(iph_batch *batch, void *stream);
+IPH_API iph_status iph_batch_run_eager(iph_batch *batch, void *stream);
+/* The autotuner's measurements and choices, as text. */
+IPH_API iph_status iph_batch_tuning(iph_batch *batch, char *buf, size_t cap);
+
 #ifdef __cplusplus
 }
 #endif

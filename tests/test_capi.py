@@ -92,3 +92,28 @@ def test_empty_images_need_no_device():
         assert gaussian_blur_f32(np.zeros(shape, np.uint8), 1.0).shape == shape
         assert ip.filter.filter_clamped(np.zeros(shape, np.uint8),
                                         ip.Kernel([2], 1, 1)).shape == shape
+
+
+def test_batch_argument_errors_before_device():
+    """Batch jobs are validated when added, with the single calls' messages
+    (filter/mod.rs:157 sigma assert, kernel.rs:24-35 Kernel::new)."""
+    import ctypes as C
+
+    from imageproc_amd._lib import Panic, check, lib
+
+    h = C.c_void_p()
+    check(lib().iph_batch_create(C.byref(h)))
+    try:
+        with pytest.raises(Panic, match="sigma must be > 0.0"):
+            check(lib().iph_batch_add_gaussian_blur(h, None, None, 8, 8, 0.0))
+        with pytest.raises(TypeError):
+            check(lib().iph_batch_add_gaussian_blur(h, None, None, 8, 8, 8.5))  # 35 taps
+        k = (C.c_float * 4)()
+        with pytest.raises(Panic, match="width and height must be non-zero"):
+            check(lib().iph_batch_add_filter_clamped(h, None, None, 8, 8, k, 0, 4))
+        with pytest.raises(Panic, match="image width must be greater than or equal"):
+            check(lib().iph_batch_add_match_template(h, None, 4, 8, None, 5, 5, 0, None))
+        with pytest.raises(Panic, match="not planned"):
+            check(lib().iph_batch_run(h, None))
+    finally:
+        lib().iph_batch_destroy(h)

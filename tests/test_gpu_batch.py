@@ -1,0 +1,130 @@
+"""Batches of uneven images (iph_batch_*): every job of a planned batch, replayed
+as a hipGraph or enqueued eagerly, equals the single call it stands for, bit
+for bit; blur and filter jobs also equal the oracle (filter/mod.rs:156-164,
+202-210; template_matching.rs:78-92 for the match jobs, whose single calls are
+checked against the oracle in test_gpu_match.py)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from imageproc_amd import device as dev
+from imageproc_amd.batch import Batch
+from imageproc_amd.kernel import Kernel
+from imageproc_amd.template_matching import MatchTemplateMethod as M
+
+pytestmark = pytest.mark.gpu
+
+
+def kernel5(rng):
+    """5x5 N(0,1) weights normalised to sum 1, redrawn while |sum| < 0.25."""
+    while True:
+        k = rng.standard_normal((5, 5)).astype(np.float32)
+        s = np.float32(k.sum(dtype=np.float32))
+        if abs(s) >= 0.25:
+            return (k / s).astype(np.float32)
+
+
+def make_batch(seed, n_img):
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(n_img):
+        h, w = (int(v) for v in rng.integers(33, 300, size=2))
+        img = torch.from_numpy(rng.integers(0, 256, (h, w), dtype=np.uint8)).cuda()
+        sigma = float(rng.uniform(0.5, 8.0))
+        k = kernel5(rng)
+        tpls = []
+        for _ in range(int(rng.integers(1, 4))):
+            th = int(rng.integers(8, min(40, h) + 1))
+            tw = int(rng.integers(8, min(40, w) + 1))
+            y, x = int(rng.integers(0, h - th + 1)), int(rng.integers(0, w - tw + 1))
+            tpls.append((img[y:y + th, x:x + tw].contiguous(), M(int(rng.integers(0, 4)))))
+        jobs.append((img, sigma, k, tpls))
+    return jobs
+
+
+def expected(jobs):
+    out = []
+    for img, sigma, k, tpls in jobs:
+        b = dev.gaussian_blur_f32(img, sigma)
+        f = dev.filter_clamped(img, Kernel(k, 5, 5), torch.uint8)
+        ms = [dev.match_template(img, t, m, "auto") for t, m in tpls]
+        out.append((b, f, ms))
+    torch.cuda.synchronize()
+    return out
+
+
+def build(jobs, autotune):
+    bt = Batch()
+    outs = []
+    for img, sigma, k, tpls in jobs:
+        b = bt.add_gaussian_blur(img, sigma)
+        f = bt.add_filter_clamped(img, k)
+        ms = [bt.add_match_template(img, t, m) for t, m in tpls]
+        outs.append((b, f, ms))
+    bt.plan(autotune=autotune)
+    return bt, outs
+
+
+def clear(outs):
+    for b, f, ms in outs:
+        b.zero_()
+        f.zero_()
+        for m in ms:
+            m.fill_(-1.0)
+
+
+def assert_same(outs, exp):
+    for (b, f, ms), (eb, ef, ems) in zip(outs, exp):
+        assert torch.equal(b, eb)
+        assert torch.equal(f, ef)
+        for m, em in zip(ms, ems):
+            assert torch.equal(m.view(torch.int32), em.view(torch.int32))
+
+
+@pytest.mark.parametrize("seed,autotune", [(11, True), (12, False)])
+def test_batch_graph_replays_equal_single_calls(seed, autotune):
+    jobs = make_batch(seed, 10)
+    exp = expected(jobs)
+    bt, outs = build(jobs, autotune)
+    for _ in range(3):
+        clear(outs)
+        torch.cuda.synchronize()
+        bt.run()
+        torch.cuda.synchronize()
+        assert_same(outs, exp)
+    clear(outs)
+    bt.run_eager()
+    torch.cuda.synchronize()
+    assert_same(outs, exp)
+    if autotune:
+        assert "chosen:" in bt.tuning()
+    bt.close()
+
+
+def test_batch_jobs_equal_oracle():
+    jobs = make_batch(13, 4)
+    bt, outs = build(jobs, False)
+    bt.run()
+    torch.cuda.synchronize()
+    for (img, sigma, k, _), (b, f, _) in zip(jobs, outs):
+        host = img.cpu().numpy()
+        assert np.array_equal(b.cpu().numpy(), O.gaussian_blur_f32(host, sigma))
+        assert np.array_equal(f.cpu().numpy(), O.filter_clamped(host, k, 5, 5, np.uint8))
+    bt.close()
+
+
+def test_batch_many_small_and_one_large_image():
+    rng = np.random.default_rng(14)
+    imgs = [torch.from_numpy(rng.integers(0, 256, (int(h), int(w)), dtype=np.uint8)).cuda()
+            for h, w in [(8, 8), (1, 300), (300, 1), (1100, 1500), (5, 7), (130, 129)]]
+    bt = Batch()
+    ks = [kernel5(rng) for _ in imgs]
+    outs = [(bt.add_gaussian_blur(im, 1.7), bt.add_filter_clamped(im, k)) for im, k in zip(imgs, ks)]
+    bt.plan(autotune=False)
+    bt.run()
+    torch.cuda.synchronize()
+    for im, k, (b, f) in zip(imgs, ks, outs):
+        assert torch.equal(b, dev.gaussian_blur_f32(im, 1.7))
+        assert torch.equal(f, dev.filter_clamped(im, Kernel(k, 5, 5), torch.uint8))
+    bt.close()
