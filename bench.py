@@ -18,6 +18,9 @@ batch's Gpix/s. The same JSON line carries:
     with 64x64 and 128x128 templates, BF16x3 MFMA vs FP32 direct, each with
     the max abs / rel error over every output against a float64 reference
     (FFT correlation, integral-image energies) and argmax agreement;
+  * batch: 256 uneven GrayImages with blur, 5x5 filter_clamped and 1-8
+    match_template jobs each, as per-call launches vs one hipGraph replay of
+    the planned batch (bit-identical outputs), with the autotuner's choices;
   * cpu_baseline: the oracle's blur (the reference's loop restated in C,
     single-threaded like the reference's gaussian_blur_f32) on host cores.
 Inputs are synthetic and seeded (SplitMix64).
@@ -41,6 +44,7 @@ INT8_PEAK_TOPS = 5000.0
 BLUR_W, BLUR_H, BLUR_N, BLUR_SIGMA = 1024, 768, 512, 2.0
 MATCH_SIDE, MATCH_T = 2048, 32
 PREC_SIDE, PREC_SIGMA, PREC_TEMPLATES = 4096, 3.0, (64, 128)
+BATCH_IMAGES, BATCH_SIDE_RANGE, BATCH_TPL_RANGE = 256, (256, 2048), (8, 96)
 BF16_PEAK_TFLOPS = 2500.0
 
 
@@ -134,6 +138,61 @@ def ccorr_normed_f64(img, tpl):
     sq[1:, 1:] = (I * I).cumsum(0).cumsum(1)
     ii = sq[th:, tw:] - sq[:-th, tw:] - sq[th:, :-tw] + sq[:-th, :-tw]
     return num / torch.sqrt(ii * (T * T).sum())
+
+
+def splitmix64_u8_cuda(seed: int, n: int):
+    """Top 8 bits of n SplitMix64 outputs, computed on the GPU (int64 with
+    masks in place of logical shifts); equal to uniform_u8 on the host."""
+    import torch
+
+    def lsr(z, k):
+        return (z >> k) & ((1 << (64 - k)) - 1)
+
+    golden = -7046029254386353131  # 0x9E3779B97F4A7C15 as int64
+    c1 = -4658895280553007687      # 0xBF58476D1CE4E5B9
+    c2 = -7723592293110705685      # 0x94D049BB133111EB
+    z = seed + torch.arange(1, n + 1, dtype=torch.int64, device="cuda") * golden
+    z = (z ^ lsr(z, 30)) * c1
+    z = (z ^ lsr(z, 27)) * c2
+    z = z ^ lsr(z, 31)
+    return lsr(z, 56).to(torch.uint8)
+
+
+def batch_workload(seed: int = 4):
+    """256 GrayImages, sides uniform in 256..2048, smooth noise (U[0,1) field,
+    this library's f32 gaussian_blur_f32 at sigma 4, min-max rescaled to u8);
+    per image 1-8 templates of sides 8..96 cropped at seeded places + N(0, 8),
+    a blur sigma in [0.5, 8] and a 5x5 N(0,1) kernel normalised to sum 1
+    (redrawn while |sum| < 0.25). All seeded (SplitMix64 on the GPU for the
+    pixels, numpy's seeded generator for the job parameters)."""
+    import torch
+
+    from imageproc_amd import device as dev
+
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(BATCH_IMAGES):
+        h, w = (int(v) for v in rng.integers(BATCH_SIDE_RANGE[0], BATCH_SIDE_RANGE[1] + 1, 2))
+        u = splitmix64_u8_cuda(seed * 1000003 + i, h * w).view(h, w).float() / 256.0
+        v = dev.gaussian_blur_f32(u.contiguous(), 4.0)
+        lo, hi = v.min(), v.max()
+        img = (255.0 * (v - lo) / (hi - lo)).to(torch.uint8).contiguous()
+        tpls = []
+        for _ in range(int(rng.integers(1, 9))):
+            th, tw = (int(t) for t in rng.integers(BATCH_TPL_RANGE[0], BATCH_TPL_RANGE[1] + 1, 2))
+            y, x = int(rng.integers(0, h - th + 1)), int(rng.integers(0, w - tw + 1))
+            noise = torch.from_numpy(8.0 * rng.standard_normal((th, tw))).cuda()
+            t = torch.clamp(torch.round(img[y:y + th, x:x + tw].double() + noise), 0, 255)
+            tpls.append(t.to(torch.uint8).contiguous())
+        while True:
+            k = rng.standard_normal((5, 5)).astype(np.float32)
+            ks = np.float32(k.sum(dtype=np.float32))
+            if abs(ks) >= 0.25:
+                k = (k / ks).astype(np.float32)
+                break
+        jobs.append((img, float(rng.uniform(0.5, 8.0)), k, tpls))
+    torch.cuda.synchronize()
+    return jobs
 
 
 # ---- measurement helpers -------------------------------------------------------------
@@ -309,6 +368,79 @@ def bench_precision(steps: int):
     return res
 
 
+def bench_batch(steps: int):
+    """The batch workload three ways: per-call launches (one library call per
+    job), the planned batch enqueued eagerly, and one hipGraph replay of it;
+    outputs compared bit for bit between per-call and graph."""
+    import torch
+
+    from imageproc_amd import device as dev
+    from imageproc_amd.batch import Batch
+    from imageproc_amd.kernel import Kernel
+    from imageproc_amd.template_matching import MatchTemplateMethod as M
+
+    print("[bench] batch: generating 256 uneven images", flush=True)
+    jobs = batch_workload()
+    ccn = M.CrossCorrelationNormalized
+    ref = []
+    for img, sigma, k, tpls in jobs:
+        h, w = img.shape
+        ref.append((torch.empty_like(img), torch.empty_like(img),
+                    [torch.empty((h - t.shape[0] + 1, w - t.shape[1] + 1), dtype=torch.float32,
+                                 device="cuda") for t in tpls]))
+    kernels = [Kernel(k, 5, 5) for _, _, k, _ in jobs]
+
+    def per_call():
+        for (img, sigma, k, tpls), (b, f, ms), kk in zip(jobs, ref, kernels):
+            dev.gaussian_blur_f32(img, sigma, out=b)
+            dev.filter_clamped(img, kk, torch.uint8, out=f)
+            for t, m in zip(tpls, ms):
+                dev.match_template(img, t, ccn, "auto", out=m)
+
+    print("[bench] batch: per-call launches", flush=True)
+    t_call, wall_call = timed(per_call, max(2, steps // 2), 1)
+    bt = Batch()
+    outs = []
+    for img, sigma, k, tpls in jobs:
+        outs.append((bt.add_gaussian_blur(img, sigma), bt.add_filter_clamped(img, k),
+                     [bt.add_match_template(img, t, ccn) for t in tpls]))
+    print("[bench] batch: plan + autotune", flush=True)
+    t0 = time.perf_counter()
+    bt.plan(autotune=True)
+    plan_s = time.perf_counter() - t0
+    t_eager, _ = timed(bt.run_eager, steps, 1)
+    t_graph, wall_graph = timed(bt.run, steps, 1)
+    torch.cuda.synchronize()
+    same = all(torch.equal(b, rb) and torch.equal(f, rf) and
+               all(torch.equal(m.view(torch.int32), rm.view(torch.int32)) for m, rm in zip(ms, rms))
+               for (b, f, ms), (rb, rf, rms) in zip(outs, ref))
+    px = sum(img.numel() for img, _, _, _ in jobs)
+    n_tpl = sum(len(t) for _, _, _, t in jobs)
+    ops = sum(2.0 * (img.shape[0] - t.shape[0] + 1) * (img.shape[1] - t.shape[1] + 1) * t.numel()
+              for img, _, _, tpls in jobs for t in tpls)
+    nbytes = 4.0 * px  # blur and filter: one read and one write of every pixel each
+    g_ms = float(np.median(t_graph))
+    ideal_ms = (nbytes / (HBM_PEAK_GBS * 1e9) + ops / (INT8_PEAK_TOPS * 1e12)) * 1e3
+    res = {
+        "workload": f"{BATCH_IMAGES} GrayImages, sides {BATCH_SIDE_RANGE[0]}-{BATCH_SIDE_RANGE[1]}, "
+                    f"{px / 1e6:.1f} Mpix; {n_tpl} templates of sides {BATCH_TPL_RANGE[0]}-"
+                    f"{BATCH_TPL_RANGE[1]} (CrossCorrelationNormalized); blur sigma 0.5-8; "
+                    "5x5 f32 filter_clamped",
+        "per_call_ms": round(float(np.median(t_call)), 3),
+        "batch_eager_ms": round(float(np.median(t_eager)), 3),
+        "graph_ms": round(g_ms, 3),
+        "graph_speedup_vs_per_call": round(float(np.median(t_call)) / g_ms, 2),
+        "outputs_identical": bool(same),
+        "plan_and_autotune_s": round(plan_s, 2),
+        "tuning": bt.tuning().strip().splitlines(),
+        "roofline": {"bound": "hbm+int8", "ideal_ms": round(ideal_ms, 3),
+                     "frac": round(ideal_ms / g_ms, 4),
+                     "work": {"hbm_bytes": nbytes, "int8_ops": ops}},
+    }
+    bt.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -343,6 +475,7 @@ def main():
     achieved = nbytes / (ms * 1e-3) / 1e9
     match = bench_match(max(5, args.steps // 2))
     precision = bench_precision(3)
+    batch = bench_batch(4)
     cpu = None if args.no_cpu else cpu_blur_baseline()
     line = {
         "metric": "blur Gpix/s & %HBM peak; match_template ms/image at equal argmax vs plain GPU",
@@ -366,6 +499,7 @@ def main():
                      "traffic": measured_traffic()},
         "match_template": match,
         "precision_f32": precision,
+        "batch": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
