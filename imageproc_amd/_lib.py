@@ -112,6 +112,38 @@ _SIGS = {
     "iph_batch_run": (C.c_int32, [C.c_void_p, C.c_void_p]),
     "iph_batch_run_eager": (C.c_int32, [C.c_void_p, C.c_void_p]),
     "iph_batch_tuning": (C.c_int32, [C.c_void_p, C.c_char_p, C.c_size_t]),
+    "iph_dev_cnn_conv3x3": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                        C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "iph_dev_cnn_conv3x3_dw_workspace": (C.c_size_t, [C.c_int32]),
+    "iph_dev_cnn_conv3x3_dw": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
+                                           C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p,
+                                           C.c_void_p]),
+    "iph_dev_cnn_flip_weights": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "iph_dev_cnn_relu_backward": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
+                                              C.c_void_p]),
+    "iph_dev_cnn_softmax_xent": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
+                                             C.c_float, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "iph_dev_cnn_adam": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                     C.c_float, C.c_float, C.c_float, C.c_float, C.c_int32,
+                                     C.c_void_p]),
+    "iph_dev_cnn_to_bf16": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    "iph_dev_downsample4_u8": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
+                                           C.c_void_p]),
+    "iph_dev_proposal_candidates_workspace": (C.c_size_t, [C.c_uint32, C.c_uint32]),
+    "iph_dev_proposal_candidates": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32,
+                                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                                C.c_void_p]),
+    "iph_dev_proposal_features": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
+                                              C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
+                                              C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    "iph_dev_proposal_top": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_uint32,
+                                         C.c_uint32, C.c_int32, C.c_void_p, C.c_void_p,
+                                         C.c_void_p]),
+    "iph_dev_proposal_refine_workspace": (C.c_size_t, [C.c_int32]),
+    "iph_dev_proposal_refine": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
+                                            C.c_uint32, C.c_uint32, C.c_int32, C.c_void_p,
+                                            C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
+                                            C.c_void_p, C.c_size_t, C.c_void_p]),
     "iph_dev_find_extremes_workspace": (C.c_size_t, []),
     "iph_dev_find_extremes": (C.c_int32, [C.c_void_p, C.c_int32, C.c_uint32, C.c_uint32,
                                           C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),

@@ -147,22 +147,6 @@ __global__ __launch_bounds__(256) void window_sums_u8_kernel(const uint8_t *__re
     }
 }
 
-template <int METHOD>
-__device__ __forceinline__ float finish_score(long long c, long long sii, long long stt,
-                                              long long n) {
-    const bool sse = (METHOD == IPH_SSE || METHOD == IPH_SSE_NORMED);
-    long long num = sse ? (stt - 2 * c + sii) : c;
-    if (METHOD == IPH_SSE || METHOD == IPH_CCORR) return (float)num;
-    if (n < 64 * 64) {
-        // the reference's normalisation in f32 (template_matching.rs:249-250)
-        float norm = sqrtf((float)sii * (float)stt);
-        float s = (float)num;
-        return norm > 0.0f ? s / norm : s;
-    }
-    double norm = sqrt((double)sii * (double)stt);
-    return norm > 0.0 ? (float)((double)num / norm) : (float)num;
-}
-
 // One wave computes a 32 x (32*kXT) output strip; RT waves stack vertically.
 template <int KB, int METHOD>
 __global__ __launch_bounds__(256) void match_i8_mfma_kernel(

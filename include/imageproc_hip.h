@@ -222,6 +222,85 @@ IPH_API iph_status iph_batch_run_eager(iph_batch *batch, void *stream);
 /* The autotuner's measurements and choices, as text. */
 IPH_API iph_status iph_batch_tuning(iph_batch *batch, char *buf, size_t cap);
 
+/*
+ * Learned match proposal: no counterpart in the reference, whose only search
+ * is the full score map of match_template (template_matching.rs:78-92) read
+ * by find_extremes (template_matching.rs:479-511). These are the building
+ * blocks of a cheaper search that ends in the same argmax: a 4x-downsampled
+ * coarse map, its best tiles, a small convolutional network that ranks
+ * candidate places, and exact scores in windows around the best of them
+ * (values identical to the full map's at the same places). All pointers are
+ * device memory; everything is enqueued on `stream`.
+ *
+ * Network tensors: activations and gradients are bf16 (uint16_t bit
+ * patterns), NHWC, IPH_CNN_PATCH x IPH_CNN_PATCH pixels by IPH_CNN_CHANNELS
+ * channels per patch; a layer's weights are bf16
+ * w[co][(ky * 3 + kx) * IPH_CNN_CHANNELS + ci], its bias f32[channels].
+ */
+#define IPH_CNN_CHANNELS 32
+#define IPH_CNN_PATCH 16
+/* y = conv3x3(x, w) + bias (zero padding) over n_patches patches; z (f32,
+ * may be NULL) gets y, a (bf16, may be NULL) relu(y) if relu else y. */
+IPH_API iph_status iph_dev_cnn_conv3x3(const uint16_t *x, const uint16_t *w, const float *bias,
+                                       int32_t relu, int32_t n_patches, float *z, uint16_t *a,
+                                       void *stream);
+/* Weight and bias gradients of that layer from its input x and output
+ * gradient dz (f32 dw[co][k], db[co]), summed over n_partials workgroups. */
+IPH_API size_t iph_dev_cnn_conv3x3_dw_workspace(int32_t n_partials);
+IPH_API iph_status iph_dev_cnn_conv3x3_dw(const uint16_t *x, const uint16_t *dz,
+                                          int32_t n_patches, int32_t n_partials, void *workspace,
+                                          size_t workspace_bytes, float *dw, float *db,
+                                          void *stream);
+/* Weights of the layer whose forward pass is the input gradient of w. */
+IPH_API iph_status iph_dev_cnn_flip_weights(const uint16_t *w, uint16_t *w_flipped, void *stream);
+/* dz = bf16(dx * (z > 0)) over n elements. */
+IPH_API iph_status iph_dev_cnn_relu_backward(const float *dx, const float *z, size_t n,
+                                             uint16_t *dz, void *stream);
+/* Softmax cross-entropy of each pair over the channel-0 logits of its
+ * patches_per_pair patches; target[pair] = patch * 256 + pixel or -1. dz_last
+ * gets (softmax - onehot) * scale in channel 0; loss[pair] the loss. */
+IPH_API iph_status iph_dev_cnn_softmax_xent(const float *z_last, int32_t n_pairs,
+                                            int32_t patches_per_pair, const int32_t *target,
+                                            float scale, float *loss, uint16_t *dz_last,
+                                            void *stream);
+/* One Adam step (step >= 1) on n f32 parameters. */
+IPH_API iph_status iph_dev_cnn_adam(float *param, const float *grad, float *m, float *v,
+                                    int32_t n, float lr, float beta1, float beta2, float eps,
+                                    int32_t step, void *stream);
+IPH_API iph_status iph_dev_cnn_to_bf16(const float *src, int32_t n, uint16_t *dst, void *stream);
+/* dst (w/4) x (h/4): rounded mean of each full 4x4 block. */
+IPH_API iph_status iph_dev_downsample4_u8(const uint8_t *src, uint32_t width, uint32_t height,
+                                          uint8_t *dst, void *stream);
+/* The k best 16x16-tile maxima of a coarse map (larger first; ties: lower
+ * index; NaN last): cand[2i], cand[2i+1] = x, y; cand_score[i]. */
+IPH_API size_t iph_dev_proposal_candidates_workspace(uint32_t cw, uint32_t ch);
+IPH_API iph_status iph_dev_proposal_candidates(const float *coarse, uint32_t cw, uint32_t ch,
+                                               int32_t k, int32_t *cand, float *cand_score,
+                                               void *workspace, size_t workspace_bytes,
+                                               void *stream);
+/* Network input: one 16x16 patch per candidate, centred on it. */
+IPH_API iph_status iph_dev_proposal_features(const float *coarse, uint32_t cw, uint32_t ch,
+                                             const uint8_t *small, uint32_t sw, uint32_t sh,
+                                             uint32_t small_tw, const int32_t *cand,
+                                             const float *cand_score, int32_t k,
+                                             uint16_t *patches, void *stream);
+/* The n_top best logits at distinct coarse places (centers: x, y) and their
+ * softmax probabilities over all k * 256 logits. */
+IPH_API iph_status iph_dev_proposal_top(const float *z_last, int32_t k, const int32_t *cand,
+                                        uint32_t cw, uint32_t ch, int32_t n_top, int32_t *centers,
+                                        float *probs, void *stream);
+/* Exact match_template scores (the full map's values) at every place within
+ * 8 pixels of each centre * scale, then result[0..3] = x, y of the best
+ * (find_extremes' order), its score, and the best score more than `sep`
+ * pixels from it. For the correlation methods (larger is better). */
+IPH_API size_t iph_dev_proposal_refine_workspace(int32_t n_centers);
+IPH_API iph_status iph_dev_proposal_refine(const uint8_t *image, uint32_t width, uint32_t height,
+                                           const uint8_t *tmpl, uint32_t tw, uint32_t th,
+                                           int32_t method, const int32_t *centers,
+                                           int32_t n_centers, int32_t scale, int32_t sep,
+                                           float *result, void *workspace, size_t workspace_bytes,
+                                           void *stream);
+
 #ifdef __cplusplus
 }
 #endif
