@@ -21,6 +21,10 @@ batch's Gpix/s. The same JSON line carries:
   * batch: 256 uneven GrayImages with blur, 5x5 filter_clamped and 1-8
     match_template jobs each, as per-call launches vs one hipGraph replay of
     the planned batch (bit-identical outputs), with the autotuner's choices;
+  * learned_proposal: 8192x8192 images with perturbed 48x48 templates, the
+    full-map search vs the proposal search (trained network + exact windows,
+    full-map fallback on low margin): ms per search, speedup and argmax
+    agreement, and what the network adds over ranking by coarse score alone;
   * cpu_baseline: the oracle's blur (the reference's loop restated in C,
     single-threaded like the reference's gaussian_blur_f32) on host cores.
 Inputs are synthetic and seeded (SplitMix64).
@@ -441,6 +445,68 @@ def bench_batch(steps: int):
     return res
 
 
+def bench_proposal(n_images: int = 12, per_image: int = 4):
+    """Learned proposal on 8192x8192 smooth noise with perturbed 48x48
+    templates (noise sigma in {0, 8, 16}, brightness x U(0.8, 1.2), 10 %
+    occlusion): the plain search (full CrossCorrelationNormalized map +
+    find_extremes) against the proposal search with the trained network, and
+    the same search ranking candidates by coarse score alone. Times are wall
+    clock per search including the read-back of the answer."""
+    import torch
+
+    from imageproc_amd.proposal import ProposalNet, ProposalSearch, WEIGHTS
+    from imageproc_amd.proposal_train import CCN, pair_stream
+
+    print("[bench] proposal: generating 8192x8192 pairs", flush=True)
+    with np.load(WEIGHTS, allow_pickle=False) as d:
+        margin = float(d["margin"])
+        trained = int(d["train_pairs"])
+    net = ProposalNet.load()
+    srch = ProposalSearch(net, CCN, margin=margin)
+    top1 = ProposalSearch(net, CCN, n_top=1, margin=-1e30, min_prob=-1.0)
+    stream = pair_stream(5, 8192, per_image=per_image)
+    rec = []
+    for _ in range(n_images * per_image):
+        img, tpl, _ = next(stream)
+        for _warm in range(1 if not rec else 0):
+            srch.full_map(img, tpl)
+            srch.search(img, tpl)
+            srch.search(img, tpl, use_net=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g = srch.full_map(img, tpl)
+        t1 = time.perf_counter()
+        pn = srch.search(img, tpl)
+        t2 = time.perf_counter()
+        pc = srch.search(img, tpl, use_net=False)
+        t3 = time.perf_counter()
+        n1 = top1.search(img, tpl)[:2]
+        c1 = top1.search(img, tpl, use_net=False)[:2]
+        rec.append((g[:2], pn, pc, n1, c1, t1 - t0, t2 - t1, t3 - t2))
+    n = len(rec)
+    full_ms = float(np.median([r[5] for r in rec])) * 1e3
+    prop_ms = float(np.median([r[6] for r in rec])) * 1e3
+    coarse_ms = float(np.median([r[7] for r in rec])) * 1e3
+    ops = 2.0 * (8192 - 47) ** 2 * 48 * 48
+    return {
+        "workload": f"{n} pairs: 8192x8192 u8 smooth noise (sigma 4), 48x48 templates with noise "
+                    "sigma in {0,8,16}, brightness x U(0.8,1.2), 10% occlusion; "
+                    "CrossCorrelationNormalized",
+        "network": f"4 layers 3x3, 32 channels, bf16; trained on {trained} pairs (2048x2048 images)",
+        "full_map_ms": round(full_ms, 3),
+        "full_map_int8_tops": round(ops / (full_ms * 1e-3) / 1e12, 1),
+        "proposal_ms": round(prop_ms, 3),
+        "speedup_vs_full_map": round(full_ms / prop_ms, 2),
+        "argmax_agreement": sum(r[1][:2] == r[0] for r in rec) / n,
+        "fallbacks": sum(1 for r in rec if r[1][3]) / n,
+        "coarse_only_ms": round(coarse_ms, 3),
+        "coarse_only_agreement": sum(r[2][:2] == r[0] for r in rec) / n,
+        "top1_agreement_network": sum(r[3] == r[0] for r in rec) / n,
+        "top1_agreement_coarse_only": sum(r[4] == r[0] for r in rec) / n,
+        "fallback_margin": margin,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -476,6 +542,7 @@ def main():
     match = bench_match(max(5, args.steps // 2))
     precision = bench_precision(3)
     batch = bench_batch(4)
+    proposal = bench_proposal()
     cpu = None if args.no_cpu else cpu_blur_baseline()
     line = {
         "metric": "blur Gpix/s & %HBM peak; match_template ms/image at equal argmax vs plain GPU",
@@ -500,6 +567,7 @@ def main():
         "match_template": match,
         "precision_f32": precision,
         "batch": batch,
+        "learned_proposal": proposal,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -136,14 +136,22 @@ _SIGS = {
     "iph_dev_proposal_features": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
                                               C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
                                               C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    "iph_dev_proposal_top_workspace": (C.c_size_t, [C.c_int32, C.c_int32]),
     "iph_dev_proposal_top": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_uint32,
                                          C.c_uint32, C.c_int32, C.c_void_p, C.c_void_p,
-                                         C.c_void_p]),
+                                         C.c_void_p, C.c_size_t, C.c_void_p]),
     "iph_dev_proposal_refine_workspace": (C.c_size_t, [C.c_int32]),
     "iph_dev_proposal_refine": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
                                             C.c_uint32, C.c_uint32, C.c_int32, C.c_void_p,
                                             C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
                                             C.c_void_p, C.c_size_t, C.c_void_p]),
+    "iph_dev_proposal_search_workspace": (C.c_size_t, [C.c_uint32, C.c_uint32, C.c_uint32,
+                                                       C.c_uint32, C.c_int32, C.c_int32,
+                                                       C.c_int32]),
+    "iph_dev_proposal_search": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
+                                            C.c_uint32, C.c_uint32, C.c_int32, C.c_void_p,
+                                            C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
+                                            C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "iph_dev_find_extremes_workspace": (C.c_size_t, []),
     "iph_dev_find_extremes": (C.c_int32, [C.c_void_p, C.c_int32, C.c_uint32, C.c_uint32,
                                           C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),

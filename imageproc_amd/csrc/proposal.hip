@@ -48,6 +48,40 @@ __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float(
 
 // ---- network layers ------------------------------------------------------------
 
+// Layer l's weights w[co][k] into LDS rows of kWPitch elements.
+__device__ __forceinline__ void stage_weights(uint16_t *ws, const uint16_t *__restrict__ w, int tid) {
+    for (int i = tid; i < kC * (kKK / 8); i += 256) {
+        const int r = i / (kKK / 8), q = i - r * (kKK / 8);
+        *reinterpret_cast<uint4 *>(ws + r * kWPitch + 8 * q) =
+            *reinterpret_cast<const uint4 *>(w + r * kKK + 8 * q);
+    }
+}
+
+// One 32-pixel x 32-channel tile (pixels 32 mt .. 32 mt + 31) of a 3x3
+// layer from a halo-padded input patch xs and staged weights ws: A row m =
+// lane n is the tile's pixel n (im2col taken straight from LDS), B column n
+// the output channel n, 18 MFMAs of K = 16 over the 9 taps x 32 inputs.
+__device__ __forceinline__ v16f conv_tile(const uint16_t *xs, const uint16_t *ws, int mt, int n,
+                                          int g) {
+    const int y = 2 * mt + (n >> 4), xq = n & 15;
+    v16f acc = {};
+#pragma unroll
+    for (int s = 0; s < 2 * 9; s++) {
+        const int tap = s >> 1, ky = tap / 3, kx = tap - 3 * ky, ci0 = 16 * (s & 1) + 8 * g;
+        const v8bf av =
+            *reinterpret_cast<const v8bf *>(xs + ((y + ky) * kXP + xq + kx) * kXPitch + ci0);
+        const v8bf bv = *reinterpret_cast<const v8bf *>(ws + n * kWPitch + 16 * s + 8 * g);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// D of conv_tile: register r of lane (n, g) holds output channel n of pixel
+// 32 mt + 8 (r / 4) + 4 g + r % 4.
+__device__ __forceinline__ int tile_pixel(int mt, int g, int r) {
+    return 32 * mt + 8 * (r >> 2) + 4 * g + (r & 3);
+}
+
 // y = conv3x3(x) + bias over one 16x16 patch per workgroup (zero padding).
 // Weights w[co][(ky*3 + kx)*32 + ci]. z (f32, optional) gets y, a (bf16,
 // optional) relu(y) or y.
@@ -59,11 +93,7 @@ __global__ __launch_bounds__(256) void cnn_conv3x3_kernel(const uint16_t *__rest
     __shared__ __align__(16) uint16_t xs[kXP * kXP * kXPitch];
     __shared__ __align__(16) uint16_t ws[kC * kWPitch];
     const int p = blockIdx.x, tid = threadIdx.x;
-    for (int i = tid; i < kC * (kKK / 8); i += 256) {
-        const int r = i / (kKK / 8), q = i - r * (kKK / 8);
-        *reinterpret_cast<uint4 *>(ws + r * kWPitch + 8 * q) =
-            *reinterpret_cast<const uint4 *>(w + r * kKK + 8 * q);
-    }
+    stage_weights(ws, w, tid);
     const uint16_t *xp = x + (size_t)p * kPix * kC;
     for (int i = tid; i < kXP * kXP * 4; i += 256) {
         const int pix = i >> 2, q = i & 3;
@@ -77,21 +107,10 @@ __global__ __launch_bounds__(256) void cnn_conv3x3_kernel(const uint16_t *__rest
     const int lane = tid & 63, wave = tid >> 6, n = lane & 31, g = lane >> 5;
     const float b = bias ? bias[n] : 0.0f;
     for (int mt = 2 * wave; mt < 2 * wave + 2; mt++) {
-        // A row m = n: pixel 32*mt + n; B column n: output channel n
-        const int y = 2 * mt + (n >> 4), xq = n & 15;
-        v16f acc = {};
-#pragma unroll
-        for (int s = 0; s < 2 * 9; s++) {
-            const int tap = s >> 1, ky = tap / 3, kx = tap - 3 * ky, ci0 = 16 * (s & 1) + 8 * g;
-            const v8bf av =
-                *reinterpret_cast<const v8bf *>(xs + ((y + ky) * kXP + xq + kx) * kXPitch + ci0);
-            const v8bf bv = *reinterpret_cast<const v8bf *>(ws + n * kWPitch + 16 * s + 8 * g);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
-        }
-        // D: column n = output channel, rows 8*(r/4) + 4*g + r%4 = pixels of the tile
+        const v16f acc = conv_tile(xs, ws, mt, n, g);
 #pragma unroll
         for (int r = 0; r < 16; r++) {
-            const int pix = 32 * mt + 8 * (r >> 2) + 4 * g + (r & 3);
+            const int pix = tile_pixel(mt, g, r);
             const float v = acc[r] + b;
             const size_t o = ((size_t)p * kPix + pix) * kC + n;
             if (z) z[o] = v;
@@ -269,110 +288,161 @@ __global__ __launch_bounds__(256) void cnn_to_bf16_kernel(const float *__restric
 // ---- search pipeline ---------------------------------------------------------------
 
 // dst[y][x] = rounded mean of the 4x4 block at (4x, 4y) (full blocks only).
+// A thread makes 4 outputs of one row from 16 bytes of each of 4 source
+// rows (one 16-byte load each when `vec`: 16-byte aligned rows).
 __global__ __launch_bounds__(256) void downsample4_kernel(const uint8_t *__restrict__ src, int W,
-                                                          int dw, int dh,
+                                                          int dw, int dh, int vec,
                                                           uint8_t *__restrict__ dst) {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (x >= dw || y >= dh) return;
-    uint32_t s = 0;
+    const int x0 = 4 * (blockIdx.x * 256 + threadIdx.x), y = blockIdx.y;
+    if (x0 >= dw || y >= dh) return;
+    uint32_t s[4] = {0u, 0u, 0u, 0u};
+    if (vec && x0 + 4 <= dw) {
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-        const uint8_t *row = src + (size_t)(4 * y + r) * W + 4 * x;
-        s += row[0] + row[1] + row[2] + row[3];
+        for (int r = 0; r < 4; r++) {
+            const uint4 v =
+                *reinterpret_cast<const uint4 *>(src + (size_t)(4 * y + r) * W + 4 * x0);
+            s[0] = __builtin_amdgcn_udot4(v.x, 0x01010101u, s[0], false);
+            s[1] = __builtin_amdgcn_udot4(v.y, 0x01010101u, s[1], false);
+            s[2] = __builtin_amdgcn_udot4(v.z, 0x01010101u, s[2], false);
+            s[3] = __builtin_amdgcn_udot4(v.w, 0x01010101u, s[3], false);
+        }
+    } else {
+        for (int j = 0; j < 4 && x0 + j < dw; j++)
+            for (int r = 0; r < 4; r++) {
+                const uint8_t *row = src + (size_t)(4 * y + r) * W + 4 * (x0 + j);
+                s[j] += row[0] + row[1] + row[2] + row[3];
+            }
     }
-    dst[(size_t)y * dw + x] = (uint8_t)((s + 8) >> 4);
+    uint8_t *o = dst + (size_t)y * dw + x0;
+    if (vec && x0 + 4 <= dw && (dw & 3) == 0) {
+        *reinterpret_cast<uint32_t *>(o) = ((s[0] + 8) >> 4) | (((s[1] + 8) >> 4) << 8) |
+                                           (((s[2] + 8) >> 4) << 16) | (((s[3] + 8) >> 4) << 24);
+    } else {
+        for (int j = 0; j < 4 && x0 + j < dw; j++) o[j] = (uint8_t)((s[j] + 8) >> 4);
+    }
 }
 
-// Larger score first; NaN counts as the lowest; equal scores: lower index.
+// Order of candidate entries (score, index): larger score first; NaN after
+// every number; equal scores: lower index. Index kNoIdx marks an empty slot,
+// after everything else.
+constexpr int kNoIdx = 0x7fffffff;
 __device__ __forceinline__ bool better(float a, int ia, float b, int ib) {
+    if (ia == kNoIdx || ib == kNoIdx) return ib == kNoIdx && ia != kNoIdx;
     const bool an = a != a, bn = b != b;
     if (an != bn) return bn;
     if (!an && a != b) return a > b;
     return ia < ib;
 }
 
-// One 16x16 tile of the coarse map per workgroup: its best value and index.
+// Best value and index of each 16x16 tile of the coarse map. A workgroup
+// covers a band of 16 map rows by 256 columns (16 tiles): each lane folds one
+// column of the band, then each group of 16 lanes folds into its tile.
 __global__ __launch_bounds__(256) void tile_max_kernel(const float *__restrict__ map, int cw, int ch,
                                                        int tiles_x, float *__restrict__ tscore,
                                                        int *__restrict__ tidx) {
-    __shared__ float sv[256];
-    __shared__ int si[256];
-    const int t = blockIdx.x, tx = t % tiles_x, ty = t / tiles_x;
-    const int x = tx * 16 + (threadIdx.x & 15), y = ty * 16 + (threadIdx.x >> 4);
+    const int x = blockIdx.x * 256 + threadIdx.x, ty = blockIdx.y;
     float v = -INFINITY;
-    int idx = 0x7fffffff;
-    if (x < cw && y < ch) {
-        v = map[(size_t)y * cw + x];
-        idx = y * cw + x;
-    }
-    sv[threadIdx.x] = v;
-    si[threadIdx.x] = idx;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o && better(sv[threadIdx.x + o], si[threadIdx.x + o], sv[threadIdx.x],
-                                      si[threadIdx.x])) {
-            sv[threadIdx.x] = sv[threadIdx.x + o];
-            si[threadIdx.x] = si[threadIdx.x + o];
+    int idx = kNoIdx;
+    if (x < cw) {
+        const int y1 = min(16 * ty + 16, ch);
+        for (int y = 16 * ty; y < y1; y++) {
+            const float u = map[(size_t)y * cw + x];
+            if (better(u, y * cw + x, v, idx)) {
+                v = u;
+                idx = y * cw + x;
+            }
         }
-        __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        tscore[t] = sv[0];
-        tidx[t] = si[0];
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+        const float u = __shfl_xor(v, o);
+        const int iu = __shfl_xor(idx, o);
+        if (better(u, iu, v, idx)) {
+            v = u;
+            idx = iu;
+        }
+    }
+    const int tx = x >> 4;
+    if ((threadIdx.x & 15) == 0 && tx < tiles_x) {
+        tscore[ty * tiles_x + tx] = v;
+        tidx[ty * tiles_x + tx] = idx;
     }
 }
 
-// The k best tile maxima (bitonic sort of up to 16384 entries in LDS).
-constexpr int kMaxTiles = 16384;
-__global__ __launch_bounds__(1024) void topk_tiles_kernel(const float *__restrict__ tscore,
-                                                          const int *__restrict__ tidx, int n,
-                                                          int k, int cw,
-                                                          int *__restrict__ cand,
-                                                          float *__restrict__ cscore) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    float *sv = reinterpret_cast<float *>(smem);
-    int *si = reinterpret_cast<int *>(sv + kMaxTiles);
-    int np = 1;
-    while (np < n) np <<= 1;
-    for (int i = threadIdx.x; i < np; i += 1024) {
-        sv[i] = i < n ? tscore[i] : -INFINITY;
-        si[i] = i < n ? tidx[i] : 0x7fffffff;
+// Sorts chunk blockIdx.x (kChunk entries) of n (score, index) entries best
+// first, bitonically in LDS, and keeps its first `keep`: at
+// out[blockIdx.x * keep] or, with cand set (a single chunk), as the
+// candidate list cand = (index % cw, index / cw), cscore = score, with (0, 0)
+// and -inf for empty slots.
+constexpr int kChunk = 1024;
+constexpr int kMaxCand = 256;
+__global__ __launch_bounds__(512) void topk_chunk_kernel(const float *__restrict__ sv_in,
+                                                         const int *__restrict__ si_in, int n,
+                                                         int keep, float *__restrict__ sv_out,
+                                                         int *__restrict__ si_out, int cw,
+                                                         int *__restrict__ cand,
+                                                         float *__restrict__ cscore) {
+    __shared__ float sv[kChunk];
+    __shared__ int si[kChunk];
+    const int base = blockIdx.x * kChunk, tid = threadIdx.x;
+    for (int i = tid; i < kChunk; i += 512) {
+        const int g = base + i;
+        sv[i] = g < n ? sv_in[g] : -INFINITY;
+        si[i] = g < n ? si_in[g] : kNoIdx;
     }
     __syncthreads();
-    for (int size = 2; size <= np; size <<= 1) {
+    for (int size = 2; size <= kChunk; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int i = threadIdx.x; i < np; i += 1024) {
-                const int j = i ^ stride;
-                if (j > i) {
-                    const bool desc = (i & size) == 0;
-                    const bool swap = desc ? better(sv[j], si[j], sv[i], si[i])
-                                           : better(sv[i], si[i], sv[j], si[j]);
-                    if (swap) {
-                        float tv = sv[i];
-                        sv[i] = sv[j];
-                        sv[j] = tv;
-                        int ti = si[i];
-                        si[i] = si[j];
-                        si[j] = ti;
-                    }
-                }
+            const int i = 2 * tid - (tid & (stride - 1)), j = i + stride;
+            const float a = sv[i], b = sv[j];
+            const int ia = si[i], ib = si[j];
+            const bool desc = (i & size) == 0;
+            if (desc ? better(b, ib, a, ia) : better(a, ia, b, ib)) {
+                sv[i] = b;
+                sv[j] = a;
+                si[i] = ib;
+                si[j] = ia;
             }
             __syncthreads();
         }
     }
-    for (int i = threadIdx.x; i < k; i += 1024) {
-        const int id = i < n ? si[i] : 0;
-        cand[2 * i] = id % cw;
-        cand[2 * i + 1] = id / cw;
-        cscore[i] = i < n ? sv[i] : -INFINITY;
+    for (int i = tid; i < keep; i += 512) {
+        if (cand) {
+            const int id = si[i];
+            cand[2 * i] = id == kNoIdx ? 0 : id % cw;
+            cand[2 * i + 1] = id == kNoIdx ? 0 : id / cw;
+            cscore[i] = id == kNoIdx ? -INFINITY : sv[i];
+        } else {
+            sv_out[(size_t)blockIdx.x * keep + i] = sv[i];
+            si_out[(size_t)blockIdx.x * keep + i] = si[i];
+        }
     }
 }
 
-// Feature patch of candidate c (one workgroup): 16x16 around its coarse
-// place, channels: 0 the coarse score minus the candidate's, x100; 1 the
-// candidate's score minus the best candidate's, x100 (the same everywhere);
-// 2 the small image's pixel at the window centre, / 255 - 0.5; 3 one inside
-// the coarse map; the rest zero.
+// Network input channels 0-3 of pixel pix of the patch around candidate
+// (cx, cy), packed as bf16 pairs: 0 the coarse score minus the candidate's,
+// x100; 1 the candidate's score minus the best candidate's, x100 (the same
+// everywhere); 2 the small image's pixel at the window centre, / 255 - 0.5;
+// 3 one inside the coarse map. Outside the map only channel 1 is set.
+__device__ __forceinline__ uint2 feature_pixel(const float *__restrict__ map, int cw, int ch,
+                                               const uint8_t *__restrict__ small, int sw, int sh,
+                                               int half_t, int cx, int cy, float peak, float best,
+                                               int pix) {
+    const int x = cx - 8 + (pix & 15), y = cy - 8 + (pix >> 4);
+    float f[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (x >= 0 && x < cw && y >= 0 && y < ch) {
+        const float v = map[(size_t)y * cw + x];
+        f[0] = (v == v && peak == peak) ? (v - peak) * 100.0f : -100.0f;
+        const int sx = min(x + half_t, sw - 1), sy = min(y + half_t, sh - 1);
+        f[2] = small[(size_t)sy * sw + sx] * (1.0f / 255.0f) - 0.5f;
+        f[3] = 1.0f;
+    }
+    f[1] = (peak == peak && best == best) ? (peak - best) * 100.0f : -100.0f;
+    return make_uint2((uint32_t)to_bf16(f[0]) | ((uint32_t)to_bf16(f[1]) << 16),
+                      (uint32_t)to_bf16(f[2]) | ((uint32_t)to_bf16(f[3]) << 16));
+}
+
+// Feature patch of candidate c (one workgroup), NHWC bf16, channels 4-31 zero.
 __global__ __launch_bounds__(256) void features_kernel(const float *__restrict__ map, int cw, int ch,
                                                        const uint8_t *__restrict__ small, int sw,
                                                        int sh, int half_t,
@@ -380,239 +450,401 @@ __global__ __launch_bounds__(256) void features_kernel(const float *__restrict__
                                                        const float *__restrict__ cscore,
                                                        uint16_t *__restrict__ patches) {
     const int c = blockIdx.x, pix = threadIdx.x;
-    const int px = pix & 15, py = pix >> 4;
-    const int x = cand[2 * c] - 8 + px, y = cand[2 * c + 1] - 8 + py;
-    const float peak = cscore[c], best = cscore[0];
-    const bool in = x >= 0 && x < cw && y >= 0 && y < ch;
-    float f[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    if (in) {
-        const float v = map[(size_t)y * cw + x];
-        f[0] = (v == v && peak == peak) ? (v - peak) * 100.0f : -100.0f;
-        const int cx = min(x + half_t, sw - 1), cy = min(y + half_t, sh - 1);
-        f[2] = small[(size_t)cy * sw + cx] * (1.0f / 255.0f) - 0.5f;
-        f[3] = 1.0f;
-    }
-    f[1] = (peak == peak && best == best) ? (peak - best) * 100.0f : -100.0f;
+    const uint2 f = feature_pixel(map, cw, ch, small, sw, sh, half_t, cand[2 * c],
+                                  cand[2 * c + 1], cscore[c], cscore[0], pix);
     uint16_t *o = patches + ((size_t)c * kPix + pix) * kC;
-    uint32_t w0 = (uint32_t)to_bf16(f[0]) | ((uint32_t)to_bf16(f[1]) << 16);
-    uint32_t w1 = (uint32_t)to_bf16(f[2]) | ((uint32_t)to_bf16(f[3]) << 16);
-    reinterpret_cast<uint4 *>(o)[0] = make_uint4(w0, w1, 0u, 0u);
+    reinterpret_cast<uint4 *>(o)[0] = make_uint4(f.x, f.y, 0u, 0u);
     for (int q = 1; q < 4; q++) reinterpret_cast<uint4 *>(o)[q] = make_uint4(0u, 0u, 0u, 0u);
 }
 
-// The n_top best logits (channel 0 of the last layer) at distinct coarse
-// places inside the map; probs get their softmax probabilities over all k*256.
-__global__ __launch_bounds__(256) void top_logits_kernel(const float *__restrict__ zl, int k,
-                                                         const int *__restrict__ cand, int cw,
-                                                         int ch, int n_top,
-                                                         int *__restrict__ centers,
-                                                         float *__restrict__ probs) {
-    __shared__ float sv[256];
-    __shared__ int si[256];
-    __shared__ int taken[2 * 64];
-    __shared__ float norm[2];
-    const int n = k * kPix, tid = threadIdx.x;
-    auto place = [&](int i, int &x, int &y) {
-        const int c = i / kPix, pix = i - c * kPix;
-        x = cand[2 * c] - 8 + (pix & 15);
-        y = cand[2 * c + 1] - 8 + (pix >> 4);
-    };
-    // softmax normaliser over every logit
-    float mx = -INFINITY;
-    for (int i = tid; i < n; i += 256) mx = fmaxf(mx, zl[(size_t)i * kC]);
-    sv[tid] = mx;
+// The whole network on candidate c's patch (one workgroup): features built
+// in LDS, every layer's activations kept in LDS, logits[c * 256 + pixel] =
+// the last layer's channel 0. The same arithmetic as features_kernel and one
+// cnn_conv3x3_kernel per layer (ReLU between layers), so the logits are
+// identical to theirs. w: `layers` weight blocks of 32 x 288 bf16, bias:
+// `layers` x 32 f32.
+__global__ __launch_bounds__(256) void cnn_proposal_forward_kernel(
+    const float *__restrict__ map, int cw, int ch, const uint8_t *__restrict__ small, int sw,
+    int sh, int half_t, const int *__restrict__ cand, const float *__restrict__ cscore,
+    const uint16_t *__restrict__ w, const float *__restrict__ bias, int layers,
+    float *__restrict__ logits) {
+    __shared__ __align__(16) uint16_t xs[kXP * kXP * kXPitch];
+    __shared__ __align__(16) uint16_t ws[kC * kWPitch];
+    const int c = blockIdx.x, tid = threadIdx.x;
+    for (int i = tid; i < kXP * kXP * kXPitch / 8; i += 256)
+        reinterpret_cast<uint4 *>(xs)[i] = make_uint4(0u, 0u, 0u, 0u);
+    stage_weights(ws, w, tid);
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (tid < o) sv[tid] = fmaxf(sv[tid], sv[tid + o]);
-        __syncthreads();
-    }
-    mx = sv[0];
-    __syncthreads();
-    float se = 0.0f;
-    for (int i = tid; i < n; i += 256) se += expf(zl[(size_t)i * kC] - mx);
-    sv[tid] = se;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (tid < o) sv[tid] += sv[tid + o];
-        __syncthreads();
-    }
-    if (tid == 0) {
-        norm[0] = mx;
-        norm[1] = sv[0];
+    {
+        const uint2 f = feature_pixel(map, cw, ch, small, sw, sh, half_t, cand[2 * c],
+                                      cand[2 * c + 1], cscore[c], cscore[0], tid);
+        const int px = tid & 15, py = tid >> 4;
+        *reinterpret_cast<uint2 *>(xs + ((py + 1) * kXP + px + 1) * kXPitch) = f;
     }
     __syncthreads();
-    for (int r = 0; r < n_top; r++) {
-        float bv = -INFINITY;
-        int bi = 0x7fffffff;
-        for (int i = tid; i < n; i += 256) {
-            int x, y;
-            place(i, x, y);
-            if (x < 0 || x >= cw || y < 0 || y >= ch) continue;
-            bool dup = false;
-            for (int q = 0; q < r; q++) dup |= (taken[2 * q] == x && taken[2 * q + 1] == y);
-            const float v = zl[(size_t)i * kC];
-            if (!dup && better(v, i, bv, bi)) {
-                bv = v;
-                bi = i;
-            }
-        }
-        sv[tid] = bv;
-        si[tid] = bi;
-        __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
-            if (tid < o && better(sv[tid + o], si[tid + o], sv[tid], si[tid])) {
-                sv[tid] = sv[tid + o];
-                si[tid] = si[tid + o];
-            }
-            __syncthreads();
-        }
-        if (tid == 0) {
-            int x = 0, y = 0;
-            if (si[0] != 0x7fffffff) place(si[0], x, y);
-            taken[2 * r] = x;
-            taken[2 * r + 1] = y;
-            centers[2 * r] = x;
-            centers[2 * r + 1] = y;
-            probs[r] = si[0] != 0x7fffffff ? expf(sv[0] - norm[0]) / norm[1] : 0.0f;
+    const int lane = tid & 63, wave = tid >> 6, n = lane & 31, g = lane >> 5;
+    for (int l = 0; l < layers; l++) {
+        const float b = bias[l * kC + n];
+        v16f acc[2];
+#pragma unroll
+        for (int t = 0; t < 2; t++) acc[t] = conv_tile(xs, ws, 2 * wave + t, n, g);
+        __syncthreads();  // every wave has read xs and ws
+        if (l + 1 < layers) {
+#pragma unroll
+            for (int t = 0; t < 2; t++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    const int pix = tile_pixel(2 * wave + t, g, r);
+                    xs[(((pix >> 4) + 1) * kXP + (pix & 15) + 1) * kXPitch + n] =
+                        to_bf16(fmaxf(acc[t][r] + b, 0.0f));
+                }
+            stage_weights(ws, w + (size_t)(l + 1) * kC * kKK, tid);
+        } else if (n == 0) {
+#pragma unroll
+            for (int t = 0; t < 2; t++)
+#pragma unroll
+                for (int r = 0; r < 16; r++)
+                    logits[(size_t)c * kPix + tile_pixel(2 * wave + t, g, r)] = acc[t][r] + b;
         }
         __syncthreads();
     }
 }
 
-// Exact scores of the (2R+1)^2 window places around one centre per
-// workgroup (places outside [0, W-tw] x [0, H-th] are skipped):
-// scores[c][(dy+R)*(2R+1) + dx+R], with NaN for skipped places.
+__device__ __forceinline__ float block_max256(float v, float *red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+}
+
+__device__ __forceinline__ float block_sum256(float v, float *red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// Per-patch half of the choice of the best logits. Entry (c, pix) has the
+// coarse place cand[c] - 8 + (pix % 16, pix / 16) and flat index
+// c * 256 + pix; it is selectable when its place is inside the map, its
+// logit is a number, and no other patch holds a better entry (larger logit;
+// equal: smaller flat index) for the same place. Writes patch c's n_top best
+// selectable entries best first to top_v / top_i[c * n_top + r] (index
+// kNoIdx past the last), and its softmax partials part[c] = (max logit, sum
+// of exp(logit - max)) over all 256 logits. Logit i is lg[i * stride].
+__global__ __launch_bounds__(256) void logit_patch_kernel(const float *__restrict__ lg, int stride,
+                                                          int k, const int *__restrict__ cand,
+                                                          int cw, int ch, int n_top,
+                                                          float *__restrict__ top_v,
+                                                          int *__restrict__ top_i,
+                                                          float2 *__restrict__ part) {
+    __shared__ int cxy[2 * kMaxCand];
+    __shared__ int2 ent[kPix];
+    __shared__ float red[4];
+    const int c = blockIdx.x, pix = threadIdx.x;
+    for (int i = pix; i < 2 * k; i += 256) cxy[i] = cand[i];
+    __syncthreads();
+    const int me = c * kPix + pix;
+    const float v = lg[(size_t)me * stride];
+    const int x = cxy[2 * c] - 8 + (pix & 15), y = cxy[2 * c + 1] - 8 + (pix >> 4);
+    bool sel = x >= 0 && x < cw && y >= 0 && y < ch && v == v;
+    if (sel) {
+        for (int c2 = 0; c2 < k; c2++) {
+            const int px = x - (cxy[2 * c2] - 8), py = y - (cxy[2 * c2 + 1] - 8);
+            if (c2 != c && (unsigned)px < 16u && (unsigned)py < 16u) {
+                const int o = c2 * kPix + py * kPS + px;
+                if (better(lg[(size_t)o * stride], o, v, me)) sel = false;
+            }
+        }
+    }
+    ent[pix] = make_int2(__float_as_int(v), sel ? me : kNoIdx);
+    const int nsel = __syncthreads_count(sel);
+    if (sel) {
+        int rank = 0;
+        for (int j = 0; j < kPix; j++) {
+            const int2 e = ent[j];
+            rank += better(__int_as_float(e.x), e.y, v, me);
+        }
+        if (rank < n_top) {
+            top_v[(size_t)c * n_top + rank] = v;
+            top_i[(size_t)c * n_top + rank] = me;
+        }
+    }
+    for (int r = pix; r < n_top; r += 256)
+        if (r >= nsel) {
+            top_v[(size_t)c * n_top + r] = -INFINITY;
+            top_i[(size_t)c * n_top + r] = kNoIdx;
+        }
+    const float mx = block_max256(fmaxf(-INFINITY, v), red);
+    const float se = block_sum256(expf(v - mx), red);
+    if (pix == 0) part[c] = make_float2(mx, se);
+}
+
+// Second half: the n_top best entries of all patches' lists (no place is
+// selectable twice). The n_top-th best patch head T bounds them from below,
+// so only entries at least as good as T are ranked. centers[r] = coarse
+// place of the r-th best ((0, 0) past the last selectable entry); probs[r] =
+// exp(logit - max) / sum(exp(logit - max)) over all k * 256 logits (0 past
+// the last).
+constexpr int kMergeCap = 2048;
+__global__ __launch_bounds__(256) void logit_merge_kernel(const float *__restrict__ top_v,
+                                                          const int *__restrict__ top_i,
+                                                          const float2 *__restrict__ part, int k,
+                                                          const int *__restrict__ cand, int n_top,
+                                                          int *__restrict__ centers,
+                                                          float *__restrict__ probs) {
+    __shared__ int2 head[kMaxCand];
+    __shared__ int2 sur[kMergeCap];
+    __shared__ int2 thr;
+    __shared__ int cnt;
+    __shared__ float norm[2];
+    const int tid = threadIdx.x, m = k * n_top;
+    if (tid == 0) {
+        thr = make_int2(0, kNoIdx);
+        cnt = 0;
+        float mx = -INFINITY;
+        for (int c = 0; c < k; c++) mx = fmaxf(mx, part[c].x);
+        float se = 0.0f;
+        for (int c = 0; c < k; c++)
+            if (part[c].x != -INFINITY) se += part[c].y * expf(part[c].x - mx);
+        norm[0] = mx;
+        norm[1] = se;
+    }
+    for (int c = tid; c < k; c += 256)
+        head[c] = make_int2(__float_as_int(top_v[(size_t)c * n_top]), top_i[(size_t)c * n_top]);
+    __syncthreads();
+    for (int c = tid; c < k; c += 256) {
+        const int2 h = head[c];
+        if (h.y == kNoIdx) continue;
+        int rank = 0;
+        for (int j = 0; j < k; j++)
+            rank += better(__int_as_float(head[j].x), head[j].y, __int_as_float(h.x), h.y);
+        if (rank == n_top - 1) thr = h;
+    }
+    __syncthreads();
+    const int2 T = thr;
+    for (int i = tid; i < m; i += 256) {
+        const float v = top_v[i];
+        const int id = top_i[i];
+        if (id == kNoIdx || better(__int_as_float(T.x), T.y, v, id)) continue;
+        const int s = atomicAdd(&cnt, 1);
+        if (s < kMergeCap) sur[s] = make_int2(__float_as_int(v), id);
+    }
+    __syncthreads();
+    const int ns = cnt;
+    auto emit = [&](int rank, float v, int id) {
+        const int c = id / kPix, pix = id - c * kPix;
+        centers[2 * rank] = cand[2 * c] - 8 + (pix & 15);
+        centers[2 * rank + 1] = cand[2 * c + 1] - 8 + (pix >> 4);
+        probs[rank] = expf(v - norm[0]) / norm[1];
+    };
+    if (ns <= kMergeCap) {
+        for (int s = tid; s < ns; s += 256) {
+            const int2 e = sur[s];
+            const float v = __int_as_float(e.x);
+            int rank = 0;
+            for (int j = 0; j < ns; j++) rank += better(__int_as_float(sur[j].x), sur[j].y, v, e.y);
+            if (rank < n_top) emit(rank, v, e.y);
+        }
+    } else {  // many equal logits: rank every selectable entry against all
+        for (int i = tid; i < m; i += 256) {
+            const float v = top_v[i];
+            const int id = top_i[i];
+            if (id == kNoIdx) continue;
+            int rank = 0;
+            for (int j = 0; j < m && rank < n_top; j++) rank += better(top_v[j], top_i[j], v, id);
+            if (rank < n_top) emit(rank, v, id);
+        }
+    }
+    for (int r = tid; r < n_top; r += 256)
+        if (r >= ns) {
+            centers[2 * r] = 0;
+            centers[2 * r + 1] = 0;
+            probs[r] = 0.0f;
+        }
+}
+
+// Exact scores of the 17 places of row dy = blockIdx.y of the window around
+// centre c = blockIdx.x: scores[c][dy * 17 + dx], NaN where the place is
+// outside [0, W - tw] x [0, H - th]. Thread (rs, dx) sums template rows
+// rs, rs + 15, ... of place dx with v_dot4 over 4-byte groups (image words
+// realigned to the place with v_alignbyte); the workgroup adds the 15
+// partial sums and normalises them as the full map does.
 constexpr int kRefR = 8;
-constexpr int kRefN = (2 * kRefR + 1) * (2 * kRefR + 1);
+constexpr int kRefS = 2 * kRefR + 1;
+constexpr int kRefN = kRefS * kRefS;
+constexpr int kRefSub = 15;
+constexpr int kRefMaxSide = 128;
+__host__ __device__ constexpr int ref_tpl_words(int tw) { return (tw + 3) >> 2; }
+__host__ __device__ constexpr int ref_img_words(int tw) { return ref_tpl_words(tw) + 5; }
 template <int METHOD>
 __global__ __launch_bounds__(256) void refine_kernel(const uint8_t *__restrict__ img, int W, int H,
                                                      const uint8_t *__restrict__ tpl, int tw,
                                                      int th, const int *__restrict__ centers,
-                                                     int scale,
-                                                     const long long *__restrict__ stt_ptr,
-                                                     float *__restrict__ scores) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    uint8_t *ts = smem;                         // th x tw
-    uint8_t *is = smem + ((tw * th + 15) & ~15);  // (th + 2R) x (tw + 2R)
-    const int c = blockIdx.x;
-    const int cx = centers[2 * c] * scale, cy = centers[2 * c + 1] * scale;
-    const int ox = cx - kRefR, oy = cy - kRefR;
-    const int iw = tw + 2 * kRefR, ih = th + 2 * kRefR;
-    for (int i = threadIdx.x; i < tw * th; i += 256) ts[i] = tpl[i];
-    for (int i = threadIdx.x; i < iw * ih; i += 256) {
-        const int r = i / iw, q = i - r * iw;
-        const int gx = ox + q, gy = oy + r;
-        is[i] = (gx >= 0 && gx < W && gy >= 0 && gy < H) ? img[(size_t)gy * W + gx] : 0;
+                                                     int scale, float *__restrict__ scores) {
+    extern __shared__ __align__(16) uint32_t sm[];
+    __shared__ int2 part[kRefSub * kRefS];
+    __shared__ long long red[4];
+    const int twq = ref_tpl_words(tw), rq = ref_img_words(tw);
+    uint32_t *ts = sm;              // th x twq template words
+    uint32_t *is = sm + th * twq;   // th x rq image words from column ox
+    const int c = blockIdx.x, dy = blockIdx.y, tid = threadIdx.x;
+    const int ox = centers[2 * c] * scale - kRefR, oy = centers[2 * c + 1] * scale - kRefR + dy;
+    long long tsq = 0;
+    for (int i = tid; i < th * twq; i += 256) {
+        const int r = i / twq, q = i - r * twq;
+        uint32_t wv = 0;
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+            if (4 * q + b < tw) wv |= (uint32_t)tpl[r * tw + 4 * q + b] << (8 * b);
+        ts[i] = wv;
+        tsq += __builtin_amdgcn_udot4(wv, wv, 0u, false);
+    }
+    for (int i = tid; i < th * rq; i += 256) {
+        const int r = i / rq, q = i - r * rq;
+        const int gy = oy + r;
+        uint32_t wv = 0;
+        if (gy >= 0 && gy < H) {
+            const uint8_t *row = img + (size_t)gy * W;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const int gx = ox + 4 * q + b;
+                if (gx >= 0 && gx < W) wv |= (uint32_t)row[gx] << (8 * b);
+            }
+        }
+        is[i] = wv;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) tsq += __shfl_xor(tsq, o);
+    if ((tid & 63) == 0) red[tid >> 6] = tsq;
+    __syncthreads();
+    const int rs = tid / kRefS, dx = tid - rs * kRefS;
+    if (rs < kRefSub) {
+        const int j0 = dx >> 2;
+        const uint32_t sh = (uint32_t)(dx & 3);
+        const uint32_t last = (tw & 3) ? (1u << (8 * (tw & 3))) - 1u : 0xffffffffu;
+        uint32_t sit = 0, sii = 0;
+        for (int r = rs; r < th; r += kRefSub) {
+            const uint32_t *ir = is + r * rq + j0;
+            const uint32_t *tr = ts + r * twq;
+            uint32_t lo = ir[0];
+            for (int q = 0; q < twq; q++) {
+                const uint32_t hi = ir[q + 1];
+                uint32_t iw = __builtin_amdgcn_alignbyte(hi, lo, sh);
+                if (q == twq - 1) iw &= last;
+                sit = __builtin_amdgcn_udot4(iw, tr[q], sit, false);
+                sii = __builtin_amdgcn_udot4(iw, iw, sii, false);
+                lo = hi;
+            }
+        }
+        part[tid] = make_int2((int)sit, (int)sii);
     }
     __syncthreads();
-    const long long stt = *stt_ptr;
-    for (int pos = threadIdx.x; pos < kRefN; pos += 256) {
-        const int dy = pos / (2 * kRefR + 1), dx = pos - dy * (2 * kRefR + 1);
-        const int x = ox + dx, y = oy + dy;
-        float s = __int_as_float(0x7fc00000);
-        if (x >= 0 && x <= W - tw && y >= 0 && y <= H - th) {
-            long long sit = 0, sii = 0;
-            for (int r = 0; r < th; r++) {
-                const uint8_t *ir = is + (dy + r) * iw + dx;
-                const uint8_t *tr = ts + r * tw;
-                int a = 0, b = 0;
-                for (int q = 0; q < tw; q++) {
-                    const int iv = ir[q];
-                    a += iv * tr[q];
-                    b += iv * iv;
-                }
-                sit += a;
-                sii += b;
-            }
-            s = finish_score<METHOD>(sit, sii, stt, (long long)tw * th);
+    if (tid < kRefS) {
+        long long sit = 0, sii = 0;
+        for (int q = 0; q < kRefSub; q++) {
+            sit += (uint32_t)part[q * kRefS + tid].x;
+            sii += (uint32_t)part[q * kRefS + tid].y;
         }
-        scores[(size_t)c * kRefN + pos] = s;
+        const long long stt = red[0] + red[1] + red[2] + red[3];
+        const int x = ox + tid;
+        float s = __int_as_float(0x7fc00000);
+        if (x >= 0 && x <= W - tw && oy >= 0 && oy <= H - th)
+            s = finish_score<METHOD>(sit, sii, stt, (long long)tw * th);
+        scores[(size_t)c * kRefN + dy * kRefS + tid] = s;
     }
 }
 
-// Best place over every refined window (larger score, then smaller y, then
-// smaller x: find_extremes' first occurrence) and the best score of the
-// places more than `sep` pixels from it (Chebyshev distance). result: x, y,
-// best, second.
-__global__ __launch_bounds__(256) void refine_best_kernel(const float *__restrict__ scores,
-                                                          const int *__restrict__ centers, int n_c,
-                                                          int scale, int W, int sep,
-                                                          float *__restrict__ result) {
-    __shared__ float sv[256];
-    __shared__ long long si[256];
-    const int n = n_c * kRefN, tid = threadIdx.x;
-    auto key = [&](int i) -> long long {
-        const int c = i / kRefN, pos = i - c * kRefN;
-        const int dy = pos / (2 * kRefR + 1), dx = pos - dy * (2 * kRefR + 1);
-        const long long x = centers[2 * c] * scale - kRefR + dx;
-        const long long y = centers[2 * c + 1] * scale - kRefR + dy;
-        return y * W + x;
-    };
-    auto bt = [](float a, long long ia, float b, long long ib) {
-        const bool an = a != a, bn = b != b;
-        if (an != bn) return bn;
-        if (!an && a != b) return a > b;
-        return ia < ib;
+// (score, y, x) order of find_extremes' maximum: larger score, then smaller
+// y, then smaller x; NaN never wins.
+__device__ __forceinline__ bool ref_better(float a, int ay, int ax, float b, int by, int bx) {
+    if (a != a) return false;
+    if (b != b) return true;
+    if (a != b) return a > b;
+    return ay != by ? ay < by : ax < bx;
+}
+
+// Best place over every refined window and the best score of the places
+// more than `sep` pixels (Chebyshev distance) from it. result: x, y, best,
+// second; with p_out, *p_out = *p_top (1 without it).
+__global__ __launch_bounds__(1024) void refine_best_kernel(const float *__restrict__ scores,
+                                                           const int *__restrict__ centers,
+                                                           int n_c, int scale, int sep,
+                                                           const float *__restrict__ p_top,
+                                                           float *__restrict__ result,
+                                                           float *__restrict__ p_out) {
+    __shared__ float sv[16];
+    __shared__ int sy[16], sx[16];
+    const int n = n_c * kRefN, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    auto place = [&](int i, int &x, int &y) {
+        const int c = i / kRefN, pos = i - c * kRefN, dy = pos / kRefS;
+        x = centers[2 * c] * scale - kRefR + (pos - dy * kRefS);
+        y = centers[2 * c + 1] * scale - kRefR + dy;
     };
     float bv = __int_as_float(0x7fc00000);
-    long long bk = 0x7fffffffffffffffLL;
-    for (int i = tid; i < n; i += 256) {
+    int bx = 0, by = 0;
+    for (int i = tid; i < n; i += 1024) {
         const float v = scores[i];
         if (v != v) continue;
-        const long long k = key(i);
-        if (bt(v, k, bv, bk)) {
+        int x, y;
+        place(i, x, y);
+        if (ref_better(v, y, x, bv, by, bx)) {
             bv = v;
-            bk = k;
+            bx = x;
+            by = y;
         }
     }
-    sv[tid] = bv;
-    si[tid] = bk;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (tid < o && bt(sv[tid + o], si[tid + o], sv[tid], si[tid])) {
-            sv[tid] = sv[tid + o];
-            si[tid] = si[tid + o];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const float v = __shfl_xor(bv, o);
+        const int x = __shfl_xor(bx, o), y = __shfl_xor(by, o);
+        if (ref_better(v, y, x, bv, by, bx)) {
+            bv = v;
+            bx = x;
+            by = y;
         }
-        __syncthreads();
     }
-    const float best = sv[0];
-    const long long bkey = si[0];
+    if (lane == 0) {
+        sv[wave] = bv;
+        sx[wave] = bx;
+        sy[wave] = by;
+    }
     __syncthreads();
-    const long long bx = bkey % W, by = bkey / W;
+    bv = sv[0];
+    bx = sx[0];
+    by = sy[0];
+    for (int w = 1; w < 16; w++)
+        if (ref_better(sv[w], sy[w], sx[w], bv, by, bx)) {
+            bv = sv[w];
+            bx = sx[w];
+            by = sy[w];
+        }
     float sec = -INFINITY;
-    for (int i = tid; i < n; i += 256) {
+    for (int i = tid; i < n; i += 1024) {
         const float v = scores[i];
         if (v != v) continue;
-        const long long k = key(i);
-        const long long x = k % W, y = k / W;
-        if (llabs(x - bx) > sep || llabs(y - by) > sep) sec = fmaxf(sec, v);
+        int x, y;
+        place(i, x, y);
+        if (abs(x - bx) > sep || abs(y - by) > sep) sec = fmaxf(sec, v);
     }
-    sv[tid] = sec;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) sec = fmaxf(sec, __shfl_xor(sec, o));
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (tid < o) sv[tid] = fmaxf(sv[tid], sv[tid + o]);
-        __syncthreads();
-    }
+    if (lane == 0) sv[wave] = sec;
+    __syncthreads();
     if (tid == 0) {
+        for (int w = 1; w < 16; w++) sec = fmaxf(sec, sv[w]);
+        sec = fmaxf(sec, sv[0]);
         result[0] = (float)bx;
         result[1] = (float)by;
-        result[2] = best;
-        result[3] = sv[0];
+        result[2] = bv;
+        result[3] = sec;
+        if (p_out) *p_out = p_top ? *p_top : 1.0f;
     }
-}
-
-__global__ void template_sq_u8_kernel(const uint8_t *__restrict__ tpl, int n,
-                                      long long *__restrict__ out) {
-    __shared__ long long red[256];
-    long long s = 0;
-    for (int i = threadIdx.x; i < n; i += 256) s += (long long)tpl[i] * tpl[i];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *out = red[0];
 }
 
 }  // namespace
@@ -623,6 +855,187 @@ using namespace iph;
 namespace {
 inline hipStream_t S(void *s) { return static_cast<hipStream_t>(s); }
 inline unsigned blocks(size_t n) { return (unsigned)((n + 255) / 256); }
+
+iph_status launch_downsample(const uint8_t *src, uint32_t w, uint32_t h, uint8_t *dst,
+                             hipStream_t s) {
+    const uint32_t dw = w / 4, dh = h / 4;
+    if (dw == 0 || dh == 0) return IPH_OK;
+    if (dh > 65535) {
+        set_error("downsampling takes images of up to 262143 rows");
+        return IPH_EINVAL;
+    }
+    const int vec = (w % 16 == 0) && ((uintptr_t)src % 16 == 0) && ((uintptr_t)dst % 4 == 0);
+    downsample4_kernel<<<dim3((dw + 1023) / 1024, dh), 256, 0, s>>>(src, (int)w, (int)dw,
+                                                                    (int)dh, vec, dst);
+    return launch_status("downsample4_kernel");
+}
+
+// Candidate search workspace: tile maxima, then two buffers for the levels
+// of chunk-wise top-k.
+struct CandPlan {
+    int tx, ty, t, nb;
+    size_t bytes;
+};
+CandPlan cand_plan(uint32_t cw, uint32_t ch) {
+    CandPlan p;
+    p.tx = (int)((cw + 15) / 16);
+    p.ty = (int)((ch + 15) / 16);
+    p.t = p.tx * p.ty;
+    p.nb = (p.t + kChunk - 1) / kChunk;
+    WorkspacePlan wp;
+    wp.take<float>(std::max(p.t, 1));
+    wp.take<int>(std::max(p.t, 1));
+    for (int b = 0; b < 2; b++) {
+        wp.take<float>((size_t)std::max(p.nb, 1) * kMaxCand);
+        wp.take<int>((size_t)std::max(p.nb, 1) * kMaxCand);
+    }
+    p.bytes = wp.used;
+    return p;
+}
+
+iph_status launch_candidates(const float *coarse, uint32_t cw, uint32_t ch, int k, int *cand,
+                             float *cscore, void *ws, size_t ws_bytes, hipStream_t s) {
+    if (cw == 0 || ch == 0 || (size_t)cw * ch > (size_t)0x7ffffffe || k <= 0 || k > kMaxCand) {
+        set_error("coarse map of %ux%u or %d candidates out of range (1 to %d)", cw, ch, k,
+                  kMaxCand);
+        return IPH_EINVAL;
+    }
+    const CandPlan p = cand_plan(cw, ch);
+    Workspace wsp(ws, ws_bytes);
+    float *ts = wsp.take<float>(std::max(p.t, 1));
+    int *ti = wsp.take<int>(std::max(p.t, 1));
+    float *lv[2];
+    int *li[2];
+    for (int b = 0; b < 2; b++) {
+        lv[b] = wsp.take<float>((size_t)std::max(p.nb, 1) * kMaxCand);
+        li[b] = wsp.take<int>((size_t)std::max(p.nb, 1) * kMaxCand);
+    }
+    if (!ts || !ti || !lv[1] || !li[1]) {
+        set_error("workspace too small for the candidate search");
+        return IPH_EINVAL;
+    }
+    tile_max_kernel<<<dim3((cw + 255) / 256, p.ty), 256, 0, s>>>(coarse, (int)cw, (int)ch, p.tx,
+                                                                 ts, ti);
+    IPH_TRY(launch_status("tile_max_kernel"));
+    const float *sv = ts;
+    const int *si = ti;
+    int n = p.t, b = 0;
+    while (n > kChunk) {
+        const int nb = (n + kChunk - 1) / kChunk;
+        topk_chunk_kernel<<<nb, 512, 0, s>>>(sv, si, n, k, lv[b], li[b], 0, nullptr, nullptr);
+        IPH_TRY(launch_status("topk_chunk_kernel"));
+        sv = lv[b];
+        si = li[b];
+        n = nb * k;
+        b ^= 1;
+    }
+    topk_chunk_kernel<<<1, 512, 0, s>>>(sv, si, n, k, nullptr, nullptr, (int)cw, cand, cscore);
+    return launch_status("topk_chunk_kernel");
+}
+
+constexpr int kMaxTop = 64;
+size_t top_workspace(int k, int n_top) {
+    WorkspacePlan wp;
+    wp.take<float>((size_t)std::max(k, 1) * std::max(n_top, 1));
+    wp.take<int>((size_t)std::max(k, 1) * std::max(n_top, 1));
+    wp.take<float2>(std::max(k, 1));
+    return wp.used;
+}
+
+iph_status launch_top(const float *lg, int stride, int k, const int *cand, uint32_t cw,
+                      uint32_t ch, int n_top, int *centers, float *probs, void *ws,
+                      size_t ws_bytes, hipStream_t s) {
+    if (k <= 0 || k > kMaxCand || n_top <= 0 || n_top > kMaxTop) {
+        set_error("1 to %d candidates and 1 to %d top places", kMaxCand, kMaxTop);
+        return IPH_EINVAL;
+    }
+    Workspace wsp(ws, ws_bytes);
+    float *tv = wsp.take<float>((size_t)k * n_top);
+    int *ti = wsp.take<int>((size_t)k * n_top);
+    float2 *part = wsp.take<float2>(k);
+    if (!tv || !ti || !part) {
+        set_error("workspace too small for the top places");
+        return IPH_EINVAL;
+    }
+    logit_patch_kernel<<<k, 256, 0, s>>>(lg, stride, k, cand, (int)cw, (int)ch, n_top, tv, ti,
+                                         part);
+    IPH_TRY(launch_status("logit_patch_kernel"));
+    logit_merge_kernel<<<1, 256, 0, s>>>(tv, ti, part, k, cand, n_top, centers, probs);
+    return launch_status("logit_merge_kernel");
+}
+
+size_t refine_workspace(int n_centers) {
+    WorkspacePlan wp;
+    wp.take<float>((size_t)std::max(n_centers, 1) * kRefN);
+    return wp.used;
+}
+
+iph_status launch_refine(const uint8_t *image, uint32_t w, uint32_t h, const uint8_t *tpl,
+                         uint32_t tw, uint32_t th, int32_t method, const int32_t *centers,
+                         int32_t n_centers, int32_t scale, int32_t sep, const float *p_top,
+                         float *result, float *p_out, void *ws, size_t ws_bytes,
+                         hipStream_t s) {
+    IPH_TRY(check_match(IPH_U8, w, h, tw, th, method, IPH_PATH_AUTO));
+    if (tw == 0 || th == 0 || tw > (uint32_t)kRefMaxSide || th > (uint32_t)kRefMaxSide ||
+        n_centers <= 0 || n_centers > 1024) {
+        set_error("refinement takes templates of 1 to %d pixels a side and 1 to 1024 centres",
+                  kRefMaxSide);
+        return IPH_EINVAL;
+    }
+    Workspace wsp(ws, ws_bytes);
+    float *scores = wsp.take<float>((size_t)n_centers * kRefN);
+    if (!scores) {
+        set_error("workspace too small for the refinement");
+        return IPH_EINVAL;
+    }
+    const size_t lds = (size_t)th * (ref_tpl_words((int)tw) + ref_img_words((int)tw)) * 4;
+    const dim3 grid(n_centers, kRefS);
+#define RF(M)                                                                                \
+    refine_kernel<M><<<grid, 256, lds, s>>>(image, (int)w, (int)h, tpl, (int)tw, (int)th,     \
+                                            centers, scale, scores)
+    switch (method) {
+    case IPH_SSE: RF(IPH_SSE); break;
+    case IPH_SSE_NORMED: RF(IPH_SSE_NORMED); break;
+    case IPH_CCORR: RF(IPH_CCORR); break;
+    default: RF(IPH_CCORR_NORMED); break;
+    }
+#undef RF
+    IPH_TRY(launch_status("refine_kernel"));
+    refine_best_kernel<<<1, 1024, 0, s>>>(scores, centers, n_centers, scale, sep, p_top, result,
+                                          p_out);
+    return launch_status("refine_best_kernel");
+}
+
+// Sizes of one proposal search: the 4x-downsampled image and template and
+// the coarse map between them.
+struct SearchPlan {
+    uint32_t sw, sh, stw, sth, cw, ch;
+};
+iph_status search_plan(uint32_t w, uint32_t h, uint32_t tw, uint32_t th, int32_t method,
+                       int32_t k, int32_t n_top, SearchPlan &p) {
+    if (method != IPH_CCORR && method != IPH_CCORR_NORMED) {
+        set_error("the proposal search serves the correlation methods");
+        return IPH_EINVAL;
+    }
+    if (tw < 4 || th < 4 || tw > (uint32_t)kRefMaxSide || th > (uint32_t)kRefMaxSide || w < tw ||
+        h < th) {
+        set_error("the proposal search takes templates of 4 to %d pixels a side, no larger "
+                  "than the image", kRefMaxSide);
+        return IPH_EINVAL;
+    }
+    if (k <= 0 || k > kMaxCand || n_top <= 0 || n_top > kMaxTop || n_top > k) {
+        set_error("1 to %d candidates and 1 to min(%d, candidates) exact windows", kMaxCand,
+                  kMaxTop);
+        return IPH_EINVAL;
+    }
+    p.sw = w / 4;
+    p.sh = h / 4;
+    p.stw = tw / 4;
+    p.sth = th / 4;
+    p.cw = p.sw - p.stw + 1;
+    p.ch = p.sh - p.sth + 1;
+    return IPH_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -703,16 +1116,11 @@ IPH_API iph_status iph_dev_cnn_to_bf16(const float *src, int32_t n, uint16_t *ds
 IPH_API iph_status iph_dev_downsample4_u8(const uint8_t *src, uint32_t w, uint32_t h,
                                           uint8_t *dst, void *stream) {
     clear_error();
-    const int dw = (int)(w / 4), dh = (int)(h / 4);
-    if (dw == 0 || dh == 0) return IPH_OK;
-    dim3 grid((dw + 63) / 64, (dh + 3) / 4);
-    downsample4_kernel<<<grid, 256, 0, S(stream)>>>(src, (int)w, dw, dh, dst);
-    return launch_status("downsample4_kernel");
+    return launch_downsample(src, w, h, dst, S(stream));
 }
 
 IPH_API size_t iph_dev_proposal_candidates_workspace(uint32_t cw, uint32_t ch) {
-    const size_t t = (size_t)((cw + 15) / 16) * ((ch + 15) / 16);
-    return std::max<size_t>(t, 1) * (sizeof(float) + sizeof(int)) + 256;
+    return cand_plan(cw, ch).bytes;
 }
 
 IPH_API iph_status iph_dev_proposal_candidates(const float *coarse, uint32_t cw, uint32_t ch,
@@ -720,22 +1128,8 @@ IPH_API iph_status iph_dev_proposal_candidates(const float *coarse, uint32_t cw,
                                                void *workspace, size_t workspace_bytes,
                                                void *stream) {
     clear_error();
-    const int tx = (int)((cw + 15) / 16), ty = (int)((ch + 15) / 16), t = tx * ty;
-    if (t <= 0 || t > kMaxTiles || k <= 0 || k > 1024) {
-        set_error("coarse map of %ux%u or %d candidates out of range", cw, ch, k);
-        return IPH_EINVAL;
-    }
-    if (workspace_bytes < iph_dev_proposal_candidates_workspace(cw, ch)) {
-        set_error("workspace too small for the candidate search");
-        return IPH_EINVAL;
-    }
-    float *ts = static_cast<float *>(workspace);
-    int *ti = reinterpret_cast<int *>(ts + t);
-    tile_max_kernel<<<t, 256, 0, S(stream)>>>(coarse, (int)cw, (int)ch, tx, ts, ti);
-    IPH_TRY(launch_status("tile_max_kernel"));
-    topk_tiles_kernel<<<1, 1024, kMaxTiles * (sizeof(float) + sizeof(int)), S(stream)>>>(
-        ts, ti, t, k, (int)cw, cand, cand_score);
-    return launch_status("topk_tiles_kernel");
+    return launch_candidates(coarse, cw, ch, k, cand, cand_score, workspace, workspace_bytes,
+                             S(stream));
 }
 
 IPH_API iph_status iph_dev_proposal_features(const float *coarse, uint32_t cw, uint32_t ch,
@@ -750,21 +1144,21 @@ IPH_API iph_status iph_dev_proposal_features(const float *coarse, uint32_t cw, u
     return launch_status("features_kernel");
 }
 
+IPH_API size_t iph_dev_proposal_top_workspace(int32_t k, int32_t n_top) {
+    return top_workspace(k, n_top);
+}
+
 IPH_API iph_status iph_dev_proposal_top(const float *z_last, int32_t k, const int32_t *cand,
                                         uint32_t cw, uint32_t ch, int32_t n_top, int32_t *centers,
-                                        float *probs, void *stream) {
+                                        float *probs, void *workspace, size_t workspace_bytes,
+                                        void *stream) {
     clear_error();
-    if (n_top <= 0 || n_top > 64) {
-        set_error("between 1 and 64 top candidates");
-        return IPH_EINVAL;
-    }
-    top_logits_kernel<<<1, 256, 0, S(stream)>>>(z_last, k, cand, (int)cw, (int)ch, n_top, centers,
-                                                probs);
-    return launch_status("top_logits_kernel");
+    return launch_top(z_last, kC, k, cand, cw, ch, n_top, centers, probs, workspace,
+                      workspace_bytes, S(stream));
 }
 
 IPH_API size_t iph_dev_proposal_refine_workspace(int32_t n_centers) {
-    return 256 + (size_t)std::max(n_centers, 1) * kRefN * sizeof(float);
+    return refine_workspace(n_centers);
 }
 
 IPH_API iph_status iph_dev_proposal_refine(const uint8_t *image, uint32_t w, uint32_t h,
@@ -774,37 +1168,91 @@ IPH_API iph_status iph_dev_proposal_refine(const uint8_t *image, uint32_t w, uin
                                            float *result, void *workspace, size_t workspace_bytes,
                                            void *stream) {
     clear_error();
-    IPH_TRY(check_match(IPH_U8, w, h, tw, th, method, IPH_PATH_AUTO));
-    if (tw == 0 || th == 0 || tw > 128 || th > 128 || n_centers <= 0 || n_centers > 1024) {
-        set_error("refinement takes templates of 1 to 128 pixels a side and 1 to 1024 centres");
+    return launch_refine(image, w, h, tpl, tw, th, method, centers, n_centers, scale, sep,
+                         nullptr, result, nullptr, workspace, workspace_bytes, S(stream));
+}
+
+IPH_API size_t iph_dev_proposal_search_workspace(uint32_t w, uint32_t h, uint32_t tw, uint32_t th,
+                                                 int32_t method, int32_t k, int32_t n_top) {
+    SearchPlan p;
+    if (search_plan(w, h, tw, th, method, k, n_top, p) != IPH_OK) {
+        clear_error();
+        return 256;
+    }
+    WorkspacePlan wp;
+    wp.take<uint8_t>((size_t)p.sw * p.sh);
+    wp.take<uint8_t>((size_t)p.stw * p.sth);
+    wp.take<float>((size_t)p.cw * p.ch);
+    wp.take<uint8_t>(mfma_match_workspace(IPH_U8, p.sw, p.sh, p.stw, p.sth, method));
+    wp.take<uint8_t>(cand_plan(p.cw, p.ch).bytes);
+    wp.take<int>(2 * (size_t)k);
+    wp.take<float>(k);
+    wp.take<float>((size_t)k * kPix);
+    wp.take<uint8_t>(top_workspace(k, n_top));
+    wp.take<int>(2 * (size_t)n_top);
+    wp.take<float>(n_top);
+    wp.take<uint8_t>(refine_workspace(n_top));
+    return wp.used;
+}
+
+IPH_API iph_status iph_dev_proposal_search(const uint8_t *image, uint32_t w, uint32_t h,
+                                           const uint8_t *tpl, uint32_t tw, uint32_t th,
+                                           int32_t method, const uint16_t *weights,
+                                           const float *bias, int32_t k, int32_t n_top,
+                                           int32_t sep, float *result, void *workspace,
+                                           size_t workspace_bytes, void *stream) {
+    clear_error();
+    SearchPlan p;
+    IPH_TRY(search_plan(w, h, tw, th, method, k, n_top, p));
+    if ((weights == nullptr) != (bias == nullptr)) {
+        set_error("weights and bias go together (both NULL: rank by coarse score)");
         return IPH_EINVAL;
     }
-    if (workspace_bytes < iph_dev_proposal_refine_workspace(n_centers)) {
-        set_error("workspace too small for the refinement");
+    if (workspace_bytes < iph_dev_proposal_search_workspace(w, h, tw, th, method, k, n_top)) {
+        set_error("workspace too small for the proposal search");
         return IPH_EINVAL;
     }
-    long long *stt = static_cast<long long *>(workspace);
-    float *scores = reinterpret_cast<float *>(static_cast<char *>(workspace) + 256);
-    template_sq_u8_kernel<<<1, 256, 0, S(stream)>>>(tpl, (int)(tw * th), stt);
-    IPH_TRY(launch_status("template_sq_u8_kernel"));
-    const size_t lds = ((tw * th + 15) & ~15u) + (size_t)(tw + 2 * kRefR) * (th + 2 * kRefR);
-#define RF(M)                                                                                \
-    refine_kernel<M><<<n_centers, 256, lds, S(stream)>>>(image, (int)w, (int)h, tpl, (int)tw, \
-                                                         (int)th, centers, scale, stt, scores)
-    switch (method) {
-    case IPH_SSE: RF(IPH_SSE); break;
-    case IPH_SSE_NORMED: RF(IPH_SSE_NORMED); break;
-    case IPH_CCORR: RF(IPH_CCORR); break;
-    default: RF(IPH_CCORR_NORMED); break;
+    Workspace wsp(workspace, workspace_bytes);
+    uint8_t *small = wsp.take<uint8_t>((size_t)p.sw * p.sh);
+    uint8_t *stpl = wsp.take<uint8_t>((size_t)p.stw * p.sth);
+    float *coarse = wsp.take<float>((size_t)p.cw * p.ch);
+    const size_t mws_bytes = mfma_match_workspace(IPH_U8, p.sw, p.sh, p.stw, p.sth, method);
+    uint8_t *mws = wsp.take<uint8_t>(mws_bytes);
+    const size_t cws_bytes = cand_plan(p.cw, p.ch).bytes;
+    uint8_t *cws = wsp.take<uint8_t>(cws_bytes);
+    int *cand = wsp.take<int>(2 * (size_t)k);
+    float *cscore = wsp.take<float>(k);
+    float *logits = wsp.take<float>((size_t)k * kPix);
+    const size_t tws_bytes = top_workspace(k, n_top);
+    uint8_t *tws = wsp.take<uint8_t>(tws_bytes);
+    int *ctr = wsp.take<int>(2 * (size_t)n_top);
+    float *probs = wsp.take<float>(n_top);
+    const size_t rws_bytes = refine_workspace(n_top);
+    uint8_t *rws = wsp.take<uint8_t>(rws_bytes);
+    if (!rws || image == nullptr || tpl == nullptr || result == nullptr) {
+        set_error("null image, template, result or workspace");
+        return IPH_EINVAL;
     }
-#undef RF
-    IPH_TRY(launch_status("refine_kernel"));
-    // SSE methods pick the smallest score: negate the search by refining
-    // the best of -score is not needed for the correlation methods this path
-    // serves; the caller reads `result` as (x, y, best, second).
-    refine_best_kernel<<<1, 256, 0, S(stream)>>>(scores, centers, n_centers, scale, (int)w, sep,
-                                                 result);
-    return launch_status("refine_best_kernel");
+    hipStream_t s = S(stream);
+    IPH_TRY(launch_downsample(image, w, h, small, s));
+    IPH_TRY(launch_downsample(tpl, tw, th, stpl, s));
+    IPH_TRY(launch_mfma_match(small, IPH_U8, p.sw, p.sh, stpl, p.stw, p.sth, method, coarse, mws,
+                              mws_bytes, s));
+    IPH_TRY(launch_candidates(coarse, p.cw, p.ch, k, cand, cscore, cws, cws_bytes, s));
+    const int *centers = cand;
+    const float *p_top = nullptr;
+    if (weights) {
+        cnn_proposal_forward_kernel<<<k, 256, 0, s>>>(coarse, (int)p.cw, (int)p.ch, small,
+                                                      (int)p.sw, (int)p.sh, (int)p.stw / 2, cand,
+                                                      cscore, weights, bias, IPH_CNN_LAYERS,
+                                                      logits);
+        IPH_TRY(launch_status("cnn_proposal_forward_kernel"));
+        IPH_TRY(launch_top(logits, 1, k, cand, p.cw, p.ch, n_top, ctr, probs, tws, tws_bytes, s));
+        centers = ctr;
+        p_top = probs;
+    }
+    return launch_refine(image, w, h, tpl, tw, th, method, centers, n_top, 4, sep, p_top, result,
+                         result + 4, rws, rws_bytes, s);
 }
 
 }  // extern "C"

@@ -5,7 +5,10 @@ No counterpart in the reference, whose only search is the full map of
 match_template (template_matching.rs:78-92) read by find_extremes
 (template_matching.rs:479-511); that search stays the judge of this one.
 
-Search (`ProposalSearch.search`), every step a kernel of libimageproc_hip.so:
+Search (`ProposalSearch.search`: one call of iph_dev_proposal_search, which
+enqueues every step below as kernels of libimageproc_hip.so, then one read of
+its 5-float result; `search_steps` makes the same search from the step-wise
+entry points, value for value):
   1. 4x-downsample image and template (rounded 4x4 means) and compute the
      coarse score map with the library's own match_template (int8 MFMA);
   2. take the best place of each 16x16 tile of the coarse map and keep the
@@ -61,14 +64,17 @@ class ProposalNet:
 
     def __init__(self, seed: int = 0):
         rng = np.random.default_rng(seed)
-        self.w, self.b = [], []
+        # one contiguous block per kind (layer l at offset l), as the fused
+        # search entry point takes them
+        self.w_all = torch.zeros((LAYERS, CH, KK), dtype=torch.float32, device="cuda")
+        self.b_all = torch.zeros((LAYERS, CH), dtype=torch.float32, device="cuda")
+        self.w, self.b = list(self.w_all.unbind(0)), list(self.b_all.unbind(0))
         for l in range(LAYERS):
             w = np.zeros((CH, 3, 3, CH), np.float32)
             fan_in = 9 * REAL_IN[l]
             w[:REAL_OUT[l], :, :, :REAL_IN[l]] = (rng.standard_normal(
                 (REAL_OUT[l], 3, 3, REAL_IN[l])) * np.sqrt(2.0 / fan_in)).astype(np.float32)
-            self.w.append(torch.from_numpy(w.reshape(CH, KK)).cuda())
-            self.b.append(torch.zeros(CH, dtype=torch.float32, device="cuda"))
+            self.w[l].copy_(torch.from_numpy(w.reshape(CH, KK)))
         self._adam = [(torch.zeros_like(t), torch.zeros_like(t)) for t in self.w + self.b]
         self.step = 0
         self._dw_parts = 64
@@ -78,13 +84,12 @@ class ProposalNet:
 
     def refresh(self) -> None:
         """bf16 and flipped copies of the f32 weights."""
-        self.wb, self.wf = [], []
-        for w in self.w:
-            wb = torch.empty(CH * KK, dtype=torch.int16, device="cuda")
-            check(lib().iph_dev_cnn_to_bf16(_p(w), CH * KK, _p(wb), _s()))
+        self.wb_all = torch.empty((LAYERS, CH * KK), dtype=torch.int16, device="cuda")
+        check(lib().iph_dev_cnn_to_bf16(_p(self.w_all), LAYERS * CH * KK, _p(self.wb_all), _s()))
+        self.wb, self.wf = list(self.wb_all.unbind(0)), []
+        for wb in self.wb:
             wf = torch.empty_like(wb)
             check(lib().iph_dev_cnn_flip_weights(_p(wb), _p(wf), _s()))
-            self.wb.append(wb)
             self.wf.append(wf)
 
     def forward(self, x: torch.Tensor, keep: bool = False):
@@ -186,9 +191,15 @@ class ProposalSearch:
                 "patches": torch.empty((self.k, PIX, CH), dtype=torch.int16, device="cuda"),
                 "centers": torch.empty((self.n_top, 2), dtype=torch.int32, device="cuda"),
                 "probs": torch.empty(self.n_top, dtype=torch.float32, device="cuda"),
+                "tws": torch.empty(lib().iph_dev_proposal_top_workspace(self.k, self.n_top),
+                                   dtype=torch.uint8, device="cuda"),
                 "rws": torch.empty(lib().iph_dev_proposal_refine_workspace(self.n_top),
                                    dtype=torch.uint8, device="cuda"),
-                "result": torch.empty(4, dtype=torch.float32, device="cuda"),
+                "sws": torch.empty(lib().iph_dev_proposal_search_workspace(
+                    w, h, tw, th, int(self.method), self.k, self.n_top), dtype=torch.uint8,
+                    device="cuda"),
+                "result": torch.empty(5, dtype=torch.float32, device="cuda"),
+                "host": torch.empty(5, dtype=torch.float32, pin_memory=True),
                 "shape": (sh, sw, sth, stw, ch_, cw_),
             }
             self._cache[key] = b
@@ -215,6 +226,14 @@ class ProposalSearch:
                                               _p(b["patches"]), _s()))
         return b["patches"]
 
+    def top(self, zl: torch.Tensor, b) -> torch.Tensor:
+        """The network's n_top best distinct coarse places (and b["probs"])."""
+        sh, sw, sth, stw, ch_, cw_ = b["shape"]
+        check(lib().iph_dev_proposal_top(_p(zl), self.k, _p(b["cand"]), cw_, ch_, self.n_top,
+                                         _p(b["centers"]), _p(b["probs"]), _p(b["tws"]),
+                                         b["tws"].numel(), _s()))
+        return b["centers"]
+
     def refine(self, image, tpl, b, centers) -> None:
         h, w = image.shape
         th, tw = tpl.shape
@@ -229,22 +248,40 @@ class ProposalSearch:
         rec = dev.find_extremes(full).cpu().numpy().view(np.uint32)
         return int(rec[2]), int(rec[3]), float(rec[0:1].view(np.float32)[0])
 
+    def search_steps(self, image: torch.Tensor, tpl: torch.Tensor, use_net: bool = True):
+        """The search from the step-wise entry points: (x, y, best, second,
+        top probability) before the fallback decision."""
+        b = self.candidates(image, tpl)
+        if use_net and self.net is not None:
+            centers = self.top(self.net.forward(self.features(b)), b)
+            p_top = float(b["probs"][0])
+        else:
+            centers, p_top = b["cand"][: self.n_top], 1.0
+        self.refine(image, tpl, b, centers)
+        r = b["result"][:4].cpu().numpy()
+        return int(r[0]), int(r[1]), float(r[2]), float(r[3]), p_top
+
+    def search_raw(self, image: torch.Tensor, tpl: torch.Tensor, use_net: bool = True):
+        """One iph_dev_proposal_search call: (x, y, best, second, top
+        probability) before the fallback decision."""
+        h, w = image.shape
+        th, tw = tpl.shape
+        b = self._buffers(h, w, th, tw)
+        net = self.net if use_net else None
+        check(lib().iph_dev_proposal_search(
+            _p(image), w, h, _p(tpl), tw, th, int(self.method),
+            _p(net.wb_all) if net is not None else None,
+            _p(net.b_all) if net is not None else None, self.k, self.n_top, self.sep,
+            _p(b["result"]), _p(b["sws"]), b["sws"].numel(), _s()))
+        host = b["host"]
+        host.copy_(b["result"], non_blocking=True)
+        torch.cuda.current_stream().synchronize()
+        r = host.numpy()
+        return int(r[0]), int(r[1]), float(r[2]), float(r[3]), float(r[4])
+
     def search(self, image: torch.Tensor, tpl: torch.Tensor, use_net: bool = True):
         """(x, y, score, used_full_map) of match_template's best place."""
-        b = self.candidates(image, tpl)
-        sh, sw, sth, stw, ch_, cw_ = b["shape"]
-        with_net = use_net and self.net is not None
-        if with_net:
-            zl = self.net.forward(self.features(b))
-            check(lib().iph_dev_proposal_top(_p(zl), self.k, _p(b["cand"]), cw_, ch_, self.n_top,
-                                             _p(b["centers"]), _p(b["probs"]), _s()))
-            centers = b["centers"]
-        else:
-            centers = b["cand"][: self.n_top]
-        self.refine(image, tpl, b, centers)
-        r = b["result"].cpu().numpy()
-        p_top = float(b["probs"][0]) if with_net else 1.0
-        x, y, best, second = int(r[0]), int(r[1]), float(r[2]), float(r[3])
+        x, y, best, second, p_top = self.search_raw(image, tpl, use_net)
         if not (best == best) or best - second < self.margin or p_top < self.min_prob:
             return (*self.full_map(image, tpl), True)
         return x, y, best, False

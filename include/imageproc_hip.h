@@ -239,6 +239,7 @@ IPH_API iph_status iph_batch_tuning(iph_batch *batch, char *buf, size_t cap);
  */
 #define IPH_CNN_CHANNELS 32
 #define IPH_CNN_PATCH 16
+#define IPH_CNN_LAYERS 4
 /* y = conv3x3(x, w) + bias (zero padding) over n_patches patches; z (f32,
  * may be NULL) gets y, a (bf16, may be NULL) relu(y) if relu else y. */
 IPH_API iph_status iph_dev_cnn_conv3x3(const uint16_t *x, const uint16_t *w, const float *bias,
@@ -271,8 +272,9 @@ IPH_API iph_status iph_dev_cnn_to_bf16(const float *src, int32_t n, uint16_t *ds
 /* dst (w/4) x (h/4): rounded mean of each full 4x4 block. */
 IPH_API iph_status iph_dev_downsample4_u8(const uint8_t *src, uint32_t width, uint32_t height,
                                           uint8_t *dst, void *stream);
-/* The k best 16x16-tile maxima of a coarse map (larger first; ties: lower
- * index; NaN last): cand[2i], cand[2i+1] = x, y; cand_score[i]. */
+/* The k best (1 <= k <= 256) 16x16-tile maxima of a coarse map (larger
+ * first; ties: lower index; NaN last): cand[2i], cand[2i+1] = x, y;
+ * cand_score[i]; (0, 0) and -inf past the number of tiles. */
 IPH_API size_t iph_dev_proposal_candidates_workspace(uint32_t cw, uint32_t ch);
 IPH_API iph_status iph_dev_proposal_candidates(const float *coarse, uint32_t cw, uint32_t ch,
                                                int32_t k, int32_t *cand, float *cand_score,
@@ -284,11 +286,15 @@ IPH_API iph_status iph_dev_proposal_features(const float *coarse, uint32_t cw, u
                                              uint32_t small_tw, const int32_t *cand,
                                              const float *cand_score, int32_t k,
                                              uint16_t *patches, void *stream);
-/* The n_top best logits at distinct coarse places (centers: x, y) and their
- * softmax probabilities over all k * 256 logits. */
+/* The n_top (1 to 64) best logits (channel 0 of z_last, k <= 256 patches)
+ * at distinct coarse places inside the map (centers: x, y; ties: lower
+ * patch * 256 + pixel) and their softmax probabilities over all k * 256
+ * logits; (0, 0) and 0 past the last such place. */
+IPH_API size_t iph_dev_proposal_top_workspace(int32_t k, int32_t n_top);
 IPH_API iph_status iph_dev_proposal_top(const float *z_last, int32_t k, const int32_t *cand,
                                         uint32_t cw, uint32_t ch, int32_t n_top, int32_t *centers,
-                                        float *probs, void *stream);
+                                        float *probs, void *workspace, size_t workspace_bytes,
+                                        void *stream);
 /* Exact match_template scores (the full map's values) at every place within
  * 8 pixels of each centre * scale, then result[0..3] = x, y of the best
  * (find_extremes' order), its score, and the best score more than `sep`
@@ -300,6 +306,23 @@ IPH_API iph_status iph_dev_proposal_refine(const uint8_t *image, uint32_t width,
                                            int32_t n_centers, int32_t scale, int32_t sep,
                                            float *result, void *workspace, size_t workspace_bytes,
                                            void *stream);
+/* The whole search on a u8 image and a 4..128 px template (CrossCorrelation
+ * or CrossCorrelationNormalized), enqueued on `stream`: downsampling, the
+ * coarse map, k candidates, the network (weights: IPH_CNN_LAYERS blocks of
+ * 32 x 288 bf16, bias: IPH_CNN_LAYERS x 32 f32; both NULL ranks by coarse
+ * score instead), exact windows around the n_top (<= k) best places.
+ * result[0..4] = x, y, best score, best score more than `sep` px away, top
+ * probability (1 without the network): the composition of the functions
+ * above, value for value. */
+IPH_API size_t iph_dev_proposal_search_workspace(uint32_t width, uint32_t height, uint32_t tw,
+                                                 uint32_t th, int32_t method, int32_t k,
+                                                 int32_t n_top);
+IPH_API iph_status iph_dev_proposal_search(const uint8_t *image, uint32_t width, uint32_t height,
+                                           const uint8_t *tmpl, uint32_t tw, uint32_t th,
+                                           int32_t method, const uint16_t *weights,
+                                           const float *bias, int32_t k, int32_t n_top,
+                                           int32_t sep, float *result, void *workspace,
+                                           size_t workspace_bytes, void *stream);
 
 #ifdef __cplusplus
 }

@@ -117,3 +117,22 @@ def test_batch_argument_errors_before_device():
             check(lib().iph_batch_run(h, None))
     finally:
         lib().iph_batch_destroy(h)
+
+
+def test_proposal_search_argument_errors_before_device():
+    """The one-call proposal search validates its sizes before any launch."""
+    from imageproc_amd._lib import Panic, check, lib
+
+    assert lib().iph_dev_proposal_search_workspace(8192, 8192, 48, 48, 3, 64, 16) > 2048 * 2048
+    with pytest.raises(Panic, match="correlation methods"):
+        check(lib().iph_dev_proposal_search(None, 64, 64, None, 8, 8, 0, None, None, 64, 16, 4,
+                                            None, None, 0, None))
+    with pytest.raises(Panic, match="templates of 4 to 128"):
+        check(lib().iph_dev_proposal_search(None, 64, 64, None, 3, 8, 3, None, None, 64, 16, 4,
+                                            None, None, 0, None))
+    with pytest.raises(Panic, match="exact windows"):
+        check(lib().iph_dev_proposal_search(None, 64, 64, None, 8, 8, 3, None, None, 8, 16, 4,
+                                            None, None, 0, None))
+    with pytest.raises(Panic, match="workspace too small"):
+        check(lib().iph_dev_proposal_search(None, 64, 64, None, 8, 8, 3, None, None, 8, 4, 4,
+                                            None, None, 0, None))
