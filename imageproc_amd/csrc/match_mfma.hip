@@ -82,9 +82,10 @@ __global__ __launch_bounds__(64) void toeplitz_i8_kernel(const uint8_t *__restri
 
 // Exact window sums (sum I, sum I^2) for every output position of a
 // kWinCols x kWinRows output tile: each staged image row gets its horizontal
-// tw-window sums by a sliding pass (one thread per row), then each output
-// column its vertical th-window sums over those (one thread per column and
-// 16-row segment), written coalesced. All arithmetic is exact u32.
+// tw-window sums by a sliding pass (one thread per 16-column segment of a
+// row), then each output column its vertical th-window sums over those (one
+// thread per column and 8-row segment), written coalesced. All arithmetic is
+// exact u32.
 __global__ __launch_bounds__(256) void window_sums_u8_kernel(const uint8_t *__restrict__ img,
                                                              int W, int tw, int th, int ow,
                                                              int oh, uint2 *__restrict__ ws) {
@@ -110,17 +111,21 @@ __global__ __launch_bounds__(256) void window_sums_u8_kernel(const uint8_t *__re
         }
     }
     __syncthreads();
-    for (int r = threadIdx.x; r < rows; r += 256) {
-        const uint8_t *p = blk + r * cols;
+    // horizontal sums: one thread per 16-column segment of a staged row
+    constexpr int kSegs = kWinCols / 16;
+    for (int it = threadIdx.x; it < rows * kSegs; it += 256) {
+        const int r = it / kSegs, q = it - r * kSegs;
+        const uint8_t *p = blk + r * cols + 16 * q;
         uint32_t a = 0, b = 0;
         for (int dx = 0; dx < tw; dx++) {
             uint32_t v = p[dx];
             a += v;
             b += v * v;
         }
-        uint2 *o = hs + r * kWinCols;
+        uint2 *o = hs + r * kWinCols + 16 * q;
         o[0] = make_uint2(a, b);
-        for (int x = 1; x < kWinCols; x++) {
+#pragma unroll
+        for (int x = 1; x < 16; x++) {
             uint32_t vin = p[x + tw - 1], vout = p[x - 1];
             a += vin - vout;
             b += vin * vin - vout * vout;

@@ -322,99 +322,96 @@ __global__ __launch_bounds__(256) void downsample4_kernel(const uint8_t *__restr
     }
 }
 
-// Order of candidate entries (score, index): larger score first; NaN after
-// every number; equal scores: lower index. Index kNoIdx marks an empty slot,
-// after everything else.
-constexpr int kNoIdx = 0x7fffffff;
-__device__ __forceinline__ bool better(float a, int ia, float b, int ib) {
-    if (ia == kNoIdx || ib == kNoIdx) return ib == kNoIdx && ia != kNoIdx;
-    const bool an = a != a, bn = b != b;
-    if (an != bn) return bn;
-    if (!an && a != b) return a > b;
-    return ia < ib;
+// Order of candidate entries (value, index): larger value first; NaN after
+// every number; equal values: lower index. entry_key maps an entry to a
+// 64-bit key that is larger exactly when the entry comes first (value bits
+// made monotone, -0 taken as +0, NaN lowest; the complemented index below);
+// key 0 is an empty slot, after everything.
+__device__ __forceinline__ uint64_t entry_key(float v, int idx) {
+    const uint32_t u = __float_as_uint(v + 0.0f);
+    const uint32_t k = (v != v) ? 0u : ((u & 0x80000000u) ? ~u : (u | 0x80000000u));
+    return ((uint64_t)k << 32) | (uint32_t)(0xffffffffu - (uint32_t)idx);
+}
+__device__ __forceinline__ int key_index(uint64_t key) {
+    return (int)(0xffffffffu - (uint32_t)key);
+}
+// The value of a key made from a number.
+__device__ __forceinline__ float key_value(uint64_t key) {
+    const uint32_t k = (uint32_t)(key >> 32);
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+__device__ __forceinline__ uint64_t max_key(uint64_t a, uint64_t b) { return a > b ? a : b; }
+__device__ __forceinline__ uint64_t shfl_xor_key(uint64_t v, int o) {
+    const uint32_t lo = __shfl_xor((uint32_t)v, o), hi = __shfl_xor((uint32_t)(v >> 32), o);
+    return ((uint64_t)hi << 32) | lo;
 }
 
-// Best value and index of each 16x16 tile of the coarse map. A workgroup
-// covers a band of 16 map rows by 256 columns (16 tiles): each lane folds one
-// column of the band, then each group of 16 lanes folds into its tile.
+// LDS exchange between the lanes of one wave: completes this wave's LDS
+// writes before its later reads (the wave's own accesses stay in order).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Best entry key of each 16x16 tile of the coarse map. A workgroup covers a
+// band of 16 map rows by 256 columns (16 tiles): each lane folds one column
+// of the band, then each group of 16 lanes folds into its tile.
 __global__ __launch_bounds__(256) void tile_max_kernel(const float *__restrict__ map, int cw, int ch,
-                                                       int tiles_x, float *__restrict__ tscore,
-                                                       int *__restrict__ tidx) {
+                                                       int tiles_x, uint64_t *__restrict__ tkey) {
     const int x = blockIdx.x * 256 + threadIdx.x, ty = blockIdx.y;
-    float v = -INFINITY;
-    int idx = kNoIdx;
+    uint64_t best = 0;
     if (x < cw) {
-        const int y1 = min(16 * ty + 16, ch);
-        for (int y = 16 * ty; y < y1; y++) {
-            const float u = map[(size_t)y * cw + x];
-            if (better(u, y * cw + x, v, idx)) {
-                v = u;
-                idx = y * cw + x;
-            }
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int y = 16 * ty + r;
+            if (y < ch) best = max_key(best, entry_key(map[(size_t)y * cw + x], y * cw + x));
         }
     }
 #pragma unroll
-    for (int o = 1; o < 16; o <<= 1) {
-        const float u = __shfl_xor(v, o);
-        const int iu = __shfl_xor(idx, o);
-        if (better(u, iu, v, idx)) {
-            v = u;
-            idx = iu;
-        }
-    }
+    for (int o = 1; o < 16; o <<= 1) best = max_key(best, shfl_xor_key(best, o));
     const int tx = x >> 4;
-    if ((threadIdx.x & 15) == 0 && tx < tiles_x) {
-        tscore[ty * tiles_x + tx] = v;
-        tidx[ty * tiles_x + tx] = idx;
-    }
+    if ((threadIdx.x & 15) == 0 && tx < tiles_x) tkey[ty * tiles_x + tx] = best;
 }
 
-// Sorts chunk blockIdx.x (kChunk entries) of n (score, index) entries best
-// first, bitonically in LDS, and keeps its first `keep`: at
-// out[blockIdx.x * keep] or, with cand set (a single chunk), as the
-// candidate list cand = (index % cw, index / cw), cscore = score, with (0, 0)
-// and -inf for empty slots.
+// Sorts chunk blockIdx.x (kChunk keys) of n keys, largest first, bitonically
+// in LDS (stages whose pairs stay inside one wave's 128 keys sync only that
+// wave), and keeps its first `keep`: at out[blockIdx.x * keep] or, with cand
+// set (a single chunk), as the candidate list cand = (index % cw,
+// index / cw), cscore = map[index], with (0, 0) and -inf for empty slots.
 constexpr int kChunk = 1024;
 constexpr int kMaxCand = 256;
-__global__ __launch_bounds__(512) void topk_chunk_kernel(const float *__restrict__ sv_in,
-                                                         const int *__restrict__ si_in, int n,
-                                                         int keep, float *__restrict__ sv_out,
-                                                         int *__restrict__ si_out, int cw,
+__global__ __launch_bounds__(512) void topk_chunk_kernel(const uint64_t *__restrict__ in, int n,
+                                                         int keep, uint64_t *__restrict__ out,
+                                                         const float *__restrict__ map, int cw,
                                                          int *__restrict__ cand,
                                                          float *__restrict__ cscore) {
-    __shared__ float sv[kChunk];
-    __shared__ int si[kChunk];
+    __shared__ uint64_t sk[kChunk];
     const int base = blockIdx.x * kChunk, tid = threadIdx.x;
-    for (int i = tid; i < kChunk; i += 512) {
-        const int g = base + i;
-        sv[i] = g < n ? sv_in[g] : -INFINITY;
-        si[i] = g < n ? si_in[g] : kNoIdx;
-    }
+    for (int i = tid; i < kChunk; i += 512) sk[i] = base + i < n ? in[base + i] : 0ull;
     __syncthreads();
     for (int size = 2; size <= kChunk; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
             const int i = 2 * tid - (tid & (stride - 1)), j = i + stride;
-            const float a = sv[i], b = sv[j];
-            const int ia = si[i], ib = si[j];
-            const bool desc = (i & size) == 0;
-            if (desc ? better(b, ib, a, ia) : better(a, ia, b, ib)) {
-                sv[i] = b;
-                sv[j] = a;
-                si[i] = ib;
-                si[j] = ia;
+            const uint64_t a = sk[i], b = sk[j];
+            if (((i & size) == 0) ? (b > a) : (a > b)) {
+                sk[i] = b;
+                sk[j] = a;
             }
-            __syncthreads();
+            const int next = stride > 1 ? stride >> 1 : (size << 1) >> 1;
+            if (stride >= 128 || next >= 128) __syncthreads();
+            else wave_sync();
         }
     }
     for (int i = tid; i < keep; i += 512) {
+        const uint64_t key = sk[i];
         if (cand) {
-            const int id = si[i];
-            cand[2 * i] = id == kNoIdx ? 0 : id % cw;
-            cand[2 * i + 1] = id == kNoIdx ? 0 : id / cw;
-            cscore[i] = id == kNoIdx ? -INFINITY : sv[i];
+            const int id = key_index(key);
+            cand[2 * i] = key ? id % cw : 0;
+            cand[2 * i + 1] = key ? id / cw : 0;
+            cscore[i] = key ? map[id] : -INFINITY;
         } else {
-            sv_out[(size_t)blockIdx.x * keep + i] = sv[i];
-            si_out[(size_t)blockIdx.x * keep + i] = si[i];
+            out[(size_t)blockIdx.x * keep + i] = key;
         }
     }
 }
@@ -483,8 +480,16 @@ __global__ __launch_bounds__(256) void cnn_proposal_forward_kernel(
     }
     __syncthreads();
     const int lane = tid & 63, wave = tid >> 6, n = lane & 31, g = lane >> 5;
+    constexpr int kWq = kC * (kKK / 8);  // uint4 per layer's weights
     for (int l = 0; l < layers; l++) {
         const float b = bias[l * kC + n];
+        uint4 pre[(kWq + 255) / 256];
+        if (l + 1 < layers) {  // next layer's weights, loaded while the MFMAs run
+            const uint4 *wn = reinterpret_cast<const uint4 *>(w + (size_t)(l + 1) * kC * kKK);
+#pragma unroll
+            for (int q = 0; q < (kWq + 255) / 256; q++)
+                if (tid + 256 * q < kWq) pre[q] = wn[tid + 256 * q];
+        }
         v16f acc[2];
 #pragma unroll
         for (int t = 0; t < 2; t++) acc[t] = conv_tile(xs, ws, 2 * wave + t, n, g);
@@ -498,7 +503,14 @@ __global__ __launch_bounds__(256) void cnn_proposal_forward_kernel(
                     xs[(((pix >> 4) + 1) * kXP + (pix & 15) + 1) * kXPitch + n] =
                         to_bf16(fmaxf(acc[t][r] + b, 0.0f));
                 }
-            stage_weights(ws, w + (size_t)(l + 1) * kC * kKK, tid);
+#pragma unroll
+            for (int q = 0; q < (kWq + 255) / 256; q++) {
+                const int i = tid + 256 * q;
+                if (i < kWq) {
+                    const int r = i / (kKK / 8), c8 = i - r * (kKK / 8);
+                    *reinterpret_cast<uint4 *>(ws + r * kWPitch + 8 * c8) = pre[q];
+                }
+            }
         } else if (n == 0) {
 #pragma unroll
             for (int t = 0; t < 2; t++)
@@ -529,27 +541,27 @@ __device__ __forceinline__ float block_sum256(float v, float *red) {
 }
 
 // Per-patch half of the choice of the best logits. Entry (c, pix) has the
-// coarse place cand[c] - 8 + (pix % 16, pix / 16) and flat index
-// c * 256 + pix; it is selectable when its place is inside the map, its
-// logit is a number, and no other patch holds a better entry (larger logit;
-// equal: smaller flat index) for the same place. Writes patch c's n_top best
-// selectable entries best first to top_v / top_i[c * n_top + r] (index
-// kNoIdx past the last), and its softmax partials part[c] = (max logit, sum
-// of exp(logit - max)) over all 256 logits. Logit i is lg[i * stride].
+// coarse place cand[c] - 8 + (pix % 16, pix / 16) and index c * 256 + pix;
+// it is selectable when its place is inside the map, its logit is a number,
+// and no other patch holds an entry for the same place that comes first
+// (entry_key order). Writes patch c's n_top best selectable entry keys, best
+// first, to top[c * n_top + r] (0 past the last), and its softmax partials
+// part[c] = (max logit, sum of exp(logit - max)) over all 256 logits.
+// Logit i is lg[i * stride].
 __global__ __launch_bounds__(256) void logit_patch_kernel(const float *__restrict__ lg, int stride,
                                                           int k, const int *__restrict__ cand,
                                                           int cw, int ch, int n_top,
-                                                          float *__restrict__ top_v,
-                                                          int *__restrict__ top_i,
+                                                          uint64_t *__restrict__ top,
                                                           float2 *__restrict__ part) {
     __shared__ int cxy[2 * kMaxCand];
-    __shared__ int2 ent[kPix];
+    __shared__ uint64_t ent[kPix];
     __shared__ float red[4];
     const int c = blockIdx.x, pix = threadIdx.x;
     for (int i = pix; i < 2 * k; i += 256) cxy[i] = cand[i];
     __syncthreads();
     const int me = c * kPix + pix;
     const float v = lg[(size_t)me * stride];
+    const uint64_t mine = entry_key(v, me);
     const int x = cxy[2 * c] - 8 + (pix & 15), y = cxy[2 * c + 1] - 8 + (pix >> 4);
     bool sel = x >= 0 && x < cw && y >= 0 && y < ch && v == v;
     if (sel) {
@@ -557,107 +569,88 @@ __global__ __launch_bounds__(256) void logit_patch_kernel(const float *__restric
             const int px = x - (cxy[2 * c2] - 8), py = y - (cxy[2 * c2 + 1] - 8);
             if (c2 != c && (unsigned)px < 16u && (unsigned)py < 16u) {
                 const int o = c2 * kPix + py * kPS + px;
-                if (better(lg[(size_t)o * stride], o, v, me)) sel = false;
+                if (entry_key(lg[(size_t)o * stride], o) > mine) sel = false;
             }
         }
     }
-    ent[pix] = make_int2(__float_as_int(v), sel ? me : kNoIdx);
+    ent[pix] = sel ? mine : 0ull;
     const int nsel = __syncthreads_count(sel);
     if (sel) {
         int rank = 0;
-        for (int j = 0; j < kPix; j++) {
-            const int2 e = ent[j];
-            rank += better(__int_as_float(e.x), e.y, v, me);
-        }
-        if (rank < n_top) {
-            top_v[(size_t)c * n_top + rank] = v;
-            top_i[(size_t)c * n_top + rank] = me;
-        }
+#pragma unroll 8
+        for (int j = 0; j < kPix; j++) rank += ent[j] > mine;
+        if (rank < n_top) top[(size_t)c * n_top + rank] = mine;
     }
     for (int r = pix; r < n_top; r += 256)
-        if (r >= nsel) {
-            top_v[(size_t)c * n_top + r] = -INFINITY;
-            top_i[(size_t)c * n_top + r] = kNoIdx;
-        }
+        if (r >= nsel) top[(size_t)c * n_top + r] = 0ull;
     const float mx = block_max256(fmaxf(-INFINITY, v), red);
     const float se = block_sum256(expf(v - mx), red);
     if (pix == 0) part[c] = make_float2(mx, se);
 }
 
-// Second half: the n_top best entries of all patches' lists (no place is
-// selectable twice). The n_top-th best patch head T bounds them from below,
-// so only entries at least as good as T are ranked. centers[r] = coarse
-// place of the r-th best ((0, 0) past the last selectable entry); probs[r] =
+// Second half: the n_top best keys of all patches' lists (no place is
+// selectable twice). The n_top-th best patch head bounds them from below,
+// so only keys at least as large are ranked. centers[r] = coarse place of
+// the r-th best ((0, 0) past the last selectable entry); probs[r] =
 // exp(logit - max) / sum(exp(logit - max)) over all k * 256 logits (0 past
 // the last).
 constexpr int kMergeCap = 2048;
-__global__ __launch_bounds__(256) void logit_merge_kernel(const float *__restrict__ top_v,
-                                                          const int *__restrict__ top_i,
+__global__ __launch_bounds__(256) void logit_merge_kernel(const uint64_t *__restrict__ top,
                                                           const float2 *__restrict__ part, int k,
                                                           const int *__restrict__ cand, int n_top,
                                                           int *__restrict__ centers,
                                                           float *__restrict__ probs) {
-    __shared__ int2 head[kMaxCand];
-    __shared__ int2 sur[kMergeCap];
-    __shared__ int2 thr;
+    __shared__ uint64_t head[kMaxCand];
+    __shared__ uint64_t sur[kMergeCap];
+    __shared__ uint64_t thr;
     __shared__ int cnt;
-    __shared__ float norm[2];
+    __shared__ float red[4];
     const int tid = threadIdx.x, m = k * n_top;
+    const float2 pc = tid < k ? part[tid] : make_float2(-INFINITY, 0.0f);
+    const float mx = block_max256(pc.x, red);
+    const float se = block_sum256(pc.x != -INFINITY ? pc.y * expf(pc.x - mx) : 0.0f, red);
     if (tid == 0) {
-        thr = make_int2(0, kNoIdx);
+        thr = 0ull;
         cnt = 0;
-        float mx = -INFINITY;
-        for (int c = 0; c < k; c++) mx = fmaxf(mx, part[c].x);
-        float se = 0.0f;
-        for (int c = 0; c < k; c++)
-            if (part[c].x != -INFINITY) se += part[c].y * expf(part[c].x - mx);
-        norm[0] = mx;
-        norm[1] = se;
     }
-    for (int c = tid; c < k; c += 256)
-        head[c] = make_int2(__float_as_int(top_v[(size_t)c * n_top]), top_i[(size_t)c * n_top]);
+    if (tid < k) head[tid] = top[(size_t)tid * n_top];
     __syncthreads();
-    for (int c = tid; c < k; c += 256) {
-        const int2 h = head[c];
-        if (h.y == kNoIdx) continue;
+    if (tid < k && head[tid] != 0ull) {
+        const uint64_t h = head[tid];
         int rank = 0;
-        for (int j = 0; j < k; j++)
-            rank += better(__int_as_float(head[j].x), head[j].y, __int_as_float(h.x), h.y);
+        for (int j = 0; j < k; j++) rank += head[j] > h;
         if (rank == n_top - 1) thr = h;
     }
     __syncthreads();
-    const int2 T = thr;
+    const uint64_t T = thr;
     for (int i = tid; i < m; i += 256) {
-        const float v = top_v[i];
-        const int id = top_i[i];
-        if (id == kNoIdx || better(__int_as_float(T.x), T.y, v, id)) continue;
+        const uint64_t key = top[i];
+        if (key == 0ull || key < T) continue;
         const int s = atomicAdd(&cnt, 1);
-        if (s < kMergeCap) sur[s] = make_int2(__float_as_int(v), id);
+        if (s < kMergeCap) sur[s] = key;
     }
     __syncthreads();
     const int ns = cnt;
-    auto emit = [&](int rank, float v, int id) {
-        const int c = id / kPix, pix = id - c * kPix;
+    auto emit = [&](int rank, uint64_t key) {
+        const int id = key_index(key), c = id / kPix, pix = id - c * kPix;
         centers[2 * rank] = cand[2 * c] - 8 + (pix & 15);
         centers[2 * rank + 1] = cand[2 * c + 1] - 8 + (pix >> 4);
-        probs[rank] = expf(v - norm[0]) / norm[1];
+        probs[rank] = expf(key_value(key) - mx) / se;
     };
     if (ns <= kMergeCap) {
         for (int s = tid; s < ns; s += 256) {
-            const int2 e = sur[s];
-            const float v = __int_as_float(e.x);
+            const uint64_t e = sur[s];
             int rank = 0;
-            for (int j = 0; j < ns; j++) rank += better(__int_as_float(sur[j].x), sur[j].y, v, e.y);
-            if (rank < n_top) emit(rank, v, e.y);
+            for (int j = 0; j < ns; j++) rank += sur[j] > e;
+            if (rank < n_top) emit(rank, e);
         }
-    } else {  // many equal logits: rank every selectable entry against all
+    } else {  // many entries tie with the bound: rank every selectable key against all
         for (int i = tid; i < m; i += 256) {
-            const float v = top_v[i];
-            const int id = top_i[i];
-            if (id == kNoIdx) continue;
+            const uint64_t e = top[i];
+            if (e == 0ull) continue;
             int rank = 0;
-            for (int j = 0; j < m && rank < n_top; j++) rank += better(top_v[j], top_i[j], v, id);
-            if (rank < n_top) emit(rank, v, id);
+            for (int j = 0; j < m && rank < n_top; j++) rank += top[j] > e;
+            if (rank < n_top) emit(rank, e);
         }
     }
     for (int r = tid; r < n_top; r += 256)
@@ -883,12 +876,8 @@ CandPlan cand_plan(uint32_t cw, uint32_t ch) {
     p.t = p.tx * p.ty;
     p.nb = (p.t + kChunk - 1) / kChunk;
     WorkspacePlan wp;
-    wp.take<float>(std::max(p.t, 1));
-    wp.take<int>(std::max(p.t, 1));
-    for (int b = 0; b < 2; b++) {
-        wp.take<float>((size_t)std::max(p.nb, 1) * kMaxCand);
-        wp.take<int>((size_t)std::max(p.nb, 1) * kMaxCand);
-    }
+    wp.take<uint64_t>(std::max(p.t, 1));
+    for (int b = 0; b < 2; b++) wp.take<uint64_t>((size_t)std::max(p.nb, 1) * kMaxCand);
     p.bytes = wp.used;
     return p;
 }
@@ -902,42 +891,34 @@ iph_status launch_candidates(const float *coarse, uint32_t cw, uint32_t ch, int 
     }
     const CandPlan p = cand_plan(cw, ch);
     Workspace wsp(ws, ws_bytes);
-    float *ts = wsp.take<float>(std::max(p.t, 1));
-    int *ti = wsp.take<int>(std::max(p.t, 1));
-    float *lv[2];
-    int *li[2];
-    for (int b = 0; b < 2; b++) {
-        lv[b] = wsp.take<float>((size_t)std::max(p.nb, 1) * kMaxCand);
-        li[b] = wsp.take<int>((size_t)std::max(p.nb, 1) * kMaxCand);
-    }
-    if (!ts || !ti || !lv[1] || !li[1]) {
+    uint64_t *tk = wsp.take<uint64_t>(std::max(p.t, 1));
+    uint64_t *lk[2];
+    for (int b = 0; b < 2; b++) lk[b] = wsp.take<uint64_t>((size_t)std::max(p.nb, 1) * kMaxCand);
+    if (!tk || !lk[1]) {
         set_error("workspace too small for the candidate search");
         return IPH_EINVAL;
     }
     tile_max_kernel<<<dim3((cw + 255) / 256, p.ty), 256, 0, s>>>(coarse, (int)cw, (int)ch, p.tx,
-                                                                 ts, ti);
+                                                                 tk);
     IPH_TRY(launch_status("tile_max_kernel"));
-    const float *sv = ts;
-    const int *si = ti;
+    const uint64_t *src = tk;
     int n = p.t, b = 0;
     while (n > kChunk) {
         const int nb = (n + kChunk - 1) / kChunk;
-        topk_chunk_kernel<<<nb, 512, 0, s>>>(sv, si, n, k, lv[b], li[b], 0, nullptr, nullptr);
+        topk_chunk_kernel<<<nb, 512, 0, s>>>(src, n, k, lk[b], nullptr, 0, nullptr, nullptr);
         IPH_TRY(launch_status("topk_chunk_kernel"));
-        sv = lv[b];
-        si = li[b];
+        src = lk[b];
         n = nb * k;
         b ^= 1;
     }
-    topk_chunk_kernel<<<1, 512, 0, s>>>(sv, si, n, k, nullptr, nullptr, (int)cw, cand, cscore);
+    topk_chunk_kernel<<<1, 512, 0, s>>>(src, n, k, nullptr, coarse, (int)cw, cand, cscore);
     return launch_status("topk_chunk_kernel");
 }
 
 constexpr int kMaxTop = 64;
 size_t top_workspace(int k, int n_top) {
     WorkspacePlan wp;
-    wp.take<float>((size_t)std::max(k, 1) * std::max(n_top, 1));
-    wp.take<int>((size_t)std::max(k, 1) * std::max(n_top, 1));
+    wp.take<uint64_t>((size_t)std::max(k, 1) * std::max(n_top, 1));
     wp.take<float2>(std::max(k, 1));
     return wp.used;
 }
@@ -950,17 +931,15 @@ iph_status launch_top(const float *lg, int stride, int k, const int *cand, uint3
         return IPH_EINVAL;
     }
     Workspace wsp(ws, ws_bytes);
-    float *tv = wsp.take<float>((size_t)k * n_top);
-    int *ti = wsp.take<int>((size_t)k * n_top);
+    uint64_t *tk = wsp.take<uint64_t>((size_t)k * n_top);
     float2 *part = wsp.take<float2>(k);
-    if (!tv || !ti || !part) {
+    if (!tk || !part) {
         set_error("workspace too small for the top places");
         return IPH_EINVAL;
     }
-    logit_patch_kernel<<<k, 256, 0, s>>>(lg, stride, k, cand, (int)cw, (int)ch, n_top, tv, ti,
-                                         part);
+    logit_patch_kernel<<<k, 256, 0, s>>>(lg, stride, k, cand, (int)cw, (int)ch, n_top, tk, part);
     IPH_TRY(launch_status("logit_patch_kernel"));
-    logit_merge_kernel<<<1, 256, 0, s>>>(tv, ti, part, k, cand, n_top, centers, probs);
+    logit_merge_kernel<<<1, 256, 0, s>>>(tk, part, k, cand, n_top, centers, probs);
     return launch_status("logit_merge_kernel");
 }
 
