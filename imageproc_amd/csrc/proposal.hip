@@ -346,14 +346,6 @@ __device__ __forceinline__ uint64_t shfl_xor_key(uint64_t v, int o) {
     return ((uint64_t)hi << 32) | lo;
 }
 
-// LDS exchange between the lanes of one wave: completes this wave's LDS
-// writes before its later reads (the wave's own accesses stay in order).
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Best entry key of each 16x16 tile of the coarse map. A workgroup covers a
 // band of 16 map rows by 256 columns (16 tiles): each lane folds one column
 // of the band, then each group of 16 lanes folds into its tile.
@@ -374,46 +366,193 @@ __global__ __launch_bounds__(256) void tile_max_kernel(const float *__restrict__
     if ((threadIdx.x & 15) == 0 && tx < tiles_x) tkey[ty * tiles_x + tx] = best;
 }
 
-// Sorts chunk blockIdx.x (kChunk keys) of n keys, largest first, bitonically
-// in LDS (stages whose pairs stay inside one wave's 128 keys sync only that
-// wave), and keeps its first `keep`: at out[blockIdx.x * keep] or, with cand
-// set (a single chunk), as the candidate list cand = (index % cw,
-// index / cw), cscore = map[index], with (0, 0) and -inf for empty slots.
 constexpr int kChunk = 1024;
 constexpr int kMaxCand = 256;
-__global__ __launch_bounds__(512) void topk_chunk_kernel(const uint64_t *__restrict__ in, int n,
-                                                         int keep, uint64_t *__restrict__ out,
-                                                         const float *__restrict__ map, int cw,
-                                                         int *__restrict__ cand,
-                                                         float *__restrict__ cscore) {
-    __shared__ uint64_t sk[kChunk];
-    const int base = blockIdx.x * kChunk, tid = threadIdx.x;
-    for (int i = tid; i < kChunk; i += 512) sk[i] = base + i < n ? in[base + i] : 0ull;
-    __syncthreads();
+
+// Sorts the kChunk = 1024 keys of a 1024-thread workgroup (thread t holds
+// position t), largest first, with a bitonic network: pairs less than 64
+// positions apart are exchanged between the lanes of a wave (shuffles),
+// farther ones through LDS (xs: 2 x 1024 keys, alternating so one barrier
+// per such stage suffices).
+__device__ __forceinline__ uint64_t sort1024(uint64_t key, uint64_t *xs, int tid) {
+    int flip = 0;
     for (int size = 2; size <= kChunk; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            const int i = 2 * tid - (tid & (stride - 1)), j = i + stride;
-            const uint64_t a = sk[i], b = sk[j];
-            if (((i & size) == 0) ? (b > a) : (a > b)) {
-                sk[i] = b;
-                sk[j] = a;
+            uint64_t o;
+            if (stride >= 64) {
+                uint64_t *buf = xs + flip * kChunk;
+                flip ^= 1;
+                buf[tid] = key;
+                __syncthreads();
+                o = buf[tid ^ stride];
+            } else {
+                o = shfl_xor_key(key, stride);
             }
-            const int next = stride > 1 ? stride >> 1 : (size << 1) >> 1;
-            if (stride >= 128 || next >= 128) __syncthreads();
-            else wave_sync();
+            const bool keep_max = ((tid & stride) == 0) == ((tid & size) == 0);
+            key = keep_max ? max_key(key, o) : (key < o ? key : o);
         }
     }
-    for (int i = tid; i < keep; i += 512) {
-        const uint64_t key = sk[i];
-        if (cand) {
-            const int id = key_index(key);
-            cand[2 * i] = key ? id % cw : 0;
-            cand[2 * i + 1] = key ? id / cw : 0;
-            cscore[i] = key ? map[id] : -INFINITY;
-        } else {
-            out[(size_t)blockIdx.x * keep + i] = key;
+    return key;
+}
+
+// Chunk-wise top-k of n keys: workgroup b sorts keys [b * kChunk, ...) and
+// keeps its first `keep` at out[b * keep] or, with cand set (a single
+// chunk), writes them as the candidate list cand = (index % cw,
+// index / cw), cscore = map[index], with (0, 0) and -inf for empty slots.
+__global__ __launch_bounds__(kChunk) void topk_chunk_kernel(const uint64_t *__restrict__ in, int n,
+                                                            int keep, uint64_t *__restrict__ out,
+                                                            const float *__restrict__ map, int cw,
+                                                            int *__restrict__ cand,
+                                                            float *__restrict__ cscore) {
+    __shared__ uint64_t xs[2 * kChunk];
+    const int tid = threadIdx.x, g = blockIdx.x * kChunk + tid;
+    const uint64_t key = sort1024(g < n ? in[g] : 0ull, xs, tid);
+    if (tid >= keep) return;
+    if (cand) {
+        const int id = key_index(key);
+        cand[2 * tid] = key ? id % cw : 0;
+        cand[2 * tid + 1] = key ? id / cw : 0;
+        cscore[tid] = key ? map[id] : -INFINITY;
+    } else {
+        out[(size_t)blockIdx.x * keep + tid] = key;
+    }
+}
+
+// The coarse map and its tile keys in one pass, for downsampled templates
+// of at most kCoarseMax pixels a side (TWQ = words per template row). A
+// workgroup covers 16 coarse rows by 256 columns (16 tiles); a thread one
+// column of them. The workgroup downsamples the full template itself
+// (rounded 4x4 means, as downsample4_kernel), stages its part of the small
+// image in LDS as 4-byte words, and sums exactly: sum(i*t) by v_dot4
+// against the template rows (image words realigned to the column with
+// v_alignbyte), sum(i^2) from each staged row's v_dot4 with itself, both
+// reused by every output row the image row falls in. Scores come from
+// finish_score, as the full map's do.
+constexpr int kCoarseMax = 16;
+constexpr int kCoarseW = 68;  // staged words per image row: 256 + 15 columns, rounded up
+template <int METHOD, int TWQ>
+__global__ __launch_bounds__(256) void coarse_tiles_kernel(
+    const uint8_t *__restrict__ small, int sw, int sh, const uint8_t *__restrict__ tpl, int tw,
+    int th, int cw, int ch, int tiles_x, int vec, float *__restrict__ coarse,
+    uint64_t *__restrict__ tkey) {
+    __shared__ uint8_t tb[kCoarseMax * kCoarseMax];
+    __shared__ uint32_t ts[kCoarseMax * 4];
+    __shared__ uint32_t iw[(16 + kCoarseMax - 1) * kCoarseW];
+    __shared__ uint32_t pre[(16 + kCoarseMax) * 256];  // per-thread prefix sums of row energies
+    __shared__ long long red[4];
+    const int tid = threadIdx.x, x0 = blockIdx.x * 256, y0 = blockIdx.y * 16;
+    const int stw = tw >> 2, sth = th >> 2, R = 16 + sth - 1;
+    long long tsq = 0;
+    for (int i = tid; i < sth * stw; i += 256) {
+        const int r = i / stw, q = i - r * stw;
+        uint32_t sum = 0;
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            const uint8_t *row = tpl + (size_t)(4 * r + a) * tw + 4 * q;
+            sum += row[0] + row[1] + row[2] + row[3];
+        }
+        const uint32_t v = (sum + 8) >> 4;
+        tb[r * kCoarseMax + q] = (uint8_t)v;
+        tsq += v * v;
+    }
+    for (int i = tid; i < (16 + kCoarseMax - 1) * kCoarseW; i += 256) {
+        const int r = i / kCoarseW, q = i - r * kCoarseW;
+        const int gy = y0 + r, gx = x0 + 4 * q;
+        uint32_t wv = 0;
+        if (r < R && gy < sh) {
+            const uint8_t *row = small + (size_t)gy * sw;
+            if (vec && gx + 4 <= sw) {
+                wv = *reinterpret_cast<const uint32_t *>(row + gx);
+            } else {
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+                    if (gx + b < sw) wv |= (uint32_t)row[gx + b] << (8 * b);
+            }
+        }
+        iw[i] = wv;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) tsq += __shfl_xor(tsq, o);
+    if ((tid & 63) == 0) red[tid >> 6] = tsq;
+    __syncthreads();
+    if (tid < kCoarseMax * 4) {
+        const int r = tid >> 2, q = tid & 3;
+        uint32_t wv = 0;
+        if (r < sth)
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+                if (4 * q + b < stw) wv |= (uint32_t)tb[r * kCoarseMax + 4 * q + b] << (8 * b);
+        ts[tid] = wv;
+    }
+    __syncthreads();
+    const long long stt = red[0] + red[1] + red[2] + red[3];
+    // the template rows, the same for every lane: scalar registers (rows past
+    // sth are zero words, so they add nothing to sum(i*t))
+    uint32_t tsw[kCoarseMax][TWQ];
+#pragma unroll
+    for (int r = 0; r < kCoarseMax; r++)
+#pragma unroll
+        for (int q = 0; q < TWQ; q++) tsw[r][q] = __builtin_amdgcn_readfirstlane(ts[r * 4 + q]);
+    const int j0 = tid >> 2;
+    const uint32_t shift = (uint32_t)(tid & 3);
+    const uint32_t last = (stw & 3) ? (1u << (8 * (stw & 3))) - 1u : 0xffffffffu;
+    uint32_t sit[TWQ][16], sii[16];
+#pragma unroll
+    for (int q = 0; q < TWQ; q++)
+#pragma unroll
+        for (int j = 0; j < 16; j++) sit[q][j] = 0u;
+    // every (image row, template row) pair, without branches: staged rows
+    // past R are zero. sum(i^2) of output row j is pre[j + sth] - pre[j],
+    // pre[rr] the sum of the row energies above image row rr.
+    uint32_t run = 0;
+    pre[tid] = 0u;
+#pragma unroll
+    for (int rr = 0; rr < 16 + kCoarseMax - 1; rr++) {
+        const uint32_t *row = iw + rr * kCoarseW + j0;
+        uint32_t w4[TWQ];
+        uint32_t lo = row[0];
+#pragma unroll
+        for (int q = 0; q < TWQ; q++) {
+            const uint32_t hi = row[q + 1];
+            w4[q] = __builtin_amdgcn_alignbyte(hi, lo, shift);
+            lo = hi;
+        }
+        w4[TWQ - 1] &= last;
+        uint32_t e = 0;
+#pragma unroll
+        for (int q = 0; q < TWQ; q++) e = __builtin_amdgcn_udot4(w4[q], w4[q], e, false);
+        run += e;
+        pre[(rr + 1) * 256 + tid] = run;
+        if (rr < 16) sii[rr] = run - e;  // pre[rr]
+        // one accumulator per (output row, template word): independent chains
+#pragma unroll
+        for (int q = 0; q < TWQ; q++)
+#pragma unroll
+            for (int r = 0; r < kCoarseMax; r++) {
+                const int j = rr - r;
+                if (j >= 0 && j < 16)
+                    sit[q][j] = __builtin_amdgcn_udot4(w4[q], tsw[r][q], sit[q][j], false);
+            }
+    }
+#pragma unroll
+    for (int j = 0; j < 16; j++) sii[j] = pre[(j + sth) * 256 + tid] - sii[j];
+    const int X = x0 + tid;
+    uint64_t best = 0;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const int Y = y0 + j;
+        if (X < cw && Y < ch) {
+            long long c = 0;
+#pragma unroll
+            for (int q = 0; q < TWQ; q++) c += sit[q][j];
+            const float v = finish_score<METHOD>(c, (long long)sii[j], stt, (long long)stw * sth);
+            coarse[(size_t)Y * cw + X] = v;
+            best = max_key(best, entry_key(v, Y * cw + X));
         }
     }
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) best = max_key(best, shfl_xor_key(best, o));
+    if ((tid & 15) == 0 && (X >> 4) < tiles_x) tkey[blockIdx.y * tiles_x + (X >> 4)] = best;
 }
 
 // Network input channels 0-3 of pixel pix of the patch around candidate
@@ -460,63 +599,65 @@ __global__ __launch_bounds__(256) void features_kernel(const float *__restrict__
 // cnn_conv3x3_kernel per layer (ReLU between layers), so the logits are
 // identical to theirs. w: `layers` weight blocks of 32 x 288 bf16, bias:
 // `layers` x 32 f32.
-__global__ __launch_bounds__(256) void cnn_proposal_forward_kernel(
+constexpr int kFwdThreads = 512;  // 8 waves: one 32-pixel tile each
+__global__ __launch_bounds__(kFwdThreads) void cnn_proposal_forward_kernel(
     const float *__restrict__ map, int cw, int ch, const uint8_t *__restrict__ small, int sw,
     int sh, int half_t, const int *__restrict__ cand, const float *__restrict__ cscore,
     const uint16_t *__restrict__ w, const float *__restrict__ bias, int layers,
     float *__restrict__ logits) {
     __shared__ __align__(16) uint16_t xs[kXP * kXP * kXPitch];
     __shared__ __align__(16) uint16_t ws[kC * kWPitch];
+    constexpr int kWq = kC * (kKK / 8);  // uint4 per layer's weights
+    constexpr int kPre = (kWq + kFwdThreads - 1) / kFwdThreads;
     const int c = blockIdx.x, tid = threadIdx.x;
-    for (int i = tid; i < kXP * kXP * kXPitch / 8; i += 256)
+    for (int i = tid; i < kWq; i += kFwdThreads) {
+        const int r = i / (kKK / 8), c8 = i - r * (kKK / 8);
+        *reinterpret_cast<uint4 *>(ws + r * kWPitch + 8 * c8) = reinterpret_cast<const uint4 *>(w)[i];
+    }
+    for (int i = tid; i < kXP * kXP * kXPitch / 8; i += kFwdThreads)
         reinterpret_cast<uint4 *>(xs)[i] = make_uint4(0u, 0u, 0u, 0u);
-    stage_weights(ws, w, tid);
     __syncthreads();
-    {
+    if (tid < kPix) {
         const uint2 f = feature_pixel(map, cw, ch, small, sw, sh, half_t, cand[2 * c],
                                       cand[2 * c + 1], cscore[c], cscore[0], tid);
         const int px = tid & 15, py = tid >> 4;
         *reinterpret_cast<uint2 *>(xs + ((py + 1) * kXP + px + 1) * kXPitch) = f;
     }
     __syncthreads();
-    const int lane = tid & 63, wave = tid >> 6, n = lane & 31, g = lane >> 5;
-    constexpr int kWq = kC * (kKK / 8);  // uint4 per layer's weights
+    const int lane = tid & 63, mt = tid >> 6, n = lane & 31, g = lane >> 5;
     for (int l = 0; l < layers; l++) {
         const float b = bias[l * kC + n];
-        uint4 pre[(kWq + 255) / 256];
-        if (l + 1 < layers) {  // next layer's weights, loaded while the MFMAs run
-            const uint4 *wn = reinterpret_cast<const uint4 *>(w + (size_t)(l + 1) * kC * kKK);
-#pragma unroll
-            for (int q = 0; q < (kWq + 255) / 256; q++)
-                if (tid + 256 * q < kWq) pre[q] = wn[tid + 256 * q];
+        // next layer's weights, in flight during the MFMAs
+        static_assert(kPre == 3, "three uint4 of weights per thread");
+        uint4 pre0 = {}, pre1 = {}, pre2 = {};
+        if (l + 1 < layers) {
+            const uint4 *src = reinterpret_cast<const uint4 *>(w + (size_t)(l + 1) * kC * kKK);
+            pre0 = src[tid];
+            pre1 = src[tid + kFwdThreads];
+            pre2 = src[min(tid + 2 * kFwdThreads, kWq - 1)];
         }
-        v16f acc[2];
-#pragma unroll
-        for (int t = 0; t < 2; t++) acc[t] = conv_tile(xs, ws, 2 * wave + t, n, g);
+        const v16f acc = conv_tile(xs, ws, mt, n, g);
         __syncthreads();  // every wave has read xs and ws
         if (l + 1 < layers) {
 #pragma unroll
-            for (int t = 0; t < 2; t++)
+            for (int r = 0; r < 16; r++) {
+                const int pix = tile_pixel(mt, g, r);
+                xs[(((pix >> 4) + 1) * kXP + (pix & 15) + 1) * kXPitch + n] =
+                    to_bf16(fmaxf(acc[r] + b, 0.0f));
+            }
 #pragma unroll
-                for (int r = 0; r < 16; r++) {
-                    const int pix = tile_pixel(2 * wave + t, g, r);
-                    xs[(((pix >> 4) + 1) * kXP + (pix & 15) + 1) * kXPitch + n] =
-                        to_bf16(fmaxf(acc[t][r] + b, 0.0f));
-                }
-#pragma unroll
-            for (int q = 0; q < (kWq + 255) / 256; q++) {
-                const int i = tid + 256 * q;
+            for (int q = 0; q < kPre; q++) {
+                const int i = tid + kFwdThreads * q;
                 if (i < kWq) {
                     const int r = i / (kKK / 8), c8 = i - r * (kKK / 8);
-                    *reinterpret_cast<uint4 *>(ws + r * kWPitch + 8 * c8) = pre[q];
+                    *reinterpret_cast<uint4 *>(ws + r * kWPitch + 8 * c8) =
+                        q == 0 ? pre0 : (q == 1 ? pre1 : pre2);
                 }
             }
         } else if (n == 0) {
 #pragma unroll
-            for (int t = 0; t < 2; t++)
-#pragma unroll
-                for (int r = 0; r < 16; r++)
-                    logits[(size_t)c * kPix + tile_pixel(2 * wave + t, g, r)] = acc[t][r] + b;
+            for (int r = 0; r < 16; r++)
+                logits[(size_t)c * kPix + tile_pixel(mt, g, r)] = acc[r] + b;
         }
         __syncthreads();
     }
@@ -593,7 +734,7 @@ __global__ __launch_bounds__(256) void logit_patch_kernel(const float *__restric
 // so only keys at least as large are ranked. centers[r] = coarse place of
 // the r-th best ((0, 0) past the last selectable entry); probs[r] =
 // exp(logit - max) / sum(exp(logit - max)) over all k * 256 logits (0 past
-// the last).
+// the last). One workgroup of 256 threads.
 constexpr int kMergeCap = 2048;
 __global__ __launch_bounds__(256) void logit_merge_kernel(const uint64_t *__restrict__ top,
                                                           const float2 *__restrict__ part, int k,
@@ -674,6 +815,95 @@ constexpr int kRefSub = 15;
 constexpr int kRefMaxSide = 128;
 __host__ __device__ constexpr int ref_tpl_words(int tw) { return (tw + 3) >> 2; }
 __host__ __device__ constexpr int ref_img_words(int tw) { return ref_tpl_words(tw) + 5; }
+// (score, y, x) order of find_extremes' maximum: larger score, then smaller
+// y, then smaller x; NaN never wins.
+__device__ __forceinline__ bool ref_better(float a, int ay, int ax, float b, int by, int bx) {
+    if (a != a) return false;
+    if (b != b) return true;
+    if (a != b) return a > b;
+    return ay != by ? ay < by : ax < bx;
+}
+
+// Best place over every refined window and the best score of the places
+// more than `sep` pixels (Chebyshev distance) from it. result: x, y, best,
+// second; with p_out, *p_out = *p_top (1 without it). One workgroup of up
+// to 1024 threads.
+__global__ __launch_bounds__(1024) void refine_best_kernel(const float *__restrict__ scores,
+                                                           const int *__restrict__ centers,
+                                                           int n_c, int scale, int sep,
+                                                           const float *__restrict__ p_top,
+                                                           float *__restrict__ result,
+                                                           float *__restrict__ p_out) {
+    __shared__ float sv[16];
+    __shared__ int sy[16], sx[16];
+    const int n = n_c * kRefN, tid = threadIdx.x, nt = blockDim.x, lane = tid & 63,
+              wave = tid >> 6, nw = nt >> 6;
+    auto place = [&](int i, int &x, int &y) {
+        const int c = i / kRefN, pos = i - c * kRefN, dy = pos / kRefS;
+        x = centers[2 * c] * scale - kRefR + (pos - dy * kRefS);
+        y = centers[2 * c + 1] * scale - kRefR + dy;
+    };
+    float bv = __int_as_float(0x7fc00000);
+    int bx = 0, by = 0;
+    for (int i = tid; i < n; i += nt) {
+        const float v = scores[i];
+        if (v != v) continue;
+        int x, y;
+        place(i, x, y);
+        if (ref_better(v, y, x, bv, by, bx)) {
+            bv = v;
+            bx = x;
+            by = y;
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const float v = __shfl_xor(bv, o);
+        const int x = __shfl_xor(bx, o), y = __shfl_xor(by, o);
+        if (ref_better(v, y, x, bv, by, bx)) {
+            bv = v;
+            bx = x;
+            by = y;
+        }
+    }
+    if (lane == 0) {
+        sv[wave] = bv;
+        sx[wave] = bx;
+        sy[wave] = by;
+    }
+    __syncthreads();
+    bv = sv[0];
+    bx = sx[0];
+    by = sy[0];
+    for (int w = 1; w < nw; w++)
+        if (ref_better(sv[w], sy[w], sx[w], bv, by, bx)) {
+            bv = sv[w];
+            bx = sx[w];
+            by = sy[w];
+        }
+    float sec = -INFINITY;
+    for (int i = tid; i < n; i += nt) {
+        const float v = scores[i];
+        if (v != v) continue;
+        int x, y;
+        place(i, x, y);
+        if (abs(x - bx) > sep || abs(y - by) > sep) sec = fmaxf(sec, v);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) sec = fmaxf(sec, __shfl_xor(sec, o));
+    __syncthreads();
+    if (lane == 0) sv[wave] = sec;
+    __syncthreads();
+    if (tid == 0) {
+        for (int w = 0; w < nw; w++) sec = fmaxf(sec, sv[w]);
+        result[0] = (float)bx;
+        result[1] = (float)by;
+        result[2] = bv;
+        result[3] = sec;
+        if (p_out) *p_out = p_top ? *p_top : 1.0f;
+    }
+}
+
 template <int METHOD>
 __global__ __launch_bounds__(256) void refine_kernel(const uint8_t *__restrict__ img, int W, int H,
                                                      const uint8_t *__restrict__ tpl, int tw,
@@ -752,94 +982,6 @@ __global__ __launch_bounds__(256) void refine_kernel(const uint8_t *__restrict__
     }
 }
 
-// (score, y, x) order of find_extremes' maximum: larger score, then smaller
-// y, then smaller x; NaN never wins.
-__device__ __forceinline__ bool ref_better(float a, int ay, int ax, float b, int by, int bx) {
-    if (a != a) return false;
-    if (b != b) return true;
-    if (a != b) return a > b;
-    return ay != by ? ay < by : ax < bx;
-}
-
-// Best place over every refined window and the best score of the places
-// more than `sep` pixels (Chebyshev distance) from it. result: x, y, best,
-// second; with p_out, *p_out = *p_top (1 without it).
-__global__ __launch_bounds__(1024) void refine_best_kernel(const float *__restrict__ scores,
-                                                           const int *__restrict__ centers,
-                                                           int n_c, int scale, int sep,
-                                                           const float *__restrict__ p_top,
-                                                           float *__restrict__ result,
-                                                           float *__restrict__ p_out) {
-    __shared__ float sv[16];
-    __shared__ int sy[16], sx[16];
-    const int n = n_c * kRefN, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    auto place = [&](int i, int &x, int &y) {
-        const int c = i / kRefN, pos = i - c * kRefN, dy = pos / kRefS;
-        x = centers[2 * c] * scale - kRefR + (pos - dy * kRefS);
-        y = centers[2 * c + 1] * scale - kRefR + dy;
-    };
-    float bv = __int_as_float(0x7fc00000);
-    int bx = 0, by = 0;
-    for (int i = tid; i < n; i += 1024) {
-        const float v = scores[i];
-        if (v != v) continue;
-        int x, y;
-        place(i, x, y);
-        if (ref_better(v, y, x, bv, by, bx)) {
-            bv = v;
-            bx = x;
-            by = y;
-        }
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const float v = __shfl_xor(bv, o);
-        const int x = __shfl_xor(bx, o), y = __shfl_xor(by, o);
-        if (ref_better(v, y, x, bv, by, bx)) {
-            bv = v;
-            bx = x;
-            by = y;
-        }
-    }
-    if (lane == 0) {
-        sv[wave] = bv;
-        sx[wave] = bx;
-        sy[wave] = by;
-    }
-    __syncthreads();
-    bv = sv[0];
-    bx = sx[0];
-    by = sy[0];
-    for (int w = 1; w < 16; w++)
-        if (ref_better(sv[w], sy[w], sx[w], bv, by, bx)) {
-            bv = sv[w];
-            bx = sx[w];
-            by = sy[w];
-        }
-    float sec = -INFINITY;
-    for (int i = tid; i < n; i += 1024) {
-        const float v = scores[i];
-        if (v != v) continue;
-        int x, y;
-        place(i, x, y);
-        if (abs(x - bx) > sep || abs(y - by) > sep) sec = fmaxf(sec, v);
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) sec = fmaxf(sec, __shfl_xor(sec, o));
-    __syncthreads();
-    if (lane == 0) sv[wave] = sec;
-    __syncthreads();
-    if (tid == 0) {
-        for (int w = 1; w < 16; w++) sec = fmaxf(sec, sv[w]);
-        sec = fmaxf(sec, sv[0]);
-        result[0] = (float)bx;
-        result[1] = (float)by;
-        result[2] = bv;
-        result[3] = sec;
-        if (p_out) *p_out = p_top ? *p_top : 1.0f;
-    }
-}
-
 }  // namespace
 }  // namespace iph
 
@@ -863,8 +1005,8 @@ iph_status launch_downsample(const uint8_t *src, uint32_t w, uint32_t h, uint8_t
     return launch_status("downsample4_kernel");
 }
 
-// Candidate search workspace: tile maxima, then two buffers for the levels
-// of chunk-wise top-k.
+// Candidate search workspace: tile keys and two buffers for the levels of
+// chunk-wise top-k.
 struct CandPlan {
     int tx, ty, t, nb;
     size_t bytes;
@@ -882,13 +1024,37 @@ CandPlan cand_plan(uint32_t cw, uint32_t ch) {
     return p;
 }
 
-iph_status launch_candidates(const float *coarse, uint32_t cw, uint32_t ch, int k, int *cand,
-                             float *cscore, void *ws, size_t ws_bytes, hipStream_t s) {
-    if (cw == 0 || ch == 0 || (size_t)cw * ch > (size_t)0x7ffffffe || k <= 0 || k > kMaxCand) {
+iph_status check_candidates(uint32_t cw, uint32_t ch, int k) {
+    if (cw == 0 || ch == 0 || (size_t)cw * ch > (size_t)0x7ffffffe || k <= 0 || k > kMaxCand ||
+        (ch + 15) / 16 > 65535) {
         set_error("coarse map of %ux%u or %d candidates out of range (1 to %d)", cw, ch, k,
                   kMaxCand);
         return IPH_EINVAL;
     }
+    return IPH_OK;
+}
+
+// Levels of chunk-wise top-k over t tile keys down to the k-candidate list.
+iph_status launch_topk(const uint64_t *tk, int t, int k, const float *coarse, uint32_t cw,
+                       int *cand, float *cscore, uint64_t *lk0, uint64_t *lk1, hipStream_t s) {
+    uint64_t *lk[2] = {lk0, lk1};
+    const uint64_t *src = tk;
+    int n = t, b = 0;
+    while (n > kChunk) {
+        const int nb = (n + kChunk - 1) / kChunk;
+        topk_chunk_kernel<<<nb, kChunk, 0, s>>>(src, n, k, lk[b], nullptr, 0, nullptr, nullptr);
+        IPH_TRY(launch_status("topk_chunk_kernel"));
+        src = lk[b];
+        n = nb * k;
+        b ^= 1;
+    }
+    topk_chunk_kernel<<<1, kChunk, 0, s>>>(src, n, k, nullptr, coarse, (int)cw, cand, cscore);
+    return launch_status("topk_chunk_kernel");
+}
+
+iph_status launch_candidates(const float *coarse, uint32_t cw, uint32_t ch, int k, int *cand,
+                             float *cscore, void *ws, size_t ws_bytes, hipStream_t s) {
+    IPH_TRY(check_candidates(cw, ch, k));
     const CandPlan p = cand_plan(cw, ch);
     Workspace wsp(ws, ws_bytes);
     uint64_t *tk = wsp.take<uint64_t>(std::max(p.t, 1));
@@ -901,18 +1067,7 @@ iph_status launch_candidates(const float *coarse, uint32_t cw, uint32_t ch, int 
     tile_max_kernel<<<dim3((cw + 255) / 256, p.ty), 256, 0, s>>>(coarse, (int)cw, (int)ch, p.tx,
                                                                  tk);
     IPH_TRY(launch_status("tile_max_kernel"));
-    const uint64_t *src = tk;
-    int n = p.t, b = 0;
-    while (n > kChunk) {
-        const int nb = (n + kChunk - 1) / kChunk;
-        topk_chunk_kernel<<<nb, 512, 0, s>>>(src, n, k, lk[b], nullptr, 0, nullptr, nullptr);
-        IPH_TRY(launch_status("topk_chunk_kernel"));
-        src = lk[b];
-        n = nb * k;
-        b ^= 1;
-    }
-    topk_chunk_kernel<<<1, 512, 0, s>>>(src, n, k, nullptr, coarse, (int)cw, cand, cscore);
-    return launch_status("topk_chunk_kernel");
+    return launch_topk(tk, p.t, k, coarse, cw, cand, cscore, lk[0], lk[1], s);
 }
 
 constexpr int kMaxTop = 64;
@@ -1014,6 +1169,13 @@ iph_status search_plan(uint32_t w, uint32_t h, uint32_t tw, uint32_t th, int32_t
     p.cw = p.sw - p.stw + 1;
     p.ch = p.sh - p.sth + 1;
     return IPH_OK;
+}
+
+// The coarse map and tile keys come from one coarse_tiles_kernel pass when
+// the downsampled template is small enough, else from the int8 MFMA match
+// and tile_max_kernel.
+bool coarse_fused(const SearchPlan &p) {
+    return p.stw <= (uint32_t)kCoarseMax && p.sth <= (uint32_t)kCoarseMax;
 }
 }  // namespace
 
@@ -1158,12 +1320,16 @@ IPH_API size_t iph_dev_proposal_search_workspace(uint32_t w, uint32_t h, uint32_
         clear_error();
         return 256;
     }
+    const CandPlan cp = cand_plan(p.cw, p.ch);
     WorkspacePlan wp;
     wp.take<uint8_t>((size_t)p.sw * p.sh);
-    wp.take<uint8_t>((size_t)p.stw * p.sth);
     wp.take<float>((size_t)p.cw * p.ch);
-    wp.take<uint8_t>(mfma_match_workspace(IPH_U8, p.sw, p.sh, p.stw, p.sth, method));
-    wp.take<uint8_t>(cand_plan(p.cw, p.ch).bytes);
+    if (!coarse_fused(p)) {
+        wp.take<uint8_t>((size_t)p.stw * p.sth);
+        wp.take<uint8_t>(mfma_match_workspace(IPH_U8, p.sw, p.sh, p.stw, p.sth, method));
+    }
+    wp.take<uint64_t>(std::max(cp.t, 1));
+    for (int b = 0; b < 2; b++) wp.take<uint64_t>((size_t)std::max(cp.nb, 1) * kMaxCand);
     wp.take<int>(2 * (size_t)k);
     wp.take<float>(k);
     wp.take<float>((size_t)k * kPix);
@@ -1183,6 +1349,7 @@ IPH_API iph_status iph_dev_proposal_search(const uint8_t *image, uint32_t w, uin
     clear_error();
     SearchPlan p;
     IPH_TRY(search_plan(w, h, tw, th, method, k, n_top, p));
+    IPH_TRY(check_candidates(p.cw, p.ch, k));
     if ((weights == nullptr) != (bias == nullptr)) {
         set_error("weights and bias go together (both NULL: rank by coarse score)");
         return IPH_EINVAL;
@@ -1191,14 +1358,21 @@ IPH_API iph_status iph_dev_proposal_search(const uint8_t *image, uint32_t w, uin
         set_error("workspace too small for the proposal search");
         return IPH_EINVAL;
     }
+    const CandPlan cp = cand_plan(p.cw, p.ch);
+    const bool fused = coarse_fused(p);
     Workspace wsp(workspace, workspace_bytes);
     uint8_t *small = wsp.take<uint8_t>((size_t)p.sw * p.sh);
-    uint8_t *stpl = wsp.take<uint8_t>((size_t)p.stw * p.sth);
     float *coarse = wsp.take<float>((size_t)p.cw * p.ch);
-    const size_t mws_bytes = mfma_match_workspace(IPH_U8, p.sw, p.sh, p.stw, p.sth, method);
-    uint8_t *mws = wsp.take<uint8_t>(mws_bytes);
-    const size_t cws_bytes = cand_plan(p.cw, p.ch).bytes;
-    uint8_t *cws = wsp.take<uint8_t>(cws_bytes);
+    uint8_t *stpl = nullptr, *mws = nullptr;
+    size_t mws_bytes = 0;
+    if (!fused) {
+        stpl = wsp.take<uint8_t>((size_t)p.stw * p.sth);
+        mws_bytes = mfma_match_workspace(IPH_U8, p.sw, p.sh, p.stw, p.sth, method);
+        mws = wsp.take<uint8_t>(mws_bytes);
+    }
+    uint64_t *tk = wsp.take<uint64_t>(std::max(cp.t, 1));
+    uint64_t *lk[2];
+    for (int b = 0; b < 2; b++) lk[b] = wsp.take<uint64_t>((size_t)std::max(cp.nb, 1) * kMaxCand);
     int *cand = wsp.take<int>(2 * (size_t)k);
     float *cscore = wsp.take<float>(k);
     float *logits = wsp.take<float>((size_t)k * kPix);
@@ -1214,19 +1388,48 @@ IPH_API iph_status iph_dev_proposal_search(const uint8_t *image, uint32_t w, uin
     }
     hipStream_t s = S(stream);
     IPH_TRY(launch_downsample(image, w, h, small, s));
-    IPH_TRY(launch_downsample(tpl, tw, th, stpl, s));
-    IPH_TRY(launch_mfma_match(small, IPH_U8, p.sw, p.sh, stpl, p.stw, p.sth, method, coarse, mws,
-                              mws_bytes, s));
-    IPH_TRY(launch_candidates(coarse, p.cw, p.ch, k, cand, cscore, cws, cws_bytes, s));
+    if (fused) {
+        const dim3 grid((p.cw + 255) / 256, cp.ty);
+        const int vec = (p.sw % 4 == 0) && ((uintptr_t)small % 4 == 0);
+        const int twq = (int)(p.stw + 3) / 4;
+#define CT(M, Q)                                                                             \
+    coarse_tiles_kernel<M, Q><<<grid, 256, 0, s>>>(small, (int)p.sw, (int)p.sh, tpl, (int)tw, \
+                                                   (int)th, (int)p.cw, (int)p.ch, cp.tx, vec,  \
+                                                   coarse, tk)
+#define CTM(M)                                                                               \
+    switch (twq) {                                                                           \
+    case 1: CT(M, 1); break;                                                                 \
+    case 2: CT(M, 2); break;                                                                 \
+    case 3: CT(M, 3); break;                                                                 \
+    default: CT(M, 4); break;                                                                \
+    }
+        if (method == IPH_CCORR) {
+            CTM(IPH_CCORR)
+        } else {
+            CTM(IPH_CCORR_NORMED)
+        }
+#undef CTM
+#undef CT
+        IPH_TRY(launch_status("coarse_tiles_kernel"));
+    } else {
+        IPH_TRY(launch_downsample(tpl, tw, th, stpl, s));
+        IPH_TRY(launch_mfma_match(small, IPH_U8, p.sw, p.sh, stpl, p.stw, p.sth, method, coarse,
+                                  mws, mws_bytes, s));
+        tile_max_kernel<<<dim3((p.cw + 255) / 256, cp.ty), 256, 0, s>>>(
+            coarse, (int)p.cw, (int)p.ch, cp.tx, tk);
+        IPH_TRY(launch_status("tile_max_kernel"));
+    }
+    IPH_TRY(launch_topk(tk, cp.t, k, coarse, p.cw, cand, cscore, lk[0], lk[1], s));
     const int *centers = cand;
     const float *p_top = nullptr;
     if (weights) {
-        cnn_proposal_forward_kernel<<<k, 256, 0, s>>>(coarse, (int)p.cw, (int)p.ch, small,
+        cnn_proposal_forward_kernel<<<k, kFwdThreads, 0, s>>>(coarse, (int)p.cw, (int)p.ch, small,
                                                       (int)p.sw, (int)p.sh, (int)p.stw / 2, cand,
                                                       cscore, weights, bias, IPH_CNN_LAYERS,
                                                       logits);
         IPH_TRY(launch_status("cnn_proposal_forward_kernel"));
-        IPH_TRY(launch_top(logits, 1, k, cand, p.cw, p.ch, n_top, ctr, probs, tws, tws_bytes, s));
+        IPH_TRY(launch_top(logits, 1, k, cand, p.cw, p.ch, n_top, ctr, probs, tws, tws_bytes,
+                           s));
         centers = ctr;
         p_top = probs;
     }

@@ -177,3 +177,22 @@ def test_one_call_search_equals_step_search(side):
         for use_net in (True, False):
             assert srch.search_raw(img, tpl, use_net) == srch.search_steps(img, tpl, use_net)
         assert plain.search_raw(img, tpl, False) == plain.search_steps(img, tpl, False)
+
+
+@pytest.mark.parametrize("tsize", [(8, 8), (20, 24), (60, 17), (67, 67), (96, 96), (128, 100)])
+def test_one_call_search_equals_step_search_across_template_sizes(tsize):
+    """Small templates take the one-pass coarse kernel (1 to 4 template words
+    per downsampled row), large ones the int8 MFMA map; both must equal the
+    step-wise search value for value."""
+    th, tw = tsize
+    rng = np.random.default_rng(th * 1000 + tw)
+    img_np = O.gaussian_blur_f32(rng.integers(0, 256, (700, 1030), dtype=np.uint8), 3.0)
+    img = torch.from_numpy(img_np).cuda()
+    net = ProposalNet.load()
+    for method in (M.CrossCorrelationNormalized, M.CrossCorrelation):
+        srch = ProposalSearch(net, method)
+        for _ in range(3):
+            y, x = int(rng.integers(0, 700 - th)), int(rng.integers(0, 1030 - tw))
+            tpl = img[y:y + th, x:x + tw].contiguous()
+            for use_net in (True, False):
+                assert srch.search_raw(img, tpl, use_net) == srch.search_steps(img, tpl, use_net)
