@@ -513,9 +513,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--leg", choices=("all", "blur", "copy"), default="all",
+    ap.add_argument("--leg", choices=("all", "blur", "copy", "proposal"), default="all",
                     help="blur: only the blur batch (for counter runs); copy: only a plain "
-                         "copy of the same size (counter calibration)")
+                         "copy of the same size (counter calibration); proposal: only the "
+                         "learned-proposal search against the full map")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X and has no "
@@ -527,6 +528,9 @@ def main():
     import imageproc_amd as ip
 
     print(f"[bench] device: {ip.device_info()}", flush=True)
+    if args.leg == "proposal":
+        print(json.dumps({"leg": "proposal", **bench_proposal()}), flush=True)
+        return
     if args.leg == "copy":
         ms, gbs = bench_copy(args.steps, args.warmup)
         print(json.dumps({"leg": "copy", "bytes": 2 * BLUR_N * BLUR_W * BLUR_H,

@@ -38,6 +38,7 @@ constexpr int kXT = 4;         // 32-column output tiles per wave
 constexpr int kMaxSide = 128;  // largest template side on the int8 path
 constexpr int kWinRows = 32;   // output rows per window-sum workgroup
 constexpr int kWinCols = 64;   // output columns per window-sum workgroup
+constexpr int kHsPitch = kWinCols + 1;  // LDS row pitch of the horizontal sums (odd: no bank clash)
 
 __device__ __forceinline__ int align_up_dev(int v) { return (v + 15) & ~15; }
 
@@ -94,7 +95,7 @@ __global__ __launch_bounds__(256) void window_sums_u8_kernel(const uint8_t *__re
     const int cols = kWinCols + tw - 1, rows = kWinRows + th - 1;
     const int H = oh + th - 1;
     uint8_t *blk = smem;                                                         // rows x cols
-    uint2 *hs = reinterpret_cast<uint2 *>(smem + align_up_dev(rows * cols));     // rows x kWinCols
+    uint2 *hs = reinterpret_cast<uint2 *>(smem + align_up_dev(rows * cols));     // rows x kHsPitch
     for (int base = threadIdx.x; base < rows * cols; base += 16 * 256) {
         uint8_t v[16];
 #pragma unroll
@@ -122,7 +123,7 @@ __global__ __launch_bounds__(256) void window_sums_u8_kernel(const uint8_t *__re
             a += v;
             b += v * v;
         }
-        uint2 *o = hs + r * kWinCols + 16 * q;
+        uint2 *o = hs + r * kHsPitch + 16 * q;
         o[0] = make_uint2(a, b);
 #pragma unroll
         for (int x = 1; x < 16; x++) {
@@ -138,15 +139,15 @@ __global__ __launch_bounds__(256) void window_sums_u8_kernel(const uint8_t *__re
     const int ya = seg * kSeg;
     uint32_t a = 0, b = 0;
     for (int dy = 0; dy < th; dy++) {
-        uint2 v = hs[(ya + dy) * kWinCols + x];
+        uint2 v = hs[(ya + dy) * kHsPitch + x];
         a += v.x;
         b += v.y;
     }
     for (int y = ya; y < ya + kSeg; y++) {
         const int gy = y0 + y, gx = x0 + x;
         if (gy < oh && gx < ow) ws[(size_t)gy * ow + gx] = make_uint2(a, b);
-        uint2 vin = hs[(y + th) * kWinCols + x < rows * kWinCols ? (y + th) * kWinCols + x : 0];
-        uint2 vout = hs[y * kWinCols + x];
+        uint2 vin = hs[y + th < rows ? (y + th) * kHsPitch + x : 0];
+        uint2 vout = hs[y * kHsPitch + x];
         a += vin.x - vout.x;
         b += vin.y - vout.y;
     }
@@ -517,7 +518,7 @@ int blocks_for(int tw) { return (31 + tw + 31) / 32; }
 
 size_t win_lds(int tw, int th) {
     int cols = kWinCols + tw - 1, rows = kWinRows + th - 1;
-    return align_up((size_t)rows * cols, 16) + (size_t)rows * kWinCols * sizeof(uint2);
+    return align_up((size_t)rows * cols, 16) + (size_t)rows * kHsPitch * sizeof(uint2);
 }
 
 thread_local int g_match_rows_hint = 0;
