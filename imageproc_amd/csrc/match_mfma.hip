@@ -34,13 +34,14 @@ namespace {
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
-constexpr int kXT = 4;         // 32-column output tiles per wave
 constexpr int kMaxSide = 128;  // largest template side on the int8 path
-constexpr int kWinRows = 32;   // output rows per window-sum workgroup
+constexpr int kWinRows = 64;   // output rows per window-sum workgroup
 constexpr int kWinCols = 64;   // output columns per window-sum workgroup
 constexpr int kHsPitch = kWinCols + 1;  // LDS row pitch of the horizontal sums (odd: no bank clash)
 
 __device__ __forceinline__ int align_up_dev(int v) { return (v + 15) & ~15; }
+// Staged bytes per row of the window-sum tile, a multiple of 4.
+__host__ __device__ __forceinline__ int win_cols(int tw) { return (kWinCols + tw - 1 + 3) & ~3; }
 
 // sum T and sum T^2 of the template (exact, 64-bit).
 __global__ __launch_bounds__(256) void template_sums_kernel(const uint8_t *__restrict__ tpl,
@@ -82,47 +83,64 @@ __global__ __launch_bounds__(64) void toeplitz_i8_kernel(const uint8_t *__restri
 }
 
 // Exact window sums (sum I, sum I^2) for every output position of a
-// kWinCols x kWinRows output tile: each staged image row gets its horizontal
-// tw-window sums by a sliding pass (one thread per 16-column segment of a
-// row), then each output column its vertical th-window sums over those (one
-// thread per column and 8-row segment), written coalesced. All arithmetic is
-// exact u32.
+// kWinCols x kWinRows output tile. The staged window (rows 4-byte aligned,
+// zeros past the image: those bytes only reach discarded outputs) gets its
+// horizontal tw-window sums per 16-column segment of a row: the first by
+// v_dot4 over whole words, then 15 sliding steps. Each output column then
+// slides its vertical th-window sum down a 16-row segment. All arithmetic
+// is exact u32; written coalesced.
 __global__ __launch_bounds__(256) void window_sums_u8_kernel(const uint8_t *__restrict__ img,
                                                              int W, int tw, int th, int ow,
                                                              int oh, uint2 *__restrict__ ws) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int x0 = blockIdx.x * kWinCols, y0 = blockIdx.y * kWinRows;
-    const int cols = kWinCols + tw - 1, rows = kWinRows + th - 1;
+    const int cols = win_cols(tw), cw4 = cols >> 2, rows = kWinRows + th - 1;
     const int H = oh + th - 1;
-    uint8_t *blk = smem;                                                         // rows x cols
-    uint2 *hs = reinterpret_cast<uint2 *>(smem + align_up_dev(rows * cols));     // rows x kHsPitch
-    for (int base = threadIdx.x; base < rows * cols; base += 16 * 256) {
-        uint8_t v[16];
+    const bool aligned = (W & 3) == 0 && (reinterpret_cast<uintptr_t>(img) & 3) == 0;
+    uint32_t *blk = reinterpret_cast<uint32_t *>(smem);                      // rows x cw4 words
+    uint2 *hs = reinterpret_cast<uint2 *>(smem + align_up_dev(rows * cols));  // rows x kHsPitch
+    for (int base = threadIdx.x; base < rows * cw4; base += 8 * 256) {
+        uint32_t v[8];
 #pragma unroll
-        for (int b = 0; b < 16; b++) {
-            const int i = base + 256 * b;
-            const int r = i / cols, c = i - r * cols;
-            const int gy = min(y0 + r, H - 1), gx = min(x0 + c, W - 1);
-            v[b] = img[(size_t)gy * W + gx];
+        for (int b = 0; b < 8; b++) {
+            const int i = min(base + 256 * b, rows * cw4 - 1);
+            const int r = i / cw4, gx = x0 + 4 * (i - r * cw4), gy = y0 + r;
+            uint32_t wv = 0;
+            if (gy < H) {
+                const uint8_t *row = img + (size_t)gy * W;
+                if (aligned && gx + 4 <= W) {
+                    wv = *reinterpret_cast<const uint32_t *>(row + gx);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 4; k++)
+                        if (gx + k < W) wv |= (uint32_t)row[gx + k] << (8 * k);
+                }
+            }
+            v[b] = wv;
         }
 #pragma unroll
-        for (int b = 0; b < 16; b++) {
-            const int i = base + 256 * b;
-            if (i < rows * cols) blk[i] = v[b];
-        }
+        for (int b = 0; b < 8; b++)
+            if (base + 256 * b < rows * cw4) blk[base + 256 * b] = v[b];
     }
     __syncthreads();
-    // horizontal sums: one thread per 16-column segment of a staged row
     constexpr int kSegs = kWinCols / 16;
+    const int nw = tw >> 2;
+    const uint32_t tail = (tw & 3) ? (1u << (8 * (tw & 3))) - 1u : 0u;
     for (int it = threadIdx.x; it < rows * kSegs; it += 256) {
         const int r = it / kSegs, q = it - r * kSegs;
-        const uint8_t *p = blk + r * cols + 16 * q;
+        const uint32_t *pw = blk + r * cw4 + 4 * q;
         uint32_t a = 0, b = 0;
-        for (int dx = 0; dx < tw; dx++) {
-            uint32_t v = p[dx];
-            a += v;
-            b += v * v;
+        for (int k = 0; k < nw; k++) {
+            const uint32_t wv = pw[k];
+            a = __builtin_amdgcn_udot4(wv, 0x01010101u, a, false);
+            b = __builtin_amdgcn_udot4(wv, wv, b, false);
         }
+        if (tail) {
+            const uint32_t wv = pw[nw] & tail;
+            a = __builtin_amdgcn_udot4(wv, 0x01010101u, a, false);
+            b = __builtin_amdgcn_udot4(wv, wv, b, false);
+        }
+        const uint8_t *p = reinterpret_cast<const uint8_t *>(pw);
         uint2 *o = hs + r * kHsPitch + 16 * q;
         o[0] = make_uint2(a, b);
 #pragma unroll
@@ -153,18 +171,18 @@ __global__ __launch_bounds__(256) void window_sums_u8_kernel(const uint8_t *__re
     }
 }
 
-// One wave computes a 32 x (32*kXT) output strip; RT waves stack vertically.
-template <int KB, int METHOD>
+// One wave computes a 32 x (32*XT) output strip; RT waves stack vertically.
+template <int KB, int METHOD, int XT>
 __global__ __launch_bounds__(256) void match_i8_mfma_kernel(
     const uint8_t *__restrict__ img, int W, int H, const v4i *__restrict__ bsw,
     const uint2 *__restrict__ wsum, const long long *__restrict__ tsums,
     float *__restrict__ out, int ow, int oh, int tw, int th, int pitch) {
     extern __shared__ __align__(16) unsigned char smem[];
     int8_t *blk = reinterpret_cast<int8_t *>(smem);
-    constexpr int BW = 32 * (kXT + KB - 1);  // bytes per staged row
+    constexpr int BW = 32 * (XT + KB - 1);  // bytes per staged row
     const int RT = blockDim.x >> 6;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int x0 = blockIdx.x * (32 * kXT), y0 = blockIdx.y * 32 * RT;
+    const int x0 = blockIdx.x * (32 * XT), y0 = blockIdx.y * 32 * RT;
     const int rows = 32 * RT + th - 1;
 
     // Stage image rows y0.. and columns x0.. as i8 = u8 - 128. Bytes outside
@@ -201,9 +219,9 @@ __global__ __launch_bounds__(256) void match_i8_mfma_kernel(
     __syncthreads();
 
     const int n = lane & 31, g = lane >> 5;
-    v16i acc[kXT];
+    v16i acc[XT];
 #pragma unroll
-    for (int t = 0; t < kXT; t++)
+    for (int t = 0; t < XT; t++)
 #pragma unroll
         for (int r = 0; r < 16; r++) acc[t][r] = 0;
 
@@ -218,10 +236,10 @@ __global__ __launch_bounds__(256) void match_i8_mfma_kernel(
 #pragma unroll
         for (int kb = 0; kb < KB; kb++) bn[kb] = bp[(dn * KB + kb) * 64];
 #pragma unroll
-        for (int s = 0; s < kXT + KB - 1; s++) {
+        for (int s = 0; s < XT + KB - 1; s++) {
             v4i a = *reinterpret_cast<const v4i *>(arow + 32 * s);
 #pragma unroll
-            for (int t = 0; t < kXT; t++) {
+            for (int t = 0; t < XT; t++) {
                 if (s - t >= 0 && s - t < KB)
                     acc[t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b[s - t], acc[t], 0, 0, 0);
             }
@@ -235,7 +253,7 @@ __global__ __launch_bounds__(256) void match_i8_mfma_kernel(
     const long long nn = (long long)tw * th;
     const long long base = 128LL * st - 16384LL * nn;
 #pragma unroll
-    for (int t = 0; t < kXT; t++) {
+    for (int t = 0; t < XT; t++) {
         const int x = x0 + 32 * t + n;
         // all 16 window-sum loads of this column are issued before any use
         // (clamped addresses, so none is conditional)
@@ -517,26 +535,26 @@ int blocks16_for(int tw) { return (31 + tw + 15) / 16; }
 int blocks_for(int tw) { return (31 + tw + 31) / 32; }
 
 size_t win_lds(int tw, int th) {
-    int cols = kWinCols + tw - 1, rows = kWinRows + th - 1;
+    int cols = win_cols(tw), rows = kWinRows + th - 1;
     return align_up((size_t)rows * cols, 16) + (size_t)rows * kHsPitch * sizeof(uint2);
 }
 
 thread_local int g_match_rows_hint = 0;
 
-template <int KB>
-iph_status launch_i8_kb(const uint8_t *img, int W, int H, const int8_t *bsw, const uint2 *wsum,
-                        const long long *tsums, float *out, int ow, int oh, int tw, int th,
-                        int method, hipStream_t s) {
-    constexpr int BW = 32 * (kXT + KB - 1);
+template <int KB, int XT>
+iph_status launch_i8_kb_xt(const uint8_t *img, int W, int H, const int8_t *bsw,
+                           const uint2 *wsum, const long long *tsums, float *out, int ow, int oh,
+                           int tw, int th, int method, hipStream_t s) {
+    constexpr int BW = 32 * (XT + KB - 1);
     int pitch = ((BW / 16) % 2 == 0) ? BW + 16 : BW;  // odd multiple of 16: no bank conflicts
     int RT = g_match_rows_hint > 0 ? std::min(g_match_rows_hint, 4) : 4;
     while (RT > 1 && (size_t)(32 * RT + th - 1) * pitch > 64 * 1024) RT /= 2;
     size_t lds = (size_t)(32 * RT + th - 1) * pitch;
-    dim3 grid((ow + 32 * kXT - 1) / (32 * kXT), (oh + 32 * RT - 1) / (32 * RT));
+    dim3 grid((ow + 32 * XT - 1) / (32 * XT), (oh + 32 * RT - 1) / (32 * RT));
     const v4i *b = reinterpret_cast<const v4i *>(bsw);
 #define L(M)                                                                                 \
-    match_i8_mfma_kernel<KB, M><<<grid, 64 * RT, lds, s>>>(img, W, H, b, wsum, tsums, out, ow, \
-                                                           oh, tw, th, pitch)
+    match_i8_mfma_kernel<KB, M, XT><<<grid, 64 * RT, lds, s>>>(img, W, H, b, wsum, tsums, out, \
+                                                               ow, oh, tw, th, pitch)
     switch (method) {
     case IPH_SSE: L(IPH_SSE); break;
     case IPH_SSE_NORMED: L(IPH_SSE_NORMED); break;
@@ -545,6 +563,30 @@ iph_status launch_i8_kb(const uint8_t *img, int W, int H, const int8_t *bsw, con
     }
 #undef L
     return launch_status("match_i8_mfma_kernel");
+}
+
+// 32-column output tiles per wave: two (each A operand feeds more MFMAs)
+// unless the map has enough 32x32 tiles to give every SIMD (1024 on the
+// chip) 16 waves of one tile each; the kernel is latency-bound, so waves
+// beat reuse there (8192x8192 with 48x48: 1.18 ms vs 1.23 ms vs 1.35 ms for
+// 1, 2, 4 tiles).
+int match_xt(int ow, int oh) {
+    const long long tiles = (long long)((oh + 31) / 32) * ((ow + 31) / 32);
+    return tiles >= 16 * 1024 ? 1 : 2;
+}
+
+template <int KB>
+iph_status launch_i8_kb(const uint8_t *img, int W, int H, const int8_t *bsw, const uint2 *wsum,
+                        const long long *tsums, float *out, int ow, int oh, int tw, int th,
+                        int method, hipStream_t s) {
+    switch (match_xt(ow, oh)) {
+    case 2:
+        return launch_i8_kb_xt<KB, 2>(img, W, H, bsw, wsum, tsums, out, ow, oh, tw, th, method,
+                                      s);
+    default:
+        return launch_i8_kb_xt<KB, 1>(img, W, H, bsw, wsum, tsums, out, ow, oh, tw, th, method,
+                                      s);
+    }
 }
 
 struct U8Plan {
