@@ -43,45 +43,6 @@ __device__ __forceinline__ int align_up_dev(int v) { return (v + 15) & ~15; }
 // Staged bytes per row of the window-sum tile, a multiple of 4.
 __host__ __device__ __forceinline__ int win_cols(int tw) { return (kWinCols + tw - 1 + 3) & ~3; }
 
-// sum T and sum T^2 of the template (exact, 64-bit).
-__global__ __launch_bounds__(256) void template_sums_kernel(const uint8_t *__restrict__ tpl,
-                                                            int n, long long *__restrict__ out) {
-    __shared__ long long s1[256], s2[256];
-    long long a = 0, b = 0;
-    for (int i = threadIdx.x; i < n; i += 256) {
-        long long t = tpl[i];
-        a += t;
-        b += t * t;
-    }
-    s1[threadIdx.x] = a;
-    s2[threadIdx.x] = b;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            s1[threadIdx.x] += s1[threadIdx.x + s];
-            s2[threadIdx.x] += s2[threadIdx.x + s];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        out[0] = s1[0];
-        out[1] = s2[0];
-    }
-}
-
-// Toeplitz operands in MFMA lane order: entry [dy][kb][lane] holds the 16
-// bytes lane `lane` feeds as B: column n = lane % 32, rows k = 16*(lane/32)+j.
-__global__ __launch_bounds__(64) void toeplitz_i8_kernel(const uint8_t *__restrict__ tpl, int tw,
-                                                         int KB, int8_t *__restrict__ bsw) {
-    const int dy = blockIdx.y, kb = blockIdx.x, lane = threadIdx.x;
-    const int n = lane & 31, g = lane >> 5;
-    int8_t *dst = bsw + ((size_t)(dy * KB + kb) * 64 + lane) * 16;
-    for (int j = 0; j < 16; j++) {
-        int dx = 32 * kb + 16 * g + j - n;
-        dst[j] = (dx >= 0 && dx < tw) ? (int8_t)((int)tpl[dy * tw + dx] - 128) : (int8_t)0;
-    }
-}
-
 // Exact window sums (sum I, sum I^2) for every output position of a
 // kWinCols x kWinRows output tile. The staged window (rows 4-byte aligned,
 // zeros past the image: those bytes only reach discarded outputs) gets its
@@ -171,13 +132,22 @@ __global__ __launch_bounds__(256) void window_sums_u8_kernel(const uint8_t *__re
     }
 }
 
+// Bytes per template row staged in LDS: 32 zeros, the row, zeros up to
+// 32 * (KB + 1) (every Toeplitz slice of a lane reads inside it).
+__host__ __device__ constexpr int tpl_pitch(int KB) { return 32 * KB + 32; }
+
 // One wave computes a 32 x (32*XT) output strip; RT waves stack vertically.
+// The workgroup stages the template (as i8, zero-padded rows) next to its
+// image rows; each lane cuts its Toeplitz B operand for template row dy and
+// K block kb, B[k][n] = T[dy][32kb + k - n], out of the padded row with
+// five dword reads and v_alignbyte, and the workgroup sums T and T^2 itself.
 template <int KB, int METHOD, int XT>
 __global__ __launch_bounds__(256) void match_i8_mfma_kernel(
-    const uint8_t *__restrict__ img, int W, int H, const v4i *__restrict__ bsw,
-    const uint2 *__restrict__ wsum, const long long *__restrict__ tsums,
-    float *__restrict__ out, int ow, int oh, int tw, int th, int pitch) {
+    const uint8_t *__restrict__ img, int W, int H, const uint8_t *__restrict__ tpl,
+    const uint2 *__restrict__ wsum, float *__restrict__ out, int ow, int oh, int tw, int th,
+    int pitch) {
     extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ long long tred[2][4];
     int8_t *blk = reinterpret_cast<int8_t *>(smem);
     constexpr int BW = 32 * (XT + KB - 1);  // bytes per staged row
     const int RT = blockDim.x >> 6;
@@ -216,6 +186,33 @@ __global__ __launch_bounds__(256) void match_i8_mfma_kernel(
             }
         }
     }
+    // template rows as i8 = u8 - 128 at [32, 32 + tw) of kTP-byte rows
+    constexpr int kTP = tpl_pitch(KB);
+    uint32_t *tp = reinterpret_cast<uint32_t *>(smem + ((rows * pitch + 15) & ~15));
+    uint32_t s1 = 0, s2 = 0;
+    for (int i = threadIdx.x; i < th * (kTP / 4) + 4; i += blockDim.x) {
+        const int r = i / (kTP / 4), c = 4 * (i - r * (kTP / 4)) - 32;
+        uint32_t wv = 0;
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+            if (r < th && c + b >= 0 && c + b < tw) {
+                const uint32_t t = tpl[r * tw + c + b];
+                s1 += t;
+                s2 += t * t;
+                wv |= (t ^ 0x80u) << (8 * b);
+            }
+        tp[i] = wv;
+    }
+    long long l1 = s1, l2 = s2;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        l1 += __shfl_xor(l1, o);
+        l2 += __shfl_xor(l2, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        tred[0][threadIdx.x >> 6] = l1;
+        tred[1][threadIdx.x >> 6] = l2;
+    }
     __syncthreads();
 
     const int n = lane & 31, g = lane >> 5;
@@ -226,15 +223,22 @@ __global__ __launch_bounds__(256) void match_i8_mfma_kernel(
         for (int r = 0; r < 16; r++) acc[t][r] = 0;
 
     const int8_t *arow = blk + (wave * 32 + n) * pitch + 16 * g;
-    const v4i *bp = bsw + lane;
-    // Toeplitz operands of the next template row are loaded one step ahead.
-    v4i b[KB], bn[KB];
-#pragma unroll
-    for (int kb = 0; kb < KB; kb++) b[kb] = bp[kb * 64];
+    // lane (n, g) feeds B rows k = 16g + j of column n: bytes
+    // 32 + 32kb + 16g - n .. +15 of the padded row, the same alignment every dy
+    const int u0 = 32 + 16 * g - n;
+    const uint32_t sh = (uint32_t)(u0 & 3);
+    const uint32_t *trow = tp + (u0 >> 2);
     for (int dy = 0; dy < th; dy++) {
-        const int dn = dy + 1 < th ? dy + 1 : dy;
+        v4i b[KB];
 #pragma unroll
-        for (int kb = 0; kb < KB; kb++) bn[kb] = bp[(dn * KB + kb) * 64];
+        for (int kb = 0; kb < KB; kb++) {
+            const uint32_t *q = trow + 8 * kb;
+            const uint32_t d0 = q[0], d1 = q[1], d2 = q[2], d3 = q[3], d4 = q[4];
+            b[kb][0] = (int)__builtin_amdgcn_alignbyte(d1, d0, sh);
+            b[kb][1] = (int)__builtin_amdgcn_alignbyte(d2, d1, sh);
+            b[kb][2] = (int)__builtin_amdgcn_alignbyte(d3, d2, sh);
+            b[kb][3] = (int)__builtin_amdgcn_alignbyte(d4, d3, sh);
+        }
 #pragma unroll
         for (int s = 0; s < XT + KB - 1; s++) {
             v4i a = *reinterpret_cast<const v4i *>(arow + 32 * s);
@@ -244,12 +248,15 @@ __global__ __launch_bounds__(256) void match_i8_mfma_kernel(
                     acc[t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b[s - t], acc[t], 0, 0, 0);
             }
         }
-#pragma unroll
-        for (int kb = 0; kb < KB; kb++) b[kb] = bn[kb];
         arow += pitch;
+        trow += kTP / 4;
     }
 
-    const long long st = tsums[0], stt = tsums[1];
+    long long st = 0, stt = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+        st += tred[0][w];
+        stt += tred[1][w];
+    }
     const long long nn = (long long)tw * th;
     const long long base = 128LL * st - 16384LL * nn;
 #pragma unroll
@@ -542,19 +549,20 @@ size_t win_lds(int tw, int th) {
 thread_local int g_match_rows_hint = 0;
 
 template <int KB, int XT>
-iph_status launch_i8_kb_xt(const uint8_t *img, int W, int H, const int8_t *bsw,
-                           const uint2 *wsum, const long long *tsums, float *out, int ow, int oh,
-                           int tw, int th, int method, hipStream_t s) {
+iph_status launch_i8_kb_xt(const uint8_t *img, int W, int H, const uint8_t *tpl,
+                           const uint2 *wsum, float *out, int ow, int oh, int tw, int th,
+                           int method, hipStream_t s) {
     constexpr int BW = 32 * (XT + KB - 1);
     int pitch = ((BW / 16) % 2 == 0) ? BW + 16 : BW;  // odd multiple of 16: no bank conflicts
     int RT = g_match_rows_hint > 0 ? std::min(g_match_rows_hint, 4) : 4;
-    while (RT > 1 && (size_t)(32 * RT + th - 1) * pitch > 64 * 1024) RT /= 2;
-    size_t lds = (size_t)(32 * RT + th - 1) * pitch;
+    const size_t tbytes = (size_t)th * tpl_pitch(KB) + 16;
+    auto lds_for = [&](int rt) { return align_up((size_t)(32 * rt + th - 1) * pitch, 16) + tbytes; };
+    while (RT > 1 && lds_for(RT) > 64 * 1024) RT /= 2;
+    const size_t lds = lds_for(RT);
     dim3 grid((ow + 32 * XT - 1) / (32 * XT), (oh + 32 * RT - 1) / (32 * RT));
-    const v4i *b = reinterpret_cast<const v4i *>(bsw);
 #define L(M)                                                                                 \
-    match_i8_mfma_kernel<KB, M, XT><<<grid, 64 * RT, lds, s>>>(img, W, H, b, wsum, tsums, out, \
-                                                               ow, oh, tw, th, pitch)
+    match_i8_mfma_kernel<KB, M, XT><<<grid, 64 * RT, lds, s>>>(img, W, H, tpl, wsum, out, ow,  \
+                                                               oh, tw, th, pitch)
     switch (method) {
     case IPH_SSE: L(IPH_SSE); break;
     case IPH_SSE_NORMED: L(IPH_SSE_NORMED); break;
@@ -576,31 +584,14 @@ int match_xt(int ow, int oh) {
 }
 
 template <int KB>
-iph_status launch_i8_kb(const uint8_t *img, int W, int H, const int8_t *bsw, const uint2 *wsum,
-                        const long long *tsums, float *out, int ow, int oh, int tw, int th,
-                        int method, hipStream_t s) {
+iph_status launch_i8_kb(const uint8_t *img, int W, int H, const uint8_t *tpl, const uint2 *wsum,
+                        float *out, int ow, int oh, int tw, int th, int method, hipStream_t s) {
     switch (match_xt(ow, oh)) {
-    case 2:
-        return launch_i8_kb_xt<KB, 2>(img, W, H, bsw, wsum, tsums, out, ow, oh, tw, th, method,
-                                      s);
-    default:
-        return launch_i8_kb_xt<KB, 1>(img, W, H, bsw, wsum, tsums, out, ow, oh, tw, th, method,
-                                      s);
+    case 2: return launch_i8_kb_xt<KB, 2>(img, W, H, tpl, wsum, out, ow, oh, tw, th, method, s);
+    default: return launch_i8_kb_xt<KB, 1>(img, W, H, tpl, wsum, out, ow, oh, tw, th, method, s);
     }
 }
 
-struct U8Plan {
-    int KB;
-    size_t bsw_bytes, win_bytes;
-};
-
-U8Plan plan_u8(uint32_t w, uint32_t h, uint32_t tw, uint32_t th) {
-    U8Plan p;
-    p.KB = blocks_for((int)tw);
-    p.bsw_bytes = (size_t)th * p.KB * 1024;
-    p.win_bytes = (size_t)(w - tw + 1) * (h - th + 1) * sizeof(uint2);
-    return p;
-}
 
 }  // namespace
 
@@ -659,10 +650,7 @@ size_t mfma_match_workspace(int32_t p_type, uint32_t w, uint32_t h, uint32_t tw,
         if (method != IPH_CCORR) wp.take<double>((size_t)(w - tw + 1) * h);
         return wp.used;
     }
-    U8Plan p = plan_u8(w, h, tw, th);
     WorkspacePlan wp;
-    wp.take<long long>(2);
-    wp.take<int8_t>(p.bsw_bytes);
     wp.take<uint2>((size_t)(w - tw + 1) * (h - th + 1));
     return wp.used;
 }
@@ -679,32 +667,26 @@ iph_status launch_mfma_match(const void *img, int32_t p_type, uint32_t w, uint32
                   kMaxSide, kMaxSide);
         return IPH_EUNSUPPORTED;
     }
-    U8Plan p = plan_u8(w, h, tw, th);
+    const int KB = blocks_for((int)tw);
     Workspace wsp(ws, ws_bytes);
-    long long *tsums = wsp.take<long long>(2);
-    int8_t *bsw = wsp.take<int8_t>(p.bsw_bytes);
     uint2 *wsum = wsp.take<uint2>((size_t)(w - tw + 1) * (h - th + 1));
-    if (!tsums || !bsw || !wsum) {
+    if (!wsum) {
         set_error("workspace too small for the int8 MFMA match_template");
         return IPH_EINVAL;
     }
     const uint8_t *im = static_cast<const uint8_t *>(img);
     const uint8_t *tp = static_cast<const uint8_t *>(tpl);
     int ow = (int)(w - tw + 1), oh = (int)(h - th + 1);
-    template_sums_kernel<<<1, 256, 0, s>>>(tp, (int)(tw * th), tsums);
-    IPH_TRY(launch_status("template_sums_kernel"));
-    toeplitz_i8_kernel<<<dim3(p.KB, th), 64, 0, s>>>(tp, (int)tw, p.KB, bsw);
-    IPH_TRY(launch_status("toeplitz_i8_kernel"));
     dim3 wg((ow + kWinCols - 1) / kWinCols, (oh + kWinRows - 1) / kWinRows);
     window_sums_u8_kernel<<<wg, 256, win_lds((int)tw, (int)th), s>>>(im, (int)w, (int)tw,
                                                                     (int)th, ow, oh, wsum);
     IPH_TRY(launch_status("window_sums_u8_kernel"));
-    switch (p.KB) {
-    case 1: return launch_i8_kb<1>(im, w, h, bsw, wsum, tsums, out, ow, oh, tw, th, method, s);
-    case 2: return launch_i8_kb<2>(im, w, h, bsw, wsum, tsums, out, ow, oh, tw, th, method, s);
-    case 3: return launch_i8_kb<3>(im, w, h, bsw, wsum, tsums, out, ow, oh, tw, th, method, s);
-    case 4: return launch_i8_kb<4>(im, w, h, bsw, wsum, tsums, out, ow, oh, tw, th, method, s);
-    default: return launch_i8_kb<5>(im, w, h, bsw, wsum, tsums, out, ow, oh, tw, th, method, s);
+    switch (KB) {
+    case 1: return launch_i8_kb<1>(im, w, h, tp, wsum, out, ow, oh, tw, th, method, s);
+    case 2: return launch_i8_kb<2>(im, w, h, tp, wsum, out, ow, oh, tw, th, method, s);
+    case 3: return launch_i8_kb<3>(im, w, h, tp, wsum, out, ow, oh, tw, th, method, s);
+    case 4: return launch_i8_kb<4>(im, w, h, tp, wsum, out, ow, oh, tw, th, method, s);
+    default: return launch_i8_kb<5>(im, w, h, tp, wsum, out, ow, oh, tw, th, method, s);
     }
 }
 
