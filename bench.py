@@ -513,10 +513,11 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--leg", choices=("all", "blur", "copy", "proposal"), default="all",
+    ap.add_argument("--leg", choices=("all", "blur", "copy", "match", "proposal"), default="all",
                     help="blur: only the blur batch (for counter runs); copy: only a plain "
-                         "copy of the same size (counter calibration); proposal: only the "
-                         "learned-proposal search against the full map")
+                         "copy of the same size (counter calibration); match: only the "
+                         "match_template leg; proposal: only the learned-proposal search "
+                         "against the full map")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X and has no "
@@ -528,6 +529,9 @@ def main():
     import imageproc_amd as ip
 
     print(f"[bench] device: {ip.device_info()}", flush=True)
+    if args.leg == "match":
+        print(json.dumps({"leg": "match", **bench_match(max(5, args.steps // 2))}), flush=True)
+        return
     if args.leg == "proposal":
         print(json.dumps({"leg": "proposal", **bench_proposal()}), flush=True)
         return

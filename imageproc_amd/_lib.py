@@ -145,6 +145,7 @@ _SIGS = {
                                             C.c_uint32, C.c_uint32, C.c_int32, C.c_void_p,
                                             C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
                                             C.c_void_p, C.c_size_t, C.c_void_p]),
+    "iph_dev_read_result": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "iph_dev_proposal_search_workspace": (C.c_size_t, [C.c_uint32, C.c_uint32, C.c_uint32,
                                                        C.c_uint32, C.c_int32, C.c_int32,
                                                        C.c_int32]),

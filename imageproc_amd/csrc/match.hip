@@ -95,11 +95,11 @@ iph_status direct_typed(const P *img, uint32_t w, const P *tpl, uint32_t tw, uin
 // later pixel replaces the current extreme only if strictly smaller/larger,
 // so the result is the first pixel in row-major order holding the extreme
 // value, NaNs never replace, and a NaN at (0,0) is never replaced.
-// Parallel form: reduce 64-bit keys (ordered value << 32 | index) with
-// atomicMin/atomicMax, skipping NaNs; -0.0 and +0.0 map to the same key
-// because the sequential compare treats them as equal. The finalize step
-// reads the value stored at the winning index and applies the NaN-at-(0,0)
-// rule.
+// Parallel form: reduce 64-bit keys (ordered value << 32 | index), skipping
+// NaNs; -0.0 and +0.0 map to the same key because the sequential compare
+// treats them as equal. Each workgroup writes its (min, max) key pair; the
+// finalize step reduces the pairs, reads the value stored at the winning
+// index and applies the NaN-at-(0,0) rule.
 
 __device__ __forceinline__ uint32_t order_key(float v) {
     uint32_t u = __float_as_uint(v);
@@ -110,13 +110,6 @@ __device__ __forceinline__ uint32_t order_key(uint8_t v) { return v; }
 __device__ __forceinline__ bool is_nan(float v) { return v != v; }
 __device__ __forceinline__ bool is_nan(uint8_t) { return false; }
 
-__global__ void extremes_init_kernel(unsigned long long *keys) {
-    if (threadIdx.x == 0) {
-        keys[0] = ~0ull;  // min key
-        keys[1] = 0ull;   // max key
-    }
-}
-
 __device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v, int m) {
     uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
     lo = __shfl_xor(lo, m);
@@ -124,22 +117,20 @@ __device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v,
     return ((unsigned long long)hi << 32) | lo;
 }
 
+// Folds pixel i = value v into the running min and max keys.
 template <typename P>
-__global__ __launch_bounds__(256) void extremes_reduce_kernel(const P *__restrict__ img,
-                                                              size_t n,
-                                                              unsigned long long *keys) {
-    unsigned long long kmin = ~0ull, kmax = 0ull;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t step = (size_t)gridDim.x * blockDim.x;
-    for (; i < n; i += step) {
-        P v = img[i];
-        if (is_nan(v)) continue;
-        unsigned long long k = (unsigned long long)order_key(v) << 32;
-        unsigned long long a = k | (uint32_t)i;
-        unsigned long long b = k | (0xffffffffu - (uint32_t)i);
-        kmin = a < kmin ? a : kmin;
-        kmax = b > kmax ? b : kmax;
-    }
+__device__ __forceinline__ void fold_extreme(P v, uint32_t i, unsigned long long &kmin,
+                                             unsigned long long &kmax) {
+    if (is_nan(v)) return;
+    const unsigned long long k = (unsigned long long)order_key(v) << 32;
+    const unsigned long long a = k | i, b = k | (0xffffffffu - i);
+    kmin = a < kmin ? a : kmin;
+    kmax = b > kmax ? b : kmax;
+}
+
+// Min and max of two key pairs over the 256 threads of a workgroup.
+__device__ __forceinline__ void block_extremes(unsigned long long &kmin, unsigned long long &kmax) {
+    __shared__ unsigned long long part[2][16];
     for (int m = 32; m >= 1; m >>= 1) {
         unsigned long long o = shfl_xor_u64(kmin, m);
         kmin = o < kmin ? o : kmin;
@@ -147,15 +138,69 @@ __global__ __launch_bounds__(256) void extremes_reduce_kernel(const P *__restric
         kmax = o > kmax ? o : kmax;
     }
     if ((threadIdx.x & 63) == 0) {
-        if (kmin != ~0ull) atomicMin(&keys[0], kmin);
-        if (kmax != 0ull) atomicMax(&keys[1], kmax);
+        part[0][threadIdx.x >> 6] = kmin;
+        part[1][threadIdx.x >> 6] = kmax;
+    }
+    __syncthreads();
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+        kmin = part[0][w] < kmin ? part[0][w] : kmin;
+        kmax = part[1][w] > kmax ? part[1][w] : kmax;
+    }
+}
+
+// Each thread reads 16-byte vectors (kUnroll in flight per iteration) when
+// `vec` (16-byte aligned map); the last n % (16 / sizeof(P)) pixels, or all
+// of them without `vec`, one at a time. pairs[2 b], pairs[2 b + 1] = the
+// min and max key of workgroup b (~0 and 0 when it saw only NaNs).
+constexpr int kExtUnroll = 4;
+constexpr int kExtMaxBlocks = 1024;
+template <typename P>
+__global__ __launch_bounds__(256) void extremes_reduce_kernel(const P *__restrict__ img,
+                                                              size_t n, int vec,
+                                                              unsigned long long *pairs) {
+    constexpr int kV = 16 / sizeof(P);
+    unsigned long long kmin = ~0ull, kmax = 0ull;
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t step = (size_t)gridDim.x * blockDim.x;
+    const size_t nv = vec ? n / kV : 0;
+    const uint4 *iv = reinterpret_cast<const uint4 *>(img);
+    for (size_t v0 = tid; v0 < nv; v0 += step * kExtUnroll) {
+        uint4 q[kExtUnroll];
+#pragma unroll
+        for (int u = 0; u < kExtUnroll; u++) {
+            const size_t v = v0 + step * u;
+            q[u] = iv[v < nv ? v : nv - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < kExtUnroll; u++) {
+            const size_t v = v0 + step * u;
+            if (v >= nv) break;
+            const P *e = reinterpret_cast<const P *>(&q[u]);
+#pragma unroll
+            for (int j = 0; j < kV; j++) fold_extreme(e[j], (uint32_t)(v * kV + j), kmin, kmax);
+        }
+    }
+    for (size_t i = nv * kV + tid; i < n; i += step) fold_extreme(img[i], (uint32_t)i, kmin, kmax);
+    block_extremes(kmin, kmax);
+    if (threadIdx.x == 0) {
+        pairs[2 * blockIdx.x] = kmin;
+        pairs[2 * blockIdx.x + 1] = kmax;
     }
 }
 
 template <typename P>
-__global__ void extremes_finalize_kernel(const P *__restrict__ img, uint32_t w,
-                                         const unsigned long long *keys, iph_extremes *out) {
+__global__ __launch_bounds__(256) void extremes_finalize_kernel(const P *__restrict__ img,
+                                                                uint32_t w,
+                                                                const unsigned long long *pairs,
+                                                                int nblocks, iph_extremes *out) {
+    unsigned long long kmin = ~0ull, kmax = 0ull;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+        kmin = pairs[2 * b] < kmin ? pairs[2 * b] : kmin;
+        kmax = pairs[2 * b + 1] > kmax ? pairs[2 * b + 1] : kmax;
+    }
+    block_extremes(kmin, kmax);
     if (threadIdx.x != 0) return;
+    const unsigned long long keys[2] = {kmin, kmax};
     iph_extremes r;
     r.max_value.raw = 0;
     r.min_value.raw = 0;
@@ -214,32 +259,31 @@ iph_status launch_match(const void *img, int32_t p_type, uint32_t w, uint32_t h,
     return launch_mfma_match(img, p_type, w, h, tpl, tw, th, method, out, ws, ws_bytes, s);
 }
 
-size_t extremes_workspace() { return 256; }
+size_t extremes_workspace() { return (size_t)kExtMaxBlocks * 2 * sizeof(unsigned long long); }
 
 iph_status launch_extremes(const void *img, int32_t p_type, uint32_t w, uint32_t h,
                            iph_extremes *out, void *ws, size_t ws_bytes, hipStream_t s) {
     Workspace wsp(ws, ws_bytes);
-    unsigned long long *keys = wsp.take<unsigned long long>(2);
-    if (!keys) {
+    unsigned long long *pairs = wsp.take<unsigned long long>(2 * kExtMaxBlocks);
+    if (!pairs) {
         set_error("workspace too small for find_extremes");
         return IPH_EINVAL;
     }
-    size_t n = (size_t)w * h;
-    extremes_init_kernel<<<1, 64, 0, s>>>(keys);
-    IPH_TRY(launch_status("extremes_init_kernel"));
-    unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 2048);
+    const size_t n = (size_t)w * h;
+    const int vec = (reinterpret_cast<uintptr_t>(img) & 15) == 0;
+    const size_t per_block = (size_t)256 * kExtUnroll * (p_type == IPH_U8 ? 16 : 4);
+    const int blocks = (int)std::max<size_t>(
+        1, std::min<size_t>((n + per_block - 1) / per_block, kExtMaxBlocks));
     if (p_type == IPH_U8) {
-        extremes_reduce_kernel<uint8_t><<<blocks, 256, 0, s>>>(static_cast<const uint8_t *>(img),
-                                                              n, keys);
+        const uint8_t *im = static_cast<const uint8_t *>(img);
+        extremes_reduce_kernel<uint8_t><<<blocks, 256, 0, s>>>(im, n, vec, pairs);
         IPH_TRY(launch_status("extremes_reduce_kernel"));
-        extremes_finalize_kernel<uint8_t><<<1, 64, 0, s>>>(static_cast<const uint8_t *>(img), w,
-                                                           keys, out);
+        extremes_finalize_kernel<uint8_t><<<1, 256, 0, s>>>(im, w, pairs, blocks, out);
     } else {
-        extremes_reduce_kernel<float><<<blocks, 256, 0, s>>>(static_cast<const float *>(img), n,
-                                                            keys);
+        const float *im = static_cast<const float *>(img);
+        extremes_reduce_kernel<float><<<blocks, 256, 0, s>>>(im, n, vec, pairs);
         IPH_TRY(launch_status("extremes_reduce_kernel"));
-        extremes_finalize_kernel<float><<<1, 64, 0, s>>>(static_cast<const float *>(img), w,
-                                                         keys, out);
+        extremes_finalize_kernel<float><<<1, 256, 0, s>>>(im, w, pairs, blocks, out);
     }
     return launch_status("extremes_finalize_kernel");
 }

@@ -418,6 +418,50 @@ __global__ __launch_bounds__(kChunk) void topk_chunk_kernel(const uint64_t *__re
     }
 }
 
+// The k best of t keys in one workgroup, when every thread's share of at
+// most ceil(t / 1024) keys keeps the candidates below kChunk: the k-th
+// largest of the 1024 per-thread maxima (keys are distinct) bounds the
+// answer from below, only the k threads reaching it hold keys that large,
+// so at most k * ceil(t / 1024) <= kChunk keys are sorted a second time.
+__global__ __launch_bounds__(kChunk) void topk_single_kernel(const uint64_t *__restrict__ keys,
+                                                             int t, int k,
+                                                             const float *__restrict__ map,
+                                                             int cw, int *__restrict__ cand,
+                                                             float *__restrict__ cscore) {
+    __shared__ uint64_t xs[2 * kChunk];
+    __shared__ uint64_t thr;
+    __shared__ int cnt;
+    const int tid = threadIdx.x;
+    uint64_t mx = 0;
+    for (int i = tid; i < t; i += kChunk) mx = max_key(mx, keys[i]);
+    if (tid == 0) {
+        thr = 0ull;
+        cnt = 0;
+    }
+    const uint64_t sorted = sort1024(mx, xs, tid);
+    __syncthreads();
+    if (tid == k - 1) thr = sorted;
+    __syncthreads();
+    const uint64_t T = thr;
+    for (int i = tid; i < t; i += kChunk) {
+        const uint64_t key = keys[i];
+        if (key != 0ull && key >= T) {
+            const int slot = atomicAdd(&cnt, 1);
+            if (slot < kChunk) xs[slot] = key;
+        }
+    }
+    __syncthreads();
+    const uint64_t mine = tid < min(cnt, kChunk) ? xs[tid] : 0ull;
+    __syncthreads();
+    const uint64_t key = sort1024(mine, xs, tid);
+    if (tid < k) {
+        const int id = key_index(key);
+        cand[2 * tid] = key ? id % cw : 0;
+        cand[2 * tid + 1] = key ? id / cw : 0;
+        cscore[tid] = key ? map[id] : -INFINITY;
+    }
+}
+
 // The coarse map and its tile keys in one pass, for downsampled templates
 // of at most kCoarseMax pixels a side (TWQ = words per template row). A
 // workgroup covers 16 coarse rows by 256 columns (16 tiles); a thread one
@@ -1034,9 +1078,14 @@ iph_status check_candidates(uint32_t cw, uint32_t ch, int k) {
     return IPH_OK;
 }
 
-// Levels of chunk-wise top-k over t tile keys down to the k-candidate list.
+// The k-candidate list from t tile keys: one workgroup when the candidate
+// bound allows it, else levels of chunk-wise top-k.
 iph_status launch_topk(const uint64_t *tk, int t, int k, const float *coarse, uint32_t cw,
                        int *cand, float *cscore, uint64_t *lk0, uint64_t *lk1, hipStream_t s) {
+    if ((long long)k * ((t + kChunk - 1) / kChunk) <= kChunk) {
+        topk_single_kernel<<<1, kChunk, 0, s>>>(tk, t, k, coarse, (int)cw, cand, cscore);
+        return launch_status("topk_single_kernel");
+    }
     uint64_t *lk[2] = {lk0, lk1};
     const uint64_t *src = tk;
     int n = t, b = 0;
@@ -1311,6 +1360,15 @@ IPH_API iph_status iph_dev_proposal_refine(const uint8_t *image, uint32_t w, uin
     clear_error();
     return launch_refine(image, w, h, tpl, tw, th, method, centers, n_centers, scale, sep,
                          nullptr, result, nullptr, workspace, workspace_bytes, S(stream));
+}
+
+IPH_API iph_status iph_dev_read_result(const float *result, float *host, size_t n,
+                                       void *stream) {
+    clear_error();
+    IPH_TRY_HIP(hipMemcpyAsync(host, result, n * sizeof(float), hipMemcpyDeviceToHost,
+                               S(stream)));
+    IPH_TRY_HIP(hipStreamSynchronize(S(stream)));
+    return IPH_OK;
 }
 
 IPH_API size_t iph_dev_proposal_search_workspace(uint32_t w, uint32_t h, uint32_t tw, uint32_t th,

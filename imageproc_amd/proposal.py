@@ -243,9 +243,21 @@ class ProposalSearch:
 
     def full_map(self, image, tpl):
         """The plain search: full score map, then find_extremes' argmax
-        (x, y, score)."""
-        full = dev.match_template(image, tpl, self.method, "auto")
-        rec = dev.find_extremes(full).cpu().numpy().view(np.uint32)
+        (x, y, score). The map and the record stay allocated per shape."""
+        h, w = image.shape
+        th, tw = tpl.shape
+        key = ("full", h, w, th, tw)
+        b = self._cache.get(key)
+        if b is None:
+            b = {"map": torch.empty((h - th + 1, w - tw + 1), dtype=torch.float32, device="cuda"),
+                 "rec": torch.empty(24, dtype=torch.uint8, device="cuda"),
+                 "host": torch.empty(24, dtype=torch.uint8, pin_memory=True)}
+            self._cache[key] = b
+        dev.match_template(image, tpl, self.method, "auto", out=b["map"])
+        dev.find_extremes(b["map"], out=b["rec"])
+        b["host"].copy_(b["rec"], non_blocking=True)
+        torch.cuda.current_stream().synchronize()
+        rec = b["host"].numpy().view(np.uint32)
         return int(rec[2]), int(rec[3]), float(rec[0:1].view(np.float32)[0])
 
     def search_steps(self, image: torch.Tensor, tpl: torch.Tensor, use_net: bool = True):
@@ -274,8 +286,7 @@ class ProposalSearch:
             _p(net.b_all) if net is not None else None, self.k, self.n_top, self.sep,
             _p(b["result"]), _p(b["sws"]), b["sws"].numel(), _s()))
         host = b["host"]
-        host.copy_(b["result"], non_blocking=True)
-        torch.cuda.current_stream().synchronize()
+        check(lib().iph_dev_read_result(_p(b["result"]), C.c_void_p(host.data_ptr()), 5, _s()))
         r = host.numpy()
         return int(r[0]), int(r[1]), float(r[2]), float(r[3]), float(r[4])
 

@@ -314,6 +314,9 @@ IPH_API iph_status iph_dev_proposal_refine(const uint8_t *image, uint32_t width,
  * result[0..4] = x, y, best score, best score more than `sep` px away, top
  * probability (1 without the network): the composition of the functions
  * above, value for value. */
+/* Copies n floats of a result (e.g. the search's 5) to host memory once the
+ * stream's work before it has run, and waits for the copy. */
+IPH_API iph_status iph_dev_read_result(const float *result, float *host, size_t n, void *stream);
 IPH_API size_t iph_dev_proposal_search_workspace(uint32_t width, uint32_t height, uint32_t tw,
                                                  uint32_t th, int32_t method, int32_t k,
                                                  int32_t n_top);
