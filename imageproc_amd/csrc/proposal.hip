@@ -418,11 +418,14 @@ __global__ __launch_bounds__(kChunk) void topk_chunk_kernel(const uint64_t *__re
     }
 }
 
-// The k best of t keys in one workgroup, when every thread's share of at
-// most ceil(t / 1024) keys keeps the candidates below kChunk: the k-th
+// The k best of t keys in one workgroup, when t <= 1024 * kTopKeysPerThread
+// and every thread's share of at most ceil(t / 1024) keys keeps the
+// candidates below kChunk: the k-th
 // largest of the 1024 per-thread maxima (keys are distinct) bounds the
 // answer from below, only the k threads reaching it hold keys that large,
-// so at most k * ceil(t / 1024) <= kChunk keys are sorted a second time.
+// so at most k * ceil(t / 1024) <= kChunk keys pass; each finds its rank
+// by counting the passing keys above it (typically a few more than k).
+constexpr int kTopKeysPerThread = 16;
 __global__ __launch_bounds__(kChunk) void topk_single_kernel(const uint64_t *__restrict__ keys,
                                                              int t, int k,
                                                              const float *__restrict__ map,
@@ -432,8 +435,15 @@ __global__ __launch_bounds__(kChunk) void topk_single_kernel(const uint64_t *__r
     __shared__ uint64_t thr;
     __shared__ int cnt;
     const int tid = threadIdx.x;
+    uint64_t kr[kTopKeysPerThread];  // this thread's keys, all loads in flight at once
+#pragma unroll
+    for (int q = 0; q < kTopKeysPerThread; q++) {
+        const int i = tid + q * kChunk;
+        kr[q] = i < t ? keys[i] : 0ull;
+    }
     uint64_t mx = 0;
-    for (int i = tid; i < t; i += kChunk) mx = max_key(mx, keys[i]);
+#pragma unroll
+    for (int q = 0; q < kTopKeysPerThread; q++) mx = max_key(mx, kr[q]);
     if (tid == 0) {
         thr = 0ull;
         cnt = 0;
@@ -443,22 +453,31 @@ __global__ __launch_bounds__(kChunk) void topk_single_kernel(const uint64_t *__r
     if (tid == k - 1) thr = sorted;
     __syncthreads();
     const uint64_t T = thr;
-    for (int i = tid; i < t; i += kChunk) {
-        const uint64_t key = keys[i];
+#pragma unroll
+    for (int q = 0; q < kTopKeysPerThread; q++) {
+        const uint64_t key = kr[q];
         if (key != 0ull && key >= T) {
             const int slot = atomicAdd(&cnt, 1);
             if (slot < kChunk) xs[slot] = key;
         }
     }
     __syncthreads();
-    const uint64_t mine = tid < min(cnt, kChunk) ? xs[tid] : 0ull;
-    __syncthreads();
-    const uint64_t key = sort1024(mine, xs, tid);
-    if (tid < k) {
-        const int id = key_index(key);
-        cand[2 * tid] = key ? id % cw : 0;
-        cand[2 * tid + 1] = key ? id / cw : 0;
-        cscore[tid] = key ? map[id] : -INFINITY;
+    const int nc = min(cnt, kChunk);
+    if (tid < nc) {
+        const uint64_t key = xs[tid];
+        int rank = 0;
+        for (int j = 0; j < nc; j++) rank += xs[j] > key;
+        if (rank < k) {
+            const int id = key_index(key);
+            cand[2 * rank] = id % cw;
+            cand[2 * rank + 1] = id / cw;
+            cscore[rank] = map[id];
+        }
+    }
+    for (int r = nc + tid; r < k; r += kChunk) {  // fewer keys than k: empty slots
+        cand[2 * r] = 0;
+        cand[2 * r + 1] = 0;
+        cscore[r] = -INFINITY;
     }
 }
 
@@ -499,21 +518,30 @@ __global__ __launch_bounds__(256) void coarse_tiles_kernel(
         tb[r * kCoarseMax + q] = (uint8_t)v;
         tsq += v * v;
     }
-    for (int i = tid; i < (16 + kCoarseMax - 1) * kCoarseW; i += 256) {
-        const int r = i / kCoarseW, q = i - r * kCoarseW;
-        const int gy = y0 + r, gx = x0 + 4 * q;
-        uint32_t wv = 0;
-        if (r < R && gy < sh) {
-            const uint8_t *row = small + (size_t)gy * sw;
-            if (vec && gx + 4 <= sw) {
-                wv = *reinterpret_cast<const uint32_t *>(row + gx);
-            } else {
+    {  // every load of the window in flight at once, then the LDS stores
+        constexpr int kWords = (16 + kCoarseMax - 1) * kCoarseW;
+        constexpr int kPer = (kWords + 255) / 256;
+        uint32_t wv[kPer];
 #pragma unroll
-                for (int b = 0; b < 4; b++)
-                    if (gx + b < sw) wv |= (uint32_t)row[gx + b] << (8 * b);
+        for (int u = 0; u < kPer; u++) {
+            const int i = tid + 256 * u;
+            const int r = i / kCoarseW, q = i - r * kCoarseW;
+            const int gy = y0 + r, gx = x0 + 4 * q;
+            wv[u] = 0;
+            if (i < kWords && r < R && gy < sh) {
+                const uint8_t *row = small + (size_t)gy * sw;
+                if (vec && gx + 4 <= sw) {
+                    wv[u] = *reinterpret_cast<const uint32_t *>(row + gx);
+                } else {
+#pragma unroll
+                    for (int b = 0; b < 4; b++)
+                        if (gx + b < sw) wv[u] |= (uint32_t)row[gx + b] << (8 * b);
+                }
             }
         }
-        iw[i] = wv;
+#pragma unroll
+        for (int u = 0; u < kPer; u++)
+            if (tid + 256 * u < kWords) iw[tid + 256 * u] = wv[u];
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) tsq += __shfl_xor(tsq, o);
@@ -887,9 +915,30 @@ __global__ __launch_bounds__(1024) void refine_best_kernel(const float *__restri
         x = centers[2 * c] * scale - kRefR + (pos - dy * kRefS);
         y = centers[2 * c + 1] * scale - kRefR + dy;
     };
+    // each thread's first kRefPer scores stay in registers (one read of
+    // them); scores past nt * kRefPer are read again for the second pass
+    constexpr int kRefPer = 8;
+    float sv_[kRefPer];
+    int sx_[kRefPer], sy_[kRefPer];
     float bv = __int_as_float(0x7fc00000);
     int bx = 0, by = 0;
-    for (int i = tid; i < n; i += nt) {
+#pragma unroll
+    for (int q = 0; q < kRefPer; q++) {
+        const int i = tid + q * nt;
+        sv_[q] = i < n ? scores[i] : __int_as_float(0x7fc00000);
+        sx_[q] = sy_[q] = 0;
+        if (i < n) place(i, sx_[q], sy_[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < kRefPer; q++) {
+        const float v = sv_[q];
+        if (v == v && ref_better(v, sy_[q], sx_[q], bv, by, bx)) {
+            bv = v;
+            bx = sx_[q];
+            by = sy_[q];
+        }
+    }
+    for (int i = tid + kRefPer * nt; i < n; i += nt) {
         const float v = scores[i];
         if (v != v) continue;
         int x, y;
@@ -926,7 +975,11 @@ __global__ __launch_bounds__(1024) void refine_best_kernel(const float *__restri
             by = sy[w];
         }
     float sec = -INFINITY;
-    for (int i = tid; i < n; i += nt) {
+#pragma unroll
+    for (int q = 0; q < kRefPer; q++)
+        if (sv_[q] == sv_[q] && (abs(sx_[q] - bx) > sep || abs(sy_[q] - by) > sep))
+            sec = fmaxf(sec, sv_[q]);
+    for (int i = tid + kRefPer * nt; i < n; i += nt) {
         const float v = scores[i];
         if (v != v) continue;
         int x, y;
@@ -961,29 +1014,46 @@ __global__ __launch_bounds__(256) void refine_kernel(const uint8_t *__restrict__
     uint32_t *is = sm + th * twq;   // th x rq image words from column ox
     const int c = blockIdx.x, dy = blockIdx.y, tid = threadIdx.x;
     const int ox = centers[2 * c] * scale - kRefR, oy = centers[2 * c + 1] * scale - kRefR + dy;
+    // loads issued kRefBatch words at a time before their LDS stores
+    constexpr int kRefBatch = 4;
     long long tsq = 0;
-    for (int i = tid; i < th * twq; i += 256) {
-        const int r = i / twq, q = i - r * twq;
-        uint32_t wv = 0;
+    for (int i0 = tid; i0 < th * twq; i0 += 256 * kRefBatch) {
+        uint32_t wv[kRefBatch];
 #pragma unroll
-        for (int b = 0; b < 4; b++)
-            if (4 * q + b < tw) wv |= (uint32_t)tpl[r * tw + 4 * q + b] << (8 * b);
-        ts[i] = wv;
-        tsq += __builtin_amdgcn_udot4(wv, wv, 0u, false);
+        for (int u = 0; u < kRefBatch; u++) {
+            const int i = i0 + 256 * u, r = i / twq, q = i - r * twq;
+            wv[u] = 0;
+            if (i < th * twq)
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+                    if (4 * q + b < tw) wv[u] |= (uint32_t)tpl[r * tw + 4 * q + b] << (8 * b);
+        }
+#pragma unroll
+        for (int u = 0; u < kRefBatch; u++)
+            if (i0 + 256 * u < th * twq) {
+                ts[i0 + 256 * u] = wv[u];
+                tsq += __builtin_amdgcn_udot4(wv[u], wv[u], 0u, false);
+            }
     }
-    for (int i = tid; i < th * rq; i += 256) {
-        const int r = i / rq, q = i - r * rq;
-        const int gy = oy + r;
-        uint32_t wv = 0;
-        if (gy >= 0 && gy < H) {
-            const uint8_t *row = img + (size_t)gy * W;
+    for (int i0 = tid; i0 < th * rq; i0 += 256 * kRefBatch) {
+        uint32_t wv[kRefBatch];
 #pragma unroll
-            for (int b = 0; b < 4; b++) {
-                const int gx = ox + 4 * q + b;
-                if (gx >= 0 && gx < W) wv |= (uint32_t)row[gx] << (8 * b);
+        for (int u = 0; u < kRefBatch; u++) {
+            const int i = i0 + 256 * u, r = i / rq, q = i - r * rq;
+            const int gy = oy + r;
+            wv[u] = 0;
+            if (i < th * rq && gy >= 0 && gy < H) {
+                const uint8_t *row = img + (size_t)gy * W;
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    const int gx = ox + 4 * q + b;
+                    if (gx >= 0 && gx < W) wv[u] |= (uint32_t)row[gx] << (8 * b);
+                }
             }
         }
-        is[i] = wv;
+#pragma unroll
+        for (int u = 0; u < kRefBatch; u++)
+            if (i0 + 256 * u < th * rq) is[i0 + 256 * u] = wv[u];
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) tsq += __shfl_xor(tsq, o);
@@ -1082,7 +1152,7 @@ iph_status check_candidates(uint32_t cw, uint32_t ch, int k) {
 // bound allows it, else levels of chunk-wise top-k.
 iph_status launch_topk(const uint64_t *tk, int t, int k, const float *coarse, uint32_t cw,
                        int *cand, float *cscore, uint64_t *lk0, uint64_t *lk1, hipStream_t s) {
-    if ((long long)k * ((t + kChunk - 1) / kChunk) <= kChunk) {
+    if (t <= kChunk * kTopKeysPerThread && (long long)k * ((t + kChunk - 1) / kChunk) <= kChunk) {
         topk_single_kernel<<<1, kChunk, 0, s>>>(tk, t, k, coarse, (int)cw, cand, cscore);
         return launch_status("topk_single_kernel");
     }
