@@ -325,6 +325,13 @@ def bench_match(steps: int):
                                        "best_location": list(best[path])}
         if len(best) == 2:
             res[f"{m.name}:argmax_agree"] = best["mfma"] == best["direct"]
+    # match_template_with_mask (reference-order direct path), seeded mask
+    mask = torch.from_numpy(uniform_u8(202, (MATCH_T, MATCH_T))).cuda()
+    out = dev.match_template_with_mask(gi, gt, M.CrossCorrelationNormalized, mask)
+    times, _ = timed(lambda: dev.match_template_with_mask(gi, gt, M.CrossCorrelationNormalized,
+                                                          mask, out=out), max(2, steps // 2), 1)
+    res["CrossCorrelationNormalized:masked_direct"] = {
+        "ms_per_image": round(float(np.median(times)), 4)}
     return res
 
 

@@ -13,7 +13,8 @@ modules over the C ABI of ``libimageproc_hip.so`` (include/imageproc_hip.h):
 from . import filter, kernel, template_matching
 from ._lib import DeviceError, Panic, device_info
 from .kernel import Kernel
-from .template_matching import Extremes, MatchTemplateMethod, find_extremes, match_template
+from .template_matching import (Extremes, MatchTemplateMethod, find_extremes, match_template,
+                                match_template_with_mask)
 
 __all__ = ["filter", "kernel", "template_matching", "Kernel", "Panic", "DeviceError",
            "MatchTemplateMethod", "match_template", "find_extremes", "Extremes", "device_info"]

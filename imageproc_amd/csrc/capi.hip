@@ -345,6 +345,42 @@ IPH_API iph_status iph_match_template(const void *image, int32_t p, uint32_t w, 
     return finish(dout, out, obytes);
 }
 
+// ImageTemplateMask::new, template_matching.rs:430-440: the mask check comes
+// before the template-vs-image checks.
+static iph_status check_masked(uint32_t w, uint32_t h, uint32_t tw, uint32_t th, uint32_t mw,
+                               uint32_t mh, int32_t method) {
+    if (mw != tw || mh != th) {
+        set_error("assertion `left == right` failed: the template and mask must be the same "
+                  "size\n  left: (%u, %u)\n right: (%u, %u)", tw, th, mw, mh);
+        return IPH_EINVAL;
+    }
+    return check_match(IPH_U8, w, h, tw, th, method, IPH_PATH_DIRECT);
+}
+
+IPH_API iph_status iph_match_template_with_mask(const uint8_t *image, uint32_t w, uint32_t h,
+                                                const uint8_t *tmpl, uint32_t tw, uint32_t th,
+                                                const uint8_t *mask, uint32_t mw, uint32_t mh,
+                                                int32_t method, float *out) {
+    clear_error();
+    IPH_TRY(check_masked(w, h, tw, th, mw, mh, method));
+    const uint32_t ow = w - tw + 1, oh = h - th + 1;
+    const size_t obytes = (size_t)ow * oh * sizeof(float);
+    if (obytes == 0) return IPH_OK;
+    IPH_TRY(require_device());
+    DevBuf di, dt, dm, dout, dws;
+    IPH_TRY(upload(di, image, (size_t)w * h));
+    IPH_TRY(upload(dt, tmpl, (size_t)tw * th));
+    IPH_TRY(upload(dm, mask, (size_t)tw * th));
+    IPH_TRY(dout.alloc(obytes));
+    IPH_TRY(dws.alloc(masked_match_workspace()));
+    IPH_TRY(launch_masked_match(static_cast<const uint8_t *>(di.p), w, h,
+                                static_cast<const uint8_t *>(dt.p), tw, th,
+                                static_cast<const uint8_t *>(dm.p), method,
+                                static_cast<float *>(dout.p), dws.p, masked_match_workspace(),
+                                nullptr));
+    return finish(dout, out, obytes);
+}
+
 IPH_API iph_status iph_find_extremes(const void *image, int32_t p, uint32_t w, uint32_t h,
                                      iph_extremes *out) {
     clear_error();
@@ -459,6 +495,23 @@ IPH_API iph_status iph_dev_box_filter(const uint8_t *src, uint8_t *dst, uint32_t
     IPH_TRY(check_dims(w, h));
     IPH_TRY(require_device());
     return launch_box_filter(src, dst, w, h, rx, ry, ws, wsb, static_cast<hipStream_t>(stream));
+}
+
+IPH_API size_t iph_dev_match_template_with_mask_workspace(void) {
+    return masked_match_workspace();
+}
+
+IPH_API iph_status iph_dev_match_template_with_mask(const uint8_t *image, uint32_t w, uint32_t h,
+                                                    const uint8_t *tmpl, uint32_t tw,
+                                                    uint32_t th, const uint8_t *mask,
+                                                    uint32_t mw, uint32_t mh, int32_t method,
+                                                    float *out, void *ws, size_t wsb,
+                                                    void *stream) {
+    clear_error();
+    IPH_TRY(check_masked(w, h, tw, th, mw, mh, method));
+    IPH_TRY(require_device());
+    return launch_masked_match(image, w, h, tmpl, tw, th, mask, method, out, ws, wsb,
+                               static_cast<hipStream_t>(stream));
 }
 
 IPH_API size_t iph_dev_match_template_workspace(int32_t p, uint32_t w, uint32_t h, uint32_t tw,

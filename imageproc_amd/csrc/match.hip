@@ -89,6 +89,66 @@ iph_status direct_typed(const P *img, uint32_t w, const P *tpl, uint32_t tw, uin
     return launch_status("match_direct_kernel");
 }
 
+// ---- match_template_with_mask ----------------------------------------------------
+//
+// template_matching.rs:125-163 with the masked methods of :295-376, one output
+// per thread in the reference's operation order (slide_window_at, :442-457:
+// dy outer, dx inner; every product and sum separately rounded f32):
+//   SumOfSquaredErrors:           score += ((t - i) * m)^2
+//   CrossCorrelation:             score += ((t * i) * m) * m
+//   the normalised methods also:  im_im += (i * m)^2, then
+//                                 score / sqrt(mult_square_sum(T, M) * im_im)
+//                                 (the raw score if that norm is 0).
+
+// mult_square_sum, template_matching.rs:381-386: sequential f32 sum of
+// (t * m)^2 (Rust's f32 Sum starts from -0.0).
+__global__ void mult_square_sum_kernel(const uint8_t *__restrict__ tpl,
+                                       const uint8_t *__restrict__ mask, size_t n,
+                                       float *__restrict__ out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    float s = -0.0f;
+    for (size_t k = 0; k < n; k++) {
+        const float q = fmul((float)tpl[k], (float)mask[k]);
+        s = fadd(s, fmul(q, q));
+    }
+    *out = s;
+}
+
+template <int METHOD>
+__global__ __launch_bounds__(256) void match_masked_direct_kernel(
+    const uint8_t *__restrict__ img, int W, const uint8_t *__restrict__ tpl,
+    const uint8_t *__restrict__ mask, int tw, int th, const float *__restrict__ tms,
+    float *__restrict__ out, int ow, int oh) {
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y * 4 + threadIdx.y;
+    if (x >= ow || y >= oh) return;
+    float score = 0.0f, imim = 0.0f;
+    for (int dy = 0; dy < th; dy++) {
+        const uint8_t *irow = img + (size_t)(y + dy) * W + x;
+        const uint8_t *trow = tpl + (size_t)dy * tw;
+        const uint8_t *mrow = mask + (size_t)dy * tw;
+        for (int dx = 0; dx < tw; dx++) {
+            const float i = (float)irow[dx], t = (float)trow[dx], m = (float)mrow[dx];
+            if (METHOD == IPH_SSE || METHOD == IPH_SSE_NORMED) {
+                const float d = fmul(fsub(t, i), m);
+                score = fadd(score, fmul(d, d));
+            } else {
+                score = fadd(score, fmul(fmul(fmul(t, i), m), m));
+            }
+            if (METHOD == IPH_SSE_NORMED || METHOD == IPH_CCORR_NORMED) {
+                const float e = fmul(i, m);
+                imim = fadd(imim, fmul(e, e));
+            }
+        }
+    }
+    if (METHOD == IPH_SSE_NORMED || METHOD == IPH_CCORR_NORMED) {
+        // template_matching.rs:329-330 / 365-366 (correctly rounded sqrt and /)
+        const float norm = sqrtf(fmul(*tms, imim));
+        score = norm > 0.0f ? score / norm : score;
+    }
+    out[(size_t)y * ow + x] = score;
+}
+
 // ---- find_extremes ------------------------------------------------------------
 //
 // Sequential semantics (template_matching.rs:488-503): start from (0,0), a
@@ -257,6 +317,35 @@ iph_status launch_match(const void *img, int32_t p_type, uint32_t w, uint32_t h,
                             tw, th, method, out, ow, oh, tss, s);
     }
     return launch_mfma_match(img, p_type, w, h, tpl, tw, th, method, out, ws, ws_bytes, s);
+}
+
+size_t masked_match_workspace() { return 256; }
+
+iph_status launch_masked_match(const uint8_t *img, uint32_t w, uint32_t h, const uint8_t *tpl,
+                               uint32_t tw, uint32_t th, const uint8_t *mask, int32_t method,
+                               float *out, void *ws, size_t ws_bytes, hipStream_t s) {
+    Workspace wsp(ws, ws_bytes);
+    float *tms = wsp.take<float>(1);
+    if (!tms) {
+        set_error("workspace too small for match_template_with_mask");
+        return IPH_EINVAL;
+    }
+    const uint32_t ow = w - tw + 1, oh = h - th + 1;
+    if ((size_t)ow * oh == 0) return IPH_OK;
+    mult_square_sum_kernel<<<1, 64, 0, s>>>(tpl, mask, (size_t)tw * th, tms);
+    IPH_TRY(launch_status("mult_square_sum_kernel"));
+    dim3 grid((ow + 63) / 64, (oh + 3) / 4), block(64, 4);
+#define LAUNCH(M)                                                                            \
+    match_masked_direct_kernel<M><<<grid, block, 0, s>>>(img, (int)w, tpl, mask, (int)tw,      \
+                                                         (int)th, tms, out, (int)ow, (int)oh)
+    switch (method) {
+    case IPH_SSE: LAUNCH(IPH_SSE); break;
+    case IPH_SSE_NORMED: LAUNCH(IPH_SSE_NORMED); break;
+    case IPH_CCORR: LAUNCH(IPH_CCORR); break;
+    default: LAUNCH(IPH_CCORR_NORMED); break;
+    }
+#undef LAUNCH
+    return launch_status("match_masked_direct_kernel");
 }
 
 size_t extremes_workspace() { return (size_t)kExtMaxBlocks * 2 * sizeof(unsigned long long); }

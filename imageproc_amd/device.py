@@ -110,6 +110,26 @@ def match_template(image: torch.Tensor, template: torch.Tensor, method, path: st
     return out
 
 
+def match_template_with_mask(image: torch.Tensor, template: torch.Tensor, method,
+                             mask: torch.Tensor, out: torch.Tensor | None = None):
+    """match_template_with_mask on device uint8 tensors (template_matching.rs:125-140)."""
+    for t in (image, template, mask):
+        if _code(t) != _TORCH_CODE[torch.uint8]:
+            raise TypeError("match_template_with_mask takes GrayImages (uint8)")
+    h, w = image.shape
+    th, tw = template.shape
+    mh, mw = mask.shape
+    if out is None:
+        out = torch.empty((max(h - th + 1, 0), max(w - tw + 1, 0)), dtype=torch.float32,
+                          device=image.device)
+    wsb = lib().iph_dev_match_template_with_mask_workspace()
+    ws = _workspace("masked", wsb, image.device)
+    check(lib().iph_dev_match_template_with_mask(
+        image.data_ptr(), w, h, template.data_ptr(), tw, th, mask.data_ptr(), mw, mh,
+        int(MatchTemplateMethod(method)), out.data_ptr(), ws.data_ptr(), ws.numel(), _stream()))
+    return out
+
+
 def find_extremes(image: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
     """Writes the 24-byte iph_extremes record into `out` (a device uint8 tensor)."""
     p = _code(image)

@@ -42,6 +42,13 @@ def _gray_or_f32(a, what):
     raise TypeError(f"{what} pixel type {a.dtype} is not u8 or f32")
 
 
+def _gray(a, what):
+    a = np.ascontiguousarray(a)
+    if a.ndim != 2 or a.dtype != np.uint8:
+        raise TypeError(f"{what} must be a 2-D uint8 GrayImage array")
+    return a
+
+
 def match_template(image, template, method: MatchTemplateMethod, path: str = "auto"):
     """template_matching.rs:78-92. Returns a float32 array of shape
     (h - th + 1, w - tw + 1). Panics if the template is larger than the image."""
@@ -62,6 +69,29 @@ def match_template(image, template, method: MatchTemplateMethod, path: str = "au
 def match_template_parallel(image, template, method: MatchTemplateMethod, path: str = "auto"):
     """template_matching.rs:96-110: the same GPU path as match_template."""
     return match_template(image, template, method, path)
+
+
+def match_template_with_mask(image, template, method: MatchTemplateMethod, mask):
+    """template_matching.rs:125-140 (GrayImages; the masked methods of
+    :295-376, bit-identical to the reference). Panics if the mask is not the
+    template's size, or the template is larger than the image."""
+    img = _gray(image, "image")
+    tpl = _gray(template, "template")
+    msk = _gray(mask, "mask")
+    h, w = img.shape
+    th, tw = tpl.shape
+    mh, mw = msk.shape
+    m = int(MatchTemplateMethod(method))
+    oh, ow = max(h - th + 1, 0), max(w - tw + 1, 0)
+    out = np.empty((oh, ow), np.float32)
+    check(lib().iph_match_template_with_mask(ptr(img), w, h, ptr(tpl), tw, th, ptr(msk), mw, mh,
+                                             m, ptr(out)))
+    return out
+
+
+def match_template_with_mask_parallel(image, template, method: MatchTemplateMethod, mask):
+    """template_matching.rs:142-163: the same GPU path."""
+    return match_template_with_mask(image, template, method, mask)
 
 
 @dataclass(frozen=True)

@@ -127,6 +127,19 @@ IPH_API iph_status iph_match_template(const void *image, int32_t p_type, uint32_
                                       uint32_t height, const void *tmpl, uint32_t tw,
                                       uint32_t th, int32_t method, int32_t path, float *out);
 
+/* match_template_with_mask, template_matching.rs:125-140, and
+ * match_template_with_mask_parallel, :142-163, over GrayImages: the masked
+ * methods of :295-376 in the reference's operation order (bit-identical
+ * f32). mask is mw x mh; unequal to the template ->
+ * IPH_EINVAL "... the template and mask must be the same size ..."
+ * (ImageTemplateMask::new, :430-437), checked before the template-vs-image
+ * asserts. out holds (width-tw+1) x (height-th+1) f32. */
+IPH_API iph_status iph_match_template_with_mask(const uint8_t *image, uint32_t width,
+                                                uint32_t height, const uint8_t *tmpl,
+                                                uint32_t tw, uint32_t th, const uint8_t *mask,
+                                                uint32_t mw, uint32_t mh, int32_t method,
+                                                float *out);
+
 /* find_extremes, template_matching.rs:479-511, for Image<Luma<u8>> and
  * Image<Luma<f32>>. Empty image -> IPH_EINVAL "image must be non-empty". */
 IPH_API iph_status iph_find_extremes(const void *image, int32_t p_type, uint32_t width,
@@ -167,6 +180,16 @@ IPH_API size_t iph_dev_box_filter_workspace(uint32_t width, uint32_t height);
 IPH_API iph_status iph_dev_box_filter(const uint8_t *src, uint8_t *dst, uint32_t width,
                                       uint32_t height, uint32_t x_radius, uint32_t y_radius,
                                       void *workspace, size_t workspace_bytes, void *stream);
+
+/* match_template_with_mask on device memory. */
+IPH_API size_t iph_dev_match_template_with_mask_workspace(void);
+IPH_API iph_status iph_dev_match_template_with_mask(const uint8_t *image, uint32_t width,
+                                                    uint32_t height, const uint8_t *tmpl,
+                                                    uint32_t tw, uint32_t th,
+                                                    const uint8_t *mask, uint32_t mw,
+                                                    uint32_t mh, int32_t method, float *out,
+                                                    void *workspace, size_t workspace_bytes,
+                                                    void *stream);
 
 /* match_template on device memory. */
 IPH_API size_t iph_dev_match_template_workspace(int32_t p_type, uint32_t width,

@@ -136,3 +136,14 @@ def test_proposal_search_argument_errors_before_device():
     with pytest.raises(Panic, match="workspace too small"):
         check(lib().iph_dev_proposal_search(None, 64, 64, None, 8, 8, 3, None, None, 8, 4, 4,
                                             None, None, 0, None))
+
+
+def test_masked_match_argument_errors_before_device():
+    """template_matching.rs:430-437: the mask check comes first."""
+    from imageproc_amd._lib import Panic, check, lib
+
+    with pytest.raises(Panic, match="the template and mask must be the same size"):
+        check(lib().iph_match_template_with_mask(None, 4, 4, None, 2, 2, None, 3, 2, 0, None))
+    with pytest.raises(Panic, match="image height must be greater than or equal"):
+        check(lib().iph_match_template_with_mask(None, 4, 1, None, 2, 2, None, 2, 2, 0, None))
+    assert lib().iph_dev_match_template_with_mask_workspace() >= 4
