@@ -253,6 +253,40 @@ IPH_API iph_status iph_gaussian_blur_f32(const void *src, void *dst, uint32_t w,
     return iph_separable_filter(src, dst, w, h, p, k.data(), n, k.data(), n, IPH_F32);
 }
 
+IPH_API iph_status iph_sharpen3x3(const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h) {
+    // identity minus the 3x3 Laplacian, filter/sharpen.rs:14
+    static const int32_t k[9] = {0, -1, 0, -1, 5, -1, 0, -1, 0};
+    return iph_filter_clamped(src, IPH_U8, w, h, k, IPH_I32, 3, 3, dst, IPH_U8);
+}
+
+IPH_API iph_status iph_sharpen_gaussian(const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h,
+                                        float sigma, float amount) {
+    clear_error();
+    if (!(sigma > 0.0f)) {  // gaussian_blur_f32's assert, filter/mod.rs:161
+        set_error("sigma must be > 0.0");
+        return IPH_EINVAL;
+    }
+    std::vector<float> k;
+    if (gaussian_kernel(sigma, k) < 0) {
+        set_error("sigma %g gives more than %u taps", (double)sigma, kMaxTaps);
+        return IPH_EINVAL;
+    }
+    IPH_TRY(check_dims(w, h));
+    const size_t bytes = (size_t)w * h;
+    if (bytes == 0) return IPH_OK;
+    IPH_TRY(require_device());
+    DevBuf din, dout, dws;
+    IPH_TRY(upload(din, src, bytes));
+    IPH_TRY(dout.alloc(bytes));
+    const uint32_t n = (uint32_t)k.size();
+    const size_t wsb = sharpen_gaussian_workspace(w, h, n);
+    IPH_TRY(dws.alloc(wsb));
+    IPH_TRY(launch_sharpen_gaussian(static_cast<const uint8_t *>(din.p),
+                                    static_cast<uint8_t *>(dout.p), w, h, k.data(), n, amount,
+                                    dws.p, wsb, nullptr));
+    return finish(dout, dst, bytes);
+}
+
 static iph_status filter1d_host(const void *src, void *dst, uint32_t w, uint32_t h, int32_t p,
                                 const void *k, uint32_t len, int32_t kt, bool vertical) {
     clear_error();

@@ -627,4 +627,59 @@ iph_status launch_filter_jobs(const void *table_dev, int n, int total_tiles, int
     return launch_status("filter2d_jobs_kernel");
 }
 
+// ---- sharpen_gaussian epilogues (filter/sharpen.rs:31-38) ------------------------
+
+namespace {
+// map_subpixels(image, |x| x as f32)
+__global__ __launch_bounds__(256) void u8_to_f32_kernel(const uint8_t *__restrict__ src,
+                                                        float *__restrict__ dst, size_t n) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = (float)src[i];
+}
+
+// v = (1 + amount) * p - amount * q, each operation rounded separately,
+// then Clamp<f32> for u8 (definitions.rs:120-136: truncation, NaN -> 255).
+__global__ __launch_bounds__(256) void sharpen_combine_kernel(const uint8_t *__restrict__ p,
+                                                              const float *__restrict__ q,
+                                                              float amount, size_t n,
+                                                              uint8_t *__restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float v = __fsub_rn(__fmul_rn(__fadd_rn(1.0f, amount), (float)p[i]),
+                              __fmul_rn(amount, q[i]));
+    out[i] = v < 255.0f ? (v > 0.0f ? (uint8_t)v : (uint8_t)0) : (uint8_t)255;
+}
+}  // namespace
+
+size_t sharpen_gaussian_workspace(uint32_t w, uint32_t h, uint32_t taps) {
+    WorkspacePlan wp;
+    wp.take<float>((size_t)w * h);
+    wp.take<float>((size_t)w * h);
+    wp.take<uint8_t>(separable_workspace(w, h, 1, IPH_F32, taps, IPH_F32));
+    return wp.used;
+}
+
+iph_status launch_sharpen_gaussian(const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h,
+                                   const float *taps, uint32_t n_taps, float amount, void *ws,
+                                   size_t ws_bytes, hipStream_t s) {
+    const size_t n = (size_t)w * h;
+    if (n == 0) return IPH_OK;
+    Workspace wsp(ws, ws_bytes);
+    float *img = wsp.take<float>(n);
+    float *smooth = wsp.take<float>(n);
+    const size_t sep_bytes = separable_workspace(w, h, 1, IPH_F32, n_taps, IPH_F32);
+    uint8_t *sep_ws = wsp.take<uint8_t>(sep_bytes);
+    if (!img || !smooth || !sep_ws) {
+        set_error("workspace too small for sharpen_gaussian");
+        return IPH_EINVAL;
+    }
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    u8_to_f32_kernel<<<blocks, 256, 0, s>>>(src, img, n);
+    IPH_TRY(launch_status("u8_to_f32_kernel"));
+    IPH_TRY(launch_separable(img, smooth, w, h, 1, n, IPH_F32, taps, taps, n_taps, IPH_F32,
+                             sep_ws, sep_bytes, s));
+    sharpen_combine_kernel<<<blocks, 256, 0, s>>>(src, smooth, amount, n, dst);
+    return launch_status("sharpen_combine_kernel");
+}
+
 }  // namespace iph

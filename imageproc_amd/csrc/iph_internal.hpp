@@ -100,6 +100,11 @@ bool blur_mfma_supported(const float *hk, const float *vk, uint32_t len);
 iph_status launch_blur_u8_mfma(const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h,
                                uint32_t n, size_t stride, const float *hk, const float *vk,
                                uint32_t len, hipStream_t s);
+// sharpen_gaussian (filter/sharpen.rs:31-38): f32 image, f32 blur, combine.
+size_t sharpen_gaussian_workspace(uint32_t w, uint32_t h, uint32_t taps);
+iph_status launch_sharpen_gaussian(const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h,
+                                   const float *taps, uint32_t n_taps, float amount, void *ws,
+                                   size_t ws_bytes, hipStream_t s);
 iph_status launch_box_filter(const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h,
                              uint32_t rx, uint32_t ry, void *ws, size_t ws_bytes,
                              hipStream_t s);

@@ -58,6 +58,38 @@ def gaussian_blur_f32(image, sigma: float) -> np.ndarray:
     return out
 
 
+def _gray_u8(image) -> np.ndarray:
+    img = np.ascontiguousarray(image)
+    if img.ndim != 2 or img.dtype != np.uint8:
+        raise TypeError("expected a 2-D uint8 GrayImage array")
+    return img
+
+
+def sharpen3x3(image) -> np.ndarray:
+    """filter/sharpen.rs:13-16: filter_clamped with identity minus the 3x3
+    Laplacian (i32 kernel), u8 out."""
+    img = _gray_u8(image)
+    out = np.empty_like(img)
+    h, w = img.shape
+    check(lib().iph_sharpen3x3(ptr(img), ptr(out), w, h))
+    return out
+
+
+def sharpen3x3_parallel(image) -> np.ndarray:
+    """filter/sharpen.rs:20-24: the same GPU path."""
+    return sharpen3x3(image)
+
+
+def sharpen_gaussian(image, sigma: float, amount: float) -> np.ndarray:
+    """filter/sharpen.rs:31-38: output = clamp((1 + amount) * input - amount *
+    gaussian_blur_f32(input as f32, sigma)). Panics if sigma <= 0."""
+    img = _gray_u8(image)
+    out = np.empty_like(img)
+    h, w = img.shape
+    check(lib().iph_sharpen_gaussian(ptr(img), ptr(out), w, h, float(sigma), float(amount)))
+    return out
+
+
 def separable_filter(image, h_kernel, v_kernel) -> np.ndarray:
     """filter/mod.rs:169-183: horizontal pass, then vertical pass on the
     pixel-typed intermediate."""

@@ -113,6 +113,15 @@ IPH_API iph_status iph_filter_clamped(const void *src, int32_t p_type, uint32_t 
                                       uint32_t height, const void *kernel, int32_t k_type,
                                       uint32_t kw, uint32_t kh, void *dst, int32_t s_type);
 
+/* sharpen3x3, filter/sharpen.rs:13-16 (and sharpen3x3_parallel, :20-24):
+ * filter_clamped with the i32 kernel identity minus the 3x3 Laplacian. */
+IPH_API iph_status iph_sharpen3x3(const uint8_t *src, uint8_t *dst, uint32_t width,
+                                  uint32_t height);
+/* sharpen_gaussian, filter/sharpen.rs:31-38: the image as f32, its f32
+ * gaussian_blur_f32 q, then clamp_u8((1 + amount) * p - amount * q). */
+IPH_API iph_status iph_sharpen_gaussian(const uint8_t *src, uint8_t *dst, uint32_t width,
+                                        uint32_t height, float sigma, float amount);
+
 /* box_filter, filter/box_filter.rs:19-70 (GrayImage only). */
 IPH_API iph_status iph_box_filter(const uint8_t *src, uint8_t *dst, uint32_t width,
                                   uint32_t height, uint32_t x_radius, uint32_t y_radius);

@@ -1,0 +1,44 @@
+"""sharpen3x3 and sharpen_gaussian (filter/sharpen.rs:13-38) on the GPU:
+the reference's truth images (tests/regression.rs:529-539) and the oracle's
+composition on random images, bit for bit."""
+import numpy as np
+import pytest
+
+import oracle as O
+from imageproc_amd import filter as flt
+from imageproc_amd._lib import Panic
+
+pytestmark = pytest.mark.gpu
+F = np.float32
+
+
+def _oracle_sharpen_gaussian(img, sigma, amount):
+    r = img.astype(F)
+    smooth = O.gaussian_blur_f32(r, sigma)
+    a = F(amount)
+    v = (F(1.0) + a) * r - a * smooth
+    return np.where(v < 255, np.where(v > 0, v, 0), 255).astype(np.uint8)
+
+
+def test_robin_truths(fixtures):
+    robin = fixtures["robin"]
+    assert np.array_equal(flt.sharpen3x3(robin), fixtures["robin_sharpen3x3"])
+    assert np.array_equal(flt.sharpen3x3_parallel(robin), fixtures["robin_sharpen3x3"])
+    assert np.array_equal(flt.sharpen_gaussian(robin, 0.7, 7.0), fixtures["robin_sharpen_gaussian"])
+
+
+@pytest.mark.parametrize("shape,sigma,amount", [((37, 53), 1.5, 2.0), ((300, 411), 0.7, 7.0),
+                                                ((128, 96), 4.0, 0.5), ((5, 200), 2.0, -1.0)])
+def test_sharpen_gaussian_random(shape, sigma, amount):
+    img = np.random.default_rng(shape[1]).integers(0, 256, shape, dtype=np.uint8)
+    assert np.array_equal(flt.sharpen_gaussian(img, sigma, amount),
+                          _oracle_sharpen_gaussian(img, sigma, amount))
+
+
+def test_sharpen3x3_random_and_errors():
+    img = np.random.default_rng(1).integers(0, 256, (61, 77), dtype=np.uint8)
+    k = np.array([0, -1, 0, -1, 5, -1, 0, -1, 0], np.int32)
+    want = O.filter_clamped(img, k, 3, 3, np.uint8)
+    assert np.array_equal(flt.sharpen3x3(img), want)
+    with pytest.raises(Panic, match="sigma must be > 0.0"):
+        flt.sharpen_gaussian(img, 0.0, 1.0)
