@@ -92,6 +92,8 @@ _SIGS = {
     "iph_dev_box_filter": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
                                        C.c_uint32, C.c_uint32, C.c_void_p, C.c_size_t,
                                        C.c_void_p]),
+    "iph_gradients": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32,
+                                  C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]),
     "iph_sharpen3x3": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     "iph_sharpen_gaussian": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
                                          C.c_float, C.c_float]),

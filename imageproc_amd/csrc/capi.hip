@@ -253,6 +253,26 @@ IPH_API iph_status iph_gaussian_blur_f32(const void *src, void *dst, uint32_t w,
     return iph_separable_filter(src, dst, w, h, p, k.data(), n, k.data(), n, IPH_F32);
 }
 
+IPH_API iph_status iph_gradients(const uint8_t *src, uint32_t w, uint32_t h, const int32_t *hk,
+                                 uint32_t hkw, uint32_t hkh, const int32_t *vk, uint32_t vkw,
+                                 uint32_t vkh, uint16_t *out) {
+    clear_error();
+    if (hkw == 0 || hkh == 0 || vkw == 0 || vkh == 0) {  // Kernel::new, kernel.rs:24-35
+        set_error("width and height must be non-zero");
+        return IPH_EINVAL;
+    }
+    IPH_TRY(check_dims(w, h));
+    const size_t n = (size_t)w * h;
+    if (n == 0) return IPH_OK;
+    IPH_TRY(require_device());
+    DevBuf din, dout;
+    IPH_TRY(upload(din, src, n));
+    IPH_TRY(dout.alloc(n * sizeof(uint16_t)));
+    IPH_TRY(launch_gradients(static_cast<const uint8_t *>(din.p), w, h, hk, hkw, hkh, vk, vkw,
+                             vkh, static_cast<uint16_t *>(dout.p), nullptr));
+    return finish(dout, out, n * sizeof(uint16_t));
+}
+
 IPH_API iph_status iph_sharpen3x3(const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h) {
     // identity minus the 3x3 Laplacian, filter/sharpen.rs:14
     static const int32_t k[9] = {0, -1, 0, -1, 5, -1, 0, -1, 0};

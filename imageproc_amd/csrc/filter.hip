@@ -682,4 +682,60 @@ iph_status launch_sharpen_gaussian(const uint8_t *src, uint8_t *dst, uint32_t w,
     return launch_status("sharpen_combine_kernel");
 }
 
+// ---- gradients (gradients.rs:99-160) -------------------------------------------
+
+namespace {
+constexpr int kGradMaxTaps = 49;
+struct GradTaps {
+    int32_t h[kGradMaxTaps], v[kGradMaxTaps];
+    int hw, hh, vw, vh;
+};
+
+// filter_clamped::<_, i32, i16> at (x, y) (filter/mod.rs:29-69: ky outer, kx
+// inner, clamp-to-edge, centre (kw/2, kh/2); the i32 sum is exact) and its
+// saturation to i16 (definitions.rs:181-188).
+__device__ __forceinline__ int grad_filter(const uint8_t *__restrict__ src, int W, int H, int x,
+                                           int y, const int32_t *k, int kw, int kh) {
+    int acc = 0;
+    for (int ky = 0; ky < kh; ky++) {
+        const uint8_t *row = src + (size_t)min(max(y + ky - kh / 2, 0), H - 1) * W;
+        for (int kx = 0; kx < kw; kx++) acc += (int)row[min(max(x + kx - kw / 2, 0), W - 1)] * k[ky * kw + kx];
+    }
+    return min(max(acc, -32768), 32767);
+}
+
+// out = (dx^2 + dy^2).sqrt() as u16 (gradient_magnitude, gradients.rs:157-160):
+// f32 squares and sum each rounded once, correctly rounded sqrt, and Rust's
+// saturating float -> u16 cast (truncation toward zero).
+__global__ __launch_bounds__(256) void gradients_kernel(const uint8_t *__restrict__ src, int W,
+                                                        int H, const GradTaps taps,
+                                                        uint16_t *__restrict__ out) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= W || y >= H) return;
+    const float dx = (float)grad_filter(src, W, H, x, y, taps.h, taps.hw, taps.hh);
+    const float dy = (float)grad_filter(src, W, H, x, y, taps.v, taps.vw, taps.vh);
+    const float m = sqrtf(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)));
+    out[(size_t)y * W + x] = m >= 65535.0f ? (uint16_t)65535 : (uint16_t)m;
+}
+}  // namespace
+
+iph_status launch_gradients(const uint8_t *src, uint32_t w, uint32_t h, const int32_t *hk,
+                            uint32_t hkw, uint32_t hkh, const int32_t *vk, uint32_t vkw,
+                            uint32_t vkh, uint16_t *out, hipStream_t s) {
+    if (hkw * hkh > (uint32_t)kGradMaxTaps || vkw * vkh > (uint32_t)kGradMaxTaps) {
+        set_error("gradients takes kernels of at most %d taps", kGradMaxTaps);
+        return IPH_EUNSUPPORTED;
+    }
+    if ((size_t)w * h == 0) return IPH_OK;
+    GradTaps t = {};
+    for (uint32_t i = 0; i < hkw * hkh; i++) t.h[i] = hk[i];
+    for (uint32_t i = 0; i < vkw * vkh; i++) t.v[i] = vk[i];
+    t.hw = (int)hkw;
+    t.hh = (int)hkh;
+    t.vw = (int)vkw;
+    t.vh = (int)vkh;
+    gradients_kernel<<<dim3((w + 63) / 64, (h + 3) / 4), 256, 0, s>>>(src, (int)w, (int)h, t, out);
+    return launch_status("gradients_kernel");
+}
+
 }  // namespace iph

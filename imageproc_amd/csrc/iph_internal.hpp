@@ -100,6 +100,10 @@ bool blur_mfma_supported(const float *hk, const float *vk, uint32_t len);
 iph_status launch_blur_u8_mfma(const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h,
                                uint32_t n, size_t stride, const float *hk, const float *vk,
                                uint32_t len, hipStream_t s);
+// gradients (gradients.rs:99-160) of a GrayImage, u16 magnitudes.
+iph_status launch_gradients(const uint8_t *src, uint32_t w, uint32_t h, const int32_t *hk,
+                            uint32_t hkw, uint32_t hkh, const int32_t *vk, uint32_t vkw,
+                            uint32_t vkh, uint16_t *out, hipStream_t s);
 // sharpen_gaussian (filter/sharpen.rs:31-38): f32 image, f32 blur, combine.
 size_t sharpen_gaussian_workspace(uint32_t w, uint32_t h, uint32_t taps);
 iph_status launch_sharpen_gaussian(const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h,

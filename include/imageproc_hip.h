@@ -113,6 +113,15 @@ IPH_API iph_status iph_filter_clamped(const void *src, int32_t p_type, uint32_t 
                                       uint32_t height, const void *kernel, int32_t k_type,
                                       uint32_t kw, uint32_t kh, void *dst, int32_t s_type);
 
+/* gradients_grayscale / gradients, gradients.rs:15-21, 99-160, for a
+ * GrayImage: filter_clamped::<_, i32, i16> with each kernel (row-major i32,
+ * at most 49 taps each), then (dx^2 + dy^2).sqrt() as u16 per pixel
+ * (sobel_gradients, :198-204, and prewitt_gradients, :207-213, pass the
+ * kernel.rs constants). */
+IPH_API iph_status iph_gradients(const uint8_t *src, uint32_t width, uint32_t height,
+                                 const int32_t *h_kernel, uint32_t hkw, uint32_t hkh,
+                                 const int32_t *v_kernel, uint32_t vkw, uint32_t vkh,
+                                 uint16_t *out);
 /* sharpen3x3, filter/sharpen.rs:13-16 (and sharpen3x3_parallel, :20-24):
  * filter_clamped with the i32 kernel identity minus the 3x3 Laplacian. */
 IPH_API iph_status iph_sharpen3x3(const uint8_t *src, uint8_t *dst, uint32_t width,
