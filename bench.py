@@ -335,6 +335,41 @@ def bench_match(steps: int):
     return res
 
 
+CONSUMER_SIDE = 8192
+
+
+def bench_consumers(steps: int):
+    """integral_image (u32 table) and adaptive_threshold (radius 41) of one
+    8192x8192 u8 image resident in HBM. Algorithmic bytes: integral 1 B read +
+    4 B written per pixel; adaptive threshold 1 B read + 1 B written per pixel
+    (its integral table is an intermediate)."""
+    import torch
+
+    import oracle as O
+    from imageproc_amd import device as dev
+
+    side = CONSUMER_SIDE
+    img = torch.from_numpy(uniform_u8(303, (side, side))).cuda()
+    res = {"workload": f"{side}x{side} u8 uniform noise"}
+    table = dev.integral_image(img)
+    for sq in (False, True):
+        times, _ = timed(lambda: dev.integral_image(img, sq, out=table), steps, 2)
+        ms = float(np.median(times))
+        res["integral_squared" if sq else "integral"] = {
+            "ms": round(ms, 4), "GB/s": round(5.0 * side * side / (ms * 1e-3) / 1e9, 1)}
+    out = dev.adaptive_threshold(img, 41, 0)
+    times, _ = timed(lambda: dev.adaptive_threshold(img, 41, 0, out=out), steps, 2)
+    ms = float(np.median(times))
+    res["adaptive_threshold_r41"] = {"ms": round(ms, 4),
+                                     "Gpix/s": round(side * side / (ms * 1e-3) / 1e9, 1)}
+    # spot check of a 1024x1024 corner against the oracle
+    corner = img[:1024, :1024].contiguous()
+    res["parity_spot_check"] = bool(
+        np.array_equal(dev.adaptive_threshold(corner, 41, 0).cpu().numpy(),
+                       O.adaptive_threshold(corner.cpu().numpy(), 41, 0)))
+    return res
+
+
 def bench_precision(steps: int):
     """CrossCorrelationNormalized on the 4096x4096 f32 field: BF16x3 MFMA vs
     FP32 direct, each with max abs / rel error over every output against the
@@ -520,11 +555,13 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--leg", choices=("all", "blur", "copy", "match", "proposal"), default="all",
+    ap.add_argument("--leg", default="all",
+                    choices=("all", "blur", "copy", "match", "proposal", "consumers"),
                     help="blur: only the blur batch (for counter runs); copy: only a plain "
                          "copy of the same size (counter calibration); match: only the "
                          "match_template leg; proposal: only the learned-proposal search "
-                         "against the full map")
+                         "against the full map; consumers: integral image and "
+                         "adaptive threshold")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X and has no "
@@ -538,6 +575,9 @@ def main():
     print(f"[bench] device: {ip.device_info()}", flush=True)
     if args.leg == "match":
         print(json.dumps({"leg": "match", **bench_match(max(5, args.steps // 2))}), flush=True)
+        return
+    if args.leg == "consumers":
+        print(json.dumps({"leg": "consumers", **bench_consumers(args.steps)}), flush=True)
         return
     if args.leg == "proposal":
         print(json.dumps({"leg": "proposal", **bench_proposal()}), flush=True)
@@ -558,6 +598,7 @@ def main():
     precision = bench_precision(3)
     batch = bench_batch(4)
     proposal = bench_proposal()
+    consumers = bench_consumers(max(5, args.steps // 2))
     cpu = None if args.no_cpu else cpu_blur_baseline()
     line = {
         "metric": "blur Gpix/s & %HBM peak; match_template ms/image at equal argmax vs plain GPU",
@@ -583,6 +624,7 @@ def main():
         "precision_f32": precision,
         "batch": batch,
         "learned_proposal": proposal,
+        "consumers": consumers,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

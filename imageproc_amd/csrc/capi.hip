@@ -253,6 +253,89 @@ IPH_API iph_status iph_gaussian_blur_f32(const void *src, void *dst, uint32_t w,
     return iph_separable_filter(src, dst, w, h, p, k.data(), n, k.data(), n, IPH_F32);
 }
 
+namespace {
+iph_status check_integral(uint32_t w, uint32_t h) {
+    if ((uint64_t)(w + 1ull) * (h + 1ull) >= (1ull << 32)) {
+        set_error("integral image of %ux%u pixels is too large", w, h);
+        return IPH_EINVAL;
+    }
+    return IPH_OK;
+}
+
+iph_status check_adaptive(uint32_t w, uint32_t h, uint32_t block_radius) {
+    if (block_radius == 0) {  // contrast.rs:20
+        set_error("assertion failed: block_radius > 0");
+        return IPH_EINVAL;
+    }
+    if (block_radius > 0x7fffffffu) {  // the reference's `block_radius as i32` turns negative
+        set_error("block_radius %u is above 2^31 - 1", block_radius);
+        return IPH_EINVAL;
+    }
+    return check_integral(w, h);
+}
+}  // namespace
+
+IPH_API iph_status iph_integral_image(const uint8_t *src, uint32_t w, uint32_t h, int32_t squared,
+                                      uint32_t *out) {
+    clear_error();
+    IPH_TRY(check_integral(w, h));
+    IPH_TRY(require_device());
+    const size_t n = (size_t)w * h, obytes = ((size_t)w + 1) * ((size_t)h + 1) * sizeof(uint32_t);
+    const size_t wsb = integral_workspace(w, h);
+    DevBuf din, dout, dws;
+    if (n) IPH_TRY(upload(din, src, n));
+    IPH_TRY(dout.alloc(obytes));
+    IPH_TRY(dws.alloc(wsb));
+    IPH_TRY(launch_integral_image(static_cast<const uint8_t *>(din.p), w, h, squared ? 1 : 0,
+                                  static_cast<uint32_t *>(dout.p), dws.p, wsb, nullptr));
+    return finish(dout, out, obytes);
+}
+
+IPH_API iph_status iph_adaptive_threshold(const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h,
+                                          uint32_t block_radius, int32_t delta) {
+    clear_error();
+    IPH_TRY(check_adaptive(w, h, block_radius));
+    const size_t n = (size_t)w * h;
+    if (n == 0) return IPH_OK;
+    IPH_TRY(require_device());
+    const size_t wsb = adaptive_threshold_workspace(w, h);
+    DevBuf din, dout, dws;
+    IPH_TRY(upload(din, src, n));
+    IPH_TRY(dout.alloc(n));
+    IPH_TRY(dws.alloc(wsb));
+    IPH_TRY(launch_adaptive_threshold(static_cast<const uint8_t *>(din.p), w, h, block_radius,
+                                      delta, static_cast<uint8_t *>(dout.p), dws.p, wsb, nullptr));
+    return finish(dout, dst, n);
+}
+
+IPH_API size_t iph_dev_integral_image_workspace(uint32_t w, uint32_t h) {
+    return integral_workspace(w, h);
+}
+
+IPH_API iph_status iph_dev_integral_image(const uint8_t *src, uint32_t w, uint32_t h,
+                                          int32_t squared, uint32_t *out, void *ws, size_t wsb,
+                                          void *stream) {
+    clear_error();
+    IPH_TRY(check_integral(w, h));
+    IPH_TRY(require_device());
+    return launch_integral_image(src, w, h, squared ? 1 : 0, out, ws, wsb,
+                                 static_cast<hipStream_t>(stream));
+}
+
+IPH_API size_t iph_dev_adaptive_threshold_workspace(uint32_t w, uint32_t h) {
+    return adaptive_threshold_workspace(w, h);
+}
+
+IPH_API iph_status iph_dev_adaptive_threshold(const uint8_t *src, uint8_t *dst, uint32_t w,
+                                              uint32_t h, uint32_t block_radius, int32_t delta,
+                                              void *ws, size_t wsb, void *stream) {
+    clear_error();
+    IPH_TRY(check_adaptive(w, h, block_radius));
+    IPH_TRY(require_device());
+    return launch_adaptive_threshold(src, w, h, block_radius, delta, dst, ws, wsb,
+                                     static_cast<hipStream_t>(stream));
+}
+
 IPH_API iph_status iph_gradients(const uint8_t *src, uint32_t w, uint32_t h, const int32_t *hk,
                                  uint32_t hkw, uint32_t hkh, const int32_t *vk, uint32_t vkw,
                                  uint32_t vkh, uint16_t *out) {

@@ -738,4 +738,297 @@ iph_status launch_gradients(const uint8_t *src, uint32_t w, uint32_t h, const in
     return launch_status("gradients_kernel");
 }
 
+// ---- integral images (integral_image.rs:50-152) and adaptive_threshold ------------
+//
+// (W+1) x (H+1) u32 tables with a zero top row and left column, every sum
+// modulo 2^32 (the reference's u32 additions wrap in release builds, and all
+// arithmetic here is u32 too, so any order of additions gives the same bits).
+// The image is cut into 64 x 64 tiles and read twice, written once:
+//   1. tile_sums: per tile, its 64 column sums and 64 row sums;
+//   2. scan_aux: column sums scanned down the tile rows (exclusive), row sums
+//      scanned across the tile columns (exclusive);
+//   3. scan_pin: each tile row's exclusive column sums scanned across the
+//      width (inclusive), giving P[ty][x] = sum of all pixels above the tile
+//      row in columns <= x;
+//   4. emit: per tile, out(y, x) = P[ty][x] + sum over the tile's rows
+//      y' <= y of (R[tx][y'] + in-tile prefix of row y' up to x), where
+//      R[tx][y'] is the sum of row y' left of the tile.
+// Workspace: tiles_y * W + tiles_x * H u32.
+
+namespace {
+constexpr int kIntTile = 64;
+
+size_t int_tiles(uint32_t n) { return ((size_t)n + kIntTile - 1) / kIntTile; }
+
+template <bool SQ>
+__device__ inline uint32_t int_px(const uint8_t *__restrict__ src, int W, int H, int x, int y) {
+    const uint32_t p = (x < W && y < H) ? (uint32_t)src[(size_t)y * W + x] : 0u;
+    return SQ ? p * p : p;
+}
+
+template <bool SQ>
+__global__ __launch_bounds__(256) void integral_tile_sums_kernel(
+    const uint8_t *__restrict__ src, int W, int H, int tiles_x, size_t ntiles,
+    uint32_t *__restrict__ colsum, uint32_t *__restrict__ rowsum) {
+    __shared__ uint32_t tile[kIntTile][kIntTile + 1];
+    __shared__ uint32_t cpart[4][kIntTile];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
+    for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int tx = (int)(t % (size_t)tiles_x), ty = (int)(t / (size_t)tiles_x);
+        const int x = tx * kIntTile + lane, y0 = ty * kIntTile + wave * 16;
+        uint32_t v[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) v[i] = int_px<SQ>(src, W, H, x, y0 + i);
+        uint32_t cs = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            cs += v[i];
+            tile[wave * 16 + i][lane] = v[i];
+        }
+        cpart[wave][lane] = cs;
+        __syncthreads();
+        uint32_t rs = 0;
+#pragma unroll
+        for (int j = 0; j < 16; j++) rs += tile[r][q * 16 + j];
+        rs += __shfl_xor(rs, 1);
+        rs += __shfl_xor(rs, 2);
+        const int yr = ty * kIntTile + r;
+        if (q == 0 && yr < H) rowsum[(size_t)tx * H + yr] = rs;
+        if (wave == 0 && x < W)
+            colsum[(size_t)ty * W + x] = cpart[0][lane] + cpart[1][lane] + cpart[2][lane] +
+                                         cpart[3][lane];
+        __syncthreads();
+    }
+}
+
+// Threads [0, W) scan colsum[.][x] down the tile rows, threads [W, W + H)
+// scan rowsum[.][y] across the tile columns; both exclusive and in place,
+// loads issued sixteen at a time.
+__global__ __launch_bounds__(256) void integral_scan_aux_kernel(uint32_t *__restrict__ colsum,
+                                                                int W, int tiles_y,
+                                                                uint32_t *__restrict__ rowsum,
+                                                                int H, int tiles_x) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    uint32_t *p;
+    size_t stride;
+    int n;
+    if (i < (size_t)W) {
+        p = colsum + i;
+        stride = (size_t)W;
+        n = tiles_y;
+    } else if (i < (size_t)W + H) {
+        p = rowsum + (i - W);
+        stride = (size_t)H;
+        n = tiles_x;
+    } else {
+        return;
+    }
+    uint32_t run = 0;
+    for (int k0 = 0; k0 < n; k0 += 16) {
+        uint32_t v[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) v[u] = k0 + u < n ? p[(size_t)(k0 + u) * stride] : 0u;
+#pragma unroll
+        for (int u = 0; u < 16; u++)
+            if (k0 + u < n) {
+                p[(size_t)(k0 + u) * stride] = run;
+                run += v[u];
+            }
+    }
+}
+
+// One workgroup per tile row: inclusive scan of colsum[ty][0..W) in place,
+// 256 columns per pass.
+__global__ __launch_bounds__(256) void integral_scan_pin_kernel(uint32_t *__restrict__ colsum,
+                                                                int W, int tiles_y) {
+    __shared__ uint32_t part[4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int ty = blockIdx.x; ty < tiles_y; ty += gridDim.x) {
+        uint32_t *row = colsum + (size_t)ty * W;
+        uint32_t carry = 0;
+        for (int x0 = 0; x0 < W; x0 += 256) {
+            const int x = x0 + tid;
+            const uint32_t v = x < W ? row[x] : 0u;
+            uint32_t inc = v;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t t = __shfl_up(inc, o);
+                if (lane >= o) inc += t;
+            }
+            if (lane == 63) part[wave] = inc;
+            __syncthreads();
+            uint32_t before = carry;
+            for (int w2 = 0; w2 < wave; w2++) before += part[w2];
+            if (x < W) row[x] = before + inc;
+            carry += part[0] + part[1] + part[2] + part[3];
+            __syncthreads();
+        }
+    }
+}
+
+template <bool SQ>
+__global__ __launch_bounds__(256) void integral_emit_kernel(
+    const uint8_t *__restrict__ src, int W, int H, int tiles_x, size_t ntiles,
+    const uint32_t *__restrict__ pin, const uint32_t *__restrict__ rleft,
+    uint32_t *__restrict__ out) {
+    __shared__ uint32_t tile[kIntTile][kIntTile + 1];
+    __shared__ uint32_t vpart[4][kIntTile];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
+    const size_t W1 = (size_t)W + 1;
+    for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int tx = (int)(t % (size_t)tiles_x), ty = (int)(t / (size_t)tiles_x);
+        const int x = tx * kIntTile + lane, y0 = ty * kIntTile + wave * 16;
+#pragma unroll
+        for (int i = 0; i < 16; i++) tile[wave * 16 + i][lane] = int_px<SQ>(src, W, H, x, y0 + i);
+        __syncthreads();
+        // row prefix inside the tile plus the part of the row left of it
+        uint32_t h[16], pre = 0;
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            pre += tile[r][q * 16 + j];
+            h[j] = pre;
+        }
+        uint32_t a = pre;
+        uint32_t u = __shfl_up(a, 1);
+        if (q >= 1) a += u;
+        u = __shfl_up(a, 2);
+        if (q >= 2) a += u;
+        const int yr = ty * kIntTile + r;
+        const uint32_t off = (a - pre) + (yr < H ? rleft[(size_t)tx * H + yr] : 0u);
+#pragma unroll
+        for (int j = 0; j < 16; j++) tile[r][q * 16 + j] = h[j] + off;
+        __syncthreads();
+        // column prefix down the tile, plus everything above the tile row
+        uint32_t v[16], run = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            run += tile[wave * 16 + i][lane];
+            v[i] = run;
+        }
+        vpart[wave][lane] = run;
+        __syncthreads();
+        uint32_t base = x < W ? pin[(size_t)ty * W + x] : 0u;
+        for (int w2 = 0; w2 < wave; w2++) base += vpart[w2][lane];
+        if (x < W) {
+#pragma unroll
+            for (int i = 0; i < 16; i++)
+                if (y0 + i < H) out[(size_t)(y0 + i + 1) * W1 + x + 1] = v[i] + base;
+        }
+        // the zero top row and left column
+        if (ty == 0 && wave == 0) {
+            if (x < W) out[x + 1] = 0u;
+            if (tx == 0 && lane == 0) out[0] = 0u;
+        }
+        if (tx == 0 && lane == 0) {
+            for (int i = 0; i < 16; i++)
+                if (y0 + i < H) out[(size_t)(y0 + i + 1) * W1] = 0u;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void zero_u32_kernel(uint32_t *__restrict__ p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        p[i] = 0u;
+}
+
+// contrast.rs:19-47: the block mean from the integral image (sum_image_pixels,
+// integral_image.rs:262-268, u32 wrapping, then u32 division) and
+// pixel >= mean - delta (i32, wrapping) -> 255, else 0.
+__global__ __launch_bounds__(256) void adaptive_threshold_kernel(const uint8_t *__restrict__ src,
+                                                                 const uint32_t *__restrict__ ii,
+                                                                 int W, int H, int r, int delta,
+                                                                 uint8_t *__restrict__ dst) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    if (x >= W) return;
+    for (int y = blockIdx.y * 4 + (threadIdx.x >> 6); y < H; y += gridDim.y * 4) {
+    const int ylo = (int)max(0LL, (long long)y - r);
+    const int yhi = (int)min((long long)H - 1, (long long)y + r);
+    const int xlo = (int)max(0LL, (long long)x - r);
+    const int xhi = (int)min((long long)W - 1, (long long)x + r);
+    const size_t W1 = (size_t)W + 1;
+    const uint32_t a = ii[(size_t)(yhi + 1) * W1 + xhi + 1], b = ii[(size_t)ylo * W1 + xlo];
+    const uint32_t c = ii[(size_t)ylo * W1 + xhi + 1], d = ii[(size_t)(yhi + 1) * W1 + xlo];
+    const uint32_t count = (uint32_t)(yhi - ylo + 1) * (uint32_t)(xhi - xlo + 1);
+    const uint32_t mean = (a + b - c - d) / count;
+    const int32_t thr = (int32_t)((uint32_t)mean - (uint32_t)delta);
+    dst[(size_t)y * W + x] = (int32_t)src[(size_t)y * W + x] >= thr ? 255 : 0;
+    }
+}
+}  // namespace
+
+size_t integral_workspace(uint32_t w, uint32_t h) {
+    const size_t n = int_tiles(h) * w + int_tiles(w) * h;
+    return (n * sizeof(uint32_t) + 255) & ~(size_t)255;
+}
+
+iph_status launch_integral_image(const uint8_t *src, uint32_t w, uint32_t h, int squared,
+                                 uint32_t *out, void *ws, size_t wsb, hipStream_t s) {
+    const size_t w1 = (size_t)w + 1;
+    if (w == 0 || h == 0) {  // all zeros, integral_image.rs:110-112
+        const size_t n = w1 * (h + 1);
+        const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+        zero_u32_kernel<<<grid, 256, 0, s>>>(out, n);
+        return launch_status("zero_u32_kernel");
+    }
+    if (wsb < integral_workspace(w, h)) {
+        set_error("integral image workspace of %zu bytes is below the %zu needed", wsb,
+                  integral_workspace(w, h));
+        return IPH_EINVAL;
+    }
+    const int tx = (int)int_tiles(w), ty = (int)int_tiles(h);
+    uint32_t *colsum = static_cast<uint32_t *>(ws);
+    uint32_t *rowsum = colsum + (size_t)ty * w;
+    const size_t ntiles = (size_t)tx * ty;
+    const unsigned grid = (unsigned)std::min<size_t>(ntiles, 1u << 20);
+    if (squared)
+        integral_tile_sums_kernel<true><<<grid, 256, 0, s>>>(src, (int)w, (int)h, tx, ntiles,
+                                                             colsum, rowsum);
+    else
+        integral_tile_sums_kernel<false><<<grid, 256, 0, s>>>(src, (int)w, (int)h, tx, ntiles,
+                                                              colsum, rowsum);
+    IPH_TRY(launch_status("integral_tile_sums_kernel"));
+    integral_scan_aux_kernel<<<(unsigned)(((size_t)w + h + 255) / 256), 256, 0, s>>>(
+        colsum, (int)w, ty, rowsum, (int)h, tx);
+    IPH_TRY(launch_status("integral_scan_aux_kernel"));
+    integral_scan_pin_kernel<<<(unsigned)std::min(ty, 1 << 20), 256, 0, s>>>(colsum, (int)w, ty);
+    IPH_TRY(launch_status("integral_scan_pin_kernel"));
+    if (squared)
+        integral_emit_kernel<true><<<grid, 256, 0, s>>>(src, (int)w, (int)h, tx, ntiles, colsum,
+                                                        rowsum, out);
+    else
+        integral_emit_kernel<false><<<grid, 256, 0, s>>>(src, (int)w, (int)h, tx, ntiles, colsum,
+                                                         rowsum, out);
+    return launch_status("integral_emit_kernel");
+}
+
+size_t adaptive_threshold_workspace(uint32_t w, uint32_t h) {
+    const size_t table = (((size_t)w + 1) * ((size_t)h + 1) * sizeof(uint32_t) + 255) & ~(size_t)255;
+    return table + integral_workspace(w, h);
+}
+
+iph_status launch_adaptive_threshold(const uint8_t *src, uint32_t w, uint32_t h,
+                                     uint32_t block_radius, int32_t delta, uint8_t *dst, void *ws,
+                                     size_t wsb, hipStream_t s) {
+    if (w == 0 || h == 0) return IPH_OK;
+    if (wsb < adaptive_threshold_workspace(w, h)) {
+        set_error("adaptive threshold workspace of %zu bytes is below the %zu needed", wsb,
+                  adaptive_threshold_workspace(w, h));
+        return IPH_EINVAL;
+    }
+    const size_t table =
+        (((size_t)w + 1) * ((size_t)h + 1) * sizeof(uint32_t) + 255) & ~(size_t)255;
+    uint32_t *ii = static_cast<uint32_t *>(ws);
+    IPH_TRY(launch_integral_image(src, w, h, 0, ii, static_cast<char *>(ws) + table, wsb - table,
+                                  s));
+    // A radius past the image's larger side clips to the same block.
+    const int r = (int)std::min<uint32_t>(block_radius, std::max(w, h));
+    const unsigned gy = std::min((h + 3) / 4, 65535u);
+    adaptive_threshold_kernel<<<dim3((w + 63) / 64, gy), 256, 0, s>>>(
+        src, ii, (int)w, (int)h, r, delta, dst);
+    return launch_status("adaptive_threshold_kernel");
+}
+
 }  // namespace iph

@@ -100,6 +100,15 @@ bool blur_mfma_supported(const float *hk, const float *vk, uint32_t len);
 iph_status launch_blur_u8_mfma(const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h,
                                uint32_t n, size_t stride, const float *hk, const float *vk,
                                uint32_t len, hipStream_t s);
+// integral_image / integral_squared_image (integral_image.rs:50-152), u32,
+// (w+1) x (h+1); adaptive_threshold (contrast.rs:19-47).
+size_t integral_workspace(uint32_t w, uint32_t h);
+iph_status launch_integral_image(const uint8_t *src, uint32_t w, uint32_t h, int squared,
+                                 uint32_t *out, void *ws, size_t wsb, hipStream_t s);
+size_t adaptive_threshold_workspace(uint32_t w, uint32_t h);
+iph_status launch_adaptive_threshold(const uint8_t *src, uint32_t w, uint32_t h,
+                                     uint32_t block_radius, int32_t delta, uint8_t *dst, void *ws,
+                                     size_t wsb, hipStream_t s);
 // gradients (gradients.rs:99-160) of a GrayImage, u16 magnitudes.
 iph_status launch_gradients(const uint8_t *src, uint32_t w, uint32_t h, const int32_t *hk,
                             uint32_t hkw, uint32_t hkh, const int32_t *vk, uint32_t vkw,

@@ -130,6 +130,39 @@ def match_template_with_mask(image: torch.Tensor, template: torch.Tensor, method
     return out
 
 
+def integral_image(image: torch.Tensor, squared: bool = False,
+                   out: torch.Tensor | None = None) -> torch.Tensor:
+    """integral_image / integral_squared_image::<_, u32> (integral_image.rs:50-95)
+    of a device uint8 tensor, as an (h + 1, w + 1) int32 tensor holding the u32
+    bits."""
+    if _code(image) != _TORCH_CODE[torch.uint8]:
+        raise TypeError("integral_image takes GrayImages (uint8)")
+    h, w = image.shape
+    if out is None:
+        out = torch.empty((h + 1, w + 1), dtype=torch.int32, device=image.device)
+    wsb = lib().iph_dev_integral_image_workspace(w, h)
+    ws = _workspace("integral", wsb, image.device)
+    check(lib().iph_dev_integral_image(image.data_ptr(), w, h, int(bool(squared)),
+                                       out.data_ptr(), ws.data_ptr(), ws.numel(), _stream()))
+    return out
+
+
+def adaptive_threshold(image: torch.Tensor, block_radius: int, delta: int,
+                       out: torch.Tensor | None = None) -> torch.Tensor:
+    """adaptive_threshold (contrast.rs:19-47) of a device uint8 tensor."""
+    if _code(image) != _TORCH_CODE[torch.uint8]:
+        raise TypeError("adaptive_threshold takes GrayImages (uint8)")
+    h, w = image.shape
+    if out is None:
+        out = torch.empty_like(image)
+    wsb = lib().iph_dev_adaptive_threshold_workspace(w, h)
+    ws = _workspace("adaptive", wsb, image.device)
+    check(lib().iph_dev_adaptive_threshold(image.data_ptr(), out.data_ptr(), w, h,
+                                           int(block_radius), int(delta), ws.data_ptr(),
+                                           ws.numel(), _stream()))
+    return out
+
+
 def find_extremes(image: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
     """Writes the 24-byte iph_extremes record into `out` (a device uint8 tensor)."""
     p = _code(image)

@@ -113,6 +113,18 @@ IPH_API iph_status iph_filter_clamped(const void *src, int32_t p_type, uint32_t 
                                       uint32_t height, const void *kernel, int32_t k_type,
                                       uint32_t kw, uint32_t kh, void *dst, int32_t s_type);
 
+/* integral_image::<_, u32> (squared = 0) and integral_squared_image::<_, u32>
+ * (squared != 0), integral_image.rs:50-57, 88-95, 98-152, of a GrayImage:
+ * out holds (width+1) x (height+1) u32, zero top row and left column, every
+ * sum modulo 2^32 (the reference's u32 additions wrap in release builds). */
+IPH_API iph_status iph_integral_image(const uint8_t *src, uint32_t width, uint32_t height,
+                                      int32_t squared, uint32_t *out);
+/* adaptive_threshold, contrast.rs:19-47: 255 where pixel >= floor(block
+ * mean) - delta over the clipped (2r+1)^2 block, else 0. block_radius 0 ->
+ * IPH_EINVAL "assertion failed: block_radius > 0"; block_radius above 2^31 - 1
+ * (where the reference's i32 cast of it goes negative) -> IPH_EINVAL. */
+IPH_API iph_status iph_adaptive_threshold(const uint8_t *src, uint8_t *dst, uint32_t width,
+                                          uint32_t height, uint32_t block_radius, int32_t delta);
 /* gradients_grayscale / gradients, gradients.rs:15-21, 99-160, for a
  * GrayImage: filter_clamped::<_, i32, i16> with each kernel (row-major i32,
  * at most 49 taps each), then (dx^2 + dy^2).sqrt() as u16 per pixel
@@ -200,6 +212,17 @@ IPH_API iph_status iph_dev_box_filter(const uint8_t *src, uint8_t *dst, uint32_t
                                       void *workspace, size_t workspace_bytes, void *stream);
 
 /* match_template_with_mask on device memory. */
+/* iph_integral_image / iph_adaptive_threshold on device buffers; workspace of
+ * the bytes the _workspace call returns, out (w+1) x (h+1) u32. */
+IPH_API size_t iph_dev_integral_image_workspace(uint32_t width, uint32_t height);
+IPH_API iph_status iph_dev_integral_image(const uint8_t *src, uint32_t width, uint32_t height,
+                                          int32_t squared, uint32_t *out, void *workspace,
+                                          size_t workspace_bytes, void *stream);
+IPH_API size_t iph_dev_adaptive_threshold_workspace(uint32_t width, uint32_t height);
+IPH_API iph_status iph_dev_adaptive_threshold(const uint8_t *src, uint8_t *dst, uint32_t width,
+                                              uint32_t height, uint32_t block_radius,
+                                              int32_t delta, void *workspace,
+                                              size_t workspace_bytes, void *stream);
 IPH_API size_t iph_dev_match_template_with_mask_workspace(void);
 IPH_API iph_status iph_dev_match_template_with_mask(const uint8_t *image, uint32_t width,
                                                     uint32_t height, const uint8_t *tmpl,

@@ -191,3 +191,15 @@ def integral_image(img, squared=False):
     out = np.zeros((h + 1, w + 1), np.uint32)
     lib().or_integral_image_u32(_p(img), w, h, _p(out), int(squared))
     return out
+
+
+def adaptive_threshold(img, block_radius, delta):
+    img = _img(img)
+    h, w = img.shape
+    out = np.zeros((h, w), np.uint8)
+    rc = lib().or_adaptive_threshold(_p(img), w, h, C.c_uint32(block_radius),
+                                     C.c_int32(delta), _p(out))
+    if rc == -1:
+        raise ValueError("assertion failed: block_radius > 0")
+    assert rc == 0
+    return out

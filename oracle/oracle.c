@@ -662,3 +662,36 @@ void or_integral_image_u32(const uint8_t *img, uint32_t w, uint32_t h, uint32_t 
         }
     }
 }
+
+/* contrast.rs:19-47: mean over the clipped block by sum_image_pixels
+ * (integral_image.rs:250-269, u32 wrapping adds/subs) divided by the pixel
+ * count, then pixel (i32) >= mean (as i32) - delta, wrapping. */
+int or_adaptive_threshold(const uint8_t *img, uint32_t w, uint32_t h, uint32_t block_radius,
+                          int32_t delta, uint8_t *out) {
+    if (block_radius == 0) return -1;
+    if (w == 0 || h == 0) return 0;
+    uint32_t *ii = malloc(sizeof(uint32_t) * ((size_t)w + 1) * ((size_t)h + 1));
+    if (!ii) return -2;
+    or_integral_image_u32(img, w, h, ii, 0);
+    const size_t ow = (size_t)w + 1;
+    for (uint32_t y = 0; y < h; y++) {
+        const int64_t ylo64 = (int64_t)y - (int64_t)block_radius;
+        const uint32_t ylo = ylo64 < 0 ? 0 : (uint32_t)ylo64;
+        const uint64_t yhi64 = (uint64_t)y + block_radius;
+        const uint32_t yhi = yhi64 > h - 1 ? h - 1 : (uint32_t)yhi64;
+        for (uint32_t x = 0; x < w; x++) {
+            const int64_t xlo64 = (int64_t)x - (int64_t)block_radius;
+            const uint32_t xlo = xlo64 < 0 ? 0 : (uint32_t)xlo64;
+            const uint64_t xhi64 = (uint64_t)x + block_radius;
+            const uint32_t xhi = xhi64 > w - 1 ? w - 1 : (uint32_t)xhi64;
+            const uint32_t n = (yhi - ylo + 1) * (xhi - xlo + 1);
+            const uint32_t sum = ii[((size_t)yhi + 1) * ow + xhi + 1] + ii[(size_t)ylo * ow + xlo] -
+                                 ii[(size_t)ylo * ow + xhi + 1] - ii[((size_t)yhi + 1) * ow + xlo];
+            const uint32_t mean = sum / n;
+            const int32_t thr = (int32_t)(mean - (uint32_t)delta);
+            out[(size_t)y * w + x] = (int32_t)img[(size_t)y * w + x] >= thr ? 255 : 0;
+        }
+    }
+    free(ii);
+    return 0;
+}

@@ -106,6 +106,11 @@ int or_find_extremes_u8(const uint8_t *img, uint32_t w, uint32_t h, uint8_t vals
 void or_integral_image_u32(const uint8_t *img, uint32_t w, uint32_t h, uint32_t *out,
                            int squared);
 
+/* adaptive_threshold, contrast.rs:19-47 (radii below 2^31 whose clipped
+ * block end does not overflow u32). Returns -1 for block_radius 0. */
+int or_adaptive_threshold(const uint8_t *img, uint32_t w, uint32_t h, uint32_t block_radius,
+                          int32_t delta, uint8_t *out);
+
 #ifdef __cplusplus
 }
 #endif

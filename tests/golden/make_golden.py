@@ -38,6 +38,7 @@ def main(ref_root):
         "robin_sharpen3x3": "truth/robin_sharpen3x3.png",
         "robin_sharpen_gaussian": "truth/robin_sharpen_gaussian.png",
         "elephant_gradients": "truth/elephant_gradients.png",
+        "zebra_adaptive_threshold": "truth/zebra_adaptive_threshold.png",
     }
     arrays, shas = {}, {}
     for key, rel in items.items():

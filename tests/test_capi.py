@@ -147,3 +147,19 @@ def test_masked_match_argument_errors_before_device():
     with pytest.raises(Panic, match="image height must be greater than or equal"):
         check(lib().iph_match_template_with_mask(None, 4, 1, None, 2, 2, None, 2, 2, 0, None))
     assert lib().iph_dev_match_template_with_mask_workspace() >= 4
+
+
+def test_adaptive_threshold_argument_errors_before_device():
+    """contrast.rs:20 asserts block_radius > 0 before any work."""
+    from imageproc_amd._lib import Panic
+    from imageproc_amd.contrast import adaptive_threshold
+
+    img = np.zeros((3, 3), np.uint8)
+    with pytest.raises(Panic, match="assertion failed: block_radius > 0"):
+        adaptive_threshold(img, 0, 0)
+    with pytest.raises(Panic, match="above 2\\^31 - 1"):
+        adaptive_threshold(img, 2 ** 31, 0)
+    with pytest.raises(TypeError):
+        adaptive_threshold(img.astype(np.uint16), 1, 0)
+    assert _lib.lib().iph_dev_integral_image_workspace(100, 70) >= 4 * (2 * 100 + 2 * 70)
+    assert _lib.lib().iph_dev_adaptive_threshold_workspace(100, 70) >= 4 * 101 * 71

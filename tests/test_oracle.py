@@ -325,3 +325,66 @@ def test_integral_image_against_cumsum(rng):
     img = gray_image([[1, 2, 3], [4, 5, 6]])
     assert np.array_equal(O.integral_image(img),
                           np.array([[0, 0, 0, 0], [0, 1, 3, 6], [0, 5, 12, 21]], np.uint32))
+    # integral_image.rs:66-90 doc example
+    assert np.array_equal(O.integral_image(img, True),
+                          np.array([[0, 0, 0, 0], [0, 1, 5, 14], [0, 17, 46, 91]], np.uint32))
+
+
+# ---- adaptive threshold (contrast.rs:19-47) ----------------------------------------------
+
+def _adaptive_brute(img, r, delta):
+    """Block means by direct summation over the clipped window (no integral
+    image), then the reference's comparison."""
+    h, w = img.shape
+    out = np.zeros_like(img)
+    for y in range(h):
+        for x in range(w):
+            blk = img[max(0, y - r):min(h - 1, y + r) + 1, max(0, x - r):min(w - 1, x + r) + 1]
+            mean = int(blk.astype(np.int64).sum()) // blk.size
+            out[y, x] = 255 if int(img[y, x]) >= mean - delta else 0
+    return out
+
+
+def test_adaptive_threshold_reference_unit_tests():
+    # contrast.rs:471-477 adaptive_threshold_constant
+    assert (O.adaptive_threshold(np.full((3, 3), 100, np.uint8), 1, 0) == 255).all()
+    # contrast.rs:479-492 one darker pixel
+    for y in range(3):
+        for x in range(3):
+            img = np.full((3, 3), 200, np.uint8)
+            img[y, x] = 100
+            exp = np.full((3, 3), 255, np.uint8)
+            exp[y, x] = 0
+            assert np.array_equal(O.adaptive_threshold(img, 1, 0), exp)
+    # contrast.rs:494-523 one lighter pixel
+    for y in range(5):
+        for x in range(5):
+            img = np.full((5, 5), 100, np.uint8)
+            img[y, x] = 200
+            got = O.adaptive_threshold(img, 1, 0)
+            for yb in range(5):
+                for xb in range(5):
+                    if (yb, xb) == (y, x):
+                        assert got[yb, xb] == 255
+                    elif abs(yb - y) <= 1 and abs(xb - x) <= 1:
+                        assert got[yb, xb] == 0
+                    else:
+                        assert got[yb, xb] == 255
+    # contrast.rs:525-540 delta
+    img = np.full((3, 3), 100, np.uint8)
+    img[2, 2] = 0
+    assert (O.adaptive_threshold(img, 1, 100) == 255).all()
+    exp = np.full((3, 3), 255, np.uint8)
+    exp[2, 2] = 0
+    assert np.array_equal(O.adaptive_threshold(img, 1, 50), exp)
+    with pytest.raises(ValueError, match="block_radius > 0"):
+        O.adaptive_threshold(img, 0, 0)
+
+
+def test_adaptive_threshold_brute_force_and_truth(rng, fixtures):
+    for shape, r, delta in [((9, 14), 2, 0), ((17, 6), 4, -7), ((12, 12), 30, 3), ((1, 9), 1, 0)]:
+        img = rng.integers(0, 256, shape, dtype=np.uint8)
+        assert np.array_equal(O.adaptive_threshold(img, r, delta), _adaptive_brute(img, r, delta))
+    # tests/regression.rs:576-582
+    assert np.array_equal(O.adaptive_threshold(fixtures["zebra"], 41, 0),
+                          fixtures["zebra_adaptive_threshold"])
