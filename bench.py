@@ -339,8 +339,8 @@ CONSUMER_SIDE = 8192
 
 
 def bench_consumers(steps: int):
-    """integral_image (u32 table) and adaptive_threshold (radius 41) of one
-    8192x8192 u8 image resident in HBM. Algorithmic bytes: integral 1 B read +
+    """integral_image (u32 table), adaptive_threshold (radius 41) and canny
+    (thresholds 250, 300) of one 8192x8192 u8 image resident in HBM. Algorithmic bytes: integral 1 B read +
     4 B written per pixel; adaptive threshold 1 B read + 1 B written per pixel
     (its integral table is an intermediate)."""
     import torch
@@ -362,11 +362,18 @@ def bench_consumers(steps: int):
     ms = float(np.median(times))
     res["adaptive_threshold_r41"] = {"ms": round(ms, 4),
                                      "Gpix/s": round(side * side / (ms * 1e-3) / 1e9, 1)}
+    edges_out = dev.canny(img, 250.0, 300.0)
+    times, _ = timed(lambda: dev.canny(img, 250.0, 300.0, out=edges_out), max(3, steps // 2), 1)
+    ms = float(np.median(times))
+    res["canny_250_300"] = {"ms": round(ms, 4),
+                            "Gpix/s": round(side * side / (ms * 1e-3) / 1e9, 1)}
     # spot check of a 1024x1024 corner against the oracle
     corner = img[:1024, :1024].contiguous()
     res["parity_spot_check"] = bool(
         np.array_equal(dev.adaptive_threshold(corner, 41, 0).cpu().numpy(),
-                       O.adaptive_threshold(corner.cpu().numpy(), 41, 0)))
+                       O.adaptive_threshold(corner.cpu().numpy(), 41, 0))
+        and np.array_equal(dev.canny(corner, 250.0, 300.0).cpu().numpy(),
+                           O.canny(corner.cpu().numpy(), 250.0, 300.0)))
     return res
 
 
@@ -560,8 +567,8 @@ def main():
                     help="blur: only the blur batch (for counter runs); copy: only a plain "
                          "copy of the same size (counter calibration); match: only the "
                          "match_template leg; proposal: only the learned-proposal search "
-                         "against the full map; consumers: integral image and "
-                         "adaptive threshold")
+                         "against the full map; consumers: integral image, "
+                         "adaptive threshold and canny")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X and has no "

@@ -336,6 +336,45 @@ IPH_API iph_status iph_dev_adaptive_threshold(const uint8_t *src, uint8_t *dst, 
                                      static_cast<hipStream_t>(stream));
 }
 
+namespace {
+iph_status check_canny(uint32_t w, uint32_t h, float low, float high) {
+    if (!(high >= low)) {  // edges.rs:31
+        set_error("assertion failed: high_threshold >= low_threshold");
+        return IPH_EINVAL;
+    }
+    if (w == 0 || h == 0) {  // edges.rs:62, 119: height - 1 / width - 1 on an empty image
+        set_error("attempt to subtract with overflow");
+        return IPH_EINVAL;
+    }
+    return IPH_OK;
+}
+}  // namespace
+
+IPH_API iph_status iph_canny(const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h, float low,
+                             float high) {
+    clear_error();
+    IPH_TRY(check_canny(w, h, low, high));
+    IPH_TRY(require_device());
+    const size_t n = (size_t)w * h, wsb = canny_workspace(w, h);
+    DevBuf din, dout, dws;
+    IPH_TRY(upload(din, src, n));
+    IPH_TRY(dout.alloc(n));
+    IPH_TRY(dws.alloc(wsb));
+    IPH_TRY(launch_canny(static_cast<const uint8_t *>(din.p), static_cast<uint8_t *>(dout.p), w, h,
+                         low, high, dws.p, wsb, nullptr));
+    return finish(dout, dst, n);
+}
+
+IPH_API size_t iph_dev_canny_workspace(uint32_t w, uint32_t h) { return canny_workspace(w, h); }
+
+IPH_API iph_status iph_dev_canny(const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h,
+                                 float low, float high, void *ws, size_t wsb, void *stream) {
+    clear_error();
+    IPH_TRY(check_canny(w, h, low, high));
+    IPH_TRY(require_device());
+    return launch_canny(src, dst, w, h, low, high, ws, wsb, static_cast<hipStream_t>(stream));
+}
+
 IPH_API iph_status iph_gradients(const uint8_t *src, uint32_t w, uint32_t h, const int32_t *hk,
                                  uint32_t hkw, uint32_t hkh, const int32_t *vk, uint32_t vkw,
                                  uint32_t vkh, uint16_t *out) {

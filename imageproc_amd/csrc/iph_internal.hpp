@@ -109,6 +109,11 @@ size_t adaptive_threshold_workspace(uint32_t w, uint32_t h);
 iph_status launch_adaptive_threshold(const uint8_t *src, uint32_t w, uint32_t h,
                                      uint32_t block_radius, int32_t delta, uint8_t *dst, void *ws,
                                      size_t wsb, hipStream_t s);
+// canny (edges.rs:30-155), implemented in edges.hip; synchronises the stream
+// once per few hysteresis launches to test for convergence.
+size_t canny_workspace(uint32_t w, uint32_t h);
+iph_status launch_canny(const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h, float low,
+                        float high, void *ws, size_t wsb, hipStream_t s);
 // gradients (gradients.rs:99-160) of a GrayImage, u16 magnitudes.
 iph_status launch_gradients(const uint8_t *src, uint32_t w, uint32_t h, const int32_t *hk,
                             uint32_t hkw, uint32_t hkh, const int32_t *vk, uint32_t vkw,

@@ -163,6 +163,22 @@ def adaptive_threshold(image: torch.Tensor, block_radius: int, delta: int,
     return out
 
 
+def canny(image: torch.Tensor, low_threshold: float, high_threshold: float,
+          out: torch.Tensor | None = None) -> torch.Tensor:
+    """canny (edges.rs:30-155) of a device uint8 tensor. Blocks until the
+    hysteresis has converged."""
+    if _code(image) != _TORCH_CODE[torch.uint8]:
+        raise TypeError("canny takes GrayImages (uint8)")
+    h, w = image.shape
+    if out is None:
+        out = torch.empty_like(image)
+    wsb = lib().iph_dev_canny_workspace(w, h)
+    ws = _workspace("canny", wsb, image.device)
+    check(lib().iph_dev_canny(image.data_ptr(), out.data_ptr(), w, h, float(low_threshold),
+                              float(high_threshold), ws.data_ptr(), ws.numel(), _stream()))
+    return out
+
+
 def find_extremes(image: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
     """Writes the 24-byte iph_extremes record into `out` (a device uint8 tensor)."""
     p = _code(image)

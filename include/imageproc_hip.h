@@ -125,6 +125,14 @@ IPH_API iph_status iph_integral_image(const uint8_t *src, uint32_t width, uint32
  * (where the reference's i32 cast of it goes negative) -> IPH_EINVAL. */
 IPH_API iph_status iph_adaptive_threshold(const uint8_t *src, uint8_t *dst, uint32_t width,
                                           uint32_t height, uint32_t block_radius, int32_t delta);
+/* canny, edges.rs:30-155: 255 on edge pixels, 0 elsewhere. high < low (or a
+ * NaN threshold) -> IPH_EINVAL "assertion failed: high_threshold >=
+ * low_threshold"; an empty image -> IPH_EINVAL (the reference's height - 1
+ * overflows); an edge that reaches the image border (possible only for
+ * low_threshold <= 0) -> IPH_EINVAL, where the reference's search indexes
+ * outside the image and panics. */
+IPH_API iph_status iph_canny(const uint8_t *src, uint8_t *dst, uint32_t width, uint32_t height,
+                             float low_threshold, float high_threshold);
 /* gradients_grayscale / gradients, gradients.rs:15-21, 99-160, for a
  * GrayImage: filter_clamped::<_, i32, i16> with each kernel (row-major i32,
  * at most 49 taps each), then (dx^2 + dy^2).sqrt() as u16 per pixel
@@ -223,6 +231,12 @@ IPH_API iph_status iph_dev_adaptive_threshold(const uint8_t *src, uint8_t *dst, 
                                               uint32_t height, uint32_t block_radius,
                                               int32_t delta, void *workspace,
                                               size_t workspace_bytes, void *stream);
+/* iph_canny on device buffers. Blocks the calling thread: the hysteresis
+ * reads a convergence flag back every few launches on `stream`. */
+IPH_API size_t iph_dev_canny_workspace(uint32_t width, uint32_t height);
+IPH_API iph_status iph_dev_canny(const uint8_t *src, uint8_t *dst, uint32_t width,
+                                 uint32_t height, float low_threshold, float high_threshold,
+                                 void *workspace, size_t workspace_bytes, void *stream);
 IPH_API size_t iph_dev_match_template_with_mask_workspace(void);
 IPH_API iph_status iph_dev_match_template_with_mask(const uint8_t *image, uint32_t width,
                                                     uint32_t height, const uint8_t *tmpl,

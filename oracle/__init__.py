@@ -203,3 +203,50 @@ def adaptive_threshold(img, block_radius, delta):
         raise ValueError("assertion failed: block_radius > 0")
     assert rc == 0
     return out
+
+
+def canny_direction(gx: int, gy: int) -> int:
+    return lib().or_canny_direction(int(gx), int(gy))
+
+
+def canny_direction_grid(r: int):
+    """(classes, magnitudes) of every (gx, gy) in [-r, r]^2, indexed [gy + r, gx + r]."""
+    n = 2 * r + 1
+    cls = np.zeros((n, n), np.uint8)
+    mag = np.zeros((n, n), np.float32)
+    lib().or_canny_direction_grid(int(r), _p(cls), _p(mag))
+    return cls, mag
+
+
+def canny(img, low, high):
+    img = _img(img)
+    h, w = img.shape
+    out = np.zeros((h, w), np.uint8)
+    rc = lib().or_canny(_p(img), w, h, C.c_float(low), C.c_float(high), _p(out))
+    if rc == -1:
+        raise ValueError("assertion failed: high_threshold >= low_threshold")
+    if rc == -4:
+        raise IndexError("hysteresis reached the image border")
+    assert rc == 0, rc
+    return out
+
+
+def canny_thinned(img):
+    img = _img(img)
+    h, w = img.shape
+    thin = np.zeros((h, w), np.float32)
+    assert lib().or_canny_thinned(_p(img), w, h, _p(thin)) == 0
+    return thin
+
+
+def canny_hysteresis(thin, low, high):
+    thin = np.ascontiguousarray(thin, np.float32)
+    h, w = thin.shape
+    out = np.zeros((h, w), np.uint8)
+    stack = np.zeros(2 * h * w, np.uint32)
+    rc = lib().or_canny_hysteresis(_p(thin), w, h, C.c_float(low), C.c_float(high), _p(stack),
+                                   _p(out))
+    if rc == -4:
+        raise IndexError("hysteresis reached the image border")
+    assert rc == 0, rc
+    return out

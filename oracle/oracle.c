@@ -695,3 +695,128 @@ int or_adaptive_threshold(const uint8_t *img, uint32_t w, uint32_t h, uint32_t b
     free(ii);
     return 0;
 }
+
+/* ---- canny, edges.rs:30-155 ------------------------------------------------------------ */
+
+/* The direction class of non_maximum_suppression, edges.rs:62-80: atan2f in
+ * degrees (f32 steps), negative angles moved up by 180, then the 22.5-degree
+ * bins. Returns 0, 45, 90 or 135. */
+int or_canny_direction(int gx, int gy) {
+    const float rad_to_deg = 180.0f / 3.14159265358979323846f;
+    float angle = atan2f((float)gy, (float)gx) * rad_to_deg;
+    if (angle < 0.0f) angle += 180.0f;
+    if (!(angle >= 22.5f && angle < 157.5f)) return 0;
+    if (angle >= 22.5f && angle < 67.5f) return 45;
+    if (angle >= 67.5f && angle < 112.5f) return 90;
+    return 135;
+}
+
+/* or_canny_direction and hypotf over every (gx, gy) in [-r, r]^2, row-major
+ * by gy then gx: the whole input domain of the Sobel responses of u8 images
+ * at r = 1020. */
+void or_canny_direction_grid(int r, uint8_t *cls, float *mag) {
+    const int n = 2 * r + 1;
+    for (int gy = -r; gy <= r; gy++)
+        for (int gx = -r; gx <= r; gx++) {
+            const size_t i = (size_t)(gy + r) * n + (gx + r);
+            cls[i] = (uint8_t)or_canny_direction(gx, gy);
+            mag[i] = hypotf((float)gx, (float)gy);
+        }
+}
+
+/* canny(image, low, high), edges.rs:30-52: gaussian_blur_f32(1.4), Sobel by
+ * filter_clamped::<_, i32, i16>, magnitude (gx as f32).hypot(gy as f32),
+ * non_maximum_suppression (edges.rs:55-108) and hysteresis (edges.rs:110-155,
+ * the same depth-first search over the same six neighbours). Returns -1 when
+ * the reference panics on its assertion, -4 when its search would index
+ * outside the image (a border pixel reached), -2 on allocation failure. */
+int or_canny_thinned(const uint8_t *img, uint32_t w, uint32_t h, float *thin_out);
+int or_canny_hysteresis(const float *thin, uint32_t w, uint32_t h, float low, float high,
+                        uint32_t *stack, uint8_t *out);
+
+int or_canny(const uint8_t *img, uint32_t w, uint32_t h, float low, float high, uint8_t *out) {
+    if (!(high >= low)) return -1;
+    if (w == 0 || h == 0) return -3;
+    const size_t n = (size_t)w * h;
+    float *thin = calloc(n, sizeof(float));
+    uint32_t *stack = malloc(n * sizeof(uint32_t) * 2);
+    int rc = thin && stack ? or_canny_thinned(img, w, h, thin) : -2;
+    if (rc == 0) rc = or_canny_hysteresis(thin, w, h, low, high, stack, out);
+    free(thin);
+    free(stack);
+    return rc;
+}
+
+/* Stages 1-3 of canny (edges.rs:34-50): the thinned magnitudes, zero on the
+ * border. */
+int or_canny_thinned(const uint8_t *img, uint32_t w, uint32_t h, float *thin) {
+    if (w == 0 || h == 0) return -3;
+    const size_t n = (size_t)w * h;
+    static const int32_t sob_h[9] = {-1, 0, 1, -2, 0, 2, -1, 0, 1};
+    static const int32_t sob_v[9] = {-1, -2, -1, 0, 0, 0, 1, 2, 1};
+    uint8_t *blur = malloc(n);
+    int16_t *gx = malloc(n * sizeof(int16_t)), *gy = malloc(n * sizeof(int16_t));
+    float *g = malloc(n * sizeof(float));
+    int rc = 0;
+    if (!blur || !gx || !gy || !g) {
+        rc = -2;
+        goto done;
+    }
+    memset(thin, 0, n * sizeof(float));
+    or_gaussian_blur_f32(img, blur, w, h, 0, 1.4f);
+    or_filter_clamped(blur, 0, w, h, sob_h, 3, 3, 3, gx, 2);
+    or_filter_clamped(blur, 0, w, h, sob_v, 3, 3, 3, gy, 2);
+    for (size_t i = 0; i < n; i++) g[i] = hypotf((float)gx[i], (float)gy[i]);
+    for (uint32_t y = 1; y + 1 < h; y++)
+        for (uint32_t x = 1; x + 1 < w; x++) {
+            const size_t i = (size_t)y * w + x;
+            float c1, c2;
+            switch (or_canny_direction(gx[i], gy[i])) {
+            case 0: c1 = g[i - 1]; c2 = g[i + 1]; break;
+            case 45: c1 = g[i + w + 1]; c2 = g[i - w - 1]; break;
+            case 90: c1 = g[i - w]; c2 = g[i + w]; break;
+            default: c1 = g[i + w - 1]; c2 = g[i - w + 1]; break;
+            }
+            thin[i] = (g[i] < c1 || g[i] < c2) ? 0.0f : g[i];
+        }
+done:
+    free(blur);
+    free(gx);
+    free(gy);
+    free(g);
+    return rc;
+}
+
+/* hysteresis, edges.rs:110-155: the reference's depth-first search. stack
+ * holds 2 * w * h entries. */
+int or_canny_hysteresis(const float *thin, uint32_t w, uint32_t h, float low, float high,
+                        uint32_t *stack, uint8_t *out) {
+    int rc = 0;
+    memset(out, 0, (size_t)w * h);
+    static const int dx[6] = {1, 1, 0, -1, -1, -1}, dy[6] = {0, 1, 1, -1, 0, 1};
+    for (uint32_t y = 1; y + 1 < h && rc == 0; y++)
+        for (uint32_t x = 1; x + 1 < w && rc == 0; x++) {
+            const size_t i = (size_t)y * w + x;
+            if (!(thin[i] >= high) || out[i] != 0) continue;
+            out[i] = 255;
+            size_t top = 0;
+            stack[top++] = x;
+            stack[top++] = y;
+            while (top && rc == 0) {
+                const uint32_t ny = stack[--top], nx = stack[--top];
+                if (nx == 0 || ny == 0 || nx + 1 >= w || ny + 1 >= h) {
+                    rc = -4; /* a neighbour of this pixel lies outside the image */
+                    break;
+                }
+                for (int k = 0; k < 6; k++) {
+                    const size_t j = (size_t)(ny + dy[k]) * w + (nx + dx[k]);
+                    if (thin[j] >= low && out[j] == 0) {
+                        out[j] = 255;
+                        stack[top++] = nx + dx[k];
+                        stack[top++] = ny + dy[k];
+                    }
+                }
+            }
+        }
+    return rc;
+}

@@ -111,6 +111,19 @@ void or_integral_image_u32(const uint8_t *img, uint32_t w, uint32_t h, uint32_t 
 int or_adaptive_threshold(const uint8_t *img, uint32_t w, uint32_t h, uint32_t block_radius,
                           int32_t delta, uint8_t *out);
 
+/* Direction class (0, 45, 90, 135) of non_maximum_suppression, edges.rs:62-80. */
+int or_canny_direction(int gx, int gy);
+/* or_canny_direction and hypotf over [-r, r]^2 (row-major by gy, then gx). */
+void or_canny_direction_grid(int r, uint8_t *cls, float *mag);
+/* canny, edges.rs:30-155. Returns -1 for high < low (or NaN), -3 for an empty
+ * image, -4 where the reference's search indexes outside the image. */
+int or_canny(const uint8_t *img, uint32_t w, uint32_t h, float low, float high, uint8_t *out);
+/* canny's stages: thinned magnitudes (edges.rs:34-50), and hysteresis
+ * (edges.rs:110-155) over them with a stack of 2 * w * h entries. */
+int or_canny_thinned(const uint8_t *img, uint32_t w, uint32_t h, float *thin);
+int or_canny_hysteresis(const float *thin, uint32_t w, uint32_t h, float low, float high,
+                        uint32_t *stack, uint8_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -39,6 +39,7 @@ def main(ref_root):
         "robin_sharpen_gaussian": "truth/robin_sharpen_gaussian.png",
         "elephant_gradients": "truth/elephant_gradients.png",
         "zebra_adaptive_threshold": "truth/zebra_adaptive_threshold.png",
+        "zebra_canny": "truth/zebra_canny.png",
     }
     arrays, shas = {}, {}
     for key, rel in items.items():

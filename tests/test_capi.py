@@ -163,3 +163,16 @@ def test_adaptive_threshold_argument_errors_before_device():
         adaptive_threshold(img.astype(np.uint16), 1, 0)
     assert _lib.lib().iph_dev_integral_image_workspace(100, 70) >= 4 * (2 * 100 + 2 * 70)
     assert _lib.lib().iph_dev_adaptive_threshold_workspace(100, 70) >= 4 * 101 * 71
+
+
+def test_canny_argument_errors_before_device():
+    """edges.rs:31 asserts high_threshold >= low_threshold first."""
+    from imageproc_amd._lib import Panic
+    from imageproc_amd.edges import canny
+
+    img = np.zeros((4, 4), np.uint8)
+    with pytest.raises(Panic, match="assertion failed: high_threshold >= low_threshold"):
+        canny(img, 5.0, 1.0)
+    with pytest.raises(Panic, match="overflow"):
+        canny(np.zeros((3, 0), np.uint8), 1.0, 5.0)
+    assert _lib.lib().iph_dev_canny_workspace(100, 70) >= 2 * 100 * 70

@@ -1,0 +1,77 @@
+"""canny (edges.rs:30-155) on the GPU, bit for bit: the reference's zebra
+truth image (tests/regression.rs:555-560) and the oracle on noise, smooth
+fields and long edges that cross many tiles, plus the reference's panics."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from imageproc_amd import device, edges
+from imageproc_amd._lib import Panic
+
+pytestmark = pytest.mark.gpu
+
+
+def _smooth(seed, shape, sigma):
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    f = gaussian_filter(rng.normal(0, 1, shape), sigma)
+    f = (f - f.min()) / (np.ptp(f) + 1e-12)
+    return (f * 255).astype(np.uint8)
+
+
+def _stripes(shape, period):
+    y, x = np.mgrid[0:shape[0], 0:shape[1]]
+    v = 127.5 + 127.5 * np.sin(2 * np.pi * (x * 0.8 + y * 0.6 + 0.002 * x * y) / period)
+    return v.astype(np.uint8)
+
+
+def test_zebra_truth(fixtures):
+    assert np.array_equal(edges.canny(fixtures["zebra"], 250.0, 300.0), fixtures["zebra_canny"])
+
+
+@pytest.mark.parametrize("shape,low,high", [((1, 1), 10.0, 20.0), ((2, 5), 10.0, 20.0),
+                                            ((3, 3), 1.0, 2.0), ((5, 200), 50.0, 100.0),
+                                            ((130, 257), 50.0, 100.0), ((64, 64), 250.0, 300.0),
+                                            ((200, 63), 0.5, 0.5), ((97, 131), 10.0, 1000.0)])
+def test_noise_vs_oracle(shape, low, high):
+    print("canny noise", shape, low, high, flush=True)
+    img = np.random.default_rng(shape[0] * 13 + shape[1]).integers(0, 256, shape, dtype=np.uint8)
+    assert np.array_equal(edges.canny(img, low, high), O.canny(img, low, high))
+
+
+@pytest.mark.parametrize("seed,shape,sigma,low,high", [(1, (300, 411), 3.0, 5.0, 20.0),
+                                                       (2, (517, 389), 6.0, 2.0, 6.0),
+                                                       (3, (1024, 1536), 12.0, 1.0, 3.0)])
+def test_smooth_vs_oracle(seed, shape, sigma, low, high):
+    print("canny smooth", seed, shape, flush=True)
+    img = _smooth(seed, shape, sigma)
+    want = O.canny(img, low, high)
+    assert want.any()
+    assert np.array_equal(edges.canny(img, low, high), want)
+
+
+def test_long_edges_cross_many_tiles():
+    img = _stripes((1100, 1700), 90.0)
+    for low, high in [(5.0, 40.0), (30.0, 200.0)]:
+        want = O.canny(img, low, high)
+        assert want.sum() > 1000 * 255
+        got = device.canny(torch.from_numpy(img).cuda(), low, high).cpu().numpy()
+        assert np.array_equal(got, want)
+
+
+def test_reference_panics():
+    img = np.zeros((9, 9), np.uint8)
+    img[3:6, 3:6] = 255
+    with pytest.raises(Panic, match="high_threshold >= low_threshold"):
+        edges.canny(img, 2.0, 1.0)
+    with pytest.raises(Panic, match="high_threshold >= low_threshold"):
+        edges.canny(img, float("nan"), 1.0)
+    with pytest.raises(Panic, match="overflow"):
+        edges.canny(np.zeros((0, 4), np.uint8), 1.0, 2.0)
+    # low <= 0 lets an edge reach the border: the reference indexes outside the image
+    with pytest.raises(Panic, match="border"):
+        edges.canny(img, 0.0, 10.0)
+    # ... but with no seed nothing is marked and nothing panics
+    assert not edges.canny(img, -1.0, 5000.0).any()
+    assert not O.canny(img, -1.0, 5000.0).any()

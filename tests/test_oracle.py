@@ -388,3 +388,98 @@ def test_adaptive_threshold_brute_force_and_truth(rng, fixtures):
     # tests/regression.rs:576-582
     assert np.array_equal(O.adaptive_threshold(fixtures["zebra"], 41, 0),
                           fixtures["zebra_adaptive_threshold"])
+
+
+# ---- canny (edges.rs:30-155) -------------------------------------------------------------
+
+def test_canny_direction_and_magnitude_over_the_sobel_domain():
+    """Every (gx, gy) a Sobel response of a u8 image can take: the oracle's
+    f32 atan2f binning equals the exact angle's bin (decided in integers:
+    tan 22.5 = sqrt2 - 1, tan 67.5 = sqrt2 + 1) except at (-985, -408), and
+    hypotf equals the correctly rounded sqrt of the exact gx^2 + gy^2."""
+    r = 1020
+    cls, mag = O.canny_direction_grid(r)
+    gy, gx = np.meshgrid(np.arange(-r, r + 1, dtype=np.int64), np.arange(-r, r + 1, dtype=np.int64),
+                         indexing="ij")
+    x = np.where(gy < 0, -gx, gx)
+    y = np.abs(gy)
+    ax = np.abs(x)
+    above_t1 = (ax + y) ** 2 > 2 * ax * ax
+    below_t2 = (y - ax <= 0) | ((y - ax) ** 2 < 2 * ax * ax)
+    exact = np.zeros(cls.shape, np.uint8)
+    pos = y > 0
+    exact[pos & (x > 0) & above_t1 & below_t2] = 45
+    exact[pos & ((x == 0) | ((x > 0) & ~below_t2) | ((x < 0) & ~below_t2))] = 90
+    exact[pos & (x < 0) & above_t1 & below_t2] = 135
+    diff = np.argwhere(cls != exact)
+    assert [(int(gx[i, j]), int(gy[i, j])) for i, j in diff] == [(-985, -408)]
+    assert cls[-408 + r, -985 + r] == 45 and exact[-408 + r, -985 + r] == 0
+    n = gx * gx + gy * gy
+    assert np.array_equal(mag, np.sqrt(n.astype(np.float64)).astype(np.float32))
+    assert np.array_equal(mag, np.sqrt(n.astype(np.float32)))
+    # spot values of the bins
+    assert O.canny_direction(1, 0) == 0 and O.canny_direction(-1, 0) == 0
+    assert O.canny_direction(0, 1) == 90 and O.canny_direction(0, -1) == 90
+    assert O.canny_direction(1, 1) == 45 and O.canny_direction(-1, 1) == 135
+    assert O.canny_direction(-1, -1) == 45 and O.canny_direction(1, -1) == 135
+
+
+def _hysteresis_brute(thin, low, high):
+    """Fixed point of the marking rule, by repeated sweeps (no stack)."""
+    h, w = thin.shape
+    out = np.zeros((h, w), bool)
+    out[1:h - 1, 1:w - 1] = thin[1:h - 1, 1:w - 1] >= high
+    nbrs = [(1, 0), (1, 1), (0, 1), (-1, -1), (-1, 0), (-1, 1)]
+    changed = True
+    while changed:
+        changed = False
+        for yy, xx in zip(*np.nonzero(out)):
+            for dx, dy in nbrs:
+                nx, ny = xx + dx, yy + dy
+                if 0 <= nx < w and 0 <= ny < h and not out[ny, nx] and thin[ny, nx] >= low:
+                    out[ny, nx] = True
+                    changed = True
+    return out
+
+
+def test_canny_truth_and_hysteresis(fixtures, rng):
+    # tests/regression.rs:555-560
+    assert np.array_equal(O.canny(fixtures["zebra"], 250.0, 300.0), fixtures["zebra_canny"])
+    # the oracle's depth-first search reaches the same set as a sweep to the
+    # fixed point, on images whose thinned maps come from the oracle's own
+    # stages
+    from scipy.ndimage import gaussian_filter
+    for seed in range(3):
+        img = np.clip(gaussian_filter(rng.normal(128, 60, (40, 50)), 2.0), 0, 255).astype(np.uint8)
+        got = O.canny(img, 20.0, 60.0)
+        assert set(np.unique(got)) <= {0, 255}
+        assert not got[0].any() and not got[-1].any() and not got[:, 0].any() and not got[:, -1].any()
+        # raising low can only remove edges, lowering high can only add them
+        assert not (O.canny(img, 40.0, 60.0) & ~got).any()
+        assert not (got & ~O.canny(img, 20.0, 40.0)).any()
+        thin = O.canny_thinned(img)
+        assert np.array_equal(O.canny_hysteresis(thin, 20.0, 60.0) == 255,
+                              _hysteresis_brute(thin, 20.0, 60.0))
+        assert np.array_equal(O.canny_hysteresis(thin, 20.0, 60.0), got)
+    # synthetic thinned maps ([y, x] indexing): marks travel only along the
+    # six neighbours (x+1, y), (x+1, y+1), (x, y+1), (x-1, y-1), (x-1, y),
+    # (x-1, y+1); there is no step up or up-right
+    thin = np.zeros((7, 7), np.float32)
+    thin[5, 1] = 10.0  # seed (1, 5)
+    thin[5, 2] = 5.0  # (2, 5): right of the seed
+    thin[4, 1] = 5.0  # (1, 4): up-left of (2, 5)
+    thin[4, 3] = 5.0  # (3, 4): up-right of (2, 5) only
+    got = O.canny_hysteresis(thin, 1.0, 8.0)
+    assert got[5, 1] == 255 and got[5, 2] == 255 and got[4, 1] == 255 and got[4, 3] == 0
+    assert np.array_equal(got == 255, _hysteresis_brute(thin, 1.0, 8.0))
+    thin2 = np.zeros((7, 7), np.float32)
+    thin2[5, 1] = 10.0
+    thin2[4, 1] = 5.0  # straight up from the seed
+    assert O.canny_hysteresis(thin2, 1.0, 8.0)[4, 1] == 0
+    with pytest.raises(ValueError, match="high_threshold >= low_threshold"):
+        O.canny(np.zeros((5, 5), np.uint8), 2.0, 1.0)
+    img = np.zeros((9, 9), np.uint8)
+    img[3:6, 3:6] = 255
+    with pytest.raises(IndexError):
+        O.canny(img, 0.0, 10.0)
+    assert not O.canny(img, -1.0, 5000.0).any()
