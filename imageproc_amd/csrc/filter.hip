@@ -760,51 +760,82 @@ constexpr int kIntTile = 64;
 
 size_t int_tiles(uint32_t n) { return ((size_t)n + kIntTile - 1) / kIntTile; }
 
-template <bool SQ>
-__device__ inline uint32_t int_px(const uint8_t *__restrict__ src, int W, int H, int x, int y) {
-    const uint32_t p = (x < W && y < H) ? (uint32_t)src[(size_t)y * W + x] : 0u;
-    return SQ ? p * p : p;
+// Tile of work item b: tiles are dealt to the 8 XCDs (workgroup b runs on XCD
+// b % 8 when the grid is a multiple of 8) in contiguous raster-order bands,
+// so horizontally adjacent tiles, whose output rows share cache lines, meet
+// in one L2.
+__device__ inline size_t int_tile_of(size_t b, size_t ntiles) {
+    const size_t per = (ntiles + 7) / 8;
+    return (b % 8) * per + b / 8;
 }
 
-template <bool SQ>
+// Thread (r, q) of a tile: the 16 pixels of row y at columns X0 + 16 q ..
+// X0 + 16 q + 15 (0 outside the image), squared for integral_squared_image.
+// VEC: W % 16 == 0 and a 16-byte aligned image, so the 16 bytes are one
+// aligned load, all inside or all outside the image.
+template <bool SQ, bool VEC>
+__device__ __forceinline__ void int_load16(const uint8_t *__restrict__ src, int W, int H, int X0,
+                                           int y, int q, uint32_t v[16]) {
+    const int x0 = X0 + q * 16;
+    if (VEC) {
+        uint4 d = make_uint4(0u, 0u, 0u, 0u);
+        if (y < H && x0 < W) d = *reinterpret_cast<const uint4 *>(src + (size_t)y * W + x0);
+        const uint32_t w4[4] = {d.x, d.y, d.z, d.w};
+#pragma unroll
+        for (int j = 0; j < 16; j++) v[j] = (w4[j >> 2] >> (8 * (j & 3))) & 0xffu;
+    } else {
+#pragma unroll
+        for (int j = 0; j < 16; j++)
+            v[j] = (y < H && x0 + j < W) ? (uint32_t)src[(size_t)y * W + x0 + j] : 0u;
+    }
+    if (SQ) {
+#pragma unroll
+        for (int j = 0; j < 16; j++) v[j] *= v[j];
+    }
+}
+
+template <bool SQ, bool VEC>
 __global__ __launch_bounds__(256) void integral_tile_sums_kernel(
     const uint8_t *__restrict__ src, int W, int H, int tiles_x, size_t ntiles,
     uint32_t *__restrict__ colsum, uint32_t *__restrict__ rowsum) {
-    __shared__ uint32_t tile[kIntTile][kIntTile + 1];
+    __shared__ uint4 tile4[kIntTile][kIntTile / 4 + 1];
     __shared__ uint32_t cpart[4][kIntTile];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
-    for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const size_t nb = 8 * ((ntiles + 7) / 8);
+    for (size_t b = blockIdx.x; b < nb; b += gridDim.x) {
+        const size_t t = int_tile_of(b, ntiles);
+        if (t >= ntiles) continue;  // the same for the whole workgroup
         const int tx = (int)(t % (size_t)tiles_x), ty = (int)(t / (size_t)tiles_x);
-        const int x = tx * kIntTile + lane, y0 = ty * kIntTile + wave * 16;
+        const int X0 = tx * kIntTile, Y0 = ty * kIntTile;
         uint32_t v[16];
-#pragma unroll
-        for (int i = 0; i < 16; i++) v[i] = int_px<SQ>(src, W, H, x, y0 + i);
-        uint32_t cs = 0;
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            cs += v[i];
-            tile[wave * 16 + i][lane] = v[i];
-        }
-        cpart[wave][lane] = cs;
-        __syncthreads();
+        int_load16<SQ, VEC>(src, W, H, X0, Y0 + r, q, v);
         uint32_t rs = 0;
 #pragma unroll
-        for (int j = 0; j < 16; j++) rs += tile[r][q * 16 + j];
+        for (int j = 0; j < 16; j++) rs += v[j];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            tile4[r][q * 4 + k] = make_uint4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
         rs += __shfl_xor(rs, 1);
         rs += __shfl_xor(rs, 2);
-        const int yr = ty * kIntTile + r;
-        if (q == 0 && yr < H) rowsum[(size_t)tx * H + yr] = rs;
-        if (wave == 0 && x < W)
-            colsum[(size_t)ty * W + x] = cpart[0][lane] + cpart[1][lane] + cpart[2][lane] +
-                                         cpart[3][lane];
+        if (q == 0 && Y0 + r < H) rowsum[(size_t)tx * H + Y0 + r] = rs;
+        __syncthreads();
+        const uint32_t *col = reinterpret_cast<const uint32_t *>(&tile4[0][0]);
+        uint32_t cs = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) cs += col[(wave * 16 + i) * (kIntTile + 4) + lane];
+        cpart[wave][lane] = cs;
+        __syncthreads();
+        if (wave == 0 && X0 + lane < W)
+            colsum[(size_t)ty * W + X0 + lane] =
+                cpart[0][lane] + cpart[1][lane] + cpart[2][lane] + cpart[3][lane];
         __syncthreads();
     }
 }
 
 // Threads [0, W) scan colsum[.][x] down the tile rows, threads [W, W + H)
 // scan rowsum[.][y] across the tile columns; both exclusive and in place,
-// loads issued sixteen at a time.
+// loads issued 32 at a time.
 __global__ __launch_bounds__(256) void integral_scan_aux_kernel(uint32_t *__restrict__ colsum,
                                                                 int W, int tiles_y,
                                                                 uint32_t *__restrict__ rowsum,
@@ -825,12 +856,12 @@ __global__ __launch_bounds__(256) void integral_scan_aux_kernel(uint32_t *__rest
         return;
     }
     uint32_t run = 0;
-    for (int k0 = 0; k0 < n; k0 += 16) {
-        uint32_t v[16];
+    for (int k0 = 0; k0 < n; k0 += 32) {
+        uint32_t v[32];
 #pragma unroll
-        for (int u = 0; u < 16; u++) v[u] = k0 + u < n ? p[(size_t)(k0 + u) * stride] : 0u;
+        for (int u = 0; u < 32; u++) v[u] = k0 + u < n ? p[(size_t)(k0 + u) * stride] : 0u;
 #pragma unroll
-        for (int u = 0; u < 16; u++)
+        for (int u = 0; u < 32; u++)
             if (k0 + u < n) {
                 p[(size_t)(k0 + u) * stride] = run;
                 run += v[u];
@@ -839,7 +870,7 @@ __global__ __launch_bounds__(256) void integral_scan_aux_kernel(uint32_t *__rest
 }
 
 // One workgroup per tile row: inclusive scan of colsum[ty][0..W) in place,
-// 256 columns per pass.
+// 4096 columns per pass (16 consecutive per thread, all loads in flight).
 __global__ __launch_bounds__(256) void integral_scan_pin_kernel(uint32_t *__restrict__ colsum,
                                                                 int W, int tiles_y) {
     __shared__ uint32_t part[4];
@@ -847,10 +878,14 @@ __global__ __launch_bounds__(256) void integral_scan_pin_kernel(uint32_t *__rest
     for (int ty = blockIdx.x; ty < tiles_y; ty += gridDim.x) {
         uint32_t *row = colsum + (size_t)ty * W;
         uint32_t carry = 0;
-        for (int x0 = 0; x0 < W; x0 += 256) {
-            const int x = x0 + tid;
-            const uint32_t v = x < W ? row[x] : 0u;
-            uint32_t inc = v;
+        for (int x0 = 0; x0 < W; x0 += 256 * 16) {
+            const int xb = x0 + tid * 16;
+            uint32_t v[16];
+#pragma unroll
+            for (int j = 0; j < 16; j++) v[j] = xb + j < W ? row[xb + j] : 0u;
+#pragma unroll
+            for (int j = 1; j < 16; j++) v[j] += v[j - 1];
+            uint32_t inc = v[15];
 #pragma unroll
             for (int o = 1; o < 64; o <<= 1) {
                 const uint32_t t = __shfl_up(inc, o);
@@ -858,72 +893,108 @@ __global__ __launch_bounds__(256) void integral_scan_pin_kernel(uint32_t *__rest
             }
             if (lane == 63) part[wave] = inc;
             __syncthreads();
-            uint32_t before = carry;
+            uint32_t before = carry + inc - v[15];
             for (int w2 = 0; w2 < wave; w2++) before += part[w2];
-            if (x < W) row[x] = before + inc;
+#pragma unroll
+            for (int j = 0; j < 16; j++)
+                if (xb + j < W) row[xb + j] = before + v[j];
             carry += part[0] + part[1] + part[2] + part[3];
             __syncthreads();
         }
     }
 }
 
-template <bool SQ>
+// V4OUT: `out` is 16-byte aligned, so every output row is written as aligned
+// 4-element groups from LDS plus at most three elements at each end.
+template <bool SQ, bool VEC, bool V4OUT>
 __global__ __launch_bounds__(256) void integral_emit_kernel(
     const uint8_t *__restrict__ src, int W, int H, int tiles_x, size_t ntiles,
     const uint32_t *__restrict__ pin, const uint32_t *__restrict__ rleft,
     uint32_t *__restrict__ out) {
-    __shared__ uint32_t tile[kIntTile][kIntTile + 1];
+    __shared__ uint4 tile4[kIntTile][kIntTile / 4 + 1];
     __shared__ uint32_t vpart[4][kIntTile];
+    uint32_t *tile = reinterpret_cast<uint32_t *>(&tile4[0][0]);
+    constexpr int P = kIntTile + 4;  // row pitch of `tile` in elements
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
     const size_t W1 = (size_t)W + 1;
-    for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const size_t nb = 8 * ((ntiles + 7) / 8);
+    for (size_t b = blockIdx.x; b < nb; b += gridDim.x) {
+        const size_t t = int_tile_of(b, ntiles);
+        if (t >= ntiles) continue;  // the same for the whole workgroup
         const int tx = (int)(t % (size_t)tiles_x), ty = (int)(t / (size_t)tiles_x);
-        const int x = tx * kIntTile + lane, y0 = ty * kIntTile + wave * 16;
+        const int X0 = tx * kIntTile, Y0 = ty * kIntTile;
+        const int cw = min(kIntTile, W - X0), chh = min(kIntTile, H - Y0);
+        // row prefix of the thread's 16 pixels, the quarter-row offsets and
+        // the part of the row left of the tile
+        uint32_t v[16];
+        int_load16<SQ, VEC>(src, W, H, X0, Y0 + r, q, v);
 #pragma unroll
-        for (int i = 0; i < 16; i++) tile[wave * 16 + i][lane] = int_px<SQ>(src, W, H, x, y0 + i);
-        __syncthreads();
-        // row prefix inside the tile plus the part of the row left of it
-        uint32_t h[16], pre = 0;
-#pragma unroll
-        for (int j = 0; j < 16; j++) {
-            pre += tile[r][q * 16 + j];
-            h[j] = pre;
-        }
-        uint32_t a = pre;
+        for (int j = 1; j < 16; j++) v[j] += v[j - 1];
+        uint32_t a = v[15];
         uint32_t u = __shfl_up(a, 1);
         if (q >= 1) a += u;
         u = __shfl_up(a, 2);
         if (q >= 2) a += u;
-        const int yr = ty * kIntTile + r;
-        const uint32_t off = (a - pre) + (yr < H ? rleft[(size_t)tx * H + yr] : 0u);
+        const uint32_t off = (a - v[15]) + (r < chh ? rleft[(size_t)tx * H + Y0 + r] : 0u);
 #pragma unroll
-        for (int j = 0; j < 16; j++) tile[r][q * 16 + j] = h[j] + off;
+        for (int k = 0; k < 4; k++)
+            tile4[r][q * 4 + k] = make_uint4(v[4 * k] + off, v[4 * k + 1] + off,
+                                             v[4 * k + 2] + off, v[4 * k + 3] + off);
         __syncthreads();
         // column prefix down the tile, plus everything above the tile row
-        uint32_t v[16], run = 0;
+        uint32_t c[16], run = 0;
 #pragma unroll
         for (int i = 0; i < 16; i++) {
-            run += tile[wave * 16 + i][lane];
-            v[i] = run;
+            run += tile[(wave * 16 + i) * P + lane];
+            c[i] = run;
         }
         vpart[wave][lane] = run;
         __syncthreads();
-        uint32_t base = x < W ? pin[(size_t)ty * W + x] : 0u;
+        uint32_t base = lane < cw ? pin[(size_t)ty * W + X0 + lane] : 0u;
         for (int w2 = 0; w2 < wave; w2++) base += vpart[w2][lane];
-        if (x < W) {
+        if (!V4OUT) {
+            if (lane < cw) {
 #pragma unroll
-            for (int i = 0; i < 16; i++)
-                if (y0 + i < H) out[(size_t)(y0 + i + 1) * W1 + x + 1] = v[i] + base;
+                for (int i = 0; i < 16; i++)
+                    if (wave * 16 + i < chh)
+                        out[(size_t)(Y0 + wave * 16 + i + 1) * W1 + X0 + lane + 1] = c[i] + base;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 16; i++) tile[(wave * 16 + i) * P + lane] = c[i] + base;
+            __syncthreads();
+            // aligned groups of 4 fully inside the row segment
+            for (int slot = threadIdx.x; slot < kIntTile * 16; slot += 256) {
+                const int rr = slot >> 4, g = slot & 15;
+                if (rr >= chh) continue;
+                const size_t e0 = (size_t)(Y0 + rr + 1) * W1 + X0 + 1;
+                const int head = (int)((4 - (e0 & 3)) & 3);
+                const int nfull = cw > head ? (cw - head) >> 2 : 0;
+                if (g >= nfull) continue;
+                const int c0 = head + 4 * g;
+                const uint32_t *s4 = tile + rr * P + c0;
+                *reinterpret_cast<uint4 *>(out + e0 + c0) = make_uint4(s4[0], s4[1], s4[2], s4[3]);
+            }
+            // up to three elements before the first group and after the last
+            for (int slot = threadIdx.x; slot < kIntTile * 6; slot += 256) {
+                const int rr = slot / 6, k = slot % 6;
+                if (rr >= chh) continue;
+                const size_t e0 = (size_t)(Y0 + rr + 1) * W1 + X0 + 1;
+                const int head = (int)((4 - (e0 & 3)) & 3);
+                const int nfull = cw > head ? (cw - head) >> 2 : 0;
+                const int col = k < 3 ? (k < head ? k : kIntTile) : head + 4 * nfull + (k - 3);
+                if (col < cw) out[e0 + col] = tile[rr * P + col];
+            }
         }
         // the zero top row and left column
         if (ty == 0 && wave == 0) {
-            if (x < W) out[x + 1] = 0u;
+            if (lane < cw) out[X0 + lane + 1] = 0u;
             if (tx == 0 && lane == 0) out[0] = 0u;
         }
         if (tx == 0 && lane == 0) {
             for (int i = 0; i < 16; i++)
-                if (y0 + i < H) out[(size_t)(y0 + i + 1) * W1] = 0u;
+                if (wave * 16 + i < chh) out[(size_t)(Y0 + wave * 16 + i + 1) * W1] = 0u;
         }
         __syncthreads();
     }
@@ -982,25 +1053,41 @@ iph_status launch_integral_image(const uint8_t *src, uint32_t w, uint32_t h, int
     uint32_t *colsum = static_cast<uint32_t *>(ws);
     uint32_t *rowsum = colsum + (size_t)ty * w;
     const size_t ntiles = (size_t)tx * ty;
-    const unsigned grid = (unsigned)std::min<size_t>(ntiles, 1u << 20);
-    if (squared)
-        integral_tile_sums_kernel<true><<<grid, 256, 0, s>>>(src, (int)w, (int)h, tx, ntiles,
-                                                             colsum, rowsum);
-    else
-        integral_tile_sums_kernel<false><<<grid, 256, 0, s>>>(src, (int)w, (int)h, tx, ntiles,
-                                                              colsum, rowsum);
+    const unsigned grid = (unsigned)std::min<size_t>(8 * ((ntiles + 7) / 8), 1u << 20);
+    const bool vec = w % 16 == 0 && (reinterpret_cast<uintptr_t>(src) & 15) == 0;
+    const bool v4out = (reinterpret_cast<uintptr_t>(out) & 15) == 0;
+    const int W = (int)w, H = (int)h;
+#define IPH_INT_SUMS(SQ, VEC)                                                                  \
+    integral_tile_sums_kernel<SQ, VEC><<<grid, 256, 0, s>>>(src, W, H, tx, ntiles, colsum, rowsum)
+    if (squared) {
+        if (vec) IPH_INT_SUMS(true, true); else IPH_INT_SUMS(true, false);
+    } else {
+        if (vec) IPH_INT_SUMS(false, true); else IPH_INT_SUMS(false, false);
+    }
+#undef IPH_INT_SUMS
     IPH_TRY(launch_status("integral_tile_sums_kernel"));
     integral_scan_aux_kernel<<<(unsigned)(((size_t)w + h + 255) / 256), 256, 0, s>>>(
-        colsum, (int)w, ty, rowsum, (int)h, tx);
+        colsum, W, ty, rowsum, H, tx);
     IPH_TRY(launch_status("integral_scan_aux_kernel"));
-    integral_scan_pin_kernel<<<(unsigned)std::min(ty, 1 << 20), 256, 0, s>>>(colsum, (int)w, ty);
+    integral_scan_pin_kernel<<<(unsigned)std::min(ty, 1 << 20), 256, 0, s>>>(colsum, W, ty);
     IPH_TRY(launch_status("integral_scan_pin_kernel"));
-    if (squared)
-        integral_emit_kernel<true><<<grid, 256, 0, s>>>(src, (int)w, (int)h, tx, ntiles, colsum,
-                                                        rowsum, out);
-    else
-        integral_emit_kernel<false><<<grid, 256, 0, s>>>(src, (int)w, (int)h, tx, ntiles, colsum,
-                                                         rowsum, out);
+#define IPH_INT_EMIT(SQ, VEC, V4)                                                               \
+    integral_emit_kernel<SQ, VEC, V4><<<grid, 256, 0, s>>>(src, W, H, tx, ntiles, colsum, rowsum, \
+                                                          out)
+    if (squared) {
+        if (vec) {
+            if (v4out) IPH_INT_EMIT(true, true, true); else IPH_INT_EMIT(true, true, false);
+        } else {
+            if (v4out) IPH_INT_EMIT(true, false, true); else IPH_INT_EMIT(true, false, false);
+        }
+    } else {
+        if (vec) {
+            if (v4out) IPH_INT_EMIT(false, true, true); else IPH_INT_EMIT(false, true, false);
+        } else {
+            if (v4out) IPH_INT_EMIT(false, false, true); else IPH_INT_EMIT(false, false, false);
+        }
+    }
+#undef IPH_INT_EMIT
     return launch_status("integral_emit_kernel");
 }
 

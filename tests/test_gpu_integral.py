@@ -14,7 +14,7 @@ from imageproc_amd._lib import Panic
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(1, 1), (2, 3), (64, 64), (65, 63), (63, 129), (130, 257), (300, 1), (1, 300),
-          (777, 1023)]
+          (777, 1023), (3, 16), (65, 128), (130, 256), (1000, 1024), (70, 4112)]
 
 
 def _cumsum_u32(img, squared):
@@ -71,6 +71,25 @@ def test_integral_device_entry_matches_host_entry():
     small = torch.from_numpy(img[:70, :90].copy()).cuda()
     got = device.integral_image(small).cpu().numpy().view(np.uint32)
     assert np.array_equal(got, _cumsum_u32(img[:70, :90], False))
+
+
+@pytest.mark.parametrize("shape", [(300, 512), (129, 400), (64, 48)])
+def test_integral_unaligned_buffers(shape):
+    """The device entry on an image and a table that are not 16-byte aligned
+    (the kernels then load bytes and store single elements)."""
+    h, w = shape
+    img = np.random.default_rng(w).integers(0, 256, shape, dtype=np.uint8)
+    flat = torch.zeros(h * w + 16, dtype=torch.uint8, device="cuda")
+    for src_off in (0, 1):
+        src = flat[src_off:src_off + h * w].view(h, w)
+        src.copy_(torch.from_numpy(img))
+        for out_off in (0, 1, 2, 3):
+            buf = torch.zeros((h + 1) * (w + 1) + 4, dtype=torch.int32, device="cuda")
+            out = buf[out_off:out_off + (h + 1) * (w + 1)].view(h + 1, w + 1)
+            for sq in (False, True):
+                device.integral_image(src, sq, out=out)
+                got = out.cpu().numpy().view(np.uint32)
+                assert np.array_equal(got, _cumsum_u32(img, sq)), (src_off, out_off, sq)
 
 
 def test_adaptive_zebra_truth(fixtures):
