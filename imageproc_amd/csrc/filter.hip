@@ -1053,6 +1053,7 @@ iph_status launch_integral_image(const uint8_t *src, uint32_t w, uint32_t h, int
     uint32_t *colsum = static_cast<uint32_t *>(ws);
     uint32_t *rowsum = colsum + (size_t)ty * w;
     const size_t ntiles = (size_t)tx * ty;
+    // one workgroup per tile up to the cap (a multiple of 8 keeps the XCD bands)
     const unsigned grid = (unsigned)std::min<size_t>(8 * ((ntiles + 7) / 8), 1u << 20);
     const bool vec = w % 16 == 0 && (reinterpret_cast<uintptr_t>(src) & 15) == 0;
     const bool v4out = (reinterpret_cast<uintptr_t>(out) & 15) == 0;

@@ -51,6 +51,15 @@ def test_smooth_vs_oracle(seed, shape, sigma, low, high):
     assert np.array_equal(edges.canny(img, low, high), want)
 
 
+def test_thresholds_on_exact_magnitudes():
+    """Thresholds equal to magnitudes that occur: sqrtf(n) >= t must hold at
+    equality on both thresholds."""
+    img = _smooth(7, (257, 333), 2.0)
+    for low, high in [(float(np.sqrt(np.float32(2000))), 100.0), (5.0, 13.0), (1.0, 1.0),
+                      (float(np.sqrt(np.float32(2))), float(np.sqrt(np.float32(50))))]:
+        assert np.array_equal(edges.canny(img, low, high), O.canny(img, low, high)), (low, high)
+
+
 def test_long_edges_cross_many_tiles():
     img = _stripes((1100, 1700), 90.0)
     for low, high in [(5.0, 40.0), (30.0, 200.0)]:

@@ -417,6 +417,10 @@ def test_canny_direction_and_magnitude_over_the_sobel_domain():
     n = gx * gx + gy * gy
     assert np.array_equal(mag, np.sqrt(n.astype(np.float64)).astype(np.float32))
     assert np.array_equal(mag, np.sqrt(n.astype(np.float32)))
+    # sqrtf is strictly increasing on 0 ..= 2 * 1020^2, so magnitudes compare
+    # like their integer squares
+    roots = np.sqrt(np.arange(0, 2 * 1020 * 1020 + 1, dtype=np.float32))
+    assert (np.diff(roots) > 0).all()
     # spot values of the bins
     assert O.canny_direction(1, 0) == 0 and O.canny_direction(-1, 0) == 0
     assert O.canny_direction(0, 1) == 90 and O.canny_direction(0, -1) == 90
