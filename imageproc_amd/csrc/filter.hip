@@ -1092,6 +1092,144 @@ iph_status launch_integral_image(const uint8_t *src, uint32_t w, uint32_t h, int
     return launch_status("integral_emit_kernel");
 }
 
+// ---- adaptive_threshold for block radii up to 64: block sums in LDS ----------------
+//
+// One 64 x 64 output tile per workgroup. The tile's source window (radius r
+// around it; rows and columns outside the image staged as zeros, which add
+// nothing to a clipped block) is staged in LDS; vertical window sums of 2r+1
+// rows are slid down each column four columns at a time (two u16 halves per
+// u32: a sum of at most 129 bytes stays below 2^16), then horizontal window
+// sums of 2r+1 of those are slid along each row. The block sums are exact
+// (below 2^32), equal to the reference's wrapping four-corner sums, and the
+// mean, threshold and comparison follow contrast.rs:19-47.
+namespace {
+constexpr int kAdT = 64;     // output tile side
+constexpr int kAdMaxR = 64;  // largest radius of this path
+
+struct AdShape {
+    int r, off, sw, sr;  // radius; (X0 - r) - xs; staged columns (multiple of 4); staged rows
+    // V rows are sw + 2 u16 apart: an odd dword pitch, so the 16 rows one
+    // wave reads at a time fall in distinct banks
+    size_t lds() const { return (size_t)sw * sr + (size_t)kAdT * (sw + 2) * 2; }
+};
+
+AdShape ad_shape(int r) {
+    AdShape a;
+    a.r = r;
+    a.off = (4 - (r & 3)) & 3;  // X0 is a multiple of 64, so (X0 - r) mod 4 = (-r) mod 4
+    a.sw = (a.off + kAdT + 2 * r + 3) & ~3;
+    a.sr = kAdT + 2 * r;
+    return a;
+}
+
+template <bool VEC>
+__global__ __launch_bounds__(256) void adaptive_block_kernel(const uint8_t *__restrict__ src,
+                                                             int W, int H, AdShape a, int delta,
+                                                             int tiles_x,
+                                                             uint8_t *__restrict__ dst) {
+    extern __shared__ __align__(16) unsigned char ad_smem[];
+    uint8_t *S = ad_smem;                                               // sr x sw bytes
+    uint16_t *V = reinterpret_cast<uint16_t *>(ad_smem + (size_t)a.sr * a.sw);  // 64 x sw
+    const int r = a.r, sw = a.sw, sr = a.sr, off = a.off;
+    const int X0 = (int)(blockIdx.x % (unsigned)tiles_x) * kAdT;
+    const int Y0 = (int)(blockIdx.x / (unsigned)tiles_x) * kAdT;
+    const int xs = X0 - r - off, ys = Y0 - r;  // image position of S[0][0] (xs % 4 == 0)
+    const int quads = sw >> 2;
+    // stage the window, one dword (4 columns) per item, twelve loads in
+    // flight per thread before their LDS stores
+    const int nitems = sr * quads;
+    for (int i0 = 0; i0 < nitems; i0 += 256 * 12) {
+        uint32_t v[12];
+#pragma unroll
+        for (int u = 0; u < 12; u++) {
+            const int i = i0 + 256 * u + (int)threadIdx.x;
+            const int k = i / quads, q = i - k * quads;
+            const int y = ys + k, x = xs + 4 * q;
+            v[u] = 0;
+            if (i < nitems && y >= 0 && y < H) {
+                const uint8_t *row = src + (size_t)y * W;
+                if (VEC && x >= 0 && x + 4 <= W) {
+                    v[u] = *reinterpret_cast<const uint32_t *>(row + x);
+                } else {
+#pragma unroll
+                    for (int b = 0; b < 4; b++)
+                        if (x + b >= 0 && x + b < W) v[u] |= (uint32_t)row[x + b] << (8 * b);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 12; u++) {
+            const int i = i0 + 256 * u + (int)threadIdx.x;
+            if (i < nitems) reinterpret_cast<uint32_t *>(S)[i] = v[u];
+        }
+    }
+    __syncthreads();
+    // vertical sums: item (quad, 16-row segment)
+    for (int i = threadIdx.x; i < quads * 4; i += 256) {
+        const int q = i % quads, seg = i / quads;
+        const uint32_t *col = reinterpret_cast<const uint32_t *>(S) + q;
+        const int qs = sw >> 2;  // dwords per staged row
+        uint32_t lo = 0, hi = 0;  // columns (0, 2) and (1, 3) of the quad
+        const int y0 = 16 * seg;
+#pragma unroll 8
+        for (int k = y0; k <= y0 + 2 * r; k++) {
+            const uint32_t w = col[k * qs];
+            lo += w & 0x00ff00ffu;
+            hi += (w >> 8) & 0x00ff00ffu;
+        }
+#pragma unroll
+        for (int yy = y0; yy < y0 + 16; yy++) {
+            uint32_t *vr = reinterpret_cast<uint32_t *>(V + (size_t)yy * (sw + 2) + 4 * q);
+            vr[0] = (lo & 0xffffu) | (hi << 16);
+            vr[1] = (lo >> 16) | (hi & 0xffff0000u);
+            if (yy + 1 < y0 + 16) {
+                const uint32_t add = col[(yy + 2 * r + 1) * qs], sub = col[yy * qs];
+                lo += (add & 0x00ff00ffu) - (sub & 0x00ff00ffu);
+                hi += ((add >> 8) & 0x00ff00ffu) - ((sub >> 8) & 0x00ff00ffu);
+            }
+        }
+    }
+    __syncthreads();
+    // horizontal sums, mean and comparison: item (row, 16-column segment)
+    const int yy = threadIdx.x >> 2, seg = threadIdx.x & 3;
+    const int y = Y0 + yy;
+    if (y >= H) return;
+    const uint16_t *vr = V + (size_t)yy * (sw + 2);
+    const int j0 = 16 * seg + off;  // staged column of the first window's left edge
+    uint32_t sum = 0;
+#pragma unroll 8
+    for (int j = j0; j <= j0 + 2 * r; j++) sum += vr[j];
+    const uint32_t rows = (uint32_t)(min(H - 1, y + r) - max(0, y - r) + 1);
+    const uint8_t *px = S + (size_t)(yy + r) * sw + off + r;
+    uint32_t packed[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+        const int xl = 16 * seg + u, x = X0 + xl;
+        const uint32_t cols = (uint32_t)(min(W - 1, x + r) - max(0, x - r) + 1);
+        // floor(sum / n): sum < 2^23 and n < 2^15 are exact in f32 and the
+        // quotient (a mean of bytes) is at most 255, so the f32 estimate is
+        // within 1e-4 of it; one correction step each way makes it exact
+        const uint32_t n = rows * cols;
+        uint32_t mean = (uint32_t)((float)sum * __builtin_amdgcn_rcpf((float)n));
+        mean -= mean * n > sum ? 1u : 0u;
+        mean += (mean + 1) * n <= sum ? 1u : 0u;
+        const int32_t thr = (int32_t)(mean - (uint32_t)delta);
+        const uint32_t o = (int32_t)px[xl] >= thr ? 255u : 0u;
+        packed[u >> 2] |= o << (8 * (u & 3));
+        if (u < 15) sum += (uint32_t)vr[j0 + u + 2 * r + 1] - (uint32_t)vr[j0 + u];
+    }
+    const int x0 = X0 + 16 * seg;
+    uint8_t *orow = dst + (size_t)y * W;
+    if (VEC && x0 + 16 <= W) {
+        *reinterpret_cast<uint4 *>(orow + x0) = make_uint4(packed[0], packed[1], packed[2], packed[3]);
+    } else {
+#pragma unroll
+        for (int u = 0; u < 16; u++)
+            if (x0 + u < W) orow[x0 + u] = (uint8_t)(packed[u >> 2] >> (8 * (u & 3)));
+    }
+}
+}  // namespace
+
 size_t adaptive_threshold_workspace(uint32_t w, uint32_t h) {
     const size_t table = (((size_t)w + 1) * ((size_t)h + 1) * sizeof(uint32_t) + 255) & ~(size_t)255;
     return table + integral_workspace(w, h);
@@ -1101,6 +1239,26 @@ iph_status launch_adaptive_threshold(const uint8_t *src, uint32_t w, uint32_t h,
                                      uint32_t block_radius, int32_t delta, uint8_t *dst, void *ws,
                                      size_t wsb, hipStream_t s) {
     if (w == 0 || h == 0) return IPH_OK;
+    // A radius past the image's larger side clips to the same block.
+    const int r = (int)std::min<uint32_t>(block_radius, std::max(w, h));
+    if (r <= kAdMaxR) {
+        const AdShape sh = ad_shape(r);
+        const int tiles_x = (int)((w + kAdT - 1) / kAdT);
+        const size_t ntiles = (size_t)tiles_x * ((h + kAdT - 1) / kAdT);
+        if (ntiles > 0x7fffffffu) {
+            set_error("adaptive threshold of %ux%u pixels is too large", w, h);
+            return IPH_EUNSUPPORTED;
+        }
+        const bool vec = w % 16 == 0 && (reinterpret_cast<uintptr_t>(src) & 15) == 0 &&
+                         (reinterpret_cast<uintptr_t>(dst) & 15) == 0;
+        if (vec)
+            adaptive_block_kernel<true><<<(unsigned)ntiles, 256, sh.lds(), s>>>(
+                src, (int)w, (int)h, sh, delta, tiles_x, dst);
+        else
+            adaptive_block_kernel<false><<<(unsigned)ntiles, 256, sh.lds(), s>>>(
+                src, (int)w, (int)h, sh, delta, tiles_x, dst);
+        return launch_status("adaptive_block_kernel");
+    }
     if (wsb < adaptive_threshold_workspace(w, h)) {
         set_error("adaptive threshold workspace of %zu bytes is below the %zu needed", wsb,
                   adaptive_threshold_workspace(w, h));
@@ -1111,8 +1269,6 @@ iph_status launch_adaptive_threshold(const uint8_t *src, uint32_t w, uint32_t h,
     uint32_t *ii = static_cast<uint32_t *>(ws);
     IPH_TRY(launch_integral_image(src, w, h, 0, ii, static_cast<char *>(ws) + table, wsb - table,
                                   s));
-    // A radius past the image's larger side clips to the same block.
-    const int r = (int)std::min<uint32_t>(block_radius, std::max(w, h));
     const unsigned gy = std::min((h + 3) / 4, 65535u);
     adaptive_threshold_kernel<<<dim3((w + 63) / 64, gy), 256, 0, s>>>(
         src, ii, (int)w, (int)h, r, delta, dst);

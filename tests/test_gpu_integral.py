@@ -134,6 +134,35 @@ def test_adaptive_vs_oracle(shape, r, delta):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("shape,r", [((200, 256), 2), ((131, 300), 4), ((300, 517), 63),
+                                     ((129, 130), 64), ((150, 160), 65), ((64, 64), 64),
+                                     ((70, 1), 3), ((1, 70), 3), ((1024, 1024), 41)])
+def test_adaptive_radii_around_the_block_path(shape, r):
+    """Radii up to 64 take the block-sum kernel, larger ones the integral
+    table; both against the oracle, through the host and the device entry."""
+    print("adaptive radius", shape, r, flush=True)
+    img = np.random.default_rng(r * 31 + shape[1]).integers(0, 256, shape, dtype=np.uint8)
+    for delta in (0, -3, 7):
+        want = O.adaptive_threshold(img, r, delta)
+        assert np.array_equal(contrast.adaptive_threshold(img, r, delta), want), delta
+    got = device.adaptive_threshold(torch.from_numpy(img).cuda(), r, 0).cpu().numpy()
+    assert np.array_equal(got, O.adaptive_threshold(img, r, 0))
+
+
+def test_adaptive_unaligned_buffers():
+    h, w = 97, 256
+    img = np.random.default_rng(3).integers(0, 256, (h, w), dtype=np.uint8)
+    want = O.adaptive_threshold(img, 9, 2)
+    flat = torch.zeros(h * w + 32, dtype=torch.uint8, device="cuda")
+    for off in (0, 1, 5):
+        src = flat[off:off + h * w].view(h, w)
+        src.copy_(torch.from_numpy(img))
+        obuf = torch.zeros(h * w + 32, dtype=torch.uint8, device="cuda")
+        out = obuf[3:3 + h * w].view(h, w)
+        device.adaptive_threshold(src, 9, 2, out=out)
+        assert np.array_equal(out.cpu().numpy(), want), off
+
+
 def test_adaptive_errors_and_empty():
     img = np.zeros((4, 4), np.uint8)
     with pytest.raises(Panic, match="block_radius > 0"):
