@@ -176,3 +176,28 @@ def test_canny_argument_errors_before_device():
     with pytest.raises(Panic, match="overflow"):
         canny(np.zeros((3, 0), np.uint8), 1.0, 5.0)
     assert _lib.lib().iph_dev_canny_workspace(100, 70) >= 2 * 100 * 70
+
+
+def test_sum_image_pixels_and_variance_on_tables():
+    """integral_image.rs:25-48, 273-301 and 513-535: the host helpers over
+    tables (built here by the oracle)."""
+    from imageproc_amd.integral_image import sum_image_pixels, variance
+
+    img = np.array([[1, 2, 3], [4, 5, 6]], np.uint8)
+    ii, sq = O.integral_image(img), O.integral_image(img, True)
+    assert sum_image_pixels(ii, 1, 0, 2, 1) == 2 + 3 + 5 + 6
+    assert sum_image_pixels(ii, 0, 0, 2, 0) == 1 + 2 + 3
+    assert sum_image_pixels(sq, 1, 0, 2, 1) == 4 + 9 + 25 + 36
+    mean = (2.0 + 3.0 + 5.0 + 6.0) / 4.0
+    var = ((2.0 - mean) ** 2 + (3.0 - mean) ** 2 + (5.0 - mean) ** 2 + (6.0 - mean) ** 2) / 4.0
+    assert variance(ii, sq, 1, 0, 2, 1) == var
+    img = np.array([[1, 2], [3, 4]], np.uint8)  # integral_image.rs:513-535
+    ii = O.integral_image(img)
+    for (l, t, r, b), want in [((0, 0, 0, 0), 1), ((0, 0, 1, 0), 3), ((0, 0, 0, 1), 4),
+                               ((0, 0, 1, 1), 10), ((1, 0, 1, 0), 2), ((1, 0, 1, 1), 6),
+                               ((0, 1, 0, 1), 3), ((0, 1, 1, 1), 7), ((1, 1, 1, 1), 4)]:
+        assert sum_image_pixels(ii, l, t, r, b) == want
+    # wrapping: a squared table of 255s past 2^32 still gives exact window sums
+    big = np.full((300, 400), 255, np.uint8)
+    sq = O.integral_image(big, True)
+    assert sum_image_pixels(sq, 10, 20, 41, 51) == 32 * 32 * 255 * 255
