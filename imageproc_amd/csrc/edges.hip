@@ -37,6 +37,7 @@ namespace iph {
 namespace {
 
 constexpr int kCannyTW = 64, kCannyTH = 16;
+constexpr int kHystT = 64;  // hysteresis tile side
 constexpr int kCannyIters = 4;  // hysteresis launches between flag checks
 constexpr int kCannyGrid = 1 << 20;  // workgroup cap of the tile kernels
 
@@ -58,56 +59,78 @@ __device__ __forceinline__ int canny_direction(int gx, int gy) {
     return lo > 0 ? 135 : 0;
 }
 
-using CannyPx = int16_t[kCannyTW + 4];
+// Staged blurred window: image rows Y0 - 2 .. Y0 + kCannyTH + 1, columns
+// X0 - 4 .. X0 + kCannyTW + 3 (18 dwords, so interior tiles load whole
+// aligned dwords), clamped to the image like filter_clamped's reads.
+constexpr int kCannyPW = kCannyTW + 8;
+using CannyPx = uint8_t[kCannyPW];
 
-// Sobel responses with taps px[ly .. ly + 2][lx .. lx + 2] (filter/mod.rs:29-69
+// Sobel responses with taps px[ly .. ly + 2][c .. c + 2] (filter/mod.rs:29-69
 // order; the i32 sums are exact and within i16 for u8 input).
-__device__ __forceinline__ void canny_sobel(const CannyPx *px, int ly, int lx, int &gx, int &gy) {
-    const int a = px[ly][lx], b = px[ly][lx + 1], c = px[ly][lx + 2];
-    const int d = px[ly + 1][lx], f = px[ly + 1][lx + 2];
-    const int g = px[ly + 2][lx], h = px[ly + 2][lx + 1], k = px[ly + 2][lx + 2];
-    gx = (c - a) + 2 * (f - d) + (k - g);  // SOBEL_HORIZONTAL_3X3
-    gy = (g - a) + 2 * (h - b) + (k - c);  // SOBEL_VERTICAL_3X3
+__device__ __forceinline__ void canny_sobel(const CannyPx *px, int ly, int c, int &gx, int &gy) {
+    const int a = px[ly][c], b = px[ly][c + 1], cc = px[ly][c + 2];
+    const int d = px[ly + 1][c], f = px[ly + 1][c + 2];
+    const int g = px[ly + 2][c], h = px[ly + 2][c + 1], k = px[ly + 2][c + 2];
+    gx = (cc - a) + 2 * (f - d) + (k - g);  // SOBEL_HORIZONTAL_3X3
+    gy = (g - a) + 2 * (h - b) + (k - cc);  // SOBEL_VERTICAL_3X3
 }
 
 // Magnitudes are compared through n = gx^2 + gy^2: sqrtf is strictly
 // increasing on the integers up to 2^21 (neighbouring square roots there are
 // at least two ulps apart), so m < c <=> n < n_c, and thinned >= t <=>
 // n >= n_t with n_t the least n whose sqrtf reaches t (found on the host).
-// seeds: per-tile flag, set when the tile holds a seed.
+// seeds: per-tile flag, set when the tile holds a seed. VEC: W % 4 == 0, so
+// interior tiles stage whole dwords and every thread stores its four states
+// as one dword. One workgroup per tile: measured faster here than persistent
+// workgroups prefetching their next tile.
+template <bool VEC>
 __global__ __launch_bounds__(256) void canny_nms_kernel(const uint8_t *__restrict__ blur, int W,
                                                         int H, int tiles_x, int ntiles,
                                                         int n_low, int n_high,
                                                         uint8_t *__restrict__ state,
                                                         uint8_t *__restrict__ seeds) {
-    // blurred tile with a two-pixel ring; n and packed (gx, gy) with a one-pixel ring
-    __shared__ int16_t px[kCannyTH + 4][kCannyTW + 4];
-    __shared__ int nn[kCannyTH + 2][kCannyTW + 2];
-    __shared__ int gxy[kCannyTH][kCannyTW];
+    __shared__ __align__(16) uint8_t px[kCannyTH + 4][kCannyPW];
+    __shared__ int nn[kCannyTH + 2][kCannyTW + 2];  // n with a one-pixel ring
+    __shared__ int gxy[kCannyTH][kCannyTW];         // packed (gx, gy)
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int X0 = (t % tiles_x) * kCannyTW, Y0 = (t / tiles_x) * kCannyTH;
-        for (int i = threadIdx.x; i < (kCannyTH + 4) * (kCannyTW + 4); i += 256) {
-            const int ly = i / (kCannyTW + 4), lx = i % (kCannyTW + 4);
-            const int y = min(max(Y0 - 2 + ly, 0), H - 1), x = min(max(X0 - 2 + lx, 0), W - 1);
-            px[ly][lx] = blur[(size_t)y * W + x];
+        if (VEC && X0 >= 4 && X0 + kCannyPW - 4 <= W) {  // the same for the whole workgroup
+            constexpr int kQ = kCannyPW / 4;
+            for (int i = threadIdx.x; i < (kCannyTH + 4) * kQ; i += 256) {
+                const int ly = i / kQ, q = i - ly * kQ;
+                const int y = min(max(Y0 - 2 + ly, 0), H - 1);
+                *reinterpret_cast<uint32_t *>(&px[ly][4 * q]) =
+                    *reinterpret_cast<const uint32_t *>(blur + (size_t)y * W + X0 - 4 + 4 * q);
+            }
+        } else {
+            for (int i = threadIdx.x; i < (kCannyTH + 4) * kCannyPW; i += 256) {
+                const int ly = i / kCannyPW, c = i - ly * kCannyPW;
+                const int y = min(max(Y0 - 2 + ly, 0), H - 1);
+                const int x = min(max(X0 - 4 + c, 0), W - 1);
+                px[ly][c] = blur[(size_t)y * W + x];
+            }
         }
         __syncthreads();
-        // ring position (ly, lx) <-> image (Y0 - 1 + ly, X0 - 1 + lx)
+        // ring position (ly, lx) <-> image (Y0 - 1 + ly, X0 - 1 + lx); its taps
+        // start at staged column lx + 2
         for (int i = threadIdx.x; i < (kCannyTH + 2) * (kCannyTW + 2); i += 256) {
             const int ly = i / (kCannyTW + 2), lx = i % (kCannyTW + 2);
             int gx, gy;
-            canny_sobel(px, ly, lx, gx, gy);
+            canny_sobel(px, ly, lx + 2, gx, gy);
             nn[ly][lx] = gx * gx + gy * gy;
             if (ly >= 1 && ly <= kCannyTH && lx >= 1 && lx <= kCannyTW)
                 gxy[ly - 1][lx - 1] = (int)(((uint32_t)gx & 0xffffu) | ((uint32_t)gy << 16));
         }
         __syncthreads();
-        const int lx = threadIdx.x & 63, x = X0 + lx;
+        // thread: row r, columns c0 .. c0 + 3
+        const int r = threadIdx.x >> 4, c0 = 4 * (threadIdx.x & 15);
+        const int y = Y0 + r;
         bool seed = false;
-        if (x < W) {
-            for (int r = threadIdx.x >> 6; r < kCannyTH; r += 4) {
-                const int y = Y0 + r;
-                if (y >= H) break;
+        if (y < H) {
+            uint32_t packed = 0;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int lx = c0 + u, x = X0 + lx;
                 int thin = 0;
                 const bool interior = x >= 1 && y >= 1 && x + 1 < W && y + 1 < H;
                 if (interior) {
@@ -126,81 +149,194 @@ __global__ __launch_bounds__(256) void canny_nms_kernel(const uint8_t *__restric
                 }
                 const bool sd = interior && thin >= n_high;
                 seed |= sd;
-                state[(size_t)y * W + x] = sd ? 2 : (thin >= n_low ? 1 : 0);
+                packed |= (uint32_t)(sd ? 2 : (thin >= n_low ? 1 : 0)) << (8 * u);
+            }
+            const int xq = X0 + c0;
+            uint8_t *row = state + (size_t)y * W;
+            if (VEC && xq + 4 <= W) {
+                *reinterpret_cast<uint32_t *>(row + xq) = packed;
+            } else {
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    if (xq + u < W) row[xq + u] = (uint8_t)(packed >> (8 * u));
             }
         }
+        // seeds: one flag per 64 x 64 hysteresis tile, cleared beforehand
         const bool any = __syncthreads_or(seed);
-        if (threadIdx.x == 0) seeds[t] = any ? 1 : 0;
+        if (threadIdx.x == 0 && any)
+            seeds[(size_t)(Y0 / kHystT) * tiles_x + (X0 / kHystT)] = 1;
     }
 }
 
-// One launch of the propagation. A tile works only when it or one of its
-// eight neighbour tiles marked a pixel on its outer ring (the pixels other
-// tiles see) in the previous launch (prev; the seed flags before the first
-// launch), and records in cur whether it did so itself: marks inside a tile
-// are already at the tile's fixed point and concern no other tile.
-// changed[0]: some tile marked a pixel; border[0]: a marked pixel lies on the
-// image border.
-__global__ __launch_bounds__(256) void canny_hysteresis_kernel(
+// Row masks of a hysteresis tile: bit i of a row is column X0 + i.
+// Occluded fills (Kogge-Stone): every bit of c reachable from s by steps
+// towards higher (fill_up) or lower (fill_down) bits through c.
+__device__ __forceinline__ uint64_t fill_up(uint64_t s, uint64_t c) {
+    s |= c & (s << 1);
+    c &= c << 1;
+    s |= c & (s << 2);
+    c &= c << 2;
+    s |= c & (s << 4);
+    c &= c << 4;
+    s |= c & (s << 8);
+    c &= c << 8;
+    s |= c & (s << 16);
+    c &= c << 16;
+    return s | (c & (s << 32));
+}
+__device__ __forceinline__ uint64_t fill_down(uint64_t s, uint64_t c) {
+    s |= c & (s >> 1);
+    c &= c >> 1;
+    s |= c & (s >> 2);
+    c &= c >> 2;
+    s |= c & (s >> 4);
+    c &= c >> 4;
+    s |= c & (s >> 8);
+    c &= c >> 8;
+    s |= c & (s >> 16);
+    c &= c >> 16;
+    return s | (c & (s >> 32));
+}
+
+// 4 bytes (each 0 or 1) -> 4 bits: b_i * 2^(8i) * 2^(7(3-i)) lands on bit 21 + i,
+// and no two terms of the product share a bit.
+__device__ __forceinline__ uint32_t bytes_to_nibble(uint32_t x) {
+    return ((x * 0x00204081u) >> 21) & 0xfu;
+}
+// The inverse: bit i of a nibble -> byte i.
+__device__ __forceinline__ uint32_t nibble_to_bytes(uint32_t n) {
+    return (n * 0x00204081u) & 0x01010101u;
+}
+
+// Candidate (state >= 1) and marked (state == 2) masks of the 64 state bytes
+// of one row starting at column x0 (columns past W read as 0).
+template <bool VEC>
+__device__ __forceinline__ void hyst_load_row(const uint8_t *__restrict__ state, int W, int y,
+                                              int x0, uint64_t &cand, uint64_t &mark) {
+    cand = 0;
+    mark = 0;
+    const uint8_t *row = state + (size_t)y * W;
+    uint32_t w[16];
+    if (VEC && x0 + 64 <= W) {
+        const uint4 *r4 = reinterpret_cast<const uint4 *>(row + x0);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint4 v = r4[k];
+            w[4 * k] = v.x;
+            w[4 * k + 1] = v.y;
+            w[4 * k + 2] = v.z;
+            w[4 * k + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            w[k] = 0;
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+                if (x0 + 4 * k + b < W) w[k] |= (uint32_t)row[x0 + 4 * k + b] << (8 * b);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        cand |= (uint64_t)bytes_to_nibble((w[k] | (w[k] >> 1)) & 0x01010101u) << (4 * k);
+        mark |= (uint64_t)bytes_to_nibble((w[k] >> 1) & 0x01010101u) << (4 * k);
+    }
+}
+
+// One launch of the propagation over 64 x 64 tiles, one wave per tile, a
+// lane per row. A tile works only when it or one of its eight neighbour
+// tiles marked a pixel on its outer ring (the pixels other tiles see) in the
+// previous launch (prev; the seed flags before the first launch), and
+// records in cur whether it did so itself: marks inside a tile are already
+// at the tile's fixed point and concern no other tile. Per step, a row
+// takes the marks that reach it from the rows above and below (the
+// predecessors of edges.rs:130-137: (x-1, y-1), (x, y-1), (x+1, y-1) and
+// (x+1, y+1)) and the neighbouring tiles' ring pixels, then fills its runs of
+// candidates in both directions ((x-1, y) and (x+1, y)); steps repeat until
+// no row changes. changed[0]: some tile marked a pixel; border[0]: a marked
+// pixel lies on the image border.
+template <bool VEC>
+__global__ __launch_bounds__(64) void canny_hysteresis_kernel(
     uint8_t *__restrict__ state, int W, int H, int tiles_x, int ntiles,
     const uint8_t *__restrict__ prev, uint8_t *__restrict__ cur, int *__restrict__ changed,
     int *__restrict__ border) {
-    __shared__ uint8_t st[kCannyTH + 2][kCannyTW + 2];
+    __shared__ uint64_t M[kHystT + 2];            // marks of rows -1 .. 64
+    __shared__ uint32_t HL[kHystT + 2], HR[kHystT + 2];  // marks at columns -1 and 64
+    const int y = threadIdx.x;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const int bx = t % tiles_x, by = t / tiles_x, tx = tiles_x, ty = ntiles / tiles_x;
+        const int bx = t % tiles_x, by = t / tiles_x, ty = ntiles / tiles_x;
         bool active = false;
         for (int j = max(by - 1, 0); j <= min(by + 1, ty - 1); j++)
-            for (int i = max(bx - 1, 0); i <= min(bx + 1, tx - 1); i++)
-                active |= prev[(size_t)j * tx + i] != 0;
+            for (int i = max(bx - 1, 0); i <= min(bx + 1, tiles_x - 1); i++)
+                active |= prev[(size_t)j * tiles_x + i] != 0;
         if (!active) {  // the same for the whole workgroup
-            if (threadIdx.x == 0) cur[t] = 0;
+            if (y == 0) cur[t] = 0;
             continue;
         }
-        const int X0 = bx * kCannyTW, Y0 = by * kCannyTH;
-        for (int i = threadIdx.x; i < (kCannyTH + 2) * (kCannyTW + 2); i += 256) {
-            const int ly = i / (kCannyTW + 2), lx = i % (kCannyTW + 2);
-            const int y = Y0 - 1 + ly, x = X0 - 1 + lx;
-            st[ly][lx] = (x >= 0 && y >= 0 && x < W && y < H) ? state[(size_t)y * W + x] : 0;
-        }
-        __syncthreads();
-        const int lx = (threadIdx.x & 63) + 1, r0 = threadIdx.x >> 6;
-        bool mine = false;  // this thread marked one of its pixels
-        for (;;) {
-            bool step = false;
-            for (int r = r0; r < kCannyTH; r += 4) {
-                const int ly = r + 1;
-                if (st[ly][lx] != 1) continue;
-                // the pixels whose neighbour list (edges.rs:130-137) holds this one
-                if (st[ly][lx - 1] == 2 || st[ly - 1][lx - 1] == 2 || st[ly - 1][lx] == 2 ||
-                    st[ly + 1][lx + 1] == 2 || st[ly][lx + 1] == 2 || st[ly - 1][lx + 1] == 2) {
-                    st[ly][lx] = 2;
-                    step = true;
-                }
+        const int X0 = bx * kHystT, Y0 = by * kHystT, gy = Y0 + y;
+        uint64_t cand = 0, mark0 = 0;
+        if (gy < H) hyst_load_row<VEC>(state, W, gy, X0, cand, mark0);
+        // ring: rows -1 and 64, columns -1 and 64 of rows -1 .. 64
+        for (int lr = y; lr < kHystT + 2; lr += 64) {
+            const int ry = Y0 - 1 + lr;
+            const bool in = ry >= 0 && ry < H;
+            if (lr == 0 || lr == kHystT + 1) {
+                uint64_t c = 0, m = 0;
+                if (in) hyst_load_row<false>(state, W, ry, X0, c, m);
+                M[lr] = m;
             }
-            mine |= step;
+            const uint8_t *row = state + (size_t)(in ? ry : 0) * W;
+            HL[lr] = (in && X0 >= 1 && row[X0 - 1] == 2) ? 1u : 0u;
+            HR[lr] = (in && X0 + kHystT < W && row[X0 + kHystT] == 2) ? 1u : 0u;
+        }
+        M[y + 1] = mark0;
+        __syncthreads();
+        const uint64_t halo = (uint64_t)(HL[y + 1] | HL[y]) |
+                              ((uint64_t)(HR[y + 1] | HR[y] | HR[y + 2]) << 63);
+        uint64_t mark = mark0;
+        for (;;) {
+            const uint64_t up = M[y], dn = M[y + 2];
+            const uint64_t s = cand & (mark | (up << 1) | up | (up >> 1) | (dn >> 1) | halo);
+            const uint64_t nm = fill_down(fill_up(s, cand), cand);
+            const bool step = nm != mark;
+            mark = nm;
+            __syncthreads();
+            M[y + 1] = mark;
             if (!__syncthreads_or(step)) break;
         }
-        const bool any = __syncthreads_or(mine);
-        bool ring = false;
-        if (any) {
-            const int x = X0 + lx - 1;
-            bool edge = false;
-            for (int r = r0; r < kCannyTH; r += 4) {
-                const int y = Y0 + r;
-                if (x < W && y < H && st[r + 1][lx] == 2) {
-                    // only pixels marked in this launch: state still holds 1 there
-                    if (state[(size_t)y * W + x] != 2) {
-                        state[(size_t)y * W + x] = 2;
-                        edge |= x == 0 || y == 0 || x == W - 1 || y == H - 1;
-                        ring |= r == 0 || r == kCannyTH - 1 || lx == 1 || lx == kCannyTW;
+        const uint64_t fresh = mark & ~mark0;
+        bool ring = false, edge = false;
+        if (fresh) {
+            uint8_t *row = state + (size_t)gy * W;
+            if (VEC && X0 + 64 <= W) {
+                uint4 *r4 = reinterpret_cast<uint4 *>(row + X0);
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    uint32_t v[4];
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        const int sh = 16 * k + 4 * e;
+                        v[e] = nibble_to_bytes((uint32_t)(cand >> sh) & 0xfu) +
+                               nibble_to_bytes((uint32_t)(mark >> sh) & 0xfu);
                     }
+                    r4[k] = make_uint4(v[0], v[1], v[2], v[3]);
                 }
+            } else {
+                for (int i = 0; i < 64 && X0 + i < W; i++)
+                    if ((fresh >> i) & 1) row[X0 + i] = 2;
             }
-            if (edge) border[0] = 1;
-            if (threadIdx.x == 0) changed[0] = 1;
+            ring = y == 0 || y == kHystT - 1 || (fresh & (1ull | (1ull << 63))) != 0;
+            const int last = W - 1 - X0;  // bit of column W - 1, when inside the tile
+            edge = gy == 0 || gy == H - 1 || (X0 == 0 && (fresh & 1)) ||
+                   (last < 64 && ((fresh >> last) & 1));
         }
+        if (edge) border[0] = 1;
+        const bool any = __syncthreads_or(fresh != 0);
         ring = __syncthreads_or(ring);
-        if (threadIdx.x == 0) cur[t] = ring ? 1 : 0;
+        if (y == 0) {
+            if (any) changed[0] = 1;
+            cur[t] = ring ? 1 : 0;
+        }
     }
 }
 
@@ -222,6 +358,10 @@ __global__ __launch_bounds__(256) void canny_finish_kernel(const uint8_t *__rest
         for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
             out[i] = (uint8_t)((state[i] >> 1) * 255);
     }
+}
+
+__global__ __launch_bounds__(256) void canny_fill_bytes_kernel(uint8_t *__restrict__ p, int n) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) p[i] = 0;
 }
 
 __global__ void canny_zero_flags_kernel(int *__restrict__ flags, int n) {
@@ -248,7 +388,7 @@ size_t canny_blur_taps(std::vector<float> &k) { return (size_t)gaussian_kernel(1
 size_t canny_workspace(uint32_t w, uint32_t h) {
     std::vector<float> k;
     const uint32_t len = (uint32_t)canny_blur_taps(k);
-    const size_t ntiles = ((size_t)w + kCannyTW - 1) / kCannyTW * (((size_t)h + kCannyTH - 1) / kCannyTH);
+    const size_t ntiles = ((size_t)w + kHystT - 1) / kHystT * (((size_t)h + kHystT - 1) / kHystT);
     WorkspacePlan p;
     p.take<int>(kCannyIters + 1);
     p.take<uint8_t>(ntiles);         // tile flags, even launches
@@ -272,7 +412,8 @@ iph_status launch_canny(const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h
     const unsigned grid = (unsigned)std::min(ntiles, kCannyGrid);
     Workspace a{static_cast<uint8_t *>(ws), wsb};
     int *flags = a.take<int>(kCannyIters + 1);
-    uint8_t *tfl[2] = {a.take<uint8_t>(ntiles), a.take<uint8_t>(ntiles)};
+    const int hyst_tiles = tiles_x * (int)((h + kHystT - 1) / kHystT);
+    uint8_t *tfl[2] = {a.take<uint8_t>(hyst_tiles), a.take<uint8_t>(hyst_tiles)};
     uint8_t *blur = a.take<uint8_t>(n);
     uint8_t *state = a.take<uint8_t>(n);
     const size_t sepb = separable_workspace(w, h, 1, IPH_U8, len, IPH_F32);
@@ -284,9 +425,16 @@ iph_status launch_canny(const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h
     int *border = flags + kCannyIters;
     IPH_TRY(launch_separable(src, blur, w, h, 1, n, IPH_U8, k.data(), k.data(), len, IPH_F32, sep,
                              sepb, s));
-    canny_nms_kernel<<<grid, 256, 0, s>>>(blur, (int)w, (int)h, tiles_x, ntiles,
-                                          canny_threshold_n(low), canny_threshold_n(high), state,
-                                          tfl[0]);
+    const int n_low = canny_threshold_n(low), n_high = canny_threshold_n(high);
+    canny_fill_bytes_kernel<<<(unsigned)std::min((hyst_tiles + 255) / 256, 4096), 256, 0, s>>>(
+        tfl[0], hyst_tiles);
+    IPH_TRY(launch_status("canny_fill_bytes_kernel"));
+    if (w % 4 == 0)
+        canny_nms_kernel<true><<<grid, 256, 0, s>>>(blur, (int)w, (int)h, tiles_x, ntiles, n_low,
+                                                    n_high, state, tfl[0]);
+    else
+        canny_nms_kernel<false><<<grid, 256, 0, s>>>(blur, (int)w, (int)h, tiles_x, ntiles, n_low,
+                                                     n_high, state, tfl[0]);
     IPH_TRY(launch_status("canny_nms_kernel"));
     canny_zero_flags_kernel<<<1, 64, 0, s>>>(border, 1);
     IPH_TRY(launch_status("canny_zero_flags_kernel"));
@@ -296,9 +444,15 @@ iph_status launch_canny(const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h
         canny_zero_flags_kernel<<<1, 64, 0, s>>>(flags, kCannyIters);
         IPH_TRY(launch_status("canny_zero_flags_kernel"));
         for (int i = 0; i < kCannyIters; i++, launch++) {
-            canny_hysteresis_kernel<<<grid, 256, 0, s>>>(state, (int)w, (int)h, tiles_x, ntiles,
-                                                         tfl[launch & 1], tfl[(launch + 1) & 1],
-                                                         flags + i, border);
+            const unsigned hgrid = (unsigned)std::min(hyst_tiles, kCannyGrid);
+            if (w % 16 == 0)
+                canny_hysteresis_kernel<true><<<hgrid, 64, 0, s>>>(
+                    state, (int)w, (int)h, tiles_x, hyst_tiles, tfl[launch & 1],
+                    tfl[(launch + 1) & 1], flags + i, border);
+            else
+                canny_hysteresis_kernel<false><<<hgrid, 64, 0, s>>>(
+                    state, (int)w, (int)h, tiles_x, hyst_tiles, tfl[launch & 1],
+                    tfl[(launch + 1) & 1], flags + i, border);
             IPH_TRY(launch_status("canny_hysteresis_kernel"));
         }
         int last = 0;
