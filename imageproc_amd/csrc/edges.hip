@@ -6,12 +6,13 @@
 //      edge reads), their squared magnitudes in LDS, non_maximum_suppression
 //      for the tile's pixels, and the hysteresis class of every pixel: 2 =
 //      seed (interior, thinned >= high), 1 = candidate (thinned >= low), 0 =
-//      none; plus a per-tile flag for tiles holding a seed;
+//      none; plus a flag for each 64 x 64 tile holding a seed;
 //   3. canny_hysteresis_kernel, repeated until a launch marks nothing: a
 //      candidate becomes marked when one of the pixels whose six-neighbour
-//      list (edges.rs:130-137) holds it is marked. Each launch iterates its
-//      tile in LDS to a local fixed point, so marks cross one tile border per
-//      launch at least; a tile whose 3 x 3 tile neighbourhood marked nothing
+//      list (edges.rs:130-137) holds it is marked. Each launch takes a 64 x 64
+//      tile as 64 row bitmasks (candidates, marks) and iterates it to a local
+//      fixed point, so marks cross one tile border per launch at least; a
+//      tile whose 3 x 3 tile neighbourhood marked nothing on its outer ring
 //      in the previous launch returns at once. The marked set is the union of
 //      what the reference's depth-first searches reach, which does not depend
 //      on visiting order. A marked pixel on the image border raises a flag:
