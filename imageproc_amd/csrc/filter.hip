@@ -745,15 +745,15 @@ iph_status launch_gradients(const uint8_t *src, uint32_t w, uint32_t h, const in
 // arithmetic here is u32 too, so any order of additions gives the same bits).
 // The image is cut into 64 x 64 tiles and read twice, written once:
 //   1. tile_sums: per tile, its 64 column sums and 64 row sums;
-//   2. scan_aux: column sums scanned down the tile rows (exclusive), row sums
-//      scanned across the tile columns (exclusive);
-//   3. scan_pin: each tile row's exclusive column sums scanned across the
-//      width (inclusive), giving P[ty][x] = sum of all pixels above the tile
-//      row in columns <= x;
-//   4. emit: per tile, out(y, x) = P[ty][x] + sum over the tile's rows
-//      y' <= y of (R[tx][y'] + in-tile prefix of row y' up to x), where
-//      R[tx][y'] is the sum of row y' left of the tile.
-// Workspace: tiles_y * W + tiles_x * H u32.
+//   2. scan_aux: column sums scanned down the tile rows (exclusive, A[ty][x]
+//      = sum of column x above tile row ty), row sums scanned across the tile
+//      columns (exclusive, R[tx][y] = sum of row y left of tile column tx),
+//      tile totals scanned down the tile rows (exclusive, T[ty][tx]);
+//   3. emit: per tile, out(y, x) = C + sum of A[ty][X0 .. x] + sum over the
+//      tile's rows y' <= y of (R[tx][y'] + in-tile prefix of row y' up to
+//      x), with C = sum over tx' < tx of T[ty][tx'] (everything above and
+//      left of the tile).
+// Workspace: tiles_y * W + tiles_x * H + tiles_x * tiles_y u32.
 
 namespace {
 constexpr int kIntTile = 64;
@@ -797,7 +797,7 @@ __device__ __forceinline__ void int_load16(const uint8_t *__restrict__ src, int 
 template <bool SQ, bool VEC>
 __global__ __launch_bounds__(256) void integral_tile_sums_kernel(
     const uint8_t *__restrict__ src, int W, int H, int tiles_x, size_t ntiles,
-    uint32_t *__restrict__ colsum, uint32_t *__restrict__ rowsum) {
+    uint32_t *__restrict__ colsum, uint32_t *__restrict__ rowsum, uint32_t *__restrict__ tsum) {
     __shared__ uint4 tile4[kIntTile][kIntTile / 4 + 1];
     __shared__ uint32_t cpart[4][kIntTile];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -826,20 +826,27 @@ __global__ __launch_bounds__(256) void integral_tile_sums_kernel(
         for (int i = 0; i < 16; i++) cs += col[(wave * 16 + i) * (kIntTile + 4) + lane];
         cpart[wave][lane] = cs;
         __syncthreads();
-        if (wave == 0 && X0 + lane < W)
-            colsum[(size_t)ty * W + X0 + lane] =
-                cpart[0][lane] + cpart[1][lane] + cpart[2][lane] + cpart[3][lane];
+        if (wave == 0) {
+            const uint32_t c = cpart[0][lane] + cpart[1][lane] + cpart[2][lane] + cpart[3][lane];
+            if (X0 + lane < W) colsum[(size_t)ty * W + X0 + lane] = c;
+            uint32_t tot = c;  // columns past W hold zeros
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
+            if (lane == 0) tsum[t] = tot;
+        }
         __syncthreads();
     }
 }
 
 // Threads [0, W) scan colsum[.][x] down the tile rows, threads [W, W + H)
-// scan rowsum[.][y] across the tile columns; both exclusive and in place,
-// loads issued 32 at a time.
+// scan rowsum[.][y] across the tile columns, threads [W + H, W + H + tiles_x)
+// scan tsum[.][tx] down the tile rows; all exclusive and in place, loads
+// issued 32 at a time.
 __global__ __launch_bounds__(256) void integral_scan_aux_kernel(uint32_t *__restrict__ colsum,
                                                                 int W, int tiles_y,
                                                                 uint32_t *__restrict__ rowsum,
-                                                                int H, int tiles_x) {
+                                                                int H, int tiles_x,
+                                                                uint32_t *__restrict__ tsum) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     uint32_t *p;
     size_t stride;
@@ -852,6 +859,10 @@ __global__ __launch_bounds__(256) void integral_scan_aux_kernel(uint32_t *__rest
         p = rowsum + (i - W);
         stride = (size_t)H;
         n = tiles_x;
+    } else if (i < (size_t)W + H + tiles_x) {
+        p = tsum + (i - W - H);
+        stride = (size_t)tiles_x;
+        n = tiles_y;
     } else {
         return;
     }
@@ -869,50 +880,16 @@ __global__ __launch_bounds__(256) void integral_scan_aux_kernel(uint32_t *__rest
     }
 }
 
-// One workgroup per tile row: inclusive scan of colsum[ty][0..W) in place,
-// 4096 columns per pass (16 consecutive per thread, all loads in flight).
-__global__ __launch_bounds__(256) void integral_scan_pin_kernel(uint32_t *__restrict__ colsum,
-                                                                int W, int tiles_y) {
-    __shared__ uint32_t part[4];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int ty = blockIdx.x; ty < tiles_y; ty += gridDim.x) {
-        uint32_t *row = colsum + (size_t)ty * W;
-        uint32_t carry = 0;
-        for (int x0 = 0; x0 < W; x0 += 256 * 16) {
-            const int xb = x0 + tid * 16;
-            uint32_t v[16];
-#pragma unroll
-            for (int j = 0; j < 16; j++) v[j] = xb + j < W ? row[xb + j] : 0u;
-#pragma unroll
-            for (int j = 1; j < 16; j++) v[j] += v[j - 1];
-            uint32_t inc = v[15];
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t t = __shfl_up(inc, o);
-                if (lane >= o) inc += t;
-            }
-            if (lane == 63) part[wave] = inc;
-            __syncthreads();
-            uint32_t before = carry + inc - v[15];
-            for (int w2 = 0; w2 < wave; w2++) before += part[w2];
-#pragma unroll
-            for (int j = 0; j < 16; j++)
-                if (xb + j < W) row[xb + j] = before + v[j];
-            carry += part[0] + part[1] + part[2] + part[3];
-            __syncthreads();
-        }
-    }
-}
-
 // V4OUT: `out` is 16-byte aligned, so every output row is written as aligned
 // 4-element groups from LDS plus at most three elements at each end.
 template <bool SQ, bool VEC, bool V4OUT>
 __global__ __launch_bounds__(256) void integral_emit_kernel(
     const uint8_t *__restrict__ src, int W, int H, int tiles_x, size_t ntiles,
-    const uint32_t *__restrict__ pin, const uint32_t *__restrict__ rleft,
-    uint32_t *__restrict__ out) {
+    const uint32_t *__restrict__ above, const uint32_t *__restrict__ rleft,
+    const uint32_t *__restrict__ tabove, uint32_t *__restrict__ out) {
     __shared__ uint4 tile4[kIntTile][kIntTile / 4 + 1];
     __shared__ uint32_t vpart[4][kIntTile];
+    __shared__ uint32_t cpart[4];
     uint32_t *tile = reinterpret_cast<uint32_t *>(&tile4[0][0]);
     constexpr int P = kIntTile + 4;  // row pitch of `tile` in elements
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -929,6 +906,12 @@ __global__ __launch_bounds__(256) void integral_emit_kernel(
         // the part of the row left of the tile
         uint32_t v[16];
         int_load16<SQ, VEC>(src, W, H, X0, Y0 + r, q, v);
+        // everything above the tile row in columns X0 .. X0 + lane: C (the
+        // tile totals above-left, summed by the workgroup) plus the prefix of
+        // A[ty][X0 ..] across the lanes
+        uint32_t corner = 0;
+        for (int k = threadIdx.x; k < tx; k += 256) corner += tabove[(size_t)ty * tiles_x + k];
+        uint32_t col_above = lane < cw ? above[(size_t)ty * W + X0 + lane] : 0u;
 #pragma unroll
         for (int j = 1; j < 16; j++) v[j] += v[j - 1];
         uint32_t a = v[15];
@@ -951,7 +934,16 @@ __global__ __launch_bounds__(256) void integral_emit_kernel(
         }
         vpart[wave][lane] = run;
         __syncthreads();
-        uint32_t base = lane < cw ? pin[(size_t)ty * W + X0 + lane] : 0u;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) corner += __shfl_xor(corner, o);
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t u2 = __shfl_up(col_above, o);
+            if (lane >= o) col_above += u2;
+        }
+        if (lane == 0) cpart[wave] = corner;
+        __syncthreads();
+        uint32_t base = cpart[0] + cpart[1] + cpart[2] + cpart[3] + col_above;
         for (int w2 = 0; w2 < wave; w2++) base += vpart[w2][lane];
         if (!V4OUT) {
             if (lane < cw) {
@@ -1031,7 +1023,7 @@ __global__ __launch_bounds__(256) void adaptive_threshold_kernel(const uint8_t *
 }  // namespace
 
 size_t integral_workspace(uint32_t w, uint32_t h) {
-    const size_t n = int_tiles(h) * w + int_tiles(w) * h;
+    const size_t n = int_tiles(h) * w + int_tiles(w) * h + int_tiles(w) * int_tiles(h);
     return (n * sizeof(uint32_t) + 255) & ~(size_t)255;
 }
 
@@ -1052,6 +1044,7 @@ iph_status launch_integral_image(const uint8_t *src, uint32_t w, uint32_t h, int
     const int tx = (int)int_tiles(w), ty = (int)int_tiles(h);
     uint32_t *colsum = static_cast<uint32_t *>(ws);
     uint32_t *rowsum = colsum + (size_t)ty * w;
+    uint32_t *tsum = rowsum + (size_t)tx * h;
     const size_t ntiles = (size_t)tx * ty;
     // one workgroup per tile up to the cap (a multiple of 8 keeps the XCD bands)
     const unsigned grid = (unsigned)std::min<size_t>(8 * ((ntiles + 7) / 8), 1u << 20);
@@ -1059,7 +1052,8 @@ iph_status launch_integral_image(const uint8_t *src, uint32_t w, uint32_t h, int
     const bool v4out = (reinterpret_cast<uintptr_t>(out) & 15) == 0;
     const int W = (int)w, H = (int)h;
 #define IPH_INT_SUMS(SQ, VEC)                                                                  \
-    integral_tile_sums_kernel<SQ, VEC><<<grid, 256, 0, s>>>(src, W, H, tx, ntiles, colsum, rowsum)
+    integral_tile_sums_kernel<SQ, VEC><<<grid, 256, 0, s>>>(src, W, H, tx, ntiles, colsum, rowsum, \
+                                                            tsum)
     if (squared) {
         if (vec) IPH_INT_SUMS(true, true); else IPH_INT_SUMS(true, false);
     } else {
@@ -1067,14 +1061,12 @@ iph_status launch_integral_image(const uint8_t *src, uint32_t w, uint32_t h, int
     }
 #undef IPH_INT_SUMS
     IPH_TRY(launch_status("integral_tile_sums_kernel"));
-    integral_scan_aux_kernel<<<(unsigned)(((size_t)w + h + 255) / 256), 256, 0, s>>>(
-        colsum, W, ty, rowsum, H, tx);
+    integral_scan_aux_kernel<<<(unsigned)(((size_t)w + h + tx + 255) / 256), 256, 0, s>>>(
+        colsum, W, ty, rowsum, H, tx, tsum);
     IPH_TRY(launch_status("integral_scan_aux_kernel"));
-    integral_scan_pin_kernel<<<(unsigned)std::min(ty, 1 << 20), 256, 0, s>>>(colsum, W, ty);
-    IPH_TRY(launch_status("integral_scan_pin_kernel"));
 #define IPH_INT_EMIT(SQ, VEC, V4)                                                               \
     integral_emit_kernel<SQ, VEC, V4><<<grid, 256, 0, s>>>(src, W, H, tx, ntiles, colsum, rowsum, \
-                                                          out)
+                                                          tsum, out)
     if (squared) {
         if (vec) {
             if (v4out) IPH_INT_EMIT(true, true, true); else IPH_INT_EMIT(true, true, false);
