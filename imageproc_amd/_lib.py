@@ -99,7 +99,7 @@ _SIGS = {
     "iph_dev_integral_image_workspace": (C.c_size_t, [C.c_uint32, C.c_uint32]),
     "iph_dev_integral_image": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32,
                                            C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
-    "iph_dev_adaptive_threshold_workspace": (C.c_size_t, [C.c_uint32, C.c_uint32]),
+    "iph_dev_adaptive_threshold_workspace": (C.c_size_t, [C.c_uint32, C.c_uint32, C.c_uint32]),
     "iph_dev_adaptive_threshold": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
                                                C.c_uint32, C.c_int32, C.c_void_p, C.c_size_t,
                                                C.c_void_p]),

@@ -298,7 +298,7 @@ IPH_API iph_status iph_adaptive_threshold(const uint8_t *src, uint8_t *dst, uint
     const size_t n = (size_t)w * h;
     if (n == 0) return IPH_OK;
     IPH_TRY(require_device());
-    const size_t wsb = adaptive_threshold_workspace(w, h);
+    const size_t wsb = adaptive_threshold_workspace(w, h, block_radius);
     DevBuf din, dout, dws;
     IPH_TRY(upload(din, src, n));
     IPH_TRY(dout.alloc(n));
@@ -322,8 +322,9 @@ IPH_API iph_status iph_dev_integral_image(const uint8_t *src, uint32_t w, uint32
                                  static_cast<hipStream_t>(stream));
 }
 
-IPH_API size_t iph_dev_adaptive_threshold_workspace(uint32_t w, uint32_t h) {
-    return adaptive_threshold_workspace(w, h);
+IPH_API size_t iph_dev_adaptive_threshold_workspace(uint32_t w, uint32_t h,
+                                                    uint32_t block_radius) {
+    return adaptive_threshold_workspace(w, h, block_radius);
 }
 
 IPH_API iph_status iph_dev_adaptive_threshold(const uint8_t *src, uint8_t *dst, uint32_t w,

@@ -1222,7 +1222,8 @@ __global__ __launch_bounds__(256) void adaptive_block_kernel(const uint8_t *__re
 }
 }  // namespace
 
-size_t adaptive_threshold_workspace(uint32_t w, uint32_t h) {
+size_t adaptive_threshold_workspace(uint32_t w, uint32_t h, uint32_t block_radius) {
+    if (std::min<uint32_t>(block_radius, std::max(w, h)) <= (uint32_t)kAdMaxR) return 0;
     const size_t table = (((size_t)w + 1) * ((size_t)h + 1) * sizeof(uint32_t) + 255) & ~(size_t)255;
     return table + integral_workspace(w, h);
 }
@@ -1251,9 +1252,9 @@ iph_status launch_adaptive_threshold(const uint8_t *src, uint32_t w, uint32_t h,
                 src, (int)w, (int)h, sh, delta, tiles_x, dst);
         return launch_status("adaptive_block_kernel");
     }
-    if (wsb < adaptive_threshold_workspace(w, h)) {
+    if (wsb < adaptive_threshold_workspace(w, h, block_radius)) {
         set_error("adaptive threshold workspace of %zu bytes is below the %zu needed", wsb,
-                  adaptive_threshold_workspace(w, h));
+                  adaptive_threshold_workspace(w, h, block_radius));
         return IPH_EINVAL;
     }
     const size_t table =

@@ -105,7 +105,7 @@ iph_status launch_blur_u8_mfma(const uint8_t *src, uint8_t *dst, uint32_t w, uin
 size_t integral_workspace(uint32_t w, uint32_t h);
 iph_status launch_integral_image(const uint8_t *src, uint32_t w, uint32_t h, int squared,
                                  uint32_t *out, void *ws, size_t wsb, hipStream_t s);
-size_t adaptive_threshold_workspace(uint32_t w, uint32_t h);
+size_t adaptive_threshold_workspace(uint32_t w, uint32_t h, uint32_t block_radius);
 iph_status launch_adaptive_threshold(const uint8_t *src, uint32_t w, uint32_t h,
                                      uint32_t block_radius, int32_t delta, uint8_t *dst, void *ws,
                                      size_t wsb, hipStream_t s);

@@ -155,7 +155,7 @@ def adaptive_threshold(image: torch.Tensor, block_radius: int, delta: int,
     h, w = image.shape
     if out is None:
         out = torch.empty_like(image)
-    wsb = lib().iph_dev_adaptive_threshold_workspace(w, h)
+    wsb = lib().iph_dev_adaptive_threshold_workspace(w, h, int(block_radius))
     ws = _workspace("adaptive", wsb, image.device)
     check(lib().iph_dev_adaptive_threshold(image.data_ptr(), out.data_ptr(), w, h,
                                            int(block_radius), int(delta), ws.data_ptr(),

@@ -226,7 +226,10 @@ IPH_API size_t iph_dev_integral_image_workspace(uint32_t width, uint32_t height)
 IPH_API iph_status iph_dev_integral_image(const uint8_t *src, uint32_t width, uint32_t height,
                                           int32_t squared, uint32_t *out, void *workspace,
                                           size_t workspace_bytes, void *stream);
-IPH_API size_t iph_dev_adaptive_threshold_workspace(uint32_t width, uint32_t height);
+/* 0 for radii up to 64 (block sums in LDS); larger radii need an integral
+ * table of (w+1) x (h+1) u32 plus its scratch. */
+IPH_API size_t iph_dev_adaptive_threshold_workspace(uint32_t width, uint32_t height,
+                                                    uint32_t block_radius);
 IPH_API iph_status iph_dev_adaptive_threshold(const uint8_t *src, uint8_t *dst, uint32_t width,
                                               uint32_t height, uint32_t block_radius,
                                               int32_t delta, void *workspace,

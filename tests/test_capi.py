@@ -162,7 +162,10 @@ def test_adaptive_threshold_argument_errors_before_device():
     with pytest.raises(TypeError):
         adaptive_threshold(img.astype(np.uint16), 1, 0)
     assert _lib.lib().iph_dev_integral_image_workspace(100, 70) >= 4 * (2 * 100 + 2 * 70)
-    assert _lib.lib().iph_dev_adaptive_threshold_workspace(100, 70) >= 4 * 101 * 71
+    # radii up to 64 (after clipping to the larger side) need no table
+    assert _lib.lib().iph_dev_adaptive_threshold_workspace(60, 50, 1000) == 0
+    assert _lib.lib().iph_dev_adaptive_threshold_workspace(100, 70, 64) == 0
+    assert _lib.lib().iph_dev_adaptive_threshold_workspace(100, 70, 65) >= 4 * 101 * 71
 
 
 def test_canny_argument_errors_before_device():
