@@ -42,7 +42,10 @@ def test_noise_vs_oracle(shape, low, high):
 
 @pytest.mark.parametrize("seed,shape,sigma,low,high", [(1, (300, 411), 3.0, 5.0, 20.0),
                                                        (2, (517, 389), 6.0, 2.0, 6.0),
-                                                       (3, (1024, 1536), 12.0, 1.0, 3.0)])
+                                                       (3, (1024, 1536), 12.0, 1.0, 3.0),
+                                                       (4, (130, 208), 2.0, 3.0, 9.0),
+                                                       (5, (333, 1040), 4.0, 2.0, 6.0),
+                                                       (6, (96, 64), 1.5, 4.0, 12.0)])
 def test_smooth_vs_oracle(seed, shape, sigma, low, high):
     print("canny smooth", seed, shape, flush=True)
     img = _smooth(seed, shape, sigma)

@@ -136,7 +136,8 @@ def test_adaptive_vs_oracle(shape, r, delta):
 
 @pytest.mark.parametrize("shape,r", [((200, 256), 2), ((131, 300), 4), ((300, 517), 63),
                                      ((129, 130), 64), ((150, 160), 65), ((64, 64), 64),
-                                     ((70, 1), 3), ((1, 70), 3), ((1024, 1024), 41)])
+                                     ((70, 1), 3), ((1, 70), 3), ((1024, 1024), 41),
+                                     ((100, 208), 5), ((77, 48), 30)])
 def test_adaptive_radii_around_the_block_path(shape, r):
     """Radii up to 64 take the block-sum kernel, larger ones the integral
     table; both against the oracle, through the host and the device entry."""
