@@ -317,6 +317,11 @@ IPH_API iph_status iph_dev_integral_image(const uint8_t *src, uint32_t w, uint32
                                           void *stream) {
     clear_error();
     IPH_TRY(check_integral(w, h));
+    const bool empty = (size_t)w * h == 0;
+    if (out == nullptr || (!empty && (src == nullptr || ws == nullptr))) {
+        set_error("null image, table or workspace");
+        return IPH_EINVAL;
+    }
     IPH_TRY(require_device());
     return launch_integral_image(src, w, h, squared ? 1 : 0, out, ws, wsb,
                                  static_cast<hipStream_t>(stream));
@@ -332,6 +337,12 @@ IPH_API iph_status iph_dev_adaptive_threshold(const uint8_t *src, uint8_t *dst, 
                                               void *ws, size_t wsb, void *stream) {
     clear_error();
     IPH_TRY(check_adaptive(w, h, block_radius));
+    if ((size_t)w * h == 0) return IPH_OK;
+    if (src == nullptr || dst == nullptr ||
+        (ws == nullptr && adaptive_threshold_workspace(w, h, block_radius) > 0)) {
+        set_error("null image, output or workspace");
+        return IPH_EINVAL;
+    }
     IPH_TRY(require_device());
     return launch_adaptive_threshold(src, w, h, block_radius, delta, dst, ws, wsb,
                                      static_cast<hipStream_t>(stream));
@@ -372,6 +383,10 @@ IPH_API iph_status iph_dev_canny(const uint8_t *src, uint8_t *dst, uint32_t w, u
                                  float low, float high, void *ws, size_t wsb, void *stream) {
     clear_error();
     IPH_TRY(check_canny(w, h, low, high));
+    if (src == nullptr || dst == nullptr || ws == nullptr) {
+        set_error("null image, output or workspace");
+        return IPH_EINVAL;
+    }
     IPH_TRY(require_device());
     return launch_canny(src, dst, w, h, low, high, ws, wsb, static_cast<hipStream_t>(stream));
 }

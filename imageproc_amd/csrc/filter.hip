@@ -1121,7 +1121,7 @@ __global__ __launch_bounds__(256) void adaptive_block_kernel(const uint8_t *__re
                                                              uint8_t *__restrict__ dst) {
     extern __shared__ __align__(16) unsigned char ad_smem[];
     uint8_t *S = ad_smem;                                               // sr x sw bytes
-    uint16_t *V = reinterpret_cast<uint16_t *>(ad_smem + (size_t)a.sr * a.sw);  // 64 x sw
+    uint16_t *V = reinterpret_cast<uint16_t *>(ad_smem + (size_t)a.sr * a.sw);  // 64 x (sw + 2)
     const int r = a.r, sw = a.sw, sr = a.sr, off = a.off;
     const int X0 = (int)(blockIdx.x % (unsigned)tiles_x) * kAdT;
     const int Y0 = (int)(blockIdx.x / (unsigned)tiles_x) * kAdT;

@@ -204,3 +204,14 @@ def test_sum_image_pixels_and_variance_on_tables():
     big = np.full((300, 400), 255, np.uint8)
     sq = O.integral_image(big, True)
     assert sum_image_pixels(sq, 10, 20, 41, 51) == 32 * 32 * 255 * 255
+
+
+def test_device_entries_reject_null_buffers_before_device():
+    from imageproc_amd._lib import Panic, check, lib
+
+    with pytest.raises(Panic, match="null"):
+        check(lib().iph_dev_integral_image(None, 4, 4, 0, None, None, 0, None))
+    with pytest.raises(Panic, match="null"):
+        check(lib().iph_dev_adaptive_threshold(None, None, 4, 4, 1, 0, None, 0, None))
+    with pytest.raises(Panic, match="null"):
+        check(lib().iph_dev_canny(None, None, 4, 4, 1.0, 2.0, None, 0, None))
