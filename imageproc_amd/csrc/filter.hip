@@ -1090,8 +1090,10 @@ iph_status launch_integral_image(const uint8_t *src, uint32_t w, uint32_t h, int
 // around it; rows and columns outside the image staged as zeros, which add
 // nothing to a clipped block) is staged in LDS; vertical window sums of 2r+1
 // rows are slid down each column four columns at a time (two u16 halves per
-// u32: a sum of at most 129 bytes stays below 2^16), then horizontal window
-// sums of 2r+1 of those are slid along each row. The block sums are exact
+// u32: a sum of at most 129 bytes stays below 2^16) into registers, then
+// written over the staged window (so the workgroup's LDS is the larger of the
+// two, not their sum); horizontal window sums of 2r+1 of those are slid along
+// each row. The block sums are exact
 // (below 2^32), equal to the reference's wrapping four-corner sums, and the
 // mean, threshold and comparison follow contrast.rs:19-47.
 namespace {
@@ -1102,7 +1104,7 @@ struct AdShape {
     int r, off, sw, sr;  // radius; (X0 - r) - xs; staged columns (multiple of 4); staged rows
     // V rows are sw + 2 u16 apart: an odd dword pitch, so the 16 rows one
     // wave reads at a time fall in distinct banks
-    size_t lds() const { return (size_t)sw * sr + (size_t)kAdT * (sw + 2) * 2; }
+    size_t lds() const { return std::max((size_t)sw * sr, (size_t)kAdT * (sw + 2) * 2); }
 };
 
 AdShape ad_shape(int r) {
@@ -1121,7 +1123,7 @@ __global__ __launch_bounds__(256) void adaptive_block_kernel(const uint8_t *__re
                                                              uint8_t *__restrict__ dst) {
     extern __shared__ __align__(16) unsigned char ad_smem[];
     uint8_t *S = ad_smem;                                               // sr x sw bytes
-    uint16_t *V = reinterpret_cast<uint16_t *>(ad_smem + (size_t)a.sr * a.sw);  // 64 x (sw + 2)
+    uint16_t *V = reinterpret_cast<uint16_t *>(ad_smem);  // 64 x (sw + 2), after S is read
     const int r = a.r, sw = a.sw, sr = a.sr, off = a.off;
     const int X0 = (int)(blockIdx.x % (unsigned)tiles_x) * kAdT;
     const int Y0 = (int)(blockIdx.x / (unsigned)tiles_x) * kAdT;
@@ -1156,9 +1158,12 @@ __global__ __launch_bounds__(256) void adaptive_block_kernel(const uint8_t *__re
         }
     }
     __syncthreads();
-    // vertical sums: item (quad, 16-row segment)
-    for (int i = threadIdx.x; i < quads * 4; i += 256) {
-        const int q = i % quads, seg = i / quads;
+    // vertical sums: item (quad, 16-row segment), at most one per thread
+    // (quads <= 49), kept in registers until every item has read S
+    const int vi = threadIdx.x, vq = vi % quads, vseg = vi / quads;
+    uint32_t vlo[16], vhi[16];
+    if (vi < quads * 4) {
+        const int q = vq, seg = vseg;
         const uint32_t *col = reinterpret_cast<const uint32_t *>(S) + q;
         const int qs = sw >> 2;  // dwords per staged row
         uint32_t lo = 0, hi = 0;  // columns (0, 2) and (1, 3) of the quad
@@ -1170,29 +1175,62 @@ __global__ __launch_bounds__(256) void adaptive_block_kernel(const uint8_t *__re
             hi += (w >> 8) & 0x00ff00ffu;
         }
 #pragma unroll
-        for (int yy = y0; yy < y0 + 16; yy++) {
-            uint32_t *vr = reinterpret_cast<uint32_t *>(V + (size_t)yy * (sw + 2) + 4 * q);
-            vr[0] = (lo & 0xffffu) | (hi << 16);
-            vr[1] = (lo >> 16) | (hi & 0xffff0000u);
-            if (yy + 1 < y0 + 16) {
-                const uint32_t add = col[(yy + 2 * r + 1) * qs], sub = col[yy * qs];
+        for (int u = 0; u < 16; u++) {
+            vlo[u] = lo;
+            vhi[u] = hi;
+            if (u < 15) {
+                const uint32_t add = col[(y0 + u + 2 * r + 1) * qs], sub = col[(y0 + u) * qs];
                 lo += (add & 0x00ff00ffu) - (sub & 0x00ff00ffu);
                 hi += ((add >> 8) & 0x00ff00ffu) - ((sub >> 8) & 0x00ff00ffu);
             }
         }
     }
+    // the tile's own pixels for the comparison, from global memory (S is
+    // about to be overwritten): thread (row, 16-column segment)
+    const int yy = threadIdx.x >> 2, seg = threadIdx.x & 3;
+    const int y = Y0 + yy, x0 = X0 + 16 * seg;
+    uint32_t pix[4] = {0u, 0u, 0u, 0u};
+    if (y < H) {
+        const uint8_t *srow = src + (size_t)y * W;
+        if (VEC && x0 + 16 <= W) {
+            const uint4 p4 = *reinterpret_cast<const uint4 *>(srow + x0);
+            pix[0] = p4.x;
+            pix[1] = p4.y;
+            pix[2] = p4.z;
+            pix[3] = p4.w;
+        } else {
+#pragma unroll
+            for (int u = 0; u < 16; u++)
+                if (x0 + u < W) pix[u >> 2] |= (uint32_t)srow[x0 + u] << (8 * (u & 3));
+        }
+    }
+    __syncthreads();  // every vertical item has read S
+    if (vi < quads * 4) {
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            uint32_t *vr = reinterpret_cast<uint32_t *>(V + (size_t)(16 * vseg + u) * (sw + 2) + 4 * vq);
+            vr[0] = (vlo[u] & 0xffffu) | (vhi[u] << 16);
+            vr[1] = (vlo[u] >> 16) | (vhi[u] & 0xffff0000u);
+        }
+    }
     __syncthreads();
     // horizontal sums, mean and comparison: item (row, 16-column segment)
-    const int yy = threadIdx.x >> 2, seg = threadIdx.x & 3;
-    const int y = Y0 + yy;
     if (y >= H) return;
     const uint16_t *vr = V + (size_t)yy * (sw + 2);
     const int j0 = 16 * seg + off;  // staged column of the first window's left edge
+    // the first window's sum, two u16 per LDS read: pairs cover [j0 & ~1,
+    // (j0 + 2r) | 1], then the one or two values outside the window come off
+    const int ja = j0 & ~1, jb = (j0 + 2 * r) | 1;
+    const uint32_t *vp = reinterpret_cast<const uint32_t *>(vr + ja);  // row pitch is even
     uint32_t sum = 0;
 #pragma unroll 8
-    for (int j = j0; j <= j0 + 2 * r; j++) sum += vr[j];
+    for (int k = 0; k <= (jb - ja) >> 1; k++) {
+        const uint32_t d = vp[k];
+        sum += (d & 0xffffu) + (d >> 16);
+    }
+    if (ja < j0) sum -= vr[ja];
+    if (jb > j0 + 2 * r) sum -= vr[jb];
     const uint32_t rows = (uint32_t)(min(H - 1, y + r) - max(0, y - r) + 1);
-    const uint8_t *px = S + (size_t)(yy + r) * sw + off + r;
     uint32_t packed[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
     for (int u = 0; u < 16; u++) {
@@ -1206,11 +1244,11 @@ __global__ __launch_bounds__(256) void adaptive_block_kernel(const uint8_t *__re
         mean -= mean * n > sum ? 1u : 0u;
         mean += (mean + 1) * n <= sum ? 1u : 0u;
         const int32_t thr = (int32_t)(mean - (uint32_t)delta);
-        const uint32_t o = (int32_t)px[xl] >= thr ? 255u : 0u;
+        const int32_t p = (int32_t)((pix[u >> 2] >> (8 * (u & 3))) & 0xffu);
+        const uint32_t o = p >= thr ? 255u : 0u;
         packed[u >> 2] |= o << (8 * (u & 3));
         if (u < 15) sum += (uint32_t)vr[j0 + u + 2 * r + 1] - (uint32_t)vr[j0 + u];
     }
-    const int x0 = X0 + 16 * seg;
     uint8_t *orow = dst + (size_t)y * W;
     if (VEC && x0 + 16 <= W) {
         *reinterpret_cast<uint4 *>(orow + x0) = make_uint4(packed[0], packed[1], packed[2], packed[3]);
