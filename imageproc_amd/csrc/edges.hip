@@ -42,22 +42,21 @@ constexpr int kHystT = 64;  // hysteresis tile side
 constexpr int kCannyIters = 4;  // hysteresis launches between flag checks
 constexpr int kCannyGrid = 1 << 20;  // workgroup cap of the tile kernels
 
+// Branch-free (selects only), so the lanes of a wave never diverge on it.
 __device__ __forceinline__ int canny_direction(int gx, int gy) {
-    if (gx == -985 && gy == -408) return 45;
-    if (gy == 0) return 0;
-    const int x = gy < 0 ? -gx : gx, y = gy < 0 ? -gy : gy;
-    if (x == 0) return 90;
+    const bool neg = gy < 0;
+    const int x = neg ? -gx : gx, y = neg ? -gy : gy;
     const int ax = x < 0 ? -x : x;
     const int lo = (ax + y) * (ax + y) - 2 * ax * ax;  // > 0 <=> y > ax * tan 22.5
     const int d = y - ax;
     const bool below_t2 = d <= 0 || d * d < 2 * ax * ax;  // y < ax * tan 67.5
-    if (x > 0) {
-        if (lo < 0) return 0;
-        return below_t2 ? 45 : 90;
-    }
-    // x < 0: the angle is 180 - atan(y / ax)
-    if (!below_t2) return 90;
-    return lo > 0 ? 135 : 0;
+    // x > 0: the angle is atan(y / ax); x < 0: 180 - atan(y / ax)
+    const int right = lo < 0 ? 0 : (below_t2 ? 45 : 90);
+    const int left = below_t2 ? (lo > 0 ? 135 : 0) : 90;
+    int cls = x > 0 ? right : left;
+    cls = x == 0 ? 90 : cls;
+    cls = gy == 0 ? 0 : cls;
+    return (gx == -985 && gy == -408) ? 45 : cls;
 }
 
 // Staged blurred window: image rows Y0 - 2 .. Y0 + kCannyTH + 1, columns
@@ -138,13 +137,13 @@ __global__ __launch_bounds__(256) void canny_nms_kernel(const uint8_t *__restric
                     const int p = gxy[r][lx];
                     const int gx = (int)(int16_t)(p & 0xffff), gy = p >> 16;
                     const int cy = r + 1, cx = lx + 1;
-                    int c1, c2;
-                    switch (canny_direction(gx, gy)) {
-                    case 0: c1 = nn[cy][cx - 1]; c2 = nn[cy][cx + 1]; break;
-                    case 45: c1 = nn[cy + 1][cx + 1]; c2 = nn[cy - 1][cx - 1]; break;
-                    case 90: c1 = nn[cy - 1][cx]; c2 = nn[cy + 1][cx]; break;
-                    default: c1 = nn[cy + 1][cx - 1]; c2 = nn[cy - 1][cx + 1]; break;
-                    }
+                    // the two neighbours across the edge (edges.rs:84-99) sit
+                    // at +(dy, dx) and -(dy, dx): 0 -> (0, -1), 45 -> (1, 1),
+                    // 90 -> (-1, 0), 135 -> (1, -1); selected without branches
+                    const int dir = canny_direction(gx, gy);
+                    const int dy = dir == 0 ? 0 : (dir == 90 ? -1 : 1);
+                    const int dx = dir == 45 ? 1 : (dir == 90 ? 0 : -1);
+                    const int c1 = nn[cy + dy][cx + dx], c2 = nn[cy - dy][cx - dx];
                     const int m = nn[cy][cx];
                     thin = (m < c1 || m < c2) ? 0 : m;
                 }
