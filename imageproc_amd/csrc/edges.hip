@@ -63,17 +63,6 @@ __device__ __forceinline__ int canny_direction(int gx, int gy) {
 // X0 - 4 .. X0 + kCannyTW + 3 (18 dwords, so interior tiles load whole
 // aligned dwords), clamped to the image like filter_clamped's reads.
 constexpr int kCannyPW = kCannyTW + 8;
-using CannyPx = uint8_t[kCannyPW];
-
-// Sobel responses with taps px[ly .. ly + 2][c .. c + 2] (filter/mod.rs:29-69
-// order; the i32 sums are exact and within i16 for u8 input).
-__device__ __forceinline__ void canny_sobel(const CannyPx *px, int ly, int c, int &gx, int &gy) {
-    const int a = px[ly][c], b = px[ly][c + 1], cc = px[ly][c + 2];
-    const int d = px[ly + 1][c], f = px[ly + 1][c + 2];
-    const int g = px[ly + 2][c], h = px[ly + 2][c + 1], k = px[ly + 2][c + 2];
-    gx = (cc - a) + 2 * (f - d) + (k - g);  // SOBEL_HORIZONTAL_3X3
-    gy = (g - a) + 2 * (h - b) + (k - cc);  // SOBEL_VERTICAL_3X3
-}
 
 // Magnitudes are compared through n = gx^2 + gy^2: sqrtf is strictly
 // increasing on the integers up to 2^21 (neighbouring square roots there are
@@ -112,14 +101,31 @@ __global__ __launch_bounds__(256) void canny_nms_kernel(const uint8_t *__restric
         }
         __syncthreads();
         // ring position (ly, lx) <-> image (Y0 - 1 + ly, X0 - 1 + lx); its taps
-        // start at staged column lx + 2
-        for (int i = threadIdx.x; i < (kCannyTH + 2) * (kCannyTW + 2); i += 256) {
-            const int ly = i / (kCannyTW + 2), lx = i % (kCannyTW + 2);
-            int gx, gy;
-            canny_sobel(px, ly, lx + 2, gx, gy);
-            nn[ly][lx] = gx * gx + gy * gy;
-            if (ly >= 1 && ly <= kCannyTH && lx >= 1 && lx <= kCannyTW)
-                gxy[ly - 1][lx - 1] = (int)(((uint32_t)gx & 0xffffu) | ((uint32_t)gy << 16));
+        // are px[ly .. ly + 2][lx + 2 .. lx + 4] (filter_clamped::<_, i32, i16>
+        // of filter/mod.rs:29-69: the i32 sums are exact and within i16 for u8
+        // input, so their order is free). An item is three positions of one
+        // column, whose taps share rows: 15 LDS reads instead of 27.
+        constexpr int kStrips = (kCannyTH + 2) / 3;
+        static_assert((kCannyTH + 2) % 3 == 0, "ring rows come in strips of three");
+        for (int i = threadIdx.x; i < kStrips * (kCannyTW + 2); i += 256) {
+            const int st = i / (kCannyTW + 2), lx = i - st * (kCannyTW + 2), ly0 = 3 * st;
+            int p[5][3];
+#pragma unroll
+            for (int k = 0; k < 5; k++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) p[k][c] = px[ly0 + k][lx + 2 + c];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const int a = p[k][0], b = p[k][1], cc = p[k][2];
+                const int d = p[k + 1][0], f = p[k + 1][2];
+                const int g = p[k + 2][0], h = p[k + 2][1], kk = p[k + 2][2];
+                const int gx = (cc - a) + 2 * (f - d) + (kk - g);  // SOBEL_HORIZONTAL_3X3
+                const int gy = (g - a) + 2 * (h - b) + (kk - cc);  // SOBEL_VERTICAL_3X3
+                const int ly = ly0 + k;
+                nn[ly][lx] = gx * gx + gy * gy;
+                if (ly >= 1 && ly <= kCannyTH && lx >= 1 && lx <= kCannyTW)
+                    gxy[ly - 1][lx - 1] = (int)(((uint32_t)gx & 0xffffu) | ((uint32_t)gy << 16));
+            }
         }
         __syncthreads();
         // thread: row r, columns c0 .. c0 + 3
