@@ -47,9 +47,11 @@ __device__ __forceinline__ int canny_direction(int gx, int gy) {
     const bool neg = gy < 0;
     const int x = neg ? -gx : gx, y = neg ? -gy : gy;
     const int ax = x < 0 ? -x : x;
-    const int lo = (ax + y) * (ax + y) - 2 * ax * ax;  // > 0 <=> y > ax * tan 22.5
+    // operands below 2^11: 24-bit multiplies (full rate) are exact
+    const int ax2 = 2 * __mul24(ax, ax);
+    const int lo = __mul24(ax + y, ax + y) - ax2;  // > 0 <=> y > ax * tan 22.5
     const int d = y - ax;
-    const bool below_t2 = d <= 0 || d * d < 2 * ax * ax;  // y < ax * tan 67.5
+    const bool below_t2 = d <= 0 || __mul24(d, d) < ax2;  // y < ax * tan 67.5
     // x > 0: the angle is atan(y / ax); x < 0: 180 - atan(y / ax)
     const int right = lo < 0 ? 0 : (below_t2 ? 45 : 90);
     const int left = below_t2 ? (lo > 0 ? 135 : 0) : 90;
@@ -122,7 +124,7 @@ __global__ __launch_bounds__(256) void canny_nms_kernel(const uint8_t *__restric
                 const int gx = (cc - a) + 2 * (f - d) + (kk - g);  // SOBEL_HORIZONTAL_3X3
                 const int gy = (g - a) + 2 * (h - b) + (kk - cc);  // SOBEL_VERTICAL_3X3
                 const int ly = ly0 + k;
-                nn[ly][lx] = gx * gx + gy * gy;
+                nn[ly][lx] = __mul24(gx, gx) + __mul24(gy, gy);
                 if (ly >= 1 && ly <= kCannyTH && lx >= 1 && lx <= kCannyTW)
                     gxy[ly - 1][lx - 1] = (int)(((uint32_t)gx & 0xffffu) | ((uint32_t)gy << 16));
             }

@@ -1129,20 +1129,21 @@ __global__ __launch_bounds__(256) void adaptive_block_kernel(const uint8_t *__re
     const int Y0 = (int)(blockIdx.x / (unsigned)tiles_x) * kAdT;
     const int xs = X0 - r - off, ys = Y0 - r;  // image position of S[0][0] (xs % 4 == 0)
     const int quads = sw >> 2;
-    // stage the window, one dword (4 columns) per item, twelve loads in
-    // flight per thread before their LDS stores
-    const int nitems = sr * quads;
-    for (int i0 = 0; i0 < nitems; i0 += 256 * 12) {
+    // stage the window, one dword (4 columns) per item: thread (tr, tq) takes
+    // column quad tq of rows tr, tr + rpp, ... (one division per thread, not
+    // per item), twelve loads in flight before their LDS stores
+    const int rpp = 256 / quads, tq = (int)threadIdx.x % quads, tr = (int)threadIdx.x / quads;
+    const int x = xs + 4 * tq;
+    const bool xin = VEC && x >= 0 && x + 4 <= W;
+    for (int k0 = tr; k0 < sr; k0 += 12 * rpp) {
         uint32_t v[12];
 #pragma unroll
         for (int u = 0; u < 12; u++) {
-            const int i = i0 + 256 * u + (int)threadIdx.x;
-            const int k = i / quads, q = i - k * quads;
-            const int y = ys + k, x = xs + 4 * q;
+            const int k = k0 + u * rpp, y = ys + k;
             v[u] = 0;
-            if (i < nitems && y >= 0 && y < H) {
+            if (tr < rpp && k < sr && y >= 0 && y < H) {
                 const uint8_t *row = src + (size_t)y * W;
-                if (VEC && x >= 0 && x + 4 <= W) {
+                if (xin) {
                     v[u] = *reinterpret_cast<const uint32_t *>(row + x);
                 } else {
 #pragma unroll
@@ -1153,8 +1154,8 @@ __global__ __launch_bounds__(256) void adaptive_block_kernel(const uint8_t *__re
         }
 #pragma unroll
         for (int u = 0; u < 12; u++) {
-            const int i = i0 + 256 * u + (int)threadIdx.x;
-            if (i < nitems) reinterpret_cast<uint32_t *>(S)[i] = v[u];
+            const int k = k0 + u * rpp;
+            if (tr < rpp && k < sr) reinterpret_cast<uint32_t *>(S)[k * quads + tq] = v[u];
         }
     }
     __syncthreads();
@@ -1239,10 +1240,11 @@ __global__ __launch_bounds__(256) void adaptive_block_kernel(const uint8_t *__re
         // floor(sum / n): sum < 2^23 and n < 2^15 are exact in f32 and the
         // quotient (a mean of bytes) is at most 255, so the f32 estimate is
         // within 1e-4 of it; one correction step each way makes it exact
-        const uint32_t n = rows * cols;
+        // (24-bit multiplies: n <= 129^2 and mean <= 256)
+        const uint32_t n = __umul24(rows, cols);
         uint32_t mean = (uint32_t)((float)sum * __builtin_amdgcn_rcpf((float)n));
-        mean -= mean * n > sum ? 1u : 0u;
-        mean += (mean + 1) * n <= sum ? 1u : 0u;
+        mean -= __umul24(mean, n) > sum ? 1u : 0u;
+        mean += __umul24(mean + 1, n) <= sum ? 1u : 0u;
         const int32_t thr = (int32_t)(mean - (uint32_t)delta);
         const int32_t p = (int32_t)((pix[u >> 2] >> (8 * (u & 3))) & 0xffu);
         const uint32_t o = p >= thr ? 255u : 0u;
