@@ -294,7 +294,7 @@ def bench_copy(steps: int, warmup: int):
     return ms, 2.0 * src.numel() / (ms * 1e-3) / 1e9
 
 
-def bench_match(steps: int):
+def bench_match(steps: int, cpu: bool = False):
     import torch
 
     from imageproc_amd import device as dev
@@ -330,19 +330,35 @@ def bench_match(steps: int):
     out = dev.match_template_with_mask(gi, gt, M.CrossCorrelationNormalized, mask)
     times, _ = timed(lambda: dev.match_template_with_mask(gi, gt, M.CrossCorrelationNormalized,
                                                           mask, out=out), max(2, steps // 2), 1)
-    res["CrossCorrelationNormalized:masked_direct"] = {
-        "ms_per_image": round(float(np.median(times)), 4)}
+    masked_ms = float(np.median(times))
+    res["CrossCorrelationNormalized:masked_direct"] = {"ms_per_image": round(masked_ms, 4)}
+    if cpu:
+        import oracle as O
+
+        crop, t_np, m_np = np.ascontiguousarray(img[:512, :512]), tpl, mask.cpu().numpy()
+        t0 = time.perf_counter()
+        O.match_template_with_mask(crop, t_np, m_np, int(M.CrossCorrelationNormalized))
+        dt = time.perf_counter() - t0
+        cpu_wps = (512 - MATCH_T + 1) ** 2 / dt
+        gpu_wps = ow * ow / (masked_ms * 1e-3)
+        res["CrossCorrelationNormalized:masked_direct"]["cpu_baseline"] = {
+            "windows/s": round(cpu_wps, 1), "cores": 1, "kind": "port",
+            "sample": f"512x512 corner of the image, the same {MATCH_T}x{MATCH_T} template and "
+                      "mask, oracle/oracle.c, one call",
+            "gpu_over_cpu": round(gpu_wps / cpu_wps, 1)}
     return res
 
 
 CONSUMER_SIDE = 8192
 
 
-def bench_consumers(steps: int):
+def bench_consumers(steps: int, cpu: bool = True):
     """integral_image (u32 table), adaptive_threshold (radius 41) and canny
-    (thresholds 250, 300) of one 8192x8192 u8 image resident in HBM. Algorithmic bytes: integral 1 B read +
-    4 B written per pixel; adaptive threshold 1 B read + 1 B written per pixel
-    (its integral table is an intermediate)."""
+    (thresholds 250, 300) of one 8192x8192 u8 image resident in HBM.
+    Algorithmic bytes: integral 1 B read + 4 B written per pixel; adaptive
+    threshold and canny 1 B read + 1 B written per pixel. With cpu, each is
+    also timed on one host core with the oracle (the reference's loops in C)
+    on the image's top-left 2048x2048 quarter, once."""
     import torch
 
     import oracle as O
@@ -367,6 +383,20 @@ def bench_consumers(steps: int):
     ms = float(np.median(times))
     res["canny_250_300"] = {"ms": round(ms, 4),
                             "Gpix/s": round(side * side / (ms * 1e-3) / 1e9, 1)}
+    if cpu:
+        crop = img[:2048, :2048].cpu().numpy()
+        npx = crop.size
+        for key, fn in [("integral", lambda: O.integral_image(crop)),
+                        ("adaptive_threshold_r41", lambda: O.adaptive_threshold(crop, 41, 0)),
+                        ("canny_250_300", lambda: O.canny(crop, 250.0, 300.0))]:
+            t0 = time.perf_counter()
+            fn()
+            dt = time.perf_counter() - t0
+            gpu_pps = side * side / (res[key]["ms"] * 1e-3)
+            res[key]["cpu_baseline"] = {
+                "Mpix/s": round(npx / dt / 1e6, 1), "cores": 1, "kind": "port",
+                "sample": "2048x2048 quarter of the same image, oracle/oracle.c, one call",
+                "gpu_over_cpu": round(gpu_pps / (npx / dt), 1)}
     # spot check of a 1024x1024 corner against the oracle
     corner = img[:1024, :1024].contiguous()
     res["parity_spot_check"] = bool(
@@ -601,11 +631,11 @@ def main():
         return
     gpix = BLUR_N * BLUR_W * BLUR_H / (ms * 1e-3) / 1e9
     achieved = nbytes / (ms * 1e-3) / 1e9
-    match = bench_match(max(5, args.steps // 2))
+    match = bench_match(max(5, args.steps // 2), cpu=not args.no_cpu)
     precision = bench_precision(3)
     batch = bench_batch(4)
     proposal = bench_proposal()
-    consumers = bench_consumers(max(5, args.steps // 2))
+    consumers = bench_consumers(max(5, args.steps // 2), cpu=not args.no_cpu)
     cpu = None if args.no_cpu else cpu_blur_baseline()
     line = {
         "metric": "blur Gpix/s & %HBM peak; match_template ms/image at equal argmax vs plain GPU",
