@@ -353,10 +353,11 @@ CONSUMER_SIDE = 8192
 
 
 def bench_consumers(steps: int, cpu: bool = True):
-    """integral_image (u32 table), adaptive_threshold (radius 41) and canny
-    (thresholds 250, 300) of one 8192x8192 u8 image resident in HBM.
-    Algorithmic bytes: integral 1 B read + 4 B written per pixel; adaptive
-    threshold and canny 1 B read + 1 B written per pixel. With cpu, each is
+    """integral_image (u32 table), adaptive_threshold (radius 41), canny
+    (thresholds 250, 300), Sobel gradients (u16) and sharpen_gaussian (0.7,
+    7.0) of one 8192x8192 u8 image resident in HBM. Algorithmic bytes:
+    integral 1 B read + 4 B written per pixel; gradients 1 B + 2 B; adaptive
+    threshold, canny and sharpen 1 B read + 1 B written per pixel. With cpu, each is
     also timed on one host core with the oracle (the reference's loops in C)
     on the image's top-left 2048x2048 quarter, once."""
     import torch
@@ -383,19 +384,49 @@ def bench_consumers(steps: int, cpu: bool = True):
     ms = float(np.median(times))
     res["canny_250_300"] = {"ms": round(ms, 4),
                             "Gpix/s": round(side * side / (ms * 1e-3) / 1e9, 1)}
+    from imageproc_amd import gradients as G
+
+    grad = dev.gradients(img, G.SOBEL_HORIZONTAL_3X3, G.SOBEL_VERTICAL_3X3)
+    times, _ = timed(lambda: dev.gradients(img, G.SOBEL_HORIZONTAL_3X3, G.SOBEL_VERTICAL_3X3,
+                                           out=grad), steps, 2)
+    ms = float(np.median(times))
+    res["sobel_gradients"] = {"ms": round(ms, 4),
+                              "GB/s": round(3.0 * side * side / (ms * 1e-3) / 1e9, 1)}
+    sharp = dev.sharpen_gaussian(img, 0.7, 7.0)
+    times, _ = timed(lambda: dev.sharpen_gaussian(img, 0.7, 7.0, out=sharp), steps, 2)
+    ms = float(np.median(times))
+    res["sharpen_gaussian_0.7_7"] = {"ms": round(ms, 4),
+                                     "Gpix/s": round(side * side / (ms * 1e-3) / 1e9, 1)}
     if cpu:
         crop = img[:2048, :2048].cpu().numpy()
         npx = crop.size
+
+        def cpu_sobel():
+            kh = np.array(G.SOBEL_HORIZONTAL_3X3.data, np.int32)
+            kv = np.array(G.SOBEL_VERTICAL_3X3.data, np.int32)
+            dx = O.filter_clamped(crop, kh, 3, 3, np.int16).astype(np.float32)
+            dy = O.filter_clamped(crop, kv, 3, 3, np.int16).astype(np.float32)
+            return np.sqrt(dx * dx + dy * dy)
+
+        def cpu_sharpen():
+            r = crop.astype(np.float32)
+            a = np.float32(7.0)
+            v = (np.float32(1.0) + a) * r - a * O.gaussian_blur_f32(r, 0.7)
+            return np.where(v < 255, np.where(v > 0, v, 0), 255).astype(np.uint8)
+
         for key, fn in [("integral", lambda: O.integral_image(crop)),
                         ("adaptive_threshold_r41", lambda: O.adaptive_threshold(crop, 41, 0)),
-                        ("canny_250_300", lambda: O.canny(crop, 250.0, 300.0))]:
+                        ("canny_250_300", lambda: O.canny(crop, 250.0, 300.0)),
+                        ("sobel_gradients", cpu_sobel),
+                        ("sharpen_gaussian_0.7_7", cpu_sharpen)]:
             t0 = time.perf_counter()
             fn()
             dt = time.perf_counter() - t0
             gpu_pps = side * side / (res[key]["ms"] * 1e-3)
             res[key]["cpu_baseline"] = {
                 "Mpix/s": round(npx / dt / 1e6, 1), "cores": 1, "kind": "port",
-                "sample": "2048x2048 quarter of the same image, oracle/oracle.c, one call",
+                "sample": "2048x2048 quarter of the same image, oracle/oracle.c (numpy for the "
+                          "gradient magnitude and the sharpen combine), one call",
                 "gpu_over_cpu": round(gpu_pps / (npx / dt), 1)}
     # spot check of a 1024x1024 corner against the oracle
     corner = img[:1024, :1024].contiguous()

@@ -108,6 +108,13 @@ _SIGS = {
     "iph_dev_canny_workspace": (C.c_size_t, [C.c_uint32, C.c_uint32]),
     "iph_dev_canny": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float,
                                   C.c_float, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "iph_dev_gradients": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32,
+                                      C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
+                                      C.c_void_p]),
+    "iph_dev_sharpen_gaussian_workspace": (C.c_size_t, [C.c_uint32, C.c_uint32, C.c_float]),
+    "iph_dev_sharpen_gaussian": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
+                                             C.c_float, C.c_float, C.c_void_p, C.c_size_t,
+                                             C.c_void_p]),
     "iph_gradients": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32,
                                   C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]),
     "iph_sharpen3x3": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),

@@ -445,6 +445,55 @@ IPH_API iph_status iph_sharpen_gaussian(const uint8_t *src, uint8_t *dst, uint32
     return finish(dout, dst, bytes);
 }
 
+IPH_API iph_status iph_dev_gradients(const uint8_t *src, uint32_t w, uint32_t h, const int32_t *hk,
+                                     uint32_t hkw, uint32_t hkh, const int32_t *vk, uint32_t vkw,
+                                     uint32_t vkh, uint16_t *out, void *stream) {
+    clear_error();
+    if (hkw == 0 || hkh == 0 || vkw == 0 || vkh == 0) {  // Kernel::new, kernel.rs:24-35
+        set_error("width and height must be non-zero");
+        return IPH_EINVAL;
+    }
+    IPH_TRY(check_dims(w, h));
+    if ((size_t)w * h == 0) return IPH_OK;
+    if (src == nullptr || out == nullptr || hk == nullptr || vk == nullptr) {
+        set_error("null image, output or kernel");
+        return IPH_EINVAL;
+    }
+    IPH_TRY(require_device());
+    return launch_gradients(src, w, h, hk, hkw, hkh, vk, vkw, vkh, out,
+                            static_cast<hipStream_t>(stream));
+}
+
+IPH_API size_t iph_dev_sharpen_gaussian_workspace(uint32_t w, uint32_t h, float sigma) {
+    std::vector<float> k;
+    if (!(sigma > 0.0f) || gaussian_kernel(sigma, k) < 0) return 0;
+    return sharpen_gaussian_workspace(w, h, (uint32_t)k.size());
+}
+
+IPH_API iph_status iph_dev_sharpen_gaussian(const uint8_t *src, uint8_t *dst, uint32_t w,
+                                            uint32_t h, float sigma, float amount, void *ws,
+                                            size_t wsb, void *stream) {
+    clear_error();
+    if (!(sigma > 0.0f)) {  // gaussian_blur_f32's assert, filter/mod.rs:161
+        set_error("sigma must be > 0.0");
+        return IPH_EINVAL;
+    }
+    std::vector<float> k;
+    if (gaussian_kernel(sigma, k) < 0) {
+        set_error("sigma %g gives more than %u taps", (double)sigma, kMaxTaps);
+        return IPH_EINVAL;
+    }
+    IPH_TRY(check_dims(w, h));
+    if ((size_t)w * h == 0) return IPH_OK;
+    if (src == nullptr || dst == nullptr || ws == nullptr) {
+        set_error("null image, output or workspace");
+        return IPH_EINVAL;
+    }
+    IPH_TRY(require_device());
+    return launch_sharpen_gaussian(src, dst, w, h, k.data(), (uint32_t)k.size(), amount, ws, wsb,
+                                   static_cast<hipStream_t>(stream));
+}
+
 static iph_status filter1d_host(const void *src, void *dst, uint32_t w, uint32_t h, int32_t p,
                                 const void *k, uint32_t len, int32_t kt, bool vertical) {
     clear_error();

@@ -631,23 +631,48 @@ iph_status launch_filter_jobs(const void *table_dev, int n, int total_tiles, int
 
 namespace {
 // map_subpixels(image, |x| x as f32)
+// Four elements per thread; VEC: src and out 4-byte aligned, so the bytes
+// move as one dword (the f32 side is always workspace, 16-byte aligned).
+template <bool VEC>
 __global__ __launch_bounds__(256) void u8_to_f32_kernel(const uint8_t *__restrict__ src,
                                                         float *__restrict__ dst, size_t n) {
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) dst[i] = (float)src[i];
+    const size_t i = ((size_t)blockIdx.x * 256 + threadIdx.x) * 4;
+    if (VEC && i + 4 <= n) {
+        const uint32_t w = *reinterpret_cast<const uint32_t *>(src + i);
+        *reinterpret_cast<float4 *>(dst + i) =
+            make_float4((float)(w & 0xffu), (float)((w >> 8) & 0xffu), (float)((w >> 16) & 0xffu),
+                        (float)(w >> 24));
+    } else {
+        for (size_t k = i; k < i + 4 && k < n; k++) dst[k] = (float)src[k];
+    }
 }
 
 // v = (1 + amount) * p - amount * q, each operation rounded separately,
 // then Clamp<f32> for u8 (definitions.rs:120-136: truncation, NaN -> 255).
+__device__ __forceinline__ uint32_t sharpen_one(float a1, float amount, uint32_t p, float q) {
+    const float v = __fsub_rn(__fmul_rn(a1, (float)p), __fmul_rn(amount, q));
+    return v < 255.0f ? (v > 0.0f ? (uint32_t)v : 0u) : 255u;
+}
+
+template <bool VEC>
 __global__ __launch_bounds__(256) void sharpen_combine_kernel(const uint8_t *__restrict__ p,
                                                               const float *__restrict__ q,
                                                               float amount, size_t n,
                                                               uint8_t *__restrict__ out) {
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const float v = __fsub_rn(__fmul_rn(__fadd_rn(1.0f, amount), (float)p[i]),
-                              __fmul_rn(amount, q[i]));
-    out[i] = v < 255.0f ? (v > 0.0f ? (uint8_t)v : (uint8_t)0) : (uint8_t)255;
+    const size_t i = ((size_t)blockIdx.x * 256 + threadIdx.x) * 4;
+    const float a1 = __fadd_rn(1.0f, amount);
+    if (VEC && i + 4 <= n) {
+        const uint32_t w = *reinterpret_cast<const uint32_t *>(p + i);
+        const float4 qq = *reinterpret_cast<const float4 *>(q + i);
+        *reinterpret_cast<uint32_t *>(out + i) =
+            sharpen_one(a1, amount, w & 0xffu, qq.x) |
+            (sharpen_one(a1, amount, (w >> 8) & 0xffu, qq.y) << 8) |
+            (sharpen_one(a1, amount, (w >> 16) & 0xffu, qq.z) << 16) |
+            (sharpen_one(a1, amount, w >> 24, qq.w) << 24);
+    } else {
+        for (size_t k = i; k < i + 4 && k < n; k++)
+            out[k] = (uint8_t)sharpen_one(a1, amount, p[k], q[k]);
+    }
 }
 }  // namespace
 
@@ -673,12 +698,19 @@ iph_status launch_sharpen_gaussian(const uint8_t *src, uint8_t *dst, uint32_t w,
         set_error("workspace too small for sharpen_gaussian");
         return IPH_EINVAL;
     }
-    const unsigned blocks = (unsigned)((n + 255) / 256);
-    u8_to_f32_kernel<<<blocks, 256, 0, s>>>(src, img, n);
+    const unsigned blocks = (unsigned)((n + 1023) / 1024);
+    const bool vec = ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 3) == 0;
+    if (vec)
+        u8_to_f32_kernel<true><<<blocks, 256, 0, s>>>(src, img, n);
+    else
+        u8_to_f32_kernel<false><<<blocks, 256, 0, s>>>(src, img, n);
     IPH_TRY(launch_status("u8_to_f32_kernel"));
     IPH_TRY(launch_separable(img, smooth, w, h, 1, n, IPH_F32, taps, taps, n_taps, IPH_F32,
                              sep_ws, sep_bytes, s));
-    sharpen_combine_kernel<<<blocks, 256, 0, s>>>(src, smooth, amount, n, dst);
+    if (vec)
+        sharpen_combine_kernel<true><<<blocks, 256, 0, s>>>(src, smooth, amount, n, dst);
+    else
+        sharpen_combine_kernel<false><<<blocks, 256, 0, s>>>(src, smooth, amount, n, dst);
     return launch_status("sharpen_combine_kernel");
 }
 
@@ -691,31 +723,93 @@ struct GradTaps {
     int hw, hh, vw, vh;
 };
 
-// filter_clamped::<_, i32, i16> at (x, y) (filter/mod.rs:29-69: ky outer, kx
-// inner, clamp-to-edge, centre (kw/2, kh/2); the i32 sum is exact) and its
-// saturation to i16 (definitions.rs:181-188).
-__device__ __forceinline__ int grad_filter(const uint8_t *__restrict__ src, int W, int H, int x,
-                                           int y, const int32_t *k, int kw, int kh) {
+// One 64 x 16 output tile per workgroup: the image window the two kernels
+// read (clamp-to-edge coordinates, filter/mod.rs:29-69, centre (kw/2, kh/2))
+// staged once in LDS, then per output both filter_clamped::<_, i32, i16>
+// sums (exact in i32, so their order is free) saturated to i16
+// (definitions.rs:181-188), and out = (dx^2 + dy^2).sqrt() as u16
+// (gradient_magnitude, gradients.rs:157-160): f32 squares and sum each rounded
+// once, correctly rounded sqrt, Rust's saturating float -> u16 cast.
+constexpr int kGradTW = 64, kGradTH = 16;
+constexpr int kGradSW = kGradTW + kGradMaxTaps - 1, kGradSH = kGradTH + kGradMaxTaps - 1;
+
+// K3: both kernels 3 x 3 (loops unrolled, taps in scalar registers); M24:
+// every |tap| < 2^23, so the u8 x tap products are exact 24-bit multiplies.
+// Otherwise runtime-sized loops with 32-bit products (wrapping like the
+// reference's i32 accumulator).
+template <bool K3, bool M24>
+__device__ __forceinline__ int grad_sum(const int *win, int pitch, int r, int c,
+                                        const int32_t *k, int kw, int kh) {
     int acc = 0;
-    for (int ky = 0; ky < kh; ky++) {
-        const uint8_t *row = src + (size_t)min(max(y + ky - kh / 2, 0), H - 1) * W;
-        for (int kx = 0; kx < kw; kx++) acc += (int)row[min(max(x + kx - kw / 2, 0), W - 1)] * k[ky * kw + kx];
+    if (K3) {
+#pragma unroll
+        for (int ky = 0; ky < 3; ky++)
+#pragma unroll
+            for (int kx = 0; kx < 3; kx++) {
+                const int p = win[(r + ky) * pitch + c + kx], t = k[ky * 3 + kx];
+                acc += M24 ? __mul24(p, t) : p * t;
+            }
+    } else {
+        for (int ky = 0; ky < kh; ky++) {
+            const int *row = win + (r + ky) * pitch + c;
+            for (int kx = 0; kx < kw; kx++) {
+                const int p = row[kx], t = k[ky * kw + kx];
+                acc += M24 ? __mul24(p, t) : p * t;
+            }
+        }
     }
     return min(max(acc, -32768), 32767);
 }
 
-// out = (dx^2 + dy^2).sqrt() as u16 (gradient_magnitude, gradients.rs:157-160):
-// f32 squares and sum each rounded once, correctly rounded sqrt, and Rust's
-// saturating float -> u16 cast (truncation toward zero).
+template <bool K3, bool M24>
 __global__ __launch_bounds__(256) void gradients_kernel(const uint8_t *__restrict__ src, int W,
                                                         int H, const GradTaps taps,
                                                         uint16_t *__restrict__ out) {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (x >= W || y >= H) return;
-    const float dx = (float)grad_filter(src, W, H, x, y, taps.h, taps.hw, taps.hh);
-    const float dy = (float)grad_filter(src, W, H, x, y, taps.v, taps.vw, taps.vh);
-    const float m = sqrtf(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)));
-    out[(size_t)y * W + x] = m >= 65535.0f ? (uint16_t)65535 : (uint16_t)m;
+    // one int per pixel: byte elements let the compiler merge neighbouring
+    // reads into misaligned 16-bit LDS accesses, which stall badly
+    __shared__ int win[K3 ? (kGradTH + 2) * (kGradTW + 2) : kGradSH * kGradSW];
+    // window origin: the farthest left / up either kernel reaches
+    const int lx0 = max(taps.hw / 2, taps.vw / 2), ly0 = max(taps.hh / 2, taps.vh / 2);
+    const int rx = max(taps.hw - 1 - taps.hw / 2, taps.vw - 1 - taps.vw / 2);
+    const int ry = max(taps.hh - 1 - taps.hh / 2, taps.vh - 1 - taps.vh / 2);
+    const int sw = kGradTW + lx0 + rx, sh = kGradTH + ly0 + ry;
+    const int X0 = blockIdx.x * kGradTW, Y0 = blockIdx.y * kGradTH;
+    // a wave per staged row, a lane per column; every load of the thread is
+    // issued before the LDS stores (a load-store loop would wait for each)
+    constexpr int kRowsPerWave = ((K3 ? kGradTH + 2 : kGradSH) + 3) / 4;
+    constexpr int kColsPerLane = ((K3 ? kGradTW + 2 : kGradSW) + 63) / 64;
+    int v[kRowsPerWave][kColsPerLane];
+#pragma unroll
+    for (int k = 0; k < kRowsPerWave; k++) {
+        const int r = (threadIdx.x >> 6) + 4 * k;
+        const uint8_t *row = src + (size_t)min(max(Y0 - ly0 + r, 0), H - 1) * W;
+#pragma unroll
+        for (int u = 0; u < kColsPerLane; u++) {
+            const int c = (threadIdx.x & 63) + 64 * u;
+            v[k][u] = (r < sh && c < sw) ? row[min(max(X0 - lx0 + c, 0), W - 1)] : 0;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kRowsPerWave; k++)
+#pragma unroll
+        for (int u = 0; u < kColsPerLane; u++) {
+            const int r = (threadIdx.x >> 6) + 4 * k, c = (threadIdx.x & 63) + 64 * u;
+            if (r < sh && c < sw) win[r * sw + c] = v[k][u];
+        }
+    __syncthreads();
+    const int lx = threadIdx.x & 63, x = X0 + lx;
+    if (x >= W) return;
+    for (int r = threadIdx.x >> 6; r < kGradTH; r += 4) {
+        const int y = Y0 + r;
+        if (y >= H) break;
+        // output (r, lx): kernel tap (0, 0) sits at window (r + ly0 - kh/2, lx + lx0 - kw/2)
+        const float dx = (float)grad_sum<K3, M24>(win, sw, r + ly0 - taps.hh / 2,
+                                                  lx + lx0 - taps.hw / 2, taps.h, taps.hw, taps.hh);
+        const float dy = (float)grad_sum<K3, M24>(win, sw, r + ly0 - taps.vh / 2,
+                                                  lx + lx0 - taps.vw / 2, taps.v, taps.vw, taps.vh);
+        const float m = sqrtf(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)));
+        out[(size_t)y * W + x] = m >= 65535.0f ? (uint16_t)65535 : (uint16_t)m;
+    }
 }
 }  // namespace
 
@@ -734,7 +828,23 @@ iph_status launch_gradients(const uint8_t *src, uint32_t w, uint32_t h, const in
     t.hh = (int)hkh;
     t.vw = (int)vkw;
     t.vh = (int)vkh;
-    gradients_kernel<<<dim3((w + 63) / 64, (h + 3) / 4), 256, 0, s>>>(src, (int)w, (int)h, t, out);
+    if ((h + kGradTH - 1) / kGradTH > 65535u) {
+        set_error("gradients supports heights up to %u", 65535u * kGradTH);
+        return IPH_EUNSUPPORTED;
+    }
+    const bool k3 = hkw == 3 && hkh == 3 && vkw == 3 && vkh == 3;
+    bool m24 = true;
+    for (uint32_t i = 0; i < hkw * hkh; i++) m24 = m24 && hk[i] > -(1 << 23) && hk[i] < (1 << 23);
+    for (uint32_t i = 0; i < vkw * vkh; i++) m24 = m24 && vk[i] > -(1 << 23) && vk[i] < (1 << 23);
+    const dim3 grid((w + kGradTW - 1) / kGradTW, (h + kGradTH - 1) / kGradTH);
+    if (k3 && m24)
+        gradients_kernel<true, true><<<grid, 256, 0, s>>>(src, (int)w, (int)h, t, out);
+    else if (k3)
+        gradients_kernel<true, false><<<grid, 256, 0, s>>>(src, (int)w, (int)h, t, out);
+    else if (m24)
+        gradients_kernel<false, true><<<grid, 256, 0, s>>>(src, (int)w, (int)h, t, out);
+    else
+        gradients_kernel<false, false><<<grid, 256, 0, s>>>(src, (int)w, (int)h, t, out);
     return launch_status("gradients_kernel");
 }
 

@@ -8,6 +8,8 @@ from __future__ import annotations
 
 import ctypes as C
 
+import numpy as np
+
 import torch
 
 from ._lib import F32, I16, I32, U8, U16, Extremes, check, lib
@@ -176,6 +178,39 @@ def canny(image: torch.Tensor, low_threshold: float, high_threshold: float,
     ws = _workspace("canny", wsb, image.device)
     check(lib().iph_dev_canny(image.data_ptr(), out.data_ptr(), w, h, float(low_threshold),
                               float(high_threshold), ws.data_ptr(), ws.numel(), _stream()))
+    return out
+
+
+def gradients(image: torch.Tensor, horizontal_kernel, vertical_kernel,
+              out: torch.Tensor | None = None) -> torch.Tensor:
+    """gradients_grayscale (gradients.rs:15-21, 99-160) of a device uint8
+    tensor with Kernel<i32> taps: u16 magnitudes (an int16 tensor holding the
+    u16 bits when `out` is not given)."""
+    if _code(image) != _TORCH_CODE[torch.uint8]:
+        raise TypeError("gradients takes GrayImages (uint8)")
+    hk = np.ascontiguousarray(horizontal_kernel.data, np.int32)
+    vk = np.ascontiguousarray(vertical_kernel.data, np.int32)
+    h, w = image.shape
+    if out is None:
+        out = torch.empty((h, w), dtype=torch.int16, device=image.device)
+    check(lib().iph_dev_gradients(image.data_ptr(), w, h, hk.ctypes.data, horizontal_kernel.width,
+                                  horizontal_kernel.height, vk.ctypes.data, vertical_kernel.width,
+                                  vertical_kernel.height, out.data_ptr(), _stream()))
+    return out
+
+
+def sharpen_gaussian(image: torch.Tensor, sigma: float, amount: float,
+                     out: torch.Tensor | None = None) -> torch.Tensor:
+    """sharpen_gaussian (filter/sharpen.rs:31-38) of a device uint8 tensor."""
+    if _code(image) != _TORCH_CODE[torch.uint8]:
+        raise TypeError("sharpen_gaussian takes GrayImages (uint8)")
+    h, w = image.shape
+    if out is None:
+        out = torch.empty_like(image)
+    wsb = lib().iph_dev_sharpen_gaussian_workspace(w, h, float(sigma))
+    ws = _workspace("sharpen", wsb, image.device)
+    check(lib().iph_dev_sharpen_gaussian(image.data_ptr(), out.data_ptr(), w, h, float(sigma),
+                                         float(amount), ws.data_ptr(), ws.numel(), _stream()))
     return out
 
 

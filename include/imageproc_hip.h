@@ -240,6 +240,16 @@ IPH_API size_t iph_dev_canny_workspace(uint32_t width, uint32_t height);
 IPH_API iph_status iph_dev_canny(const uint8_t *src, uint8_t *dst, uint32_t width,
                                  uint32_t height, float low_threshold, float high_threshold,
                                  void *workspace, size_t workspace_bytes, void *stream);
+/* iph_gradients and iph_sharpen_gaussian on device buffers (kernels of
+ * iph_dev_gradients stay host arrays, copied into the launch). */
+IPH_API iph_status iph_dev_gradients(const uint8_t *src, uint32_t width, uint32_t height,
+                                     const int32_t *h_kernel, uint32_t hkw, uint32_t hkh,
+                                     const int32_t *v_kernel, uint32_t vkw, uint32_t vkh,
+                                     uint16_t *out, void *stream);
+IPH_API size_t iph_dev_sharpen_gaussian_workspace(uint32_t width, uint32_t height, float sigma);
+IPH_API iph_status iph_dev_sharpen_gaussian(const uint8_t *src, uint8_t *dst, uint32_t width,
+                                            uint32_t height, float sigma, float amount,
+                                            void *workspace, size_t workspace_bytes, void *stream);
 IPH_API size_t iph_dev_match_template_with_mask_workspace(void);
 IPH_API iph_status iph_dev_match_template_with_mask(const uint8_t *image, uint32_t width,
                                                     uint32_t height, const uint8_t *tmpl,

@@ -29,6 +29,10 @@ def test_elephant_sobel_truth(fixtures):
     (G.PREWITT_HORIZONTAL_3X3, G.PREWITT_VERTICAL_3X3),
     (Kernel([1, 0, 0, -1], 2, 2), Kernel([0, 1, -1, 0], 2, 2)),          # Roberts, even sides
     (Kernel(np.arange(-12, 13), 5, 5), Kernel([3, -7, 200, -1, 9], 1, 5)),  # i16 saturation
+    # taps of 2^24 and more: 32-bit products that wrap like the reference's i32
+    (Kernel([1 << 24, 0, -(1 << 24), 3, 0, -3, 1 << 30, 5, -(1 << 30)], 3, 3),
+     G.SOBEL_VERTICAL_3X3),
+    (Kernel([1 << 27, -1, 7, -(1 << 29)], 2, 2), Kernel([1 << 23, 2, -3, 4], 2, 2)),
 ])
 def test_gradients_random(hk, vk):
     img = np.random.default_rng(hk.width * 7 + vk.height).integers(0, 256, (67, 131), dtype=np.uint8)
@@ -43,3 +47,16 @@ def test_directional_filters_and_prewitt():
                           O.filter_clamped(img, G.SCHARR_VERTICAL_3X3.data, 3, 3, np.int16))
     assert np.array_equal(G.prewitt_gradients(img),
                           _magnitude(img, G.PREWITT_HORIZONTAL_3X3, G.PREWITT_VERTICAL_3X3))
+
+
+def test_device_entry_matches_host_entry():
+    import torch
+
+    from imageproc_amd import device
+
+    img = np.random.default_rng(17).integers(0, 256, (301, 517), dtype=np.uint8)
+    for hk, vk in [(G.SOBEL_HORIZONTAL_3X3, G.SOBEL_VERTICAL_3X3),
+                   (G.SCHARR_HORIZONTAL_3X3, G.SCHARR_VERTICAL_3X3)]:
+        want = G.gradients_grayscale(img, hk, vk)
+        got = device.gradients(torch.from_numpy(img).cuda(), hk, vk).cpu().numpy().view(np.uint16)
+        assert np.array_equal(got, want)

@@ -42,3 +42,15 @@ def test_sharpen3x3_random_and_errors():
     assert np.array_equal(flt.sharpen3x3(img), want)
     with pytest.raises(Panic, match="sigma must be > 0.0"):
         flt.sharpen_gaussian(img, 0.0, 1.0)
+
+
+def test_device_entry_matches_host_entry():
+    import torch
+
+    from imageproc_amd import device
+
+    img = np.random.default_rng(23).integers(0, 256, (257, 389), dtype=np.uint8)
+    for sigma, amount in [(0.7, 7.0), (2.0, 1.5)]:
+        got = device.sharpen_gaussian(torch.from_numpy(img).cuda(), sigma, amount).cpu().numpy()
+        assert np.array_equal(got, flt.sharpen_gaussian(img, sigma, amount))
+        assert np.array_equal(got, _oracle_sharpen_gaussian(img, sigma, amount))
