@@ -111,12 +111,15 @@ template <typename K> struct FusedTaps {
     K v[kFusedMaxTaps];
 };
 
-template <typename P, typename K>
+// LT > 0: the tap count is the compile-time LT (loops unrolled, taps in
+// scalar registers); LT == 0: Lrt at run time.
+template <typename P, typename K, int LT>
 __global__ __launch_bounds__(256) void sep_fused_kernel(const P *__restrict__ src,
                                                         P *__restrict__ dst, int W, int H,
-                                                        size_t stride, int L,
+                                                        size_t stride, int Lrt,
                                                         FusedTaps<K> taps) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int L = LT > 0 ? LT : Lrt;
     const int R = kSepTH + L - 1;
     const int C = kSepTW + L - 1;
     P *in_s = reinterpret_cast<P *>(smem_raw);
@@ -135,7 +138,11 @@ __global__ __launch_bounds__(256) void sep_fused_kernel(const P *__restrict__ sr
         int r = i / kSepTW, c = i - r * kSepTW;
         const P *row = in_s + r * C + c;
         K acc = K(0);
-        for (int t = 0; t < L; t++) acc = mac(acc, to_acc(row[t], K(0)), taps.h[t]);
+#pragma unroll
+        for (int t = 0; t < (LT > 0 ? LT : kFusedMaxTaps); t++) {
+            if (LT == 0 && t >= L) break;
+            acc = mac(acc, to_acc(row[t], K(0)), taps.h[t]);
+        }
         mid_s[i] = ClampTo<P>::from(acc);
     }
     __syncthreads();
@@ -146,7 +153,11 @@ __global__ __launch_bounds__(256) void sep_fused_kernel(const P *__restrict__ sr
         if (gy >= H || gx >= W) continue;
         const P *col = mid_s + r * kSepTW + c;
         K acc = K(0);
-        for (int t = 0; t < L; t++) acc = mac(acc, to_acc(col[t * kSepTW], K(0)), taps.v[t]);
+#pragma unroll
+        for (int t = 0; t < (LT > 0 ? LT : kFusedMaxTaps); t++) {
+            if (LT == 0 && t >= L) break;
+            acc = mac(acc, to_acc(col[t * kSepTW], K(0)), taps.v[t]);
+        }
         out[(size_t)gy * W + gx] = ClampTo<P>::from(acc);
     }
 }
@@ -379,7 +390,18 @@ iph_status separable_typed(const void *src, void *dst, uint32_t w, uint32_t h, u
         size_t lds = sizeof(P) * ((size_t)(kSepTH + L - 1) * (kSepTW + L - 1) +
                                   (size_t)(kSepTH + L - 1) * kSepTW);
         dim3 grid((w + kSepTW - 1) / kSepTW, (h + kSepTH - 1) / kSepTH, n);
-        sep_fused_kernel<P, K><<<grid, 256, lds, s>>>(in, out, (int)w, (int)h, stride, L, taps);
+#define IPH_SEP(LT)                                                                          \
+    sep_fused_kernel<P, K, LT><<<grid, 256, lds, s>>>(in, out, (int)w, (int)h, stride, L, taps)
+        switch (L) {
+        case 3: IPH_SEP(3); break;
+        case 5: IPH_SEP(5); break;
+        case 7: IPH_SEP(7); break;
+        case 9: IPH_SEP(9); break;
+        case 11: IPH_SEP(11); break;
+        case 13: IPH_SEP(13); break;
+        default: IPH_SEP(0); break;
+        }
+#undef IPH_SEP
         return launch_status("sep_fused_kernel");
     }
     // Long kernels: two passes through a P-typed intermediate in the workspace.
