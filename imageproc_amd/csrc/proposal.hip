@@ -501,7 +501,9 @@ __global__ __launch_bounds__(256) void coarse_tiles_kernel(
     __shared__ uint8_t tb[kCoarseMax * kCoarseMax];
     __shared__ uint32_t ts[kCoarseMax * 4];
     __shared__ uint32_t iw[(16 + kCoarseMax - 1) * kCoarseW];
-    __shared__ uint32_t pre[(16 + kCoarseMax) * 256];  // per-thread prefix sums of row energies
+    // per-thread prefix sums of row energies, rows 1 .. 16 + kCoarseMax - 1
+    // (row 0 is zero and never read)
+    __shared__ uint32_t pre[(16 + kCoarseMax - 1) * 256];
     __shared__ long long red[4];
     const int tid = threadIdx.x, x0 = blockIdx.x * 256, y0 = blockIdx.y * 16;
     const int stw = tw >> 2, sth = th >> 2, R = 16 + sth - 1;
@@ -568,16 +570,16 @@ __global__ __launch_bounds__(256) void coarse_tiles_kernel(
     const int j0 = tid >> 2;
     const uint32_t shift = (uint32_t)(tid & 3);
     const uint32_t last = (stw & 3) ? (1u << (8 * (stw & 3))) - 1u : 0xffffffffu;
-    uint32_t sit[TWQ][16], sii[16];
+    // sum(i * t) per output row: at most 256 products below 2^16 each, so
+    // a u32 accumulator is exact
+    uint32_t sit[16], sii[16];
 #pragma unroll
-    for (int q = 0; q < TWQ; q++)
-#pragma unroll
-        for (int j = 0; j < 16; j++) sit[q][j] = 0u;
+    for (int j = 0; j < 16; j++) sit[j] = 0u;
     // every (image row, template row) pair, without branches: staged rows
     // past R are zero. sum(i^2) of output row j is pre[j + sth] - pre[j],
-    // pre[rr] the sum of the row energies above image row rr.
+    // pre[rr] the sum of the row energies above image row rr (stored at
+    // row rr - 1).
     uint32_t run = 0;
-    pre[tid] = 0u;
 #pragma unroll
     for (int rr = 0; rr < 16 + kCoarseMax - 1; rr++) {
         const uint32_t *row = iw + rr * kCoarseW + j0;
@@ -594,30 +596,31 @@ __global__ __launch_bounds__(256) void coarse_tiles_kernel(
 #pragma unroll
         for (int q = 0; q < TWQ; q++) e = __builtin_amdgcn_udot4(w4[q], w4[q], e, false);
         run += e;
-        pre[(rr + 1) * 256 + tid] = run;
+        pre[rr * 256 + tid] = run;
         if (rr < 16) sii[rr] = run - e;  // pre[rr]
-        // one accumulator per (output row, template word): independent chains
+        // one accumulator per output row: up to 16 independent chains
 #pragma unroll
         for (int q = 0; q < TWQ; q++)
 #pragma unroll
             for (int r = 0; r < kCoarseMax; r++) {
                 const int j = rr - r;
                 if (j >= 0 && j < 16)
-                    sit[q][j] = __builtin_amdgcn_udot4(w4[q], tsw[r][q], sit[q][j], false);
+                    sit[j] = __builtin_amdgcn_udot4(w4[q], tsw[r][q], sit[j], false);
             }
     }
 #pragma unroll
-    for (int j = 0; j < 16; j++) sii[j] = pre[(j + sth) * 256 + tid] - sii[j];
+    for (int j = 0; j < 16; j++) sii[j] = pre[(j + sth - 1) * 256 + tid] - sii[j];
     const int X = x0 + tid;
     uint64_t best = 0;
 #pragma unroll
     for (int j = 0; j < 16; j++) {
         const int Y = y0 + j;
         if (X < cw && Y < ch) {
-            long long c = 0;
-#pragma unroll
-            for (int q = 0; q < TWQ; q++) c += sit[q][j];
-            const float v = finish_score<METHOD>(c, (long long)sii[j], stt, (long long)stw * sth);
+            const long long c = sit[j];
+            // windows of at most kCoarseMax^2 < 64^2 pixels: finish_score's f32
+            // branch, chosen at compile time (n = 0 selects the same branch)
+            static_assert(kCoarseMax * kCoarseMax < 64 * 64, "coarse windows stay below 64x64");
+            const float v = finish_score<METHOD>(c, (long long)sii[j], stt, 0);
             coarse[(size_t)Y * cw + X] = v;
             best = max_key(best, entry_key(v, Y * cw + X));
         }
