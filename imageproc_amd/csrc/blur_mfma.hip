@@ -635,7 +635,9 @@ constexpr int fx_waves(int nkb, int taps, bool aligned) {
 }
 
 // TAPS: 0 = distinct horizontal and vertical taps; 1 = equal taps; 2 = equal
-// taps and no window shift (m = 0), so both passes use the same B operands.
+// taps with the rows staged under the same shift m as the columns (L + m <=
+// 33 keeps the shifted rows inside the staged window), so both passes use the
+// same B operands.
 template <int NKB, int TAPS, bool ALIGNED, typename Jobs>
 __device__ __forceinline__ void blur_u8_fx_body(const Jobs &jobs, int ntiles) {
     using S = RowStage<NKB, kRows2>;
@@ -676,7 +678,7 @@ __device__ __forceinline__ void blur_u8_fx_body(const Jobs &jobs, int ntiles) {
     auto setup_job = [&](const FxJob &jb) {
         L = jb.L;
         half = L / 2;
-        m = SHARE_B ? 0 : jb.m;
+        m = jb.m;
         thr_h = 2u * (uint32_t)jb.ch;
         thr_v = 2u * (uint32_t)jb.cv;
         build_b_fx<NKB>(taps_h, lane, m, bh_hi, bh_lo);
@@ -717,11 +719,13 @@ __device__ __forceinline__ void blur_u8_fx_body(const Jobs &jobs, int ntiles) {
         const int per_img = jb.tiles_x * jb.tiles_y;
         const int lt = tt - jb.tile0;
         const int im = lt / per_img, rem = lt - im * per_img, ty = rem / jb.tiles_x;
-        const int hf = jb.L / 2, mm = SHARE_B ? 0 : jb.m;
+        // shared B operands: the rows are staged with the same shift m as the
+        // columns, so both passes use one Toeplitz matrix
+        const int hf = jb.L / 2, mm = jb.m;
         const int xs = (rem - ty * jb.tiles_x) * kTW - hf - mm;
         interior = xs >= 0 && xs + S::CIN <= jb.W;
-        load_rows<S, ALIGNED>(jb.src + (size_t)im * jb.stride, jb.W, jb.H, xs, ty * kTH2 - hf,
-                              interior, regs);
+        load_rows<S, ALIGNED>(jb.src + (size_t)im * jb.stride, jb.W, jb.H, xs,
+                              ty * kTH2 - hf - (SHARE_B ? mm : 0), interior, regs);
     };
     if (t < t_end) {
         jc = job_of(t, 0);
@@ -739,7 +743,7 @@ __device__ __forceinline__ void blur_u8_fx_body(const Jobs &jobs, int ntiles) {
 
         __syncthreads();  // the previous tile is done with in_s / out_s, ht and the taps
         if (new_job) stage_taps(jb);
-        store_rows<S, ALIGNED>(in_s, W, x0 - jb.L / 2 - (SHARE_B ? 0 : jb.m), interior, regs);
+        store_rows<S, ALIGNED>(in_s, W, x0 - jb.L / 2 - jb.m, interior, regs);
         const int tn = t + per_xcd;
         int jn = jc;
         if (tn < t_end) {
@@ -841,7 +845,7 @@ __device__ __forceinline__ void blur_u8_fx_body(const Jobs &jobs, int ntiles) {
             for (int e = lane; e < qn; e += 64) {
                 const uint32_t ent = queue[e];
                 const int y2 = (int)(ent >> 8), x2 = (int)(ent & 255);
-                float r = exact_chain8<LMAX>(ht + x2 * kPitchB + y2, vts, L);
+                float r = exact_chain8<LMAX>(ht + x2 * kPitchB + y2 + (SHARE_B ? m : 0), vts, L);
                 out_s[y2 * kOPitch + x2] = clamp_u8_exact(r);
             }
             qn = 0;
@@ -1120,7 +1124,7 @@ iph_status launch_blur_u8_mfma(const uint8_t *src, uint8_t *dst, uint32_t w, uin
 #define FA(K, SAME)                                                                          \
     if (aligned) FB(K, SAME, true); else FB(K, SAME, false)
 #define FK(K)                                                                                \
-    if (same && m == 0) { FA(K, 2); } else if (same) { FA(K, 1); } else { FA(K, 0); }       \
+    if (same && L + m <= 33) { FA(K, 2); } else if (same) { FA(K, 1); } else { FA(K, 0); }  \
     break
         switch (nkb) {
         case 2: FK(2);
@@ -1172,7 +1176,7 @@ iph_status blur_jobs_table(const BlurSpec *specs, int n, void *host_table,
         }
         const bool aligned = b.w >= 4 && (b.w % 4) == 0 &&
                              (reinterpret_cast<uintptr_t>(b.src) & 3) == 0;
-        const int cls = (same && m == 0) ? 2 : 1;
+        const int cls = (same && L + m <= 33) ? 2 : 1;
         FxJob f{};
         f.src = b.src;
         f.dst = b.dst;
