@@ -490,20 +490,21 @@ __global__ __launch_bounds__(kChunk) void topk_single_kernel(const uint64_t *__r
 // against the template rows (image words realigned to the column with
 // v_alignbyte), sum(i^2) from each staged row's v_dot4 with itself, both
 // reused by every output row the image row falls in. Scores come from
-// finish_score, as the full map's do.
+// finish_score, as the full map's do. TR >= sth template rows are summed
+// (rows past sth are zero words).
 constexpr int kCoarseMax = 16;
 constexpr int kCoarseW = 68;  // staged words per image row: 256 + 15 columns, rounded up
-template <int METHOD, int TWQ>
+template <int METHOD, int TWQ, int TR>
 __global__ __launch_bounds__(256) void coarse_tiles_kernel(
     const uint8_t *__restrict__ small, int sw, int sh, const uint8_t *__restrict__ tpl, int tw,
     int th, int cw, int ch, int tiles_x, int vec, float *__restrict__ coarse,
     uint64_t *__restrict__ tkey) {
     __shared__ uint8_t tb[kCoarseMax * kCoarseMax];
     __shared__ uint32_t ts[kCoarseMax * 4];
-    __shared__ uint32_t iw[(16 + kCoarseMax - 1) * kCoarseW];
-    // per-thread prefix sums of row energies, rows 1 .. 16 + kCoarseMax - 1
+    __shared__ uint32_t iw[(16 + TR - 1) * kCoarseW];
+    // per-thread prefix sums of row energies, rows 1 .. 16 + TR - 1
     // (row 0 is zero and never read)
-    __shared__ uint32_t pre[(16 + kCoarseMax - 1) * 256];
+    __shared__ uint32_t pre[(16 + TR - 1) * 256];
     __shared__ long long red[4];
     const int tid = threadIdx.x, x0 = blockIdx.x * 256, y0 = blockIdx.y * 16;
     const int stw = tw >> 2, sth = th >> 2, R = 16 + sth - 1;
@@ -521,7 +522,7 @@ __global__ __launch_bounds__(256) void coarse_tiles_kernel(
         tsq += v * v;
     }
     {  // every load of the window in flight at once, then the LDS stores
-        constexpr int kWords = (16 + kCoarseMax - 1) * kCoarseW;
+        constexpr int kWords = (16 + TR - 1) * kCoarseW;
         constexpr int kPer = (kWords + 255) / 256;
         uint32_t wv[kPer];
 #pragma unroll
@@ -562,9 +563,9 @@ __global__ __launch_bounds__(256) void coarse_tiles_kernel(
     const long long stt = red[0] + red[1] + red[2] + red[3];
     // the template rows, the same for every lane: scalar registers (rows past
     // sth are zero words, so they add nothing to sum(i*t))
-    uint32_t tsw[kCoarseMax][TWQ];
+    uint32_t tsw[TR][TWQ];
 #pragma unroll
-    for (int r = 0; r < kCoarseMax; r++)
+    for (int r = 0; r < TR; r++)
 #pragma unroll
         for (int q = 0; q < TWQ; q++) tsw[r][q] = __builtin_amdgcn_readfirstlane(ts[r * 4 + q]);
     const int j0 = tid >> 2;
@@ -581,7 +582,7 @@ __global__ __launch_bounds__(256) void coarse_tiles_kernel(
     // row rr - 1).
     uint32_t run = 0;
 #pragma unroll
-    for (int rr = 0; rr < 16 + kCoarseMax - 1; rr++) {
+    for (int rr = 0; rr < 16 + TR - 1; rr++) {
         const uint32_t *row = iw + rr * kCoarseW + j0;
         uint32_t w4[TWQ];
         uint32_t lo = row[0];
@@ -602,7 +603,7 @@ __global__ __launch_bounds__(256) void coarse_tiles_kernel(
 #pragma unroll
         for (int q = 0; q < TWQ; q++)
 #pragma unroll
-            for (int r = 0; r < kCoarseMax; r++) {
+            for (int r = 0; r < TR; r++) {
                 const int j = rr - r;
                 if (j >= 0 && j < 16)
                     sit[j] = __builtin_amdgcn_udot4(w4[q], tsw[r][q], sit[j], false);
@@ -1523,10 +1524,13 @@ IPH_API iph_status iph_dev_proposal_search(const uint8_t *image, uint32_t w, uin
         const dim3 grid((p.cw + 255) / 256, cp.ty);
         const int vec = (p.sw % 4 == 0) && ((uintptr_t)small % 4 == 0);
         const int twq = (int)(p.stw + 3) / 4;
+#define CTR(M, Q, R)                                                                          \
+    coarse_tiles_kernel<M, Q, R><<<grid, 256, 0, s>>>(small, (int)p.sw, (int)p.sh, tpl,         \
+                                                      (int)tw, (int)th, (int)p.cw, (int)p.ch,  \
+                                                      cp.tx, vec, coarse, tk)
+        // templates no taller than wide sum 4 * Q rows, others all kCoarseMax
 #define CT(M, Q)                                                                             \
-    coarse_tiles_kernel<M, Q><<<grid, 256, 0, s>>>(small, (int)p.sw, (int)p.sh, tpl, (int)tw, \
-                                                   (int)th, (int)p.cw, (int)p.ch, cp.tx, vec,  \
-                                                   coarse, tk)
+    if ((int)p.sth <= 4 * (Q)) CTR(M, Q, 4 * (Q)); else CTR(M, Q, kCoarseMax)
 #define CTM(M)                                                                               \
     switch (twq) {                                                                           \
     case 1: CT(M, 1); break;                                                                 \
@@ -1541,6 +1545,7 @@ IPH_API iph_status iph_dev_proposal_search(const uint8_t *image, uint32_t w, uin
         }
 #undef CTM
 #undef CT
+#undef CTR
         IPH_TRY(launch_status("coarse_tiles_kernel"));
     } else {
         IPH_TRY(launch_downsample(tpl, tw, th, stpl, s));
