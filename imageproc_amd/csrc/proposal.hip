@@ -466,6 +466,7 @@ __global__ __launch_bounds__(kChunk) void topk_single_kernel(const uint64_t *__r
     if (tid < nc) {
         const uint64_t key = xs[tid];
         int rank = 0;
+#pragma unroll 8
         for (int j = 0; j < nc; j++) rank += xs[j] > key;
         if (rank < k) {
             const int id = key_index(key);
@@ -782,6 +783,7 @@ __global__ __launch_bounds__(256) void logit_patch_kernel(const float *__restric
     const int x = cxy[2 * c] - 8 + (pix & 15), y = cxy[2 * c + 1] - 8 + (pix >> 4);
     bool sel = x >= 0 && x < cw && y >= 0 && y < ch && v == v;
     if (sel) {
+#pragma unroll 8
         for (int c2 = 0; c2 < k; c2++) {
             const int px = x - (cxy[2 * c2] - 8), py = y - (cxy[2 * c2 + 1] - 8);
             if (c2 != c && (unsigned)px < 16u && (unsigned)py < 16u) {
@@ -835,6 +837,7 @@ __global__ __launch_bounds__(256) void logit_merge_kernel(const uint64_t *__rest
     if (tid < k && head[tid] != 0ull) {
         const uint64_t h = head[tid];
         int rank = 0;
+#pragma unroll 8
         for (int j = 0; j < k; j++) rank += head[j] > h;
         if (rank == n_top - 1) thr = h;
     }
@@ -858,6 +861,7 @@ __global__ __launch_bounds__(256) void logit_merge_kernel(const uint64_t *__rest
         for (int s = tid; s < ns; s += 256) {
             const uint64_t e = sur[s];
             int rank = 0;
+#pragma unroll 8
             for (int j = 0; j < ns; j++) rank += sur[j] > e;
             if (rank < n_top) emit(rank, e);
         }
