@@ -35,6 +35,7 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
 constexpr int kMaxSide = 128;  // largest template side on the int8 path
+static_assert(kMaxSide * 255 < 65536, "a template row's pixel sum fits 16 bits");
 constexpr int kWinRows = 64;   // output rows per window-sum workgroup
 constexpr int kWinCols = 64;   // output columns per window-sum workgroup
 constexpr int kHsPitch = kWinCols + 1;  // LDS row pitch of the horizontal sums (odd: no bank clash)
@@ -59,7 +60,10 @@ __global__ __launch_bounds__(256) void window_sums_u8_kernel(const uint8_t *__re
     const int H = oh + th - 1;
     const bool aligned = (W & 3) == 0 && (reinterpret_cast<uintptr_t>(img) & 3) == 0;
     uint32_t *blk = reinterpret_cast<uint32_t *>(smem);                      // rows x cw4 words
-    uint2 *hs = reinterpret_cast<uint2 *>(smem + align_up_dev(rows * cols));  // rows x kHsPitch
+    // horizontal sums: sum I (below 2^15 for tw <= 128) and sum I^2, rows x kHsPitch each
+    uint16_t *hsa = reinterpret_cast<uint16_t *>(smem + align_up_dev(rows * cols));
+    uint32_t *hsb = reinterpret_cast<uint32_t *>(smem + align_up_dev(rows * cols) +
+                                                 align_up_dev(rows * kHsPitch * 2));
     for (int base = threadIdx.x; base < rows * cw4; base += 8 * 256) {
         uint32_t v[8];
 #pragma unroll
@@ -91,6 +95,7 @@ __global__ __launch_bounds__(256) void window_sums_u8_kernel(const uint8_t *__re
         const int r = it / kSegs, q = it - r * kSegs;
         const uint32_t *pw = blk + r * cw4 + 4 * q;
         uint32_t a = 0, b = 0;
+#pragma unroll 4
         for (int k = 0; k < nw; k++) {
             const uint32_t wv = pw[k];
             a = __builtin_amdgcn_udot4(wv, 0x01010101u, a, false);
@@ -102,14 +107,17 @@ __global__ __launch_bounds__(256) void window_sums_u8_kernel(const uint8_t *__re
             b = __builtin_amdgcn_udot4(wv, wv, b, false);
         }
         const uint8_t *p = reinterpret_cast<const uint8_t *>(pw);
-        uint2 *o = hs + r * kHsPitch + 16 * q;
-        o[0] = make_uint2(a, b);
+        uint16_t *oa = hsa + r * kHsPitch + 16 * q;
+        uint32_t *ob = hsb + r * kHsPitch + 16 * q;
+        oa[0] = (uint16_t)a;
+        ob[0] = b;
 #pragma unroll
         for (int x = 1; x < 16; x++) {
             uint32_t vin = p[x + tw - 1], vout = p[x - 1];
             a += vin - vout;
             b += vin * vin - vout * vout;
-            o[x] = make_uint2(a, b);
+            oa[x] = (uint16_t)a;
+            ob[x] = b;
         }
     }
     __syncthreads();
@@ -117,18 +125,18 @@ __global__ __launch_bounds__(256) void window_sums_u8_kernel(const uint8_t *__re
     constexpr int kSeg = kWinRows / (256 / kWinCols);
     const int ya = seg * kSeg;
     uint32_t a = 0, b = 0;
+#pragma unroll 8
     for (int dy = 0; dy < th; dy++) {
-        uint2 v = hs[(ya + dy) * kHsPitch + x];
-        a += v.x;
-        b += v.y;
+        a += hsa[(ya + dy) * kHsPitch + x];
+        b += hsb[(ya + dy) * kHsPitch + x];
     }
+#pragma unroll
     for (int y = ya; y < ya + kSeg; y++) {
         const int gy = y0 + y, gx = x0 + x;
         if (gy < oh && gx < ow) ws[(size_t)gy * ow + gx] = make_uint2(a, b);
-        uint2 vin = hs[y + th < rows ? (y + th) * kHsPitch + x : 0];
-        uint2 vout = hs[y * kHsPitch + x];
-        a += vin.x - vout.x;
-        b += vin.y - vout.y;
+        const int in = y + th < rows ? (y + th) * kHsPitch + x : 0, out = y * kHsPitch + x;
+        a += (uint32_t)hsa[in] - (uint32_t)hsa[out];
+        b += hsb[in] - hsb[out];
     }
 }
 
@@ -543,7 +551,8 @@ int blocks_for(int tw) { return (31 + tw + 31) / 32; }
 
 size_t win_lds(int tw, int th) {
     int cols = win_cols(tw), rows = kWinRows + th - 1;
-    return align_up((size_t)rows * cols, 16) + (size_t)rows * kHsPitch * sizeof(uint2);
+    return align_up((size_t)rows * cols, 16) + align_up((size_t)rows * kHsPitch * 2, 16) +
+           (size_t)rows * kHsPitch * sizeof(uint32_t);
 }
 
 thread_local int g_match_rows_hint = 0;
