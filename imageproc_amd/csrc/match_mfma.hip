@@ -64,10 +64,12 @@ __global__ __launch_bounds__(256) void window_sums_u8_kernel(const uint8_t *__re
     uint16_t *hsa = reinterpret_cast<uint16_t *>(smem + align_up_dev(rows * cols));
     uint32_t *hsb = reinterpret_cast<uint32_t *>(smem + align_up_dev(rows * cols) +
                                                  align_up_dev(rows * kHsPitch * 2));
-    for (int base = threadIdx.x; base < rows * cw4; base += 8 * 256) {
-        uint32_t v[8];
+    // 16 loads per thread in flight: one round for templates up to 48 rows
+    constexpr int kStage = 16;
+    for (int base = threadIdx.x; base < rows * cw4; base += kStage * 256) {
+        uint32_t v[kStage];
 #pragma unroll
-        for (int b = 0; b < 8; b++) {
+        for (int b = 0; b < kStage; b++) {
             const int i = min(base + 256 * b, rows * cw4 - 1);
             const int r = i / cw4, gx = x0 + 4 * (i - r * cw4), gy = y0 + r;
             uint32_t wv = 0;
@@ -84,7 +86,7 @@ __global__ __launch_bounds__(256) void window_sums_u8_kernel(const uint8_t *__re
             v[b] = wv;
         }
 #pragma unroll
-        for (int b = 0; b < 8; b++)
+        for (int b = 0; b < kStage; b++)
             if (base + 256 * b < rows * cw4) blk[base + 256 * b] = v[b];
     }
     __syncthreads();
