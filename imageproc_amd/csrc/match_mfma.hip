@@ -39,6 +39,7 @@ static_assert(kMaxSide * 255 < 65536, "a template row's pixel sum fits 16 bits")
 constexpr int kWinRows = 64;   // output rows per window-sum workgroup
 constexpr int kWinCols = 64;   // output columns per window-sum workgroup
 constexpr int kHsPitch = kWinCols + 1;  // LDS row pitch of the horizontal sums (odd: no bank clash)
+constexpr int kWinThreads = 512;  // threads per window-sum workgroup (8 rows per column segment)
 
 __device__ __forceinline__ int align_up_dev(int v) { return (v + 15) & ~15; }
 // Staged bytes per row of the window-sum tile, a multiple of 4.
@@ -51,7 +52,7 @@ __host__ __device__ __forceinline__ int win_cols(int tw) { return (kWinCols + tw
 // v_dot4 over whole words, then 15 sliding steps. Each output column then
 // slides its vertical th-window sum down a 16-row segment. All arithmetic
 // is exact u32; written coalesced.
-__global__ __launch_bounds__(256) void window_sums_u8_kernel(const uint8_t *__restrict__ img,
+__global__ __launch_bounds__(kWinThreads) void window_sums_u8_kernel(const uint8_t *__restrict__ img,
                                                              int W, int tw, int th, int ow,
                                                              int oh, uint2 *__restrict__ ws) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -64,13 +65,13 @@ __global__ __launch_bounds__(256) void window_sums_u8_kernel(const uint8_t *__re
     uint16_t *hsa = reinterpret_cast<uint16_t *>(smem + align_up_dev(rows * cols));
     uint32_t *hsb = reinterpret_cast<uint32_t *>(smem + align_up_dev(rows * cols) +
                                                  align_up_dev(rows * kHsPitch * 2));
-    // 16 loads per thread in flight: one round for templates up to 48 rows
+    // 16 loads per thread in flight
     constexpr int kStage = 16;
-    for (int base = threadIdx.x; base < rows * cw4; base += kStage * 256) {
+    for (int base = threadIdx.x; base < rows * cw4; base += kStage * kWinThreads) {
         uint32_t v[kStage];
 #pragma unroll
         for (int b = 0; b < kStage; b++) {
-            const int i = min(base + 256 * b, rows * cw4 - 1);
+            const int i = min(base + kWinThreads * b, rows * cw4 - 1);
             const int r = i / cw4, gx = x0 + 4 * (i - r * cw4), gy = y0 + r;
             uint32_t wv = 0;
             if (gy < H) {
@@ -87,13 +88,13 @@ __global__ __launch_bounds__(256) void window_sums_u8_kernel(const uint8_t *__re
         }
 #pragma unroll
         for (int b = 0; b < kStage; b++)
-            if (base + 256 * b < rows * cw4) blk[base + 256 * b] = v[b];
+            if (base + kWinThreads * b < rows * cw4) blk[base + kWinThreads * b] = v[b];
     }
     __syncthreads();
     constexpr int kSegs = kWinCols / 16;
     const int nw = tw >> 2;
     const uint32_t tail = (tw & 3) ? (1u << (8 * (tw & 3))) - 1u : 0u;
-    for (int it = threadIdx.x; it < rows * kSegs; it += 256) {
+    for (int it = threadIdx.x; it < rows * kSegs; it += kWinThreads) {
         const int r = it / kSegs, q = it - r * kSegs;
         const uint32_t *pw = blk + r * cw4 + 4 * q;
         uint32_t a = 0, b = 0;
@@ -124,7 +125,7 @@ __global__ __launch_bounds__(256) void window_sums_u8_kernel(const uint8_t *__re
     }
     __syncthreads();
     const int x = threadIdx.x & (kWinCols - 1), seg = threadIdx.x / kWinCols;
-    constexpr int kSeg = kWinRows / (256 / kWinCols);
+    constexpr int kSeg = kWinRows / (kWinThreads / kWinCols);
     const int ya = seg * kSeg;
     uint32_t a = 0, b = 0;
 #pragma unroll 8
@@ -689,7 +690,7 @@ iph_status launch_mfma_match(const void *img, int32_t p_type, uint32_t w, uint32
     const uint8_t *tp = static_cast<const uint8_t *>(tpl);
     int ow = (int)(w - tw + 1), oh = (int)(h - th + 1);
     dim3 wg((ow + kWinCols - 1) / kWinCols, (oh + kWinRows - 1) / kWinRows);
-    window_sums_u8_kernel<<<wg, 256, win_lds((int)tw, (int)th), s>>>(im, (int)w, (int)tw,
+    window_sums_u8_kernel<<<wg, kWinThreads, win_lds((int)tw, (int)th), s>>>(im, (int)w, (int)tw,
                                                                     (int)th, ow, oh, wsum);
     IPH_TRY(launch_status("window_sums_u8_kernel"));
     switch (KB) {
