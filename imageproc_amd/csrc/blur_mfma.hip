@@ -527,6 +527,27 @@ __device__ __forceinline__ void build_b_fx(const float *taps, int lane, int shif
     }
 }
 
+// Smallest fraction field of four accumulator bit patterns: one compare
+// against the margin then tells whether any of the four needs the exact chain.
+__device__ __forceinline__ uint32_t min_frac4(const uint32_t (&u)[4]) {
+    return min(min(u[0] & kFxFracMask, u[1] & kFxFracMask),
+               min(u[2] & kFxFracMask, u[3] & kFxFracMask));
+}
+
+// Queues the outputs among four whose fraction lies below the margin (entry
+// e0 + i * step for element i). Taken rarely; the compares are redone here
+// (the empty asm keeps the compiler from holding their masks across the
+// caller's common path).
+__device__ __forceinline__ void enqueue_flags4(uint32_t (&u)[4], uint32_t thr, uint32_t e0,
+                                               uint32_t step, uint16_t *q, int &qn) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        asm volatile("" : "+v"(u[i]));
+        const bool f = (u[i] & kFxFracMask) < thr;
+        enqueue_ballot(f, __ballot(f), e0 + i * step, q, qn);
+    }
+}
+
 typedef unsigned short v2u16 __attribute__((ext_vector_type(2)));
 
 // Bits kFxShift..kFxShift+7 of four accumulator bit patterns, packed into one
@@ -790,21 +811,15 @@ __device__ __forceinline__ void blur_u8_fx_body(const Jobs &jobs, int ntiles) {
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 uint32_t u[4];
-                bool f[4];
-                unsigned long long bal[4], any = 0;
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
                     // (a bit_cast of a vector element reads element 0 with this compiler)
                     u[i] = __float_as_uint(acc[4 * j + i] + acl[4 * j + i]);
-                    f[i] = (u[i] & kFxFracMask) < thr_h;
-                    bal[i] = __ballot(f[i]);
-                    any |= bal[i];
                 }
                 *reinterpret_cast<uint32_t *>(col + 8 * j) = pack_bits15(u[0], u[1], u[2], u[3]);
-                if (any) {
+                if (__ballot(min_frac4(u) < thr_h)) {
                     const uint32_t e0 = (uint32_t)((32 * rb + 8 * j + 4 * g) << 8) | (32 * cb + n);
-#pragma unroll
-                    for (int i = 0; i < 4; i++) enqueue_ballot(f[i], bal[i], e0 + (i << 8), queue, qn);
+                    enqueue_flags4(u, thr_h, e0, 1u << 8, queue, qn);
                 }
                 if ((j & 1) && qn > kFxQDrain) h_drain();
             }
@@ -859,20 +874,12 @@ __device__ __forceinline__ void blur_u8_fx_body(const Jobs &jobs, int ntiles) {
             for (int j = 0; j < 4; j++) {
                 const int xl = 32 * cb + 8 * j + 4 * g;
                 uint32_t u[4];
-                bool f[4];
-                unsigned long long bal[4], any = 0;
 #pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    u[i] = __float_as_uint(acc[4 * j + i] + acl[4 * j + i]);
-                    f[i] = (u[i] & kFxFracMask) < thr_v;
-                    bal[i] = __ballot(f[i]);
-                    any |= bal[i];
-                }
+                for (int i = 0; i < 4; i++) u[i] = __float_as_uint(acc[4 * j + i] + acl[4 * j + i]);
                 *reinterpret_cast<uint32_t *>(orow + xl) = pack_bits15(u[0], u[1], u[2], u[3]);
-                if (any) {
+                if (__ballot(min_frac4(u) < thr_v)) {
                     const uint32_t e0 = (uint32_t)(yl << 8) | (uint32_t)xl;
-#pragma unroll
-                    for (int i = 0; i < 4; i++) enqueue_ballot(f[i], bal[i], e0 + i, queue, qn);
+                    enqueue_flags4(u, thr_v, e0, 1u, queue, qn);
                 }
                 if ((j & 1) && qn > kFxQDrain) v_drain();
             }
