@@ -485,7 +485,8 @@ IPH_API iph_status iph_dev_sharpen_gaussian(const uint8_t *src, uint8_t *dst, ui
     }
     IPH_TRY(check_dims(w, h));
     if ((size_t)w * h == 0) return IPH_OK;
-    if (src == nullptr || dst == nullptr || ws == nullptr) {
+    if (src == nullptr || dst == nullptr ||
+        (ws == nullptr && sharpen_gaussian_workspace(w, h, (uint32_t)k.size()) > 0)) {
         set_error("null image, output or workspace");
         return IPH_EINVAL;
     }

@@ -92,6 +92,26 @@ __device__ __forceinline__ void stage_window(const P *__restrict__ img, int W, i
     }
 }
 
+// stage_window for a u8 image into an f32 window (the conversion is exact).
+__device__ __forceinline__ void stage_window_f32(const uint8_t *__restrict__ img, int W, int H,
+                                                 int oy, int ox, int R, int C, float *dst) {
+    const int total = R * C;
+    for (int base = threadIdx.x; base < total; base += 8 * (int)blockDim.x) {
+        uint32_t v[8];
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const int i = min(base + b * (int)blockDim.x, total - 1);
+            const int r = i / C, c = i - r * C;
+            v[b] = img[(size_t)clampi(oy + r, 0, H - 1) * W + clampi(ox + c, 0, W - 1)];
+        }
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const int i = base + b * (int)blockDim.x;
+            if (i < total) dst[i] = (float)v[b];
+        }
+    }
+}
+
 // ---- fused separable filter ------------------------------------------------
 //
 // One workgroup produces a TW x TH output tile of one image:
@@ -676,6 +696,93 @@ __device__ __forceinline__ uint32_t sharpen_one(float a1, float amount, uint32_t
     return v < 255.0f ? (v > 0.0f ? (uint32_t)v : 0u) : 255u;
 }
 
+// sharpen_gaussian in one pass over a kSepTW x kSepTH output tile
+// (sharpen.rs:27-38): the u8 window is staged once, as f32; the f32 blur of
+// that conversion runs as sep_fused_kernel<float, float> does on the converted
+// image (the same products and sums in the same order, so the same bits),
+// its horizontal pass kept as f32 in LDS; the combine reads the centre pixel
+// from the staged window.
+template <int LT>
+__global__ __launch_bounds__(256) void sharpen_fused_kernel(const uint8_t *__restrict__ src,
+                                                            uint8_t *__restrict__ dst, int W,
+                                                            int H, int Lrt,
+                                                            FusedTaps<float> taps,
+                                                            float amount) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int L = LT > 0 ? LT : Lrt;
+    const int R = kSepTH + L - 1;
+    const int C = kSepTW + L - 1;
+    float *in_s = reinterpret_cast<float *>(smem_raw);  // the window as f32 (exact)
+    float *mid_s = in_s + R * C;
+    const int half = L / 2;
+    const int x0 = blockIdx.x * kSepTW;
+    const int y0 = blockIdx.y * kSepTH;
+
+    stage_window_f32(src, W, H, y0 - half, x0 - half, R, C, in_s);
+    __syncthreads();
+    const float a1 = __fadd_rn(1.0f, amount);
+
+    if constexpr (LT > 0) {
+        // 4 outputs per item from 4 + LT - 1 window values held in registers
+        constexpr int NV = 4 + LT - 1;
+        for (int i = threadIdx.x; i < R * (kSepTW / 4); i += blockDim.x) {
+            const int r = i / (kSepTW / 4), c0 = 4 * (i - r * (kSepTW / 4));
+            const float *row = in_s + r * C + c0;
+            float x[NV];
+#pragma unroll
+            for (int k = 0; k < NV; k++) x[k] = row[k];
+            float acc[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                acc[j] = 0.0f;
+#pragma unroll
+                for (int t = 0; t < LT; t++) acc[j] = mac(acc[j], x[j + t], taps.h[t]);
+            }
+            *reinterpret_cast<float4 *>(mid_s + r * kSepTW + c0) =
+                make_float4(acc[0], acc[1], acc[2], acc[3]);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < (kSepTH / 4) * kSepTW; i += blockDim.x) {
+            const int r0 = 4 * (i / kSepTW), c = i % kSepTW;
+            const int gx = x0 + c;
+            if (gx >= W) continue;
+            float x[NV];
+#pragma unroll
+            for (int k = 0; k < NV; k++) x[k] = mid_s[(r0 + k) * kSepTW + c];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int gy = y0 + r0 + j;
+                float q = 0.0f;
+#pragma unroll
+                for (int t = 0; t < LT; t++) q = mac(q, x[j + t], taps.v[t]);
+                const uint32_t p = (uint32_t)in_s[(r0 + j + half) * C + c + half];
+                if (gy < H) dst[(size_t)gy * W + gx] = (uint8_t)sharpen_one(a1, amount, p, q);
+            }
+        }
+        return;
+    }
+
+    for (int i = threadIdx.x; i < R * kSepTW; i += blockDim.x) {
+        int r = i / kSepTW, c = i - r * kSepTW;
+        const float *row = in_s + r * C + c;
+        float acc = 0.0f;
+        for (int t = 0; t < L; t++) acc = mac(acc, row[t], taps.h[t]);
+        mid_s[i] = acc;
+    }
+    __syncthreads();
+
+    for (int i = threadIdx.x; i < kSepTH * kSepTW; i += blockDim.x) {
+        int r = i / kSepTW, c = i - r * kSepTW;
+        int gy = y0 + r, gx = x0 + c;
+        if (gy >= H || gx >= W) continue;
+        const float *col = mid_s + r * kSepTW + c;
+        float q = 0.0f;
+        for (int t = 0; t < L; t++) q = mac(q, col[t * kSepTW], taps.v[t]);
+        const uint32_t p = (uint32_t)in_s[(r + half) * C + c + half];
+        dst[(size_t)gy * W + gx] = (uint8_t)sharpen_one(a1, amount, p, q);
+    }
+}
+
 template <bool VEC>
 __global__ __launch_bounds__(256) void sharpen_combine_kernel(const uint8_t *__restrict__ p,
                                                               const float *__restrict__ q,
@@ -699,6 +806,7 @@ __global__ __launch_bounds__(256) void sharpen_combine_kernel(const uint8_t *__r
 }  // namespace
 
 size_t sharpen_gaussian_workspace(uint32_t w, uint32_t h, uint32_t taps) {
+    if (taps <= (uint32_t)kFusedMaxTaps) return 0;  // one fused pass
     WorkspacePlan wp;
     wp.take<float>((size_t)w * h);
     wp.take<float>((size_t)w * h);
@@ -711,6 +819,29 @@ iph_status launch_sharpen_gaussian(const uint8_t *src, uint8_t *dst, uint32_t w,
                                    size_t ws_bytes, hipStream_t s) {
     const size_t n = (size_t)w * h;
     if (n == 0) return IPH_OK;
+    if (n_taps <= (uint32_t)kFusedMaxTaps) {
+        FusedTaps<float> ft{};
+        copy_taps(ft.h, taps, n_taps);
+        copy_taps(ft.v, taps, n_taps);
+        const int L = (int)n_taps;
+        const size_t lds = sizeof(float) * ((size_t)(kSepTH + L - 1) * (kSepTW + L - 1) +
+                                            (size_t)(kSepTH + L - 1) * kSepTW);
+        const dim3 grid((w + kSepTW - 1) / kSepTW, (h + kSepTH - 1) / kSepTH);
+#define IPH_SHARP(LT)                                                                       \
+    sharpen_fused_kernel<LT><<<grid, 256, lds, s>>>(src, dst, (int)w, (int)h, L, ft, amount)
+        switch (L) {
+        case 3: IPH_SHARP(3); break;
+        case 5: IPH_SHARP(5); break;
+        case 7: IPH_SHARP(7); break;
+        case 9: IPH_SHARP(9); break;
+        case 11: IPH_SHARP(11); break;
+        case 13: IPH_SHARP(13); break;
+        case 17: IPH_SHARP(17); break;
+        default: IPH_SHARP(0); break;
+        }
+#undef IPH_SHARP
+        return launch_status("sharpen_fused_kernel");
+    }
     Workspace wsp(ws, ws_bytes);
     float *img = wsp.take<float>(n);
     float *smooth = wsp.take<float>(n);

@@ -246,6 +246,8 @@ IPH_API iph_status iph_dev_gradients(const uint8_t *src, uint32_t width, uint32_
                                      const int32_t *h_kernel, uint32_t hkw, uint32_t hkh,
                                      const int32_t *v_kernel, uint32_t vkw, uint32_t vkh,
                                      uint16_t *out, void *stream);
+/* 0 when the blur has at most 48 taps (sigma <= 11.5: one fused pass, the
+ * workspace may be NULL); else two f32 images plus the blur's workspace. */
 IPH_API size_t iph_dev_sharpen_gaussian_workspace(uint32_t width, uint32_t height, float sigma);
 IPH_API iph_status iph_dev_sharpen_gaussian(const uint8_t *src, uint8_t *dst, uint32_t width,
                                             uint32_t height, float sigma, float amount,

@@ -27,8 +27,13 @@ def test_robin_truths(fixtures):
     assert np.array_equal(flt.sharpen_gaussian(robin, 0.7, 7.0), fixtures["robin_sharpen_gaussian"])
 
 
+# sigma 3.5 gives 15 taps (the fused kernel's loop over any tap count),
+# sigma 13 gives 53 taps (the unfused path through f32 images); the others
+# take the fused kernel's 4-outputs-per-thread form
 @pytest.mark.parametrize("shape,sigma,amount", [((37, 53), 1.5, 2.0), ((300, 411), 0.7, 7.0),
-                                                ((128, 96), 4.0, 0.5), ((5, 200), 2.0, -1.0)])
+                                                ((128, 96), 4.0, 0.5), ((5, 200), 2.0, -1.0),
+                                                ((70, 1), 1.0, 3.0), ((90, 140), 13.0, 1.0),
+                                                ((65, 129), 0.4, 2.5), ((33, 70), 3.5, 1.0)])
 def test_sharpen_gaussian_random(shape, sigma, amount):
     img = np.random.default_rng(shape[1]).integers(0, 256, shape, dtype=np.uint8)
     assert np.array_equal(flt.sharpen_gaussian(img, sigma, amount),
@@ -49,8 +54,13 @@ def test_device_entry_matches_host_entry():
 
     from imageproc_amd import device
 
+    from imageproc_amd._lib import lib
+
+    # blurs of up to 48 taps run fused and need no workspace
+    assert lib().iph_dev_sharpen_gaussian_workspace(389, 257, 11.5) == 0
+    assert lib().iph_dev_sharpen_gaussian_workspace(389, 257, 13.0) >= 2 * 4 * 389 * 257
     img = np.random.default_rng(23).integers(0, 256, (257, 389), dtype=np.uint8)
-    for sigma, amount in [(0.7, 7.0), (2.0, 1.5)]:
+    for sigma, amount in [(0.7, 7.0), (2.0, 1.5), (13.0, 0.5)]:
         got = device.sharpen_gaussian(torch.from_numpy(img).cuda(), sigma, amount).cpu().numpy()
         assert np.array_equal(got, flt.sharpen_gaussian(img, sigma, amount))
         assert np.array_equal(got, _oracle_sharpen_gaussian(img, sigma, amount))
