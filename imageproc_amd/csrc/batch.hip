@@ -8,8 +8,9 @@
 // blur kernel (one launch per kernel variant, each walking the tiles of all
 // its images), the filter jobs one table of the ragged 2-D filter kernel (one
 // launch), the match jobs one int8-MFMA pipeline each with a workspace slice.
-// The three kinds run on three streams forked from the caller's stream and
-// joined back, captured once into a graph. The autotuner times the candidate
+// The blurs run on the caller's stream, the filters and the matches on side
+// streams forked from it (the matches spread over three, largest first onto
+// the least loaded) and joined back, captured once into a graph. The autotuner times the candidate
 // launch shapes of each kind on the real jobs (filter tile shape, rows of
 // 32 outputs per match workgroup, persistent blur workgroups per CU) and
 // keeps the fastest before capture.
@@ -55,9 +56,12 @@ struct iph_batch {
     int filt_tiles = 0;
     uint32_t filt_max_kw = 1, filt_max_kh = 1;
     int blur_grid_pct = 100, filt_variant = 0, match_rows = 0;
-    hipStream_t side[2] = {nullptr, nullptr};
+    // side[0]: the filters; side[1 .. kSide - 1]: the matches
+    static constexpr int kSide = 4;
+    hipStream_t side[kSide] = {};
     hipStream_t capture = nullptr;  // the graph is captured here (the caller's may be the null stream)
-    hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
+    hipEvent_t fork = nullptr, join[kSide] = {};
+    std::vector<int> match_lane;  // side stream (1 ..) of each match job
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::string tuning;
@@ -69,7 +73,7 @@ namespace {
 void unplan(iph_batch *b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    for (int i = 0; i < 2; i++) {
+    for (int i = 0; i < iph_batch::kSide; i++) {
         if (b->side[i]) (void)hipStreamDestroy(b->side[i]);
         if (b->join[i]) (void)hipEventDestroy(b->join[i]);
         b->side[i] = nullptr;
@@ -103,12 +107,15 @@ iph_status enqueue_filters(iph_batch *b, hipStream_t s) {
                               b->filt_variant, b->filt_max_kw, b->filt_max_kh, s);
 }
 
-iph_status enqueue_matches(iph_batch *b, hipStream_t s) {
+// The match jobs; with `spread`, each on its side stream, else all on s.
+iph_status enqueue_matches(iph_batch *b, hipStream_t s, bool spread = false) {
     set_match_rows_hint(b->match_rows);
-    for (const MatchSpec &m : b->match) {
+    for (size_t j = 0; j < b->match.size(); j++) {
+        const MatchSpec &m = b->match[j];
+        hipStream_t ms = spread ? b->side[b->match_lane[j]] : s;
         iph_status st = launch_match(m.img, IPH_U8, m.w, m.h, m.tpl, m.tw, m.th, m.method,
                                      resolve_path(IPH_PATH_AUTO, m.tw, m.th), m.out,
-                                     b->mem + m.ws_off, m.ws_bytes, s);
+                                     b->mem + m.ws_off, m.ws_bytes, ms);
         if (st != IPH_OK) {
             set_match_rows_hint(0);
             return st;
@@ -118,15 +125,16 @@ iph_status enqueue_matches(iph_batch *b, hipStream_t s) {
     return IPH_OK;
 }
 
-// All jobs: blurs on `s`, filters and matches on the two side streams forked
+// All jobs: blurs on `s`, filters and matches on the side streams forked
 // from `s` and joined back into it.
 iph_status enqueue_all(iph_batch *b, hipStream_t s) {
     IPH_TRY_HIP(hipEventRecord(b->fork, s));
-    for (int i = 0; i < 2; i++) IPH_TRY_HIP(hipStreamWaitEvent(b->side[i], b->fork, 0));
+    for (int i = 0; i < iph_batch::kSide; i++)
+        IPH_TRY_HIP(hipStreamWaitEvent(b->side[i], b->fork, 0));
     IPH_TRY(enqueue_blur(b, s));
     IPH_TRY(enqueue_filters(b, b->side[0]));
-    IPH_TRY(enqueue_matches(b, b->side[1]));
-    for (int i = 0; i < 2; i++) {
+    IPH_TRY(enqueue_matches(b, s, true));
+    for (int i = 0; i < iph_batch::kSide; i++) {
         IPH_TRY_HIP(hipEventRecord(b->join[i], b->side[i]));
         IPH_TRY_HIP(hipStreamWaitEvent(s, b->join[i], 0));
     }
@@ -264,7 +272,26 @@ iph_status plan(iph_batch *b, int32_t tune, hipStream_t s) {
     IPH_TRY_HIP(hipMemcpy(b->mem + b->blur_table_off, blur_host.data(), blur_host.size(),
                           hipMemcpyHostToDevice));
     IPH_TRY(upload_filter_table(b, 0));
-    for (int i = 0; i < 2; i++) {
+    // match jobs to the match streams: largest first, each to the least loaded
+    {
+        const int nl = iph_batch::kSide - 1;
+        std::vector<size_t> order(b->match.size());
+        std::vector<double> cost(b->match.size()), load(nl, 0.0);
+        for (size_t j = 0; j < order.size(); j++) {
+            const MatchSpec &m = b->match[j];
+            order[j] = j;
+            cost[j] = (double)(m.w - m.tw + 1) * (m.h - m.th + 1) * ((double)m.tw * m.th + 64.0);
+        }
+        std::stable_sort(order.begin(), order.end(),
+                         [&](size_t a, size_t c) { return cost[a] > cost[c]; });
+        b->match_lane.assign(b->match.size(), 1);
+        for (size_t j : order) {
+            const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            b->match_lane[j] = 1 + l;
+            load[l] += cost[j];
+        }
+    }
+    for (int i = 0; i < iph_batch::kSide; i++) {
         IPH_TRY_HIP(hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking));
         IPH_TRY_HIP(hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming));
     }
