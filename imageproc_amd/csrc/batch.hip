@@ -9,7 +9,7 @@
 // its images), the filter jobs one table of the ragged 2-D filter kernel (one
 // launch), the match jobs one int8-MFMA pipeline each with a workspace slice.
 // The blurs run on the caller's stream, the filters and the matches on side
-// streams forked from it (the matches spread over three, largest first onto
+// streams forked from it (the matches spread over seven, largest first onto
 // the least loaded) and joined back, captured once into a graph. The autotuner times the candidate
 // launch shapes of each kind on the real jobs (filter tile shape, rows of
 // 32 outputs per match workgroup, persistent blur workgroups per CU) and
@@ -57,7 +57,7 @@ struct iph_batch {
     uint32_t filt_max_kw = 1, filt_max_kh = 1;
     int blur_grid_pct = 100, filt_variant = 0, match_rows = 0;
     // side[0]: the filters; side[1 .. kSide - 1]: the matches
-    static constexpr int kSide = 4;
+    static constexpr int kSide = 8;
     hipStream_t side[kSide] = {};
     hipStream_t capture = nullptr;  // the graph is captured here (the caller's may be the null stream)
     hipEvent_t fork = nullptr, join[kSide] = {};
