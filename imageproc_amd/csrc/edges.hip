@@ -444,12 +444,12 @@ iph_status launch_canny(const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h
         canny_nms_kernel<false><<<grid, 256, 0, s>>>(blur, (int)w, (int)h, tiles_x, ntiles, n_low,
                                                      n_high, state, tfl[0]);
     IPH_TRY(launch_status("canny_nms_kernel"));
-    canny_zero_flags_kernel<<<1, 64, 0, s>>>(border, 1);
-    IPH_TRY(launch_status("canny_zero_flags_kernel"));
     // Hysteresis: launches in groups of kCannyIters, each with its own flag;
-    // the group's last flag clear means a launch changed nothing.
+    // the group's last flag clear means a launch changed nothing. The flags
+    // and the border flag after them are cleared together before the first
+    // group; one read-back per group returns the last flag and the border flag.
     for (int launch = 0;;) {
-        canny_zero_flags_kernel<<<1, 64, 0, s>>>(flags, kCannyIters);
+        canny_zero_flags_kernel<<<1, 64, 0, s>>>(flags, launch == 0 ? kCannyIters + 1 : kCannyIters);
         IPH_TRY(launch_status("canny_zero_flags_kernel"));
         for (int i = 0; i < kCannyIters; i++, launch++) {
             const unsigned hgrid = (unsigned)std::min(hyst_tiles, kCannyGrid);
@@ -463,11 +463,16 @@ iph_status launch_canny(const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h
                     tfl[(launch + 1) & 1], flags + i, border);
             IPH_TRY(launch_status("canny_hysteresis_kernel"));
         }
-        int last = 0;
-        IPH_TRY_HIP(hipMemcpyAsync(&last, flags + kCannyIters - 1, sizeof(int),
+        int last_border[2] = {0, 0};  // flags[kCannyIters - 1], *border
+        IPH_TRY_HIP(hipMemcpyAsync(last_border, flags + kCannyIters - 1, 2 * sizeof(int),
                                    hipMemcpyDeviceToHost, s));
         IPH_TRY_HIP(hipStreamSynchronize(s));
-        if (last == 0) break;
+        if (last_border[1]) {
+            set_error("canny: hysteresis reached the image border, where the reference's "
+                      "search indexes outside the image and panics (low_threshold <= 0)");
+            return IPH_EINVAL;
+        }
+        if (last_border[0] == 0) break;
     }
     const bool vec = n % 16 == 0 && (reinterpret_cast<uintptr_t>(dst) & 15) == 0;
     const unsigned fgrid = (unsigned)std::min<size_t>((vec ? n / 16 : n) / 256 + 1, 1u << 16);
@@ -476,14 +481,6 @@ iph_status launch_canny(const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h
     else
         canny_finish_kernel<false><<<fgrid, 256, 0, s>>>(state, n, dst);
     IPH_TRY(launch_status("canny_finish_kernel"));
-    int hit = 0;
-    IPH_TRY_HIP(hipMemcpyAsync(&hit, border, sizeof(int), hipMemcpyDeviceToHost, s));
-    IPH_TRY_HIP(hipStreamSynchronize(s));
-    if (hit) {
-        set_error("canny: hysteresis reached the image border, where the reference's search "
-                  "indexes outside the image and panics (low_threshold <= 0)");
-        return IPH_EINVAL;
-    }
     return IPH_OK;
 }
 
