@@ -141,10 +141,13 @@ __global__ __launch_bounds__(256) void canny_nms_kernel(const uint8_t *__restric
                 const int lx = c0 + u, x = X0 + lx;
                 int thin = 0;
                 const bool interior = x >= 1 && y >= 1 && x + 1 < W && y + 1 < H;
-                if (interior) {
+                const int cy = r + 1, cx = lx + 1;
+                const int m = nn[cy][cx];
+                // below the low threshold the class is 0 whether or not the
+                // output is suppressed (n_low <= n_high): no direction needed
+                if (interior && m >= n_low) {
                     const int p = gxy[r][lx];
                     const int gx = (int)(int16_t)(p & 0xffff), gy = p >> 16;
-                    const int cy = r + 1, cx = lx + 1;
                     // the two neighbours across the edge (edges.rs:84-99) sit
                     // at +(dy, dx) and -(dy, dx): 0 -> (0, -1), 45 -> (1, 1),
                     // 90 -> (-1, 0), 135 -> (1, -1); selected without branches
@@ -152,7 +155,6 @@ __global__ __launch_bounds__(256) void canny_nms_kernel(const uint8_t *__restric
                     const int dy = dir == 0 ? 0 : (dir == 90 ? -1 : 1);
                     const int dx = dir == 45 ? 1 : (dir == 90 ? 0 : -1);
                     const int c1 = nn[cy + dy][cx + dx], c2 = nn[cy - dy][cx - dx];
-                    const int m = nn[cy][cx];
                     thin = (m < c1 || m < c2) ? 0 : m;
                 }
                 const bool sd = interior && thin >= n_high;
