@@ -876,14 +876,14 @@ struct GradTaps {
     int hw, hh, vw, vh;
 };
 
-// One 64 x 16 output tile per workgroup: the image window the two kernels
+// One 64 x 32 output tile per workgroup: the image window the two kernels
 // read (clamp-to-edge coordinates, filter/mod.rs:29-69, centre (kw/2, kh/2))
 // staged once in LDS, then per output both filter_clamped::<_, i32, i16>
 // sums (exact in i32, so their order is free) saturated to i16
 // (definitions.rs:181-188), and out = (dx^2 + dy^2).sqrt() as u16
 // (gradient_magnitude, gradients.rs:157-160): f32 squares and sum each rounded
 // once, correctly rounded sqrt, Rust's saturating float -> u16 cast.
-constexpr int kGradTW = 64, kGradTH = 16;
+constexpr int kGradTW = 64, kGradTH = 32;
 constexpr int kGradSW = kGradTW + kGradMaxTaps - 1, kGradSH = kGradTH + kGradMaxTaps - 1;
 
 // K3: both kernels 3 x 3 (loops unrolled, taps in scalar registers); M24:
