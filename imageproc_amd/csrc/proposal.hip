@@ -369,30 +369,60 @@ __global__ __launch_bounds__(256) void tile_max_kernel(const float *__restrict__
 constexpr int kChunk = 1024;
 constexpr int kMaxCand = 256;
 
+// Lane l's value of lane l ^ S: DPP moves inside a row of 16 lanes for
+// S = 1, 2, 4 (row half mirror = xor 7, then quad reverse = xor 3) and 8
+// (row rotate by 8), a shuffle otherwise.
+template <int S> __device__ __forceinline__ uint32_t xor_lane(uint32_t v) {
+    if constexpr (S == 1) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);
+    else if constexpr (S == 2) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);
+    else if constexpr (S == 4) {
+        const int t = __builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false);
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, t, 0x1B, 0xF, 0xF, false);
+    } else if constexpr (S == 8) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, false);
+    else return __shfl_xor(v, S);
+}
+template <int S> __device__ __forceinline__ uint64_t xor_lane_key(uint64_t v) {
+    return ((uint64_t)xor_lane<S>((uint32_t)(v >> 32)) << 32) | xor_lane<S>((uint32_t)v);
+}
+
+// One bitonic stage of sort1024: pair (tid, tid ^ STRIDE) inside the
+// size-SIZE blocks.
+template <int SIZE, int STRIDE>
+__device__ __forceinline__ uint64_t bitonic_stage(uint64_t key, uint64_t *xs, int tid, int &flip) {
+    uint64_t o;
+    if constexpr (STRIDE >= 64) {
+        uint64_t *buf = xs + flip * kChunk;
+        flip ^= 1;
+        buf[tid] = key;
+        __syncthreads();
+        o = buf[tid ^ STRIDE];
+    } else {
+        o = xor_lane_key<STRIDE>(key);
+    }
+    const bool keep_max = ((tid & STRIDE) == 0) == ((tid & SIZE) == 0);
+    return keep_max ? max_key(key, o) : (key < o ? key : o);
+}
+template <int SIZE, int STRIDE>
+__device__ __forceinline__ uint64_t bitonic_merge(uint64_t key, uint64_t *xs, int tid, int &flip) {
+    key = bitonic_stage<SIZE, STRIDE>(key, xs, tid, flip);
+    if constexpr (STRIDE > 1) key = bitonic_merge<SIZE, STRIDE / 2>(key, xs, tid, flip);
+    return key;
+}
+template <int SIZE>
+__device__ __forceinline__ uint64_t bitonic_sort(uint64_t key, uint64_t *xs, int tid, int &flip) {
+    key = bitonic_merge<SIZE, SIZE / 2>(key, xs, tid, flip);
+    if constexpr (SIZE < kChunk) key = bitonic_sort<SIZE * 2>(key, xs, tid, flip);
+    return key;
+}
+
 // Sorts the kChunk = 1024 keys of a 1024-thread workgroup (thread t holds
 // position t), largest first, with a bitonic network: pairs less than 64
-// positions apart are exchanged between the lanes of a wave (shuffles),
-// farther ones through LDS (xs: 2 x 1024 keys, alternating so one barrier
-// per such stage suffices).
+// positions apart are exchanged between the lanes of a wave (DPP moves or
+// shuffles), farther ones through LDS (xs: 2 x 1024 keys, alternating so
+// one barrier per such stage suffices).
 __device__ __forceinline__ uint64_t sort1024(uint64_t key, uint64_t *xs, int tid) {
     int flip = 0;
-    for (int size = 2; size <= kChunk; size <<= 1) {
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            uint64_t o;
-            if (stride >= 64) {
-                uint64_t *buf = xs + flip * kChunk;
-                flip ^= 1;
-                buf[tid] = key;
-                __syncthreads();
-                o = buf[tid ^ stride];
-            } else {
-                o = shfl_xor_key(key, stride);
-            }
-            const bool keep_max = ((tid & stride) == 0) == ((tid & size) == 0);
-            key = keep_max ? max_key(key, o) : (key < o ? key : o);
-        }
-    }
-    return key;
+    return bitonic_sort<2>(key, xs, tid, flip);
 }
 
 // Chunk-wise top-k of n keys: workgroup b sorts keys [b * kChunk, ...) and
