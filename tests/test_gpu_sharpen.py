@@ -33,7 +33,8 @@ def test_robin_truths(fixtures):
 @pytest.mark.parametrize("shape,sigma,amount", [((37, 53), 1.5, 2.0), ((300, 411), 0.7, 7.0),
                                                 ((128, 96), 4.0, 0.5), ((5, 200), 2.0, -1.0),
                                                 ((70, 1), 1.0, 3.0), ((90, 140), 13.0, 1.0),
-                                                ((65, 129), 0.4, 2.5), ((33, 70), 3.5, 1.0)])
+                                                ((65, 129), 0.4, 2.5), ((33, 70), 3.5, 1.0),
+                                                ((50, 77), 2.5, 1.0), ((41, 90), 3.0, 2.0)])
 def test_sharpen_gaussian_random(shape, sigma, amount):
     img = np.random.default_rng(shape[1]).integers(0, 256, shape, dtype=np.uint8)
     assert np.array_equal(flt.sharpen_gaussian(img, sigma, amount),
