@@ -1,5 +1,6 @@
 """Build step for the gfx950 library: compiles imageproc_amd/csrc/*.hip into
 imageproc_amd/libimageproc_hip.so with hipcc (cross-compiles without a GPU).
+Each source is compiled to its own object in parallel, then linked.
 
     python -m imageproc_amd.build
 """
@@ -7,8 +8,11 @@ from __future__ import annotations
 
 import glob
 import os
+import shutil
 import subprocess
 import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -19,7 +23,6 @@ HIPCC_FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     "-fvisibility=hidden",
     # Parity kernels: never contract a*b+c into an FMA behind the source's back.
     "-ffp-contract=off",
@@ -32,18 +35,35 @@ HIPCC_FLAGS = [
 ]
 
 
-def build(verbose: bool = False) -> str:
-    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    srcs = sorted(glob.glob(os.path.join(HERE, "csrc", "*.hip")))
-    tmp = LIB + ".tmp"
-    cmd = [hipcc, *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", tmp, *srcs]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libimageproc_hip.so")
-    os.replace(tmp, LIB)
+    return r.returncode
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> str:
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    srcs = sorted(glob.glob(os.path.join(HERE, "csrc", "*.hip")))
+    inc = ["-I", os.path.join(ROOT, "include")]
+    tmpdir = tempfile.mkdtemp(prefix="iph_build_")
+    try:
+        objs = [os.path.join(tmpdir, os.path.basename(s) + ".o") for s in srcs]
+        cmds = [[hipcc, *HIPCC_FLAGS, *inc, "-c", "-o", o, s] for s, o in zip(srcs, objs)]
+        n = jobs or min(len(cmds), max(1, min(os.cpu_count() or 1, 16)))
+        with ThreadPoolExecutor(max_workers=n) as ex:
+            codes = list(ex.map(lambda c: _run(c, verbose), cmds))
+        if any(codes):
+            bad = [os.path.basename(s) for s, c in zip(srcs, codes) if c]
+            raise RuntimeError(f"hipcc failed on {', '.join(bad)}")
+        tmp = LIB + ".tmp"
+        if _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs], verbose):
+            raise RuntimeError("hipcc failed linking libimageproc_hip.so")
+        os.replace(tmp, LIB)
+    finally:
+        shutil.rmtree(tmpdir, ignore_errors=True)
     return LIB
 
 

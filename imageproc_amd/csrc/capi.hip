@@ -175,9 +175,13 @@ iph_status check_match(int32_t p, uint32_t w, uint32_t h, uint32_t tw, uint32_t 
 }
 
 // Empty templates have no window to correlate: the direct path handles them.
+// AUTO takes the MFMA engines where they apply (template sides up to
+// kMfmaMaxSide) and the direct kernel otherwise, so every template the
+// reference accepts (template_matching.rs:394-404) has a GPU path.
 int32_t resolve_path(int32_t path, uint32_t tw, uint32_t th) {
     if (tw == 0 || th == 0) return IPH_PATH_DIRECT;
-    return path == IPH_PATH_AUTO ? IPH_PATH_MFMA : path;
+    if (path != IPH_PATH_AUTO) return path;
+    return (tw <= kMfmaMaxSide && th <= kMfmaMaxSide) ? IPH_PATH_MFMA : IPH_PATH_DIRECT;
 }
 
 }  // namespace iph

@@ -136,6 +136,8 @@ int32_t gaussian_kernel(float sigma, std::vector<float> &k);
 iph_status check_dims(uint32_t w, uint32_t h);
 iph_status check_match(int32_t p, uint32_t w, uint32_t h, uint32_t tw, uint32_t th,
                        int32_t method, int32_t path);
+// Largest template side of the MFMA match engines (u8 and f32).
+constexpr uint32_t kMfmaMaxSide = 128;
 int32_t resolve_path(int32_t path, uint32_t tw, uint32_t th);
 
 // ---- batches of uneven images (filter.hip, blur_mfma.hip, batch.hip) ------

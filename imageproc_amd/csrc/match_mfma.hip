@@ -34,7 +34,7 @@ namespace {
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
-constexpr int kMaxSide = 128;  // largest template side on the int8 path
+constexpr int kMaxSide = (int)kMfmaMaxSide;  // largest template side on the int8 path
 static_assert(kMaxSide * 255 < 65536, "a template row's pixel sum fits 16 bits");
 constexpr int kWinRows = 64;   // output rows per window-sum workgroup
 constexpr int kWinCols = 64;   // output columns per window-sum workgroup
@@ -546,7 +546,7 @@ iph_status launch_bf16_kb(const float *img, int W, int H, const uint16_t *bsw, c
     return launch_status("match_bf16x3_mfma_kernel");
 }
 
-constexpr int kMaxSideF32 = 128;
+constexpr int kMaxSideF32 = (int)kMfmaMaxSide;
 
 int blocks16_for(int tw) { return (31 + tw + 15) / 16; }
 

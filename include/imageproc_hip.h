@@ -57,8 +57,12 @@ enum {
 
 /* Correlation engine for match_template. */
 enum {
-    IPH_PATH_AUTO = 0,    /* MFMA path */
-    IPH_PATH_MFMA = 1,    /* u8: int8 MFMA, exact i32 sums; f32: BF16x3 MFMA, FP32 accumulation */
+    IPH_PATH_AUTO = 0,    /* MFMA path for templates up to 128 x 128, else the direct path:
+                             every template the reference accepts (no larger than the
+                             image, template_matching.rs:394-404) runs */
+    IPH_PATH_MFMA = 1,    /* u8: int8 MFMA, exact i32 sums; f32: BF16x3 MFMA, FP32
+                             accumulation; templates up to 128 x 128 (larger ->
+                             IPH_EUNSUPPORTED) */
     IPH_PATH_DIRECT = 2   /* FP32 per-window loop in the reference's order (bit-exact) */
 };
 

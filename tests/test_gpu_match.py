@@ -210,3 +210,33 @@ def test_bf16x3_argmax_matches_direct_on_smooth_field(rng):
     d = match_template(img, tpl, M.CrossCorrelationNormalized, "direct")
     assert np.argmax(a) == np.argmax(d)
     assert np.unravel_index(np.argmax(a), a.shape) == (y, x)
+
+
+# ---- default path for every template the reference accepts ----------------------------
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.float32])
+@pytest.mark.parametrize("shape,tshape", [((90, 300), (40, 129)), ((230, 70), (200, 40)),
+                                          ((220, 240), (200, 200)), ((129, 129), (129, 129))])
+def test_auto_path_takes_templates_above_128(rng, dtype, shape, tshape):
+    """template_matching.rs:394-404 accepts any template no larger than the
+    image: the default path runs templates with a side above 128 on the direct
+    kernel, bit-identical to the oracle, and the MFMA path refuses them."""
+    if dtype == np.uint8:
+        img = rng.integers(0, 256, shape, dtype=np.uint8)
+        tpl = rng.integers(0, 256, tshape, dtype=np.uint8)
+    else:
+        img = rng.standard_normal(shape).astype(F)
+        tpl = rng.standard_normal(tshape).astype(F)
+    for m in M:
+        _bits_equal(match_template(img, tpl, m), O.match_template(img, tpl, int(m), nthreads=8))
+    with pytest.raises(TypeError, match="MFMA path takes templates up to 128x128"):
+        match_template(img, tpl, M.CrossCorrelation, "mfma")
+
+
+@pytest.mark.parametrize("tshape", [(128, 128), (128, 5), (3, 128)])
+def test_auto_path_keeps_mfma_up_to_128(rng, tshape):
+    img = rng.integers(0, 256, (150, 170), dtype=np.uint8)
+    tpl = rng.integers(0, 256, tshape, dtype=np.uint8)
+    for m in M:
+        np.testing.assert_array_equal(match_template(img, tpl, m),
+                                      match_template(img, tpl, m, "mfma"))
