@@ -194,6 +194,8 @@ _SIGS = {
     "iph_dev_find_extremes_workspace": (C.c_size_t, []),
     "iph_dev_find_extremes": (C.c_int32, [C.c_void_p, C.c_int32, C.c_uint32, C.c_uint32,
                                           C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "iph_dev_find_extremes_noalloc": (C.c_int32, [C.c_void_p, C.c_int32, C.c_uint32,
+                                                  C.c_uint32, C.c_void_p, C.c_void_p]),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -796,4 +796,23 @@ IPH_API iph_status iph_dev_find_extremes(const void *image, int32_t p, uint32_t 
     return launch_extremes(image, p, w, h, out, ws, wsb, static_cast<hipStream_t>(stream));
 }
 
+IPH_API iph_status iph_dev_find_extremes_noalloc(const void *image, int32_t p, uint32_t w,
+                                                 uint32_t h, iph_extremes *out, void *stream) {
+    clear_error();
+    if (w == 0 || h == 0) {
+        set_error("image must be non-empty");
+        return IPH_EINVAL;
+    }
+    if (p != IPH_U8 && p != IPH_F32) {
+        set_error("find_extremes is built for Luma<u8> and Luma<f32>, not type %d", p);
+        return IPH_EUNSUPPORTED;
+    }
+    if (image == nullptr || out == nullptr) {
+        set_error("null image or result");
+        return IPH_EINVAL;
+    }
+    IPH_TRY(require_device());
+    return launch_extremes_noalloc(image, p, w, h, out, static_cast<hipStream_t>(stream));
+}
+
 }  // extern "C"

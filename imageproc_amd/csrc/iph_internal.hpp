@@ -199,5 +199,9 @@ iph_status launch_masked_match(const uint8_t *img, uint32_t w, uint32_t h, const
 size_t extremes_workspace();
 iph_status launch_extremes(const void *img, int32_t p_type, uint32_t w, uint32_t h,
                            iph_extremes *out, void *ws, size_t ws_bytes, hipStream_t s);
+// find_extremes with its key pairs in the library's own fixed device array
+// (no allocation); calls that overlap on one device race.
+iph_status launch_extremes_noalloc(const void *img, int32_t p_type, uint32_t w, uint32_t h,
+                                   iph_extremes *out, hipStream_t s);
 
 }  // namespace iph

@@ -13,6 +13,8 @@
 
 #include "match_common.hpp"
 
+#include <mutex>
+
 namespace iph {
 namespace {
 
@@ -349,6 +351,60 @@ iph_status launch_masked_match(const uint8_t *img, uint32_t w, uint32_t h, const
 }
 
 size_t extremes_workspace() { return (size_t)kExtMaxBlocks * 2 * sizeof(unsigned long long); }
+
+// ---- find_extremes without allocation -------------------------------------------
+//
+// The per-workgroup key pairs of find_extremes live in a fixed device array
+// (kExtMaxBlocks pairs, 16 KiB per device, held until the process ends), so
+// the caller passes only the image and the result. The reduction grid never
+// exceeds kExtMaxBlocks workgroups, so every image up to 2^32 - 1 pixels fits.
+// Each workgroup writes its pair before the finalize kernel reads it: nothing
+// needs clearing between calls. All calls on one device share the array, so
+// calls that overlap on one device (other streams, other threads) race.
+static __device__ __align__(256) unsigned long long scratch[2 * kExtMaxBlocks];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current (each device holds its own copy).
+iph_status scratch_address(void **ptr) {
+    int dev = 0;
+    IPH_TRY_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("device %d is beyond the %d devices the scratch table tracks", dev,
+                  kMaxDevices);
+        return IPH_EDEVICE;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            set_error("hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                      hipGetErrorString(e));
+            return IPH_EDEVICE;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return IPH_OK;
+}
+}  // namespace
+
+iph_status launch_extremes_noalloc(const void *img, int32_t p_type, uint32_t w, uint32_t h,
+                                   iph_extremes *out, hipStream_t s) {
+    if ((uint64_t)w * h >= (1ull << 32)) {
+        set_error("image of %ux%u pixels exceeds the 2^32 - 1 pixels find_extremes indexes",
+                  w, h);
+        return IPH_EINVAL;
+    }
+    void *ptr = nullptr;
+    IPH_TRY(scratch_address(&ptr));
+    return launch_extremes(img, p_type, w, h, out, ptr, sizeof(scratch), s);
+}
 
 iph_status launch_extremes(const void *img, int32_t p_type, uint32_t w, uint32_t h,
                            iph_extremes *out, void *ws, size_t ws_bytes, hipStream_t s) {

@@ -281,6 +281,19 @@ IPH_API iph_status iph_dev_find_extremes(const void *image, int32_t p_type, uint
                                          uint32_t height, iph_extremes *out, void *workspace,
                                          size_t workspace_bytes, void *stream);
 
+/* find_extremes on device memory without allocating: no counterpart in the
+ * reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * per-workgroup partial results live in the library's own 16 KiB device
+ * array (one per device, held until the process ends); every image the
+ * workspace entry above takes (up to 2^32 - 1 pixels) fits, larger ->
+ * IPH_EINVAL. Calls no hipMalloc/hipFree. All calls on one device share the
+ * array: calls that overlap on one device (other streams or threads) race.
+ * Same result as iph_dev_find_extremes; *out is device memory. */
+IPH_API iph_status iph_dev_find_extremes_noalloc(const void *image, int32_t p_type,
+                                                 uint32_t width, uint32_t height,
+                                                 iph_extremes *out, void *stream);
+
 /*
  * Batches of uneven GrayImages. The reference has no batch API: a batch
  * replaces a loop of per-image calls of gaussian_blur_f32
