@@ -453,15 +453,7 @@ constexpr int kRows2 = 160;         // staged rows: kTH2 + 32 covers L <= 33
 constexpr int kPitchB = kRows2 + 8; // bytes per intermediate column (42 dwords)
 constexpr int kOPitch = kTW + 16;   // bytes per output-tile row
 
-// The A operand is fed as 1024 + p (the byte-to-f16 permute without the
-// subtraction); taps are scaled by 2^13 so that every partial sum of the
-// integer hi products (1024 + p) * hi stays below 2^24 in magnitude, and the
-// 1024 * sum(hi) offset is folded into the accumulator's start value.
-#ifndef IPH_FX_BIASED_A
-#define IPH_FX_BIASED_A 0
-#endif
-constexpr bool kFxBiasedA = IPH_FX_BIASED_A;
-constexpr int kFxShift = kFxBiasedA ? 13 : 15;  // estimates in units of 2^-kFxShift
+constexpr int kFxShift = 15;  // estimates in units of 2^-kFxShift
 constexpr uint32_t kFxFracMask = (1u << kFxShift) - 1u;
 
 // One blur job of the fixed-point kernel: n_img images of W x H (rows W bytes
@@ -493,20 +485,7 @@ struct FxJobTable {
     __device__ __forceinline__ const FxJob &operator[](int i) const { return p[i]; }
 };
 
-// 8 u8 pixels -> 8 f16 values 1024 + p (exact).
-__device__ __forceinline__ v8h bytes_to_half8_biased(uint2 b) {
-    const uint32_t magic = 0x64646464u;
-    uint32_t q[4] = {__builtin_amdgcn_perm(magic, b.x, 0x04010400u),
-                     __builtin_amdgcn_perm(magic, b.x, 0x04030402u),
-                     __builtin_amdgcn_perm(magic, b.y, 0x04010400u),
-                     __builtin_amdgcn_perm(magic, b.y, 0x04030402u)};
-    return __builtin_bit_cast(v8h, q);
-}
-
-__device__ __forceinline__ v8h fx_a_operand(uint2 b) {
-    if constexpr (kFxBiasedA) return bytes_to_half8_biased(b);
-    else return bytes_to_half8(b);
-}
+__device__ __forceinline__ v8h fx_a_operand(uint2 b) { return bytes_to_half8(b); }
 
 // Toeplitz B operand (see build_b) with taps * 2^kFxShift split into an
 // integer hi and an f16 lo.
@@ -952,10 +931,8 @@ float blur_eps(const float *w, int L, int nkb) {
 // Readout margin c of the fixed-point kernel (units of 2^-kFxShift) and its
 // accumulator start values, or c = -1 when the taps do not qualify. c bounds
 // |E - R| in units, E the kernel's estimate and R the reference's f32 result:
-//   * the hi accumulator is exact; it starts at 2^23 + c - 1024 * sum(hi)
-//     + rint(-1024 * sum(lo)) (the A operand's offset, folded in as an
-//     integer: at most half a unit off); every partial sum of the lo
-//     accumulator (started at 0) stays below B_lo = max(A) * sum|lo|, so each
+//   * the hi accumulator is exact; it starts at 2^23 + c; every partial sum
+//     of the lo accumulator (started at 0) stays below B_lo = max(A) * sum|lo|, so each
 //     of its at most 2L + 2*NKB roundings costs at most B_lo * 2^-23; the
 //     final f32 add rounds once (half a unit);
 //   * the split leaves |w * 2^kFxShift - hi - lo| per tap, times a pixel
@@ -973,8 +950,8 @@ struct FxPlan {
 
 FxPlan blur_fx_plan(const float *w, int L, int nkb) {
     const double unit = std::ldexp(1.0, kFxShift);
-    const double amax = kFxBiasedA ? 1279.0 : 255.0;  // largest A operand
-    double sum = 0.0, sum_hi = 0.0, sum_lo = 0.0, lo_abs = 0.0, split = 0.0, chain = 0.0;
+    const double amax = 255.0;  // largest A operand
+    double sum = 0.0, sum_hi = 0.0, lo_abs = 0.0, split = 0.0, chain = 0.0;
     for (int i = 0; i < L; i++) {
         if (!(w[i] >= 0.0f) || !std::isfinite(w[i])) return {-1, 0.0f};
         const float ws = w[i] * (float)unit;
@@ -982,23 +959,17 @@ FxPlan blur_fx_plan(const float *w, int L, int nkb) {
         const _Float16 lo = (_Float16)(ws - (float)hi);
         sum += (double)w[i];
         sum_hi += (double)(float)hi;
-        sum_lo += (double)(float)lo;
         lo_abs += std::fabs((double)(float)lo);
         double res = std::fabs((double)ws - (double)(float)hi - (double)(float)lo);
         if (std::fabs((double)(float)lo) < std::ldexp(1.0, -14)) res += std::fabs((double)(float)lo);
         split += res;
         chain += (double)w[i] * (1.0 + (double)(L - std::max(i, 1)));
     }
-    // offset of the biased A operand: 1024 * (sum(hi) + sum(lo)), the lo part
-    // rounded to an integer (error at most half a unit)
-    const double off_hi = kFxBiasedA ? 1024.0 * sum_hi : 0.0;
-    const double off_lo = kFxBiasedA ? std::rint(1024.0 * sum_lo) : 0.0;
     const double b_lo = amax * lo_abs;
-    double e = 0.5 + (2.0 * L + 2.0 * nkb) * b_lo * std::ldexp(1.0, -23) +
-               (kFxBiasedA ? std::fabs(off_lo - 1024.0 * sum_lo) : 0.0) + 255.0 * split +
+    double e = 0.5 + (2.0 * L + 2.0 * nkb) * b_lo * std::ldexp(1.0, -23) + 255.0 * split +
                unit * 255.0 * chain * std::ldexp(1.0, -24) * 1.01;
     const double c = std::ceil(1.25 * e + 1.0);
-    const double init_hi = 8388608.0 + c - off_hi - off_lo;
+    const double init_hi = 8388608.0 + c;
     // the hi accumulator is exact: every partial sum is an integer of
     // magnitude below 2^24; the final value stays in [2^23, 2^24) with the
     // estimate's integer part in 8 bits
@@ -1007,6 +978,401 @@ FxPlan blur_fx_plan(const float *w, int L, int nkb) {
         return {-1, 0.0f};
     if (c >= unit / 8.0) return {-1, 0.0f};
     return {(int)c, (float)init_hi};
+}
+
+// ---- streaming fixed-point kernel ---------------------------------------------------
+//
+// Each workgroup owns a 128-column strip of one image over a band of rows and
+// walks down it 32 output rows per step. A step's horizontal pass computes the
+// 32 intermediate rows that become available (each intermediate row exactly
+// once per strip) into a 64-row ring of f16 columns; its vertical pass reads
+// the 16 * NKB ring rows that its 32 output rows need. The input window of
+// the next step is loaded into registers while the current step is computed,
+// and the strips of one image go to one XCD, so a strip's halo columns are
+// read from the L2 its neighbours just filled.
+//
+// Readout: the fixed-point scheme of blur_u8_fx_kernel (blur_fx_plan): an
+// exact hi accumulator started at 2^23 + c, a lo accumulator started at 0,
+// one f32 add; bits 15..22 of the sum are the reference's byte whenever bits
+// 0..14 are at least 2c, else the output is recomputed with the reference's
+// chain from the same LDS data.
+constexpr int kSR = 32;     // output rows per step
+constexpr int kSW = 128;    // strip width (output columns)
+constexpr int kRing = 64;   // intermediate rows held (2*half + m + d + 32 <= 64 for L <= 33)
+constexpr int kSOP = kSW + 16;  // bytes per output-step row in LDS
+
+template <int NKB> struct StreamShape {
+    static constexpr int CIN = kSW - 32 + 16 * NKB;  // staged input columns (multiple of 16)
+    static constexpr int CH = CIN / 16;              // 16-byte chunks per staged row
+    static constexpr int NCH = kSR * CH;             // chunks per step
+    static constexpr int NLD = (NCH + 255) / 256;    // chunk loads per thread
+    // f16 row pitch in dwords = 4 * odd: the 8 rows one 16-byte A-operand read
+    // of 8 lanes touches fall in distinct bank groups
+    static constexpr int PIN = CIN * 2 + 16;         // bytes per staged f16 row
+    static constexpr int LMAX = 16 * NKB - 31;
+};
+constexpr int kRingPitch = kRing * 2 + 16;  // bytes per ring column (36 dwords)
+
+struct StreamJob {
+    const uint8_t *src;
+    uint8_t *dst;
+    long long stride;
+    int W, H, n_img, strips, bands, band_rows;
+    int L, m, d;                 // taps, window shift, horizontal run-ahead rows
+    int ch, cv;                  // readout margins (units of 2^-15)
+    float init_h, init_v;        // accumulator start values 2^23 + c
+    float h[kFxMaxL], v[kFxMaxL];
+};
+
+// 16 u8 pixels -> 16 exact f16, written as 32 bytes.
+__device__ __forceinline__ void store_half16(_Float16 *dst, uint4 b) {
+    const v8h lo = bytes_to_half8(make_uint2(b.x, b.y));
+    const v8h hi = bytes_to_half8(make_uint2(b.z, b.w));
+    *reinterpret_cast<v8h *>(dst) = lo;
+    *reinterpret_cast<v8h *>(dst + 8) = hi;
+}
+
+// Bits 15..22 of two accumulator bit patterns as two exact f16 values.
+__device__ __forceinline__ uint32_t half2_bits15(uint32_t u0, uint32_t u1) {
+    const uint32_t p = __builtin_amdgcn_perm(u1, u0, 0x06050201u);
+    const v2u16 sh = {7, 7};
+    const uint32_t b = __builtin_bit_cast(uint32_t, __builtin_bit_cast(v2u16, p) >> sh);
+    const v2h f = __builtin_bit_cast(v2h, b | 0x64006400u);
+    const v2h off = {(_Float16)-1024.0f, (_Float16)-1024.0f};
+    return __builtin_bit_cast(uint32_t, f + off);
+}
+
+// The reference's chain down one ring column: rows (base + q) mod kRing.
+template <int LMAX>
+__device__ __forceinline__ float ring_chain(const _Float16 *col, int base, const float *taps,
+                                            int L) {
+    float s = 0.0f;
+#pragma unroll
+    for (int q = 0; q < LMAX; q++)
+        if (q < L) s = __fadd_rn(s, __fmul_rn((float)col[(base + q) & (kRing - 1)], taps[kTapPad + q]));
+    return s;
+}
+
+template <int NKB, bool SHARE_B, bool ALIGNED>
+__device__ __forceinline__ void blur_u8_stream_body(const StreamJob &jb) {
+    using S = StreamShape<NKB>;
+    constexpr int PIN = S::PIN, LMAX = S::LMAX;
+    constexpr int PINH = PIN / 2, RPH = kRingPitch / 2;  // pitches in f16 elements
+    extern __shared__ __align__(16) unsigned char smem[];
+    _Float16 *in_s = reinterpret_cast<_Float16 *>(smem);                       // kSR x PIN
+    _Float16 *ring = reinterpret_cast<_Float16 *>(smem + kSR * PIN);           // kSW x kRingPitch
+    uint8_t *out_s = smem + kSR * PIN + kSW * kRingPitch;                      // kSR x kSOP
+    float *taps_h = reinterpret_cast<float *>(out_s + kSR * kSOP);
+    float *taps_v = taps_h + (2 * kTapPad + kMaxTaps);
+    uint16_t *queue = reinterpret_cast<uint16_t *>(taps_v + (2 * kTapPad + kMaxTaps)) +
+                      (threadIdx.x >> 6) * kFxQCap;
+
+    // unit = (image, band, strip), strip fastest; contiguous eighths of the
+    // unit list go to the eight XCDs (workgroups are dealt round-robin)
+    const int units = jb.n_img * jb.bands * jb.strips;
+    const int chunk = (units + 7) >> 3;
+    const int u = (int)(blockIdx.x & 7) * chunk + (int)(blockIdx.x >> 3);
+    if ((int)(blockIdx.x >> 3) >= chunk || u >= units) return;
+    const int per_img = jb.bands * jb.strips;
+    const int im = u / per_img, rem = u - im * per_img;
+    const int band = rem / jb.strips, strip = rem - band * jb.strips;
+    const int W = jb.W, H = jb.H, L = jb.L, m = jb.m, half = L / 2;
+    const int x0 = strip * kSW, yb0 = band * jb.band_rows;
+    const int yb1 = min(H, yb0 + jb.band_rows);
+    const int nsteps = (yb1 - yb0 + kSR - 1) / kSR;
+    const uint8_t *src = jb.src + (size_t)im * jb.stride;
+    uint8_t *dst = jb.dst + (size_t)im * jb.stride;
+    const int xs = x0 - half - m;               // first staged column (multiple of 4)
+    const bool interior = ALIGNED && xs >= 0 && xs + S::CIN <= W;
+    const int ahead = half + jb.d;              // H(s) makes intermediate rows y(s) + ahead ..
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = lane & 31, g = lane >> 5;
+
+    // taps (zero padded), and a zeroed ring: rows the vertical pass reads
+    // with zero weight must hold finite values
+    if (threadIdx.x < 2 * kTapPad + kMaxTaps) {
+        const int t = (int)threadIdx.x - kTapPad;
+        const bool in = t >= 0 && t < L;
+        taps_h[threadIdx.x] = in ? jb.h[t] : 0.0f;
+        taps_v[threadIdx.x] = in ? jb.v[t] : 0.0f;
+    }
+    for (int i = threadIdx.x; i < kSW * kRingPitch / 16; i += 256)
+        reinterpret_cast<uint4 *>(ring)[i] = make_uint4(0, 0, 0, 0);
+
+    // input rows of step s: y(s) + ahead + r, r < kSR (clamped to the image)
+    uint4 regs[S::NLD];
+    auto issue_loads = [&](int s) {
+        const int ybase = yb0 + kSR * s + ahead;
+#pragma unroll
+        for (int k = 0; k < S::NLD; k++) {
+            const int c = min((int)threadIdx.x + 256 * k, S::NCH - 1);
+            const int r = c / S::CH, q = c - r * S::CH;
+            // 32-bit offsets from the image's (uniform) base: W * H < 2^32
+            const uint32_t row = (uint32_t)clampi(ybase + r, 0, H - 1) * (uint32_t)W;
+            const int gx = xs + 16 * q;
+            if (interior) {
+                regs[k] = *reinterpret_cast<const uint4 *>(src + (row + (uint32_t)gx));
+            } else {
+                uint32_t dw[4];
+#pragma unroll
+                for (int p = 0; p < 4; p++) {
+                    const int x4 = gx + 4 * p;
+                    if (ALIGNED) {
+                        dw[p] = *reinterpret_cast<const uint32_t *>(
+                            src + (row + (uint32_t)clampi(x4, 0, W - 4)));
+                    } else {
+                        dw[p] = (uint32_t)src[row + (uint32_t)clampi(x4, 0, W - 1)] |
+                                ((uint32_t)src[row + (uint32_t)clampi(x4 + 1, 0, W - 1)] << 8) |
+                                ((uint32_t)src[row + (uint32_t)clampi(x4 + 2, 0, W - 1)] << 16) |
+                                ((uint32_t)src[row + (uint32_t)clampi(x4 + 3, 0, W - 1)] << 24);
+                    }
+                }
+                regs[k] = make_uint4(dw[0], dw[1], dw[2], dw[3]);
+            }
+        }
+    };
+    // the loaded window into in_s as f16; at the image edges, aligned
+    // dwords left of the row become its first byte, right of it its last
+    auto store_rows = [&]() {
+#pragma unroll
+        for (int k = 0; k < S::NLD; k++) {
+            const int c = (int)threadIdx.x + 256 * k;
+            if (S::NCH % 256 != 0 && c >= S::NCH) break;
+            const int r = c / S::CH, q = c - r * S::CH;
+            uint4 v = regs[k];
+            if (ALIGNED && !interior) {
+                uint32_t dw[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int p = 0; p < 4; p++) {
+                    const int x4 = xs + 16 * q + 4 * p;
+                    const uint32_t left = (dw[p] & 0xffu) * 0x01010101u;
+                    const uint32_t right = (dw[p] >> 24) * 0x01010101u;
+                    dw[p] = x4 < 0 ? left : dw[p];
+                    dw[p] = x4 >= W ? right : dw[p];
+                }
+                v = make_uint4(dw[0], dw[1], dw[2], dw[3]);
+            }
+            store_half16(in_s + r * PINH + 16 * q, v);
+        }
+    };
+
+    issue_loads(-1);
+    __syncthreads();  // taps staged
+    v8h bh_hi[NKB], bh_lo[NKB], bv_hi_[SHARE_B ? 1 : NKB], bv_lo_[SHARE_B ? 1 : NKB];
+    build_b_fx<NKB>(taps_h, lane, m, bh_hi, bh_lo);
+    if constexpr (!SHARE_B) build_b_fx<NKB>(taps_v, lane, m, bv_hi_, bv_lo_);
+    auto &bv_hi = *reinterpret_cast<v8h(*)[NKB]>(SHARE_B ? (void *)bh_hi : (void *)bv_hi_);
+    auto &bv_lo = *reinterpret_cast<v8h(*)[NKB]>(SHARE_B ? (void *)bh_lo : (void *)bv_lo_);
+    const float *vts = SHARE_B ? taps_h : taps_v;
+    const uint32_t thr_h = 2u * (uint32_t)jb.ch, thr_v = 2u * (uint32_t)jb.cv;
+    v16f init_h, init_v_;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        float a = jb.init_h, b = jb.init_v;
+        asm volatile("" : "+v"(a));
+        init_h[r] = a;
+        if constexpr (!SHARE_B) {
+            asm volatile("" : "+v"(b));
+            init_v_[r] = b;
+        }
+    }
+    const v16f &init_v = SHARE_B ? init_h : init_v_;
+    store_rows();
+    if (nsteps > 0) issue_loads(0);
+
+    // the output rows y .. y + 31 of a finished step, from out_s; 32-bit
+    // offsets from the image base (W * H < 2^32)
+    const bool vec_out = ((W & 15) == 0) && ((reinterpret_cast<uintptr_t>(dst) & 15) == 0);
+    auto copy_out = [&](int y) {
+        const int yl = threadIdx.x >> 3, xl = (threadIdx.x & 7) * 16;
+        const int gy = y + yl, gx = x0 + xl;
+        if (gy < yb1 && gx < W) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(out_s + yl * kSOP + xl);
+            const uint32_t o = (uint32_t)gy * (uint32_t)W + (uint32_t)gx;
+            if (vec_out && gx + 16 <= W) {
+                *reinterpret_cast<uint4 *>(dst + o) = v;
+            } else {
+                const uint8_t *b = out_s + yl * kSOP + xl;
+                for (int q = 0; q < 16 && gx + q < W; q++) dst[o + q] = b[q];
+            }
+        }
+    };
+    // exact recomputation of the queued outputs an integer boundary could flip
+    int qn = 0;
+    auto h_drain = [&](int posh) {
+        for (int e = lane; e < qn; e += 64) {
+            const uint32_t ent = queue[e];
+            const int row = (int)(ent >> 8), cl = (int)(ent & 255);
+            const float r = exact_chain8<LMAX>(in_s + row * PINH + m + cl, taps_h, L);
+            ring[cl * RPH + ((posh + row) & (kRing - 1))] = (_Float16)(float)clamp_u8_exact(r);
+        }
+        qn = 0;
+    };
+    auto v_drain = [&](int pos0) {
+        for (int e = lane; e < qn; e += 64) {
+            const uint32_t ent = queue[e];
+            const int yl = (int)(ent >> 8), x2 = (int)(ent & 255);
+            const float r = ring_chain<LMAX>(ring + x2 * RPH, pos0 + yl + m, vts, L);
+            out_s[yl * kSOP + x2] = clamp_u8_exact(r);
+        }
+        qn = 0;
+    };
+    for (int s = -1; s < nsteps; s++) {
+        const int y = yb0 + kSR * s;
+        __syncthreads();  // in_s holds H(s)'s rows; out_s holds V(s-1)
+
+        // ---- copy V(s-1) out: one 16-byte row chunk per thread
+        if (s >= 1) copy_out(y - kSR);
+
+        // ---- horizontal pass: rows y + ahead .. + 31 into the ring (wave = column block)
+        {
+            const int posh = (y + ahead + half + m) & (kRing - 1);  // a multiple of 4
+            const _Float16 *arow = in_s + n * PINH + 32 * wave + 8 * g;
+            v8h a[NKB];
+#pragma unroll
+            for (int kb = 0; kb < NKB; kb++) a[kb] = *reinterpret_cast<const v8h *>(arow + 16 * kb);
+            v16f acc, acl;
+            const v16f zero = {};
+#pragma unroll
+            for (int kb = 0; kb < NKB; kb++) {
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[kb], bh_hi[kb], kb == 0 ? init_h : acc, 0, 0, 0);
+                acl = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[kb], bh_lo[kb], kb == 0 ? zero : acl, 0, 0, 0);
+            }
+            // lane holds column 32*wave + n, step rows 8*(r/4) + 4*g + r%4
+            _Float16 *col = ring + (32 * wave + n) * RPH;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                uint32_t uu[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) uu[i] = __float_as_uint(acc[4 * j + i] + acl[4 * j + i]);
+                const uint2 pk = make_uint2(half2_bits15(uu[0], uu[1]), half2_bits15(uu[2], uu[3]));
+                *reinterpret_cast<uint2 *>(col + ((posh + 8 * j + 4 * g) & (kRing - 1))) = pk;
+                if (__ballot(min_frac4(uu) < thr_h)) {
+                    const uint32_t e0 = (uint32_t)((8 * j + 4 * g) << 8) | (32 * wave + n);
+                    enqueue_flags4(uu, thr_h, e0, 1u << 8, queue, qn);
+                }
+                // a j-group adds at most 256 entries: draining above kFxQDrain after
+                // every second group keeps the queue within kFxQCap
+                if ((j & 1) && qn > kFxQDrain) h_drain(posh);
+            }
+            h_drain(posh);
+        }
+        __syncthreads();  // the ring holds every row V(s) reads; in_s is free
+
+        if (s + 1 < nsteps) {
+            store_rows();
+            if (s + 2 < nsteps) issue_loads(s + 2);
+        }
+
+        // ---- vertical pass: output rows y .. y + 31 into out_s (wave = column block)
+        if (s >= 0) {
+            const int pos0 = y & (kRing - 1);  // ring row of intermediate row y - half - m
+            const _Float16 *acol = ring + (32 * wave + n) * RPH;
+            v8h a[NKB];
+#pragma unroll
+            for (int kb = 0; kb < NKB; kb++)
+                a[kb] = *reinterpret_cast<const v8h *>(acol + ((pos0 + 16 * kb + 8 * g) & (kRing - 1)));
+            v16f acc, acl;
+            const v16f zero = {};
+#pragma unroll
+            for (int kb = 0; kb < NKB; kb++) {
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[kb], bv_hi[kb], kb == 0 ? init_v : acc, 0, 0, 0);
+                acl = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[kb], bv_lo[kb], kb == 0 ? zero : acl, 0, 0, 0);
+            }
+            // lane holds output row n, columns 32*wave + 8*(r/4) + 4*g + r%4
+            uint8_t *orow = out_s + n * kSOP;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int xl = 32 * wave + 8 * j + 4 * g;
+                uint32_t uu[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) uu[i] = __float_as_uint(acc[4 * j + i] + acl[4 * j + i]);
+                *reinterpret_cast<uint32_t *>(orow + xl) = pack_bits15(uu[0], uu[1], uu[2], uu[3]);
+                if (__ballot(min_frac4(uu) < thr_v)) {
+                    const uint32_t e0 = (uint32_t)(n << 8) | (uint32_t)xl;
+                    enqueue_flags4(uu, thr_v, e0, 1u, queue, qn);
+                }
+                if ((j & 1) && qn > kFxQDrain) v_drain(pos0);
+            }
+            v_drain(pos0);
+        }
+    }
+    __syncthreads();
+    if (nsteps > 0) copy_out(yb0 + kSR * (nsteps - 1));
+}
+
+// The job is read through a reference to the kernel argument: its taps are
+// indexed per thread, which must not copy the argument into registers.
+template <int NKB, bool SHARE_B, bool ALIGNED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void
+blur_u8_stream_kernel(StreamJob job) {
+    blur_u8_stream_body<NKB, SHARE_B, ALIGNED>(job);
+}
+
+constexpr size_t stream_lds(int nkb) {
+    return (size_t)kSR * (2 * (kSW - 32 + 16 * nkb) + 16) + (size_t)kSW * kRingPitch +
+           (size_t)kSR * kSOP + 2 * (2 * kTapPad + kMaxTaps) * sizeof(float) +
+           4 * kFxQCap * sizeof(uint16_t);
+}
+
+iph_status launch_blur_stream(const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h,
+                              uint32_t n, size_t stride, const float *hk, const float *vk,
+                              int L, int m, int nkb, const FxPlan &ph, const FxPlan &pv,
+                              bool same, bool aligned, hipStream_t s) {
+    StreamJob jb{};
+    jb.src = src;
+    jb.dst = dst;
+    jb.stride = (long long)stride;
+    jb.W = (int)w;
+    jb.H = (int)h;
+    jb.n_img = (int)n;
+    jb.strips = (int)((w + kSW - 1) / kSW);
+    // bands of rows: enough units to fill the chip a few times over
+    const long long per_band = (long long)n * jb.strips;
+    const int steps_total = (int)((h + kSR - 1) / kSR);
+    int bands = 1;
+    if (per_band < 4096) {
+        bands = (int)std::min<long long>(steps_total / 2 > 0 ? steps_total / 2 : 1,
+                                         (4096 + per_band - 1) / per_band);
+        bands = std::max(bands, 1);
+    }
+    const int steps_per_band = (steps_total + bands - 1) / bands;
+    jb.band_rows = steps_per_band * kSR;
+    jb.bands = (int)((h + jb.band_rows - 1) / jb.band_rows);
+    const long long units = per_band * jb.bands;
+    if (units == 0) return IPH_OK;
+    if (units > (1LL << 30)) {
+        set_error("batch of %u images of %ux%u is too large for one launch", n, w, h);
+        return IPH_EINVAL;
+    }
+    jb.L = L;
+    jb.m = m;
+    jb.d = (4 - ((2 * (L / 2) + m) & 3)) & 3;
+    jb.ch = ph.c;
+    jb.cv = pv.c;
+    jb.init_h = ph.init;
+    jb.init_v = pv.init;
+    for (int i = 0; i < L; i++) {
+        jb.h[i] = hk[i];
+        jb.v[i] = vk[i];
+    }
+    const unsigned grid = (unsigned)((units + 7) / 8 * 8);
+#define SB(K, SAME, AL)                                                                      \
+    blur_u8_stream_kernel<K, SAME, AL><<<grid, 256, stream_lds(K), s>>>(jb)
+#define SA(K, SAME)                                                                          \
+    if (aligned) SB(K, SAME, true); else SB(K, SAME, false)
+#define SK(K)                                                                                \
+    if (same) { SA(K, true); } else { SA(K, false); }                                        \
+    break
+    switch (nkb) {
+    case 2: SK(2);
+    case 3: SK(3);
+    default: SK(4);
+    }
+#undef SK
+#undef SA
+#undef SB
+    return launch_status("blur_u8_stream_kernel");
 }
 
 }  // namespace
@@ -1089,6 +1455,12 @@ iph_status launch_blur_u8_mfma(const uint8_t *src, uint8_t *dst, uint32_t w, uin
 #define LK(K)                                                                                \
     if (same) { LA(K, true); } else { LA(K, false); }                                        \
     break
+    if (L <= kFxMaxL && nkb >= 2 && nkb <= 4) {
+        const FxPlan sh = blur_fx_plan(hk, L, nkb), sv = blur_fx_plan(vk, L, nkb);
+        if (sh.c > 0 && sv.c > 0)
+            return launch_blur_stream(src, dst, w, h, n, stride, hk, vk, L, m, nkb, sh, sv, same,
+                                      aligned, s);
+    }
     const FxPlan ph = blur_fx_plan(hk, L, nkb), pv = blur_fx_plan(vk, L, nkb);
     if (ph.c > 0 && pv.c > 0) {
         FxJob f{};

@@ -237,3 +237,31 @@ def test_mfma_blur_batch_device_api():
     out = dev.gaussian_blur_f32(torch.from_numpy(imgs).cuda(), 2.0).cpu().numpy()
     for i in range(5):
         _same(out[i], O.gaussian_blur_f32(imgs[i], 2.0))
+
+
+# ---- streaming u8 blur: strips, bands, ragged edges, unequal taps ---------------------
+
+def test_stream_blur_bands_strips_and_ragged_edges():
+    """Shapes that cut the streaming kernel into several row bands and column
+    strips, with widths and heights that are not multiples of 4, 16, 32 or
+    128, and single-row / single-column images."""
+    rng = np.random.default_rng(21)
+    for h, w, sigma in [(1000, 37, 2.0), (2050, 131, 1.0), (77, 1029, 3.3), (4099, 260, 8.0),
+                        (33, 4, 0.5), (300, 1, 2.0), (1, 300, 2.0), (31, 129, 7.6),
+                        (768, 1024, 2.0), (65, 2049, 4.0)]:
+        img = rng.integers(0, 256, (h, w), dtype=np.uint8)
+        _same(flt.gaussian_blur_f32(img, sigma), O.gaussian_blur_f32(img, sigma))
+
+
+@pytest.mark.parametrize("L", [1, 2, 3, 4, 7, 9, 16, 17, 24, 27, 29, 31, 32, 33])
+def test_stream_separable_unequal_positive_taps(L):
+    """Distinct non-negative horizontal and vertical taps (separate Toeplitz
+    operands for the two passes), odd and even lengths up to 33."""
+    rng = np.random.default_rng(100 + L)
+    img = rng.integers(0, 256, (203, 301), dtype=np.uint8)
+    hk = rng.uniform(0.0, 1.0, L).astype(F)
+    vk = rng.uniform(0.0, 1.0, L).astype(F)
+    hk = (hk / np.float32(hk.sum() * 1.001)).astype(F)
+    vk = (vk / np.float32(vk.sum() * 1.001)).astype(F)
+    _same(flt.separable_filter(img, hk, vk), O.separable_filter(img, hk, vk))
+    _same(flt.separable_filter(img, hk, hk), O.separable_filter(img, hk, hk))
