@@ -27,6 +27,7 @@
 #include "match_common.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace iph {
 namespace {
@@ -605,6 +606,283 @@ iph_status launch_i8_kb(const uint8_t *img, int W, int H, const uint8_t *tpl, co
 }
 
 
+// ---- fused int8 correlation + window sums (u8 images) -------------------------------
+//
+// One workgroup computes a (32 RT) x 32 output tile, wave w its rows
+// 32w .. 32w + 31, from one LDS-staged image block (rows y0 .. y0 + 32 RT +
+// th - 2, columns x0 .. x0 + 32 KB - 1, as i8 = u8 - 128) and the staged
+// template:
+//   * C' = sum I'T' on MFMA against the template's Toeplitz slices (as in
+//     match_i8_mfma_kernel);
+//   * S' = sum I' over each window on MFMA too, with the same A operands
+//     against an all-ones Toeplitz band held in registers: exact in i32;
+//   * sum I^2 by exact u32 sliding sums: each staged row's horizontal
+//     tw-window sums of squares go to LDS (16 outputs per task), and each lane
+//     slides the vertical th-window sum down its own output column, so the
+//     sums land in the accumulators' layout without an exchange.
+// Then sum I T = C' + 128 S' + 128 sum T and every method is finished from
+// exact integers (finish_score). Nothing but the map leaves the workgroup.
+constexpr int kFusedHP = 33;  // u32 per row of the horizontal sums (odd: column reads hit distinct banks)
+
+__host__ __device__ constexpr int fused_pitch(int KB) {
+    return ((KB * 32 / 16) % 2 == 0) ? KB * 32 + 16 : KB * 32;  // odd multiple of 16 bytes
+}
+
+// finish_score over 32-bit exact sums: for templates up to 128 x 128, sum I T
+// and sum I^2, sum T^2 stay below 255^2 * 128^2 < 2^31, and SSE = sum T^2 -
+// 2 sum I T + sum I^2 is exact in u32. F64: normalise in f64 (windows of at
+// least 64 x 64 pixels), else in f32 like the reference.
+template <int METHOD, bool F64>
+__device__ __forceinline__ float finish_u32(uint32_t c, uint32_t sii, uint32_t stt) {
+    const bool sse = (METHOD == IPH_SSE || METHOD == IPH_SSE_NORMED);
+    const uint32_t num = sse ? stt + sii - 2u * c : c;
+    if (METHOD == IPH_SSE || METHOD == IPH_CCORR) return (float)num;
+    if constexpr (!F64) {
+        const float norm = sqrtf((float)sii * (float)stt);
+        const float sn = (float)num;
+        return norm > 0.0f ? sn / norm : sn;
+    } else {
+        const double norm = sqrt((double)sii * (double)stt);
+        return norm > 0.0 ? (float)((double)num / norm) : (float)num;
+    }
+}
+
+size_t fused_lds(int KB, int tw, int th, int RT, bool sums) {
+    const int rows = 32 * RT + th - 1;
+    size_t b = align_up((size_t)rows * fused_pitch(KB), 16) + (size_t)th * tpl_pitch(KB) + 16;
+    if (sums) b = align_up(b, 16) + (size_t)rows * kFusedHP * sizeof(uint32_t);
+    return b;
+}
+
+template <int KB, int METHOD>
+__global__ __launch_bounds__(256) void match_i8_fused_kernel(
+    const uint8_t *__restrict__ img, int W, int H, const uint8_t *__restrict__ tpl,
+    float *__restrict__ out, int ow, int oh, int tw, int th) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ long long tred[2][4];
+    constexpr bool kSums = METHOD != IPH_CCORR;  // sum I^2 needed
+    constexpr int BW = 32 * KB, PITCH = fused_pitch(KB), kTP = tpl_pitch(KB);
+    const int RT = blockDim.x >> 6;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32 * RT;
+    const int rows = 32 * RT + th - 1;
+    int8_t *blk = reinterpret_cast<int8_t *>(smem);
+    uint32_t *tp = reinterpret_cast<uint32_t *>(smem + ((rows * PITCH + 15) & ~15));
+    uint32_t *h2 = reinterpret_cast<uint32_t *>(
+        smem + ((((rows * PITCH + 15) & ~15) + th * kTP + 16 + 15) & ~15));
+
+    // image rows y0.. (clamped) and columns x0.. (clamped), as i8; bytes
+    // outside the image meet zero Toeplitz entries or discarded outputs
+    const bool aligned = (W & 3) == 0 && (reinterpret_cast<uintptr_t>(img) & 3) == 0 && W >= 4;
+    const int total = rows * (BW / 4);
+    for (int base = threadIdx.x; base < total; base += 8 * (int)blockDim.x) {
+        uint32_t v[8];
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const int i = min(base + b * (int)blockDim.x, total - 1);
+            const int r = i / (BW / 4), c = (i - r * (BW / 4)) * 4;
+            const uint8_t *row = img + (size_t)min(y0 + r, H - 1) * W;
+            const int gx = x0 + c;
+            if (aligned) {
+                v[b] = *reinterpret_cast<const uint32_t *>(row + min(gx, W - 4));
+            } else {
+                v[b] = (uint32_t)row[min(gx, W - 1)] | ((uint32_t)row[min(gx + 1, W - 1)] << 8) |
+                       ((uint32_t)row[min(gx + 2, W - 1)] << 16) |
+                       ((uint32_t)row[min(gx + 3, W - 1)] << 24);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const int i = base + b * (int)blockDim.x;
+            if (i < total) {
+                const int r = i / (BW / 4), c = (i - r * (BW / 4)) * 4;
+                *reinterpret_cast<uint32_t *>(blk + r * PITCH + c) = v[b] ^ 0x80808080u;
+            }
+        }
+    }
+    // template rows as i8 at [32, 32 + tw) of kTP-byte rows (zeros around),
+    // and sum T, sum T^2
+    for (int i = threadIdx.x; i < th * (kTP / 4) + 4; i += blockDim.x) tp[i] = 0;
+    __syncthreads();
+    uint32_t s1 = 0, s2 = 0;
+    uint8_t *tpb = reinterpret_cast<uint8_t *>(tp);
+    for (int i = threadIdx.x; i < th * tw; i += blockDim.x) {
+        const int r = i / tw, c = i - r * tw;
+        const uint32_t t = tpl[i];
+        s1 += t;
+        s2 += t * t;
+        tpb[r * kTP + 32 + c] = (uint8_t)(t ^ 0x80u);
+    }
+    long long l1 = s1, l2 = s2;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        l1 += __shfl_xor(l1, o);
+        l2 += __shfl_xor(l2, o);
+    }
+    if (lane == 0) {
+        tred[0][wave] = l1;
+        tred[1][wave] = l2;
+    }
+    __syncthreads();
+
+    // horizontal tw-window sums of squares of every staged row, 16 outputs
+    // per task: columns 16q .. 16q + 15 (the tile's 32 output columns)
+    if constexpr (kSums) {
+        const int nw = tw >> 2;
+        const uint32_t tail = (tw & 3) ? (1u << (8 * (tw & 3))) - 1u : 0u;
+        for (int it = threadIdx.x; it < rows * 2; it += blockDim.x) {
+            const int r = it >> 1, q = it & 1;
+            const uint32_t *pw = reinterpret_cast<const uint32_t *>(blk + r * PITCH + 16 * q);
+            uint32_t b = 0;
+            for (int k = 0; k < nw; k++) {
+                const uint32_t wv = pw[k] ^ 0x80808080u;
+                b = __builtin_amdgcn_udot4(wv, wv, b, false);
+            }
+            if (tail) {
+                const uint32_t wv = (pw[nw] ^ 0x80808080u) & tail;
+                b = __builtin_amdgcn_udot4(wv, wv, b, false);
+            }
+            const uint8_t *p = reinterpret_cast<const uint8_t *>(pw);
+            uint32_t *o = h2 + r * kFusedHP + 16 * q;
+            o[0] = b;
+#pragma unroll
+            for (int x = 1; x < 16; x++) {
+                const uint32_t vin = p[x + tw - 1] ^ 0x80u, vout = p[x - 1] ^ 0x80u;
+                b += vin * vin - vout * vout;
+                o[x] = b;
+            }
+        }
+    }
+
+    // correlation and window sums of I' on MFMA
+    const int n = lane & 31, g = lane >> 5;
+    v16i acc, acs;
+#pragma unroll
+    for (int r = 0; r < 16; r++) acc[r] = acs[r] = 0;
+    // all-ones band: B[k][n] = 1 for 0 <= 32 kb + k - n < tw
+    v4i ones[KB];
+#pragma unroll
+    for (int kb = 0; kb < KB; kb++)
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            uint32_t wv = 0;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const int dx = 32 * kb + 16 * g + 4 * d + b - n;
+                if (dx >= 0 && dx < tw) wv |= 1u << (8 * b);
+            }
+            ones[kb][d] = (int)wv;
+        }
+    const int8_t *arow = blk + (wave * 32 + n) * PITCH + 16 * g;
+    const int u0 = 32 + 16 * g - n;
+    const uint32_t sh = (uint32_t)(u0 & 3);
+    const uint32_t *trow = tp + (u0 >> 2);
+    for (int dy = 0; dy < th; dy++) {
+#pragma unroll
+        for (int kb = 0; kb < KB; kb++) {
+            const uint32_t *q = trow + 8 * kb;
+            const uint32_t d0 = q[0], d1 = q[1], d2 = q[2], d3 = q[3], d4 = q[4];
+            v4i bt;
+            bt[0] = (int)__builtin_amdgcn_alignbyte(d1, d0, sh);
+            bt[1] = (int)__builtin_amdgcn_alignbyte(d2, d1, sh);
+            bt[2] = (int)__builtin_amdgcn_alignbyte(d3, d2, sh);
+            bt[3] = (int)__builtin_amdgcn_alignbyte(d4, d3, sh);
+            const v4i a = *reinterpret_cast<const v4i *>(arow + 32 * kb);
+            acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, bt, acc, 0, 0, 0);
+            acs = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, ones[kb], acs, 0, 0, 0);
+        }
+        arow += PITCH;
+        trow += kTP / 4;
+    }
+
+    uint32_t st = 0, stt = 0;
+    for (int w = 0; w < RT; w++) {
+        st += (uint32_t)tred[0][w];
+        stt += (uint32_t)tred[1][w];
+    }
+    if constexpr (kSums) __syncthreads();  // every row's horizontal sums are in h2
+    // lane (n, g) holds rows 32w + 8j + 4g + i (register 4j + i) of column n:
+    // slide the vertical th-window sum of h2 down rows 32w + 4g .. 32w + 4g + 27
+    uint32_t sq = 0;
+    const uint32_t *hc = h2 + (wave * 32 + 4 * g) * kFusedHP + n;
+    if constexpr (kSums)
+        for (int dy = 0; dy < th; dy++) sq += hc[dy * kFusedHP];
+    const int x = x0 + n;
+    const int ybase = y0 + wave * 32 + 4 * g;
+    // the 16 window energies first (no branch between the LDS reads), then
+    // the scores, then the stores
+    uint32_t sqv[16];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int yl = 8 * j + i;  // row offset from 32w + 4g
+            sqv[4 * j + i] = sq;
+            if constexpr (kSums) {
+                // to row yl + 1 (past the other half's four rows after i = 3)
+                if (j < 3 || i < 3) {
+#pragma unroll
+                    for (int k = 0; k < ((i == 3) ? 5 : 1); k++) {
+                        const int r = yl + k;
+                        sq += hc[(r + th) * kFusedHP] - hc[r * kFusedHP];
+                    }
+                }
+            }
+        }
+    }
+    auto emit = [&](auto f64) {
+        constexpr bool F64 = decltype(f64)::value;
+        float v[16];
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            // sum I T = C' + 128 (S' + sum T), exact in 32 bits
+            const uint32_t c = (uint32_t)(acc[r] + 128 * (acs[r] + (int)st));
+            v[r] = finish_u32<METHOD, F64>(c, sqv[r], stt);
+        }
+        if (x < ow) {
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int y = ybase + 8 * (r >> 2) + (r & 3);
+                if (y < oh) out[(uint32_t)y * (uint32_t)ow + (uint32_t)x] = v[r];
+            }
+        }
+    };
+    const bool f64 = (METHOD == IPH_SSE_NORMED || METHOD == IPH_CCORR_NORMED) && tw * th >= 64 * 64;
+    if (f64) emit(std::true_type{});
+    else emit(std::false_type{});
+}
+
+constexpr size_t kFusedMaxLds = 64 * 1024;
+
+// Output rows per fused workgroup (32 * RT): the batch hint or 4 waves,
+// halved while the LDS footprint is above kFusedMaxLds; 0 when even one
+// wave does not fit (the template is too tall or wide for the fused kernel).
+int fused_rt(int tw, int th, int method) {
+    const int KB = (31 + tw + 31) / 32;
+    int RT = g_match_rows_hint > 0 ? std::min(g_match_rows_hint, 4) : 4;
+    while (RT > 0 && fused_lds(KB, tw, th, RT, method != IPH_CCORR) > kFusedMaxLds) RT /= 2;
+    return RT;
+}
+
+template <int KB>
+iph_status launch_i8_fused_kb(const uint8_t *img, int W, int H, const uint8_t *tpl, float *out,
+                              int ow, int oh, int tw, int th, int method, int RT,
+                              hipStream_t s) {
+    const size_t lds = fused_lds(KB, tw, th, RT, method != IPH_CCORR);
+    dim3 grid((ow + 31) / 32, (oh + 32 * RT - 1) / (32 * RT));
+#define L(M)                                                                                 \
+    match_i8_fused_kernel<KB, M><<<grid, 64 * RT, lds, s>>>(img, W, H, tpl, out, ow, oh, tw, th)
+    switch (method) {
+    case IPH_SSE: L(IPH_SSE); break;
+    case IPH_SSE_NORMED: L(IPH_SSE_NORMED); break;
+    case IPH_CCORR: L(IPH_CCORR); break;
+    default: L(IPH_CCORR_NORMED); break;
+    }
+#undef L
+    return launch_status("match_i8_fused_kernel");
+}
+
+
 }  // namespace
 
 iph_status launch_bf16x3_match(const float *img, uint32_t w, uint32_t h, const float *tpl,
@@ -662,6 +940,8 @@ size_t mfma_match_workspace(int32_t p_type, uint32_t w, uint32_t h, uint32_t tw,
         if (method != IPH_CCORR) wp.take<double>((size_t)(w - tw + 1) * h);
         return wp.used;
     }
+    if (tw <= (uint32_t)kMaxSide && th <= (uint32_t)kMaxSide && fused_rt((int)tw, (int)th, method) > 0)
+        return 256;  // the fused kernel needs no workspace
     WorkspacePlan wp;
     wp.take<uint2>((size_t)(w - tw + 1) * (h - th + 1));
     return wp.used;
@@ -680,6 +960,19 @@ iph_status launch_mfma_match(const void *img, int32_t p_type, uint32_t w, uint32
         return IPH_EUNSUPPORTED;
     }
     const int KB = blocks_for((int)tw);
+    const uint8_t *im8 = static_cast<const uint8_t *>(img);
+    const uint8_t *tp8 = static_cast<const uint8_t *>(tpl);
+    const int RT = fused_rt((int)tw, (int)th, method);
+    if (RT > 0) {
+        const int ow = (int)(w - tw + 1), oh = (int)(h - th + 1);
+        switch (KB) {
+        case 1: return launch_i8_fused_kb<1>(im8, w, h, tp8, out, ow, oh, tw, th, method, RT, s);
+        case 2: return launch_i8_fused_kb<2>(im8, w, h, tp8, out, ow, oh, tw, th, method, RT, s);
+        case 3: return launch_i8_fused_kb<3>(im8, w, h, tp8, out, ow, oh, tw, th, method, RT, s);
+        case 4: return launch_i8_fused_kb<4>(im8, w, h, tp8, out, ow, oh, tw, th, method, RT, s);
+        default: return launch_i8_fused_kb<5>(im8, w, h, tp8, out, ow, oh, tw, th, method, RT, s);
+        }
+    }
     Workspace wsp(ws, ws_bytes);
     uint2 *wsum = wsp.take<uint2>((size_t)(w - tw + 1) * (h - th + 1));
     if (!wsum) {
