@@ -624,12 +624,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--leg", default="all",
-                    choices=("all", "blur", "copy", "match", "proposal", "consumers"),
+                    choices=("all", "blur", "copy", "match", "proposal", "consumers", "batch"),
                     help="blur: only the blur batch (for counter runs); copy: only a plain "
                          "copy of the same size (counter calibration); match: only the "
                          "match_template leg; proposal: only the learned-proposal search "
                          "against the full map; consumers: integral image, "
-                         "adaptive threshold and canny")
+                         "adaptive threshold and canny; batch: the ragged batch")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X and has no "
@@ -649,6 +649,9 @@ def main():
         return
     if args.leg == "proposal":
         print(json.dumps({"leg": "proposal", **bench_proposal()}), flush=True)
+        return
+    if args.leg == "batch":
+        print(json.dumps({"leg": "batch", **bench_batch(4)}), flush=True)
         return
     if args.leg == "copy":
         ms, gbs = bench_copy(args.steps, args.warmup)

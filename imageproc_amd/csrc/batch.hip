@@ -7,19 +7,27 @@
 // Plan: the blur jobs become device job tables of the persistent fixed-point
 // blur kernel (one launch per kernel variant, each walking the tiles of all
 // its images), the filter jobs one table of the ragged 2-D filter kernel (one
-// launch), the match jobs one int8-MFMA pipeline each with a workspace slice.
-// The blurs run on the caller's stream, the filters and the matches on side
-// streams forked from it (the matches spread over seven, largest first onto
-// the least loaded) and joined back, captured once into a graph. The autotuner times the candidate
-// launch shapes of each kind on the real jobs (filter tile shape, rows of
-// 32 outputs per match workgroup, persistent blur workgroups per CU) and
-// keeps the fastest before capture.
+// launch), the match jobs device tables of the fused int8 match kernel (one
+// launch per K-block count and method, each walking the tiles of all its
+// pairs); pairs that kernel does not take (templates whose staged block
+// exceeds its LDS budget) run the two-kernel path, one pipeline each. The
+// blurs run on the caller's stream, the filters and the match tables on side
+// streams forked from it (the remaining pairs spread over six more, largest
+// first onto the least loaded) and joined back, captured once into a graph.
+// The autotuner times the candidate launch shapes of each kind on the real
+// jobs (filter tile shape, rows of 32 outputs per match workgroup for each
+// match table, persistent blur workgroups per CU), keeps the fastest before
+// capture, and remembers each match table's choice for its shape class.
 #include "iph_internal.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
 #include <string>
+#include <tuple>
 #include <vector>
 
 namespace {
@@ -39,12 +47,27 @@ struct FallbackBlur {
     size_t ws_off, ws_bytes;
 };
 
+// Match jobs of one fused-kernel launch (one K-block count and method),
+// consecutive in the job table.
+struct MatchGroup {
+    int kb, method, first, count, rt, rt_max;
+    int tc;  // template rows per chunk, at most
+    long long ntiles;
+    size_t lds;
+    uint32_t max_tw, max_th;
+    double outputs;  // map pixels of all the group's matches
+};
+
 }  // namespace
 
 struct iph_batch {
     std::vector<iph::BlurSpec> blur;
     std::vector<iph::FilterSpec> filt;
     std::vector<MatchSpec> match;
+    std::vector<iph::MatchJob> mjobs;     // fused-kernel matches, grouped (host copy)
+    std::vector<MatchGroup> mgroups;
+    std::vector<int> match_fallback;      // matches on the two-kernel path
+    size_t mjob_table_off = 0;
 
     bool planned = false;
     int device = -1;
@@ -56,12 +79,13 @@ struct iph_batch {
     int filt_tiles = 0;
     uint32_t filt_max_kw = 1, filt_max_kh = 1;
     int blur_grid_pct = 100, filt_variant = 0, match_rows = 0;
-    // side[0]: the filters; side[1 .. kSide - 1]: the matches
+    // side[0]: the filters; side[1]: the match tables; side[2 .. kSide - 1]: the
+    // matches on the two-kernel path
     static constexpr int kSide = 8;
     hipStream_t side[kSide] = {};
     hipStream_t capture = nullptr;  // the graph is captured here (the caller's may be the null stream)
     hipEvent_t fork = nullptr, join[kSide] = {};
-    std::vector<int> match_lane;  // side stream (1 ..) of each match job
+    std::vector<int> match_lane;  // side stream (2 ..) of each two-kernel-path match
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::string tuning;
@@ -107,12 +131,22 @@ iph_status enqueue_filters(iph_batch *b, hipStream_t s) {
                               b->filt_variant, b->filt_max_kw, b->filt_max_kh, s);
 }
 
-// The match jobs; with `spread`, each on its side stream, else all on s.
+iph_status enqueue_match_group(iph_batch *b, const MatchGroup &g, hipStream_t s) {
+    const MatchJob *table = reinterpret_cast<const MatchJob *>(b->mem + b->mjob_table_off);
+    return launch_match_jobs(table + g.first, g.count, g.ntiles, g.kb, g.method, g.rt, g.tc,
+                             g.lds, s);
+}
+
+// The match jobs: the fused-kernel tables, then the two-kernel-path pairs;
+// with `spread`, the tables on side[1] and each pair on its side stream,
+// else all on s.
 iph_status enqueue_matches(iph_batch *b, hipStream_t s, bool spread = false) {
+    for (const MatchGroup &g : b->mgroups)
+        IPH_TRY(enqueue_match_group(b, g, spread ? b->side[1] : s));
     set_match_rows_hint(b->match_rows);
-    for (size_t j = 0; j < b->match.size(); j++) {
-        const MatchSpec &m = b->match[j];
-        hipStream_t ms = spread ? b->side[b->match_lane[j]] : s;
+    for (size_t k = 0; k < b->match_fallback.size(); k++) {
+        const MatchSpec &m = b->match[b->match_fallback[k]];
+        hipStream_t ms = spread ? b->side[b->match_lane[k]] : s;
         iph_status st = launch_match(m.img, IPH_U8, m.w, m.h, m.tpl, m.tw, m.th, m.method,
                                      resolve_path(IPH_PATH_AUTO, m.tw, m.th), m.out,
                                      b->mem + m.ws_off, m.ws_bytes, ms);
@@ -123,6 +157,37 @@ iph_status enqueue_matches(iph_batch *b, hipStream_t s, bool spread = false) {
     }
     set_match_rows_hint(0);
     return IPH_OK;
+}
+
+// Lays out and uploads the job-table slice of group g for g.rt waves per
+// workgroup (its tile list, chunk height and LDS size).
+iph_status layout_match_group(iph_batch *b, MatchGroup &g) {
+    g.ntiles = match_jobs_layout(b->mjobs.data() + g.first, g.count, g.rt);
+    g.tc = fused_match_tc(g.kb, g.max_th, g.rt, g.method);
+    g.lds = 0;
+    for (int i = g.first; i < g.first + g.count; i++) {
+        const MatchJob &j = b->mjobs[i];
+        g.lds = std::max(g.lds, fused_match_lds(g.kb, (uint32_t)j.tw, (uint32_t)j.th, g.rt,
+                                                g.method, g.tc));
+    }
+    IPH_TRY_HIP(hipMemcpy(b->mem + b->mjob_table_off + (size_t)g.first * sizeof(MatchJob),
+                          b->mjobs.data() + g.first, (size_t)g.count * sizeof(MatchJob),
+                          hipMemcpyHostToDevice));
+    return IPH_OK;
+}
+
+// Waves per workgroup the autotuner chose for a shape class of match tables:
+// (K blocks, method, tallest and widest template rounded up to 16, log2 of
+// the map pixels).
+using MatchClass = std::tuple<int, int, int, int, int>;
+std::mutex g_match_cache_mu;
+std::map<MatchClass, int> g_match_cache;
+
+MatchClass match_class(const MatchGroup &g) {
+    int lg = 0;
+    while (std::ldexp(1.0, lg + 1) <= std::max(g.outputs, 1.0)) lg++;
+    return MatchClass(g.kb, g.method, (int)((g.max_th + 15) / 16 * 16),
+                      (int)((g.max_tw + 15) / 16 * 16), lg);
 }
 
 // All jobs: blurs on `s`, filters and matches on the side streams forked
@@ -192,14 +257,39 @@ iph_status autotune(iph_batch *b, hipStream_t s) {
         }
         IPH_TRY(upload_filter_table(b, best_v));
     }
-    if (!b->match.empty()) {
+    for (MatchGroup &g : b->mgroups) {
+        int best_r = g.rt_max;
+        float best_ms = 1e30f;
+        for (int rt : {4, 2, 1}) {
+            if (rt > g.rt_max) continue;
+            g.rt = rt;
+            IPH_TRY(layout_match_group(b, g));
+            float ms;
+            IPH_TRY(time_min(s, 3, [&] { return enqueue_match_group(b, g, s); }, &ms));
+            std::snprintf(line, sizeof line,
+                          "match table (kb %d, %d pairs, templates <= %ux%u) rows/workgroup "
+                          "%d: %.3f ms\n",
+                          g.kb, g.count, g.max_tw, g.max_th, 32 * rt, ms);
+            b->tuning += line;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_r = rt;
+            }
+        }
+        g.rt = best_r;
+        IPH_TRY(layout_match_group(b, g));
+        std::lock_guard<std::mutex> lock(g_match_cache_mu);
+        g_match_cache[match_class(g)] = best_r;
+    }
+    if (!b->match_fallback.empty()) {
         int best_r = 0;
         float best_ms = 1e30f;
         for (int rows : {4, 2, 1}) {
             b->match_rows = rows;
             float ms;
             IPH_TRY(time_min(s, 3, [&] { return enqueue_matches(b, s); }, &ms));
-            std::snprintf(line, sizeof line, "match rows/workgroup %d: %.3f ms\n", 32 * rows, ms);
+            std::snprintf(line, sizeof line, "two-kernel match rows/workgroup %d: %.3f ms\n",
+                          32 * rows, ms);
             b->tuning += line;
             if (ms < best_ms) {
                 best_ms = ms;
@@ -224,8 +314,10 @@ iph_status autotune(iph_batch *b, hipStream_t s) {
         }
         b->blur_grid_pct = best_p;
     }
-    std::snprintf(line, sizeof line, "chosen: filter %s, match %d rows, blur %d%%\n",
-                  filter_jobs_variant_name(b->filt_variant), 32 * std::max(b->match_rows, 1),
+    std::string rows;
+    for (const MatchGroup &g : b->mgroups) rows += " " + std::to_string(32 * g.rt);
+    std::snprintf(line, sizeof line, "chosen: filter %s, match tables rows%s, blur %d%%\n",
+                  filter_jobs_variant_name(b->filt_variant), rows.empty() ? " -" : rows.c_str(),
                   b->blur_grid_pct);
     b->tuning += line;
     return IPH_OK;
@@ -257,12 +349,57 @@ iph_status plan(iph_batch *b, int32_t tune, hipStream_t s) {
         b->filt_max_kw = std::max(b->filt_max_kw, f.kw);
         b->filt_max_kh = std::max(b->filt_max_kh, f.kh);
     }
-    for (MatchSpec &m : b->match) {
-        m.ws_bytes = match_workspace(IPH_U8, m.w, m.h, m.tw, m.th, m.method,
-                                     resolve_path(IPH_PATH_AUTO, m.tw, m.th));
-        m.ws_off = align_up(wp.used, 256);
-        wp.take<unsigned char>(std::max<size_t>(m.ws_bytes, 1));
+    // matches: the fused kernel's tables, grouped by (K blocks, method); the
+    // rest on the two-kernel path with a workspace slice each
+    b->mjobs.clear();
+    b->mgroups.clear();
+    b->match_fallback.clear();
+    {
+        struct Entry {
+            int key, idx, rt_max;
+        };
+        std::vector<Entry> entries;
+        for (size_t i = 0; i < b->match.size(); i++) {
+            MatchSpec &m = b->match[i];
+            const int rt_max = fused_match_rt_max(m.tw, m.th, m.method);
+            if (rt_max == 0) {
+                b->match_fallback.push_back((int)i);
+                m.ws_bytes = match_workspace(IPH_U8, m.w, m.h, m.tw, m.th, m.method,
+                                             resolve_path(IPH_PATH_AUTO, m.tw, m.th));
+                m.ws_off = align_up(wp.used, 256);
+                wp.take<unsigned char>(std::max<size_t>(m.ws_bytes, 1));
+                continue;
+            }
+            entries.push_back({fused_match_kb(m.tw) * 4 + m.method, (int)i, rt_max});
+        }
+        std::stable_sort(entries.begin(), entries.end(),
+                         [](const Entry &a, const Entry &c) { return a.key < c.key; });
+        for (const Entry &e : entries) {
+            const MatchSpec &m = b->match[e.idx];
+            if (b->mgroups.empty() || b->mgroups.back().kb * 4 + b->mgroups.back().method != e.key)
+                b->mgroups.push_back({e.key / 4, e.key % 4, (int)b->mjobs.size(), 0, 4, 4, 32,
+                                      0, 0, 0u, 0u, 0.0});
+            MatchGroup &g = b->mgroups.back();
+            MatchJob j{};
+            j.img = m.img;
+            j.tpl = m.tpl;
+            j.out = m.out;
+            j.W = (int)m.w;
+            j.H = (int)m.h;
+            j.tw = (int)m.tw;
+            j.th = (int)m.th;
+            j.ow = (int)(m.w - m.tw + 1);
+            j.oh = (int)(m.h - m.th + 1);
+            b->mjobs.push_back(j);
+            g.count++;
+            g.rt_max = std::min(g.rt_max, e.rt_max);
+            g.max_tw = std::max(g.max_tw, m.tw);
+            g.max_th = std::max(g.max_th, m.th);
+            g.outputs += (double)j.ow * j.oh;
+        }
     }
+    b->mjob_table_off = align_up(wp.used, 256);
+    wp.take<unsigned char>(std::max<size_t>(b->mjobs.size() * sizeof(MatchJob), 1));
     b->mem_bytes = std::max<size_t>(wp.used, 256);
     if (hipMalloc(reinterpret_cast<void **>(&b->mem), b->mem_bytes) != hipSuccess) {
         b->mem = nullptr;
@@ -272,23 +409,33 @@ iph_status plan(iph_batch *b, int32_t tune, hipStream_t s) {
     IPH_TRY_HIP(hipMemcpy(b->mem + b->blur_table_off, blur_host.data(), blur_host.size(),
                           hipMemcpyHostToDevice));
     IPH_TRY(upload_filter_table(b, 0));
-    // match jobs to the match streams: largest first, each to the least loaded
+    // match tables: the cached choice for their shape class, else the largest fit
+    for (MatchGroup &g : b->mgroups) {
+        g.rt = g.rt_max;
+        g.ntiles = match_jobs_layout(b->mjobs.data() + g.first, g.count, g.rt);
+        std::lock_guard<std::mutex> lock(g_match_cache_mu);
+        auto it = g_match_cache.find(match_class(g));
+        if (it != g_match_cache.end() && it->second <= g.rt_max) g.rt = it->second;
+    }
+    for (MatchGroup &g : b->mgroups) IPH_TRY(layout_match_group(b, g));
+    // two-kernel-path pairs to their streams: largest first, each to the least loaded
     {
-        const int nl = iph_batch::kSide - 1;
-        std::vector<size_t> order(b->match.size());
-        std::vector<double> cost(b->match.size()), load(nl, 0.0);
-        for (size_t j = 0; j < order.size(); j++) {
-            const MatchSpec &m = b->match[j];
-            order[j] = j;
-            cost[j] = (double)(m.w - m.tw + 1) * (m.h - m.th + 1) * ((double)m.tw * m.th + 64.0);
+        const int nl = iph_batch::kSide - 2;
+        const size_t nf = b->match_fallback.size();
+        std::vector<size_t> order(nf);
+        std::vector<double> cost(nf), load(nl, 0.0);
+        for (size_t k = 0; k < nf; k++) {
+            const MatchSpec &m = b->match[b->match_fallback[k]];
+            order[k] = k;
+            cost[k] = (double)(m.w - m.tw + 1) * (m.h - m.th + 1) * ((double)m.tw * m.th + 64.0);
         }
         std::stable_sort(order.begin(), order.end(),
                          [&](size_t a, size_t c) { return cost[a] > cost[c]; });
-        b->match_lane.assign(b->match.size(), 1);
-        for (size_t j : order) {
+        b->match_lane.assign(nf, 2);
+        for (size_t k : order) {
             const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-            b->match_lane[j] = 1 + l;
-            load[l] += cost[j];
+            b->match_lane[k] = 2 + l;
+            load[l] += cost[k];
         }
     }
     for (int i = 0; i < iph_batch::kSide; i++) {

@@ -185,6 +185,31 @@ iph_status launch_blur_jobs(const void *table_dev, const std::vector<BlurGroup> 
 // largest that fits in LDS); set by the batch autotuner for its launches.
 void set_match_rows_hint(int rows);
 
+// match_template (GrayImage, int8 MFMA with in-kernel window sums) of a batch:
+// one entry of the device job table of the fused kernel.
+struct MatchJob {
+    const uint8_t *img;
+    const uint8_t *tpl;
+    float *out;
+    int W, H, tw, th, ow, oh;
+    int tiles_x, tile0;  // 32-column tiles per tile row; first tile in the launch
+};
+// K blocks of the fused kernel for a template width.
+int fused_match_kb(uint32_t tw);
+// Largest waves per workgroup (4, 2, 1) whose LDS footprint the fused kernel
+// takes for this template and method, 0 if none (the two-kernel path then).
+int fused_match_rt_max(uint32_t tw, uint32_t th, int32_t method);
+// Template rows per chunk for templates up to th rows high (the footprint then
+// fits three workgroups per CU where it can), and the LDS bytes of one job.
+int fused_match_tc(int kb, uint32_t th, int rt, int32_t method);
+size_t fused_match_lds(int kb, uint32_t tw, uint32_t th, int rt, int32_t method, int tc);
+// Sets tiles_x and tile0 of n consecutive jobs for rt waves per workgroup;
+// returns the number of tiles.
+long long match_jobs_layout(MatchJob *jobs, int n, int rt);
+// One launch over n jobs of one (kb, method) group, device table `jobs`.
+iph_status launch_match_jobs(const MatchJob *jobs, int n, long long ntiles, int kb,
+                             int32_t method, int rt, int tc, size_t lds, hipStream_t s);
+
 // ---- launchers implemented in match.hip ---------------------------------
 
 size_t match_workspace(int32_t p_type, uint32_t w, uint32_t h, uint32_t tw, uint32_t th,

@@ -647,118 +647,47 @@ __device__ __forceinline__ float finish_u32(uint32_t c, uint32_t sii, uint32_t s
     }
 }
 
-size_t fused_lds(int KB, int tw, int th, int RT, bool sums) {
-    const int rows = 32 * RT + th - 1;
-    size_t b = align_up((size_t)rows * fused_pitch(KB), 16) + (size_t)th * tpl_pitch(KB) + 16;
+// LDS of a fused workgroup that takes template rows at most tc at a time.
+size_t fused_lds(int KB, int tw, int th, int RT, bool sums, int tc) {
+    tc = std::min(th, tc);
+    const int rows = 32 * RT + tc - 1;
+    size_t b = align_up((size_t)rows * fused_pitch(KB), 16) + (size_t)tc * tpl_pitch(KB) + 16;
     if (sums) b = align_up(b, 16) + (size_t)rows * kFusedHP * sizeof(uint32_t);
     return b;
 }
 
+// The output tile at (x0, y0) of one match (the body of the single-image and
+// the job-table kernels). Template rows are taken in ceil(th / tcm) chunks of
+// near-equal height (at most tcm rows): per chunk the workgroup stages the
+// image rows those template rows meet and the rows themselves, correlates,
+// and adds the chunk's rows of every window's sum of squares, so the LDS
+// footprint is bounded by tcm rather than by the template height.
+
 template <int KB, int METHOD>
-__global__ __launch_bounds__(256) void match_i8_fused_kernel(
-    const uint8_t *__restrict__ img, int W, int H, const uint8_t *__restrict__ tpl,
-    float *__restrict__ out, int ow, int oh, int tw, int th) {
+__device__ __forceinline__ void match_i8_fused_tile(const uint8_t *__restrict__ img, int W,
+                                                    int H, const uint8_t *__restrict__ tpl,
+                                                    float *__restrict__ out, int ow, int oh,
+                                                    int tw, int th, int tcm, int x0,
+                                                    int y0) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ long long tred[2][4];
     constexpr bool kSums = METHOD != IPH_CCORR;  // sum I^2 needed
     constexpr int BW = 32 * KB, PITCH = fused_pitch(KB), kTP = tpl_pitch(KB);
     const int RT = blockDim.x >> 6;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32 * RT;
-    const int rows = 32 * RT + th - 1;
-    int8_t *blk = reinterpret_cast<int8_t *>(smem);
-    uint32_t *tp = reinterpret_cast<uint32_t *>(smem + ((rows * PITCH + 15) & ~15));
-    uint32_t *h2 = reinterpret_cast<uint32_t *>(
-        smem + ((((rows * PITCH + 15) & ~15) + th * kTP + 16 + 15) & ~15));
-
-    // image rows y0.. (clamped) and columns x0.. (clamped), as i8; bytes
-    // outside the image meet zero Toeplitz entries or discarded outputs
-    const bool aligned = (W & 3) == 0 && (reinterpret_cast<uintptr_t>(img) & 3) == 0 && W >= 4;
-    const int total = rows * (BW / 4);
-    for (int base = threadIdx.x; base < total; base += 8 * (int)blockDim.x) {
-        uint32_t v[8];
-#pragma unroll
-        for (int b = 0; b < 8; b++) {
-            const int i = min(base + b * (int)blockDim.x, total - 1);
-            const int r = i / (BW / 4), c = (i - r * (BW / 4)) * 4;
-            const uint8_t *row = img + (size_t)min(y0 + r, H - 1) * W;
-            const int gx = x0 + c;
-            if (aligned) {
-                v[b] = *reinterpret_cast<const uint32_t *>(row + min(gx, W - 4));
-            } else {
-                v[b] = (uint32_t)row[min(gx, W - 1)] | ((uint32_t)row[min(gx + 1, W - 1)] << 8) |
-                       ((uint32_t)row[min(gx + 2, W - 1)] << 16) |
-                       ((uint32_t)row[min(gx + 3, W - 1)] << 24);
-            }
-        }
-#pragma unroll
-        for (int b = 0; b < 8; b++) {
-            const int i = base + b * (int)blockDim.x;
-            if (i < total) {
-                const int r = i / (BW / 4), c = (i - r * (BW / 4)) * 4;
-                *reinterpret_cast<uint32_t *>(blk + r * PITCH + c) = v[b] ^ 0x80808080u;
-            }
-        }
-    }
-    // template rows as i8 at [32, 32 + tw) of kTP-byte rows (zeros around),
-    // and sum T, sum T^2
-    for (int i = threadIdx.x; i < th * (kTP / 4) + 4; i += blockDim.x) tp[i] = 0;
-    __syncthreads();
-    uint32_t s1 = 0, s2 = 0;
-    uint8_t *tpb = reinterpret_cast<uint8_t *>(tp);
-    for (int i = threadIdx.x; i < th * tw; i += blockDim.x) {
-        const int r = i / tw, c = i - r * tw;
-        const uint32_t t = tpl[i];
-        s1 += t;
-        s2 += t * t;
-        tpb[r * kTP + 32 + c] = (uint8_t)(t ^ 0x80u);
-    }
-    long long l1 = s1, l2 = s2;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        l1 += __shfl_xor(l1, o);
-        l2 += __shfl_xor(l2, o);
-    }
-    if (lane == 0) {
-        tred[0][wave] = l1;
-        tred[1][wave] = l2;
-    }
-    __syncthreads();
-
-    // horizontal tw-window sums of squares of every staged row, 16 outputs
-    // per task: columns 16q .. 16q + 15 (the tile's 32 output columns)
-    if constexpr (kSums) {
-        const int nw = tw >> 2;
-        const uint32_t tail = (tw & 3) ? (1u << (8 * (tw & 3))) - 1u : 0u;
-        for (int it = threadIdx.x; it < rows * 2; it += blockDim.x) {
-            const int r = it >> 1, q = it & 1;
-            const uint32_t *pw = reinterpret_cast<const uint32_t *>(blk + r * PITCH + 16 * q);
-            uint32_t b = 0;
-            for (int k = 0; k < nw; k++) {
-                const uint32_t wv = pw[k] ^ 0x80808080u;
-                b = __builtin_amdgcn_udot4(wv, wv, b, false);
-            }
-            if (tail) {
-                const uint32_t wv = (pw[nw] ^ 0x80808080u) & tail;
-                b = __builtin_amdgcn_udot4(wv, wv, b, false);
-            }
-            const uint8_t *p = reinterpret_cast<const uint8_t *>(pw);
-            uint32_t *o = h2 + r * kFusedHP + 16 * q;
-            o[0] = b;
-#pragma unroll
-            for (int x = 1; x < 16; x++) {
-                const uint32_t vin = p[x + tw - 1] ^ 0x80u, vout = p[x - 1] ^ 0x80u;
-                b += vin * vin - vout * vout;
-                o[x] = b;
-            }
-        }
-    }
-
-    // correlation and window sums of I' on MFMA
     const int n = lane & 31, g = lane >> 5;
-    v16i acc, acs;
-#pragma unroll
-    for (int r = 0; r < 16; r++) acc[r] = acs[r] = 0;
+    const int nchunk = (th + tcm - 1) / tcm;
+    const int tc = (th + nchunk - 1) / nchunk;  // rows per chunk (the last may have fewer)
+    const int rmax = 32 * RT + min(th, tcm) - 1;  // staged rows of a chunk, at most
+    int8_t *blk = reinterpret_cast<int8_t *>(smem);
+    const int tp_off = (rmax * PITCH + 15) & ~15;
+    uint32_t *tp = reinterpret_cast<uint32_t *>(smem + tp_off);
+    uint32_t *h2 = reinterpret_cast<uint32_t *>(
+        smem + ((tp_off + min(th, tcm) * kTP + 16 + 15) & ~15));
+    const bool aligned = (W & 3) == 0 && (reinterpret_cast<uintptr_t>(img) & 3) == 0 && W >= 4;
+    const int nw = tw >> 2;
+    const uint32_t tail = (tw & 3) ? (1u << (8 * (tw & 3))) - 1u : 0u;
+
     // all-ones band: B[k][n] = 1 for 0 <= 32 kb + k - n < tw
     v4i ones[KB];
 #pragma unroll
@@ -773,63 +702,156 @@ __global__ __launch_bounds__(256) void match_i8_fused_kernel(
             }
             ones[kb][d] = (int)wv;
         }
-    const int8_t *arow = blk + (wave * 32 + n) * PITCH + 16 * g;
     const int u0 = 32 + 16 * g - n;
     const uint32_t sh = (uint32_t)(u0 & 3);
-    const uint32_t *trow = tp + (u0 >> 2);
-    for (int dy = 0; dy < th; dy++) {
+    v16i acc, acs;
 #pragma unroll
-        for (int kb = 0; kb < KB; kb++) {
-            const uint32_t *q = trow + 8 * kb;
-            const uint32_t d0 = q[0], d1 = q[1], d2 = q[2], d3 = q[3], d4 = q[4];
-            v4i bt;
-            bt[0] = (int)__builtin_amdgcn_alignbyte(d1, d0, sh);
-            bt[1] = (int)__builtin_amdgcn_alignbyte(d2, d1, sh);
-            bt[2] = (int)__builtin_amdgcn_alignbyte(d3, d2, sh);
-            bt[3] = (int)__builtin_amdgcn_alignbyte(d4, d3, sh);
-            const v4i a = *reinterpret_cast<const v4i *>(arow + 32 * kb);
-            acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, bt, acc, 0, 0, 0);
-            acs = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, ones[kb], acs, 0, 0, 0);
-        }
-        arow += PITCH;
-        trow += kTP / 4;
-    }
-
-    uint32_t st = 0, stt = 0;
-    for (int w = 0; w < RT; w++) {
-        st += (uint32_t)tred[0][w];
-        stt += (uint32_t)tred[1][w];
-    }
-    if constexpr (kSums) __syncthreads();  // every row's horizontal sums are in h2
-    // lane (n, g) holds rows 32w + 8j + 4g + i (register 4j + i) of column n:
-    // slide the vertical th-window sum of h2 down rows 32w + 4g .. 32w + 4g + 27
-    uint32_t sq = 0;
-    const uint32_t *hc = h2 + (wave * 32 + 4 * g) * kFusedHP + n;
-    if constexpr (kSums)
-        for (int dy = 0; dy < th; dy++) sq += hc[dy * kFusedHP];
-    const int x = x0 + n;
-    const int ybase = y0 + wave * 32 + 4 * g;
-    // the 16 window energies first (no branch between the LDS reads), then
-    // the scores, then the stores
+    for (int r = 0; r < 16; r++) acc[r] = acs[r] = 0;
     uint32_t sqv[16];
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
+    for (int r = 0; r < 16; r++) sqv[r] = 0;
+    uint32_t s1 = 0, s2 = 0;  // this thread's share of sum T, sum T^2
+
+    for (int c0 = 0; c0 < th; c0 += tc) {
+        const int dyn = min(tc, th - c0);
+        const int rows = 32 * RT + dyn - 1;
+        if (c0 > 0) __syncthreads();  // the previous chunk is done with blk, tp and h2
+        // image rows y0 + c0.. (clamped) and columns x0.. (clamped), as i8; bytes
+        // outside the image meet zero Toeplitz entries or discarded outputs
+        const int total = rows * (BW / 4);
+        for (int base = threadIdx.x; base < total; base += 8 * (int)blockDim.x) {
+            uint32_t v[8];
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int yl = 8 * j + i;  // row offset from 32w + 4g
-            sqv[4 * j + i] = sq;
-            if constexpr (kSums) {
-                // to row yl + 1 (past the other half's four rows after i = 3)
-                if (j < 3 || i < 3) {
+            for (int b = 0; b < 8; b++) {
+                const int i = min(base + b * (int)blockDim.x, total - 1);
+                const int r = i / (BW / 4), c = (i - r * (BW / 4)) * 4;
+                const uint8_t *row = img + (size_t)min(y0 + c0 + r, H - 1) * W;
+                const int gx = x0 + c;
+                if (aligned) {
+                    v[b] = *reinterpret_cast<const uint32_t *>(row + min(gx, W - 4));
+                } else {
+                    v[b] = (uint32_t)row[min(gx, W - 1)] | ((uint32_t)row[min(gx + 1, W - 1)] << 8) |
+                           ((uint32_t)row[min(gx + 2, W - 1)] << 16) |
+                           ((uint32_t)row[min(gx + 3, W - 1)] << 24);
+                }
+            }
 #pragma unroll
-                    for (int k = 0; k < ((i == 3) ? 5 : 1); k++) {
-                        const int r = yl + k;
-                        sq += hc[(r + th) * kFusedHP] - hc[r * kFusedHP];
+            for (int b = 0; b < 8; b++) {
+                const int i = base + b * (int)blockDim.x;
+                if (i < total) {
+                    const int r = i / (BW / 4), c = (i - r * (BW / 4)) * 4;
+                    *reinterpret_cast<uint32_t *>(blk + r * PITCH + c) = v[b] ^ 0x80808080u;
+                }
+            }
+        }
+        // template rows c0.. as i8 at [32, 32 + tw) of kTP-byte rows (zeros around)
+        for (int i = threadIdx.x; i < dyn * (kTP / 4) + 4; i += blockDim.x) {
+            const int r = i / (kTP / 4), c = 4 * (i - r * (kTP / 4)) - 32;
+            uint32_t wv = 0;
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+                if (r < dyn && c + b >= 0 && c + b < tw) {
+                    const uint32_t t = tpl[(c0 + r) * tw + c + b];
+                    s1 += t;
+                    s2 += t * t;
+                    wv |= (t ^ 0x80u) << (8 * b);
+                }
+            tp[i] = wv;
+        }
+        __syncthreads();
+
+        // horizontal tw-window sums of squares of the chunk's rows, 16 outputs
+        // per task: columns 16q .. 16q + 15 (the tile's 32 output columns)
+        if constexpr (kSums) {
+            for (int it = threadIdx.x; it < rows * 2; it += blockDim.x) {
+                const int r = it >> 1, q = it & 1;
+                const uint32_t *pw = reinterpret_cast<const uint32_t *>(blk + r * PITCH + 16 * q);
+                uint32_t b = 0;
+                for (int k = 0; k < nw; k++) {
+                    const uint32_t wv = pw[k] ^ 0x80808080u;
+                    b = __builtin_amdgcn_udot4(wv, wv, b, false);
+                }
+                if (tail) {
+                    const uint32_t wv = (pw[nw] ^ 0x80808080u) & tail;
+                    b = __builtin_amdgcn_udot4(wv, wv, b, false);
+                }
+                const uint8_t *p = reinterpret_cast<const uint8_t *>(pw);
+                uint32_t *o = h2 + r * kFusedHP + 16 * q;
+                o[0] = b;
+#pragma unroll
+                for (int x = 1; x < 16; x++) {
+                    const uint32_t vin = p[x + tw - 1] ^ 0x80u, vout = p[x - 1] ^ 0x80u;
+                    b += vin * vin - vout * vout;
+                    o[x] = b;
+                }
+            }
+        }
+
+        // correlation and window sums of I' on MFMA over the chunk's template rows
+        const int8_t *arow = blk + (wave * 32 + n) * PITCH + 16 * g;
+        const uint32_t *trow = tp + (u0 >> 2);
+        for (int dy = 0; dy < dyn; dy++) {
+#pragma unroll
+            for (int kb = 0; kb < KB; kb++) {
+                const uint32_t *q = trow + 8 * kb;
+                const uint32_t d0 = q[0], d1 = q[1], d2 = q[2], d3 = q[3], d4 = q[4];
+                v4i bt;
+                bt[0] = (int)__builtin_amdgcn_alignbyte(d1, d0, sh);
+                bt[1] = (int)__builtin_amdgcn_alignbyte(d2, d1, sh);
+                bt[2] = (int)__builtin_amdgcn_alignbyte(d3, d2, sh);
+                bt[3] = (int)__builtin_amdgcn_alignbyte(d4, d3, sh);
+                const v4i a = *reinterpret_cast<const v4i *>(arow + 32 * kb);
+                acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, bt, acc, 0, 0, 0);
+                acs = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, ones[kb], acs, 0, 0, 0);
+            }
+            arow += PITCH;
+            trow += kTP / 4;
+        }
+
+        if constexpr (kSums) {
+            __syncthreads();  // every row's horizontal sums are in h2
+            // lane (n, g) holds rows 32w + 8j + 4g + i (register 4j + i) of column
+            // n: slide the chunk's dyn-row sum of h2 down rows 32w + 4g .. + 27
+            const uint32_t *hc = h2 + (wave * 32 + 4 * g) * kFusedHP + n;
+            uint32_t sq = 0;
+            for (int d = 0; d < dyn; d++) sq += hc[d * kFusedHP];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int yl = 8 * j + i;  // row offset from 32w + 4g
+                    sqv[4 * j + i] += sq;
+                    // to row yl + 1 (past the other half's four rows after i = 3)
+                    if (j < 3 || i < 3) {
+#pragma unroll
+                        for (int k = 0; k < ((i == 3) ? 5 : 1); k++) {
+                            const int r = yl + k;
+                            sq += hc[(r + dyn) * kFusedHP] - hc[r * kFusedHP];
+                        }
                     }
                 }
             }
         }
     }
+
+    long long l1 = s1, l2 = s2;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        l1 += __shfl_xor(l1, o);
+        l2 += __shfl_xor(l2, o);
+    }
+    if (lane == 0) {
+        tred[0][wave] = l1;
+        tred[1][wave] = l2;
+    }
+    __syncthreads();
+    uint32_t st = 0, stt = 0;
+    for (int w = 0; w < RT; w++) {
+        st += (uint32_t)tred[0][w];
+        stt += (uint32_t)tred[1][w];
+    }
+    const int x = x0 + n;
+    const int ybase = y0 + wave * 32 + 4 * g;
     auto emit = [&](auto f64) {
         constexpr bool F64 = decltype(f64)::value;
         float v[16];
@@ -852,26 +874,76 @@ __global__ __launch_bounds__(256) void match_i8_fused_kernel(
     else emit(std::false_type{});
 }
 
+template <int KB, int METHOD>
+__global__ __launch_bounds__(256) void match_i8_fused_kernel(
+    const uint8_t *__restrict__ img, int W, int H, const uint8_t *__restrict__ tpl,
+    float *__restrict__ out, int ow, int oh, int tw, int th, int tcm) {
+    match_i8_fused_tile<KB, METHOD>(img, W, H, tpl, out, ow, oh, tw, th, tcm, blockIdx.x * 32,
+                                    blockIdx.y * 32 * (int)(blockDim.x >> 6));
+}
+
+// The same over a device table of matches (one KB and method): job j owns
+// tiles [tile0, tile0 + tiles_x * tiles_y) of the launch's tile list, which
+// is dealt to the XCDs in eight contiguous ranges (a job's neighbouring tiles,
+// which share image rows, meet in one L2).
+template <int KB, int METHOD>
+__global__ __launch_bounds__(256) void match_i8_fused_jobs_kernel(const MatchJob *__restrict__ jobs,
+                                                                  int njobs, int ntiles,
+                                                                  int tcm) {
+    const int chunk = (ntiles + 7) >> 3;
+    const int t = (int)(blockIdx.x & 7) * chunk + (int)(blockIdx.x >> 3);
+    if ((int)(blockIdx.x >> 3) >= chunk || t >= ntiles) return;
+    int lo = 0, hi = njobs - 1;  // the last job with tile0 <= t
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].tile0 <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const MatchJob &jb = jobs[lo];
+    const int lt = t - jb.tile0, ty = lt / jb.tiles_x, tx = lt - ty * jb.tiles_x;
+    match_i8_fused_tile<KB, METHOD>(jb.img, jb.W, jb.H, jb.tpl, jb.out, jb.ow, jb.oh, jb.tw,
+                                    jb.th, tcm, tx * 32, ty * 32 * (int)(blockDim.x >> 6));
+}
+
 constexpr size_t kFusedMaxLds = 64 * 1024;
+// Footprint that still lets three 4-wave workgroups share a CU's 160 KiB.
+constexpr size_t kFusedOccLds = 53 * 1024;
+constexpr int kFusedMinChunk = 32;  // chunks are never forced below this
 
 // Output rows per fused workgroup (32 * RT): the batch hint or 4 waves,
-// halved while the LDS footprint is above kFusedMaxLds; 0 when even one
-// wave does not fit (the template is too tall or wide for the fused kernel).
+// halved while the footprint of 32-row chunks is above kFusedMaxLds; 0 when
+// even one wave does not fit.
 int fused_rt(int tw, int th, int method) {
     const int KB = (31 + tw + 31) / 32;
     int RT = g_match_rows_hint > 0 ? std::min(g_match_rows_hint, 4) : 4;
-    while (RT > 0 && fused_lds(KB, tw, th, RT, method != IPH_CCORR) > kFusedMaxLds) RT /= 2;
+    while (RT > 0 &&
+           fused_lds(KB, tw, th, RT, method != IPH_CCORR, kFusedMinChunk) > kFusedMaxLds)
+        RT /= 2;
     return RT;
+}
+
+// Largest chunk height (near-equal chunks of th) whose footprint is at most
+// kFusedOccLds, else kFusedMinChunk: every chunk restages the image rows it
+// meets, so fewer chunks save staging and horizontal sums.
+int fused_tc(int KB, int tw, int th, int RT, bool sums) {
+    for (int nc = 1; nc <= th; nc++) {
+        const int tc = (th + nc - 1) / nc;
+        if (tc < kFusedMinChunk && nc > 1) break;
+        if (fused_lds(KB, tw, th, RT, sums, tc) <= kFusedOccLds) return tc;
+    }
+    return std::min(th, kFusedMinChunk);
 }
 
 template <int KB>
 iph_status launch_i8_fused_kb(const uint8_t *img, int W, int H, const uint8_t *tpl, float *out,
                               int ow, int oh, int tw, int th, int method, int RT,
                               hipStream_t s) {
-    const size_t lds = fused_lds(KB, tw, th, RT, method != IPH_CCORR);
+    const bool sums = method != IPH_CCORR;
+    const int tc = fused_tc(KB, tw, th, RT, sums);
+    const size_t lds = fused_lds(KB, tw, th, RT, sums, tc);
     dim3 grid((ow + 31) / 32, (oh + 32 * RT - 1) / (32 * RT));
 #define L(M)                                                                                 \
-    match_i8_fused_kernel<KB, M><<<grid, 64 * RT, lds, s>>>(img, W, H, tpl, out, ow, oh, tw, th)
+    match_i8_fused_kernel<KB, M><<<grid, 64 * RT, lds, s>>>(img, W, H, tpl, out, ow, oh, tw, th, tc)
     switch (method) {
     case IPH_SSE: L(IPH_SSE); break;
     case IPH_SSE_NORMED: L(IPH_SSE_NORMED); break;
@@ -996,5 +1068,68 @@ iph_status launch_mfma_match(const void *img, int32_t p_type, uint32_t w, uint32
 }
 
 void set_match_rows_hint(int rows) { g_match_rows_hint = rows; }
+
+int fused_match_kb(uint32_t tw) { return blocks_for((int)tw); }
+
+int fused_match_rt_max(uint32_t tw, uint32_t th, int32_t method) {
+    if (tw == 0 || th == 0 || tw > (uint32_t)kMaxSide || th > (uint32_t)kMaxSide) return 0;
+    const int KB = blocks_for((int)tw);
+    for (int rt = 4; rt > 0; rt /= 2)
+        if (fused_lds(KB, (int)tw, (int)th, rt, method != IPH_CCORR, kFusedMinChunk) <=
+            kFusedMaxLds)
+            return rt;
+    return 0;
+}
+
+int fused_match_tc(int kb, uint32_t th, int rt, int32_t method) {
+    return fused_tc(kb, 0, (int)th, rt, method != IPH_CCORR);
+}
+
+size_t fused_match_lds(int kb, uint32_t tw, uint32_t th, int rt, int32_t method, int tc) {
+    return fused_lds(kb, (int)tw, (int)th, rt, method != IPH_CCORR, tc);
+}
+
+long long match_jobs_layout(MatchJob *jobs, int n, int rt) {
+    long long t = 0;
+    for (int i = 0; i < n; i++) {
+        MatchJob &j = jobs[i];
+        j.tiles_x = (j.ow + 31) / 32;
+        j.tile0 = (int)t;
+        t += (long long)j.tiles_x * ((j.oh + 32 * rt - 1) / (32 * rt));
+    }
+    return t;
+}
+
+iph_status launch_match_jobs(const MatchJob *jobs, int n, long long ntiles, int kb,
+                             int32_t method, int rt, int tc, size_t lds, hipStream_t s) {
+    if (n <= 0 || ntiles <= 0) return IPH_OK;
+    if (ntiles > (1LL << 30) || kb < 1 || kb > 5 || (rt != 1 && rt != 2 && rt != 4) || tc < 1 ||
+        lds > kFusedMaxLds) {
+        set_error("match job table out of range (%lld tiles, kb %d, rt %d, chunk %d, lds %zu)",
+                  ntiles, kb, rt, tc, lds);
+        return IPH_EINVAL;
+    }
+    const unsigned grid = (unsigned)((ntiles + 7) / 8 * 8);
+#define J(K, M)                                                                              \
+    match_i8_fused_jobs_kernel<K, M><<<grid, 64 * rt, lds, s>>>(jobs, n, (int)ntiles, tc)
+#define JK(K)                                                                                \
+    switch (method) {                                                                        \
+    case IPH_SSE: J(K, IPH_SSE); break;                                                      \
+    case IPH_SSE_NORMED: J(K, IPH_SSE_NORMED); break;                                        \
+    case IPH_CCORR: J(K, IPH_CCORR); break;                                                  \
+    default: J(K, IPH_CCORR_NORMED); break;                                                  \
+    }                                                                                        \
+    break
+    switch (kb) {
+    case 1: JK(1);
+    case 2: JK(2);
+    case 3: JK(3);
+    case 4: JK(4);
+    default: JK(5);
+    }
+#undef JK
+#undef J
+    return launch_status("match_i8_fused_jobs_kernel");
+}
 
 }  // namespace iph

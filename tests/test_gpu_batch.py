@@ -128,3 +128,46 @@ def test_batch_many_small_and_one_large_image():
         assert torch.equal(b, dev.gaussian_blur_f32(im, 1.7))
         assert torch.equal(f, dev.filter_clamped(im, Kernel(k, 5, 5), torch.uint8))
     bt.close()
+
+
+def _check_match_vs_oracle(got, img, tpl, m):
+    """The int8 MFMA bounds of test_gpu_match.py: SSE and CC are the exact
+    integer sums rounded once to f32; the normalised methods are within 4 f32
+    ulps of FP64 below 64x64 windows and 0.5 ulp from there on."""
+    exact = O.match_template_f64(img, tpl, int(m))
+    if m in (M.SumOfSquaredErrors, M.CrossCorrelation):
+        assert np.array_equal(got.view(np.uint32), exact.astype(np.float32).view(np.uint32)), m
+    else:
+        rel = np.abs(got - exact) / np.maximum(np.abs(exact), 1e-30)
+        assert rel.max() <= (4.0 if tpl.size < 4096 else 0.5) * 2.0 ** -23 + 1e-30, (m, rel.max())
+    if tpl.size * 65025 < 2 ** 24:
+        assert np.array_equal(got.view(np.uint32),
+                              O.match_template(img, tpl, int(m), nthreads=8).view(np.uint32)), m
+
+
+def test_batch_match_jobs_equal_oracle():
+    """Batched matches against the oracle: templates of 8 to 128 px (every K-block
+    count of the fused job-table kernel, and 128 x 128, which takes the
+    two-kernel path), all four methods, images of uneven sizes."""
+    rng = np.random.default_rng(15)
+    shapes = [((150, 170), (8, 8)), ((200, 160), (40, 33)), ((180, 210), (65, 66)),
+              ((230, 190), (96, 90)), ((240, 260), (100, 97)), ((140, 150), (128, 128)),
+              ((97, 301), (31, 64)), ((260, 75), (17, 70))]
+    bt = Batch()
+    cases = []
+    for k, ((h, w), (th, tw)) in enumerate(shapes):
+        img = rng.integers(0, 256, (h, w), dtype=np.uint8)
+        tpl = rng.integers(0, 256, (th, tw), dtype=np.uint8)
+        gi, gt = torch.from_numpy(img).cuda(), torch.from_numpy(tpl).cuda()
+        for m in M:
+            cases.append((img, tpl, m, bt.add_match_template(gi, gt, m)))
+    bt.plan(autotune=True)
+    for _ in range(2):
+        for _, _, _, o in cases:
+            o.fill_(-1.0)
+        bt.run()
+        torch.cuda.synchronize()
+        for img, tpl, m, o in cases:
+            _check_match_vs_oracle(o.cpu().numpy(), img, tpl, m)
+    assert "match table" in bt.tuning()
+    bt.close()
