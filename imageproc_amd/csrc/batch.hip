@@ -9,8 +9,8 @@
 // its images), the filter jobs one table of the ragged 2-D filter kernel (one
 // launch), the match jobs device tables of the fused int8 match kernel (one
 // launch per K-block count and method, each walking the tiles of all its
-// pairs); pairs that kernel does not take (templates whose staged block
-// exceeds its LDS budget) run the two-kernel path, one pipeline each. The
+// pairs); pairs that kernel does not take (templates above 128 px, images
+// beyond its 32-bit offsets) run the direct kernel, one launch each. The
 // blurs run on the caller's stream, the filters and the match tables on side
 // streams forked from it (the remaining pairs spread over six more, largest
 // first onto the least loaded) and joined back, captured once into a graph.
@@ -66,7 +66,7 @@ struct iph_batch {
     std::vector<MatchSpec> match;
     std::vector<iph::MatchJob> mjobs;     // fused-kernel matches, grouped (host copy)
     std::vector<MatchGroup> mgroups;
-    std::vector<int> match_fallback;      // matches on the two-kernel path
+    std::vector<int> match_fallback;      // matches on the direct path
     size_t mjob_table_off = 0;
 
     bool planned = false;
@@ -78,14 +78,14 @@ struct iph_batch {
     std::vector<FallbackBlur> blur_fallback;
     int filt_tiles = 0;
     uint32_t filt_max_kw = 1, filt_max_kh = 1;
-    int blur_grid_pct = 100, filt_variant = 0, match_rows = 0;
+    int blur_grid_pct = 100, filt_variant = 0;
     // side[0]: the filters; side[1]: the match tables; side[2 .. kSide - 1]: the
-    // matches on the two-kernel path
+    // matches on the direct path
     static constexpr int kSide = 8;
     hipStream_t side[kSide] = {};
     hipStream_t capture = nullptr;  // the graph is captured here (the caller's may be the null stream)
     hipEvent_t fork = nullptr, join[kSide] = {};
-    std::vector<int> match_lane;  // side stream (2 ..) of each two-kernel-path match
+    std::vector<int> match_lane;  // side stream (2 ..) of each direct-path match
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::string tuning;
@@ -137,25 +137,20 @@ iph_status enqueue_match_group(iph_batch *b, const MatchGroup &g, hipStream_t s)
                              g.lds, s);
 }
 
-// The match jobs: the fused-kernel tables, then the two-kernel-path pairs;
-// with `spread`, the tables on side[1] and each pair on its side stream,
-// else all on s.
+// The match jobs: the fused-kernel tables, then the pairs the fused kernel
+// does not take (templates above 128 px, or images beyond its offsets: the
+// direct path); with `spread`, the tables on side[1] and each pair on its
+// side stream, else all on s.
 iph_status enqueue_matches(iph_batch *b, hipStream_t s, bool spread = false) {
     for (const MatchGroup &g : b->mgroups)
         IPH_TRY(enqueue_match_group(b, g, spread ? b->side[1] : s));
-    set_match_rows_hint(b->match_rows);
     for (size_t k = 0; k < b->match_fallback.size(); k++) {
         const MatchSpec &m = b->match[b->match_fallback[k]];
         hipStream_t ms = spread ? b->side[b->match_lane[k]] : s;
-        iph_status st = launch_match(m.img, IPH_U8, m.w, m.h, m.tpl, m.tw, m.th, m.method,
-                                     resolve_path(IPH_PATH_AUTO, m.tw, m.th), m.out,
-                                     b->mem + m.ws_off, m.ws_bytes, ms);
-        if (st != IPH_OK) {
-            set_match_rows_hint(0);
-            return st;
-        }
+        IPH_TRY(launch_match(m.img, IPH_U8, m.w, m.h, m.tpl, m.tw, m.th, m.method,
+                             resolve_path(IPH_PATH_AUTO, m.w, m.h, m.tw, m.th), m.out,
+                             b->mem + m.ws_off, m.ws_bytes, ms));
     }
-    set_match_rows_hint(0);
     return IPH_OK;
 }
 
@@ -260,7 +255,7 @@ iph_status autotune(iph_batch *b, hipStream_t s) {
     for (MatchGroup &g : b->mgroups) {
         int best_r = g.rt_max;
         float best_ms = 1e30f;
-        for (int rt : {4, 2, 1}) {
+        for (int rt : {8, 4, 2, 1}) {
             if (rt > g.rt_max) continue;
             g.rt = rt;
             IPH_TRY(layout_match_group(b, g));
@@ -280,23 +275,6 @@ iph_status autotune(iph_batch *b, hipStream_t s) {
         IPH_TRY(layout_match_group(b, g));
         std::lock_guard<std::mutex> lock(g_match_cache_mu);
         g_match_cache[match_class(g)] = best_r;
-    }
-    if (!b->match_fallback.empty()) {
-        int best_r = 0;
-        float best_ms = 1e30f;
-        for (int rows : {4, 2, 1}) {
-            b->match_rows = rows;
-            float ms;
-            IPH_TRY(time_min(s, 3, [&] { return enqueue_matches(b, s); }, &ms));
-            std::snprintf(line, sizeof line, "two-kernel match rows/workgroup %d: %.3f ms\n",
-                          32 * rows, ms);
-            b->tuning += line;
-            if (ms < best_ms) {
-                best_ms = ms;
-                best_r = rows;
-            }
-        }
-        b->match_rows = best_r;
     }
     if (!b->blur_groups.empty()) {
         int best_p = 100;
@@ -350,7 +328,7 @@ iph_status plan(iph_batch *b, int32_t tune, hipStream_t s) {
         b->filt_max_kh = std::max(b->filt_max_kh, f.kh);
     }
     // matches: the fused kernel's tables, grouped by (K blocks, method); the
-    // rest on the two-kernel path with a workspace slice each
+    // rest on the direct path with a workspace slice each
     b->mjobs.clear();
     b->mgroups.clear();
     b->match_fallback.clear();
@@ -361,11 +339,12 @@ iph_status plan(iph_batch *b, int32_t tune, hipStream_t s) {
         std::vector<Entry> entries;
         for (size_t i = 0; i < b->match.size(); i++) {
             MatchSpec &m = b->match[i];
-            const int rt_max = fused_match_rt_max(m.tw, m.th, m.method);
+            const int rt_max =
+                mfma_match_fits(m.w, m.h) ? fused_match_rt_max(m.tw, m.th, m.method) : 0;
             if (rt_max == 0) {
                 b->match_fallback.push_back((int)i);
                 m.ws_bytes = match_workspace(IPH_U8, m.w, m.h, m.tw, m.th, m.method,
-                                             resolve_path(IPH_PATH_AUTO, m.tw, m.th));
+                                             resolve_path(IPH_PATH_AUTO, m.w, m.h, m.tw, m.th));
                 m.ws_off = align_up(wp.used, 256);
                 wp.take<unsigned char>(std::max<size_t>(m.ws_bytes, 1));
                 continue;
@@ -377,7 +356,7 @@ iph_status plan(iph_batch *b, int32_t tune, hipStream_t s) {
         for (const Entry &e : entries) {
             const MatchSpec &m = b->match[e.idx];
             if (b->mgroups.empty() || b->mgroups.back().kb * 4 + b->mgroups.back().method != e.key)
-                b->mgroups.push_back({e.key / 4, e.key % 4, (int)b->mjobs.size(), 0, 4, 4, 32,
+                b->mgroups.push_back({e.key / 4, e.key % 4, (int)b->mjobs.size(), 0, 8, 8, 32,
                                       0, 0, 0u, 0u, 0.0});
             MatchGroup &g = b->mgroups.back();
             MatchJob j{};
@@ -409,16 +388,17 @@ iph_status plan(iph_batch *b, int32_t tune, hipStream_t s) {
     IPH_TRY_HIP(hipMemcpy(b->mem + b->blur_table_off, blur_host.data(), blur_host.size(),
                           hipMemcpyHostToDevice));
     IPH_TRY(upload_filter_table(b, 0));
-    // match tables: the cached choice for their shape class, else the largest fit
+    // match tables: the cached choice for their shape class, else the shape a
+    // single call of the group's largest template would take
     for (MatchGroup &g : b->mgroups) {
-        g.rt = g.rt_max;
+        g.rt = std::min(g.rt_max, fused_match_rt_default(g.max_tw, g.max_th, g.method));
         g.ntiles = match_jobs_layout(b->mjobs.data() + g.first, g.count, g.rt);
         std::lock_guard<std::mutex> lock(g_match_cache_mu);
         auto it = g_match_cache.find(match_class(g));
         if (it != g_match_cache.end() && it->second <= g.rt_max) g.rt = it->second;
     }
     for (MatchGroup &g : b->mgroups) IPH_TRY(layout_match_group(b, g));
-    // two-kernel-path pairs to their streams: largest first, each to the least loaded
+    // direct-path pairs to their streams: largest first, each to the least loaded
     {
         const int nl = iph_batch::kSide - 2;
         const size_t nf = b->match_fallback.size();
@@ -445,7 +425,6 @@ iph_status plan(iph_batch *b, int32_t tune, hipStream_t s) {
     IPH_TRY_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
     IPH_TRY_HIP(hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking));
     b->blur_grid_pct = 100;
-    b->match_rows = 0;
     b->tuning = "not tuned\n";
     if (tune) IPH_TRY(autotune(b, s));
     IPH_TRY_HIP(hipStreamSynchronize(s));
@@ -568,10 +547,6 @@ IPH_API iph_status iph_batch_add_match_template(iph_batch *b, const uint8_t *ima
     clear_error();
     IPH_TRY(check_batch(b));
     IPH_TRY(check_match(IPH_U8, w, h, tw, th, method, IPH_PATH_AUTO));
-    if (tw > 128 || th > 128) {
-        set_error("batched match_template takes templates up to 128x128");
-        return IPH_EUNSUPPORTED;
-    }
     unplan(b);
     b->match.push_back({image, w, h, tpl, tw, th, method, out, 0, 0});
     return IPH_OK;

@@ -176,12 +176,15 @@ iph_status check_match(int32_t p, uint32_t w, uint32_t h, uint32_t tw, uint32_t 
 
 // Empty templates have no window to correlate: the direct path handles them.
 // AUTO takes the MFMA engines where they apply (template sides up to
-// kMfmaMaxSide) and the direct kernel otherwise, so every template the
-// reference accepts (template_matching.rs:394-404) has a GPU path.
-int32_t resolve_path(int32_t path, uint32_t tw, uint32_t th) {
+// kMfmaMaxSide, images within mfma_match_fits) and the direct kernel
+// otherwise, so every template the reference accepts
+// (template_matching.rs:394-404) has a GPU path.
+int32_t resolve_path(int32_t path, uint32_t w, uint32_t h, uint32_t tw, uint32_t th) {
     if (tw == 0 || th == 0) return IPH_PATH_DIRECT;
     if (path != IPH_PATH_AUTO) return path;
-    return (tw <= kMfmaMaxSide && th <= kMfmaMaxSide) ? IPH_PATH_MFMA : IPH_PATH_DIRECT;
+    return (tw <= kMfmaMaxSide && th <= kMfmaMaxSide && mfma_match_fits(w, h))
+               ? IPH_PATH_MFMA
+               : IPH_PATH_DIRECT;
 }
 
 }  // namespace iph
@@ -575,7 +578,7 @@ IPH_API iph_status iph_match_template(const void *image, int32_t p, uint32_t w, 
                                       int32_t method, int32_t path, float *out) {
     clear_error();
     IPH_TRY(check_match(p, w, h, tw, th, method, path));
-    path = resolve_path(path, tw, th);
+    path = resolve_path(path, w, h, tw, th);
     uint32_t ow = w - tw + 1, oh = h - th + 1;
     size_t obytes = (size_t)ow * oh * sizeof(float);
     if (obytes == 0) return IPH_OK;
@@ -763,7 +766,7 @@ IPH_API iph_status iph_dev_match_template_with_mask(const uint8_t *image, uint32
 IPH_API size_t iph_dev_match_template_workspace(int32_t p, uint32_t w, uint32_t h, uint32_t tw,
                                                 uint32_t th, int32_t method, int32_t path) {
     if (w < tw || h < th) return 0;
-    return match_workspace(p, w, h, tw, th, method, resolve_path(path, tw, th));
+    return match_workspace(p, w, h, tw, th, method, resolve_path(path, w, h, tw, th));
 }
 
 IPH_API iph_status iph_dev_match_template(const void *image, int32_t p, uint32_t w, uint32_t h,
@@ -773,7 +776,7 @@ IPH_API iph_status iph_dev_match_template(const void *image, int32_t p, uint32_t
     clear_error();
     IPH_TRY(check_match(p, w, h, tw, th, method, path));
     IPH_TRY(require_device());
-    return launch_match(image, p, w, h, tmpl, tw, th, method, resolve_path(path, tw, th), out,
+    return launch_match(image, p, w, h, tmpl, tw, th, method, resolve_path(path, w, h, tw, th), out,
                         ws, wsb, static_cast<hipStream_t>(stream));
 }
 

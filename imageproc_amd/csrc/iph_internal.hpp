@@ -138,7 +138,12 @@ iph_status check_match(int32_t p, uint32_t w, uint32_t h, uint32_t tw, uint32_t 
                        int32_t method, int32_t path);
 // Largest template side of the MFMA match engines (u8 and f32).
 constexpr uint32_t kMfmaMaxSide = 128;
-int32_t resolve_path(int32_t path, uint32_t tw, uint32_t th);
+// Images the MFMA match engines take: 32-bit byte offsets into the image and
+// into 256 rows of the score map.
+inline bool mfma_match_fits(uint32_t w, uint32_t h) {
+    return (uint64_t)w * h + 64 < (1ull << 32) && w < (1u << 22);
+}
+int32_t resolve_path(int32_t path, uint32_t w, uint32_t h, uint32_t tw, uint32_t th);
 
 // ---- batches of uneven images (filter.hip, blur_mfma.hip, batch.hip) ------
 
@@ -181,10 +186,6 @@ iph_status blur_jobs_table(const BlurSpec *specs, int n, void *host_table,
 iph_status launch_blur_jobs(const void *table_dev, const std::vector<BlurGroup> &groups,
                             int grid_pct, hipStream_t s);
 
-// Rows of 32 outputs per workgroup of the int8 MFMA match kernel (0 = the
-// largest that fits in LDS); set by the batch autotuner for its launches.
-void set_match_rows_hint(int rows);
-
 // match_template (GrayImage, int8 MFMA with in-kernel window sums) of a batch:
 // one entry of the device job table of the fused kernel.
 struct MatchJob {
@@ -196,9 +197,11 @@ struct MatchJob {
 };
 // K blocks of the fused kernel for a template width.
 int fused_match_kb(uint32_t tw);
-// Largest waves per workgroup (4, 2, 1) whose LDS footprint the fused kernel
-// takes for this template and method, 0 if none (the two-kernel path then).
+// Largest waves per workgroup (8, 4, 2, 1) whose LDS footprint the fused
+// kernel takes for this template and method, 0 if none; and the shape a
+// single call picks.
 int fused_match_rt_max(uint32_t tw, uint32_t th, int32_t method);
+int fused_match_rt_default(uint32_t tw, uint32_t th, int32_t method);
 // Template rows per chunk for templates up to th rows high (the footprint then
 // fits three workgroups per CU where it can), and the LDS bytes of one job.
 int fused_match_tc(int kb, uint32_t th, int rt, int32_t method);

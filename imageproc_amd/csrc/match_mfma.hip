@@ -10,7 +10,8 @@
 // (zero outside the row). Both operands are offset by -128 to fit i8:
 //   sum I T = sum I'T' + 128 sum_window I + 128 sum T - 128^2 N,
 // with I' = I - 128, T' = T - 128, every term an exact integer. The window
-// sums sum I and sum I^2 come from a separate exact integer pass, so
+// sums sum I and sum I^2 come from the same workgroup (an all-ones Toeplitz
+// band on MFMA, and exact u32 sliding sums), so
 //   SumOfSquaredErrors  = sum T^2 - 2 C + sum I^2      (exact)
 //   CrossCorrelation    = C                             (exact)
 // each rounded once to f32, and the normalised methods divide by
@@ -36,264 +37,9 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
 constexpr int kMaxSide = (int)kMfmaMaxSide;  // largest template side on the int8 path
-static_assert(kMaxSide * 255 < 65536, "a template row's pixel sum fits 16 bits");
-constexpr int kWinRows = 64;   // output rows per window-sum workgroup
-constexpr int kWinCols = 64;   // output columns per window-sum workgroup
-constexpr int kHsPitch = kWinCols + 1;  // LDS row pitch of the horizontal sums (odd: no bank clash)
-constexpr int kWinThreads = 512;  // threads per window-sum workgroup (8 rows per column segment)
-
-__device__ __forceinline__ int align_up_dev(int v) { return (v + 15) & ~15; }
-// Staged bytes per row of the window-sum tile, a multiple of 4.
-__host__ __device__ __forceinline__ int win_cols(int tw) { return (kWinCols + tw - 1 + 3) & ~3; }
-
-// Exact window sums (sum I, sum I^2) for every output position of a
-// kWinCols x kWinRows output tile. The staged window (rows 4-byte aligned,
-// zeros past the image: those bytes only reach discarded outputs) gets its
-// horizontal tw-window sums per 16-column segment of a row: the first by
-// v_dot4 over whole words, then 15 sliding steps. Each output column then
-// slides its vertical th-window sum down a 16-row segment. All arithmetic
-// is exact u32; written coalesced.
-__global__ __launch_bounds__(kWinThreads) void window_sums_u8_kernel(const uint8_t *__restrict__ img,
-                                                             int W, int tw, int th, int ow,
-                                                             int oh, uint2 *__restrict__ ws) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int x0 = blockIdx.x * kWinCols, y0 = blockIdx.y * kWinRows;
-    const int cols = win_cols(tw), cw4 = cols >> 2, rows = kWinRows + th - 1;
-    const int H = oh + th - 1;
-    const bool aligned = (W & 3) == 0 && (reinterpret_cast<uintptr_t>(img) & 3) == 0;
-    uint32_t *blk = reinterpret_cast<uint32_t *>(smem);                      // rows x cw4 words
-    // horizontal sums: sum I (below 2^15 for tw <= 128) and sum I^2, rows x kHsPitch each
-    uint16_t *hsa = reinterpret_cast<uint16_t *>(smem + align_up_dev(rows * cols));
-    uint32_t *hsb = reinterpret_cast<uint32_t *>(smem + align_up_dev(rows * cols) +
-                                                 align_up_dev(rows * kHsPitch * 2));
-    // 16 loads per thread in flight
-    constexpr int kStage = 16;
-    for (int base = threadIdx.x; base < rows * cw4; base += kStage * kWinThreads) {
-        uint32_t v[kStage];
-#pragma unroll
-        for (int b = 0; b < kStage; b++) {
-            const int i = min(base + kWinThreads * b, rows * cw4 - 1);
-            const int r = i / cw4, gx = x0 + 4 * (i - r * cw4), gy = y0 + r;
-            uint32_t wv = 0;
-            if (gy < H) {
-                const uint8_t *row = img + (size_t)gy * W;
-                if (aligned && gx + 4 <= W) {
-                    wv = *reinterpret_cast<const uint32_t *>(row + gx);
-                } else {
-#pragma unroll
-                    for (int k = 0; k < 4; k++)
-                        if (gx + k < W) wv |= (uint32_t)row[gx + k] << (8 * k);
-                }
-            }
-            v[b] = wv;
-        }
-#pragma unroll
-        for (int b = 0; b < kStage; b++)
-            if (base + kWinThreads * b < rows * cw4) blk[base + kWinThreads * b] = v[b];
-    }
-    __syncthreads();
-    constexpr int kSegs = kWinCols / 16;
-    const int nw = tw >> 2;
-    const uint32_t tail = (tw & 3) ? (1u << (8 * (tw & 3))) - 1u : 0u;
-    for (int it = threadIdx.x; it < rows * kSegs; it += kWinThreads) {
-        const int r = it / kSegs, q = it - r * kSegs;
-        const uint32_t *pw = blk + r * cw4 + 4 * q;
-        uint32_t a = 0, b = 0;
-#pragma unroll 4
-        for (int k = 0; k < nw; k++) {
-            const uint32_t wv = pw[k];
-            a = __builtin_amdgcn_udot4(wv, 0x01010101u, a, false);
-            b = __builtin_amdgcn_udot4(wv, wv, b, false);
-        }
-        if (tail) {
-            const uint32_t wv = pw[nw] & tail;
-            a = __builtin_amdgcn_udot4(wv, 0x01010101u, a, false);
-            b = __builtin_amdgcn_udot4(wv, wv, b, false);
-        }
-        const uint8_t *p = reinterpret_cast<const uint8_t *>(pw);
-        uint16_t *oa = hsa + r * kHsPitch + 16 * q;
-        uint32_t *ob = hsb + r * kHsPitch + 16 * q;
-        oa[0] = (uint16_t)a;
-        ob[0] = b;
-#pragma unroll
-        for (int x = 1; x < 16; x++) {
-            uint32_t vin = p[x + tw - 1], vout = p[x - 1];
-            a += vin - vout;
-            b += vin * vin - vout * vout;
-            oa[x] = (uint16_t)a;
-            ob[x] = b;
-        }
-    }
-    __syncthreads();
-    const int x = threadIdx.x & (kWinCols - 1), seg = threadIdx.x / kWinCols;
-    constexpr int kSeg = kWinRows / (kWinThreads / kWinCols);
-    const int ya = seg * kSeg;
-    uint32_t a = 0, b = 0;
-#pragma unroll 8
-    for (int dy = 0; dy < th; dy++) {
-        a += hsa[(ya + dy) * kHsPitch + x];
-        b += hsb[(ya + dy) * kHsPitch + x];
-    }
-#pragma unroll
-    for (int y = ya; y < ya + kSeg; y++) {
-        const int gy = y0 + y, gx = x0 + x;
-        if (gy < oh && gx < ow) ws[(size_t)gy * ow + gx] = make_uint2(a, b);
-        const int in = y + th < rows ? (y + th) * kHsPitch + x : 0, out = y * kHsPitch + x;
-        a += (uint32_t)hsa[in] - (uint32_t)hsa[out];
-        b += hsb[in] - hsb[out];
-    }
-}
-
 // Bytes per template row staged in LDS: 32 zeros, the row, zeros up to
 // 32 * (KB + 1) (every Toeplitz slice of a lane reads inside it).
 __host__ __device__ constexpr int tpl_pitch(int KB) { return 32 * KB + 32; }
-
-// One wave computes a 32 x (32*XT) output strip; RT waves stack vertically.
-// The workgroup stages the template (as i8, zero-padded rows) next to its
-// image rows; each lane cuts its Toeplitz B operand for template row dy and
-// K block kb, B[k][n] = T[dy][32kb + k - n], out of the padded row with
-// five dword reads and v_alignbyte, and the workgroup sums T and T^2 itself.
-template <int KB, int METHOD, int XT>
-__global__ __launch_bounds__(256) void match_i8_mfma_kernel(
-    const uint8_t *__restrict__ img, int W, int H, const uint8_t *__restrict__ tpl,
-    const uint2 *__restrict__ wsum, float *__restrict__ out, int ow, int oh, int tw, int th,
-    int pitch) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ long long tred[2][4];
-    int8_t *blk = reinterpret_cast<int8_t *>(smem);
-    constexpr int BW = 32 * (XT + KB - 1);  // bytes per staged row
-    const int RT = blockDim.x >> 6;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int x0 = blockIdx.x * (32 * XT), y0 = blockIdx.y * 32 * RT;
-    const int rows = 32 * RT + th - 1;
-
-    // Stage image rows y0.. and columns x0.. as i8 = u8 - 128. Bytes outside
-    // the image only meet zero Toeplitz entries or produce discarded outputs.
-    const bool aligned = (W & 3) == 0 && (reinterpret_cast<uintptr_t>(img) & 3) == 0 && W >= 4;
-    const int total = rows * (BW / 4);
-    // loads are issued in batches of 8 before their LDS stores, so a
-    // workgroup waits for global latency once per batch, not per dword
-    for (int base = threadIdx.x; base < total; base += 8 * (int)blockDim.x) {
-        uint32_t v[8];
-#pragma unroll
-        for (int b = 0; b < 8; b++) {
-            const int i = min(base + b * (int)blockDim.x, total - 1);
-            const int r = i / (BW / 4), c = (i - r * (BW / 4)) * 4;
-            const uint8_t *row = img + (size_t)min(y0 + r, H - 1) * W;
-            const int gx = x0 + c;
-            if (aligned) {
-                v[b] = *reinterpret_cast<const uint32_t *>(row + min(gx, W - 4));
-            } else {
-                v[b] = (uint32_t)row[min(gx, W - 1)] | ((uint32_t)row[min(gx + 1, W - 1)] << 8) |
-                       ((uint32_t)row[min(gx + 2, W - 1)] << 16) |
-                       ((uint32_t)row[min(gx + 3, W - 1)] << 24);
-            }
-        }
-#pragma unroll
-        for (int b = 0; b < 8; b++) {
-            const int i = base + b * (int)blockDim.x;
-            if (i < total) {
-                const int r = i / (BW / 4), c = (i - r * (BW / 4)) * 4;
-                *reinterpret_cast<uint32_t *>(blk + r * pitch + c) = v[b] ^ 0x80808080u;
-            }
-        }
-    }
-    // template rows as i8 = u8 - 128 at [32, 32 + tw) of kTP-byte rows
-    constexpr int kTP = tpl_pitch(KB);
-    uint32_t *tp = reinterpret_cast<uint32_t *>(smem + ((rows * pitch + 15) & ~15));
-    uint32_t s1 = 0, s2 = 0;
-    for (int i = threadIdx.x; i < th * (kTP / 4) + 4; i += blockDim.x) {
-        const int r = i / (kTP / 4), c = 4 * (i - r * (kTP / 4)) - 32;
-        uint32_t wv = 0;
-#pragma unroll
-        for (int b = 0; b < 4; b++)
-            if (r < th && c + b >= 0 && c + b < tw) {
-                const uint32_t t = tpl[r * tw + c + b];
-                s1 += t;
-                s2 += t * t;
-                wv |= (t ^ 0x80u) << (8 * b);
-            }
-        tp[i] = wv;
-    }
-    long long l1 = s1, l2 = s2;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        l1 += __shfl_xor(l1, o);
-        l2 += __shfl_xor(l2, o);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        tred[0][threadIdx.x >> 6] = l1;
-        tred[1][threadIdx.x >> 6] = l2;
-    }
-    __syncthreads();
-
-    const int n = lane & 31, g = lane >> 5;
-    v16i acc[XT];
-#pragma unroll
-    for (int t = 0; t < XT; t++)
-#pragma unroll
-        for (int r = 0; r < 16; r++) acc[t][r] = 0;
-
-    const int8_t *arow = blk + (wave * 32 + n) * pitch + 16 * g;
-    // lane (n, g) feeds B rows k = 16g + j of column n: bytes
-    // 32 + 32kb + 16g - n .. +15 of the padded row, the same alignment every dy
-    const int u0 = 32 + 16 * g - n;
-    const uint32_t sh = (uint32_t)(u0 & 3);
-    const uint32_t *trow = tp + (u0 >> 2);
-    for (int dy = 0; dy < th; dy++) {
-        v4i b[KB];
-#pragma unroll
-        for (int kb = 0; kb < KB; kb++) {
-            const uint32_t *q = trow + 8 * kb;
-            const uint32_t d0 = q[0], d1 = q[1], d2 = q[2], d3 = q[3], d4 = q[4];
-            b[kb][0] = (int)__builtin_amdgcn_alignbyte(d1, d0, sh);
-            b[kb][1] = (int)__builtin_amdgcn_alignbyte(d2, d1, sh);
-            b[kb][2] = (int)__builtin_amdgcn_alignbyte(d3, d2, sh);
-            b[kb][3] = (int)__builtin_amdgcn_alignbyte(d4, d3, sh);
-        }
-#pragma unroll
-        for (int s = 0; s < XT + KB - 1; s++) {
-            v4i a = *reinterpret_cast<const v4i *>(arow + 32 * s);
-#pragma unroll
-            for (int t = 0; t < XT; t++) {
-                if (s - t >= 0 && s - t < KB)
-                    acc[t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b[s - t], acc[t], 0, 0, 0);
-            }
-        }
-        arow += pitch;
-        trow += kTP / 4;
-    }
-
-    long long st = 0, stt = 0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
-        st += tred[0][w];
-        stt += tred[1][w];
-    }
-    const long long nn = (long long)tw * th;
-    const long long base = 128LL * st - 16384LL * nn;
-#pragma unroll
-    for (int t = 0; t < XT; t++) {
-        const int x = x0 + 32 * t + n;
-        // all 16 window-sum loads of this column are issued before any use
-        // (clamped addresses, so none is conditional)
-        uint2 sums[16];
-#pragma unroll
-        for (int r = 0; r < 16; r++) {
-            // 32x32 accumulator layout: register r of lane l holds row
-            // 8*(r/4) + 4*(l/32) + r%4, column l%32.
-            const int y = y0 + wave * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
-            sums[r] = wsum[(size_t)min(y, oh - 1) * ow + min(x, ow - 1)];
-        }
-#pragma unroll
-        for (int r = 0; r < 16; r++) {
-            const int y = y0 + wave * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
-            long long c = (long long)acc[t][r] + 128LL * sums[r].x + base;
-            float v = finish_score<METHOD>(c, (long long)sums[r].y, stt, nn);
-            if (x < ow && y < oh) out[(size_t)y * ow + x] = v;
-        }
-    }
-}
-
 
 // ---- BF16x3 path for f32 images --------------------------------------------------
 
@@ -553,67 +299,15 @@ int blocks16_for(int tw) { return (31 + tw + 15) / 16; }
 
 int blocks_for(int tw) { return (31 + tw + 31) / 32; }
 
-size_t win_lds(int tw, int th) {
-    int cols = win_cols(tw), rows = kWinRows + th - 1;
-    return align_up((size_t)rows * cols, 16) + align_up((size_t)rows * kHsPitch * 2, 16) +
-           (size_t)rows * kHsPitch * sizeof(uint32_t);
-}
-
-thread_local int g_match_rows_hint = 0;
-
-template <int KB, int XT>
-iph_status launch_i8_kb_xt(const uint8_t *img, int W, int H, const uint8_t *tpl,
-                           const uint2 *wsum, float *out, int ow, int oh, int tw, int th,
-                           int method, hipStream_t s) {
-    constexpr int BW = 32 * (XT + KB - 1);
-    int pitch = ((BW / 16) % 2 == 0) ? BW + 16 : BW;  // odd multiple of 16: no bank conflicts
-    int RT = g_match_rows_hint > 0 ? std::min(g_match_rows_hint, 4) : 4;
-    const size_t tbytes = (size_t)th * tpl_pitch(KB) + 16;
-    auto lds_for = [&](int rt) { return align_up((size_t)(32 * rt + th - 1) * pitch, 16) + tbytes; };
-    while (RT > 1 && lds_for(RT) > 64 * 1024) RT /= 2;
-    const size_t lds = lds_for(RT);
-    dim3 grid((ow + 32 * XT - 1) / (32 * XT), (oh + 32 * RT - 1) / (32 * RT));
-#define L(M)                                                                                 \
-    match_i8_mfma_kernel<KB, M, XT><<<grid, 64 * RT, lds, s>>>(img, W, H, tpl, wsum, out, ow,  \
-                                                               oh, tw, th, pitch)
-    switch (method) {
-    case IPH_SSE: L(IPH_SSE); break;
-    case IPH_SSE_NORMED: L(IPH_SSE_NORMED); break;
-    case IPH_CCORR: L(IPH_CCORR); break;
-    default: L(IPH_CCORR_NORMED); break;
-    }
-#undef L
-    return launch_status("match_i8_mfma_kernel");
-}
-
-// 32-column output tiles per wave: two (each A operand feeds more MFMAs)
-// unless the map has enough 32x32 tiles to give every SIMD (1024 on the
-// chip) 16 waves of one tile each; the kernel is latency-bound, so waves
-// beat reuse there (8192x8192 with 48x48: 1.18 ms vs 1.23 ms vs 1.35 ms for
-// 1, 2, 4 tiles).
-int match_xt(int ow, int oh) {
-    const long long tiles = (long long)((oh + 31) / 32) * ((ow + 31) / 32);
-    return tiles >= 16 * 1024 ? 1 : 2;
-}
-
-template <int KB>
-iph_status launch_i8_kb(const uint8_t *img, int W, int H, const uint8_t *tpl, const uint2 *wsum,
-                        float *out, int ow, int oh, int tw, int th, int method, hipStream_t s) {
-    switch (match_xt(ow, oh)) {
-    case 2: return launch_i8_kb_xt<KB, 2>(img, W, H, tpl, wsum, out, ow, oh, tw, th, method, s);
-    default: return launch_i8_kb_xt<KB, 1>(img, W, H, tpl, wsum, out, ow, oh, tw, th, method, s);
-    }
-}
-
-
 // ---- fused int8 correlation + window sums (u8 images) -------------------------------
 //
 // One workgroup computes a (32 RT) x 32 output tile, wave w its rows
 // 32w .. 32w + 31, from one LDS-staged image block (rows y0 .. y0 + 32 RT +
 // th - 2, columns x0 .. x0 + 32 KB - 1, as i8 = u8 - 128) and the staged
 // template:
-//   * C' = sum I'T' on MFMA against the template's Toeplitz slices (as in
-//     match_i8_mfma_kernel);
+//   * C' = sum I'T' on MFMA against the template's Toeplitz slices, which
+//     each lane cuts from the LDS-staged template rows (zero padded) with
+//     five dword reads and v_alignbyte: B[k][n] = T'[dy][32 kb + k - n];
 //   * S' = sum I' over each window on MFMA too, with the same A operands
 //     against an all-ones Toeplitz band held in registers: exact in i32;
 //   * sum I^2 by exact u32 sliding sums: each staged row's horizontal
@@ -651,9 +345,55 @@ __device__ __forceinline__ float finish_u32(uint32_t c, uint32_t sii, uint32_t s
 size_t fused_lds(int KB, int tw, int th, int RT, bool sums, int tc) {
     tc = std::min(th, tc);
     const int rows = 32 * RT + tc - 1;
-    size_t b = align_up((size_t)rows * fused_pitch(KB), 16) + (size_t)tc * tpl_pitch(KB) + 16;
+    // image rows, template rows + the all-ones row + 16 zero bytes
+    size_t b = align_up((size_t)rows * fused_pitch(KB), 16) + (size_t)(tc + 1) * tpl_pitch(KB) + 16;
     if (sums) b = align_up(b, 16) + (size_t)rows * kFusedHP * sizeof(uint32_t);
     return b;
+}
+
+// Raw buffer over `bytes` bytes from `base`: dword loads past the end read
+// zero and stores past it are dropped (the gfx9 descriptor word 3 for raw
+// 32-bit access).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const void *base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, (int)bytes, 0x00020000);
+}
+
+// sqrt(x) correctly rounded, for x = 0 or normal x: v_sqrt_f32 is within
+// 1 ulp, and the sign of x - s' s for the neighbours s' of s picks the
+// rounded root (Tuckerman's test; x and those products are multiples of the
+// squared ulp, so the test is exact).
+__device__ __forceinline__ float sqrt_rn(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sm = __uint_as_float(__float_as_uint(s) - 1u);
+    const float sp = __uint_as_float(__float_as_uint(s) + 1u);
+    const float rm = __builtin_fmaf(-sm, s, x), rp = __builtin_fmaf(-sp, s, x);
+    const float t = rm <= 0.0f ? sm : s;
+    return rp > 0.0f ? sp : t;
+}
+
+// a / b correctly rounded for 0 <= a < 2^32 and 1 <= b < 2^32, where no
+// intermediate leaves the normal range: the reciprocal refined once by
+// Newton's step, then two residual corrections of the quotient (Markstein).
+__device__ __forceinline__ float div_rn(float a, float b) {
+    float r = __builtin_amdgcn_rcpf(b);
+    r = __builtin_fmaf(__builtin_fmaf(-b, r, 1.0f), r, r);
+    float q = a * r;
+    q = __builtin_fmaf(__builtin_fmaf(-b, q, a), r, q);
+    return __builtin_fmaf(__builtin_fmaf(-b, q, a), r, q);
+}
+
+// finish_u32 with the f32 normalisation spelled out (same values: the
+// reference's sqrt and division are IEEE, and so are sqrt_rn and div_rn in
+// the ranges here: sum I^2 * sum T^2 is 0 or at least 1, the quotient below
+// 2^32).
+template <int METHOD>
+__device__ __forceinline__ float finish_u32_fast(uint32_t c, uint32_t sii, uint32_t stt) {
+    const bool sse = (METHOD == IPH_SSE || METHOD == IPH_SSE_NORMED);
+    const uint32_t num = sse ? stt + sii - 2u * c : c;
+    if (METHOD == IPH_SSE || METHOD == IPH_CCORR) return (float)num;
+    const float norm = sqrt_rn((float)sii * (float)stt);
+    const float sn = (float)num;
+    return norm > 0.0f ? div_rn(sn, norm) : sn;
 }
 
 // The output tile at (x0, y0) of one match (the body of the single-image and
@@ -662,7 +402,10 @@ size_t fused_lds(int KB, int tw, int th, int RT, bool sums, int tc) {
 // image rows those template rows meet and the rows themselves, correlates,
 // and adds the chunk's rows of every window's sum of squares, so the LDS
 // footprint is bounded by tcm rather than by the template height.
-
+// Loads and stores go through raw buffers with 32-bit offsets (the host
+// keeps images within mfma_match_fits): the image and the template from
+// their 4-byte aligned starts, two dwords and a byte funnel per unaligned
+// word, and the score map from the tile's first row.
 template <int KB, int METHOD>
 __device__ __forceinline__ void match_i8_fused_tile(const uint8_t *__restrict__ img, int W,
                                                     int H, const uint8_t *__restrict__ tpl,
@@ -670,40 +413,51 @@ __device__ __forceinline__ void match_i8_fused_tile(const uint8_t *__restrict__ 
                                                     int tw, int th, int tcm, int x0,
                                                     int y0) {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ long long tred[2][4];
+    __shared__ uint32_t tred[2][8];
     constexpr bool kSums = METHOD != IPH_CCORR;  // sum I^2 needed
-    constexpr int BW = 32 * KB, PITCH = fused_pitch(KB), kTP = tpl_pitch(KB);
-    const int RT = blockDim.x >> 6;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    constexpr int PITCH = fused_pitch(KB), kTP = tpl_pitch(KB);
+    constexpr int kTW = kTP / 4;       // u32 words per staged template row
+    constexpr int kLPR = 2 * KB;       // lanes per staged image row, 16 bytes each
+    const int nthr = (int)blockDim.x, RT = nthr >> 6;
+    const int tid = (int)threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int n = lane & 31, g = lane >> 5;
     const int nchunk = (th + tcm - 1) / tcm;
     const int tc = (th + nchunk - 1) / nchunk;  // rows per chunk (the last may have fewer)
-    const int rmax = 32 * RT + min(th, tcm) - 1;  // staged rows of a chunk, at most
+    const int tprows = min(th, tcm);
+    const int rmax = 32 * RT + tprows - 1;  // staged image rows of a chunk, at most
     int8_t *blk = reinterpret_cast<int8_t *>(smem);
     const int tp_off = (rmax * PITCH + 15) & ~15;
     uint32_t *tp = reinterpret_cast<uint32_t *>(smem + tp_off);
+    // one more staged row after the template rows: 0x01 where a template
+    // column is (the all-ones band), then 16 zero bytes
+    uint32_t *onesrow = tp + tprows * kTW;
     uint32_t *h2 = reinterpret_cast<uint32_t *>(
-        smem + ((tp_off + min(th, tcm) * kTP + 16 + 15) & ~15));
-    const bool aligned = (W & 3) == 0 && (reinterpret_cast<uintptr_t>(img) & 3) == 0 && W >= 4;
-    const int nw = tw >> 2;
-    const uint32_t tail = (tw & 3) ? (1u << (8 * (tw & 3))) - 1u : 0u;
+        smem + ((tp_off + (tprows + 1) * kTP + 16 + 15) & ~15));
 
-    // all-ones band: B[k][n] = 1 for 0 <= 32 kb + k - n < tw
-    v4i ones[KB];
+    const uintptr_t ia = reinterpret_cast<uintptr_t>(img), ta = reinterpret_cast<uintptr_t>(tpl);
+    const uint32_t ib = (uint32_t)ia & 3u, tb = (uint32_t)ta & 3u;
+    const auto irs = raw_rsrc(reinterpret_cast<const void *>(ia - ib),
+                              ((uint32_t)W * (uint32_t)H + ib + 3u) & ~3u);
+    const auto trs = raw_rsrc(reinterpret_cast<const void *>(ta - tb),
+                              ((uint32_t)(tw * th) + tb + 3u) & ~3u);
+
+    // image staging: lane (row r0 + k RPP, 16 bytes at column 16 cl)
+    const int rpp = nthr / kLPR;
+    const int cl = tid % kLPR, r0 = tid / kLPR;
+    const uint32_t gxc = (uint32_t)min(x0 + 16 * cl, W - 1);
+    // template staging: lane (row rt0 + k TPP, word ct); bytes 4 ct - 32 ..
+    // hold template columns, the rest of the row is zero
+    const int tpp = nthr / kTW;
+    const int ct = tid % kTW, rt0 = tid / kTW;
+    const int jcol = 4 * ct - 32;
+    uint32_t vm = 0;
 #pragma unroll
-    for (int kb = 0; kb < KB; kb++)
-#pragma unroll
-        for (int d = 0; d < 4; d++) {
-            uint32_t wv = 0;
-#pragma unroll
-            for (int b = 0; b < 4; b++) {
-                const int dx = 32 * kb + 16 * g + 4 * d + b - n;
-                if (dx >= 0 && dx < tw) wv |= 1u << (8 * b);
-            }
-            ones[kb][d] = (int)wv;
-        }
-    const int u0 = 32 + 16 * g - n;
-    const uint32_t sh = (uint32_t)(u0 & 3);
+    for (int b = 0; b < 4; b++)
+        if (jcol + b >= 0 && jcol + b < tw) vm |= 0xFFu << (8 * b);
+    const uint32_t tcol = (uint32_t)max(jcol, 0) + tb;
+    if (rt0 == 0) onesrow[ct] = vm & 0x01010101u;
+    if (tid < 4) onesrow[kTW + tid] = 0u;
+
     v16i acc, acs;
 #pragma unroll
     for (int r = 0; r < 16; r++) acc[r] = acs[r] = 0;
@@ -711,59 +465,92 @@ __device__ __forceinline__ void match_i8_fused_tile(const uint8_t *__restrict__ 
 #pragma unroll
     for (int r = 0; r < 16; r++) sqv[r] = 0;
     uint32_t s1 = 0, s2 = 0;  // this thread's share of sum T, sum T^2
+    v4i ones[KB];
+    const int u0 = 32 + 16 * g - n;
+    const uint32_t sh = (uint32_t)(u0 & 3);
 
     for (int c0 = 0; c0 < th; c0 += tc) {
         const int dyn = min(tc, th - c0);
         const int rows = 32 * RT + dyn - 1;
         if (c0 > 0) __syncthreads();  // the previous chunk is done with blk, tp and h2
-        // image rows y0 + c0.. (clamped) and columns x0.. (clamped), as i8; bytes
-        // outside the image meet zero Toeplitz entries or discarded outputs
-        const int total = rows * (BW / 4);
-        for (int base = threadIdx.x; base < total; base += 8 * (int)blockDim.x) {
-            uint32_t v[8];
+        // image rows y0 + c0 .. (clamped), bytes x0 .. x0 + 32 KB - 1 as i8
+        // (bytes past a row's end, or the image's, meet zero Toeplitz
+        // entries or discarded outputs only), and template rows c0 ..; each
+        // round issues all its loads before its LDS writes
+        {
+            constexpr int kIB = 3, kTB = 2;  // image and template rows per lane and round
+            const int ni = (rows - r0 + kIB * rpp - 1) / (kIB * rpp);
+            const int nt = max((dyn - rt0 + kTB * tpp - 1) / (kTB * tpp), 0);
+            const int rounds = max(r0 < rpp ? ni : 0, rt0 < tpp ? nt : 0);
+            for (int k = 0; k < rounds; k++) {
+                uint32_t d[kIB][5], e[kTB][2], osh[kIB], tsh8[kTB];
 #pragma unroll
-            for (int b = 0; b < 8; b++) {
-                const int i = min(base + b * (int)blockDim.x, total - 1);
-                const int r = i / (BW / 4), c = (i - r * (BW / 4)) * 4;
-                const uint8_t *row = img + (size_t)min(y0 + c0 + r, H - 1) * W;
-                const int gx = x0 + c;
-                if (aligned) {
-                    v[b] = *reinterpret_cast<const uint32_t *>(row + min(gx, W - 4));
-                } else {
-                    v[b] = (uint32_t)row[min(gx, W - 1)] | ((uint32_t)row[min(gx + 1, W - 1)] << 8) |
-                           ((uint32_t)row[min(gx + 2, W - 1)] << 16) |
-                           ((uint32_t)row[min(gx + 3, W - 1)] << 24);
+                for (int u = 0; u < kIB; u++) {
+                    const int rr = min(r0 + (k * kIB + u) * rpp, rows - 1);
+                    const uint32_t y = (uint32_t)min(y0 + c0 + rr, H - 1);
+                    const uint32_t o = y * (uint32_t)W + gxc + ib;
+                    osh[u] = o & 3u;
+#pragma unroll
+                    for (int q = 0; q < 5; q++)
+                        d[u][q] = __builtin_amdgcn_raw_buffer_load_b32(irs, (int)((o & ~3u) + 4 * q),
+                                                                       0, 0);
+                }
+#pragma unroll
+                for (int u = 0; u < kTB; u++) {
+                    const int r = min(rt0 + (k * kTB + u) * tpp, dyn - 1);
+                    const uint32_t o = (uint32_t)((c0 + r) * tw) + tcol;
+                    tsh8[u] = o & 3u;
+                    e[u][0] = __builtin_amdgcn_raw_buffer_load_b32(trs, (int)(o & ~3u), 0, 0);
+                    e[u][1] = __builtin_amdgcn_raw_buffer_load_b32(trs, (int)((o & ~3u) + 4), 0, 0);
+                }
+#pragma unroll
+                for (int u = 0; u < kIB; u++) {
+                    const int rr = r0 + (k * kIB + u) * rpp;
+                    if (r0 < rpp && rr < rows) {
+                        v4i w;
+#pragma unroll
+                        for (int q = 0; q < 4; q++)
+                            w[q] = (int)(__builtin_amdgcn_alignbyte(d[u][q + 1], d[u][q], osh[u]) ^
+                                         0x80808080u);
+                        *reinterpret_cast<v4i *>(blk + rr * PITCH + 16 * cl) = w;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kTB; u++) {
+                    const int r = rt0 + (k * kTB + u) * tpp;
+                    if (rt0 < tpp && r < dyn) {
+                        const uint32_t wv = __builtin_amdgcn_alignbyte(e[u][1], e[u][0], tsh8[u]) & vm;
+                        s1 = __builtin_amdgcn_udot4(wv, 0x01010101u, s1, false);
+                        s2 = __builtin_amdgcn_udot4(wv, wv, s2, false);
+                        tp[r * kTW + ct] = wv ^ (vm & 0x80808080u);
+                    }
                 }
             }
-#pragma unroll
-            for (int b = 0; b < 8; b++) {
-                const int i = base + b * (int)blockDim.x;
-                if (i < total) {
-                    const int r = i / (BW / 4), c = (i - r * (BW / 4)) * 4;
-                    *reinterpret_cast<uint32_t *>(blk + r * PITCH + c) = v[b] ^ 0x80808080u;
-                }
-            }
-        }
-        // template rows c0.. as i8 at [32, 32 + tw) of kTP-byte rows (zeros around)
-        for (int i = threadIdx.x; i < dyn * (kTP / 4) + 4; i += blockDim.x) {
-            const int r = i / (kTP / 4), c = 4 * (i - r * (kTP / 4)) - 32;
-            uint32_t wv = 0;
-#pragma unroll
-            for (int b = 0; b < 4; b++)
-                if (r < dyn && c + b >= 0 && c + b < tw) {
-                    const uint32_t t = tpl[(c0 + r) * tw + c + b];
-                    s1 += t;
-                    s2 += t * t;
-                    wv |= (t ^ 0x80u) << (8 * b);
-                }
-            tp[i] = wv;
         }
         __syncthreads();
+        if (c0 == 0) {
+            // all-ones band: B[k][n] = 1 for 0 <= 32 kb + k - n < tw, cut from
+            // the ones row like the template's Toeplitz slices
+            const uint32_t *q0 = onesrow + (u0 >> 2);
+#pragma unroll
+            for (int kb = 0; kb < KB; kb++) {
+                const uint32_t *q = q0 + 8 * kb;
+#pragma unroll
+                for (int d = 0; d < 4; d++)
+                    ones[kb][d] = (int)__builtin_amdgcn_alignbyte(q[d + 1], q[d], sh);
+            }
+        }
 
         // horizontal tw-window sums of squares of the chunk's rows, 16 outputs
-        // per task: columns 16q .. 16q + 15 (the tile's 32 output columns)
+        // per task: columns 16q .. 16q + 15 (the tile's 32 output columns);
+        // the first from v_dot4 over whole words, then 15 steps adding the
+        // byte entering (16q + tw + x - 1) and dropping the one leaving
+        // (16q + x - 1), both taken from words
         if constexpr (kSums) {
-            for (int it = threadIdx.x; it < rows * 2; it += blockDim.x) {
+            const int nw = tw >> 2;
+            const uint32_t tail = (tw & 3) ? (1u << (8 * (tw & 3))) - 1u : 0u;
+            const uint32_t tsh = (uint32_t)(tw & 3);
+            for (int it = tid; it < rows * 2; it += nthr) {
                 const int r = it >> 1, q = it & 1;
                 const uint32_t *pw = reinterpret_cast<const uint32_t *>(blk + r * PITCH + 16 * q);
                 uint32_t b = 0;
@@ -775,13 +562,21 @@ __device__ __forceinline__ void match_i8_fused_tile(const uint8_t *__restrict__ 
                     const uint32_t wv = (pw[nw] ^ 0x80808080u) & tail;
                     b = __builtin_amdgcn_udot4(wv, wv, b, false);
                 }
-                const uint8_t *p = reinterpret_cast<const uint8_t *>(pw);
+                uint32_t vo[4], vi[4], e[5];
+#pragma unroll
+                for (int k = 0; k < 4; k++) vo[k] = pw[k] ^ 0x80808080u;
+#pragma unroll
+                for (int k = 0; k < 5; k++) e[k] = pw[nw + k];
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    vi[k] = __builtin_amdgcn_alignbyte(e[k + 1], e[k], tsh) ^ 0x80808080u;
                 uint32_t *o = h2 + r * kFusedHP + 16 * q;
                 o[0] = b;
 #pragma unroll
                 for (int x = 1; x < 16; x++) {
-                    const uint32_t vin = p[x + tw - 1] ^ 0x80u, vout = p[x - 1] ^ 0x80u;
-                    b += vin * vin - vout * vout;
+                    const uint32_t bi = (vi[(x - 1) >> 2] >> (8 * ((x - 1) & 3))) & 0xFFu;
+                    const uint32_t bo = (vo[(x - 1) >> 2] >> (8 * ((x - 1) & 3))) & 0xFFu;
+                    b += bi * bi - bo * bo;
                     o[x] = b;
                 }
             }
@@ -805,7 +600,7 @@ __device__ __forceinline__ void match_i8_fused_tile(const uint8_t *__restrict__ 
                 acs = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, ones[kb], acs, 0, 0, 0);
             }
             arow += PITCH;
-            trow += kTP / 4;
+            trow += kTW;
         }
 
         if constexpr (kSums) {
@@ -834,7 +629,7 @@ __device__ __forceinline__ void match_i8_fused_tile(const uint8_t *__restrict__ 
         }
     }
 
-    long long l1 = s1, l2 = s2;
+    uint32_t l1 = s1, l2 = s2;  // sum T <= 2^22, sum T^2 < 2^31: no wrap
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         l1 += __shfl_xor(l1, o);
@@ -847,11 +642,15 @@ __device__ __forceinline__ void match_i8_fused_tile(const uint8_t *__restrict__ 
     __syncthreads();
     uint32_t st = 0, stt = 0;
     for (int w = 0; w < RT; w++) {
-        st += (uint32_t)tred[0][w];
-        stt += (uint32_t)tred[1][w];
+        st += tred[0][w];
+        stt += tred[1][w];
     }
+    // the score map from the tile's first row: offsets below 32 RT rows
     const int x = x0 + n;
-    const int ybase = y0 + wave * 32 + 4 * g;
+    float *obase = out + (size_t)y0 * (uint32_t)ow;
+    const uint32_t orows = (uint32_t)min(oh - y0, 32 * RT);
+    const auto ors = raw_rsrc(obase, orows * (uint32_t)ow * 4u);
+    const uint32_t voff = ((uint32_t)(wave * 32 + 4 * g) * (uint32_t)ow + (uint32_t)x) * 4u;
     auto emit = [&](auto f64) {
         constexpr bool F64 = decltype(f64)::value;
         float v[16];
@@ -859,13 +658,15 @@ __device__ __forceinline__ void match_i8_fused_tile(const uint8_t *__restrict__ 
         for (int r = 0; r < 16; r++) {
             // sum I T = C' + 128 (S' + sum T), exact in 32 bits
             const uint32_t c = (uint32_t)(acc[r] + 128 * (acs[r] + (int)st));
-            v[r] = finish_u32<METHOD, F64>(c, sqv[r], stt);
+            if constexpr (F64) v[r] = finish_u32<METHOD, true>(c, sqv[r], stt);
+            else v[r] = finish_u32_fast<METHOD>(c, sqv[r], stt);
         }
         if (x < ow) {
 #pragma unroll
             for (int r = 0; r < 16; r++) {
-                const int y = ybase + 8 * (r >> 2) + (r & 3);
-                if (y < oh) out[(uint32_t)y * (uint32_t)ow + (uint32_t)x] = v[r];
+                const uint32_t soff = (uint32_t)(8 * (r >> 2) + (r & 3)) * (uint32_t)ow * 4u;
+                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v[r]), ors, (int)voff,
+                                                      (int)soff, 0);
             }
         }
     };
@@ -875,7 +676,7 @@ __device__ __forceinline__ void match_i8_fused_tile(const uint8_t *__restrict__ 
 }
 
 template <int KB, int METHOD>
-__global__ __launch_bounds__(256) void match_i8_fused_kernel(
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void match_i8_fused_kernel(
     const uint8_t *__restrict__ img, int W, int H, const uint8_t *__restrict__ tpl,
     float *__restrict__ out, int ow, int oh, int tw, int th, int tcm) {
     match_i8_fused_tile<KB, METHOD>(img, W, H, tpl, out, ow, oh, tw, th, tcm, blockIdx.x * 32,
@@ -887,7 +688,7 @@ __global__ __launch_bounds__(256) void match_i8_fused_kernel(
 // is dealt to the XCDs in eight contiguous ranges (a job's neighbouring tiles,
 // which share image rows, meet in one L2).
 template <int KB, int METHOD>
-__global__ __launch_bounds__(256) void match_i8_fused_jobs_kernel(const MatchJob *__restrict__ jobs,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void match_i8_fused_jobs_kernel(const MatchJob *__restrict__ jobs,
                                                                   int njobs, int ntiles,
                                                                   int tcm) {
     const int chunk = (ntiles + 7) >> 3;
@@ -905,42 +706,77 @@ __global__ __launch_bounds__(256) void match_i8_fused_jobs_kernel(const MatchJob
                                     jb.th, tcm, tx * 32, ty * 32 * (int)(blockDim.x >> 6));
 }
 
-constexpr size_t kFusedMaxLds = 64 * 1024;
-// Footprint that still lets three 4-wave workgroups share a CU's 160 KiB.
-constexpr size_t kFusedOccLds = 53 * 1024;
-constexpr int kFusedMinChunk = 32;  // chunks are never forced below this
+// LDS per workgroup and per CU on the current device (64 KiB and 160 KiB if
+// the queries fail).
+size_t device_lds(hipDeviceAttribute_t what, size_t dflt) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&v, what, dev) != hipSuccess ||
+        v <= 0)
+        return dflt;
+    return (size_t)v;
+}
+size_t fused_lds_limit() {
+    static const size_t v = device_lds(hipDeviceAttributeMaxSharedMemoryPerBlock, 64 * 1024);
+    return v;
+}
+size_t fused_lds_cu() {
+    static const size_t v = device_lds(hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                                       160 * 1024);
+    return v;
+}
+constexpr int kFusedMinChunk = 16;  // chunks are never cut below this many rows
+constexpr int kFusedMaxWaves = 16;  // per CU: the kernel's registers allow 4 per SIMD
 
-// Output rows per fused workgroup (32 * RT): the batch hint or 4 waves,
-// halved while the footprint of 32-row chunks is above kFusedMaxLds; 0 when
-// even one wave does not fit.
-int fused_rt(int tw, int th, int method) {
-    const int KB = (31 + tw + 31) / 32;
-    int RT = g_match_rows_hint > 0 ? std::min(g_match_rows_hint, 4) : 4;
-    while (RT > 0 &&
-           fused_lds(KB, tw, th, RT, method != IPH_CCORR, kFusedMinChunk) > kFusedMaxLds)
-        RT /= 2;
-    return RT;
+// Waves per CU that a workgroup of RT waves and `lds` bytes allows.
+int fused_waves(size_t lds, int RT) {
+    return std::min<int>(kFusedMaxWaves, (int)(fused_lds_cu() / std::max<size_t>(lds, 1)) * RT);
 }
 
-// Largest chunk height (near-equal chunks of th) whose footprint is at most
-// kFusedOccLds, else kFusedMinChunk: every chunk restages the image rows it
-// meets, so fewer chunks save staging and horizontal sums.
-int fused_tc(int KB, int tw, int th, int RT, bool sums) {
-    for (int nc = 1; nc <= th; nc++) {
-        const int tc = (th + nc - 1) / nc;
-        if (tc < kFusedMinChunk && nc > 1) break;
-        if (fused_lds(KB, tw, th, RT, sums, tc) <= kFusedOccLds) return tc;
+// Template rows per chunk for RT waves: the fewest near-equal chunks whose
+// footprint allows 16 waves per CU (else 12, else 8); 0 if no chunk of at
+// least kFusedMinChunk rows fits a workgroup. Every chunk restages the image
+// rows it meets, so fewer chunks save staging and horizontal sums.
+int fused_tc(int KB, int th, int RT, bool sums) {
+    for (int waves : {16, 12, 8}) {
+        for (int nc = 1; nc <= th; nc++) {
+            const int tc = (th + nc - 1) / nc;
+            if (tc < kFusedMinChunk && nc > 1) break;
+            const size_t lds = fused_lds(KB, 0, th, RT, sums, tc);
+            if (lds <= fused_lds_limit() && fused_waves(lds, RT) >= waves) return tc;
+        }
     }
-    return std::min(th, kFusedMinChunk);
+    const int tc = std::min(th, kFusedMinChunk);
+    return fused_lds(KB, 0, th, RT, sums, tc) <= fused_lds_limit() ? tc : 0;
+}
+
+// Workgroup shape for one match: waves (32 output rows each) and chunk
+// height. Most waves per CU first, then the fewest staged image rows per
+// output row (taller tiles spread the th - 1 halo rows over more outputs).
+void fused_shape(int KB, int th, bool sums, int &RT, int &tc) {
+    RT = 0;
+    tc = 0;
+    int best_w = 0;
+    double best_rows = 1e30;
+    for (int rt : {8, 4, 2, 1}) {
+        const int c = fused_tc(KB, th, rt, sums);
+        if (c == 0) continue;
+        const int w = fused_waves(fused_lds(KB, 0, th, rt, sums, c), rt);
+        const int nc = (th + c - 1) / c;
+        const double rows = (double)nc * (32 * rt + c - 1) / (32.0 * rt);
+        if (w > best_w || (w == best_w && rows < best_rows - 1e-9)) {
+            best_w = w;
+            best_rows = rows;
+            RT = rt;
+            tc = c;
+        }
+    }
 }
 
 template <int KB>
 iph_status launch_i8_fused_kb(const uint8_t *img, int W, int H, const uint8_t *tpl, float *out,
-                              int ow, int oh, int tw, int th, int method, int RT,
+                              int ow, int oh, int tw, int th, int method, int RT, int tc,
                               hipStream_t s) {
-    const bool sums = method != IPH_CCORR;
-    const int tc = fused_tc(KB, tw, th, RT, sums);
-    const size_t lds = fused_lds(KB, tw, th, RT, sums, tc);
+    const size_t lds = fused_lds(KB, tw, th, RT, method != IPH_CCORR, tc);
     dim3 grid((ow + 31) / 32, (oh + 32 * RT - 1) / (32 * RT));
 #define L(M)                                                                                 \
     match_i8_fused_kernel<KB, M><<<grid, 64 * RT, lds, s>>>(img, W, H, tpl, out, ow, oh, tw, th, tc)
@@ -1012,16 +848,17 @@ size_t mfma_match_workspace(int32_t p_type, uint32_t w, uint32_t h, uint32_t tw,
         if (method != IPH_CCORR) wp.take<double>((size_t)(w - tw + 1) * h);
         return wp.used;
     }
-    if (tw <= (uint32_t)kMaxSide && th <= (uint32_t)kMaxSide && fused_rt((int)tw, (int)th, method) > 0)
-        return 256;  // the fused kernel needs no workspace
-    WorkspacePlan wp;
-    wp.take<uint2>((size_t)(w - tw + 1) * (h - th + 1));
-    return wp.used;
+    return 256;  // the fused int8 kernel needs no workspace
 }
 
 iph_status launch_mfma_match(const void *img, int32_t p_type, uint32_t w, uint32_t h,
                              const void *tpl, uint32_t tw, uint32_t th, int32_t method,
                              float *out, void *ws, size_t ws_bytes, hipStream_t s) {
+    if (!mfma_match_fits(w, h)) {
+        set_error("the MFMA match paths take images below 2^32 - 64 pixels and 2^23 columns; "
+                  "use the direct path");
+        return IPH_EUNSUPPORTED;
+    }
     if (p_type == IPH_F32)
         return launch_bf16x3_match(static_cast<const float *>(img), w, h,
                                    static_cast<const float *>(tpl), tw, th, method, out, ws,
@@ -1034,55 +871,42 @@ iph_status launch_mfma_match(const void *img, int32_t p_type, uint32_t w, uint32
     const int KB = blocks_for((int)tw);
     const uint8_t *im8 = static_cast<const uint8_t *>(img);
     const uint8_t *tp8 = static_cast<const uint8_t *>(tpl);
-    const int RT = fused_rt((int)tw, (int)th, method);
-    if (RT > 0) {
-        const int ow = (int)(w - tw + 1), oh = (int)(h - th + 1);
-        switch (KB) {
-        case 1: return launch_i8_fused_kb<1>(im8, w, h, tp8, out, ow, oh, tw, th, method, RT, s);
-        case 2: return launch_i8_fused_kb<2>(im8, w, h, tp8, out, ow, oh, tw, th, method, RT, s);
-        case 3: return launch_i8_fused_kb<3>(im8, w, h, tp8, out, ow, oh, tw, th, method, RT, s);
-        case 4: return launch_i8_fused_kb<4>(im8, w, h, tp8, out, ow, oh, tw, th, method, RT, s);
-        default: return launch_i8_fused_kb<5>(im8, w, h, tp8, out, ow, oh, tw, th, method, RT, s);
-        }
+    int RT, tc;
+    fused_shape(KB, (int)th, method != IPH_CCORR, RT, tc);
+    if (RT == 0) {
+        set_error("no fused int8 workgroup shape fits %zu bytes of LDS", fused_lds_limit());
+        return IPH_EUNSUPPORTED;
     }
-    Workspace wsp(ws, ws_bytes);
-    uint2 *wsum = wsp.take<uint2>((size_t)(w - tw + 1) * (h - th + 1));
-    if (!wsum) {
-        set_error("workspace too small for the int8 MFMA match_template");
-        return IPH_EINVAL;
-    }
-    const uint8_t *im = static_cast<const uint8_t *>(img);
-    const uint8_t *tp = static_cast<const uint8_t *>(tpl);
-    int ow = (int)(w - tw + 1), oh = (int)(h - th + 1);
-    dim3 wg((ow + kWinCols - 1) / kWinCols, (oh + kWinRows - 1) / kWinRows);
-    window_sums_u8_kernel<<<wg, kWinThreads, win_lds((int)tw, (int)th), s>>>(im, (int)w, (int)tw,
-                                                                    (int)th, ow, oh, wsum);
-    IPH_TRY(launch_status("window_sums_u8_kernel"));
+    const int ow = (int)(w - tw + 1), oh = (int)(h - th + 1);
+#define C(K)                                                                                 \
+    case K: return launch_i8_fused_kb<K>(im8, w, h, tp8, out, ow, oh, tw, th, method, RT, tc, s)
     switch (KB) {
-    case 1: return launch_i8_kb<1>(im, w, h, tp, wsum, out, ow, oh, tw, th, method, s);
-    case 2: return launch_i8_kb<2>(im, w, h, tp, wsum, out, ow, oh, tw, th, method, s);
-    case 3: return launch_i8_kb<3>(im, w, h, tp, wsum, out, ow, oh, tw, th, method, s);
-    case 4: return launch_i8_kb<4>(im, w, h, tp, wsum, out, ow, oh, tw, th, method, s);
-    default: return launch_i8_kb<5>(im, w, h, tp, wsum, out, ow, oh, tw, th, method, s);
+        C(1); C(2); C(3); C(4);
+    default: return launch_i8_fused_kb<5>(im8, w, h, tp8, out, ow, oh, tw, th, method, RT, tc, s);
     }
+#undef C
 }
 
-void set_match_rows_hint(int rows) { g_match_rows_hint = rows; }
+
 
 int fused_match_kb(uint32_t tw) { return blocks_for((int)tw); }
 
 int fused_match_rt_max(uint32_t tw, uint32_t th, int32_t method) {
     if (tw == 0 || th == 0 || tw > (uint32_t)kMaxSide || th > (uint32_t)kMaxSide) return 0;
     const int KB = blocks_for((int)tw);
-    for (int rt = 4; rt > 0; rt /= 2)
-        if (fused_lds(KB, (int)tw, (int)th, rt, method != IPH_CCORR, kFusedMinChunk) <=
-            kFusedMaxLds)
-            return rt;
+    for (int rt : {8, 4, 2, 1})
+        if (fused_tc(KB, (int)th, rt, method != IPH_CCORR) > 0) return rt;
     return 0;
 }
 
+int fused_match_rt_default(uint32_t tw, uint32_t th, int32_t method) {
+    int rt, tc;
+    fused_shape(blocks_for((int)tw), (int)th, method != IPH_CCORR, rt, tc);
+    return rt;
+}
+
 int fused_match_tc(int kb, uint32_t th, int rt, int32_t method) {
-    return fused_tc(kb, 0, (int)th, rt, method != IPH_CCORR);
+    return fused_tc(kb, (int)th, rt, method != IPH_CCORR);
 }
 
 size_t fused_match_lds(int kb, uint32_t tw, uint32_t th, int rt, int32_t method, int tc) {
@@ -1103,8 +927,8 @@ long long match_jobs_layout(MatchJob *jobs, int n, int rt) {
 iph_status launch_match_jobs(const MatchJob *jobs, int n, long long ntiles, int kb,
                              int32_t method, int rt, int tc, size_t lds, hipStream_t s) {
     if (n <= 0 || ntiles <= 0) return IPH_OK;
-    if (ntiles > (1LL << 30) || kb < 1 || kb > 5 || (rt != 1 && rt != 2 && rt != 4) || tc < 1 ||
-        lds > kFusedMaxLds) {
+    if (ntiles > (1LL << 30) || kb < 1 || kb > 5 || (rt != 1 && rt != 2 && rt != 4 && rt != 8) ||
+        tc < 1 || lds > fused_lds_limit()) {
         set_error("match job table out of range (%lld tiles, kb %d, rt %d, chunk %d, lds %zu)",
                   ntiles, kb, rt, tc, lds);
         return IPH_EINVAL;

@@ -321,7 +321,8 @@ IPH_API iph_status iph_batch_add_gaussian_blur(iph_batch *batch, const uint8_t *
 IPH_API iph_status iph_batch_add_filter_clamped(iph_batch *batch, const uint8_t *src,
                                                 uint8_t *dst, uint32_t width, uint32_t height,
                                                 const float *kernel, uint32_t kw, uint32_t kh);
-/* templates up to 128x128; out: (width - tw + 1) x (height - th + 1) f32. */
+/* any template the single call takes (above 128x128 on the direct path);
+ * out: (width - tw + 1) x (height - th + 1) f32. */
 IPH_API iph_status iph_batch_add_match_template(iph_batch *batch, const uint8_t *image,
                                                 uint32_t width, uint32_t height,
                                                 const uint8_t *tmpl, uint32_t tw, uint32_t th,

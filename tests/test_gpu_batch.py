@@ -133,7 +133,12 @@ def test_batch_many_small_and_one_large_image():
 def _check_match_vs_oracle(got, img, tpl, m):
     """The int8 MFMA bounds of test_gpu_match.py: SSE and CC are the exact
     integer sums rounded once to f32; the normalised methods are within 4 f32
-    ulps of FP64 below 64x64 windows and 0.5 ulp from there on."""
+    ulps of FP64 below 64x64 windows and 0.5 ulp from there on. Templates
+    above 128 px take the direct path: bit-identical to the oracle."""
+    if max(tpl.shape) > 128:
+        assert np.array_equal(got.view(np.uint32),
+                              O.match_template(img, tpl, int(m), nthreads=8).view(np.uint32)), m
+        return
     exact = O.match_template_f64(img, tpl, int(m))
     if m in (M.SumOfSquaredErrors, M.CrossCorrelation):
         assert np.array_equal(got.view(np.uint32), exact.astype(np.float32).view(np.uint32)), m
@@ -147,12 +152,13 @@ def _check_match_vs_oracle(got, img, tpl, m):
 
 def test_batch_match_jobs_equal_oracle():
     """Batched matches against the oracle: templates of 8 to 128 px (every K-block
-    count of the fused job-table kernel, and 128 x 128, which takes the
-    two-kernel path), all four methods, images of uneven sizes."""
+    count of the fused job-table kernel, one and several template-row chunks)
+    and one of 130 columns (the direct path inside the batch), all four
+    methods, images of uneven sizes."""
     rng = np.random.default_rng(15)
     shapes = [((150, 170), (8, 8)), ((200, 160), (40, 33)), ((180, 210), (65, 66)),
               ((230, 190), (96, 90)), ((240, 260), (100, 97)), ((140, 150), (128, 128)),
-              ((97, 301), (31, 64)), ((260, 75), (17, 70))]
+              ((97, 301), (31, 64)), ((260, 75), (17, 70)), ((60, 150), (20, 130))]
     bt = Batch()
     cases = []
     for k, ((h, w), (th, tw)) in enumerate(shapes):
