@@ -11,9 +11,13 @@ batch's Gpix/s. The same JSON line carries:
     stream the kernel runs on, against the 8 TB/s HBM peak; `traffic` is the
     HBM bytes per launch from the FETCH_SIZE/WRITE_SIZE counter runs recorded
     in profiles/blur_u8_sigma2_512x1024x768.md (copy-calibrated);
+  * single-image blur latency (one 1024x768 image per call) and bit-exact
+    parity of all 512 images against the oracle;
   * match_template: ms per 2048x2048 u8 image with a 32x32 template, for each
     of the four methods on the MFMA path and on the direct FP32 path, with
-    argmax/argmin agreement between them;
+    argmax/argmin agreement, max abs / rel error over every output against an
+    FP64 reference, the MFMA kernel's int8 roofline, find_extremes, and the
+    oracle's match_template_parallel restatement on the host cores;
   * precision_f32: CrossCorrelationNormalized on a 4096x4096 Luma<f32> field
     with 64x64 and 128x128 templates, BF16x3 MFMA vs FP32 direct, each with
     the max abs / rel error over every output against a float64 reference
@@ -26,7 +30,9 @@ batch's Gpix/s. The same JSON line carries:
     full-map fallback on low margin): ms per search, speedup and argmax
     agreement, and what the network adds over ranking by coarse score alone;
   * cpu_baseline: the oracle's blur (the reference's loop restated in C,
-    single-threaded like the reference's gaussian_blur_f32) on host cores.
+    single-threaded like the reference's gaussian_blur_f32) on host cores;
+    the match, precision, batch and proposal legs carry their own, on up to
+    16 host threads (the CPU model and nproc are stated).
 Inputs are synthetic and seeded (SplitMix64).
 """
 from __future__ import annotations
@@ -144,6 +150,56 @@ def ccorr_normed_f64(img, tpl):
     return num / torch.sqrt(ii * (T * T).sum())
 
 
+def match_f64_u8(gi, gt):
+    """FP64 reference of the four methods for a u8 image and template, on the
+    GPU: the correlation by FFT rounded to the nearest integer (every sum is
+    an integer below 2^31, and the FFT's error stays far below 0.5: the
+    largest rounding residual is returned), the window energies from an
+    exact float64 integral image; then the oracle's f64 formulas
+    (or_match_template_u8_f64). Returns ({method value: map}, residual)."""
+    import torch
+
+    I = gi.double()
+    T = gt.double()
+    H, W = I.shape
+    th, tw = T.shape
+    s = (H + th, W + tw)
+    num = torch.fft.irfft2(torch.fft.rfft2(I, s) * torch.conj(torch.fft.rfft2(T, s)), s)
+    num = num[:H - th + 1, :W - tw + 1]
+    c = torch.round(num)
+    residual = float((num - c).abs().max())
+    sq = torch.zeros((H + 1, W + 1), dtype=torch.float64, device=I.device)
+    sq[1:, 1:] = (I * I).cumsum(0).cumsum(1)
+    ii = sq[th:, tw:] - sq[:-th, tw:] - sq[th:, :-tw] + sq[:-th, :-tw]
+    tss = (T * T).sum()
+    sse = tss - 2.0 * c + ii
+    norm = torch.sqrt(ii * tss)
+    pos = norm > 0
+    return {0: sse, 1: torch.where(pos, sse / norm, sse), 2: c,
+            3: torch.where(pos, c / norm, c)}, residual
+
+
+def cpu_threads() -> int:
+    """Host threads for the parallel CPU baselines: the box's CPU share (16)
+    or fewer if the machine has fewer."""
+    return max(1, min(os.cpu_count() or 1, 16))
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def cpu_desc(threads: int) -> str:
+    return f"{threads} threads of {cpu_model()} (nproc {os.cpu_count()})"
+
+
 def splitmix64_u8_cuda(seed: int, n: int):
     """Top 8 bits of n SplitMix64 outputs, computed on the GPU (int64 with
     masks in place of logical shifts); equal to uniform_u8 on the host."""
@@ -253,7 +309,8 @@ def cpu_blur_baseline(budget_s: float = 12.0):
         "cores": 1,
         "kind": "port",
         "sample": f"{n} images of {BLUR_W}x{BLUR_H} u8, sigma {BLUR_SIGMA}, {dt:.1f} s on one core "
-                  f"of {platform.processor() or platform.machine()} (oracle/oracle.c, gcc -O3)",
+                  f"of {cpu_model()} (nproc {os.cpu_count()}; oracle/oracle.c, gcc -O3; the "
+                  "reference's gaussian_blur_f32 is single-threaded)",
     }
 
 
@@ -272,14 +329,22 @@ def bench_blur(steps: int, warmup: int):
     torch.cuda.synchronize()
     print("[bench] blur: timing", flush=True)
     times, wall = timed(lambda: dev.gaussian_blur_f32(src, BLUR_SIGMA, out=dst), steps, warmup)
-    # spot-check parity of one image against the oracle
-    import oracle as O
-
-    k = BLUR_N // 3
-    ok = np.array_equal(dst[k].cpu().numpy(), O.gaussian_blur_f32(src[k].cpu().numpy(), BLUR_SIGMA))
     ms = float(np.mean(times))
+    # one 1024x768 image per call (the reference's unit of work), latency
+    one, one_out = src[0], torch.empty_like(src[0])
+    t1, _ = timed(lambda: dev.gaussian_blur_f32(one, BLUR_SIGMA, out=one_out), 50, 5)
+    single_ms = float(np.median(t1))
+    # parity of every image of the batch against the oracle, on host threads
+    import oracle as O
+    from concurrent.futures import ThreadPoolExecutor
+
+    host_src, host_dst = src.cpu().numpy(), dst.cpu().numpy()
+    with ThreadPoolExecutor(cpu_threads()) as ex:
+        same = list(ex.map(lambda i: np.array_equal(host_dst[i], O.gaussian_blur_f32(
+            host_src[i], BLUR_SIGMA)), range(BLUR_N)))
     bytes_per_step = 2.0 * BLUR_W * BLUR_H * BLUR_N
-    return ms, wall, bytes_per_step, ok
+    return ms, wall, bytes_per_step, {"images_checked": BLUR_N, "bit_exact": int(sum(same)),
+                                      "single_image_ms": round(single_ms, 4)}
 
 
 def bench_copy(steps: int, warmup: int):
@@ -295,8 +360,14 @@ def bench_copy(steps: int, warmup: int):
 
 
 def bench_match(steps: int, cpu: bool = False):
+    """Config 2: the four methods on the int8 MFMA path and the FP32 direct
+    path, each with ms per image (HIP events on the stream the kernel runs
+    on), argmax agreement, and max abs / rel error over every output against
+    the FP64 reference; the MFMA kernel's int8 roofline; find_extremes; and
+    the oracle's match_template_parallel restatement on the host cores."""
     import torch
 
+    import oracle as O
     from imageproc_amd import device as dev
     from imageproc_amd.template_matching import MatchTemplateMethod as M
 
@@ -307,24 +378,48 @@ def bench_match(steps: int, cpu: bool = False):
     res = {"workload": f"{MATCH_SIDE}x{MATCH_SIDE} u8 smooth noise, {MATCH_T}x{MATCH_T} template",
            "planted_location": list(loc)}
     ow = MATCH_SIDE - MATCH_T + 1
+    ops = 2.0 * ow * ow * MATCH_T * MATCH_T
+    ref, residual = match_f64_u8(gi, gt)
+    # the FP64 reference against the oracle's int64/f64 restatement on a crop
+    crop = np.ascontiguousarray(img[:200, :260])
+    ref_ok = all(np.array_equal(match_f64_u8(torch.from_numpy(crop).cuda(), gt)[0][int(m)]
+                                .cpu().numpy(), O.match_template_f64(crop, tpl, int(m)))
+                 for m in M)
+    res["fp64_reference"] = {
+        "how": "FFT correlation rounded to integers + exact integral-image energies, float64 "
+               "normalisation as in oracle match_template_f64",
+        "max_fft_rounding_residual": residual,
+        "equals_oracle_f64_on_200x260_crop": bool(ref_ok)}
     for m in M:
         best = {}
+        r = ref[int(m)]
         for path in ("mfma", "direct"):
-            try:
-                out = dev.match_template(gi, gt, m, path)
-            except TypeError as e:  # path not built
-                res[f"{m.name}:{path}"] = {"error": str(e)}
-                continue
-            times, _ = timed(lambda: dev.match_template(gi, gt, m, path, out=out),
-                             steps, 2)
+            out = dev.match_template(gi, gt, m, path)
+            times, _ = timed(lambda: dev.match_template(gi, gt, m, path, out=out), steps, 2)
+            ms = float(np.median(times))
             o = out.cpu().numpy()
             flat = int(np.argmin(o) if m in (M.SumOfSquaredErrors,
                                            M.SumOfSquaredErrorsNormalized) else np.argmax(o))
             best[path] = (flat % ow, flat // ow)
-            res[f"{m.name}:{path}"] = {"ms_per_image": round(float(np.median(times)), 4),
-                                       "best_location": list(best[path])}
-        if len(best) == 2:
-            res[f"{m.name}:argmax_agree"] = best["mfma"] == best["direct"]
+            d = (out.double() - r).abs()
+            nz = r.abs() > 0
+            entry = {"ms_per_image": round(ms, 4), "best_location": list(best[path]),
+                     "max_abs_err_vs_fp64": float(d.max()),
+                     "max_rel_err_vs_fp64": float((d[nz] / r.abs()[nz]).max()) if bool(nz.any())
+                     else 0.0}
+            if path == "mfma":
+                entry["roofline"] = {"bound": "mfma", "achieved": round(ops / (ms * 1e-3) / 1e12, 1),
+                                     "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
+                                     "frac": round(ops / (ms * 1e-3) / 1e12 / INT8_PEAK_TOPS, 4),
+                                     "ops": ops}
+            res[f"{m.name}:{path}"] = entry
+        res[f"{m.name}:argmax_agree"] = best["mfma"] == best["direct"]
+    # find_extremes of the CrossCorrelationNormalized map
+    out = dev.match_template(gi, gt, M.CrossCorrelationNormalized, "mfma")
+    ex = dev.find_extremes(out)
+    times, _ = timed(lambda: dev.find_extremes(out, out=ex), steps, 2)
+    res["find_extremes"] = {"ms": round(float(np.median(times)), 4),
+                            "GB/s": round(4.0 * ow * ow / (float(np.median(times)) * 1e-3) / 1e9, 1)}
     # match_template_with_mask (reference-order direct path), seeded mask
     mask = torch.from_numpy(uniform_u8(202, (MATCH_T, MATCH_T))).cuda()
     out = dev.match_template_with_mask(gi, gt, M.CrossCorrelationNormalized, mask)
@@ -333,8 +428,21 @@ def bench_match(steps: int, cpu: bool = False):
     masked_ms = float(np.median(times))
     res["CrossCorrelationNormalized:masked_direct"] = {"ms_per_image": round(masked_ms, 4)}
     if cpu:
-        import oracle as O
-
+        nt = cpu_threads()
+        cpu_ms = {}
+        for m in M:
+            t0 = time.perf_counter()
+            O.match_template(img, tpl, int(m), nthreads=nt)
+            cpu_ms[m.name] = (time.perf_counter() - t0) * 1e3
+        ccn = cpu_ms[M.CrossCorrelationNormalized.name]
+        res["cpu_baseline"] = {
+            "value": round(ccn, 1), "unit": "ms/image (CrossCorrelationNormalized)",
+            "cores": nt, "kind": "port",
+            "per_method_ms": {k: round(v, 1) for k, v in cpu_ms.items()},
+            "sample": f"the full {MATCH_SIDE}x{MATCH_SIDE} image and {MATCH_T}x{MATCH_T} template, "
+                      "each method once, oracle/oracle.c's match_template restated as "
+                      f"match_template_parallel (output rows over threads) on {cpu_desc(nt)}",
+            "gpu_mfma_over_cpu": round(ccn / res["CrossCorrelationNormalized:mfma"]["ms_per_image"], 1)}
         crop, t_np, m_np = np.ascontiguousarray(img[:512, :512]), tpl, mask.cpu().numpy()
         t0 = time.perf_counter()
         O.match_template_with_mask(crop, t_np, m_np, int(M.CrossCorrelationNormalized))
@@ -344,7 +452,7 @@ def bench_match(steps: int, cpu: bool = False):
         res["CrossCorrelationNormalized:masked_direct"]["cpu_baseline"] = {
             "windows/s": round(cpu_wps, 1), "cores": 1, "kind": "port",
             "sample": f"512x512 corner of the image, the same {MATCH_T}x{MATCH_T} template and "
-                      "mask, oracle/oracle.c, one call",
+                      f"mask, oracle/oracle.c, one call on one core of {cpu_model()}",
             "gpu_over_cpu": round(gpu_wps / cpu_wps, 1)}
     return res
 
@@ -438,7 +546,7 @@ def bench_consumers(steps: int, cpu: bool = True):
     return res
 
 
-def bench_precision(steps: int):
+def bench_precision(steps: int, cpu: bool = False):
     """CrossCorrelationNormalized on the 4096x4096 f32 field: BF16x3 MFMA vs
     FP32 direct, each with max abs / rel error over every output against the
     float64 reference, and argmax agreement."""
@@ -477,12 +585,28 @@ def bench_precision(steps: int):
                     "bound": "mfma", "achieved": round(flops / (ms * 1e-3) / 1e12, 2),
                     "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": round(flops / (ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4)}
+        if cpu:
+            import oracle as O
+
+            nt = cpu_threads()
+            rows = 256 + t - 1
+            a = img[:rows].cpu().numpy()
+            tt = tpl.cpu().numpy()
+            t0 = time.perf_counter()
+            O.match_template(a, tt, int(M.CrossCorrelationNormalized), nthreads=nt)
+            dt = time.perf_counter() - t0
+            entry["cpu_baseline"] = {
+                "value": round(dt * (ow / 256) * 1e3, 1), "unit": "ms/image (estimated)",
+                "cores": nt, "kind": "port",
+                "sample": f"256 output rows (the top {rows} image rows) in {dt:.2f} s, the "
+                          f"reference's f32 loop (oracle/oracle.c match_template_parallel "
+                          f"restatement, {cpu_desc(nt)}), scaled to {ow} rows"}
         res[f"template_{t}"] = entry
         del ref
     return res
 
 
-def bench_batch(steps: int):
+def bench_batch(steps: int, cpu: bool = False):
     """The batch workload three ways: per-call launches (one library call per
     job), the planned batch enqueued eagerly, and one hipGraph replay of it;
     outputs compared bit for bit between per-call and graph."""
@@ -532,9 +656,14 @@ def bench_batch(steps: int):
     n_tpl = sum(len(t) for _, _, _, t in jobs)
     ops = sum(2.0 * (img.shape[0] - t.shape[0] + 1) * (img.shape[1] - t.shape[1] + 1) * t.numel()
               for img, _, _, tpls in jobs for t in tpls)
-    nbytes = 4.0 * px  # blur and filter: one read and one write of every pixel each
+    map_bytes = sum(4.0 * (img.shape[0] - t.shape[0] + 1) * (img.shape[1] - t.shape[1] + 1)
+                    for img, _, _, tpls in jobs for t in tpls)
+    # blur and filter: one read and one write of every pixel each; every match
+    # reads its image once and writes its f32 score map
+    nbytes = 4.0 * px + sum(img.numel() * len(tpls) for img, _, _, tpls in jobs) + map_bytes
     g_ms = float(np.median(t_graph))
     ideal_ms = (nbytes / (HBM_PEAK_GBS * 1e9) + ops / (INT8_PEAK_TOPS * 1e12)) * 1e3
+    cpu_line = batch_cpu_baseline(jobs) if cpu else None
     res = {
         "workload": f"{BATCH_IMAGES} GrayImages, sides {BATCH_SIDE_RANGE[0]}-{BATCH_SIDE_RANGE[1]}, "
                     f"{px / 1e6:.1f} Mpix; {n_tpl} templates of sides {BATCH_TPL_RANGE[0]}-"
@@ -549,13 +678,62 @@ def bench_batch(steps: int):
         "tuning": bt.tuning().strip().splitlines(),
         "roofline": {"bound": "hbm+int8", "ideal_ms": round(ideal_ms, 3),
                      "frac": round(ideal_ms / g_ms, 4),
-                     "work": {"hbm_bytes": nbytes, "int8_ops": ops}},
+                     "work": {"hbm_bytes": nbytes, "score_map_bytes": map_bytes,
+                              "int8_ops": ops}},
+        "cpu_baseline": cpu_line,
     }
     bt.close()
     return res
 
 
-def bench_proposal(n_images: int = 12, per_image: int = 4):
+def batch_cpu_baseline(jobs, budget_s: float = 12.0):
+    """The per-call loop the reference runs for the batch (gaussian_blur_f32
+    single-threaded, filter_clamped_parallel and match_template_parallel over
+    the host threads), restated by the oracle, on the batch's first images
+    until the budget is spent; the rest of the batch is scaled by the rates
+    measured (s per blurred and per filtered pixel, s per correlation MAC)."""
+    import oracle as O
+    from imageproc_amd.template_matching import MatchTemplateMethod as M
+
+    nt = cpu_threads()
+    t_blur = t_filt = t_match = 0.0
+    px_done = mac_done = 0.0
+    n_img = n_match = 0
+    t0 = time.perf_counter()
+    for img, sigma, k, tpls in jobs:
+        if time.perf_counter() - t0 > budget_s:
+            break
+        a = img.cpu().numpy()
+        t1 = time.perf_counter()
+        O.gaussian_blur_f32(a, sigma)
+        t2 = time.perf_counter()
+        O.filter_clamped(a, k, 5, 5, np.uint8, nthreads=nt)
+        t3 = time.perf_counter()
+        for t in tpls:
+            tt = t.cpu().numpy()
+            O.match_template(a, tt, int(M.CrossCorrelationNormalized), nthreads=nt)
+            mac_done += float((a.shape[0] - tt.shape[0] + 1) * (a.shape[1] - tt.shape[1] + 1) *
+                              tt.size)
+            n_match += 1
+        t4 = time.perf_counter()
+        t_blur += t2 - t1
+        t_filt += t3 - t2
+        t_match += t4 - t3
+        px_done += a.size
+        n_img += 1
+    px_all = float(sum(img.numel() for img, _, _, _ in jobs))
+    mac_all = float(sum((img.shape[0] - t.shape[0] + 1) * (img.shape[1] - t.shape[1] + 1) *
+                        t.numel() for img, _, _, tpls in jobs for t in tpls))
+    est_s = (t_blur + t_filt) / px_done * px_all + t_match / mac_done * mac_all
+    return {"value": round(est_s * 1e3, 1), "unit": "ms per batch (estimated)", "cores": nt,
+            "kind": "port",
+            "sample": f"the first {n_img} images of the batch with their blur, 5x5 filter and "
+                      f"{n_match} matches in {t_blur + t_filt + t_match:.1f} s (oracle/oracle.c per "
+                      f"call, {cpu_desc(nt)}), scaled to all {len(jobs)} images by the measured "
+                      "per-pixel and per-MAC rates"}
+
+
+def bench_proposal(n_images: int = 12, per_image: int = 4, cpu: bool = False):
     """Learned proposal on 8192x8192 smooth noise with perturbed 48x48
     templates (noise sigma in {0, 8, 16}, brightness x U(0.8, 1.2), 10 %
     occlusion): the plain search (full CrossCorrelationNormalized map +
@@ -598,6 +776,23 @@ def bench_proposal(n_images: int = 12, per_image: int = 4):
     prop_ms = float(np.median([r[6] for r in rec])) * 1e3
     coarse_ms = float(np.median([r[7] for r in rec])) * 1e3
     ops = 2.0 * (8192 - 47) ** 2 * 48 * 48
+    cpu_line = None
+    if cpu:
+        import oracle as O
+
+        nt = cpu_threads()
+        a = img[:512].cpu().numpy()
+        tt = tpl.cpu().numpy()
+        t0 = time.perf_counter()
+        O.match_template(a, tt, int(CCN), nthreads=nt)
+        dt = time.perf_counter() - t0
+        scale = (8192 - 47) / (512 - 47)
+        cpu_line = {"value": round(dt * scale * 1e3, 1),
+                    "unit": "ms per full-map search (estimated)", "cores": nt, "kind": "port",
+                    "sample": f"CrossCorrelationNormalized full map of the top 512 rows of one "
+                              f"8192x8192 pair in {dt:.2f} s (oracle/oracle.c "
+                              f"match_template_parallel restatement, {cpu_desc(nt)}), scaled by "
+                              "output rows; argmax by a scan is not included"}
     return {
         "workload": f"{n} pairs: 8192x8192 u8 smooth noise (sigma 4), 48x48 templates with noise "
                     "sigma in {0,8,16}, brightness x U(0.8,1.2), 10% occlusion; "
@@ -614,6 +809,7 @@ def bench_proposal(n_images: int = 12, per_image: int = 4):
         "top1_agreement_network": sum(r[3] == r[0] for r in rec) / n,
         "top1_agreement_coarse_only": sum(r[4] == r[0] for r in rec) / n,
         "fallback_margin": margin,
+        "cpu_baseline": cpu_line,
     }
 
 
@@ -624,7 +820,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--leg", default="all",
-                    choices=("all", "blur", "copy", "match", "proposal", "consumers", "batch"),
+                    choices=("all", "blur", "copy", "match", "precision", "proposal", "consumers",
+                             "batch"),
                     help="blur: only the blur batch (for counter runs); copy: only a plain "
                          "copy of the same size (counter calibration); match: only the "
                          "match_template leg; proposal: only the learned-proposal search "
@@ -642,7 +839,12 @@ def main():
 
     print(f"[bench] device: {ip.device_info()}", flush=True)
     if args.leg == "match":
-        print(json.dumps({"leg": "match", **bench_match(max(5, args.steps // 2))}), flush=True)
+        print(json.dumps({"leg": "match", **bench_match(max(5, args.steps // 2),
+                                                        cpu=not args.no_cpu)}), flush=True)
+        return
+    if args.leg == "precision":
+        print(json.dumps({"leg": "precision", **bench_precision(3, cpu=not args.no_cpu)}),
+              flush=True)
         return
     if args.leg == "consumers":
         print(json.dumps({"leg": "consumers", **bench_consumers(args.steps)}), flush=True)
@@ -651,24 +853,25 @@ def main():
         print(json.dumps({"leg": "proposal", **bench_proposal()}), flush=True)
         return
     if args.leg == "batch":
-        print(json.dumps({"leg": "batch", **bench_batch(4)}), flush=True)
+        print(json.dumps({"leg": "batch", **bench_batch(4, cpu=not args.no_cpu)}), flush=True)
         return
     if args.leg == "copy":
         ms, gbs = bench_copy(args.steps, args.warmup)
         print(json.dumps({"leg": "copy", "bytes": 2 * BLUR_N * BLUR_W * BLUR_H,
                           "ms": round(ms, 4), "GB/s": round(gbs, 1)}), flush=True)
         return
-    ms, wall, nbytes, ok = bench_blur(args.steps, args.warmup)
+    ms, wall, nbytes, par = bench_blur(args.steps, args.warmup)
+    ok = par["bit_exact"] == par["images_checked"]
     if args.leg == "blur":
-        print(json.dumps({"leg": "blur", "ms": round(ms, 4), "parity_spot_check": bool(ok),
+        print(json.dumps({"leg": "blur", "ms": round(ms, 4), "parity": par,
                           "GB/s": round(nbytes / (ms * 1e-3) / 1e9, 1)}), flush=True)
         return
     gpix = BLUR_N * BLUR_W * BLUR_H / (ms * 1e-3) / 1e9
     achieved = nbytes / (ms * 1e-3) / 1e9
     match = bench_match(max(5, args.steps // 2), cpu=not args.no_cpu)
-    precision = bench_precision(3)
-    batch = bench_batch(4)
-    proposal = bench_proposal()
+    precision = bench_precision(3, cpu=not args.no_cpu)
+    batch = bench_batch(4, cpu=not args.no_cpu)
+    proposal = bench_proposal(cpu=not args.no_cpu)
     consumers = bench_consumers(max(5, args.steps // 2), cpu=not args.no_cpu)
     cpu = None if args.no_cpu else cpu_blur_baseline()
     line = {
@@ -687,10 +890,16 @@ def main():
         "config": {"workload": f"gaussian_blur_f32 sigma {BLUR_SIGMA} over {BLUR_N} distinct "
                                f"{BLUR_W}x{BLUR_H} GrayImages resident in HBM",
                    "images": BLUR_N, "width": BLUR_W, "height": BLUR_H, "sigma": BLUR_SIGMA,
-                   "parity_spot_check": bool(ok), "wall_s": round(wall, 3)},
+                   "parity": par, "all_bit_exact": bool(ok), "wall_s": round(wall, 3),
+                   "single_image_ms": par["single_image_ms"],
+                   "single_image_gpix_s": round(BLUR_W * BLUR_H / (par["single_image_ms"] * 1e-3)
+                                                / 1e9, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic()},
+                     "traffic": measured_traffic(),
+                     "traffic_source": "read from " + os.path.relpath(TRAFFIC_PROFILE, ROOT) +
+                                       " (rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this "
+                                       "workload, copy-calibrated), not measured in this run"},
         "match_template": match,
         "precision_f32": precision,
         "batch": batch,
