@@ -308,8 +308,11 @@ int blocks_for(int tw) { return (31 + tw + 31) / 32; }
 //   * C' = sum I'T' on MFMA against the template's Toeplitz slices, which
 //     each lane cuts from the LDS-staged template rows (zero padded) with
 //     five dword reads and v_alignbyte: B[k][n] = T'[dy][32 kb + k - n];
-//   * S' = sum I' over each window on MFMA too, with the same A operands
-//     against an all-ones Toeplitz band held in registers: exact in i32;
+//   * S' = sum I' over each window: every staged row's horizontal tw-window
+//     sums on MFMA (A = 32 staged rows, B = an all-ones Toeplitz band held
+//     in registers), written over the image block as i16 once the
+//     correlation is done, then slid vertically like sum I^2 (for KB = 1,
+//     whose image block is too narrow, the band MFMA runs per template row);
 //   * sum I^2 by exact u32 sliding sums: each staged row's horizontal
 //     tw-window sums of squares go to LDS (16 outputs per task), and each lane
 //     slides the vertical th-window sum down its own output column, so the
@@ -317,29 +320,14 @@ int blocks_for(int tw) { return (31 + tw + 31) / 32; }
 // Then sum I T = C' + 128 S' + 128 sum T and every method is finished from
 // exact integers (finish_score). Nothing but the map leaves the workgroup.
 constexpr int kFusedHP = 33;  // u32 per row of the horizontal sums (odd: column reads hit distinct banks)
+// i16 per row of the horizontal sums of I' (rows 4 apart 16 banks apart); 80
+// bytes per row fit the image block's row pitch for KB >= 2.
+constexpr int kH1P = 40;
 
 __host__ __device__ constexpr int fused_pitch(int KB) {
     return ((KB * 32 / 16) % 2 == 0) ? KB * 32 + 16 : KB * 32;  // odd multiple of 16 bytes
 }
-
-// finish_score over 32-bit exact sums: for templates up to 128 x 128, sum I T
-// and sum I^2, sum T^2 stay below 255^2 * 128^2 < 2^31, and SSE = sum T^2 -
-// 2 sum I T + sum I^2 is exact in u32. F64: normalise in f64 (windows of at
-// least 64 x 64 pixels), else in f32 like the reference.
-template <int METHOD, bool F64>
-__device__ __forceinline__ float finish_u32(uint32_t c, uint32_t sii, uint32_t stt) {
-    const bool sse = (METHOD == IPH_SSE || METHOD == IPH_SSE_NORMED);
-    const uint32_t num = sse ? stt + sii - 2u * c : c;
-    if (METHOD == IPH_SSE || METHOD == IPH_CCORR) return (float)num;
-    if constexpr (!F64) {
-        const float norm = sqrtf((float)sii * (float)stt);
-        const float sn = (float)num;
-        return norm > 0.0f ? sn / norm : sn;
-    } else {
-        const double norm = sqrt((double)sii * (double)stt);
-        return norm > 0.0 ? (float)((double)num / norm) : (float)num;
-    }
-}
+static_assert(fused_pitch(2) >= 2 * kH1P, "the i16 row sums fit over the image block");
 
 // LDS of a fused workgroup that takes template rows at most tc at a time.
 size_t fused_lds(int KB, int tw, int th, int RT, bool sums, int tc) {
@@ -382,18 +370,51 @@ __device__ __forceinline__ float div_rn(float a, float b) {
     return __builtin_fmaf(__builtin_fmaf(-b, q, a), r, q);
 }
 
-// finish_u32 with the f32 normalisation spelled out (same values: the
-// reference's sqrt and division are IEEE, and so are sqrt_rn and div_rn in
-// the ranges here: sum I^2 * sum T^2 is 0 or at least 1, the quotient below
-// 2^32).
-template <int METHOD>
-__device__ __forceinline__ float finish_u32_fast(uint32_t c, uint32_t sii, uint32_t stt) {
+// sqrt(x) and a / b correctly rounded in f64 for 1 <= x < 2^64, 0 <= a <
+// 2^32, 1 <= b < 2^32 (no scaling: every intermediate is normal). sqrt: the
+// v_rsq_f64 seed, one coupled Newton step for sqrt(x) and 1 / (2 sqrt(x))
+// (Goldschmidt), then two residual corrections; division: the v_rcp_f64
+// seed refined by two Newton steps, then one residual correction
+// (Markstein). Both match IEEE sqrt and division on every one of 2e7 random
+// operands in these ranges with seeds perturbed by up to 2^-22.
+__device__ __forceinline__ double sqrt_rn64(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    g = __builtin_fma(__builtin_fma(-g, g, x), h, g);
+    return __builtin_fma(__builtin_fma(-g, g, x), h, g);
+}
+__device__ __forceinline__ double div_rn64(double a, double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    r = __builtin_fma(__builtin_fma(-b, r, 1.0), r, r);
+    r = __builtin_fma(__builtin_fma(-b, r, 1.0), r, r);
+    const double q = a * r;
+    return __builtin_fma(__builtin_fma(-b, q, a), r, q);
+}
+
+// finish_score over 32-bit exact sums: for templates up to 128 x 128, sum I T
+// and sum I^2, sum T^2 stay below 255^2 * 128^2 < 2^31, and SSE = sum T^2 -
+// 2 sum I T + sum I^2 is exact in u32. F64: normalise in f64 (windows of at
+// least 64 x 64 pixels), else in f32 like the reference. The sqrt and the
+// division are IEEE (sqrt_rn, div_rn and their f64 forms in the ranges here:
+// sum I^2 * sum T^2 is 0 or at least 1, the quotient below 2^32).
+template <int METHOD, bool F64>
+__device__ __forceinline__ float finish_u32(uint32_t c, uint32_t sii, uint32_t stt) {
     const bool sse = (METHOD == IPH_SSE || METHOD == IPH_SSE_NORMED);
     const uint32_t num = sse ? stt + sii - 2u * c : c;
     if (METHOD == IPH_SSE || METHOD == IPH_CCORR) return (float)num;
-    const float norm = sqrt_rn((float)sii * (float)stt);
-    const float sn = (float)num;
-    return norm > 0.0f ? div_rn(sn, norm) : sn;
+    if constexpr (F64) {
+        const double p = (double)sii * (double)stt;  // 0 or at least 1
+        const bool pos = p > 0.0;
+        const double q = div_rn64((double)num, sqrt_rn64(pos ? p : 1.0));
+        return pos ? (float)q : (float)num;
+    } else {
+        const float norm = sqrt_rn((float)sii * (float)stt);
+        const float sn = (float)num;
+        return norm > 0.0f ? div_rn(sn, norm) : sn;
+    }
 }
 
 // The output tile at (x0, y0) of one match (the body of the single-image and
@@ -415,6 +436,7 @@ __device__ __forceinline__ void match_i8_fused_tile(const uint8_t *__restrict__ 
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ uint32_t tred[2][8];
     constexpr bool kSums = METHOD != IPH_CCORR;  // sum I^2 needed
+    constexpr bool kH1 = KB >= 2;  // window sums of I' from per-row sums (fits the image block)
     constexpr int PITCH = fused_pitch(KB), kTP = tpl_pitch(KB);
     constexpr int kTW = kTP / 4;       // u32 words per staged template row
     constexpr int kLPR = 2 * KB;       // lanes per staged image row, 16 bytes each
@@ -528,7 +550,7 @@ __device__ __forceinline__ void match_i8_fused_tile(const uint8_t *__restrict__ 
             }
         }
         __syncthreads();
-        if (c0 == 0) {
+        if (!kH1 && c0 == 0) {
             // all-ones band: B[k][n] = 1 for 0 <= 32 kb + k - n < tw, cut from
             // the ones row like the template's Toeplitz slices
             const uint32_t *q0 = onesrow + (u0 >> 2);
@@ -582,9 +604,11 @@ __device__ __forceinline__ void match_i8_fused_tile(const uint8_t *__restrict__ 
             }
         }
 
-        // correlation and window sums of I' on MFMA over the chunk's template rows
+        // correlation on MFMA over the chunk's template rows (with KB = 1, the
+        // window sums of I' too, against the all-ones band)
         const int8_t *arow = blk + (wave * 32 + n) * PITCH + 16 * g;
         const uint32_t *trow = tp + (u0 >> 2);
+#pragma unroll 1
         for (int dy = 0; dy < dyn; dy++) {
 #pragma unroll
             for (int kb = 0; kb < KB; kb++) {
@@ -597,31 +621,85 @@ __device__ __forceinline__ void match_i8_fused_tile(const uint8_t *__restrict__ 
                 bt[3] = (int)__builtin_amdgcn_alignbyte(d4, d3, sh);
                 const v4i a = *reinterpret_cast<const v4i *>(arow + 32 * kb);
                 acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, bt, acc, 0, 0, 0);
-                acs = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, ones[kb], acs, 0, 0, 0);
+                if constexpr (!kH1) acs = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, ones[kb], acs, 0, 0, 0);
             }
             arow += PITCH;
             trow += kTW;
         }
 
-        if constexpr (kSums) {
-            __syncthreads();  // every row's horizontal sums are in h2
+        // KB >= 2: the horizontal tw-window sums H[r][n] of I' of every staged
+        // row, once per row, on MFMA against the all-ones band (row blocks
+        // wave and wave + RT: chunks are at most 32 RT + 1 rows high, so there
+        // are at most 2 RT blocks), then written over the image block as i16
+        // (|H| <= 128 * 128) for the vertical slide
+        const int16_t *h1s = reinterpret_cast<const int16_t *>(blk);
+        if constexpr (kH1) {
+            const int nrb = (rows + 31) >> 5;
+            // both blocks on every wave (rows clamped): a block past the last
+            // one is computed and dropped, which keeps the MFMAs unconditional
+            const int8_t *ar0 = blk + min(32 * wave + n, rows - 1) * PITCH + 16 * g;
+            const int8_t *ar1 = blk + min(32 * (wave + RT) + n, rows - 1) * PITCH + 16 * g;
+            v16i hb0, hb1;
+#pragma unroll
+            for (int r = 0; r < 16; r++) hb0[r] = hb1[r] = 0;
+#pragma unroll
+            for (int kb = 0; kb < KB; kb++) {
+                // the all-ones band B[k][n] = 1 for 0 <= 32 kb + k - n < tw,
+                // cut from the ones row like the template's Toeplitz slices
+                const uint32_t *q = onesrow + (u0 >> 2) + 8 * kb;
+                v4i ob;
+#pragma unroll
+                for (int d = 0; d < 4; d++)
+                    ob[d] = (int)__builtin_amdgcn_alignbyte(q[d + 1], q[d], sh);
+                hb0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+                    *reinterpret_cast<const v4i *>(ar0 + 32 * kb), ob, hb0, 0, 0, 0);
+                hb1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+                    *reinterpret_cast<const v4i *>(ar1 + 32 * kb), ob, hb1, 0, 0, 0);
+            }
+            __syncthreads();  // every wave is done reading the image block
+            // register r holds row 32 b + 4 g + 8 (r >> 2) + (r & 3) of column n
+            int16_t *hw = reinterpret_cast<int16_t *>(blk) + (32 * wave + 4 * g) * kH1P + n;
+            const int lim = rows - (32 * wave + 4 * g);  // rows left from the lane's first
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int k = 8 * (r >> 2) + (r & 3);
+                if (k < lim) hw[k * kH1P] = (int16_t)hb0[r];
+            }
+            if (wave + RT < nrb) {
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    const int k = 32 * RT + 8 * (r >> 2) + (r & 3);
+                    if (k < lim) hw[k * kH1P] = (int16_t)hb1[r];
+                }
+            }
+        }
+
+        if constexpr (kSums || kH1) {
+            __syncthreads();  // every row's horizontal sums are in LDS
             // lane (n, g) holds rows 32w + 8j + 4g + i (register 4j + i) of column
-            // n: slide the chunk's dyn-row sum of h2 down rows 32w + 4g .. + 27
+            // n: slide the chunk's dyn-row sums down rows 32w + 4g .. + 27
             const uint32_t *hc = h2 + (wave * 32 + 4 * g) * kFusedHP + n;
+            const int16_t *hc1 = h1s + (wave * 32 + 4 * g) * kH1P + n;
             uint32_t sq = 0;
-            for (int d = 0; d < dyn; d++) sq += hc[d * kFusedHP];
+            int s1w = 0;
+            for (int d = 0; d < dyn; d++) {
+                if constexpr (kSums) sq += hc[d * kFusedHP];
+                if constexpr (kH1) s1w += hc1[d * kH1P];
+            }
 #pragma unroll
             for (int j = 0; j < 4; j++) {
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
                     const int yl = 8 * j + i;  // row offset from 32w + 4g
-                    sqv[4 * j + i] += sq;
+                    if constexpr (kSums) sqv[4 * j + i] += sq;
+                    if constexpr (kH1) acs[4 * j + i] += s1w;
                     // to row yl + 1 (past the other half's four rows after i = 3)
                     if (j < 3 || i < 3) {
 #pragma unroll
                         for (int k = 0; k < ((i == 3) ? 5 : 1); k++) {
                             const int r = yl + k;
-                            sq += hc[(r + dyn) * kFusedHP] - hc[r * kFusedHP];
+                            if constexpr (kSums) sq += hc[(r + dyn) * kFusedHP] - hc[r * kFusedHP];
+                            if constexpr (kH1) s1w += hc1[(r + dyn) * kH1P] - hc1[r * kH1P];
                         }
                     }
                 }
@@ -658,8 +736,7 @@ __device__ __forceinline__ void match_i8_fused_tile(const uint8_t *__restrict__ 
         for (int r = 0; r < 16; r++) {
             // sum I T = C' + 128 (S' + sum T), exact in 32 bits
             const uint32_t c = (uint32_t)(acc[r] + 128 * (acs[r] + (int)st));
-            if constexpr (F64) v[r] = finish_u32<METHOD, true>(c, sqv[r], stt);
-            else v[r] = finish_u32_fast<METHOD>(c, sqv[r], stt);
+            v[r] = finish_u32<METHOD, F64>(c, sqv[r], stt);
         }
         if (x < ow) {
 #pragma unroll
@@ -735,17 +812,20 @@ int fused_waves(size_t lds, int RT) {
 // Template rows per chunk for RT waves: the fewest near-equal chunks whose
 // footprint allows 16 waves per CU (else 12, else 8); 0 if no chunk of at
 // least kFusedMinChunk rows fits a workgroup. Every chunk restages the image
-// rows it meets, so fewer chunks save staging and horizontal sums.
+// rows it meets, so fewer chunks save staging and horizontal sums. Chunks
+// are at most 32 RT + 1 rows (the staged rows then form at most 2 RT blocks
+// of 32 for the per-row window sums).
 int fused_tc(int KB, int th, int RT, bool sums) {
     for (int waves : {16, 12, 8}) {
         for (int nc = 1; nc <= th; nc++) {
             const int tc = (th + nc - 1) / nc;
+            if (tc > 32 * RT + 1) continue;
             if (tc < kFusedMinChunk && nc > 1) break;
             const size_t lds = fused_lds(KB, 0, th, RT, sums, tc);
             if (lds <= fused_lds_limit() && fused_waves(lds, RT) >= waves) return tc;
         }
     }
-    const int tc = std::min(th, kFusedMinChunk);
+    const int tc = std::min({th, kFusedMinChunk, 32 * RT + 1});
     return fused_lds(KB, 0, th, RT, sums, tc) <= fused_lds_limit() ? tc : 0;
 }
 
