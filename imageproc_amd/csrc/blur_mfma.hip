@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 namespace iph {
 namespace {
@@ -1375,6 +1376,530 @@ iph_status launch_blur_stream(const uint8_t *src, uint8_t *dst, uint32_t w, uint
     return launch_status("blur_u8_stream_kernel");
 }
 
+
+// ---- per-wave streaming kernel ------------------------------------------------------
+//
+// Each wave owns a 32-column strip of one image over a band of rows and walks
+// down it 32 rows per step with no workgroup barrier: the columns its
+// vertical pass reads are exactly the intermediate columns its own horizontal
+// pass wrote, so a wave's LDS (its staged input rows, a 64-row ring of
+// intermediate columns and its output rows) is private to it. The four waves
+// of a workgroup take four neighbouring strips and the workgroups of one
+// image go to one XCD, so the 16 halo columns a strip loads beside its own
+// come from the L2 (or the L1) its neighbours filled.
+//
+// Step s of a wave (y = first output row of the band + 32 s):
+//   V(s):   output rows y .. y + 31 from ring rows 32 (s & 1) + [0, 16 NKB)
+//           (intermediate rows y - half - m + k), into the output rows in LDS,
+//           then 16 bytes per lane to the image;
+//   H(s+1): intermediate rows y + 64 - half - m + [0, 32) into ring slot
+//           s & 1, from input rows loaded two steps ahead straight into
+//           registers (converted to f16 in registers for the A operand, kept
+//           as bytes in LDS for the exact recomputations).
+// Ring offsets depend only on the step's parity: the loop is unrolled by two
+// and every LDS address is a per-lane base plus a constant.
+//
+// Readout: the fixed-point scheme of blur_fx_plan (exact hi accumulator from
+// 2^23 + c, lo accumulator from 0, one f32 add; bits 15..22 are the
+// reference's byte whenever bits 0..14 are at least 2c). The start value
+// enters through the last two K rows, which the taps leave zero (L + m + 33
+// <= 16 NKB): A = 2048 and 1 there against hi B rows 4096 and c, so every
+// partial sum of the hi accumulator is an integer below 2^24 and exact
+// whatever the order of the MFMA's additions. One ballot per pass tells
+// whether any output lies inside the margin; if one does, the wave re-reads
+// its operands from LDS, recomputes the accumulators, and each lane
+// recomputes its flagged outputs with the reference's chain and overwrites
+// its estimates in LDS.
+constexpr int kWaveOutPitch = 48;  // bytes per output row in LDS (16-byte aligned rows)
+
+// staged input row pitch: 8 bytes past a multiple of 16 (18 dwords for
+// NKB = 3), so the 8-byte A-operand reads of 32 rows spread over the banks
+template <int NKB> constexpr int wave_in_pitch() { return 16 * NKB + 8 + (NKB == 2 ? 16 : 0); }
+template <int NKB> constexpr int wave_lds_bytes() {
+    return 32 * wave_in_pitch<NKB>() + 32 * kRingPitch + 32 * kWaveOutPitch;
+}
+// LDS shared by the workgroup's waves: the zero-padded taps and the
+// Toeplitz tables (one, or two for distinct horizontal and vertical taps).
+template <int NKB, bool SHARE_B> constexpr int wave_shared_lds() {
+    return 2 * (2 * kTapPad + kMaxTaps) * (int)sizeof(float) + (SHARE_B ? 1 : 2) * 2 * NKB * 64 * 16;
+}
+
+struct WaveJob {
+    const uint8_t *src;
+    uint8_t *dst;
+    long long stride;
+    int W, H, n_img, strips, bands, band_rows;
+    int L, m;                    // taps, window shift (the staged columns start 4-aligned)
+    int ch, cv;                  // readout margins (units of 2^-15)
+    float h[kFxMaxL], v[kFxMaxL];
+};
+
+// The reference's chain over taps t < L (zero taps pad it to a multiple of
+// four: with non-negative taps every partial sum is >= +0, and adding +0
+// leaves it unchanged). `taps` is the zero-padded LDS copy (tap t at
+// kTapPad + t); `p(t)` returns input t as f32.
+// The chunk loop stays rolled: its tap loads then cannot be hoisted out of
+// the caller's loop over flagged outputs into registers.
+template <typename F>
+__device__ __forceinline__ float wave_chain(const float *taps, int L, F p) {
+    float s = 0.0f;
+#pragma unroll 1
+    for (int t0 = 0; t0 < L; t0 += 4) {
+#pragma unroll
+        for (int t = t0; t < t0 + 4; t++) s = __fadd_rn(s, __fmul_rn(p(t), taps[kTapPad + t]));
+    }
+    return s;
+}
+
+__device__ __forceinline__ void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// The accumulator start value in the last two K rows: B rows 4096 and c
+// (exact f16 integers, c < 4096) in the hi operand, zero in the lo operand.
+template <int NKB>
+__device__ __forceinline__ void wave_b_init(int g, int c, v8h (&bhi)[NKB], v8h (&blo)[NKB]) {
+    if (g == 1) {
+        bhi[NKB - 1][6] = (_Float16)4096.0f;
+        bhi[NKB - 1][7] = (_Float16)(float)c;
+        blo[NKB - 1][6] = (_Float16)0.0f;
+        blo[NKB - 1][7] = (_Float16)0.0f;
+    }
+}
+// ... and their A values 2048 and 1 (f16 0x6800, 0x3c00) in the lanes of
+// the upper K half of the last block.
+template <int NKB>
+__device__ __forceinline__ void wave_a_init(int g, v8h (&a)[NKB]) {
+    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+    v4u w = __builtin_bit_cast(v4u, a[NKB - 1]);
+    w[3] = g ? 0x3c006800u : w[3];
+    a[NKB - 1] = __builtin_bit_cast(v8h, w);
+}
+
+// The Toeplitz operands live in LDS, one 16-byte fragment per lane, laid out
+// [kb][hi, lo][lane] (built once per workgroup by wave_b_table): each pass
+// reads its 2 NKB fragments just before its MFMAs, so they take no registers
+// across the step.
+template <int NKB>
+__device__ __forceinline__ void wave_mfma(const v8h (&a)[NKB], const v8h *bt, int lane,
+                                          v16f &acc, v16f &acl) {
+    const v16f zero = {};
+    v8h bhi[NKB], blo[NKB];
+#pragma unroll
+    for (int kb = 0; kb < NKB; kb++) {
+        bhi[kb] = bt[(2 * kb) * 64 + lane];
+        blo[kb] = bt[(2 * kb + 1) * 64 + lane];
+    }
+#pragma unroll
+    for (int kb = 0; kb < NKB; kb++) {
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[kb], bhi[kb], kb == 0 ? zero : acc, 0, 0, 0);
+        acl = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[kb], blo[kb], kb == 0 ? zero : acl, 0, 0, 0);
+    }
+}
+
+// One wave writes the Toeplitz table of one tap set (with the start value
+// in its last two K rows) to LDS.
+template <int NKB>
+__device__ __forceinline__ void wave_b_table(const float *taps, int lane, int m, int c, v8h *bt) {
+    v8h bhi[NKB], blo[NKB];
+    build_b_fx<NKB>(taps, lane, m, bhi, blo);
+    wave_b_init<NKB>(lane >> 5, c, bhi, blo);
+#pragma unroll
+    for (int kb = 0; kb < NKB; kb++) {
+        bt[(2 * kb) * 64 + lane] = bhi[kb];
+        bt[(2 * kb + 1) * 64 + lane] = blo[kb];
+    }
+}
+
+// Strip modes: interior (the staged window lies inside 4-aligned rows),
+// edge (4-aligned rows, window crossing the left or right image edge) and
+// unaligned rows (every byte loaded from its clamped column). Each mode is
+// its own inlined copy of the strip loop, so the rarer modes' registers do
+// not weigh on the interior one.
+enum { kWaveInterior = 0, kWaveEdge = 1, kWaveUnaligned = 2 };
+
+template <int NKB, bool SHARE_B, int MODE>
+__device__ __forceinline__ void blur_u8_wave_strip(const WaveJob &jb, const float *taps_h,
+                                                   const float *taps_v, const v8h *bt_h,
+                                                   const v8h *bt_v, uint8_t *in_w,
+                                                   _Float16 *ring, uint8_t *out_w, int im,
+                                                   int band, int strip) {
+    constexpr int CIN = 16 * NKB, PI = wave_in_pitch<NKB>(), RPH = kRingPitch / 2;
+    constexpr int PO = kWaveOutPitch;
+    constexpr bool ALIGNED = MODE != kWaveUnaligned;
+    constexpr bool interior = MODE == kWaveInterior;
+    const int L = jb.L, m = jb.m, half = L / 2;
+    const int lane = threadIdx.x & 63;
+    const int n = lane & 31, g = lane >> 5;
+    const int W = jb.W, H = jb.H;
+    const int x0 = strip * 32, yb0 = band * jb.band_rows;
+    const int yb1 = min(H, yb0 + jb.band_rows);
+    const int nsteps = (yb1 - yb0 + 31) >> 5;
+    const uint8_t *src = jb.src + (size_t)im * jb.stride;
+    uint8_t *dst = jb.dst + (size_t)im * jb.stride;
+    const int xs = x0 - half - m;  // first staged column (a multiple of 4)
+    const int yh0 = yb0 + 32 - half - m;  // first intermediate row of H(0)
+
+    // zero ring: rows read with zero weight must hold finite values
+    for (int i = lane; i < 32 * kRingPitch / 16; i += 64)
+        reinterpret_cast<uint4 *>(ring)[i] = make_uint4(0, 0, 0, 0);
+
+    const uint32_t thr_h = 2u * (uint32_t)jb.ch, thr_v = 2u * (uint32_t)jb.cv;
+
+    // input row yh0 + 32 t + n of H(t), staged columns xs + 8 g + 16 kb + [0, 8),
+    // through raw buffers with 32-bit offsets from the image base (W * H < 2^31)
+    const auto srs = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t *>(src), 0,
+                                                       W * H, 0x00020000);
+    const auto drs = __builtin_amdgcn_make_buffer_rsrc(dst, 0, W * H, 0x00020000);
+    // H(t)'s window, input rows yh0 + 32 t + [0, 32) by staged columns
+    // xs + [0, CIN), is loaded as 16-byte row chunks (NLD per lane): chunk c
+    // is row c / NKB, columns 16 (c % NKB) + [0, 16). Rows then take few
+    // cache lines per load; the A operands are cut from the bytes in LDS.
+    constexpr int NCH = 32 * NKB, NLD = (NCH + 63) / 64;
+    using Chunks = uint4[NLD];
+    auto load_h = [&](int t, Chunks &d) {
+#pragma unroll
+        for (int k = 0; k < NLD; k++) {
+            const int c = min(lane + 64 * k, NCH - 1);
+            const int r = c / NKB, q = c - r * NKB;
+            const int row = clampi(yh0 + 32 * t + r, 0, H - 1) * W;
+            const int x = xs + 16 * q;
+            if (interior) {
+                // two 8-byte loads: a 16-byte load at an address 12 past a
+                // multiple of 16 (as xs is for 9 taps) returned wrong bytes
+                // at times on gfx950 here, 8-byte and 4-byte loads never did
+                const auto v0 = __builtin_amdgcn_raw_buffer_load_b64(srs, row + x, 0, 0);
+                const auto v1 = __builtin_amdgcn_raw_buffer_load_b64(srs, row + x + 8, 0, 0);
+                d[k] = make_uint4(v0[0], v0[1], v1[0], v1[1]);
+                continue;
+            }
+            uint32_t dw[4];
+#pragma unroll
+            for (int p = 0; p < 4; p++) {
+                const int x4 = x + 4 * p;
+                if (ALIGNED) {
+                    dw[p] = __builtin_amdgcn_raw_buffer_load_b32(srs, row + clampi(x4, 0, W - 4), 0, 0);
+                } else {
+                    dw[p] = 0;
+#pragma unroll
+                    for (int b = 0; b < 4; b++)
+                        dw[p] |= (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(
+                                     srs, row + clampi(x4 + b, 0, W - 1), 0, 0)
+                                 << (8 * b);
+                }
+            }
+            d[k] = make_uint4(dw[0], dw[1], dw[2], dw[3]);
+        }
+    };
+    // the loaded chunks into in_w (at the image edges, aligned dwords left of
+    // the row become its first byte, right of it its last), then this lane's
+    // A operand, row n, columns 8 g + 16 kb + [0, 8), as exact f16
+    auto stage_h = [&](Chunks &d, v8h (&a)[NKB]) {
+#pragma unroll
+        for (int k = 0; k < NLD; k++) {
+            const int c = lane + 64 * k;
+            if (NCH % 64 != 0 && c >= NCH) break;
+            const int r = c / NKB, q = c - r * NKB;
+            uint32_t dw[4] = {d[k].x, d[k].y, d[k].z, d[k].w};
+            if (ALIGNED && !interior) {
+#pragma unroll
+                for (int p = 0; p < 4; p++) {
+                    const int x4 = xs + 16 * q + 4 * p;
+                    const uint32_t left = (dw[p] & 0xffu) * 0x01010101u;
+                    const uint32_t right = (dw[p] >> 24) * 0x01010101u;
+                    dw[p] = x4 < 0 ? left : dw[p];
+                    dw[p] = x4 >= W ? right : dw[p];
+                }
+            }
+            uint8_t *o = in_w + r * PI + 16 * q;
+            *reinterpret_cast<uint2 *>(o) = make_uint2(dw[0], dw[1]);
+            *reinterpret_cast<uint2 *>(o + 8) = make_uint2(dw[2], dw[3]);
+        }
+        wave_lds_fence();  // the rows were written by other lanes
+#pragma unroll
+        for (int kb = 0; kb < NKB; kb++)
+            a[kb] = bytes_to_half8(*reinterpret_cast<const uint2 *>(in_w + n * PI + 8 * g + 16 * kb));
+        wave_a_init<NKB>(g, a);
+    };
+    // H readout into ring slot SLOT: the lane holds intermediate column n,
+    // rows 8 (r / 4) + 4 g + r % 4 of the step. In a flagged pass only the
+    // groups of four whose smallest fraction was below the margin are
+    // checked again, from the accumulators (still live).
+    auto h_readout = [&](auto slot_c, const v16f &acc, const v16f &acl) {
+        constexpr int SLOT = decltype(slot_c)::value;
+        _Float16 *col = ring + n * RPH + 32 * SLOT;
+        uint32_t mg[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            uint32_t q[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) q[i] = __float_as_uint(acc[4 * j + i] + acl[4 * j + i]);
+            *reinterpret_cast<uint2 *>(col + 8 * j + 4 * g) =
+                make_uint2(half2_bits15(q[0], q[1]), half2_bits15(q[2], q[3]));
+            mg[j] = min_frac4(q);
+        }
+        if (__ballot(min(min(mg[0], mg[1]), min(mg[2], mg[3])) < thr_h)) {
+            wave_lds_fence();  // in_w rows were written by other lanes
+            uint32_t fl = 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if (mg[j] < thr_h) {
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+                        fl |= ((__float_as_uint(acc[4 * j + i] + acl[4 * j + i]) & kFxFracMask) < thr_h)
+                                  ? (1u << (4 * j + i)) : 0u;
+                }
+            }
+            while (fl) {
+                const int r = __builtin_ctz(fl);
+                fl &= fl - 1u;
+                const int i = 8 * (r >> 2) + 4 * g + (r & 3);
+                const uint8_t *p = in_w + i * PI + n + m;
+                const float e = wave_chain(taps_h, L, [&](int t) { return (float)p[t]; });
+                col[i] = (_Float16)(float)clamp_u8_exact(e);
+            }
+        }
+    };
+    // V pass of step s (parity P): the lane holds output row n, columns
+    // 8 (r / 4) + 4 g + r % 4; the rows go to out_w, then to the image
+    const bool vec_out = ((W & 15) == 0) && ((reinterpret_cast<uintptr_t>(dst) & 15) == 0);
+    const bool dw_out = ALIGNED && ((reinterpret_cast<uintptr_t>(dst) & 3) == 0);
+    auto v_mfma = [&](auto par_c, v16f &acc, v16f &acl) {
+        constexpr int P = decltype(par_c)::value;
+        wave_lds_fence();  // ring columns were written by other lanes
+        const _Float16 *acol = ring + n * RPH + 8 * g;
+        v8h a[NKB];
+#pragma unroll
+        for (int kb = 0; kb < NKB; kb++)
+            a[kb] = *reinterpret_cast<const v8h *>(acol + ((32 * P + 16 * kb) & (kRing - 1)));
+        wave_a_init<NKB>(g, a);
+        wave_mfma<NKB>(a, bt_v, lane, acc, acl);
+    };
+    auto v_readout = [&](auto par_c, int s, const v16f &acc, const v16f &acl) {
+        constexpr int P = decltype(par_c)::value;
+        uint8_t *orow = out_w + n * PO;
+        uint32_t mg[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            uint32_t q[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) q[i] = __float_as_uint(acc[4 * j + i] + acl[4 * j + i]);
+            *reinterpret_cast<uint32_t *>(orow + 8 * j + 4 * g) = pack_bits15(q[0], q[1], q[2], q[3]);
+            mg[j] = min_frac4(q);
+        }
+        if (__ballot(min(min(mg[0], mg[1]), min(mg[2], mg[3])) < thr_v)) {
+            uint32_t fl = 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if (mg[j] < thr_v) {
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+                        fl |= ((__float_as_uint(acc[4 * j + i] + acl[4 * j + i]) & kFxFracMask) < thr_v)
+                                  ? (1u << (4 * j + i)) : 0u;
+                }
+            }
+            while (fl) {
+                const int r = __builtin_ctz(fl);
+                fl &= fl - 1u;
+                const int i = 8 * (r >> 2) + 4 * g + (r & 3);
+                const _Float16 *c = ring + i * RPH;
+                const int base = 32 * P + n + m;
+                const float e =
+                    wave_chain(taps_v, L, [&](int t) { return (float)c[(base + t) & (kRing - 1)]; });
+                orow[i] = clamp_u8_exact(e);
+            }
+        }
+        wave_lds_fence();  // out_w rows were written by other lanes
+        // rows y + lane / 2, columns x0 + 16 (lane % 2) + [0, 16)
+        const int yl = lane >> 1, xl = (lane & 1) * 16;
+        const int gy = yb0 + 32 * s + yl, gx = x0 + xl;
+        if (gy < yb1 && gx < W) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(out_w + yl * PO + xl);
+            const int o = gy * W + gx;
+            if (vec_out && gx + 16 <= W) {
+                typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+                const v4u vv = {v.x, v.y, v.z, v.w};
+                __builtin_amdgcn_raw_buffer_store_b128(vv, drs, o, 0, 0);
+            } else if (dw_out) {
+                const uint32_t dws[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    if (gx + 4 * q < W) __builtin_amdgcn_raw_buffer_store_b32(dws[q], drs, o + 4 * q, 0, 0);
+            } else {
+                const uint8_t *b = out_w + yl * PO + xl;
+                for (int q = 0; q < 16 && gx + q < W; q++) dst[o + q] = b[q];
+            }
+        }
+    };
+
+    // Software pipeline over two accumulator sets: X for H passes, Y for V
+    // passes. Iteration s converts H(s + 1)'s bytes and issues its MFMAs,
+    // reads V(s) out while they run, reads H(s + 1) out and issues V(s + 1)'s
+    // MFMAs, which run under the next iteration's conversion. H(t) goes to
+    // ring slot (t + 1) & 1; V(s) starts at ring row 32 (s & 1); H(t)'s bytes
+    // are loaded two passes ahead, into dA for odd t and dB for even t.
+    using P0 = std::integral_constant<int, 0>;
+    using P1 = std::integral_constant<int, 1>;
+    Chunks dA, dB;
+    v16f xc, xl, yc, yl;
+    auto h_mfma = [&](Chunks &d) {
+        v8h a[NKB];
+        stage_h(d, a);
+        wave_mfma<NKB>(a, bt_h, lane, xc, xl);
+    };
+    // (the loads run two passes ahead unconditionally: rows past the image
+    // are clamped, so the last ones read rows that are cached already, and
+    // the wait for each load stays a count of the loads issued after it)
+    load_h(-1, dA);
+    load_h(0, dB);
+    h_mfma(dA);
+    load_h(1, dA);
+    h_readout(P0{}, xc, xl);  // H(-1)
+    h_mfma(dB);
+    load_h(2, dB);
+    h_readout(P1{}, xc, xl);  // H(0)
+    v_mfma(P0{}, yc, yl);     // V(0)
+    auto iter = [&](auto par_c, int s, Chunks &dx) {
+        constexpr int P = decltype(par_c)::value;
+        using Q = std::integral_constant<int, 1 - P>;
+        const bool more = s + 1 < nsteps;
+        if (more) {
+            h_mfma(dx);  // H(s + 1)
+            load_h(s + 3, dx);
+        }
+        v_readout(par_c, s, yc, yl);  // V(s)
+        if (more) {
+            h_readout(par_c, xc, xl);  // H(s + 1) -> slot (s + 2) & 1
+            v_mfma(Q{}, yc, yl);       // V(s + 1)
+        }
+    };
+    for (int s = 0; s < nsteps; s += 2) {
+        iter(P0{}, s, dA);
+        if (s + 1 < nsteps) iter(P1{}, s + 1, dB);
+    }
+}
+
+template <int NKB, bool SHARE_B, bool ALIGNED>
+__device__ __forceinline__ void blur_u8_wave_body(const WaveJob &jb) {
+    constexpr int CIN = 16 * NKB, PI = wave_in_pitch<NKB>(), WL = wave_lds_bytes<NKB>();
+    extern __shared__ __align__(16) unsigned char smem[];
+    float *taps_h = reinterpret_cast<float *>(smem);
+    float *taps_v = taps_h + (2 * kTapPad + kMaxTaps);
+    v8h *bt_h = reinterpret_cast<v8h *>(smem + 2 * (2 * kTapPad + kMaxTaps) * sizeof(float));
+    v8h *bt_v = SHARE_B ? bt_h : bt_h + 2 * NKB * 64;
+    unsigned char *wbase = smem + wave_shared_lds<NKB, SHARE_B>() +
+                           __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) * WL;
+    uint8_t *in_w = wbase;                                           // 32 rows x PI bytes
+    _Float16 *ring = reinterpret_cast<_Float16 *>(wbase + 32 * PI);  // 32 columns x kRingPitch
+    uint8_t *out_w = wbase + 32 * PI + 32 * kRingPitch;              // 32 rows x PO bytes
+
+    const int L = jb.L;
+    if (threadIdx.x < 2 * kTapPad + kMaxTaps) {
+        const int t = (int)threadIdx.x - kTapPad;
+        const bool in = t >= 0 && t < L;
+        taps_h[threadIdx.x] = in ? jb.h[t] : 0.0f;
+        taps_v[threadIdx.x] = in ? jb.v[t] : 0.0f;
+    }
+    __syncthreads();  // taps staged
+    if (threadIdx.x < 64) wave_b_table<NKB>(taps_h, (int)threadIdx.x, jb.m, jb.ch, bt_h);
+    if (!SHARE_B && threadIdx.x >= 64 && threadIdx.x < 128)
+        wave_b_table<NKB>(taps_v, (int)threadIdx.x - 64, jb.m, jb.cv, bt_v);
+    __syncthreads();  // Toeplitz tables written; no workgroup barrier after this one
+
+    // wave unit = (image, band, strip), strip fastest; a workgroup takes four
+    // neighbouring units, and contiguous eighths of the workgroups go to the
+    // eight XCDs (workgroups are dealt round-robin). The wave index is read
+    // as a wave-uniform value, so the unit, the strip's geometry and the
+    // buffer descriptors live in scalar registers.
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int units = jb.n_img * jb.bands * jb.strips;
+    const int chunk = (((units + 3) >> 2) + 7) >> 3;
+    const int u = ((int)(blockIdx.x & 7) * chunk + (int)(blockIdx.x >> 3)) * 4 + wave;
+    if ((int)(blockIdx.x >> 3) >= chunk || u >= units) return;
+    const int per_img = jb.bands * jb.strips;
+    const int im = u / per_img, rem = u - im * per_img;
+    const int band = rem / jb.strips, strip = rem - band * jb.strips;
+    const int xs = strip * 32 - L / 2 - jb.m;
+    if (!ALIGNED)
+        blur_u8_wave_strip<NKB, SHARE_B, kWaveUnaligned>(jb, taps_h, taps_v, bt_h, bt_v, in_w,
+                                                         ring, out_w, im, band, strip);
+    else if (xs >= 0 && xs + CIN <= jb.W)
+        blur_u8_wave_strip<NKB, SHARE_B, kWaveInterior>(jb, taps_h, taps_v, bt_h, bt_v, in_w,
+                                                        ring, out_w, im, band, strip);
+    else
+        blur_u8_wave_strip<NKB, SHARE_B, kWaveEdge>(jb, taps_h, taps_v, bt_h, bt_v, in_w, ring,
+                                                    out_w, im, band, strip);
+}
+
+// The job is read through a reference to the kernel argument: its taps are
+// indexed per thread, which must not copy the argument into registers.
+template <int NKB, bool SHARE_B, bool ALIGNED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void
+blur_u8_wave_kernel(WaveJob job) {
+    blur_u8_wave_body<NKB, SHARE_B, ALIGNED>(job);
+}
+
+template <int NKB, bool SHARE_B> constexpr size_t wave_kernel_lds() {
+    return (size_t)wave_shared_lds<NKB, SHARE_B>() + 4 * (size_t)wave_lds_bytes<NKB>();
+}
+
+iph_status launch_blur_wave(const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h, uint32_t n,
+                            size_t stride, const float *hk, const float *vk, int L, int m,
+                            int nkb, const FxPlan &ph, const FxPlan &pv, bool same, bool aligned,
+                            hipStream_t s) {
+    WaveJob jb{};
+    jb.src = src;
+    jb.dst = dst;
+    jb.stride = (long long)stride;
+    jb.W = (int)w;
+    jb.H = (int)h;
+    jb.n_img = (int)n;
+    jb.strips = (int)((w + 31) / 32);
+    // bands of rows: enough waves to fill the chip a few times over
+    const long long per_band = (long long)n * jb.strips;
+    const int steps_total = (int)((h + 31) / 32);
+    int bands = 1;
+    if (per_band < 8192)
+        bands = (int)std::max<long long>(
+            1, std::min<long long>(steps_total, (8192 + per_band - 1) / per_band));
+    const int steps_per_band = (steps_total + bands - 1) / bands;
+    jb.band_rows = steps_per_band * 32;
+    jb.bands = (int)((h + jb.band_rows - 1) / jb.band_rows);
+    const long long units = per_band * jb.bands;
+    if (units == 0) return IPH_OK;
+    if (units > (1LL << 30)) {
+        set_error("batch of %u images of %ux%u is too large for one launch", n, w, h);
+        return IPH_EINVAL;
+    }
+    jb.L = L;
+    jb.m = m;
+    jb.ch = ph.c;
+    jb.cv = pv.c;
+    for (int i = 0; i < L; i++) {
+        jb.h[i] = hk[i];
+        jb.v[i] = vk[i];
+    }
+    const long long wgs = (units + 3) / 4;
+    const unsigned grid = (unsigned)((wgs + 7) / 8 * 8);
+#define WB(K, SAME, AL)                                                                      \
+    blur_u8_wave_kernel<K, SAME, AL><<<grid, 256, wave_kernel_lds<K, SAME>(), s>>>(jb)
+#define WA(K, SAME)                                                                          \
+    if (aligned) WB(K, SAME, true); else WB(K, SAME, false)
+#define WK(K)                                                                                \
+    if (same) { WA(K, true); } else { WA(K, false); }                                        \
+    break
+    switch (nkb) {
+    case 2: WK(2);
+    case 3: WK(3);
+    default: WK(4);
+    }
+#undef WK
+#undef WA
+#undef WB
+    return launch_status("blur_u8_wave_kernel");
+}
+
 }  // namespace
 
 // Taps the MFMA path handles: finite, at most 33 (sigma <= 8 for a Gaussian;
@@ -1455,6 +1980,14 @@ iph_status launch_blur_u8_mfma(const uint8_t *src, uint8_t *dst, uint32_t w, uin
 #define LK(K)                                                                                \
     if (same) { LA(K, true); } else { LA(K, false); }                                        \
     break
+    // the per-wave kernel: two K rows beyond the taps carry the start value
+    const int nkb_w = (L + m + 33 + 15) / 16;
+    if (L <= kFxMaxL && nkb_w <= 4 && (uint64_t)w * h < (1ull << 31)) {
+        const FxPlan sh = blur_fx_plan(hk, L, nkb_w), sv = blur_fx_plan(vk, L, nkb_w);
+        if (sh.c > 0 && sv.c > 0)
+            return launch_blur_wave(src, dst, w, h, n, stride, hk, vk, L, m, nkb_w, sh, sv, same,
+                                    aligned, s);
+    }
     if (L <= kFxMaxL && nkb >= 2 && nkb <= 4) {
         const FxPlan sh = blur_fx_plan(hk, L, nkb), sv = blur_fx_plan(vk, L, nkb);
         if (sh.c > 0 && sv.c > 0)

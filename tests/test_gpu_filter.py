@@ -265,3 +265,18 @@ def test_stream_separable_unequal_positive_taps(L):
     vk = (vk / np.float32(vk.sum() * 1.001)).astype(F)
     _same(flt.separable_filter(img, hk, vk), O.separable_filter(img, hk, vk))
     _same(flt.separable_filter(img, hk, hk), O.separable_filter(img, hk, hk))
+
+
+def test_wave_blur_device_batches_aligned_widths():
+    """Device batches whose widths take each store form of the per-wave
+    kernel (16-byte rows, 4-byte rows, byte rows), strips at both image
+    edges, odd and even step counts and several row bands per image."""
+    import torch
+    from imageproc_amd import device as dev
+    rng = np.random.default_rng(31)
+    for n, h, w, sigma in [(3, 70, 1024, 2.0), (2, 96, 100, 1.2), (4, 33, 64, 4.0),
+                           (2, 257, 36, 2.0), (1, 1000, 4, 0.8), (6, 45, 1040, 6.5)]:
+        imgs = rng.integers(0, 256, (n, h, w), dtype=np.uint8)
+        out = dev.gaussian_blur_f32(torch.from_numpy(imgs).cuda(), sigma).cpu().numpy()
+        for i in range(n):
+            _same(out[i], O.gaussian_blur_f32(imgs[i], sigma))
