@@ -362,6 +362,79 @@ __global__ __launch_bounds__(256) void box_rows_kernel(const uint8_t *__restrict
     box_line(src + (size_t)y * W, mid + (size_t)y * W, 1, W, seg * kBoxSeg, rx, kw);
 }
 
+// Rows pass for rows of up to kBoxLdsMaxSeg * kBoxSeg pixels: a workgroup
+// loads whole rows (256 / segments of them) into LDS with coalesced dword
+// loads, each thread slides its window over one kBoxSeg-pixel segment there,
+// and the rows are stored back coalesced. Segments are kBoxSegPitch bytes
+// apart in LDS (17 dwords), so the byte reads of a wave's threads, one
+// segment each, fall in distinct banks. Same sums as box_line.
+constexpr int kBoxSegPitch = kBoxSeg + 4;
+constexpr int kBoxLdsMaxSeg = 256;
+
+template <typename G>
+__device__ __forceinline__ uint32_t window_sum_first_f(G get, int n, int c, int r) {
+    long long lo = (long long)c - r, hi = (long long)c + r;
+    uint32_t s = 0;
+    long long nl = min(hi, -1LL) - lo + 1;
+    if (nl > 0) s += (uint32_t)nl * get(0);
+    long long nr = hi - max(lo, (long long)n) + 1;
+    if (nr > 0) s += (uint32_t)nr * get(n - 1);
+    long long a = max(lo, 0LL), b = min(hi, (long long)n - 1);
+    for (long long j = a; j <= b; j++) s += get((int)j);
+    return s;
+}
+
+__global__ __launch_bounds__(256) void box_rows_lds_kernel(const uint8_t *__restrict__ src,
+                                                           uint8_t *__restrict__ mid, int W,
+                                                           int H, int rx, uint32_t kw, int nseg,
+                                                           int rows_per_wg, bool aligned) {
+    extern __shared__ __align__(16) uint8_t box_sm[];
+    const int rowp = nseg * kBoxSegPitch;
+    uint8_t *in_s = box_sm, *out_s = box_sm + rows_per_wg * rowp;
+    const int y0 = blockIdx.x * rows_per_wg;
+    const int nrows = min(rows_per_wg, H - y0);
+    auto at = [&](int r, int c) { return r * rowp + (c >> 6) * kBoxSegPitch + (c & (kBoxSeg - 1)); };
+    const int dpr = (W + 3) >> 2;  // dwords per row
+    for (int i = threadIdx.x; i < nrows * dpr; i += 256) {
+        const int r = i / dpr, c = 4 * (i - r * dpr);
+        const uint8_t *row = src + (size_t)(y0 + r) * W;
+        uint32_t v;
+        if (aligned && c + 4 <= W) {
+            v = *reinterpret_cast<const uint32_t *>(row + c);
+        } else {
+            v = 0;
+            for (int b = 0; b < 4 && c + b < W; b++) v |= (uint32_t)row[c + b] << (8 * b);
+        }
+        *reinterpret_cast<uint32_t *>(in_s + at(r, c)) = v;
+    }
+    __syncthreads();
+    const int r = (int)threadIdx.x / nseg, seg = (int)threadIdx.x - r * nseg;
+    if (r < nrows) {
+        const uint8_t *in_r = in_s + r * rowp;
+        uint8_t *out_r = out_s + r * rowp;
+        auto get = [&](int c) { return (uint32_t)in_r[(c >> 6) * kBoxSegPitch + (c & (kBoxSeg - 1))]; };
+        const int c0 = seg * kBoxSeg;
+        uint32_t sum = window_sum_first_f(get, W, c0, rx);
+        const int c1 = min(c0 + kBoxSeg, W);
+        for (int c = c0; c < c1; c++) {
+            out_r[(c >> 6) * kBoxSegPitch + (c & (kBoxSeg - 1))] = (uint8_t)(sum / kw);
+            sum += get(min(c + 1 + rx, W - 1));
+            sum -= get(max(c - rx, 0));
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nrows * dpr; i += 256) {
+        const int rr = i / dpr, c = 4 * (i - rr * dpr);
+        uint8_t *row = mid + (size_t)(y0 + rr) * W;
+        const uint32_t v = *reinterpret_cast<const uint32_t *>(out_s + at(rr, c));
+        if (aligned && c + 4 <= W) {
+            *reinterpret_cast<uint32_t *>(row + c) = v;
+        } else {
+            for (int b = 0; b < 4 && c + b < W; b++) row[c + b] = (uint8_t)(v >> (8 * b));
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void box_cols_kernel(const uint8_t *__restrict__ mid,
                                                        uint8_t *__restrict__ dst, int W, int H,
                                                        int ry, uint32_t kh) {
@@ -591,10 +664,21 @@ iph_status launch_box_filter(const uint8_t *src, uint8_t *dst, uint32_t w, uint3
         return IPH_EINVAL;
     }
     uint32_t kw = 2u * rx + 1u, kh = 2u * ry + 1u;  // box_filter.rs:26-27 (u32)
-    long long nrow = (long long)((w + kBoxSeg - 1) / kBoxSeg) * h;
-    box_rows_kernel<<<(unsigned)((nrow + 255) / 256), 256, 0, s>>>(src, mid, (int)w, (int)h,
-                                                                   (int)rx, kw);
-    IPH_TRY(launch_status("box_rows_kernel"));
+    const uint32_t nseg = (w + kBoxSeg - 1) / kBoxSeg;
+    if (nseg <= (uint32_t)kBoxLdsMaxSeg) {
+        const int rows_per_wg = (int)(256 / nseg);
+        const size_t lds = 2 * (size_t)rows_per_wg * nseg * kBoxSegPitch;
+        const bool aligned = (w % 4) == 0 && (reinterpret_cast<uintptr_t>(src) & 3) == 0 &&
+                             (reinterpret_cast<uintptr_t>(mid) & 3) == 0;
+        box_rows_lds_kernel<<<(unsigned)((h + rows_per_wg - 1) / rows_per_wg), 256, lds, s>>>(
+            src, mid, (int)w, (int)h, (int)rx, kw, (int)nseg, rows_per_wg, aligned);
+        IPH_TRY(launch_status("box_rows_lds_kernel"));
+    } else {
+        long long nrow = (long long)nseg * h;
+        box_rows_kernel<<<(unsigned)((nrow + 255) / 256), 256, 0, s>>>(src, mid, (int)w, (int)h,
+                                                                       (int)rx, kw);
+        IPH_TRY(launch_status("box_rows_kernel"));
+    }
     dim3 grid((w + 255) / 256, (h + kBoxSeg - 1) / kBoxSeg);
     box_cols_kernel<<<grid, 256, 0, s>>>(mid, dst, (int)w, (int)h, (int)ry, kh);
     return launch_status("box_cols_kernel");
@@ -893,25 +977,27 @@ constexpr int kGradSW = kGradTW + kGradMaxTaps - 1, kGradSH = kGradTH + kGradMax
 template <bool K3, bool M24>
 __device__ __forceinline__ int grad_sum(const int *win, int pitch, int r, int c,
                                         const int32_t *k, int kw, int kh) {
-    int acc = 0;
+    // products and sums in uint32: two's-complement wrapping, defined in C++,
+    // the same bits as the reference's wrapping i32 arithmetic
+    uint32_t acc = 0;
     if (K3) {
 #pragma unroll
         for (int ky = 0; ky < 3; ky++)
 #pragma unroll
             for (int kx = 0; kx < 3; kx++) {
                 const int p = win[(r + ky) * pitch + c + kx], t = k[ky * 3 + kx];
-                acc += M24 ? __mul24(p, t) : p * t;
+                acc += M24 ? (uint32_t)__mul24(p, t) : (uint32_t)p * (uint32_t)t;
             }
     } else {
         for (int ky = 0; ky < kh; ky++) {
             const int *row = win + (r + ky) * pitch + c;
             for (int kx = 0; kx < kw; kx++) {
                 const int p = row[kx], t = k[ky * kw + kx];
-                acc += M24 ? __mul24(p, t) : p * t;
+                acc += M24 ? (uint32_t)__mul24(p, t) : (uint32_t)p * (uint32_t)t;
             }
         }
     }
-    return min(max(acc, -32768), 32767);
+    return min(max((int)acc, -32768), 32767);
 }
 
 template <bool K3, bool M24>

@@ -185,7 +185,13 @@ def test_box_filter_kat():
 
 @pytest.mark.parametrize("h,w,rx,ry", [(1, 1, 0, 0), (5, 7, 2, 3), (17, 3, 9, 1),
                                        (200, 199, 7, 7), (3, 50, 60, 4), (130, 70, 0, 99),
-                                       (500, 500, 7, 7)])
+                                       (500, 500, 7, 7),
+                                       # rows staged in LDS: one row of 256 segments per
+                                       # workgroup, 16 rows of 1024, windows wider than the
+                                       # row, window sums that wrap u32; 257 segments take
+                                       # the per-thread rows pass
+                                       (9, 16384, 3, 2), (64, 1024, 5, 5), (33, 260, 100, 1),
+                                       (5, 100, (1 << 24) - 1, 1), (3, 16448, 2, 2)])
 def test_box_filter_random(rng, h, w, rx, ry):
     img = rng.integers(0, 256, (h, w), dtype=np.uint8)
     _same(flt.box_filter(img, rx, ry), O.box_filter(img, rx, ry))

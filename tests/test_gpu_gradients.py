@@ -33,6 +33,11 @@ def test_elephant_sobel_truth(fixtures):
     (Kernel([1 << 24, 0, -(1 << 24), 3, 0, -3, 1 << 30, 5, -(1 << 30)], 3, 3),
      G.SOBEL_VERTICAL_3X3),
     (Kernel([1 << 27, -1, 7, -(1 << 29)], 2, 2), Kernel([1 << 23, 2, -3, 4], 2, 2)),
+    # the 24-bit product path's edges: +-(2^23 - 1) stay on it (255 of them in
+    # one window already wrap the 32-bit sum), 2^23 leaves it
+    (Kernel([(1 << 23) - 1] * 9, 3, 3), Kernel([-((1 << 23) - 1)] * 9, 3, 3)),
+    (Kernel([(1 << 23) - 1, -((1 << 23) - 1), 1 << 23, -(1 << 23)], 2, 2),
+     Kernel([-((1 << 23) - 1), 5, (1 << 23) - 1, -6], 2, 2)),
 ])
 def test_gradients_random(hk, vk):
     img = np.random.default_rng(hk.width * 7 + vk.height).integers(0, 256, (67, 131), dtype=np.uint8)

@@ -27,14 +27,17 @@ def test_robin_truths(fixtures):
     assert np.array_equal(flt.sharpen_gaussian(robin, 0.7, 7.0), fixtures["robin_sharpen_gaussian"])
 
 
-# sigma 3.5 gives 15 taps (the fused kernel's loop over any tap count),
-# sigma 13 gives 53 taps (the unfused path through f32 images); the others
-# take the fused kernel's 4-outputs-per-thread form
+# sigma 3.5 gives 15 taps, 6.0 gives 25 and 11.5 gives 47 (the fused
+# kernel's loop over any tap count, up to its 48-tap limit, on shapes whose
+# last 64 x 32 tile is partial), sigma 13 gives 53 taps (the unfused path
+# through f32 images); the others take the fused kernel's
+# 4-outputs-per-thread form
 @pytest.mark.parametrize("shape,sigma,amount", [((37, 53), 1.5, 2.0), ((300, 411), 0.7, 7.0),
                                                 ((128, 96), 4.0, 0.5), ((5, 200), 2.0, -1.0),
                                                 ((70, 1), 1.0, 3.0), ((90, 140), 13.0, 1.0),
                                                 ((65, 129), 0.4, 2.5), ((33, 70), 3.5, 1.0),
-                                                ((50, 77), 2.5, 1.0), ((41, 90), 3.0, 2.0)])
+                                                ((50, 77), 2.5, 1.0), ((41, 90), 3.0, 2.0),
+                                                ((97, 131), 6.0, 1.5), ((75, 150), 11.5, 0.8)])
 def test_sharpen_gaussian_random(shape, sigma, amount):
     img = np.random.default_rng(shape[1]).integers(0, 256, shape, dtype=np.uint8)
     assert np.array_equal(flt.sharpen_gaussian(img, sigma, amount),
