@@ -22,9 +22,10 @@
 // Each operand is split x = hi + lo (hi = bf16(x), lo = bf16(x - hi)) and
 //   sum I T ~= sum (I_hi T_hi + I_hi T_lo + I_lo T_hi),
 // with the same image-block x Toeplitz decomposition at K = 16. Template
-// rows are processed in chunks of 32, restaging the image rows of each chunk
-// in LDS, so any template height fits. The window energy sum I^2 is summed
-// in f64.
+// rows are processed in chunks, restaging the image rows of each chunk and
+// the chunk's template rows in LDS (the Toeplitz slices are cut from the
+// staged template rows), so any template height fits. The window energy
+// sum I^2 is summed in f64 by one separate kernel.
 #include "match_common.hpp"
 
 #include <algorithm>
@@ -46,8 +47,6 @@ __host__ __device__ constexpr int tpl_pitch(int KB) { return 32 * KB + 32; }
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 
-constexpr int kXT16 = 2;     // 32-column output tiles per wave (f32 images)
-constexpr int kDyChunk = 32; // template rows per LDS staging round
 
 // Round-to-nearest-even f32 -> bf16 bits (finite inputs).
 __device__ __forceinline__ uint16_t bf16_bits(float x) {
@@ -78,82 +77,69 @@ __global__ __launch_bounds__(256) void template_sq_f32_kernel(const float *__res
     if (threadIdx.x == 0) out[0] = s[0];
 }
 
-// Toeplitz operands at K = 16 in MFMA lane order, hi and lo parts:
-// entry [dy][kb][part][lane] holds 8 bf16: column n = lane % 32,
-// rows k = 8*(lane/32) + j, value T[dy][16kb + k - n].
-__global__ __launch_bounds__(64) void toeplitz_bf16_kernel(const float *__restrict__ tpl, int tw,
-                                                           int KB, uint16_t *__restrict__ bsw) {
-    const int dy = blockIdx.y, kb = blockIdx.x, lane = threadIdx.x;
-    const int n = lane & 31, g = lane >> 5;
-    uint16_t *hi = bsw + (((size_t)(dy * KB + kb) * 2 + 0) * 64 + lane) * 8;
-    uint16_t *lo = bsw + (((size_t)(dy * KB + kb) * 2 + 1) * 64 + lane) * 8;
-    for (int j = 0; j < 8; j++) {
-        int dx = 16 * kb + 8 * g + j - n;
-        uint16_t h = 0, l = 0;
-        if (dx >= 0 && dx < tw) split_bf16(tpl[dy * tw + dx], h, l);
-        hi[j] = h;
-        lo[j] = l;
-    }
-}
-
-// Window energy sum I^2 in f64 for every output: vertical sums over th rows,
-// then horizontal sums over tw, each evaluated directly (no running
-// differences, so no cancellation).
-// Window energies sum(i^2) of an f32 image in f64, as two sliding-window
-// passes. Horizontal: one workgroup per (row, 1024 outputs); p^2 of the
-// needed columns is staged in LDS, each thread sums its first window and
-// slides over 3 more (4 consecutive outputs).
-constexpr int kHsOut = 1024;  // outputs per horizontal workgroup
-__global__ __launch_bounds__(256) void window_sq_rows_f32_kernel(const float *__restrict__ img,
-                                                                 int W, int tw, int ow,
-                                                                 double *__restrict__ hs) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    double *v = reinterpret_cast<double *>(smem);  // kHsOut + tw - 1
-    const int y = blockIdx.y, x0 = blockIdx.x * kHsOut;
-    const int ncols = min(kHsOut + tw - 1, W - x0);
-    const float *row = img + (size_t)y * W + x0;
-    for (int c = threadIdx.x; c < ncols; c += 256) {
-        const double p = row[c];
-        v[c] = p * p;
-    }
-    __syncthreads();
-    const int xl = threadIdx.x * 4;
-    if (x0 + xl >= ow) return;
-    double a = 0.0;
-    for (int d = 0; d < tw; d++) a += v[xl + d];
-    double *o = hs + (size_t)y * ow + x0 + xl;
-    o[0] = a;
+// Window energies sum(i^2) of an f32 image in f64, one kernel: a workgroup
+// owns kWsqCols output columns by kWsqRows output rows. Each thread holds the
+// vertical sums over th rows of p^2 for its image columns (summed directly
+// once, then slid down a row at a time: the row entering added, the row
+// leaving subtracted, all f64); per output row they go to LDS, and each
+// thread sums the horizontal windows of kWsqRun consecutive outputs (the
+// first directly, the rest slid). p^2 is exact in f64; only the f64
+// energies are written.
+constexpr int kWsqRun = 8;                    // outputs per thread per row
+constexpr int kWsqCols = 256 * kWsqRun;       // output columns per workgroup
+constexpr int kWsqRows = 32;                  // output rows per workgroup
+constexpr int kWsqOwn = (kWsqCols + 128 + 255) / 256;  // image columns per thread (tw <= 129)
+__global__ __launch_bounds__(256) void window_sq_f32_kernel(const float *__restrict__ img, int W,
+                                                            int tw, int th, int ow, int oh,
+                                                            double *__restrict__ wsq) {
+    extern __shared__ __align__(16) double usm[];  // kWsqCols + tw - 1
+    const int x0 = blockIdx.x * kWsqCols, y0 = blockIdx.y * kWsqRows;
+    const int ncols = min(kWsqCols + tw - 1, W - x0);
+    const int yend = min(y0 + kWsqRows, oh);
+    const float *base = img + x0;
+    double u[kWsqOwn];
 #pragma unroll
-    for (int q = 1; q < 4; q++) {
-        if (x0 + xl + q >= ow) break;
-        a += v[xl + q - 1 + tw] - v[xl + q - 1];
-        o[q] = a;
+    for (int k = 0; k < kWsqOwn; k++) {
+        const int c = (int)threadIdx.x + 256 * k;
+        u[k] = 0.0;
+        if (c < ncols)
+            for (int dy = 0; dy < th; dy++) {
+                const double p = base[(size_t)(y0 + dy) * W + c];
+                u[k] += p * p;
+            }
     }
-}
-
-// Vertical: one workgroup per 32 columns x 128 output rows; each thread sums
-// its first window of th horizontal sums and slides over 15 more rows.
-constexpr int kVsCols = 32, kVsRows = 128;
-__global__ __launch_bounds__(256) void window_sq_cols_f32_kernel(const double *__restrict__ hs,
-                                                                 int th, int ow, int oh,
-                                                                 double *__restrict__ ws) {
-    const int x = blockIdx.x * kVsCols + (threadIdx.x & (kVsCols - 1));
-    const int y0 = blockIdx.y * kVsRows + (threadIdx.x / kVsCols) * (kVsRows / 8);
-    if (x >= ow || y0 >= oh) return;
-    const double *col = hs + x;
-    double a = 0.0;
-    for (int r0 = 0; r0 < th; r0 += 8) {
-        double t[8];
+    for (int y = y0; y < yend; y++) {
 #pragma unroll
-        for (int q = 0; q < 8; q++) t[q] = (r0 + q < th) ? col[(size_t)(y0 + r0 + q) * ow] : 0.0;
+        for (int k = 0; k < kWsqOwn; k++) {
+            const int c = (int)threadIdx.x + 256 * k;
+            if (c < ncols) usm[c] = u[k];
+        }
+        __syncthreads();
+        const int xl = (int)threadIdx.x * kWsqRun;
+        if (x0 + xl < ow) {
+            double a = 0.0;
+            for (int d = 0; d < tw; d++) a += usm[xl + d];
+            double *o = wsq + (size_t)y * ow + x0 + xl;
+            o[0] = a;
 #pragma unroll
-        for (int q = 0; q < 8; q++) a += t[q];
-    }
-    ws[(size_t)y0 * ow + x] = a;
-    const int yend = min(y0 + kVsRows / 8, oh);
-    for (int y = y0 + 1; y < yend; y++) {
-        a += col[(size_t)(y - 1 + th) * ow] - col[(size_t)(y - 1) * ow];
-        ws[(size_t)y * ow + x] = a;
+            for (int q = 1; q < kWsqRun; q++) {
+                if (x0 + xl + q >= ow) break;
+                a += usm[xl + q - 1 + tw] - usm[xl + q - 1];
+                o[q] = a;
+            }
+        }
+        __syncthreads();
+        if (y + 1 < yend) {
+#pragma unroll
+            for (int k = 0; k < kWsqOwn; k++) {
+                const int c = (int)threadIdx.x + 256 * k;
+                if (c < ncols) {
+                    const double pin = base[(size_t)(y + th) * W + c];
+                    const double pout = base[(size_t)y * W + c];
+                    u[k] += pin * pin - pout * pout;
+                }
+            }
+        }
     }
 }
 
@@ -172,73 +158,118 @@ __device__ __forceinline__ float finish_score_f32(float c, double sii, double st
     return norm > 0.0 ? (float)(num / norm) : (float)num;
 }
 
-// One wave computes a 32 x (32*kXT16) output strip; RT waves stack vertically.
-// LDS holds the (32*RT + kDyChunk - 1) image rows of one template-row chunk,
-// split into bf16 hi and lo planes.
+// BF16x3 correlation. A workgroup of kBfWaves waves side by side computes 32
+// output rows by kBfWaves * 64 columns (each wave two 32x32 tiles). Template
+// rows are taken in chunks of C: per chunk the workgroup stages the
+// (32 + C - 1) image rows it needs as bf16 hi and lo planes, and the chunk's
+// template rows as bf16 hi and lo rows padded with 32 zeros on the left (and
+// zeros to the right). Per template row dy and K block kb, each lane cuts its
+// Toeplitz slice B[k][n] = T[dy][16 kb + k - n], k = 8 g + 0..7, from the
+// padded row: 8 consecutive bf16 starting at element 32 + 16 kb + 8 g - n,
+// read as 5 dwords and realigned by v_alignbyte (the shift, 0 or 2 bytes, is
+// the lane's column parity). Each slice pair feeds both tiles (6 MFMAs).
+constexpr int kBfWaves = 4;
+constexpr int kBfWaveCols = 64;
+
+__host__ __device__ constexpr int bf_img_cols(int KB) { return kBfWaves * kBfWaveCols + 16 * (KB - 2); }
+// staged image row pitch in bf16: an odd multiple of 8 (16 bytes), so the 16-byte
+// A-operand reads of 8 consecutive rows fall in distinct bank groups
+__host__ __device__ constexpr int bf_img_pitch(int KB) {
+    return ((bf_img_cols(KB) / 8) % 2 == 0) ? bf_img_cols(KB) + 8 : bf_img_cols(KB);
+}
+__host__ __device__ constexpr int bf_tpl_pitch(int KB) { return 16 * KB + 64; }
+__host__ __device__ constexpr size_t bf_lds(int KB, int C) {
+    return (size_t)(32 + C - 1) * bf_img_pitch(KB) * 4 + (size_t)C * bf_tpl_pitch(KB) * 4;
+}
+
+// 8 consecutive bf16 of a padded template row from element e (e >= 0):
+// 5 dwords from e / 2, realigned by (e & 1) * 2 bytes.
+__device__ __forceinline__ v8bf bf_slice(const uint16_t *row, int e) {
+    const uint32_t *d = reinterpret_cast<const uint32_t *>(row) + (e >> 1);
+    const uint32_t sh = (uint32_t)(e & 1) * 2u;
+    uint32_t w[5];
+#pragma unroll
+    for (int i = 0; i < 5; i++) w[i] = d[i];
+    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+    v4u r;
+#pragma unroll
+    for (int i = 0; i < 4; i++) r[i] = __builtin_amdgcn_alignbyte(w[i + 1], w[i], sh);
+    return __builtin_bit_cast(v8bf, r);
+}
+
 template <int KB, int METHOD>
-__global__ __launch_bounds__(128) void match_bf16x3_mfma_kernel(
-    const float *__restrict__ img, int W, int H, const v8bf *__restrict__ bsw,
+__global__ __launch_bounds__(256) void match_bf16x3_mfma_kernel(
+    const float *__restrict__ img, int W, int H, const float *__restrict__ tpl,
     const double *__restrict__ wsq, const double *__restrict__ tsq, float *__restrict__ out,
-    int ow, int oh, int tw, int th, int pitch) {
+    int ow, int oh, int tw, int th, int C) {
+    constexpr int BW = bf_img_cols(KB), PI = bf_img_pitch(KB), PT = bf_tpl_pitch(KB);
     extern __shared__ __align__(16) unsigned char smem[];
-    constexpr int BW = 16 * (2 * kXT16 + KB - 1);  // staged columns
-    const int RT = blockDim.x >> 6;
-    const int rows = 32 * RT + kDyChunk - 1;
-    uint16_t *hi_s = reinterpret_cast<uint16_t *>(smem);
-    uint16_t *lo_s = hi_s + (size_t)rows * pitch;
+    const int rows = 32 + C - 1;
+    uint16_t *ih = reinterpret_cast<uint16_t *>(smem);
+    uint16_t *il = ih + (size_t)rows * PI;
+    uint16_t *th_s = il + (size_t)rows * PI;
+    uint16_t *tl_s = th_s + (size_t)C * PT;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int n = lane & 31, g = lane >> 5;
-    const int x0 = blockIdx.x * (32 * kXT16), y0 = blockIdx.y * 32 * RT;
+    const int x0 = blockIdx.x * (kBfWaves * kBfWaveCols), y0 = blockIdx.y * 32;
+    const bool vec = (W & 3) == 0;
 
-    v16f acc[kXT16];
+    v16f acc[2];
 #pragma unroll
-    for (int t = 0; t < kXT16; t++)
+    for (int t = 0; t < 2; t++)
 #pragma unroll
         for (int r = 0; r < 16; r++) acc[t][r] = 0.0f;
 
-    const v8bf *bp = bsw + lane;
-    for (int c0 = 0; c0 < th; c0 += kDyChunk) {
-        __syncthreads();
-        for (int base = threadIdx.x; base < rows * BW; base += 8 * (int)blockDim.x) {
-            float v[8];
+    for (int c0 = 0; c0 < th; c0 += C) {
+        __syncthreads();  // the previous chunk's operands are consumed
+        // image rows y0 + c0 + [0, rows), columns x0 + [0, BW), 4 per thread
+        for (int i = threadIdx.x; i < rows * (BW / 4); i += 256) {
+            const int r = i / (BW / 4), c = 4 * (i - r * (BW / 4));
+            const int gy = min(y0 + c0 + r, H - 1), gx = x0 + c;
+            const float *src = img + (size_t)gy * W;
+            float v[4];
+            if (vec && gx + 3 < W) {
+                const float4 q = *reinterpret_cast<const float4 *>(src + gx);
+                v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+            } else {
 #pragma unroll
-            for (int b = 0; b < 8; b++) {
-                const int i = base + b * (int)blockDim.x;
-                const int r = i / BW, c = i - r * BW;
-                const int gy = min(y0 + c0 + r, H - 1), gx = min(x0 + c, W - 1);
-                v[b] = img[(size_t)gy * W + gx];
+                for (int b = 0; b < 4; b++) v[b] = src[min(gx + b, W - 1)];
             }
+            uint16_t h4[4], l4[4];
 #pragma unroll
-            for (int b = 0; b < 8; b++) {
-                const int i = base + b * (int)blockDim.x;
-                if (i < rows * BW) {
-                    const int r = i / BW, c = i - r * BW;
-                    uint16_t h, l;
-                    split_bf16(v[b], h, l);
-                    hi_s[r * pitch + c] = h;
-                    lo_s[r * pitch + c] = l;
-                }
-            }
+            for (int b = 0; b < 4; b++) split_bf16(v[b], h4[b], l4[b]);
+            *reinterpret_cast<uint2 *>(ih + r * PI + c) =
+                make_uint2(h4[0] | ((uint32_t)h4[1] << 16), h4[2] | ((uint32_t)h4[3] << 16));
+            *reinterpret_cast<uint2 *>(il + r * PI + c) =
+                make_uint2(l4[0] | ((uint32_t)l4[1] << 16), l4[2] | ((uint32_t)l4[3] << 16));
+        }
+        // template rows c0 + [0, C): 32 zeros, the row, zeros to PT
+        for (int i = threadIdx.x; i < C * PT; i += 256) {
+            const int r = i / PT, e = i - r * PT, dx = e - 32;
+            uint16_t h = 0, l = 0;
+            if (c0 + r < th && dx >= 0 && dx < tw) split_bf16(tpl[(size_t)(c0 + r) * tw + dx], h, l);
+            th_s[i] = h;
+            tl_s[i] = l;
         }
         __syncthreads();
-        const int dy_end = min(kDyChunk, th - c0);
-        const uint16_t *ah = hi_s + (wave * 32 + n) * pitch + 8 * g;
-        const uint16_t *al = lo_s + (wave * 32 + n) * pitch + 8 * g;
-        for (int d = 0; d < dy_end; d++) {
-            const int dy = c0 + d;
+        const int dend = min(C, th - c0);
+        const uint16_t *ah = ih + n * PI + kBfWaveCols * wave + 8 * g;
+        const uint16_t *al = il + n * PI + kBfWaveCols * wave + 8 * g;
+        const int e0 = 32 + 8 * g - n;
+        for (int d = 0; d < dend; d++) {
             v8bf bh[KB], bl[KB];
 #pragma unroll
             for (int kb = 0; kb < KB; kb++) {
-                bh[kb] = bp[((size_t)(dy * KB + kb) * 2 + 0) * 64];
-                bl[kb] = bp[((size_t)(dy * KB + kb) * 2 + 1) * 64];
+                bh[kb] = bf_slice(th_s + d * PT, e0 + 16 * kb);
+                bl[kb] = bf_slice(tl_s + d * PT, e0 + 16 * kb);
             }
 #pragma unroll
-            for (int s = 0; s < 2 * kXT16 + KB - 1; s++) {
-                v8bf a_h = *reinterpret_cast<const v8bf *>(ah + d * pitch + 16 * s);
-                v8bf a_l = *reinterpret_cast<const v8bf *>(al + d * pitch + 16 * s);
+            for (int sblk = 0; sblk < KB + 2; sblk++) {
+                const v8bf a_h = *reinterpret_cast<const v8bf *>(ah + d * PI + 16 * sblk);
+                const v8bf a_l = *reinterpret_cast<const v8bf *>(al + d * PI + 16 * sblk);
 #pragma unroll
-                for (int t = 0; t < kXT16; t++) {
-                    const int kb = s - 2 * t;
+                for (int t = 0; t < 2; t++) {
+                    const int kb = sblk - 2 * t;
                     if (kb >= 0 && kb < KB) {
                         acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_l, bh[kb], acc[t], 0, 0, 0);
                         acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_h, bl[kb], acc[t], 0, 0, 0);
@@ -252,37 +283,41 @@ __global__ __launch_bounds__(128) void match_bf16x3_mfma_kernel(
     const double stt = tsq[0];
     const long long nn = (long long)tw * th;
 #pragma unroll
-    for (int t = 0; t < kXT16; t++) {
-        const int x = x0 + 32 * t + n;
+    for (int t = 0; t < 2; t++) {
+        const int x = x0 + kBfWaveCols * wave + 32 * t + n;
         double sq[16];
 #pragma unroll
         for (int r = 0; r < 16; r++) {
-            const int y = y0 + wave * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
+            const int y = y0 + 8 * (r >> 2) + 4 * g + (r & 3);
             sq[r] = (METHOD == IPH_CCORR) ? 0.0 : wsq[(size_t)min(y, oh - 1) * ow + min(x, ow - 1)];
         }
 #pragma unroll
         for (int r = 0; r < 16; r++) {
-            const int y = y0 + wave * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
+            const int y = y0 + 8 * (r >> 2) + 4 * g + (r & 3);
             float v = finish_score_f32<METHOD>(acc[t][r], sq[r], stt, nn);
             if (x < ow && y < oh) out[(size_t)y * ow + x] = v;
         }
     }
 }
 
+// Template rows per chunk: the largest of 32, 16, 8 whose LDS lets two
+// workgroups share a CU.
+int bf_chunk(int KB) {
+    for (int C = 32; C >= 8; C /= 2)
+        if (bf_lds(KB, C) <= 80 * 1024) return C;
+    return 4;
+}
+
 template <int KB>
-iph_status launch_bf16_kb(const float *img, int W, int H, const uint16_t *bsw, const double *wsq,
+iph_status launch_bf16_kb(const float *img, int W, int H, const float *tpl, const double *wsq,
                           const double *tsq, float *out, int ow, int oh, int tw, int th,
                           int method, hipStream_t s) {
-    constexpr int BW = 16 * (2 * kXT16 + KB - 1);
-    int pitch = ((BW / 8) % 2 == 0) ? BW + 8 : BW;  // odd multiple of 16 bytes
-    int RT = 2;
-    while (RT > 1 && (size_t)(32 * RT + kDyChunk - 1) * pitch * 4 > 64 * 1024) RT /= 2;
-    size_t lds = (size_t)(32 * RT + kDyChunk - 1) * pitch * 4;
-    dim3 grid((ow + 32 * kXT16 - 1) / (32 * kXT16), (oh + 32 * RT - 1) / (32 * RT));
-    const v8bf *b = reinterpret_cast<const v8bf *>(bsw);
+    const int C = bf_chunk(KB);
+    const size_t lds = bf_lds(KB, C);
+    dim3 grid((ow + kBfWaves * kBfWaveCols - 1) / (kBfWaves * kBfWaveCols), (oh + 31) / 32);
 #define L(M)                                                                                 \
-    match_bf16x3_mfma_kernel<KB, M><<<grid, 64 * RT, lds, s>>>(img, W, H, b, wsq, tsq, out,   \
-                                                               ow, oh, tw, th, pitch)
+    match_bf16x3_mfma_kernel<KB, M><<<grid, 64 * kBfWaves, lds, s>>>(img, W, H, tpl, wsq, tsq, \
+                                                                      out, ow, oh, tw, th, C)
     switch (method) {
     case IPH_SSE: L(IPH_SSE); break;
     case IPH_SSE_NORMED: L(IPH_SSE_NORMED); break;
@@ -884,31 +919,24 @@ iph_status launch_bf16x3_match(const float *img, uint32_t w, uint32_t h, const f
     const int KB = blocks16_for((int)tw);
     Workspace wsp(ws, ws_bytes);
     double *tsq = wsp.take<double>(1);
-    uint16_t *bsw = wsp.take<uint16_t>((size_t)th * KB * 2 * 64 * 8);
-    double *wsq = wsp.take<double>((size_t)(w - tw + 1) * (h - th + 1));
-    double *hsq = method == IPH_CCORR ? nullptr : wsp.take<double>((size_t)(w - tw + 1) * h);
-    if (!tsq || !bsw || !wsq || (method != IPH_CCORR && !hsq)) {
+    double *wsq = method == IPH_CCORR ? nullptr : wsp.take<double>((size_t)(w - tw + 1) * (h - th + 1));
+    if (!tsq || (method != IPH_CCORR && !wsq)) {
         set_error("workspace too small for the BF16x3 MFMA match_template");
         return IPH_EINVAL;
     }
     int ow = (int)(w - tw + 1), oh = (int)(h - th + 1);
     template_sq_f32_kernel<<<1, 256, 0, s>>>(tpl, (int)(tw * th), tsq);
     IPH_TRY(launch_status("template_sq_f32_kernel"));
-    toeplitz_bf16_kernel<<<dim3(KB, th), 64, 0, s>>>(tpl, (int)tw, KB, bsw);
-    IPH_TRY(launch_status("toeplitz_bf16_kernel"));
     if (method != IPH_CCORR) {
-        dim3 hg((ow + kHsOut - 1) / kHsOut, h);
-        size_t lds = (size_t)(kHsOut + tw - 1) * sizeof(double);
-        window_sq_rows_f32_kernel<<<hg, 256, lds, s>>>(img, (int)w, (int)tw, ow, hsq);
-        IPH_TRY(launch_status("window_sq_rows_f32_kernel"));
-        dim3 vg((ow + kVsCols - 1) / kVsCols, (oh + kVsRows - 1) / kVsRows);
-        window_sq_cols_f32_kernel<<<vg, 256, 0, s>>>(hsq, (int)th, ow, oh, wsq);
-        IPH_TRY(launch_status("window_sq_cols_f32_kernel"));
+        dim3 eg((ow + kWsqCols - 1) / kWsqCols, (oh + kWsqRows - 1) / kWsqRows);
+        size_t lds = (size_t)(kWsqCols + tw - 1) * sizeof(double);
+        window_sq_f32_kernel<<<eg, 256, lds, s>>>(img, (int)w, (int)tw, (int)th, ow, oh, wsq);
+        IPH_TRY(launch_status("window_sq_f32_kernel"));
     }
     switch (KB) {
 #define C(K)                                                                                 \
     case K:                                                                                  \
-        return launch_bf16_kb<K>(img, w, h, bsw, wsq, tsq, out, ow, oh, tw, th, method, s);
+        return launch_bf16_kb<K>(img, w, h, tpl, wsq, tsq, out, ow, oh, tw, th, method, s);
         C(2) C(3) C(4) C(5) C(6) C(7) C(8) C(9) C(10)
 #undef C
     default:
@@ -923,9 +951,7 @@ size_t mfma_match_workspace(int32_t p_type, uint32_t w, uint32_t h, uint32_t tw,
     if (p_type == IPH_F32) {
         WorkspacePlan wp;
         wp.take<double>(1);
-        wp.take<uint16_t>((size_t)th * blocks16_for((int)tw) * 2 * 64 * 8);
-        wp.take<double>((size_t)(w - tw + 1) * (h - th + 1));
-        if (method != IPH_CCORR) wp.take<double>((size_t)(w - tw + 1) * h);
+        if (method != IPH_CCORR) wp.take<double>((size_t)(w - tw + 1) * (h - th + 1));
         return wp.used;
     }
     return 256;  // the fused int8 kernel needs no workspace
