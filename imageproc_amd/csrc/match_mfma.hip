@@ -85,9 +85,9 @@ __global__ __launch_bounds__(256) void template_sq_f32_kernel(const float *__res
 // thread sums the horizontal windows of kWsqRun consecutive outputs (the
 // first directly, the rest slid). p^2 is exact in f64; only the f64
 // energies are written.
-constexpr int kWsqRun = 8;                    // outputs per thread per row
+constexpr int kWsqRun = 4;                    // outputs per thread per row
 constexpr int kWsqCols = 256 * kWsqRun;       // output columns per workgroup
-constexpr int kWsqRows = 32;                  // output rows per workgroup
+constexpr int kWsqRows = 16;                  // output rows per workgroup
 constexpr int kWsqOwn = (kWsqCols + 128 + 255) / 256;  // image columns per thread (tw <= 129)
 __global__ __launch_bounds__(256) void window_sq_f32_kernel(const float *__restrict__ img, int W,
                                                             int tw, int th, int ow, int oh,
@@ -102,11 +102,22 @@ __global__ __launch_bounds__(256) void window_sq_f32_kernel(const float *__restr
     for (int k = 0; k < kWsqOwn; k++) {
         const int c = (int)threadIdx.x + 256 * k;
         u[k] = 0.0;
-        if (c < ncols)
-            for (int dy = 0; dy < th; dy++) {
-                const double p = base[(size_t)(y0 + dy) * W + c];
+        if (c < ncols) {
+            // eight rows' loads in flight, summed in row order
+            const float *col = base + (size_t)y0 * W + c;
+            int dy = 0;
+            for (; dy + 8 <= th; dy += 8) {
+                float v[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) v[q] = col[(size_t)(dy + q) * W];
+#pragma unroll
+                for (int q = 0; q < 8; q++) u[k] += (double)v[q] * (double)v[q];
+            }
+            for (; dy < th; dy++) {
+                const double p = col[(size_t)dy * W];
                 u[k] += p * p;
             }
+        }
     }
     for (int y = y0; y < yend; y++) {
 #pragma unroll
