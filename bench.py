@@ -749,6 +749,18 @@ def bench_proposal(n_images: int = 12, per_image: int = 4, cpu: bool = False):
     with np.load(WEIGHTS, allow_pickle=False) as d:
         margin = float(d["margin"])
         trained = int(d["train_pairs"])
+        held_out = None
+        if "eval_pairs" in d.files:
+            held_out = {
+                "pairs": int(d["eval_pairs"]), "calibration_pairs": int(d["cal_pairs"]),
+                "agreement_with_fallback": float(d["eval_agree_with_fallback"]),
+                "network_only_agreement": float(d["eval_net_agree"]),
+                "coarse_only_agreement": float(d["eval_coarse_agree"]),
+                "fallback_rate": float(d["eval_fallback_rate"]),
+                "source": "recorded in proposal_weights.npz by python -m "
+                          "imageproc_amd.proposal_train --calibrate-only: margin calibrated on "
+                          "one seeded set of 8192x8192 pairs, agreement measured on a separate "
+                          "held-out set (not re-measured in this run)"}
     net = ProposalNet.load()
     srch = ProposalSearch(net, CCN, margin=margin)
     top1 = ProposalSearch(net, CCN, n_top=1, margin=-1e30, min_prob=-1.0)
@@ -809,6 +821,7 @@ def bench_proposal(n_images: int = 12, per_image: int = 4, cpu: bool = False):
         "top1_agreement_network": sum(r[3] == r[0] for r in rec) / n,
         "top1_agreement_coarse_only": sum(r[4] == r[0] for r in rec) / n,
         "fallback_margin": margin,
+        "held_out_evaluation": held_out,
         "cpu_baseline": cpu_line,
     }
 

@@ -165,9 +165,14 @@ class ProposalSearch:
     cached per image / template shape)."""
 
     def __init__(self, net: ProposalNet | None, method=MatchTemplateMethod.CrossCorrelationNormalized,
-                 k: int = 64, n_top: int = 16, margin: float = 0.0, min_prob: float = 0.0,
+                 k: int = 64, n_top: int = 16, margin: float | None = None, min_prob: float = 0.0,
                  sep: int = 4):
+        """margin None takes the margin calibrated with the shipped weights
+        (proposal_train --calibrate-only): below it the full map decides."""
         self.net, self.method = net, MatchTemplateMethod(method)
+        if margin is None:
+            with np.load(WEIGHTS, allow_pickle=False) as d:
+                margin = float(d["margin"]) if "margin" in d.files else 0.0
         if self.method not in (MatchTemplateMethod.CrossCorrelation,
                                MatchTemplateMethod.CrossCorrelationNormalized):
             raise ValueError("the proposal search serves the correlation methods")

@@ -18,6 +18,12 @@ validation pairs of 8192x8192 images: the smallest exact-score margin such
 that the searches that skip the full map agree with it on at least 99.5 % of
 the pairs (and the rest fall back). Weights and thresholds go to
 proposal_weights.npz next to this file.
+
+    python -m imageproc_amd.proposal_train --calibrate-only [--cal-pairs 1200] [--eval-pairs 1200]
+
+keeps the saved network and recalibrates its margin on one seeded set of
+8192x8192 pairs (margin_rule), then evaluates the search on a separate
+held-out set, recording both in the weights file.
 """
 from __future__ import annotations
 
@@ -123,7 +129,9 @@ def evaluate(net: ProposalNet | None, n: int, side: int, seed: int, margin: floa
     srch = ProposalSearch(net, CCN, margin=-1e30, min_prob=-1.0)
     out = []
     stream = pair_stream(seed, side, per_image=4)
-    for _ in range(n):
+    for i in range(n):
+        if i and i % 200 == 0:
+            print(f"[evaluate] {i}/{n} pairs", flush=True)
         img, tpl, _ = next(stream)
         gx, gy, _ = srch.full_map(img, tpl)
         nx, ny, best, second, p = srch.search_raw(img, tpl, use_net=net is not None)
@@ -145,14 +153,70 @@ def calibrate(records, target: float = 0.995):
     return float("inf"), 1.0, 1.0
 
 
+def margin_rule(records, floor_q: float = 0.01):
+    """Fallback margin from calibration records: just above the largest
+    best - second margin at which the network search disagreed with the
+    judge (so every such pair would fall back), and at least the floor_q
+    quantile of all margins (so the most ambiguous pairs fall back even when
+    the calibration set has no disagreement)."""
+    ms = np.array([r[3] for r in records], np.float64)
+    bad = [r[3] for r in records if r[0] != r[2]]
+    m = float(np.quantile(ms, floor_q)) if len(ms) else 0.0
+    if bad:
+        m = max(m, float(np.nextafter(np.float32(max(bad)), np.float32(np.inf))))
+    return m
+
+
+def summarise(records, margin: float):
+    """Agreement with the judge of the network search with fallback, of the
+    network search alone, of the coarse-only search, and the fallback rate."""
+    n = len(records)
+    fb = [r[3] < margin for r in records]
+    with_fb = sum(1 for r, f in zip(records, fb) if f or r[0] == r[2]) / n
+    return {"pairs": n, "agreement_with_fallback": with_fb,
+            "net_only_agreement": sum(1 for r in records if r[0] == r[2]) / n,
+            "coarse_only_agreement": sum(1 for r in records if r[1] == r[2]) / n,
+            "fallback_rate": sum(fb) / n}
+
+
+def calibrate_main(args) -> None:
+    """Calibrates the fallback margin of the saved network on one seeded set
+    of 8192x8192 pairs and evaluates the search on a separate held-out set,
+    then records both in the weights file."""
+    net = ProposalNet.load(args.out)
+    cal = evaluate(net, args.cal_pairs, args.val_side, seed=90001)
+    margin = margin_rule(cal)
+    ev = evaluate(net, args.eval_pairs, args.val_side, seed=777013)
+    sc, se = summarise(cal, margin), summarise(ev, margin)
+    print(f"[calibrate] margin {margin:.6g}; calibration {sc}; held-out {se}", flush=True)
+    with np.load(args.out, allow_pickle=False) as d:
+        arrays = {k: d[k] for k in d.files}
+    arrays["margin"] = np.float64(margin)
+    arrays["cal_pairs"] = np.int64(sc["pairs"])
+    arrays["eval_pairs"] = np.int64(se["pairs"])
+    arrays["eval_agree_with_fallback"] = np.float64(se["agreement_with_fallback"])
+    arrays["eval_net_agree"] = np.float64(se["net_only_agreement"])
+    arrays["eval_coarse_agree"] = np.float64(se["coarse_only_agreement"])
+    arrays["eval_fallback_rate"] = np.float64(se["fallback_rate"])
+    np.savez(args.out, **arrays)
+    print(f"[calibrate] wrote {args.out}", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--calibrate-only", action="store_true",
+                    help="keep the saved network; calibrate its margin and evaluate it")
+    ap.add_argument("--cal-pairs", type=int, default=1200)
+    ap.add_argument("--eval-pairs", type=int, default=1200)
     ap.add_argument("--pairs", type=int, default=200_000)
     ap.add_argument("--side", type=int, default=2048)
     ap.add_argument("--val-pairs", type=int, default=200)
     ap.add_argument("--val-side", type=int, default=8192)
     ap.add_argument("--out", default=WEIGHTS)
     args = ap.parse_args()
+    if args.calibrate_only:
+        calibrate_main(args)
+        return
     net = train(args.pairs, args.side)
     print("[train] calibrating on validation pairs", flush=True)
     rec = evaluate(net, args.val_pairs, args.val_side, seed=90001)

@@ -50,3 +50,37 @@ def test_fallback_returns_full_map_answer(pairs):
     for (img, tpl, _), j in list(zip(ps, judge))[:4]:
         x, y, v, fell_back = srch.search(img, tpl)
         assert fell_back and (x, y, v) == j
+
+
+def test_search_agreement_on_held_out_8192_pairs():
+    """8192x8192 pairs from a seed used neither in training nor in the margin
+    calibration: the search with the calibrated fallback agrees with the full
+    map on at least 99.5 % of them, and the network's ranking alone agrees at
+    least as often as ranking by coarse score alone."""
+    stream = pair_stream(555001, 8192, per_image=4)
+    srch = ProposalSearch(ProposalNet.load(), CCN)
+    raw = ProposalSearch(ProposalNet.load(), CCN, margin=-1e30, min_prob=-1.0)
+    n, agree, net_only, coarse_only = 240, 0, 0, 0
+    for _ in range(n):
+        img, tpl, _ = next(stream)
+        j = srch.full_map(img, tpl)[:2]
+        agree += srch.search(img, tpl)[:2] == j
+        net_only += raw.search(img, tpl)[:2] == j
+        coarse_only += raw.search(img, tpl, use_net=False)[:2] == j
+    assert agree >= 0.995 * n, (agree, n)
+    assert net_only >= coarse_only, (net_only, coarse_only)
+
+
+def test_low_margin_pair_takes_the_full_map():
+    """A template that occurs twice in the image (an unperturbed crop pasted
+    at a second place far away) leaves no margin between the best window and
+    the runner-up: the calibrated search falls back to the full map and
+    returns its answer, the first of the two places in row-major order."""
+    img = next(pair_stream(555002, 2048, per_image=1))[0].clone()
+    tpl = img[700:748, 900:948].clone().contiguous()
+    img[1500:1548, 300:348] = tpl
+    srch = ProposalSearch(ProposalNet.load(), CCN)
+    x, y, v, fell_back = srch.search(img, tpl)
+    assert fell_back
+    assert (x, y, v) == srch.full_map(img, tpl)
+    assert (x, y) == (900, 700)
