@@ -29,6 +29,8 @@ batch's Gpix/s. The same JSON line carries:
     full-map search vs the proposal search (trained network + exact windows,
     full-map fallback on low margin): ms per search, speedup and argmax
     agreement, and what the network adds over ranking by coarse score alone;
+  * filters: box_filter, a 5x5 f32 filter_clamped and the f32 separable_filter
+    on one 8192x8192 image, each with its HBM roofline and a CPU baseline;
   * cpu_baseline: the oracle's blur (the reference's loop restated in C,
     single-threaded like the reference's gaussian_blur_f32) on host cores;
     the match, precision, batch and proposal legs carry their own, on up to
@@ -460,6 +462,91 @@ def bench_match(steps: int, cpu: bool = False):
 CONSUMER_SIDE = 8192
 
 
+def bench_filters(steps: int, cpu: bool = True):
+    """The other filters of the path on one 8192x8192 image resident in HBM:
+    box_filter (u8, radii 5, 5), filter_clamped with a 5x5 f32 kernel of
+    N(0,1) weights normalised to sum 1 (u8 -> u8, the batch workload's
+    kernel), and separable_filter with the sigma 2.0 Gaussian taps on a
+    Luma<f32> image (the reference's f32 path). Algorithmic bytes: one read
+    and one write of the image per call (2 B per pixel for u8, 8 B for f32);
+    the roofline divides them by the call's time against the 8 TB/s peak.
+    With cpu, each is timed with the oracle (the reference's loops in C) on
+    the image's top-left 2048x2048 quarter: box_filter and separable_filter
+    on one core (the reference's are single-threaded), filter_clamped as
+    filter_clamped_parallel over the host threads."""
+    import torch
+
+    import oracle as O
+    from imageproc_amd import device as dev
+    from imageproc_amd.kernel import Kernel
+
+    side = CONSUMER_SIDE
+    px = side * side
+    u8 = torch.from_numpy(uniform_u8(404, (side, side))).cuda()
+    f32 = (u8.float() / 255.0).contiguous()
+    rng = np.random.default_rng(4)
+    while True:
+        kw = rng.standard_normal(25).astype(np.float32)
+        if abs(float(kw.sum())) >= 0.25:
+            break
+    kw = (kw / kw.sum()).astype(np.float32)
+    taps = O.gaussian_kernel_f32(2.0)
+    res = {"workload": f"{side}x{side} image (u8 uniform noise; f32 = u8 / 255)"}
+
+    def leg(name, fn, out, nbytes, extra):
+        times, _ = timed(lambda: fn(out), steps, 2)
+        ms = float(np.median(times))
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        res[name] = {"ms": round(ms, 4), "GB/s": round(gbs, 1),
+                     "roofline": {"bound": "hbm", "achieved": round(gbs, 1),
+                                  "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": round(gbs / HBM_PEAK_GBS, 4)}, **extra}
+
+    box_out = torch.empty_like(u8)
+    leg("box_filter_r5", lambda o: dev.box_filter(u8, 5, 5, out=o), box_out, 2.0 * px, {})
+    k5 = Kernel(kw, 5, 5)
+    f2_out = torch.empty_like(u8)
+    leg("filter_clamped_5x5_f32", lambda o: dev.filter_clamped(u8, k5, out=o), f2_out, 2.0 * px,
+        {})
+    sep_out = torch.empty_like(f32)
+    leg("separable_filter_f32_sigma2", lambda o: dev.separable_filter(f32, taps, taps, out=o),
+        sep_out, 8.0 * px, {"taps": int(taps.size)})
+    quarter = 2048
+    a8 = u8[:quarter, :quarter].cpu().numpy()
+    af = f32[:quarter, :quarter].cpu().numpy()
+    # parity spot checks: every output of the quarter against the oracle on the quarter
+    # (clamped edges differ only along the quarter's right and bottom border: compare inside)
+    inner = np.s_[: quarter - 16, : quarter - 16]
+    checks = {
+        "box_filter_r5": np.array_equal(box_out[:quarter, :quarter].cpu().numpy()[inner],
+                                        O.box_filter(a8, 5, 5)[inner]),
+        "filter_clamped_5x5_f32": np.array_equal(
+            f2_out[:quarter, :quarter].cpu().numpy()[inner],
+            O.filter_clamped(a8, kw, 5, 5, np.uint8)[inner]),
+        "separable_filter_f32_sigma2": np.array_equal(
+            sep_out[:quarter, :quarter].cpu().numpy()[inner].view(np.uint32),
+            O.separable_filter(af, taps, taps)[inner].view(np.uint32)),
+    }
+    res["parity_inner_quarter_bit_exact"] = checks
+    if cpu:
+        nt = cpu_threads()
+        for name, fn, threads in [
+                ("box_filter_r5", lambda: O.box_filter(a8, 5, 5), 1),
+                ("filter_clamped_5x5_f32", lambda: O.filter_clamped(a8, kw, 5, 5, np.uint8,
+                                                                    nthreads=nt), nt),
+                ("separable_filter_f32_sigma2", lambda: O.separable_filter(af, taps, taps), 1)]:
+            t0 = time.perf_counter()
+            fn()
+            dt = time.perf_counter() - t0
+            mpx = quarter * quarter / dt / 1e6
+            res[name]["cpu_baseline"] = {
+                "Mpix/s": round(mpx, 1), "cores": threads, "kind": "port",
+                "sample": f"2048x2048 quarter of the same image, oracle/oracle.c, one call on "
+                          f"{cpu_desc(threads)}",
+                "gpu_over_cpu": round(px / (res[name]["ms"] * 1e-3) / 1e6 / mpx, 1)}
+    return res
+
+
 def bench_consumers(steps: int, cpu: bool = True):
     """integral_image (u32 table), adaptive_threshold (radius 41), canny
     (thresholds 250, 300), Sobel gradients (u16) and sharpen_gaussian (0.7,
@@ -834,12 +921,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--leg", default="all",
                     choices=("all", "blur", "copy", "match", "precision", "proposal", "consumers",
-                             "batch"),
+                             "batch", "filters"),
                     help="blur: only the blur batch (for counter runs); copy: only a plain "
                          "copy of the same size (counter calibration); match: only the "
                          "match_template leg; proposal: only the learned-proposal search "
                          "against the full map; consumers: integral image, "
-                         "adaptive threshold and canny; batch: the ragged batch")
+                         "adaptive threshold and canny; batch: the ragged batch; filters: "
+                         "box_filter, 5x5 filter_clamped and f32 separable_filter")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X and has no "
@@ -865,6 +953,10 @@ def main():
     if args.leg == "proposal":
         print(json.dumps({"leg": "proposal", **bench_proposal()}), flush=True)
         return
+    if args.leg == "filters":
+        print(json.dumps({"leg": "filters", **bench_filters(args.steps, cpu=not args.no_cpu)}),
+              flush=True)
+        return
     if args.leg == "batch":
         print(json.dumps({"leg": "batch", **bench_batch(4, cpu=not args.no_cpu)}), flush=True)
         return
@@ -886,6 +978,7 @@ def main():
     batch = bench_batch(4, cpu=not args.no_cpu)
     proposal = bench_proposal(cpu=not args.no_cpu)
     consumers = bench_consumers(max(5, args.steps // 2), cpu=not args.no_cpu)
+    filters = bench_filters(max(5, args.steps // 2), cpu=not args.no_cpu)
     cpu = None if args.no_cpu else cpu_blur_baseline()
     line = {
         "metric": "blur Gpix/s & %HBM peak; match_template ms/image at equal argmax vs plain GPU",
@@ -918,6 +1011,7 @@ def main():
         "batch": batch,
         "learned_proposal": proposal,
         "consumers": consumers,
+        "filters": filters,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -249,6 +249,62 @@ __global__ __launch_bounds__(256) void filter2d_tiled_kernel(const P *__restrict
     }
 }
 
+// filter_clamped<u8, f32, u8> with a KW x KH kernel known at compile time
+// (3 or 5 per side): one 64 x 32 output tile per workgroup, the clamp-to-edge
+// window staged once as f32 (rows 68 floats apart), each thread eight
+// consecutive outputs of one row from three 16-byte reads per kernel row,
+// taps in scalar registers, each output's chain in the reference's order
+// (ky outer, kx inner, separately rounded multiply then add); eight threads
+// write a row's 64 outputs as one contiguous run.
+constexpr int kF2STW = 64, kF2STH = 32, kF2SP = 68;
+
+template <int KW, int KH>
+__global__ __launch_bounds__(256) void filter2d_small_kernel(const uint8_t *__restrict__ src,
+                                                             uint8_t *__restrict__ dst, int W,
+                                                             int H, Taps<float> k) {
+    static_assert(kF2STW + KW - 1 <= kF2SP, "the window row fits the pitch");
+    __shared__ __align__(16) float win[(kF2STH + KH - 1) * kF2SP];
+    const int x0 = blockIdx.x * kF2STW, y0 = blockIdx.y * kF2STH;
+    const int ox = x0 - KW / 2, oy = y0 - KH / 2;  // filter/mod.rs:50,53
+    constexpr int R = kF2STH + KH - 1;
+    for (int i = threadIdx.x; i < R * kF2SP; i += 256) {
+        const int r = i / kF2SP, c = i - r * kF2SP;
+        win[i] = (float)src[(size_t)clampi(oy + r, 0, H - 1) * W + clampi(ox + c, 0, W - 1)];
+    }
+    __syncthreads();
+    const int cg = threadIdx.x & 7, r = threadIdx.x >> 3;
+    float acc[8];
+#pragma unroll
+    for (int o = 0; o < 8; o++) acc[o] = 0.0f;
+#pragma unroll
+    for (int ky = 0; ky < KH; ky++) {
+        const float4 *row = reinterpret_cast<const float4 *>(win + (r + ky) * kF2SP + 8 * cg);
+        float v[12];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const float4 f = row[q];
+            v[4 * q] = f.x; v[4 * q + 1] = f.y; v[4 * q + 2] = f.z; v[4 * q + 3] = f.w;
+        }
+#pragma unroll
+        for (int o = 0; o < 8; o++)
+#pragma unroll
+            for (int kx = 0; kx < KW; kx++) acc[o] = mac(acc[o], v[o + kx], k.w[ky * KW + kx]);
+    }
+    const int gy = y0 + r, gx = x0 + 8 * cg;
+    if (gy >= H || gx >= W) return;
+    uint8_t b[8];
+#pragma unroll
+    for (int o = 0; o < 8; o++) b[o] = ClampTo<uint8_t>::from(acc[o]);
+    uint8_t *o8 = dst + (size_t)gy * W + gx;
+    if (gx + 8 <= W && ((reinterpret_cast<uintptr_t>(o8) & 7) == 0)) {
+        *reinterpret_cast<uint2 *>(o8) =
+            make_uint2(b[0] | (b[1] << 8) | (b[2] << 16) | ((uint32_t)b[3] << 24),
+                       b[4] | (b[5] << 8) | (b[6] << 16) | ((uint32_t)b[7] << 24));
+    } else {
+        for (int o = 0; o < 8 && gx + o < W; o++) o8[o] = b[o];
+    }
+}
+
 template <typename P, typename K, typename S>
 __global__ __launch_bounds__(256) void filter2d_naive_kernel(const P *__restrict__ src,
                                                              S *__restrict__ dst, int W, int H,
@@ -551,6 +607,21 @@ iph_status filter2d_typed(const void *src, uint32_t w, uint32_t h, const void *k
                           hipStream_t s) {
     size_t taps = (size_t)kw * kh;
     size_t lds = sizeof(P) * ((size_t)kF2TH + kh - 1) * ((size_t)kF2TW + kw - 1);
+    if constexpr (std::is_same<P, uint8_t>::value && std::is_same<K, float>::value &&
+                  std::is_same<S, uint8_t>::value) {
+        if ((kw == 3 || kw == 5) && (kh == 3 || kh == 5)) {
+            Taps<float> k{};
+            copy_taps(k.w, kernel, (uint32_t)taps);
+            dim3 grid((w + kF2STW - 1) / kF2STW, (h + kF2STH - 1) / kF2STH, 1);
+            const uint8_t *sp = static_cast<const uint8_t *>(src);
+            uint8_t *dp = static_cast<uint8_t *>(dst);
+            if (kw == 3 && kh == 3) filter2d_small_kernel<3, 3><<<grid, 256, 0, s>>>(sp, dp, (int)w, (int)h, k);
+            else if (kw == 3) filter2d_small_kernel<3, 5><<<grid, 256, 0, s>>>(sp, dp, (int)w, (int)h, k);
+            else if (kh == 3) filter2d_small_kernel<5, 3><<<grid, 256, 0, s>>>(sp, dp, (int)w, (int)h, k);
+            else filter2d_small_kernel<5, 5><<<grid, 256, 0, s>>>(sp, dp, (int)w, (int)h, k);
+            return launch_status("filter2d_small_kernel");
+        }
+    }
     if (taps <= (size_t)kMaxArgTaps && lds <= kMaxTiledLds) {
         Taps<K> k{};
         copy_taps(k.w, kernel, (uint32_t)taps);

@@ -144,7 +144,8 @@ def test_filter_clamped_regression_truths(fixtures):
     _same(np.minimum(mag, 255).astype(np.uint8), fixtures["elephant_gradients"])
 
 
-@pytest.mark.parametrize("kw,kh", [(1, 1), (3, 3), (5, 5), (2, 7), (16, 16), (17, 17), (40, 3)])
+@pytest.mark.parametrize("kw,kh", [(1, 1), (3, 3), (5, 5), (3, 5), (5, 3), (2, 7), (16, 16),
+                                   (17, 17), (40, 3)])
 def test_filter_clamped_random(rng, kw, kh):
     img = rng.integers(0, 256, (61, 90), dtype=np.uint8)
     kf = rng.standard_normal(kw * kh).astype(F)
@@ -161,6 +162,19 @@ def test_filter_clamped_random(rng, kw, kh):
     for st in (np.uint8, np.float32):
         _same(flt.filter_clamped(imgf, ip.Kernel(kf, kw, kh), st),
               O.filter_clamped(imgf, kf, kw, kh, st))
+
+
+def test_filter_clamped_small_kernels_tile_edges():
+    """u8 images, f32 3/5-sided kernels, u8 outputs (the compile-time kernel):
+    widths that take its 8-byte stores and its byte stores, partial last
+    tiles, single rows and images narrower than the kernel."""
+    rng = np.random.default_rng(23)
+    for h, w in [(70, 256), (97, 131), (1, 8), (40, 3), (33, 64), (65, 1000)]:
+        img = rng.integers(0, 256, (h, w), dtype=np.uint8)
+        for kw, kh in [(3, 3), (3, 5), (5, 3), (5, 5)]:
+            kf = (rng.standard_normal(kw * kh) * 0.4).astype(F)
+            _same(flt.filter_clamped(img, ip.Kernel(kf, kw, kh), np.uint8),
+                  O.filter_clamped(img, kf, kw, kh, np.uint8))
 
 
 def test_filter_clamped_config_5x5_normalised():
