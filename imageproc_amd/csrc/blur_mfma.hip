@@ -942,7 +942,11 @@ float blur_eps(const float *w, int L, int nkb) {
 //   * the reference's chain: product i is rounded once and then passes
 //     through L - max(i, 1) rounded additions (the first addition, 0 + x0,
 //     is exact), each off by at most 2^-24 relative, all terms >= 0 and
-//     pixels <= 255 (1 % slack for second-order terms).
+//     pixels <= 255 (1 % slack for second-order terms); or, where smaller,
+//     the sum of the half-ulps of every rounded value's largest possible
+//     magnitude: product i is at most 255 w_i and partial sum k at most
+//     255 (w_0 + ... + w_k), and a round-to-nearest result is off by at
+//     most half an ulp of any bound on its magnitude.
 // c exceeds the bound by 25 % plus one unit.
 struct FxPlan {
     int c;
@@ -952,7 +956,13 @@ struct FxPlan {
 FxPlan blur_fx_plan(const float *w, int L, int nkb) {
     const double unit = std::ldexp(1.0, kFxShift);
     const double amax = 255.0;  // largest A operand
-    double sum = 0.0, sum_hi = 0.0, lo_abs = 0.0, split = 0.0, chain = 0.0;
+    double sum = 0.0, sum_hi = 0.0, lo_abs = 0.0, split = 0.0, chain = 0.0, chain_ulp = 0.0;
+    // half an ulp of every f32 value of magnitude at most b (b > 0)
+    auto half_ulp = [](double b) {
+        int ex = 0;
+        std::frexp(b, &ex);  // b in [2^(ex-1), 2^ex): ulp 2^(ex-24)
+        return std::ldexp(1.0, ex - 25);
+    };
     for (int i = 0; i < L; i++) {
         if (!(w[i] >= 0.0f) || !std::isfinite(w[i])) return {-1, 0.0f};
         const float ws = w[i] * (float)unit;
@@ -965,10 +975,15 @@ FxPlan blur_fx_plan(const float *w, int L, int nkb) {
         if (std::fabs((double)(float)lo) < std::ldexp(1.0, -14)) res += std::fabs((double)(float)lo);
         split += res;
         chain += (double)w[i] * (1.0 + (double)(L - std::max(i, 1)));
+        const double slack = 1.0 + 1e-5;  // rounded terms exceed exact ones by < 34 * 2^-24
+        if (w[i] > 0.0f) chain_ulp += half_ulp(amax * (double)w[i] * slack);
+        if (i >= 1 && sum > 0.0) chain_ulp += half_ulp(amax * sum * slack);
     }
     const double b_lo = amax * lo_abs;
+    const double chain_err =
+        unit * std::min(255.0 * chain * std::ldexp(1.0, -24) * 1.01, chain_ulp);
     double e = 0.5 + (2.0 * L + 2.0 * nkb) * b_lo * std::ldexp(1.0, -23) + 255.0 * split +
-               unit * 255.0 * chain * std::ldexp(1.0, -24) * 1.01;
+               chain_err;
     const double c = std::ceil(1.25 * e + 1.0);
     const double init_hi = 8388608.0 + c;
     // the hi accumulator is exact: every partial sum is an integer of
@@ -1434,19 +1449,33 @@ struct WaveJob {
     float h[kFxMaxL], v[kFxMaxL];
 };
 
-// The reference's chain over taps t < L (zero taps pad it to a multiple of
-// four: with non-negative taps every partial sum is >= +0, and adding +0
-// leaves it unchanged). `taps` is the zero-padded LDS copy (tap t at
-// kTapPad + t); `p(t)` returns input t as f32.
-// The chunk loop stays rolled: its tap loads then cannot be hoisted out of
-// the caller's loop over flagged outputs into registers.
+// The reference's chain over taps t < L in chunks of four: a chunk's pixel
+// and tap loads are issued together, then its products and sums run in
+// order. Zero taps pad the last chunk: with non-negative taps every partial
+// sum is >= +0, and adding +0 leaves it unchanged. `taps` is the
+// zero-padded LDS copy (tap t at kTapPad + t); `p(t)` returns input t as
+// f32 and must return a finite value for every t below L rounded up to 4.
+// The chunk loop stays rolled (unrolled chains take registers the strip
+// loop needs).
 template <typename F>
 __device__ __forceinline__ float wave_chain(const float *taps, int L, F p) {
+    // an opaque zero in the tap index keeps the tap loads inside the caller's
+    // loop over flagged outputs (hoisted, they would hold LMAX registers
+    // across the whole strip loop)
+    int z = 0;
+    asm volatile("" : "+v"(z));
+    const float *tp = taps + kTapPad + z;
     float s = 0.0f;
 #pragma unroll 1
     for (int t0 = 0; t0 < L; t0 += 4) {
+        float x[4], w[4];
 #pragma unroll
-        for (int t = t0; t < t0 + 4; t++) s = __fadd_rn(s, __fmul_rn(p(t), taps[kTapPad + t]));
+        for (int t = 0; t < 4; t++) {
+            x[t] = p(t0 + t);
+            w[t] = tp[t0 + t];
+        }
+#pragma unroll
+        for (int t = 0; t < 4; t++) s = __fadd_rn(s, __fmul_rn(x[t], w[t]));
     }
     return s;
 }
@@ -1477,20 +1506,13 @@ __device__ __forceinline__ void wave_a_init(int g, v8h (&a)[NKB]) {
     a[NKB - 1] = __builtin_bit_cast(v8h, w);
 }
 
-// The Toeplitz operands live in LDS, one 16-byte fragment per lane, laid out
-// [kb][hi, lo][lane] (built once per workgroup by wave_b_table): each pass
-// reads its 2 NKB fragments just before its MFMAs, so they take no registers
-// across the step.
+// The 2 NKB MFMAs of one pass. The Toeplitz operands (one 16-byte fragment
+// per lane and K block, hi and lo) are built once per workgroup into LDS
+// tables laid out [kb][hi, lo][lane] (wave_b_table).
 template <int NKB>
-__device__ __forceinline__ void wave_mfma(const v8h (&a)[NKB], const v8h *bt, int lane,
-                                          v16f &acc, v16f &acl) {
+__device__ __forceinline__ void wave_mfma_r(const v8h (&a)[NKB], const v8h (&bhi)[NKB],
+                                            const v8h (&blo)[NKB], v16f &acc, v16f &acl) {
     const v16f zero = {};
-    v8h bhi[NKB], blo[NKB];
-#pragma unroll
-    for (int kb = 0; kb < NKB; kb++) {
-        bhi[kb] = bt[(2 * kb) * 64 + lane];
-        blo[kb] = bt[(2 * kb + 1) * 64 + lane];
-    }
 #pragma unroll
     for (int kb = 0; kb < NKB; kb++) {
         acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[kb], bhi[kb], kb == 0 ? zero : acc, 0, 0, 0);
@@ -1546,6 +1568,32 @@ __device__ __forceinline__ void blur_u8_wave_strip(const WaveJob &jb, const floa
         reinterpret_cast<uint4 *>(ring)[i] = make_uint4(0, 0, 0, 0);
 
     const uint32_t thr_h = 2u * (uint32_t)jb.ch, thr_v = 2u * (uint32_t)jb.cv;
+    // shared taps, aligned rows: the Toeplitz operands, from the workgroup's
+    // LDS table into registers for the whole strip (two sets, or one beside
+    // the byte-wise loads of unaligned rows, would not fit; those read them
+    // from LDS in each pass)
+    constexpr bool BREG = SHARE_B && ALIGNED;
+    v8h rbh[BREG ? NKB : 1], rbl[BREG ? NKB : 1];
+    if constexpr (BREG) {
+#pragma unroll
+        for (int kb = 0; kb < NKB; kb++) {
+            rbh[kb] = bt_h[(2 * kb) * 64 + lane];
+            rbl[kb] = bt_h[(2 * kb + 1) * 64 + lane];
+        }
+    }
+    auto pass_mfma = [&](const v8h(&a)[NKB], const v8h *bt, v16f &acc, v16f &acl) {
+        if constexpr (BREG) {
+            wave_mfma_r<NKB>(a, rbh, rbl, acc, acl);
+        } else {
+            v8h bh[NKB], bl[NKB];
+#pragma unroll
+            for (int kb = 0; kb < NKB; kb++) {
+                bh[kb] = bt[(2 * kb) * 64 + lane];
+                bl[kb] = bt[(2 * kb + 1) * 64 + lane];
+            }
+            wave_mfma_r<NKB>(a, bh, bl, acc, acl);
+        }
+    };
 
     // input row yh0 + 32 t + n of H(t), staged columns xs + 8 g + 16 kb + [0, 8),
     // through raw buffers with 32-bit offsets from the image base (W * H < 2^31)
@@ -1674,7 +1722,7 @@ __device__ __forceinline__ void blur_u8_wave_strip(const WaveJob &jb, const floa
         for (int kb = 0; kb < NKB; kb++)
             a[kb] = *reinterpret_cast<const v8h *>(acol + ((32 * P + 16 * kb) & (kRing - 1)));
         wave_a_init<NKB>(g, a);
-        wave_mfma<NKB>(a, bt_v, lane, acc, acl);
+        pass_mfma(a, bt_v, acc, acl);
     };
     auto v_readout = [&](auto par_c, int s, const v16f &acc, const v16f &acl) {
         constexpr int P = decltype(par_c)::value;
@@ -1704,9 +1752,16 @@ __device__ __forceinline__ void blur_u8_wave_strip(const WaveJob &jb, const floa
                 fl &= fl - 1u;
                 const int i = 8 * (r >> 2) + 4 * g + (r & 3);
                 const _Float16 *c = ring + i * RPH;
-                const int base = 32 * P + n + m;
+                // ring rows r0 .. r0 + L - 1 (mod 64); read straight when they
+                // do not wrap (the rows past L up to the chunk end then lie in
+                // the column's zeroed padding rows 64 .. 67)
+                const int r0 = (32 * P + n + m) & (kRing - 1);
                 const float e =
-                    wave_chain(taps_v, L, [&](int t) { return (float)c[(base + t) & (kRing - 1)]; });
+                    r0 + L <= kRing
+                        ? wave_chain(taps_v, L, [&](int t) { return (float)c[r0 + t]; })
+                        : wave_chain(taps_v, L, [&](int t) {
+                              return (float)c[(r0 + t) & (kRing - 1)];
+                          });
                 orow[i] = clamp_u8_exact(e);
             }
         }
@@ -1746,7 +1801,7 @@ __device__ __forceinline__ void blur_u8_wave_strip(const WaveJob &jb, const floa
     auto h_mfma = [&](Chunks &d) {
         v8h a[NKB];
         stage_h(d, a);
-        wave_mfma<NKB>(a, bt_h, lane, xc, xl);
+        pass_mfma(a, bt_h, xc, xl);
     };
     // (the loads run two passes ahead unconditionally: rows past the image
     // are clamped, so the last ones read rows that are cached already, and

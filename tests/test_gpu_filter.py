@@ -268,7 +268,8 @@ def test_stream_blur_bands_strips_and_ragged_edges():
     rng = np.random.default_rng(21)
     for h, w, sigma in [(1000, 37, 2.0), (2050, 131, 1.0), (77, 1029, 3.3), (4099, 260, 8.0),
                         (33, 4, 0.5), (300, 1, 2.0), (1, 300, 2.0), (31, 129, 7.6),
-                        (768, 1024, 2.0), (65, 2049, 4.0)]:
+                        (768, 1024, 2.0), (65, 2049, 4.0), (4099, 1024, 2.0),
+                        (1001, 528, 3.0)]:
         img = rng.integers(0, 256, (h, w), dtype=np.uint8)
         _same(flt.gaussian_blur_f32(img, sigma), O.gaussian_blur_f32(img, sigma))
 
