@@ -116,7 +116,8 @@ void unplan(iph_batch *b) {
 }
 
 iph_status enqueue_blur(iph_batch *b, hipStream_t s) {
-    IPH_TRY(launch_blur_jobs(b->mem + b->blur_table_off, b->blur_groups, b->blur_grid_pct, s));
+    IPH_TRY(launch_blur_jobs(b->mem + b->blur_table_off, (int)b->blur.size(), b->blur_groups,
+                             b->blur_grid_pct, s));
     for (const FallbackBlur &f : b->blur_fallback) {
         const BlurSpec &sp = b->blur[f.spec];
         IPH_TRY(launch_separable(sp.src, sp.dst, sp.w, sp.h, 1, (size_t)sp.w * sp.h, IPH_U8,
@@ -276,7 +277,9 @@ iph_status autotune(iph_batch *b, hipStream_t s) {
         std::lock_guard<std::mutex> lock(g_match_cache_mu);
         g_match_cache[match_class(g)] = best_r;
     }
-    if (!b->blur_groups.empty()) {
+    // the tile kernel's persistent grid (the per-wave kernel's grid is fixed)
+    if (std::any_of(b->blur_groups.begin(), b->blur_groups.end(),
+                    [](const BlurGroup &g) { return !g.wave; })) {
         int best_p = 100;
         float best_ms = 1e30f;
         for (int pct : {100, 67, 34}) {

@@ -1446,6 +1446,7 @@ struct WaveJob {
     int W, H, n_img, strips, bands, band_rows;
     int L, m;                    // taps, window shift (the staged columns start 4-aligned)
     int ch, cv;                  // readout margins (units of 2^-15)
+    int unit0;                   // a table job's first wave unit (a multiple of 4)
     float h[kFxMaxL], v[kFxMaxL];
 };
 
@@ -1835,8 +1836,11 @@ __device__ __forceinline__ void blur_u8_wave_strip(const WaveJob &jb, const floa
     }
 }
 
+// The workgroup's shared tables for job `jb` (uniform across the workgroup),
+// then wave unit u of the job, unless `valid` is false (the wave only takes
+// part in the table set-up).
 template <int NKB, bool SHARE_B, bool ALIGNED>
-__device__ __forceinline__ void blur_u8_wave_body(const WaveJob &jb) {
+__device__ __forceinline__ void blur_u8_wave_unit(const WaveJob &jb, int u, bool valid) {
     constexpr int CIN = 16 * NKB, PI = wave_in_pitch<NKB>(), WL = wave_lds_bytes<NKB>();
     extern __shared__ __align__(16) unsigned char smem[];
     float *taps_h = reinterpret_cast<float *>(smem);
@@ -1861,17 +1865,9 @@ __device__ __forceinline__ void blur_u8_wave_body(const WaveJob &jb) {
     if (!SHARE_B && threadIdx.x >= 64 && threadIdx.x < 128)
         wave_b_table<NKB>(taps_v, (int)threadIdx.x - 64, jb.m, jb.cv, bt_v);
     __syncthreads();  // Toeplitz tables written; no workgroup barrier after this one
+    if (!valid) return;
 
-    // wave unit = (image, band, strip), strip fastest; a workgroup takes four
-    // neighbouring units, and contiguous eighths of the workgroups go to the
-    // eight XCDs (workgroups are dealt round-robin). The wave index is read
-    // as a wave-uniform value, so the unit, the strip's geometry and the
-    // buffer descriptors live in scalar registers.
-    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int units = jb.n_img * jb.bands * jb.strips;
-    const int chunk = (((units + 3) >> 2) + 7) >> 3;
-    const int u = ((int)(blockIdx.x & 7) * chunk + (int)(blockIdx.x >> 3)) * 4 + wave;
-    if ((int)(blockIdx.x >> 3) >= chunk || u >= units) return;
+    // wave unit = (image, band, strip), strip fastest
     const int per_img = jb.bands * jb.strips;
     const int im = u / per_img, rem = u - im * per_img;
     const int band = rem / jb.strips, strip = rem - band * jb.strips;
@@ -1887,12 +1883,48 @@ __device__ __forceinline__ void blur_u8_wave_body(const WaveJob &jb) {
                                                     out_w, im, band, strip);
 }
 
+// A workgroup takes four neighbouring wave units, and contiguous eighths of
+// the workgroups go to the eight XCDs (workgroups are dealt round-robin).
+// The wave index is read as a wave-uniform value, so the unit, the strip's
+// geometry and the buffer descriptors live in scalar registers.
+template <int NKB, bool SHARE_B, bool ALIGNED>
+__device__ __forceinline__ void blur_u8_wave_body(const WaveJob &jb) {
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int units = jb.n_img * jb.bands * jb.strips;
+    const int chunk = (((units + 3) >> 2) + 7) >> 3;
+    if ((int)(blockIdx.x >> 3) >= chunk) return;  // the whole workgroup
+    const int u = ((int)(blockIdx.x & 7) * chunk + (int)(blockIdx.x >> 3)) * 4 + wave;
+    blur_u8_wave_unit<NKB, SHARE_B, ALIGNED>(jb, u, u < units);
+}
+
 // The job is read through a reference to the kernel argument: its taps are
 // indexed per thread, which must not copy the argument into registers.
 template <int NKB, bool SHARE_B, bool ALIGNED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void
 blur_u8_wave_kernel(WaveJob job) {
     blur_u8_wave_body<NKB, SHARE_B, ALIGNED>(job);
+}
+
+// The same over a device table of single-image jobs (ragged batches): job j
+// owns wave units [unit0, unit0 + bands * strips), unit0 a multiple of 4, so
+// the four waves of a workgroup always belong to one job and share its
+// tables. nwg = total units / 4.
+template <int NKB, bool SHARE_B, bool ALIGNED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void
+blur_u8_wave_jobs_kernel(const WaveJob *__restrict__ jobs, int njobs, int nwg) {
+    const int chunk = (nwg + 7) >> 3;
+    const int wg = (int)(blockIdx.x & 7) * chunk + (int)(blockIdx.x >> 3);
+    if ((int)(blockIdx.x >> 3) >= chunk || wg >= nwg) return;  // the whole workgroup
+    int lo = 0, hi = njobs - 1;  // the last job with unit0 <= 4 wg
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].unit0 <= 4 * wg) lo = mid;
+        else hi = mid - 1;
+    }
+    const WaveJob &jb = jobs[lo];
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int u = 4 * wg + wave - jb.unit0;
+    blur_u8_wave_unit<NKB, SHARE_B, ALIGNED>(jb, u, u < jb.bands * jb.strips);
 }
 
 template <int NKB, bool SHARE_B> constexpr size_t wave_kernel_lds() {
@@ -2114,9 +2146,15 @@ iph_status launch_blur_u8_mfma(const uint8_t *src, uint8_t *dst, uint32_t w, uin
     return launch_status("blur_u8_mfma_kernel");
 }
 
-// ---- batches: fixed-point jobs grouped by kernel variant -------------------------
+// ---- batches: jobs grouped by kernel variant -------------------------------------
+//
+// The device table holds the fixed-point tile kernel's FxJob entries (n slots)
+// followed by the per-wave kernel's WaveJob entries (n slots); each group's
+// entries are consecutive in their part.
 
-size_t blur_jobs_table_bytes(int n) { return (size_t)std::max(n, 1) * sizeof(FxJob); }
+size_t blur_jobs_table_bytes(int n) {
+    return (size_t)std::max(n, 1) * (sizeof(FxJob) + sizeof(WaveJob));
+}
 
 iph_status blur_jobs_table(const BlurSpec *specs, int n, void *host_table,
                            std::vector<BlurGroup> &groups, std::vector<int> &fallback) {
@@ -2124,7 +2162,12 @@ iph_status blur_jobs_table(const BlurSpec *specs, int n, void *host_table,
         int key, idx;
         FxJob job;
     };
+    struct WEntry {
+        int key, idx;
+        WaveJob job;
+    };
     std::vector<Entry> entries;
+    std::vector<WEntry> wentries;
     groups.clear();
     fallback.clear();
     for (int i = 0; i < n; i++) {
@@ -2136,13 +2179,40 @@ iph_status blur_jobs_table(const BlurSpec *specs, int n, void *host_table,
             continue;
         }
         const int half = L / 2, m = (4 - half % 4) % 4, nkb = (31 + L + m + 15) / 16;
+        const bool aligned = b.w >= 4 && (b.w % 4) == 0 &&
+                             (reinterpret_cast<uintptr_t>(b.src) & 3) == 0;
+        // the per-wave kernel where it takes the taps (two K rows beyond them
+        // carry the start value)
+        const int nkb_w = (L + m + 33 + 15) / 16;
+        if (nkb_w <= 4 && (uint64_t)b.w * b.h < (1ull << 31)) {
+            const FxPlan pw = blur_fx_plan(b.taps, L, nkb_w);
+            if (pw.c > 0) {
+                WaveJob j{};
+                j.src = b.src;
+                j.dst = b.dst;
+                j.stride = (long long)b.w * b.h;
+                j.W = (int)b.w;
+                j.H = (int)b.h;
+                j.n_img = 1;
+                j.strips = (int)((b.w + 31) / 32);
+                // bands of about 24 steps of 32 rows
+                const int steps = (int)((b.h + 31) / 32);
+                const int bands0 = std::max(1, steps / 24);
+                j.band_rows = (steps + bands0 - 1) / bands0 * 32;
+                j.bands = (int)((b.h + j.band_rows - 1) / j.band_rows);
+                j.L = L;
+                j.m = m;
+                j.ch = j.cv = pw.c;
+                for (int q = 0; q < L; q++) j.h[q] = j.v[q] = b.taps[q];
+                wentries.push_back({nkb_w * 2 + (aligned ? 1 : 0), i, j});
+                continue;
+            }
+        }
         const FxPlan pl = blur_fx_plan(b.taps, L, nkb);
         if (pl.c <= 0 || nkb < 2 || nkb > 4) {
             fallback.push_back(i);
             continue;
         }
-        const bool aligned = b.w >= 4 && (b.w % 4) == 0 &&
-                             (reinterpret_cast<uintptr_t>(b.src) & 3) == 0;
         const int cls = (same && L + m <= 33) ? 2 : 1;
         FxJob f{};
         f.src = b.src;
@@ -2162,12 +2232,14 @@ iph_status blur_jobs_table(const BlurSpec *specs, int n, void *host_table,
     }
     std::stable_sort(entries.begin(), entries.end(),
                      [](const Entry &a, const Entry &b) { return a.key < b.key; });
+    std::stable_sort(wentries.begin(), wentries.end(),
+                     [](const WEntry &a, const WEntry &b) { return a.key < b.key; });
     FxJob *table = static_cast<FxJob *>(host_table);
     for (size_t e = 0; e < entries.size(); e++) {
         const int key = entries[e].key;
         if (groups.empty() || groups.back().nkb * 8 + groups.back().taps_class * 2 +
                                       (groups.back().aligned ? 1 : 0) != key) {
-            groups.push_back({key / 8, (key % 8) / 2, (key & 1) != 0, (int)e, 0, 0});
+            groups.push_back({key / 8, (key % 8) / 2, (key & 1) != 0, false, (int)e, 0, 0});
         }
         BlurGroup &g = groups.back();
         FxJob f = entries[e].job;
@@ -2181,17 +2253,53 @@ iph_status blur_jobs_table(const BlurSpec *specs, int n, void *host_table,
         g.count++;
         table[e] = f;
     }
+    WaveJob *wtable = reinterpret_cast<WaveJob *>(
+        static_cast<unsigned char *>(host_table) + (size_t)std::max(n, 1) * sizeof(FxJob));
+    const size_t nfx = groups.size();
+    long long units = 0;
+    for (size_t e = 0; e < wentries.size(); e++) {
+        const int key = wentries[e].key;
+        if (groups.size() == nfx || groups.back().nkb * 2 + (groups.back().aligned ? 1 : 0) != key) {
+            groups.push_back({key / 2, 2, (key & 1) != 0, true, (int)e, 0, 0});
+            units = 0;
+        }
+        BlurGroup &g = groups.back();
+        WaveJob j = wentries[e].job;
+        j.unit0 = (int)units;
+        units += ((long long)j.bands * j.strips + 3) / 4 * 4;
+        if (units > (1LL << 30)) {
+            set_error("batch too large for one launch");
+            return IPH_EINVAL;
+        }
+        g.ntiles = (int)(units / 4);
+        g.count++;
+        wtable[e] = j;
+    }
     return IPH_OK;
 }
 
-iph_status launch_blur_jobs(const void *table_dev, const std::vector<BlurGroup> &groups,
+iph_status launch_blur_jobs(const void *table_dev, int n, const std::vector<BlurGroup> &groups,
                             int grid_pct, hipStream_t s) {
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess)
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     const FxJob *table = static_cast<const FxJob *>(table_dev);
+    const WaveJob *wtable = reinterpret_cast<const WaveJob *>(
+        static_cast<const unsigned char *>(table_dev) + (size_t)std::max(n, 1) * sizeof(FxJob));
     for (const BlurGroup &g : groups) {
         if (g.ntiles == 0) continue;
+        if (g.wave) {
+            const unsigned grid = (unsigned)(((long long)g.ntiles + 7) / 8 * 8);
+#define WJ(K, AL)                                                                                blur_u8_wave_jobs_kernel<K, true, AL><<<grid, 256, wave_kernel_lds<K, true>(), s>>>(               wtable + g.first, g.count, g.ntiles)
+            if (g.nkb == 3) {
+                if (g.aligned) WJ(3, true); else WJ(3, false);
+            } else {
+                if (g.aligned) WJ(4, true); else WJ(4, false);
+            }
+#undef WJ
+            IPH_TRY(launch_status("blur_u8_wave_jobs_kernel"));
+            continue;
+        }
 #define JB(K, CLS, AL)                                                                       \
     do {                                                                                     \
         using Sh = RowStage<K, kRows2>;                                                     \

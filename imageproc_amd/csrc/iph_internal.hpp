@@ -171,10 +171,12 @@ struct BlurSpec {
     uint32_t w, h, len;
     float taps[kBlurSpecMaxTaps];
 };
-// Jobs of one fixed-point kernel variant, consecutive in the device table.
+// Jobs of one kernel variant, consecutive in the device table: the per-wave
+// kernel (wave: `first` indexes the table's per-wave entries, ntiles counts
+// its workgroups) or the fixed-point tile kernel (ntiles counts its tiles).
 struct BlurGroup {
     int nkb, taps_class;
-    bool aligned;
+    bool aligned, wave;
     int first, count, ntiles;
 };
 size_t blur_jobs_table_bytes(int n);
@@ -182,8 +184,10 @@ size_t blur_jobs_table_bytes(int n);
 // fixed-point kernel does not take in `fallback`.
 iph_status blur_jobs_table(const BlurSpec *specs, int n, void *host_table,
                            std::vector<BlurGroup> &groups, std::vector<int> &fallback);
-// grid_pct: persistent workgroups as a percentage of the resident maximum.
-iph_status launch_blur_jobs(const void *table_dev, const std::vector<BlurGroup> &groups,
+// n: the specs the table was built from (its per-wave part starts after n
+// FxJob slots); grid_pct: persistent workgroups of the tile kernel as a
+// percentage of the resident maximum.
+iph_status launch_blur_jobs(const void *table_dev, int n, const std::vector<BlurGroup> &groups,
                             int grid_pct, hipStream_t s);
 
 // match_template (GrayImage, int8 MFMA with in-kernel window sums) of a batch:
