@@ -1456,16 +1456,11 @@ struct WaveJob {
 // sum is >= +0, and adding +0 leaves it unchanged. `taps` is the
 // zero-padded LDS copy (tap t at kTapPad + t); `p(t)` returns input t as
 // f32 and must return a finite value for every t below L rounded up to 4.
-// The chunk loop stays rolled (unrolled chains take registers the strip
-// loop needs).
+// The chunk loop stays rolled: unrolled, its tap loads would be hoisted out
+// of the caller's loop over flagged outputs into registers the strip loop
+// needs.
 template <typename F>
 __device__ __forceinline__ float wave_chain(const float *taps, int L, F p) {
-    // an opaque zero in the tap index keeps the tap loads inside the caller's
-    // loop over flagged outputs (hoisted, they would hold LMAX registers
-    // across the whole strip loop)
-    int z = 0;
-    asm volatile("" : "+v"(z));
-    const float *tp = taps + kTapPad + z;
     float s = 0.0f;
 #pragma unroll 1
     for (int t0 = 0; t0 < L; t0 += 4) {
@@ -1473,7 +1468,7 @@ __device__ __forceinline__ float wave_chain(const float *taps, int L, F p) {
 #pragma unroll
         for (int t = 0; t < 4; t++) {
             x[t] = p(t0 + t);
-            w[t] = tp[t0 + t];
+            w[t] = taps[kTapPad + t0 + t];
         }
 #pragma unroll
         for (int t = 0; t < 4; t++) s = __fadd_rn(s, __fmul_rn(x[t], w[t]));
