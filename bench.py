@@ -422,13 +422,24 @@ def bench_match(steps: int, cpu: bool = False):
     times, _ = timed(lambda: dev.find_extremes(out, out=ex), steps, 2)
     res["find_extremes"] = {"ms": round(float(np.median(times)), 4),
                             "GB/s": round(4.0 * ow * ow / (float(np.median(times)) * 1e-3) / 1e9, 1)}
-    # match_template_with_mask (reference-order direct path), seeded mask
+    # match_template_with_mask, seeded mask: the reference-order direct path
+    # and the MFMA path (BF16x3 correlations), argmax and max abs difference
     mask = torch.from_numpy(uniform_u8(202, (MATCH_T, MATCH_T))).cuda()
-    out = dev.match_template_with_mask(gi, gt, M.CrossCorrelationNormalized, mask)
-    times, _ = timed(lambda: dev.match_template_with_mask(gi, gt, M.CrossCorrelationNormalized,
-                                                          mask, out=out), max(2, steps // 2), 1)
-    masked_ms = float(np.median(times))
+    mres = {}
+    for path in ("direct", "mfma"):
+        out = dev.match_template_with_mask(gi, gt, M.CrossCorrelationNormalized, mask, path=path)
+        times, _ = timed(lambda: dev.match_template_with_mask(
+            gi, gt, M.CrossCorrelationNormalized, mask, out=out, path=path), max(2, steps // 2), 1)
+        host = out.cpu().numpy()
+        mres[path] = (float(np.median(times)), host)
+    masked_ms = mres["direct"][0]
     res["CrossCorrelationNormalized:masked_direct"] = {"ms_per_image": round(masked_ms, 4)}
+    loc = {p: np.unravel_index(int(np.argmax(v[1])), v[1].shape) for p, v in mres.items()}
+    res["CrossCorrelationNormalized:masked_mfma"] = {
+        "ms_per_image": round(mres["mfma"][0], 4),
+        "best_location": [int(loc["mfma"][1]), int(loc["mfma"][0])],
+        "argmax_agrees_with_direct": loc["mfma"] == loc["direct"],
+        "max_abs_diff_vs_direct": float(np.max(np.abs(mres["mfma"][1] - mres["direct"][1])))}
     if cpu:
         nt = cpu_threads()
         cpu_ms = {}

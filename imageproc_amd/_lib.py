@@ -123,13 +123,15 @@ _SIGS = {
     "iph_match_template_with_mask": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32,
                                                  C.c_void_p, C.c_uint32, C.c_uint32,
                                                  C.c_void_p, C.c_uint32, C.c_uint32,
-                                                 C.c_int32, C.c_void_p]),
-    "iph_dev_match_template_with_mask_workspace": (C.c_size_t, []),
+                                                 C.c_int32, C.c_int32, C.c_void_p]),
+    "iph_dev_match_template_with_mask_workspace": (C.c_size_t, [C.c_uint32, C.c_uint32,
+                                                               C.c_uint32, C.c_uint32,
+                                                               C.c_int32, C.c_int32]),
     "iph_dev_match_template_with_mask": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32,
                                                      C.c_void_p, C.c_uint32, C.c_uint32,
                                                      C.c_void_p, C.c_uint32, C.c_uint32,
-                                                     C.c_int32, C.c_void_p, C.c_void_p,
-                                                     C.c_size_t, C.c_void_p]),
+                                                     C.c_int32, C.c_int32, C.c_void_p,
+                                                     C.c_void_p, C.c_size_t, C.c_void_p]),
     "iph_dev_match_template_workspace": (C.c_size_t, [C.c_int32, C.c_uint32, C.c_uint32,
                                                       C.c_uint32, C.c_uint32, C.c_int32,
                                                       C.c_int32]),

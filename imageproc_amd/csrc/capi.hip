@@ -597,21 +597,34 @@ IPH_API iph_status iph_match_template(const void *image, int32_t p, uint32_t w, 
 // ImageTemplateMask::new, template_matching.rs:430-440: the mask check comes
 // before the template-vs-image checks.
 static iph_status check_masked(uint32_t w, uint32_t h, uint32_t tw, uint32_t th, uint32_t mw,
-                               uint32_t mh, int32_t method) {
+                               uint32_t mh, int32_t method, int32_t path) {
     if (mw != tw || mh != th) {
         set_error("assertion `left == right` failed: the template and mask must be the same "
                   "size\n  left: (%u, %u)\n right: (%u, %u)", tw, th, mw, mh);
         return IPH_EINVAL;
     }
-    return check_match(IPH_U8, w, h, tw, th, method, IPH_PATH_DIRECT);
+    return check_match(IPH_U8, w, h, tw, th, method, path);
+}
+
+// The default masked path: the MFMA correlations for CrossCorrelation and
+// CrossCorrelationNormalized with templates up to 128 px, else the direct
+// kernel in the reference's order.
+static int32_t resolve_masked_path(int32_t path, uint32_t w, uint32_t h, uint32_t tw,
+                                   uint32_t th, int32_t method) {
+    if (path != IPH_PATH_AUTO) return path;
+    if (tw == 0 || th == 0 || (method != IPH_CCORR && method != IPH_CCORR_NORMED))
+        return IPH_PATH_DIRECT;
+    return (tw <= kMfmaMaxSide && th <= kMfmaMaxSide && mfma_match_fits(w, h)) ? IPH_PATH_MFMA
+                                                                               : IPH_PATH_DIRECT;
 }
 
 IPH_API iph_status iph_match_template_with_mask(const uint8_t *image, uint32_t w, uint32_t h,
                                                 const uint8_t *tmpl, uint32_t tw, uint32_t th,
                                                 const uint8_t *mask, uint32_t mw, uint32_t mh,
-                                                int32_t method, float *out) {
+                                                int32_t method, int32_t path, float *out) {
     clear_error();
-    IPH_TRY(check_masked(w, h, tw, th, mw, mh, method));
+    IPH_TRY(check_masked(w, h, tw, th, mw, mh, method, path));
+    path = resolve_masked_path(path, w, h, tw, th, method);
     const uint32_t ow = w - tw + 1, oh = h - th + 1;
     const size_t obytes = (size_t)ow * oh * sizeof(float);
     if (obytes == 0) return IPH_OK;
@@ -621,12 +634,12 @@ IPH_API iph_status iph_match_template_with_mask(const uint8_t *image, uint32_t w
     IPH_TRY(upload(dt, tmpl, (size_t)tw * th));
     IPH_TRY(upload(dm, mask, (size_t)tw * th));
     IPH_TRY(dout.alloc(obytes));
-    IPH_TRY(dws.alloc(masked_match_workspace()));
+    const size_t wsb = masked_match_workspace(w, h, tw, th, method, path);
+    IPH_TRY(dws.alloc(wsb));
     IPH_TRY(launch_masked_match(static_cast<const uint8_t *>(di.p), w, h,
                                 static_cast<const uint8_t *>(dt.p), tw, th,
-                                static_cast<const uint8_t *>(dm.p), method,
-                                static_cast<float *>(dout.p), dws.p, masked_match_workspace(),
-                                nullptr));
+                                static_cast<const uint8_t *>(dm.p), method, path,
+                                static_cast<float *>(dout.p), dws.p, wsb, nullptr));
     return finish(dout, out, obytes);
 }
 
@@ -746,20 +759,24 @@ IPH_API iph_status iph_dev_box_filter(const uint8_t *src, uint8_t *dst, uint32_t
     return launch_box_filter(src, dst, w, h, rx, ry, ws, wsb, static_cast<hipStream_t>(stream));
 }
 
-IPH_API size_t iph_dev_match_template_with_mask_workspace(void) {
-    return masked_match_workspace();
+IPH_API size_t iph_dev_match_template_with_mask_workspace(uint32_t w, uint32_t h, uint32_t tw,
+                                                          uint32_t th, int32_t method,
+                                                          int32_t path) {
+    return masked_match_workspace(w, h, tw, th, method,
+                                  resolve_masked_path(path, w, h, tw, th, method));
 }
 
 IPH_API iph_status iph_dev_match_template_with_mask(const uint8_t *image, uint32_t w, uint32_t h,
                                                     const uint8_t *tmpl, uint32_t tw,
                                                     uint32_t th, const uint8_t *mask,
                                                     uint32_t mw, uint32_t mh, int32_t method,
-                                                    float *out, void *ws, size_t wsb,
-                                                    void *stream) {
+                                                    int32_t path, float *out, void *ws,
+                                                    size_t wsb, void *stream) {
     clear_error();
-    IPH_TRY(check_masked(w, h, tw, th, mw, mh, method));
+    IPH_TRY(check_masked(w, h, tw, th, mw, mh, method, path));
     IPH_TRY(require_device());
-    return launch_masked_match(image, w, h, tmpl, tw, th, mask, method, out, ws, wsb,
+    return launch_masked_match(image, w, h, tmpl, tw, th, mask, method,
+                               resolve_masked_path(path, w, h, tw, th, method), out, ws, wsb,
                                static_cast<hipStream_t>(stream));
 }
 

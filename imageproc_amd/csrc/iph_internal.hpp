@@ -224,10 +224,15 @@ size_t match_workspace(int32_t p_type, uint32_t w, uint32_t h, uint32_t tw, uint
 iph_status launch_match(const void *img, int32_t p_type, uint32_t w, uint32_t h,
                         const void *tpl, uint32_t tw, uint32_t th, int32_t method,
                         int32_t path, float *out, void *ws, size_t ws_bytes, hipStream_t s);
-size_t masked_match_workspace();
+// path: IPH_PATH_DIRECT (the reference's order, bit-identical) or
+// IPH_PATH_MFMA (CrossCorrelation and CrossCorrelationNormalized, templates
+// up to 128 px: BF16x3 correlations); resolved by the caller.
+size_t masked_match_workspace(uint32_t w, uint32_t h, uint32_t tw, uint32_t th, int32_t method,
+                              int32_t path);
 iph_status launch_masked_match(const uint8_t *img, uint32_t w, uint32_t h, const uint8_t *tpl,
                                uint32_t tw, uint32_t th, const uint8_t *mask, int32_t method,
-                               float *out, void *ws, size_t ws_bytes, hipStream_t s);
+                               int32_t path, float *out, void *ws, size_t ws_bytes,
+                               hipStream_t s);
 size_t extremes_workspace();
 iph_status launch_extremes(const void *img, int32_t p_type, uint32_t w, uint32_t h,
                            iph_extremes *out, void *ws, size_t ws_bytes, hipStream_t s);

@@ -13,6 +13,7 @@
 
 #include "match_common.hpp"
 
+#include <algorithm>
 #include <mutex>
 
 namespace iph {
@@ -287,6 +288,54 @@ __global__ __launch_bounds__(256) void extremes_finalize_kernel(const P *__restr
     *out = r;
 }
 
+// ---- match_template_with_mask on MFMA ----------------------------------------------
+//
+// The masked cross-correlation sum t i m^2 (template_matching.rs:337-349) is
+// the correlation of the image I with the template T M^2, and the normalised
+// method's image term sum (i m)^2 (:351-374) the correlation of I^2 with M^2:
+// two BF16x3 correlations of f32 planes (I and I^2 are exact in a bf16 hi +
+// lo pair, so is M^2; T M^2 < 2^24 keeps 16 of its 24 bits), then the
+// reference's f32 finish with its sequential mult_square_sum. The SSE
+// methods are not taken: their sum t^2 m^2 - 2 sum t i m^2 + sum i^2 m^2
+// cancels to nearly nothing at a match, far below the error of the parts.
+
+// I and (optionally) I^2 of a u8 image as f32 planes.
+__global__ __launch_bounds__(256) void mask_image_planes_kernel(const uint8_t *__restrict__ img,
+                                                                size_t n, float *__restrict__ i1,
+                                                                float *__restrict__ i2) {
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256) {
+        const float v = (float)img[k];
+        i1[k] = v;
+        if (i2) i2[k] = v * v;  // <= 65025: exact
+    }
+}
+
+// T M^2 and (w2 not null: the normalised method) M^2, exact integers below
+// 2^24 in f32.
+__global__ __launch_bounds__(256) void mask_template_planes_kernel(const uint8_t *__restrict__ tpl,
+                                                                   const uint8_t *__restrict__ mask,
+                                                                   int n, float *__restrict__ w1,
+                                                                   float *__restrict__ w2) {
+    for (int k = threadIdx.x; k < n; k += 256) {
+        const float m2 = (float)mask[k] * (float)mask[k];
+        w1[k] = (float)tpl[k] * m2;
+        if (w2) w2[k] = m2;
+    }
+}
+
+// CrossCorrelationNormalizedWithMask's finish (:362-373) in f32: score /
+// sqrt(mult_square_sum * im_im), the raw score when that norm is 0.
+__global__ __launch_bounds__(256) void mask_ccorr_normed_finish_kernel(
+    float *__restrict__ out, const float *__restrict__ imim, const float *__restrict__ tms,
+    size_t n) {
+    const float t = *tms;
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256) {
+        const float norm = sqrtf(t * imim[k]);
+        const float sc = out[k];
+        out[k] = norm > 0.0f ? sc / norm : sc;
+    }
+}
+
 }  // namespace
 
 size_t match_workspace(int32_t p_type, uint32_t w, uint32_t h, uint32_t tw, uint32_t th,
@@ -321,11 +370,47 @@ iph_status launch_match(const void *img, int32_t p_type, uint32_t w, uint32_t h,
     return launch_mfma_match(img, p_type, w, h, tpl, tw, th, method, out, ws, ws_bytes, s);
 }
 
-size_t masked_match_workspace() { return 256; }
+// Workspace of the masked path: the mult_square_sum scalar, and on the MFMA
+// path the f32 image planes, the template planes, the im_im map and the
+// BF16x3 launcher's own workspace.
+template <typename WS>
+static void masked_plan(WS &wp, uint32_t w, uint32_t h, uint32_t tw, uint32_t th, int32_t method,
+                        int32_t path) {
+    wp.template take<float>(1);
+    if (path != IPH_PATH_MFMA) return;
+    const bool nrm = method == IPH_CCORR_NORMED;
+    const size_t n = (size_t)w * h, nt = (size_t)tw * th, no = (size_t)(w - tw + 1) * (h - th + 1);
+    wp.template take<float>(n);
+    if (nrm) wp.template take<float>(n);
+    wp.template take<float>(nt);
+    if (nrm) wp.template take<float>(nt);
+    if (nrm) wp.template take<float>(no);
+    wp.template take<unsigned char>(match_workspace(IPH_F32, w, h, tw, th, IPH_CCORR,
+                                                    IPH_PATH_MFMA));
+}
+
+size_t masked_match_workspace(uint32_t w, uint32_t h, uint32_t tw, uint32_t th, int32_t method,
+                              int32_t path) {
+    if (w < tw || h < th) return 256;
+    WorkspacePlan wp;
+    masked_plan(wp, w, h, tw, th, method, path);
+    return std::max<size_t>(wp.used, 256);
+}
 
 iph_status launch_masked_match(const uint8_t *img, uint32_t w, uint32_t h, const uint8_t *tpl,
                                uint32_t tw, uint32_t th, const uint8_t *mask, int32_t method,
-                               float *out, void *ws, size_t ws_bytes, hipStream_t s) {
+                               int32_t path, float *out, void *ws, size_t ws_bytes,
+                               hipStream_t s) {
+    if (path == IPH_PATH_MFMA && method != IPH_CCORR && method != IPH_CCORR_NORMED) {
+        set_error("the MFMA path of match_template_with_mask takes CrossCorrelation and "
+                  "CrossCorrelationNormalized; use the direct path");
+        return IPH_EUNSUPPORTED;
+    }
+    if (path == IPH_PATH_MFMA && (tw > kMfmaMaxSide || th > kMfmaMaxSide || !mfma_match_fits(w, h))) {
+        set_error("the MFMA path of match_template_with_mask takes templates up to %ux%u; use "
+                  "the direct path", kMfmaMaxSide, kMfmaMaxSide);
+        return IPH_EUNSUPPORTED;
+    }
     Workspace wsp(ws, ws_bytes);
     float *tms = wsp.take<float>(1);
     if (!tms) {
@@ -336,6 +421,34 @@ iph_status launch_masked_match(const uint8_t *img, uint32_t w, uint32_t h, const
     if ((size_t)ow * oh == 0) return IPH_OK;
     mult_square_sum_kernel<<<1, 64, 0, s>>>(tpl, mask, (size_t)tw * th, tms);
     IPH_TRY(launch_status("mult_square_sum_kernel"));
+    if (path == IPH_PATH_MFMA) {
+        const bool nrm = method == IPH_CCORR_NORMED;
+        const size_t n = (size_t)w * h, nt = (size_t)tw * th, no = (size_t)ow * oh;
+        float *i1 = wsp.take<float>(n);
+        float *i2 = nrm ? wsp.take<float>(n) : nullptr;
+        float *w1 = wsp.take<float>(nt);
+        float *w2 = nrm ? wsp.take<float>(nt) : nullptr;
+        float *imim = nrm ? wsp.take<float>(no) : nullptr;
+        const size_t bw = match_workspace(IPH_F32, w, h, tw, th, IPH_CCORR, IPH_PATH_MFMA);
+        unsigned char *bws = wsp.take<unsigned char>(bw);
+        if (!i1 || !w1 || !bws || (nrm && (!i2 || !w2 || !imim))) {
+            set_error("workspace too small for match_template_with_mask");
+            return IPH_EINVAL;
+        }
+        const unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+        mask_image_planes_kernel<<<g, 256, 0, s>>>(img, n, i1, i2);
+        IPH_TRY(launch_status("mask_image_planes_kernel"));
+        mask_template_planes_kernel<<<1, 256, 0, s>>>(tpl, mask, (int)nt, w1, w2);
+        IPH_TRY(launch_status("mask_template_planes_kernel"));
+        IPH_TRY(launch_match(i1, IPH_F32, w, h, w1, tw, th, IPH_CCORR, IPH_PATH_MFMA, out, bws,
+                             bw, s));
+        if (!nrm) return IPH_OK;
+        IPH_TRY(launch_match(i2, IPH_F32, w, h, w2, tw, th, IPH_CCORR, IPH_PATH_MFMA, imim, bws,
+                             bw, s));
+        const unsigned go = (unsigned)std::min<size_t>((no + 255) / 256, 8192);
+        mask_ccorr_normed_finish_kernel<<<go, 256, 0, s>>>(out, imim, tms, no);
+        return launch_status("mask_ccorr_normed_finish_kernel");
+    }
     dim3 grid((ow + 63) / 64, (oh + 3) / 4), block(64, 4);
 #define LAUNCH(M)                                                                            \
     match_masked_direct_kernel<M><<<grid, block, 0, s>>>(img, (int)w, tpl, mask, (int)tw,      \

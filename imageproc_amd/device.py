@@ -113,8 +113,10 @@ def match_template(image: torch.Tensor, template: torch.Tensor, method, path: st
 
 
 def match_template_with_mask(image: torch.Tensor, template: torch.Tensor, method,
-                             mask: torch.Tensor, out: torch.Tensor | None = None):
-    """match_template_with_mask on device uint8 tensors (template_matching.rs:125-140)."""
+                             mask: torch.Tensor, out: torch.Tensor | None = None,
+                             path: str = "auto"):
+    """match_template_with_mask on device uint8 tensors (template_matching.rs:125-140);
+    path as in template_matching.match_template_with_mask."""
     for t in (image, template, mask):
         if _code(t) != _TORCH_CODE[torch.uint8]:
             raise TypeError("match_template_with_mask takes GrayImages (uint8)")
@@ -124,11 +126,12 @@ def match_template_with_mask(image: torch.Tensor, template: torch.Tensor, method
     if out is None:
         out = torch.empty((max(h - th + 1, 0), max(w - tw + 1, 0)), dtype=torch.float32,
                           device=image.device)
-    wsb = lib().iph_dev_match_template_with_mask_workspace()
-    ws = _workspace("masked", wsb, image.device)
+    m = int(MatchTemplateMethod(method))
+    wsb = lib().iph_dev_match_template_with_mask_workspace(w, h, tw, th, m, _PATHS[path])
+    ws = _workspace(("masked", path), wsb, image.device)
     check(lib().iph_dev_match_template_with_mask(
-        image.data_ptr(), w, h, template.data_ptr(), tw, th, mask.data_ptr(), mw, mh,
-        int(MatchTemplateMethod(method)), out.data_ptr(), ws.data_ptr(), ws.numel(), _stream()))
+        image.data_ptr(), w, h, template.data_ptr(), tw, th, mask.data_ptr(), mw, mh, m,
+        _PATHS[path], out.data_ptr(), ws.data_ptr(), ws.numel(), _stream()))
     return out
 
 

@@ -71,9 +71,13 @@ def match_template_parallel(image, template, method: MatchTemplateMethod, path: 
     return match_template(image, template, method, path)
 
 
-def match_template_with_mask(image, template, method: MatchTemplateMethod, mask):
+def match_template_with_mask(image, template, method: MatchTemplateMethod, mask,
+                             path: str = "auto"):
     """template_matching.rs:125-140 (GrayImages; the masked methods of
-    :295-376, bit-identical to the reference). Panics if the mask is not the
+    :295-376). path "direct": the reference's operation order, bit-identical;
+    "mfma": CrossCorrelation(Normalized) as BF16x3 MFMA correlations of the
+    image and its square with T*M^2 and M^2 (templates up to 128 px); "auto":
+    "mfma" where it applies, else "direct". Panics if the mask is not the
     template's size, or the template is larger than the image."""
     img = _gray(image, "image")
     tpl = _gray(template, "template")
@@ -85,13 +89,14 @@ def match_template_with_mask(image, template, method: MatchTemplateMethod, mask)
     oh, ow = max(h - th + 1, 0), max(w - tw + 1, 0)
     out = np.empty((oh, ow), np.float32)
     check(lib().iph_match_template_with_mask(ptr(img), w, h, ptr(tpl), tw, th, ptr(msk), mw, mh,
-                                             m, ptr(out)))
+                                             m, _PATHS[path], ptr(out)))
     return out
 
 
-def match_template_with_mask_parallel(image, template, method: MatchTemplateMethod, mask):
+def match_template_with_mask_parallel(image, template, method: MatchTemplateMethod, mask,
+                                      path: str = "auto"):
     """template_matching.rs:142-163: the same GPU path."""
-    return match_template_with_mask(image, template, method, mask)
+    return match_template_with_mask(image, template, method, mask, path)
 
 
 @dataclass(frozen=True)

@@ -170,17 +170,22 @@ IPH_API iph_status iph_match_template(const void *image, int32_t p_type, uint32_
                                       uint32_t th, int32_t method, int32_t path, float *out);
 
 /* match_template_with_mask, template_matching.rs:125-140, and
- * match_template_with_mask_parallel, :142-163, over GrayImages: the masked
- * methods of :295-376 in the reference's operation order (bit-identical
- * f32). mask is mw x mh; unequal to the template ->
- * IPH_EINVAL "... the template and mask must be the same size ..."
- * (ImageTemplateMask::new, :430-437), checked before the template-vs-image
- * asserts. out holds (width-tw+1) x (height-th+1) f32. */
+ * match_template_with_mask_parallel, :142-163, over GrayImages (the masked
+ * methods of :295-376). path IPH_PATH_DIRECT: the reference's operation order
+ * (bit-identical f32); IPH_PATH_MFMA (CrossCorrelation and
+ * CrossCorrelationNormalized, templates up to 128x128): sum t i m^2 and
+ * sum (i m)^2 as BF16x3 MFMA correlations of the image and its square with
+ * T M^2 and M^2, then the reference's f32 finish (the SSE methods ->
+ * IPH_EUNSUPPORTED: their decomposed sum cancels at a match);
+ * IPH_PATH_AUTO: MFMA where it applies, else direct. mask is mw x mh; unequal
+ * to the template -> IPH_EINVAL "... the template and mask must be the same
+ * size ..." (ImageTemplateMask::new, :430-437), checked before the
+ * template-vs-image asserts. out holds (width-tw+1) x (height-th+1) f32. */
 IPH_API iph_status iph_match_template_with_mask(const uint8_t *image, uint32_t width,
                                                 uint32_t height, const uint8_t *tmpl,
                                                 uint32_t tw, uint32_t th, const uint8_t *mask,
                                                 uint32_t mw, uint32_t mh, int32_t method,
-                                                float *out);
+                                                int32_t path, float *out);
 
 /* find_extremes, template_matching.rs:479-511, for Image<Luma<u8>> and
  * Image<Luma<f32>>. Empty image -> IPH_EINVAL "image must be non-empty". */
@@ -256,14 +261,16 @@ IPH_API size_t iph_dev_sharpen_gaussian_workspace(uint32_t width, uint32_t heigh
 IPH_API iph_status iph_dev_sharpen_gaussian(const uint8_t *src, uint8_t *dst, uint32_t width,
                                             uint32_t height, float sigma, float amount,
                                             void *workspace, size_t workspace_bytes, void *stream);
-IPH_API size_t iph_dev_match_template_with_mask_workspace(void);
+IPH_API size_t iph_dev_match_template_with_mask_workspace(uint32_t width, uint32_t height,
+                                                          uint32_t tw, uint32_t th,
+                                                          int32_t method, int32_t path);
 IPH_API iph_status iph_dev_match_template_with_mask(const uint8_t *image, uint32_t width,
                                                     uint32_t height, const uint8_t *tmpl,
                                                     uint32_t tw, uint32_t th,
                                                     const uint8_t *mask, uint32_t mw,
-                                                    uint32_t mh, int32_t method, float *out,
-                                                    void *workspace, size_t workspace_bytes,
-                                                    void *stream);
+                                                    uint32_t mh, int32_t method, int32_t path,
+                                                    float *out, void *workspace,
+                                                    size_t workspace_bytes, void *stream);
 
 /* match_template on device memory. */
 IPH_API size_t iph_dev_match_template_workspace(int32_t p_type, uint32_t width,

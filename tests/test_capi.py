@@ -143,10 +143,11 @@ def test_masked_match_argument_errors_before_device():
     from imageproc_amd._lib import Panic, check, lib
 
     with pytest.raises(Panic, match="the template and mask must be the same size"):
-        check(lib().iph_match_template_with_mask(None, 4, 4, None, 2, 2, None, 3, 2, 0, None))
+        check(lib().iph_match_template_with_mask(None, 4, 4, None, 2, 2, None, 3, 2, 0, 0, None))
     with pytest.raises(Panic, match="image height must be greater than or equal"):
-        check(lib().iph_match_template_with_mask(None, 4, 1, None, 2, 2, None, 2, 2, 0, None))
-    assert lib().iph_dev_match_template_with_mask_workspace() >= 4
+        check(lib().iph_match_template_with_mask(None, 4, 1, None, 2, 2, None, 2, 2, 0, 0, None))
+    assert lib().iph_dev_match_template_with_mask_workspace(64, 64, 8, 8, 3, 2) >= 4
+    assert lib().iph_dev_match_template_with_mask_workspace(64, 64, 8, 8, 3, 1) >= 64 * 64 * 8
 
 
 def test_adaptive_threshold_argument_errors_before_device():
