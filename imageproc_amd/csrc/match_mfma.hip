@@ -291,7 +291,7 @@ __global__ __launch_bounds__(256) void match_bf16x3_mfma_kernel(
         }
     }
 
-    const double stt = tsq[0];
+    const double stt = METHOD == IPH_CCORR ? 0.0 : tsq[0];  // CCORR needs no sum T^2
     const long long nn = (long long)tw * th;
 #pragma unroll
     for (int t = 0; t < 2; t++) {
@@ -936,8 +936,10 @@ iph_status launch_bf16x3_match(const float *img, uint32_t w, uint32_t h, const f
         return IPH_EINVAL;
     }
     int ow = (int)(w - tw + 1), oh = (int)(h - th + 1);
-    template_sq_f32_kernel<<<1, 256, 0, s>>>(tpl, (int)(tw * th), tsq);
-    IPH_TRY(launch_status("template_sq_f32_kernel"));
+    if (method != IPH_CCORR) {
+        template_sq_f32_kernel<<<1, 256, 0, s>>>(tpl, (int)(tw * th), tsq);
+        IPH_TRY(launch_status("template_sq_f32_kernel"));
+    }
     if (method != IPH_CCORR) {
         dim3 eg((ow + kWsqCols - 1) / kWsqCols, (oh + kWsqRows - 1) / kWsqRows);
         size_t lds = (size_t)(kWsqCols + tw - 1) * sizeof(double);
