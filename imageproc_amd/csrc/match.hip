@@ -19,20 +19,32 @@
 namespace iph {
 namespace {
 
+constexpr int kSumChunk = 4096;  // products staged per round of the sequential sums
+
 __device__ __forceinline__ float fmul(float a, float b) { return __fmul_rn(a, b); }
 __device__ __forceinline__ float fadd(float a, float b) { return __fadd_rn(a, b); }
 __device__ __forceinline__ float fsub(float a, float b) { return __fsub_rn(a, b); }
 
 // square_sum, template_matching.rs:378-380: sequential f32 sum of t*t.
 template <typename P>
-__global__ void square_sum_kernel(const P *__restrict__ tpl, size_t n, float *__restrict__ out) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+__global__ __launch_bounds__(256) void square_sum_kernel(const P *__restrict__ tpl, size_t n,
+                                                        float *__restrict__ out) {
+    // the products in parallel (each rounded once, as in the reference), then
+    // their sequential sum from LDS, kSumChunk at a time
+    __shared__ float sq[kSumChunk];
     float s = -0.0f;
-    for (size_t i = 0; i < n; i++) {
-        float t = (float)tpl[i];
-        s = fadd(s, fmul(t, t));
+    for (size_t base = 0; base < n; base += kSumChunk) {
+        const int m = (int)min((size_t)kSumChunk, n - base);
+        for (int i = (int)threadIdx.x; i < m; i += (int)blockDim.x) {
+            const float t = (float)tpl[base + i];
+            sq[i] = fmul(t, t);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int i = 0; i < m; i++) s = fadd(s, sq[i]);
+        __syncthreads();
     }
-    *out = s;
+    if (threadIdx.x == 0) *out = s;
 }
 
 // One output per thread, reference operation order. The template element
@@ -76,7 +88,7 @@ template <typename P>
 iph_status direct_typed(const P *img, uint32_t w, const P *tpl, uint32_t tw, uint32_t th,
                         int32_t method, float *out, uint32_t ow, uint32_t oh, float *tss,
                         hipStream_t s) {
-    square_sum_kernel<P><<<1, 64, 0, s>>>(tpl, (size_t)tw * th, tss);
+    square_sum_kernel<P><<<1, 256, 0, s>>>(tpl, (size_t)tw * th, tss);
     IPH_TRY(launch_status("square_sum_kernel"));
     dim3 grid((ow + 63) / 64, (oh + 3) / 4), block(64, 4);
 #define LAUNCH(M)                                                                            \
@@ -105,16 +117,25 @@ iph_status direct_typed(const P *img, uint32_t w, const P *tpl, uint32_t tw, uin
 
 // mult_square_sum, template_matching.rs:381-386: sequential f32 sum of
 // (t * m)^2 (Rust's f32 Sum starts from -0.0).
-__global__ void mult_square_sum_kernel(const uint8_t *__restrict__ tpl,
-                                       const uint8_t *__restrict__ mask, size_t n,
-                                       float *__restrict__ out) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+__global__ __launch_bounds__(256) void mult_square_sum_kernel(const uint8_t *__restrict__ tpl,
+                                                             const uint8_t *__restrict__ mask,
+                                                             size_t n, float *__restrict__ out) {
+    // (t * m)^2 in parallel (each product rounded as in the reference), then
+    // the sequential sum from LDS
+    __shared__ float sq[kSumChunk];
     float s = -0.0f;
-    for (size_t k = 0; k < n; k++) {
-        const float q = fmul((float)tpl[k], (float)mask[k]);
-        s = fadd(s, fmul(q, q));
+    for (size_t base = 0; base < n; base += kSumChunk) {
+        const int m = (int)min((size_t)kSumChunk, n - base);
+        for (int i = (int)threadIdx.x; i < m; i += (int)blockDim.x) {
+            const float q = fmul((float)tpl[base + i], (float)mask[base + i]);
+            sq[i] = fmul(q, q);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int i = 0; i < m; i++) s = fadd(s, sq[i]);
+        __syncthreads();
     }
-    *out = s;
+    if (threadIdx.x == 0) *out = s;
 }
 
 template <int METHOD>
@@ -419,7 +440,7 @@ iph_status launch_masked_match(const uint8_t *img, uint32_t w, uint32_t h, const
     }
     const uint32_t ow = w - tw + 1, oh = h - th + 1;
     if ((size_t)ow * oh == 0) return IPH_OK;
-    mult_square_sum_kernel<<<1, 64, 0, s>>>(tpl, mask, (size_t)tw * th, tms);
+    mult_square_sum_kernel<<<1, 256, 0, s>>>(tpl, mask, (size_t)tw * th, tms);
     IPH_TRY(launch_status("mult_square_sum_kernel"));
     if (path == IPH_PATH_MFMA) {
         const bool nrm = method == IPH_CCORR_NORMED;
