@@ -62,6 +62,19 @@ __device__ __forceinline__ void split_bf16(float x, uint16_t &hi, uint16_t &lo) 
     lo = bf16_bits(x - bf16_value(hi));
 }
 
+// The same split for two values on the conversion unit (v_cvt_pk_bf16_f32
+// rounds to nearest even like bf16_bits); x - hi is exact in f32. Returns
+// the packed hi pair and writes the packed lo pair.
+__device__ __forceinline__ uint32_t split2_bf16(float x0, float x1, uint32_t &lo) {
+    typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+    const bf2 h = {(__bf16)x0, (__bf16)x1};
+    const uint32_t hu = __builtin_bit_cast(uint32_t, h);
+    const bf2 l = {(__bf16)(x0 - __uint_as_float(hu << 16)),
+                   (__bf16)(x1 - __uint_as_float(hu & 0xffff0000u))};
+    lo = __builtin_bit_cast(uint32_t, l);
+    return hu;
+}
+
 // sum T^2 of the template in f64 (single block tree reduction).
 __global__ __launch_bounds__(256) void template_sq_f32_kernel(const float *__restrict__ tpl, int n,
                                                               double *__restrict__ out) {
@@ -171,16 +184,20 @@ __device__ __forceinline__ float finish_score_f32(float c, double sii, double st
 
 // BF16x3 correlation. A workgroup of kBfWaves waves side by side computes 32
 // output rows by kBfWaves * 64 columns (each wave two 32x32 tiles). Template
-// rows are taken in chunks of C: per chunk the workgroup stages the
-// (32 + C - 1) image rows it needs as bf16 hi and lo planes, and the chunk's
-// template rows as bf16 hi and lo rows padded with 32 zeros on the left (and
-// zeros to the right). Per template row dy and K block kb, each lane cuts its
+// rows are taken in chunks of C. The image rows live in a ring of 32 + C - 1
+// rows of bf16 hi and lo planes (image row r in slot r mod (32 + C - 1)):
+// the first chunk stages all the rows it meets, every later chunk only its C
+// new ones, over the slots of the rows the previous chunk was the last to
+// use. Per chunk the template rows are staged as bf16 hi and lo rows padded
+// with 32 zeros on the left (and zeros to the right). Per template row dy and K block kb, each lane cuts its
 // Toeplitz slice B[k][n] = T[dy][16 kb + k - n], k = 8 g + 0..7, from the
 // padded row: 8 consecutive bf16 starting at element 32 + 16 kb + 8 g - n,
 // read as 5 dwords and realigned by v_alignbyte (the shift, 0 or 2 bytes, is
 // the lane's column parity). Each slice pair feeds both tiles (6 MFMAs).
 constexpr int kBfWaves = 4;
 constexpr int kBfWaveCols = 64;
+constexpr int kBfTiles = kBfWaveCols / 32;  // 32x32 output tiles per wave
+constexpr int kBfThreads = 64 * kBfWaves;
 
 __host__ __device__ constexpr int bf_img_cols(int KB) { return kBfWaves * kBfWaveCols + 16 * (KB - 2); }
 // staged image row pitch in bf16: an odd multiple of 8 (16 bytes), so the 16-byte
@@ -209,53 +226,58 @@ __device__ __forceinline__ v8bf bf_slice(const uint16_t *row, int e) {
 }
 
 template <int KB, int METHOD>
-__global__ __launch_bounds__(256) void match_bf16x3_mfma_kernel(
+__global__ __launch_bounds__(kBfThreads) void match_bf16x3_mfma_kernel(
     const float *__restrict__ img, int W, int H, const float *__restrict__ tpl,
     const double *__restrict__ wsq, const double *__restrict__ tsq, float *__restrict__ out,
     int ow, int oh, int tw, int th, int C) {
     constexpr int BW = bf_img_cols(KB), PI = bf_img_pitch(KB), PT = bf_tpl_pitch(KB);
     extern __shared__ __align__(16) unsigned char smem[];
-    const int rows = 32 + C - 1;
+    const int R = 32 + C - 1;  // ring rows
     uint16_t *ih = reinterpret_cast<uint16_t *>(smem);
-    uint16_t *il = ih + (size_t)rows * PI;
-    uint16_t *th_s = il + (size_t)rows * PI;
+    uint16_t *il = ih + (size_t)R * PI;
+    uint16_t *th_s = il + (size_t)R * PI;
     uint16_t *tl_s = th_s + (size_t)C * PT;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int n = lane & 31, g = lane >> 5;
     const int x0 = blockIdx.x * (kBfWaves * kBfWaveCols), y0 = blockIdx.y * 32;
     const bool vec = (W & 3) == 0;
 
-    v16f acc[2];
+    v16f acc[kBfTiles];
 #pragma unroll
-    for (int t = 0; t < 2; t++)
+    for (int t = 0; t < kBfTiles; t++)
 #pragma unroll
         for (int r = 0; r < 16; r++) acc[t][r] = 0.0f;
 
+    int base = 0;  // ring slot of image row c0 (relative to y0)
     for (int c0 = 0; c0 < th; c0 += C) {
+        const int dend = min(C, th - c0);
         __syncthreads();  // the previous chunk's operands are consumed
-        // image rows y0 + c0 + [0, rows), columns x0 + [0, BW), 4 per thread
-        for (int i = threadIdx.x; i < rows * (BW / 4); i += 256) {
-            const int r = i / (BW / 4), c = 4 * (i - r * (BW / 4));
-            const int gy = min(y0 + c0 + r, H - 1), gx = x0 + c;
+        // image rows y0 + [ra, c0 + dend + 31), columns x0 + [0, BW), 4 per
+        // thread: all of them for the first chunk, else the rows past the
+        // previous chunk's, into slots (ra mod R) + q, wrapped
+        const int ra = c0 == 0 ? 0 : c0 + 31;
+        const int nr = c0 + dend + 31 - ra;
+        const int sa = c0 == 0 ? 0 : (base + 31 < R ? base + 31 : base + 31 - R);
+        for (int i = threadIdx.x; i < nr * (BW / 4); i += kBfThreads) {
+            const int q = i / (BW / 4), c = 4 * (i - q * (BW / 4));
+            const int gy = min(y0 + ra + q, H - 1), gx = x0 + c;
+            const int slot = sa + q < R ? sa + q : sa + q - R;
             const float *src = img + (size_t)gy * W;
             float v[4];
             if (vec && gx + 3 < W) {
-                const float4 q = *reinterpret_cast<const float4 *>(src + gx);
-                v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+                const float4 f = *reinterpret_cast<const float4 *>(src + gx);
+                v[0] = f.x; v[1] = f.y; v[2] = f.z; v[3] = f.w;
             } else {
 #pragma unroll
                 for (int b = 0; b < 4; b++) v[b] = src[min(gx + b, W - 1)];
             }
-            uint16_t h4[4], l4[4];
-#pragma unroll
-            for (int b = 0; b < 4; b++) split_bf16(v[b], h4[b], l4[b]);
-            *reinterpret_cast<uint2 *>(ih + r * PI + c) =
-                make_uint2(h4[0] | ((uint32_t)h4[1] << 16), h4[2] | ((uint32_t)h4[3] << 16));
-            *reinterpret_cast<uint2 *>(il + r * PI + c) =
-                make_uint2(l4[0] | ((uint32_t)l4[1] << 16), l4[2] | ((uint32_t)l4[3] << 16));
+            uint32_t l0, l1;
+            const uint32_t h0 = split2_bf16(v[0], v[1], l0), h1 = split2_bf16(v[2], v[3], l1);
+            *reinterpret_cast<uint2 *>(ih + slot * PI + c) = make_uint2(h0, h1);
+            *reinterpret_cast<uint2 *>(il + slot * PI + c) = make_uint2(l0, l1);
         }
         // template rows c0 + [0, C): 32 zeros, the row, zeros to PT
-        for (int i = threadIdx.x; i < C * PT; i += 256) {
+        for (int i = threadIdx.x; i < C * PT; i += kBfThreads) {
             const int r = i / PT, e = i - r * PT, dx = e - 32;
             uint16_t h = 0, l = 0;
             if (c0 + r < th && dx >= 0 && dx < tw) split_bf16(tpl[(size_t)(c0 + r) * tw + dx], h, l);
@@ -263,11 +285,12 @@ __global__ __launch_bounds__(256) void match_bf16x3_mfma_kernel(
             tl_s[i] = l;
         }
         __syncthreads();
-        const int dend = min(C, th - c0);
-        const uint16_t *ah = ih + n * PI + kBfWaveCols * wave + 8 * g;
-        const uint16_t *al = il + n * PI + kBfWaveCols * wave + 8 * g;
         const int e0 = 32 + 8 * g - n;
+        const int s0 = base + n;  // ring slot of this lane's row at d = 0, before wrapping (< 2R)
         for (int d = 0; d < dend; d++) {
+            const int sl = s0 + d < R ? s0 + d : s0 + d - R;
+            const uint16_t *ah = ih + sl * PI + kBfWaveCols * wave + 8 * g;
+            const uint16_t *al = il + sl * PI + kBfWaveCols * wave + 8 * g;
             v8bf bh[KB], bl[KB];
 #pragma unroll
             for (int kb = 0; kb < KB; kb++) {
@@ -275,11 +298,11 @@ __global__ __launch_bounds__(256) void match_bf16x3_mfma_kernel(
                 bl[kb] = bf_slice(tl_s + d * PT, e0 + 16 * kb);
             }
 #pragma unroll
-            for (int sblk = 0; sblk < KB + 2; sblk++) {
-                const v8bf a_h = *reinterpret_cast<const v8bf *>(ah + d * PI + 16 * sblk);
-                const v8bf a_l = *reinterpret_cast<const v8bf *>(al + d * PI + 16 * sblk);
+            for (int sblk = 0; sblk < KB + 2 * (kBfTiles - 1); sblk++) {
+                const v8bf a_h = *reinterpret_cast<const v8bf *>(ah + 16 * sblk);
+                const v8bf a_l = *reinterpret_cast<const v8bf *>(al + 16 * sblk);
 #pragma unroll
-                for (int t = 0; t < 2; t++) {
+                for (int t = 0; t < kBfTiles; t++) {
                     const int kb = sblk - 2 * t;
                     if (kb >= 0 && kb < KB) {
                         acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_l, bh[kb], acc[t], 0, 0, 0);
@@ -289,12 +312,13 @@ __global__ __launch_bounds__(256) void match_bf16x3_mfma_kernel(
                 }
             }
         }
+        base = base + C < R ? base + C : base + C - R;
     }
 
     const double stt = METHOD == IPH_CCORR ? 0.0 : tsq[0];  // CCORR needs no sum T^2
     const long long nn = (long long)tw * th;
 #pragma unroll
-    for (int t = 0; t < 2; t++) {
+    for (int t = 0; t < kBfTiles; t++) {
         const int x = x0 + kBfWaveCols * wave + 32 * t + n;
         double sq[16];
 #pragma unroll
@@ -327,7 +351,7 @@ iph_status launch_bf16_kb(const float *img, int W, int H, const float *tpl, cons
     const size_t lds = bf_lds(KB, C);
     dim3 grid((ow + kBfWaves * kBfWaveCols - 1) / (kBfWaves * kBfWaveCols), (oh + 31) / 32);
 #define L(M)                                                                                 \
-    match_bf16x3_mfma_kernel<KB, M><<<grid, 64 * kBfWaves, lds, s>>>(img, W, H, tpl, wsq, tsq, \
+    match_bf16x3_mfma_kernel<KB, M><<<grid, kBfThreads, lds, s>>>(img, W, H, tpl, wsq, tsq, \
                                                                       out, ow, oh, tw, th, C)
     switch (method) {
     case IPH_SSE: L(IPH_SSE); break;
