@@ -184,8 +184,8 @@ __device__ __forceinline__ float finish_score_f32(float c, double sii, double st
 
 // BF16x3 correlation. A workgroup of kBfWaves waves side by side computes 32
 // output rows by kBfWaves * 64 columns (each wave two 32x32 tiles). Template
-// rows are taken in chunks of C. The image rows live in a ring of 32 + C - 1
-// rows of bf16 hi and lo planes (image row r in slot r mod (32 + C - 1)):
+// rows are taken in chunks of C. The image rows live in a ring of R >= 32 +
+// C - 1 rows of bf16 hi and lo planes (image row r in slot r mod R):
 // the first chunk stages all the rows it meets, every later chunk only its C
 // new ones, over the slots of the rows the previous chunk was the last to
 // use. Per chunk the template rows are staged as bf16 hi and lo rows padded
@@ -206,8 +206,12 @@ __host__ __device__ constexpr int bf_img_pitch(int KB) {
     return ((bf_img_cols(KB) / 8) % 2 == 0) ? bf_img_cols(KB) + 8 : bf_img_cols(KB);
 }
 __host__ __device__ constexpr int bf_tpl_pitch(int KB) { return 16 * KB + 64; }
+// ring rows for chunks of C template rows: at least 32 + C - 1, a multiple
+// of 8 so that the 8 rows one A-operand read meets keep distinct bank groups
+// across the wrap
+__host__ __device__ constexpr int bf_ring_rows(int C) { return (32 + C - 1 + 7) & ~7; }
 __host__ __device__ constexpr size_t bf_lds(int KB, int C) {
-    return (size_t)(32 + C - 1) * bf_img_pitch(KB) * 4 + (size_t)C * bf_tpl_pitch(KB) * 4;
+    return (size_t)bf_ring_rows(C) * bf_img_pitch(KB) * 4 + (size_t)C * bf_tpl_pitch(KB) * 4;
 }
 
 // 8 consecutive bf16 of a padded template row from element e (e >= 0):
@@ -232,7 +236,7 @@ __global__ __launch_bounds__(kBfThreads) void match_bf16x3_mfma_kernel(
     int ow, int oh, int tw, int th, int C) {
     constexpr int BW = bf_img_cols(KB), PI = bf_img_pitch(KB), PT = bf_tpl_pitch(KB);
     extern __shared__ __align__(16) unsigned char smem[];
-    const int R = 32 + C - 1;  // ring rows
+    const int R = bf_ring_rows(C);
     uint16_t *ih = reinterpret_cast<uint16_t *>(smem);
     uint16_t *il = ih + (size_t)R * PI;
     uint16_t *th_s = il + (size_t)R * PI;
