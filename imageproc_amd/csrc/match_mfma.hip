@@ -884,13 +884,13 @@ int fused_waves(size_t lds, int RT) {
 }
 
 // Template rows per chunk for RT waves: the fewest near-equal chunks whose
-// footprint allows 16 waves per CU (else 12, else 8); 0 if no chunk of at
+// footprint allows 12 waves per CU (else 8); 0 if no chunk of at
 // least kFusedMinChunk rows fits a workgroup. Every chunk restages the image
 // rows it meets, so fewer chunks save staging and horizontal sums. Chunks
 // are at most 32 RT + 1 rows (the staged rows then form at most 2 RT blocks
 // of 32 for the per-row window sums).
 int fused_tc(int KB, int th, int RT, bool sums) {
-    for (int waves : {16, 12, 8}) {
+    for (int waves : {12, 8}) {
         for (int nc = 1; nc <= th; nc++) {
             const int tc = (th + nc - 1) / nc;
             if (tc > 32 * RT + 1) continue;
@@ -904,8 +904,9 @@ int fused_tc(int KB, int th, int RT, bool sums) {
 }
 
 // Workgroup shape for one match: waves (32 output rows each) and chunk
-// height. Most waves per CU first, then the fewest staged image rows per
-// output row (taller tiles spread the th - 1 halo rows over more outputs).
+// height. Most waves per CU first, counting 12 or more as equal, then the
+// fewest staged image rows per output row (taller tiles spread the th - 1
+// halo rows over more outputs; fewer chunks restage fewer rows).
 void fused_shape(int KB, int th, bool sums, int &RT, int &tc) {
     RT = 0;
     tc = 0;
@@ -917,8 +918,9 @@ void fused_shape(int KB, int th, bool sums, int &RT, int &tc) {
         const int w = fused_waves(fused_lds(KB, 0, th, rt, sums, c), rt);
         const int nc = (th + c - 1) / c;
         const double rows = (double)nc * (32 * rt + c - 1) / (32.0 * rt);
-        if (w > best_w || (w == best_w && rows < best_rows - 1e-9)) {
-            best_w = w;
+        const int wc = std::min(w, 12);
+        if (wc > best_w || (wc == best_w && rows < best_rows - 1e-9)) {
+            best_w = wc;
             best_rows = rows;
             RT = rt;
             tc = c;
