@@ -51,7 +51,8 @@ struct FallbackBlur {
 // consecutive in the job table.
 struct MatchGroup {
     int kb, method, first, count, rt, rt_max;
-    int tc;  // template rows per chunk, at most
+    int waves;  // waves per CU the chunk height aims for (12 or 8)
+    int tc;     // template rows per chunk, at most
     long long ntiles;
     size_t lds;
     uint32_t max_tw, max_th;
@@ -159,7 +160,7 @@ iph_status enqueue_matches(iph_batch *b, hipStream_t s, bool spread = false) {
 // workgroup (its tile list, chunk height and LDS size).
 iph_status layout_match_group(iph_batch *b, MatchGroup &g) {
     g.ntiles = match_jobs_layout(b->mjobs.data() + g.first, g.count, g.rt);
-    g.tc = fused_match_tc(g.kb, g.max_th, g.rt, g.method);
+    g.tc = fused_match_tc(g.kb, g.max_th, g.rt, g.method, g.waves);
     g.lds = 0;
     for (int i = g.first; i < g.first + g.count; i++) {
         const MatchJob &j = b->mjobs[i];
@@ -172,12 +173,12 @@ iph_status layout_match_group(iph_batch *b, MatchGroup &g) {
     return IPH_OK;
 }
 
-// Waves per workgroup the autotuner chose for a shape class of match tables:
-// (K blocks, method, tallest and widest template rounded up to 16, log2 of
-// the map pixels).
+// Waves per workgroup and per CU the autotuner chose for a shape class of
+// match tables: (K blocks, method, tallest and widest template rounded up to
+// 16, log2 of the map pixels).
 using MatchClass = std::tuple<int, int, int, int, int>;
 std::mutex g_match_cache_mu;
-std::map<MatchClass, int> g_match_cache;
+std::map<MatchClass, std::pair<int, int>> g_match_cache;
 
 MatchClass match_class(const MatchGroup &g) {
     int lg = 0;
@@ -254,28 +255,37 @@ iph_status autotune(iph_batch *b, hipStream_t s) {
         IPH_TRY(upload_filter_table(b, best_v));
     }
     for (MatchGroup &g : b->mgroups) {
-        int best_r = g.rt_max;
+        int best_r = g.rt_max, best_w = 12;
         float best_ms = 1e30f;
         for (int rt : {8, 4, 2, 1}) {
             if (rt > g.rt_max) continue;
-            g.rt = rt;
-            IPH_TRY(layout_match_group(b, g));
-            float ms;
-            IPH_TRY(time_min(s, 3, [&] { return enqueue_match_group(b, g, s); }, &ms));
-            std::snprintf(line, sizeof line,
-                          "match table (kb %d, %d pairs, templates <= %ux%u) rows/workgroup "
-                          "%d: %.3f ms\n",
-                          g.kb, g.count, g.max_tw, g.max_th, 32 * rt, ms);
-            b->tuning += line;
-            if (ms < best_ms) {
-                best_ms = ms;
-                best_r = rt;
+            for (int waves : {12, 8}) {
+                // 8 only where it gives taller chunks than 12
+                if (waves == 8 && fused_match_tc(g.kb, g.max_th, rt, g.method, 8) ==
+                                      fused_match_tc(g.kb, g.max_th, rt, g.method, 12))
+                    continue;
+                g.rt = rt;
+                g.waves = waves;
+                IPH_TRY(layout_match_group(b, g));
+                float ms;
+                IPH_TRY(time_min(s, 3, [&] { return enqueue_match_group(b, g, s); }, &ms));
+                std::snprintf(line, sizeof line,
+                              "match table (kb %d, %d pairs, templates <= %ux%u) rows/workgroup "
+                              "%d, chunk %d: %.3f ms\n",
+                              g.kb, g.count, g.max_tw, g.max_th, 32 * rt, g.tc, ms);
+                b->tuning += line;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best_r = rt;
+                    best_w = waves;
+                }
             }
         }
         g.rt = best_r;
+        g.waves = best_w;
         IPH_TRY(layout_match_group(b, g));
         std::lock_guard<std::mutex> lock(g_match_cache_mu);
-        g_match_cache[match_class(g)] = best_r;
+        g_match_cache[match_class(g)] = {best_r, best_w};
     }
     // the tile kernel's persistent grid (the per-wave kernel's grid is fixed)
     if (std::any_of(b->blur_groups.begin(), b->blur_groups.end(),
@@ -359,8 +369,8 @@ iph_status plan(iph_batch *b, int32_t tune, hipStream_t s) {
         for (const Entry &e : entries) {
             const MatchSpec &m = b->match[e.idx];
             if (b->mgroups.empty() || b->mgroups.back().kb * 4 + b->mgroups.back().method != e.key)
-                b->mgroups.push_back({e.key / 4, e.key % 4, (int)b->mjobs.size(), 0, 8, 8, 32,
-                                      0, 0, 0u, 0u, 0.0});
+                b->mgroups.push_back({e.key / 4, e.key % 4, (int)b->mjobs.size(), 0, 8, 8, 12,
+                                      32, 0, 0, 0u, 0u, 0.0});
             MatchGroup &g = b->mgroups.back();
             MatchJob j{};
             j.img = m.img;
@@ -395,10 +405,14 @@ iph_status plan(iph_batch *b, int32_t tune, hipStream_t s) {
     // single call of the group's largest template would take
     for (MatchGroup &g : b->mgroups) {
         g.rt = std::min(g.rt_max, fused_match_rt_default(g.max_tw, g.max_th, g.method));
+        g.waves = 12;
         g.ntiles = match_jobs_layout(b->mjobs.data() + g.first, g.count, g.rt);
         std::lock_guard<std::mutex> lock(g_match_cache_mu);
         auto it = g_match_cache.find(match_class(g));
-        if (it != g_match_cache.end() && it->second <= g.rt_max) g.rt = it->second;
+        if (it != g_match_cache.end() && it->second.first <= g.rt_max) {
+            g.rt = it->second.first;
+            g.waves = it->second.second;
+        }
     }
     for (MatchGroup &g : b->mgroups) IPH_TRY(layout_match_group(b, g));
     // direct-path pairs to their streams: largest first, each to the least loaded

@@ -206,9 +206,10 @@ int fused_match_kb(uint32_t tw);
 // single call picks.
 int fused_match_rt_max(uint32_t tw, uint32_t th, int32_t method);
 int fused_match_rt_default(uint32_t tw, uint32_t th, int32_t method);
-// Template rows per chunk for templates up to th rows high (the footprint then
-// fits three workgroups per CU where it can), and the LDS bytes of one job.
-int fused_match_tc(int kb, uint32_t th, int rt, int32_t method);
+// Template rows per chunk for templates up to th rows high: the fewest
+// chunks whose footprint leaves min_waves (12 or 8) waves per CU, else 8;
+// and the LDS bytes of one job.
+int fused_match_tc(int kb, uint32_t th, int rt, int32_t method, int min_waves = 12);
 size_t fused_match_lds(int kb, uint32_t tw, uint32_t th, int rt, int32_t method, int tc);
 // Sets tiles_x and tile0 of n consecutive jobs for rt waves per workgroup;
 // returns the number of tiles.

@@ -884,13 +884,14 @@ int fused_waves(size_t lds, int RT) {
 }
 
 // Template rows per chunk for RT waves: the fewest near-equal chunks whose
-// footprint allows 12 waves per CU (else 8); 0 if no chunk of at
+// footprint allows `min_waves` (12 or 8) waves per CU (else 8); 0 if no chunk of at
 // least kFusedMinChunk rows fits a workgroup. Every chunk restages the image
 // rows it meets, so fewer chunks save staging and horizontal sums. Chunks
 // are at most 32 RT + 1 rows (the staged rows then form at most 2 RT blocks
 // of 32 for the per-row window sums).
-int fused_tc(int KB, int th, int RT, bool sums) {
+int fused_tc(int KB, int th, int RT, bool sums, int min_waves = 12) {
     for (int waves : {12, 8}) {
+        if (waves > min_waves) continue;
         for (int nc = 1; nc <= th; nc++) {
             const int tc = (th + nc - 1) / nc;
             if (tc > 32 * RT + 1) continue;
@@ -1054,8 +1055,8 @@ int fused_match_rt_default(uint32_t tw, uint32_t th, int32_t method) {
     return rt;
 }
 
-int fused_match_tc(int kb, uint32_t th, int rt, int32_t method) {
-    return fused_tc(kb, (int)th, rt, method != IPH_CCORR);
+int fused_match_tc(int kb, uint32_t th, int rt, int32_t method, int min_waves) {
+    return fused_tc(kb, (int)th, rt, method != IPH_CCORR, min_waves);
 }
 
 size_t fused_match_lds(int kb, uint32_t tw, uint32_t th, int rt, int32_t method, int tc) {
