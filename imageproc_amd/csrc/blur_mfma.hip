@@ -1596,12 +1596,31 @@ __device__ __forceinline__ void blur_u8_wave_strip(const WaveJob &jb, const floa
     const auto srs = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t *>(src), 0,
                                                        W * H, 0x00020000);
     const auto drs = __builtin_amdgcn_make_buffer_rsrc(dst, 0, W * H, 0x00020000);
+    // dword stores where dst or its rows are not 4-aligned: a descriptor from
+    // the 4-aligned address at or below dst (stores stay inside the image)
+    const uint32_t ibd = (uint32_t)reinterpret_cast<uintptr_t>(dst) & 3u;
+    const auto drs4 = __builtin_amdgcn_make_buffer_rsrc(dst - ibd, 0,
+                                                        (int)(((uint32_t)(W * H) + ibd + 3u) & ~3u),
+                                                        0x00020000);
     // H(t)'s window, input rows yh0 + 32 t + [0, 32) by staged columns
     // xs + [0, CIN), is loaded as 16-byte row chunks (NLD per lane): chunk c
     // is row c / NKB, columns 16 (c % NKB) + [0, 16). Rows then take few
     // cache lines per load; the A operands are cut from the bytes in LDS.
     constexpr int NCH = 32 * NKB, NLD = (NCH + 63) / 64;
-    using Chunks = uint4[NLD];
+    // a chunk as loaded: four dwords (aligned rows), or five dwords from the
+    // 4-aligned address at or below its first byte and the byte shift that
+    // realigns them (rows that are not 4-aligned; shift 0 where the chunk
+    // was assembled byte by byte)
+    struct Chunks {
+        uint32_t w[NLD][5];
+        uint32_t sh[NLD];
+    };
+    // unaligned rows: dword loads through a descriptor whose base is the
+    // 4-aligned address at or below src, over every dword holding an image byte
+    const uint32_t ib = (uint32_t)reinterpret_cast<uintptr_t>(src) & 3u;
+    const auto srs4 = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t *>(src - ib), 0,
+                                                        (int)(((uint32_t)(W * H) + ib + 3u) & ~3u),
+                                                        0x00020000);
     auto load_h = [&](int t, Chunks &d) {
 #pragma unroll
         for (int k = 0; k < NLD; k++) {
@@ -1615,25 +1634,41 @@ __device__ __forceinline__ void blur_u8_wave_strip(const WaveJob &jb, const floa
                 // at times on gfx950 here, 8-byte and 4-byte loads never did
                 const auto v0 = __builtin_amdgcn_raw_buffer_load_b64(srs, row + x, 0, 0);
                 const auto v1 = __builtin_amdgcn_raw_buffer_load_b64(srs, row + x + 8, 0, 0);
-                d[k] = make_uint4(v0[0], v0[1], v1[0], v1[1]);
+                d.w[k][0] = v0[0];
+                d.w[k][1] = v0[1];
+                d.w[k][2] = v1[0];
+                d.w[k][3] = v1[1];
                 continue;
             }
-            uint32_t dw[4];
+            if (ALIGNED) {
 #pragma unroll
-            for (int p = 0; p < 4; p++) {
-                const int x4 = x + 4 * p;
-                if (ALIGNED) {
-                    dw[p] = __builtin_amdgcn_raw_buffer_load_b32(srs, row + clampi(x4, 0, W - 4), 0, 0);
-                } else {
-                    dw[p] = 0;
+                for (int p = 0; p < 4; p++)
+                    d.w[k][p] = __builtin_amdgcn_raw_buffer_load_b32(
+                        srs, row + clampi(x + 4 * p, 0, W - 4), 0, 0);
+                continue;
+            }
+            if (x >= 0 && x + 16 <= W) {
+                // inside the row: five aligned dwords, realigned when staged
+                const uint32_t o = (uint32_t)(row + x) + ib;
+                d.sh[k] = o & 3u;
+#pragma unroll
+                for (int p = 0; p < 5; p++)
+                    d.w[k][p] = __builtin_amdgcn_raw_buffer_load_b32(srs4, (int)((o & ~3u) + 4 * p), 0, 0);
+            } else {
+                // across an image edge: byte by byte from the clamped columns
+                d.sh[k] = 0;
+                d.w[k][4] = 0;
+#pragma unroll
+                for (int p = 0; p < 4; p++) {
+                    uint32_t v = 0;
 #pragma unroll
                     for (int b = 0; b < 4; b++)
-                        dw[p] |= (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(
-                                     srs, row + clampi(x4 + b, 0, W - 1), 0, 0)
-                                 << (8 * b);
+                        v |= (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(
+                                 srs, row + clampi(x + 4 * p + b, 0, W - 1), 0, 0)
+                             << (8 * b);
+                    d.w[k][p] = v;
                 }
             }
-            d[k] = make_uint4(dw[0], dw[1], dw[2], dw[3]);
         }
     };
     // the loaded chunks into in_w (at the image edges, aligned dwords left of
@@ -1645,7 +1680,11 @@ __device__ __forceinline__ void blur_u8_wave_strip(const WaveJob &jb, const floa
             const int c = lane + 64 * k;
             if (NCH % 64 != 0 && c >= NCH) break;
             const int r = c / NKB, q = c - r * NKB;
-            uint32_t dw[4] = {d[k].x, d[k].y, d[k].z, d[k].w};
+            uint32_t dw[4];
+#pragma unroll
+            for (int p = 0; p < 4; p++)
+                dw[p] = ALIGNED ? d.w[k][p]
+                                : __builtin_amdgcn_alignbyte(d.w[k][p + 1], d.w[k][p], d.sh[k]);
             if (ALIGNED && !interior) {
 #pragma unroll
                 for (int p = 0; p < 4; p++) {
@@ -1777,9 +1816,30 @@ __device__ __forceinline__ void blur_u8_wave_strip(const WaveJob &jb, const floa
 #pragma unroll
                 for (int q = 0; q < 4; q++)
                     if (gx + 4 * q < W) __builtin_amdgcn_raw_buffer_store_b32(dws[q], drs, o + 4 * q, 0, 0);
-            } else {
+            } else if constexpr (ALIGNED) {
+                // 4-aligned rows from a dst that is not (the register-lean
+                // byte loop keeps the aligned kernels free of spills)
                 const uint8_t *b = out_w + yl * PO + xl;
                 for (int q = 0; q < 16 && gx + q < W; q++) dst[o + q] = b[q];
+            } else {
+                // rows not 4-aligned: the bytes before the first 4-aligned
+                // address, whole dwords, then the bytes left (the dwords at
+                // a piece's ends are shared with the neighbouring piece, so
+                // those take byte stores)
+                const uint8_t *b = out_w + yl * PO + xl;
+                const int nb = min(16, W - gx);
+                const uint32_t a = (uint32_t)o + ibd;  // from the 4-aligned base dst - ibd
+                const int sh = (int)((4u - (a & 3u)) & 3u);
+                const int hb = min(sh, nb), nd = (nb - hb) >> 2;
+                const uint32_t w5[5] = {v.x, v.y, v.z, v.w, 0u};
+                for (int q = 0; q < hb; q++) dst[o + q] = b[q];
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (k < nd)
+                        __builtin_amdgcn_raw_buffer_store_b32(
+                            __builtin_amdgcn_alignbyte(w5[k + 1], w5[k], (uint32_t)sh), drs4,
+                            (int)(a + (uint32_t)sh + 4u * (uint32_t)k), 0, 0);
+                for (int q = hb + 4 * nd; q < nb; q++) dst[o + q] = b[q];
             }
         }
     };
