@@ -1647,17 +1647,21 @@ __device__ __forceinline__ void blur_u8_wave_strip(const WaveJob &jb, const floa
                         srs, row + clampi(x + 4 * p, 0, W - 4), 0, 0);
                 continue;
             }
+            // (the branches leave values, not stores: stores to the chunk
+            // array in both arms would be merged into one at a computed
+            // element, which moves the array to scratch memory)
+            uint32_t wv[5], shv;
             if (x >= 0 && x + 16 <= W) {
                 // inside the row: five aligned dwords, realigned when staged
                 const uint32_t o = (uint32_t)(row + x) + ib;
-                d.sh[k] = o & 3u;
+                shv = o & 3u;
 #pragma unroll
                 for (int p = 0; p < 5; p++)
-                    d.w[k][p] = __builtin_amdgcn_raw_buffer_load_b32(srs4, (int)((o & ~3u) + 4 * p), 0, 0);
+                    wv[p] = __builtin_amdgcn_raw_buffer_load_b32(srs4, (int)((o & ~3u) + 4 * p), 0, 0);
             } else {
                 // across an image edge: byte by byte from the clamped columns
-                d.sh[k] = 0;
-                d.w[k][4] = 0;
+                shv = 0;
+                wv[4] = 0;
 #pragma unroll
                 for (int p = 0; p < 4; p++) {
                     uint32_t v = 0;
@@ -1666,9 +1670,12 @@ __device__ __forceinline__ void blur_u8_wave_strip(const WaveJob &jb, const floa
                         v |= (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(
                                  srs, row + clampi(x + 4 * p + b, 0, W - 1), 0, 0)
                              << (8 * b);
-                    d.w[k][p] = v;
+                    wv[p] = v;
                 }
             }
+#pragma unroll
+            for (int p = 0; p < 5; p++) d.w[k][p] = wv[p];
+            d.sh[k] = shv;
         }
     };
     // the loaded chunks into in_w (at the image edges, aligned dwords left of
