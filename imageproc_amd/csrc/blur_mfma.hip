@@ -1716,6 +1716,14 @@ __device__ __forceinline__ void blur_u8_wave_strip(const WaveJob &jb, const floa
     // rows 8 (r / 4) + 4 g + r % 4 of the step. In a flagged pass only the
     // groups of four whose smallest fraction was below the margin are
     // checked again, from the accumulators (still live).
+    // Intermediate columns right of the image and output rows below it are
+    // never stored, so their flagged outputs are dropped before the exact
+    // chain: such columns repeat the edge byte and such rows the last row,
+    // and the blur of a constant run lands on an integer, so nearly all of
+    // them are flagged. (Interior strips hold no column right of the image;
+    // the V pass of such a column blurs the edge column's values, as near
+    // integers as any other.)
+    const bool hclip = MODE != kWaveInterior && x0 + n >= W;
     auto h_readout = [&](auto slot_c, const v16f &acc, const v16f &acl) {
         constexpr int SLOT = decltype(slot_c)::value;
         _Float16 *col = ring + n * RPH + 32 * SLOT;
@@ -1741,6 +1749,7 @@ __device__ __forceinline__ void blur_u8_wave_strip(const WaveJob &jb, const floa
                                   ? (1u << (4 * j + i)) : 0u;
                 }
             }
+            if (hclip) fl = 0;
             while (fl) {
                 const int r = __builtin_ctz(fl);
                 fl &= fl - 1u;
@@ -1789,6 +1798,7 @@ __device__ __forceinline__ void blur_u8_wave_strip(const WaveJob &jb, const floa
                                   ? (1u << (4 * j + i)) : 0u;
                 }
             }
+            if (yb0 + 32 * s + n >= yb1) fl = 0;
             while (fl) {
                 const int r = __builtin_ctz(fl);
                 fl &= fl - 1u;
