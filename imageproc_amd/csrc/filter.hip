@@ -500,6 +500,92 @@ __global__ __launch_bounds__(256) void box_cols_kernel(const uint8_t *__restrict
     box_line(mid + x, dst + x, (size_t)W, H, seg * kBoxSeg, ry, kh);
 }
 
+// One-kernel box filter for radii up to kBoxTileMaxR: a workgroup stages the
+// input rows and columns its 128 x 64 output tile meets (rows and columns
+// clamped, as the reference's running sums pad with the first and last
+// pixel), computes the tile's u8 row means for every staged row into LDS,
+// then the column means of those. Both passes slide exact integer window
+// sums (any order gives the same integers) and divide by multiplying with
+// ceil(2^32 / k): for sums below 2^16 and k <= 255 the high word is the
+// floor of sum / k. Same bytes as box_rows_* + box_cols_kernel.
+constexpr int kBoxTX = 128, kBoxTY = 64, kBoxRun = 8;
+constexpr int kBoxTileMaxR = 32;
+
+// staged bytes per row: the tile's columns and rx either side, a multiple of 4
+__host__ __device__ constexpr int box_tile_pin(int rx) { return (kBoxTX + 2 * rx + 3) & ~3; }
+__host__ __device__ constexpr size_t box_tile_lds(int rx, int ry) {
+    return (size_t)(kBoxTY + 2 * ry) * (size_t)(box_tile_pin(rx) + kBoxTX);
+}
+
+__device__ __forceinline__ uint32_t box_div(uint32_t s, uint32_t magic) {
+    return magic ? __umulhi(s, magic) : s;  // magic 0: k = 1
+}
+
+__global__ __launch_bounds__(256) void box_tile_kernel(const uint8_t *__restrict__ src,
+                                                       uint8_t *__restrict__ dst, int W, int H,
+                                                       int rx, int ry, uint32_t mw, uint32_t mh) {
+    extern __shared__ __align__(16) uint8_t box_tsm[];
+    const int x0 = (int)blockIdx.x * kBoxTX, y0 = (int)blockIdx.y * kBoxTY;
+    const int NR = kBoxTY + 2 * ry, PIN = box_tile_pin(rx);
+    uint8_t *in_s = box_tsm;               // NR x PIN: global columns cs + [0, PIN)
+    uint8_t *mid = box_tsm + NR * PIN;     // NR x kBoxTX: row means of the tile's columns
+    const int cs = max(0, x0 - rx);
+    // staging: four bytes per task from two aligned dwords (rows need not be
+    // 4-aligned); bytes past the image's last column are never read
+    const uint32_t ib = (uint32_t)reinterpret_cast<uintptr_t>(src) & 3u;
+    const auto rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t *>(src - ib), 0,
+                                                      (int)(((uint32_t)(W * H) + ib + 3u) & ~3u),
+                                                      0x00020000);
+    const int dpr = PIN >> 2;
+    for (int i = (int)threadIdx.x; i < NR * dpr; i += 256) {
+        const int k = i / dpr, d = i - k * dpr;
+        const int gy = min(max(y0 - ry + k, 0), H - 1);
+        const uint32_t a = (uint32_t)(gy * W + cs + 4 * d) + ib;
+        const uint32_t lo = __builtin_amdgcn_raw_buffer_load_b32(rs, (int)(a & ~3u), 0, 0);
+        const uint32_t hi = __builtin_amdgcn_raw_buffer_load_b32(rs, (int)((a & ~3u) + 4u), 0, 0);
+        *reinterpret_cast<uint32_t *>(in_s + k * PIN + 4 * d) =
+            __builtin_amdgcn_alignbyte(hi, lo, a & 3u);
+    }
+    __syncthreads();
+    // row means: task = (staged row, run of kBoxRun output columns)
+    auto col = [&](int g) { return min(max(g, 0), W - 1) - cs; };
+    constexpr int RPR = kBoxTX / kBoxRun;
+    for (int i = (int)threadIdx.x; i < NR * RPR; i += 256) {
+        const int k = i / RPR, c0 = (i - k * RPR) * kBoxRun;
+        const uint8_t *row = in_s + k * PIN;
+        const int g0 = x0 + c0;
+        uint32_t sum = 0;
+        for (int j = -rx; j <= rx; j++) sum += row[col(g0 + j)];
+        uint32_t b[kBoxRun];
+#pragma unroll
+        for (int c = 0; c < kBoxRun; c++) {
+            b[c] = box_div(sum, mw);
+            if (c + 1 < kBoxRun) {
+                sum += (uint32_t)row[col(g0 + c + rx + 1)];
+                sum -= (uint32_t)row[col(g0 + c - rx)];
+            }
+        }
+        *reinterpret_cast<uint2 *>(mid + k * kBoxTX + c0) =
+            make_uint2(b[0] | (b[1] << 8) | (b[2] << 16) | (b[3] << 24),
+                       b[4] | (b[5] << 8) | (b[6] << 16) | (b[7] << 24));
+    }
+    __syncthreads();
+    // column means: thread = (column, half of the tile's rows); staged row
+    // r + j holds the row means of global row clamp(y0 + r + j - ry)
+    const int c = (int)threadIdx.x & (kBoxTX - 1), r0 = ((int)threadIdx.x >> 7) * (kBoxTY / 2);
+    const int gx = x0 + c;
+    uint32_t sum = 0;
+    for (int j = 0; j <= 2 * ry; j++) sum += mid[(r0 + j) * kBoxTX + c];
+    const int rend = min(r0 + kBoxTY / 2, H - y0);
+    for (int r = r0; r < rend; r++) {
+        if (gx < W) dst[(size_t)(y0 + r) * W + gx] = (uint8_t)box_div(sum, mh);
+        if (r + 1 < rend) {
+            sum += (uint32_t)mid[(r + 2 * ry + 1) * kBoxTX + c];
+            sum -= (uint32_t)mid[r * kBoxTX + c];
+        }
+    }
+}
+
 // ---- host-side dispatch helpers -----------------------------------------------
 
 template <typename K> void copy_taps(K *dst, const void *src, uint32_t len) {
@@ -735,6 +821,17 @@ iph_status launch_box_filter(const uint8_t *src, uint8_t *dst, uint32_t w, uint3
         return IPH_EINVAL;
     }
     uint32_t kw = 2u * rx + 1u, kh = 2u * ry + 1u;  // box_filter.rs:26-27 (u32)
+    if (rx <= (uint32_t)kBoxTileMaxR && ry <= (uint32_t)kBoxTileMaxR &&
+        (uint64_t)w * h + 512u < (1ull << 31)) {
+        // ceil(2^32 / k), 0 for k = 1 (the quotient is the sum)
+        auto magic = [](uint32_t k) {
+            return k == 1u ? 0u : (uint32_t)(((1ull << 32) + k - 1u) / k);
+        };
+        dim3 grid((w + kBoxTX - 1) / kBoxTX, (h + kBoxTY - 1) / kBoxTY);
+        box_tile_kernel<<<grid, 256, box_tile_lds((int)rx, (int)ry), s>>>(
+            src, dst, (int)w, (int)h, (int)rx, (int)ry, magic(kw), magic(kh));
+        return launch_status("box_tile_kernel");
+    }
     const uint32_t nseg = (w + kBoxSeg - 1) / kBoxSeg;
     if (nseg <= (uint32_t)kBoxLdsMaxSeg) {
         const int rows_per_wg = (int)(256 / nseg);

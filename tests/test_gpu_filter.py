@@ -205,7 +205,12 @@ def test_box_filter_kat():
                                        # row, window sums that wrap u32; 257 segments take
                                        # the per-thread rows pass
                                        (9, 16384, 3, 2), (64, 1024, 5, 5), (33, 260, 100, 1),
-                                       (5, 100, (1 << 24) - 1, 1), (3, 16448, 2, 2)])
+                                       (5, 100, (1 << 24) - 1, 1), (3, 16448, 2, 2),
+                                       # one-kernel tiles (radii up to 32): partial tiles,
+                                       # rows that are not 4-aligned, the largest radii,
+                                       # a zero radius on either axis
+                                       (130, 1001, 32, 32), (257, 300, 0, 32),
+                                       (66, 129, 32, 0), (1000, 1003, 5, 5), (65, 4097, 31, 2)])
 def test_box_filter_random(rng, h, w, rx, ry):
     img = rng.integers(0, 256, (h, w), dtype=np.uint8)
     _same(flt.box_filter(img, rx, ry), O.box_filter(img, rx, ry))
