@@ -521,6 +521,7 @@ __device__ __forceinline__ uint32_t box_div(uint32_t s, uint32_t magic) {
     return magic ? __umulhi(s, magic) : s;  // magic 0: k = 1
 }
 
+template <bool VEC>
 __global__ __launch_bounds__(256) void box_tile_kernel(const uint8_t *__restrict__ src,
                                                        uint8_t *__restrict__ dst, int W, int H,
                                                        int rx, int ry, uint32_t mw, uint32_t mh) {
@@ -537,8 +538,15 @@ __global__ __launch_bounds__(256) void box_tile_kernel(const uint8_t *__restrict
                                                       (int)(((uint32_t)(W * H) + ib + 3u) & ~3u),
                                                       0x00020000);
     const int dpr = PIN >> 2;
-    for (int i = (int)threadIdx.x; i < NR * dpr; i += 256) {
-        const int k = i / dpr, d = i - k * dpr;
+    // task (k, d) = (staged row, dword), stepped by 256 tasks without a division
+    const int sk = 256 / dpr, sd = 256 - sk * dpr;
+    int k = (int)threadIdx.x / dpr, d = (int)threadIdx.x - k * dpr;
+    for (; k < NR; k += sk, d += sd) {
+        if (d >= dpr) {
+            d -= dpr;
+            k++;
+            if (k >= NR) break;
+        }
         const int gy = min(max(y0 - ry + k, 0), H - 1);
         const uint32_t a = (uint32_t)(gy * W + cs + 4 * d) + ib;
         const uint32_t lo = __builtin_amdgcn_raw_buffer_load_b32(rs, (int)(a & ~3u), 0, 0);
@@ -548,6 +556,8 @@ __global__ __launch_bounds__(256) void box_tile_kernel(const uint8_t *__restrict
     }
     __syncthreads();
     // row means: task = (staged row, run of kBoxRun output columns)
+    // (tiles whose windows stay inside the image read without clamping)
+    const bool inner = x0 - rx >= 0 && x0 + kBoxTX + rx <= W;
     auto col = [&](int g) { return min(max(g, 0), W - 1) - cs; };
     constexpr int RPR = kBoxTX / kBoxRun;
     for (int i = (int)threadIdx.x; i < NR * RPR; i += 256) {
@@ -555,14 +565,26 @@ __global__ __launch_bounds__(256) void box_tile_kernel(const uint8_t *__restrict
         const uint8_t *row = in_s + k * PIN;
         const int g0 = x0 + c0;
         uint32_t sum = 0;
-        for (int j = -rx; j <= rx; j++) sum += row[col(g0 + j)];
         uint32_t b[kBoxRun];
+        if (inner) {
+            const uint8_t *p = row + (g0 - cs);  // column g0
+#pragma unroll 4
+            for (int j = -rx; j <= rx; j++) sum += p[j];
 #pragma unroll
-        for (int c = 0; c < kBoxRun; c++) {
-            b[c] = box_div(sum, mw);
-            if (c + 1 < kBoxRun) {
-                sum += (uint32_t)row[col(g0 + c + rx + 1)];
-                sum -= (uint32_t)row[col(g0 + c - rx)];
+            for (int c = 0; c < kBoxRun; c++) {
+                b[c] = box_div(sum, mw);
+                if (c + 1 < kBoxRun) sum += (uint32_t)p[c + rx + 1] - (uint32_t)p[c - rx];
+            }
+        } else {
+#pragma unroll 4
+            for (int j = -rx; j <= rx; j++) sum += row[col(g0 + j)];
+#pragma unroll
+            for (int c = 0; c < kBoxRun; c++) {
+                b[c] = box_div(sum, mw);
+                if (c + 1 < kBoxRun) {
+                    sum += (uint32_t)row[col(g0 + c + rx + 1)];
+                    sum -= (uint32_t)row[col(g0 + c - rx)];
+                }
             }
         }
         *reinterpret_cast<uint2 *>(mid + k * kBoxTX + c0) =
@@ -575,14 +597,38 @@ __global__ __launch_bounds__(256) void box_tile_kernel(const uint8_t *__restrict
     const int c = (int)threadIdx.x & (kBoxTX - 1), r0 = ((int)threadIdx.x >> 7) * (kBoxTY / 2);
     const int gx = x0 + c;
     uint32_t sum = 0;
+#pragma unroll 4
     for (int j = 0; j <= 2 * ry; j++) sum += mid[(r0 + j) * kBoxTX + c];
     const int rend = min(r0 + kBoxTY / 2, H - y0);
+    if (!VEC) {
+#pragma unroll 8
+        for (int r = r0; r < rend; r++) {
+            if (gx < W) dst[(size_t)(y0 + r) * W + gx] = (uint8_t)box_div(sum, mh);
+            if (r + 1 < rend) {
+                sum += (uint32_t)mid[(r + 2 * ry + 1) * kBoxTX + c];
+                sum -= (uint32_t)mid[r * kBoxTX + c];
+            }
+        }
+        return;
+    }
+    // rows 16-aligned: the tile's output bytes into LDS (over the staged
+    // input, done with), then 16-byte row pieces out
+    uint8_t *out_s = in_s;
+#pragma unroll 8
     for (int r = r0; r < rend; r++) {
-        if (gx < W) dst[(size_t)(y0 + r) * W + gx] = (uint8_t)box_div(sum, mh);
+        out_s[r * kBoxTX + c] = (uint8_t)box_div(sum, mh);
         if (r + 1 < rend) {
             sum += (uint32_t)mid[(r + 2 * ry + 1) * kBoxTX + c];
             sum -= (uint32_t)mid[r * kBoxTX + c];
         }
+    }
+    __syncthreads();
+    const int nrows = min(kBoxTY, H - y0);
+    for (int i = (int)threadIdx.x; i < nrows * (kBoxTX / 16); i += 256) {
+        const int r = i >> 3, cx = (i & 7) * 16;
+        if (x0 + cx < W)
+            *reinterpret_cast<uint4 *>(dst + (size_t)(y0 + r) * W + x0 + cx) =
+                *reinterpret_cast<const uint4 *>(out_s + r * kBoxTX + cx);
     }
 }
 
@@ -828,8 +874,13 @@ iph_status launch_box_filter(const uint8_t *src, uint8_t *dst, uint32_t w, uint3
             return k == 1u ? 0u : (uint32_t)(((1ull << 32) + k - 1u) / k);
         };
         dim3 grid((w + kBoxTX - 1) / kBoxTX, (h + kBoxTY - 1) / kBoxTY);
-        box_tile_kernel<<<grid, 256, box_tile_lds((int)rx, (int)ry), s>>>(
-            src, dst, (int)w, (int)h, (int)rx, (int)ry, magic(kw), magic(kh));
+        const size_t lds = box_tile_lds((int)rx, (int)ry);
+        if (w % 16 == 0 && (reinterpret_cast<uintptr_t>(dst) & 15) == 0)
+            box_tile_kernel<true><<<grid, 256, lds, s>>>(src, dst, (int)w, (int)h, (int)rx,
+                                                         (int)ry, magic(kw), magic(kh));
+        else
+            box_tile_kernel<false><<<grid, 256, lds, s>>>(src, dst, (int)w, (int)h, (int)rx,
+                                                          (int)ry, magic(kw), magic(kh));
         return launch_status("box_tile_kernel");
     }
     const uint32_t nseg = (w + kBoxSeg - 1) / kBoxSeg;
