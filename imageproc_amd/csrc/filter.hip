@@ -125,6 +125,7 @@ __device__ __forceinline__ void stage_window_f32(const uint8_t *__restrict__ img
 constexpr int kFusedMaxTaps = 48;
 constexpr int kSepTW = 64;
 constexpr int kSepTH = 32;
+constexpr int kSepRun = 8;  // outputs per task with compile-time taps
 
 template <typename K> struct FusedTaps {
     K h[kFusedMaxTaps];
@@ -153,6 +154,46 @@ __global__ __launch_bounds__(256) void sep_fused_kernel(const P *__restrict__ sr
 
     stage_window(img, W, H, y0 - half, x0 - half, R, C, in_s);
     __syncthreads();
+
+    if constexpr (LT > 0) {
+        // compile-time taps: each task computes kSepRun outputs from the
+        // kSepRun + L - 1 values it loads once (the same products and sums,
+        // in the same order, as one output per thread)
+        constexpr int NV = kSepRun + LT - 1;
+        constexpr int HT = kSepTW / kSepRun;  // tasks per row
+        for (int i = threadIdx.x; i < R * HT; i += blockDim.x) {
+            const int r = i / HT, c0 = (i - r * HT) * kSepRun;
+            const P *row = in_s + r * C + c0;
+            P v[NV];
+#pragma unroll
+            for (int j = 0; j < NV; j++) v[j] = row[j];
+#pragma unroll
+            for (int o = 0; o < kSepRun; o++) {
+                K acc = K(0);
+#pragma unroll
+                for (int t = 0; t < LT; t++) acc = mac(acc, to_acc(v[o + t], K(0)), taps.h[t]);
+                mid_s[r * kSepTW + c0 + o] = ClampTo<P>::from(acc);
+            }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < (kSepTH / kSepRun) * kSepTW; i += blockDim.x) {
+            const int rb = i / kSepTW, c = i - rb * kSepTW;
+            const int r0 = rb * kSepRun, gx = x0 + c;
+            const P *col = mid_s + r0 * kSepTW + c;
+            P v[NV];
+#pragma unroll
+            for (int j = 0; j < NV; j++) v[j] = col[j * kSepTW];
+#pragma unroll
+            for (int o = 0; o < kSepRun; o++) {
+                K acc = K(0);
+#pragma unroll
+                for (int t = 0; t < LT; t++) acc = mac(acc, to_acc(v[o + t], K(0)), taps.v[t]);
+                const int gy = y0 + r0 + o;
+                if (gy < H && gx < W) out[(size_t)gy * W + gx] = ClampTo<P>::from(acc);
+            }
+        }
+        return;
+    }
 
     for (int i = threadIdx.x; i < R * kSepTW; i += blockDim.x) {
         int r = i / kSepTW, c = i - r * kSepTW;
