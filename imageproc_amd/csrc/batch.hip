@@ -400,7 +400,7 @@ iph_status plan(iph_batch *b, int32_t tune, hipStream_t s) {
     }
     IPH_TRY_HIP(hipMemcpy(b->mem + b->blur_table_off, blur_host.data(), blur_host.size(),
                           hipMemcpyHostToDevice));
-    IPH_TRY(upload_filter_table(b, 0));
+    IPH_TRY(upload_filter_table(b, 1));  // 64x32 tiles until the autotuner says otherwise
     // match tables: the cached choice for their shape class, else the shape a
     // single call of the group's largest template would take
     for (MatchGroup &g : b->mgroups) {

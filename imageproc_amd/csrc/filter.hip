@@ -299,13 +299,15 @@ __global__ __launch_bounds__(256) void filter2d_tiled_kernel(const P *__restrict
 // write a row's 64 outputs as one contiguous run.
 constexpr int kF2STW = 64, kF2STH = 32, kF2SP = 68;
 
-template <int KW, int KH>
-__global__ __launch_bounds__(256) void filter2d_small_kernel(const uint8_t *__restrict__ src,
-                                                             uint8_t *__restrict__ dst, int W,
-                                                             int H, Taps<float> k) {
+// One 64 x 32 output tile of filter_clamped<u8, f32, u8> with a KW x KH
+// kernel (the reference's ky-outer, kx-inner f32 chain, each product and sum
+// rounded): the window staged as f32 in `win`, eight outputs per thread.
+// tap(q) returns weight q (row-major).
+template <int KW, int KH, typename TapF>
+__device__ __forceinline__ void f2s_tile(const uint8_t *__restrict__ src,
+                                         uint8_t *__restrict__ dst, int W, int H, int x0,
+                                         int y0, float *win, TapF tap) {
     static_assert(kF2STW + KW - 1 <= kF2SP, "the window row fits the pitch");
-    __shared__ __align__(16) float win[(kF2STH + KH - 1) * kF2SP];
-    const int x0 = blockIdx.x * kF2STW, y0 = blockIdx.y * kF2STH;
     const int ox = x0 - KW / 2, oy = y0 - KH / 2;  // filter/mod.rs:50,53
     constexpr int R = kF2STH + KH - 1;
     for (int i = threadIdx.x; i < R * kF2SP; i += 256) {
@@ -329,7 +331,7 @@ __global__ __launch_bounds__(256) void filter2d_small_kernel(const uint8_t *__re
 #pragma unroll
         for (int o = 0; o < 8; o++)
 #pragma unroll
-            for (int kx = 0; kx < KW; kx++) acc[o] = mac(acc[o], v[o + kx], k.w[ky * KW + kx]);
+            for (int kx = 0; kx < KW; kx++) acc[o] = mac(acc[o], v[o + kx], tap(ky * KW + kx));
     }
     const int gy = y0 + r, gx = x0 + 8 * cg;
     if (gy >= H || gx >= W) return;
@@ -344,6 +346,15 @@ __global__ __launch_bounds__(256) void filter2d_small_kernel(const uint8_t *__re
     } else {
         for (int o = 0; o < 8 && gx + o < W; o++) o8[o] = b[o];
     }
+}
+
+template <int KW, int KH>
+__global__ __launch_bounds__(256) void filter2d_small_kernel(const uint8_t *__restrict__ src,
+                                                             uint8_t *__restrict__ dst, int W,
+                                                             int H, Taps<float> k) {
+    __shared__ __align__(16) float win[(kF2STH + KH - 1) * kF2SP];
+    f2s_tile<KW, KH>(src, dst, W, H, blockIdx.x * kF2STW, blockIdx.y * kF2STH, win,
+                     [&](int q) { return k.w[q]; });
 }
 
 template <typename P, typename K, typename S>
@@ -399,6 +410,48 @@ __global__ __launch_bounds__(256) void filter2d_jobs_kernel(const F2Job *__restr
     __syncthreads();
     for (int i = threadIdx.x; i < TH * TW; i += blockDim.x) {
         int r = i / TW, c = i - r * TW;
+        int gy = y0 + r, gx = x0 + c;
+        if (gy >= H || gx >= W) continue;
+        float acc = 0.0f;
+        int q = 0;
+        for (int ky = 0; ky < kh; ky++) {
+            const uint8_t *row = in_s + (r + ky) * C + c;
+            for (int kx = 0; kx < kw; kx++, q++) acc = mac(acc, to_acc(row[kx], 0.0f), jb.w[q]);
+        }
+        jb.dst[(size_t)gy * W + gx] = ClampTo<uint8_t>::from(acc);
+    }
+}
+
+// The job table of 64 x 32 tiles when every job's kernel is at most KW x KH:
+// jobs of exactly KW x KH take f2s_tile, the others the generic loop over a
+// u8 window (the same arithmetic either way).
+template <int KW, int KH>
+__global__ __launch_bounds__(256) void filter2d_jobs_small_kernel(const F2Job *__restrict__ jobs,
+                                                                  int njobs) {
+    static_assert((kF2STH + KH - 1) * (kF2STW + KW - 1) <= (kF2STH + KH - 1) * kF2SP * 4,
+                  "a u8 window fits the f32 window's LDS");
+    __shared__ __align__(16) float win[(kF2STH + KH - 1) * kF2SP];
+    const int t = (int)blockIdx.x;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].tile0 <= t) lo = mid; else hi = mid - 1;
+    }
+    const F2Job &jb = jobs[lo];
+    const int lt = t - jb.tile0, ty = lt / jb.tiles_x, tx = lt - ty * jb.tiles_x;
+    const int x0 = tx * kF2STW, y0 = ty * kF2STH;
+    if (jb.kw == KW && jb.kh == KH) {
+        f2s_tile<KW, KH>(jb.src, jb.dst, jb.W, jb.H, x0, y0, win,
+                         [&](int q) { return jb.w[q]; });
+        return;
+    }
+    const int W = jb.W, H = jb.H, kw = jb.kw, kh = jb.kh;
+    const int R = kF2STH + kh - 1, C = kF2STW + kw - 1;
+    uint8_t *in_s = reinterpret_cast<uint8_t *>(win);
+    stage_window(jb.src, W, H, y0 - kh / 2, x0 - kw / 2, R, C, in_s);
+    __syncthreads();
+    for (int i = threadIdx.x; i < kF2STH * kF2STW; i += blockDim.x) {
+        int r = i / kF2STW, c = i - r * kF2STW;
         int gy = y0 + r, gx = x0 + c;
         if (gy >= H || gx >= W) continue;
         float acc = 0.0f;
@@ -1004,6 +1057,16 @@ iph_status launch_filter_jobs(const void *table_dev, int n, int total_tiles, int
         size_t lds = (size_t)(TH + max_kh - 1) * (TW + max_kw - 1);                          \
         filter2d_jobs_kernel<TW, TH><<<total_tiles, 256, lds, s>>>(jobs, n);                 \
     } while (0)
+    // 64 x 32 tiles with kernels of at most 3x3 or 5x5: eight outputs per thread
+    // from an f32 window for the jobs of exactly that size
+    static_assert(kF2STW == 64 && kF2STH == 32, "variant 1's tiles are f2s_tile's");
+    if (variant == 1 && max_kw <= 5 && max_kh <= 5) {
+        if (max_kw <= 3 && max_kh <= 3)
+            filter2d_jobs_small_kernel<3, 3><<<total_tiles, 256, 0, s>>>(jobs, n);
+        else
+            filter2d_jobs_small_kernel<5, 5><<<total_tiles, 256, 0, s>>>(jobs, n);
+        return launch_status("filter2d_jobs_small_kernel");
+    }
     switch (variant) {
     case 0: FV(64, 16); break;
     case 1: FV(64, 32); break;

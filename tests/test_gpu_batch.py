@@ -114,6 +114,29 @@ def test_batch_jobs_equal_oracle():
     bt.close()
 
 
+@pytest.mark.parametrize("shapes", [[(3, 3), (5, 5), (1, 5), (5, 3), (2, 4)], [(3, 3), (1, 3), (3, 1)]])
+def test_batch_filter_kernel_sizes_equal_oracle(shapes):
+    """Batched filter_clamped with kernels of several sizes in one table: the
+    jobs of the table's largest size (5x5 or 3x3) take the eight-outputs-per-
+    thread path, the others the generic one; every output equals the oracle."""
+    rng = np.random.default_rng(21)
+    bt = Batch()
+    cases = []
+    for i, (kh, kw) in enumerate(shapes):
+        h, w = int(rng.integers(20, 150)), int(rng.integers(20, 150))
+        img = torch.from_numpy(rng.integers(0, 256, (h, w), dtype=np.uint8)).cuda()
+        k = rng.normal(0.0, 0.3, (kh, kw)).astype(np.float32)
+        cases.append((img, k, bt.add_filter_clamped(img, k)))
+    bt.plan(autotune=False)
+    bt.run()
+    torch.cuda.synchronize()
+    for img, k, out in cases:
+        kh, kw = k.shape
+        assert np.array_equal(out.cpu().numpy(),
+                              O.filter_clamped(img.cpu().numpy(), k.reshape(-1), kw, kh, np.uint8))
+    bt.close()
+
+
 def test_batch_many_small_and_one_large_image():
     rng = np.random.default_rng(14)
     imgs = [torch.from_numpy(rng.integers(0, 256, (int(h), int(w)), dtype=np.uint8)).cuda()
