@@ -1535,7 +1535,7 @@ __device__ __forceinline__ void wave_b_table(const float *taps, int lane, int m,
 // unaligned rows (every byte loaded from its clamped column). Each mode is
 // its own inlined copy of the strip loop, so the rarer modes' registers do
 // not weigh on the interior one.
-enum { kWaveInterior = 0, kWaveEdge = 1, kWaveUnaligned = 2 };
+enum { kWaveInterior = 0, kWaveEdge = 1, kWaveUnaligned = 2, kWaveInteriorVec = 3 };
 
 template <int NKB, bool SHARE_B, int MODE>
 __device__ __forceinline__ void blur_u8_wave_strip(const WaveJob &jb, const float *taps_h,
@@ -1546,7 +1546,10 @@ __device__ __forceinline__ void blur_u8_wave_strip(const WaveJob &jb, const floa
     constexpr int CIN = 16 * NKB, PI = wave_in_pitch<NKB>(), RPH = kRingPitch / 2;
     constexpr int PO = kWaveOutPitch;
     constexpr bool ALIGNED = MODE != kWaveUnaligned;
-    constexpr bool interior = MODE == kWaveInterior;
+    constexpr bool interior = MODE == kWaveInterior || MODE == kWaveInteriorVec;
+    // interior strips of images whose rows and dst are 16-aligned: 16-byte
+    // stores only (the other store forms are not compiled into this copy)
+    constexpr bool VEC = MODE == kWaveInteriorVec;
     const int L = jb.L, m = jb.m, half = L / 2;
     const int lane = threadIdx.x & 63;
     const int n = lane & 31, g = lane >> 5;
@@ -1723,7 +1726,7 @@ __device__ __forceinline__ void blur_u8_wave_strip(const WaveJob &jb, const floa
     // them are flagged. (Interior strips hold no column right of the image;
     // the V pass of such a column blurs the edge column's values, as near
     // integers as any other.)
-    const bool hclip = MODE != kWaveInterior && x0 + n >= W;
+    const bool hclip = !interior && x0 + n >= W;
     auto h_readout = [&](auto slot_c, const v16f &acc, const v16f &acl) {
         constexpr int SLOT = decltype(slot_c)::value;
         _Float16 *col = ring + n * RPH + 32 * SLOT;
@@ -1824,15 +1827,35 @@ __device__ __forceinline__ void blur_u8_wave_strip(const WaveJob &jb, const floa
         if (gy < yb1 && gx < W) {
             const uint4 v = *reinterpret_cast<const uint4 *>(out_w + yl * PO + xl);
             const int o = gy * W + gx;
-            if (vec_out && gx + 16 <= W) {
+            if constexpr (VEC) {
+                typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+                const v4u vv = {v.x, v.y, v.z, v.w};
+                __builtin_amdgcn_raw_buffer_store_b128(vv, drs, o, 0, 0);
+            } else if (vec_out && gx + 16 <= W) {
                 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
                 const v4u vv = {v.x, v.y, v.z, v.w};
                 __builtin_amdgcn_raw_buffer_store_b128(vv, drs, o, 0, 0);
             } else if (dw_out) {
+                // rows 4-aligned but not all 16-aligned: each piece as one
+                // 16-byte or two 8-byte stores where its address allows
+                // (fewer, wider writes), else four dword stores
                 const uint32_t dws[4] = {v.x, v.y, v.z, v.w};
+                const uint32_t al = (uint32_t)reinterpret_cast<uintptr_t>(dst + o) & 15u;
+                typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+                typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+                if (gx + 16 <= W && al == 0) {
+                    const v4u vv = {v.x, v.y, v.z, v.w};
+                    __builtin_amdgcn_raw_buffer_store_b128(vv, drs, o, 0, 0);
+                } else if (gx + 16 <= W && (al & 7u) == 0) {
+                    const v2u lo = {v.x, v.y}, hi = {v.z, v.w};
+                    __builtin_amdgcn_raw_buffer_store_b64(lo, drs, o, 0, 0);
+                    __builtin_amdgcn_raw_buffer_store_b64(hi, drs, o + 8, 0, 0);
+                } else {
 #pragma unroll
-                for (int q = 0; q < 4; q++)
-                    if (gx + 4 * q < W) __builtin_amdgcn_raw_buffer_store_b32(dws[q], drs, o + 4 * q, 0, 0);
+                    for (int q = 0; q < 4; q++)
+                        if (gx + 4 * q < W)
+                            __builtin_amdgcn_raw_buffer_store_b32(dws[q], drs, o + 4 * q, 0, 0);
+                }
             } else if constexpr (ALIGNED) {
                 // 4-aligned rows from a dst that is not (the register-lean
                 // byte loop keeps the aligned kernels free of spills)
@@ -1947,9 +1970,16 @@ __device__ __forceinline__ void blur_u8_wave_unit(const WaveJob &jb, int u, bool
     if (!ALIGNED)
         blur_u8_wave_strip<NKB, SHARE_B, kWaveUnaligned>(jb, taps_h, taps_v, bt_h, bt_v, in_w,
                                                          ring, out_w, im, band, strip);
-    else if (xs >= 0 && xs + CIN <= jb.W)
-        blur_u8_wave_strip<NKB, SHARE_B, kWaveInterior>(jb, taps_h, taps_v, bt_h, bt_v, in_w,
-                                                        ring, out_w, im, band, strip);
+    else if (xs >= 0 && xs + CIN <= jb.W) {
+        const bool vec = (jb.W & 15) == 0 && (jb.stride & 15) == 0 &&
+                         (reinterpret_cast<uintptr_t>(jb.dst) & 15) == 0;
+        if (vec)
+            blur_u8_wave_strip<NKB, SHARE_B, kWaveInteriorVec>(jb, taps_h, taps_v, bt_h, bt_v,
+                                                               in_w, ring, out_w, im, band, strip);
+        else
+            blur_u8_wave_strip<NKB, SHARE_B, kWaveInterior>(jb, taps_h, taps_v, bt_h, bt_v, in_w,
+                                                            ring, out_w, im, band, strip);
+    }
     else
         blur_u8_wave_strip<NKB, SHARE_B, kWaveEdge>(jb, taps_h, taps_v, bt_h, bt_v, in_w, ring,
                                                     out_w, im, band, strip);
