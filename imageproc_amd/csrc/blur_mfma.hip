@@ -1567,11 +1567,10 @@ __device__ __forceinline__ void blur_u8_wave_strip(const WaveJob &jb, const floa
         reinterpret_cast<uint4 *>(ring)[i] = make_uint4(0, 0, 0, 0);
 
     const uint32_t thr_h = 2u * (uint32_t)jb.ch, thr_v = 2u * (uint32_t)jb.cv;
-    // shared taps, aligned rows: the Toeplitz operands, from the workgroup's
-    // LDS table into registers for the whole strip (two sets, or one beside
-    // the byte-wise loads of unaligned rows, would not fit; those read them
-    // from LDS in each pass)
-    constexpr bool BREG = SHARE_B && ALIGNED;
+    // shared taps: the Toeplitz operands, from the workgroup's LDS table into
+    // registers for the whole strip (two sets would not fit; different H and
+    // V taps read them from LDS in each pass)
+    constexpr bool BREG = SHARE_B;
     v8h rbh[BREG ? NKB : 1], rbl[BREG ? NKB : 1];
     if constexpr (BREG) {
 #pragma unroll
@@ -1999,10 +1998,14 @@ __device__ __forceinline__ void blur_u8_wave_body(const WaveJob &jb) {
     blur_u8_wave_unit<NKB, SHARE_B, ALIGNED>(jb, u, u < units);
 }
 
+// Waves per SIMD: 3 (168 VGPRs) for 4-aligned rows; 2 for the others, whose
+// five-dword chunks and funnel shifts would spill at 3.
+constexpr int wave_kernel_waves(bool aligned) { return aligned ? 3 : 2; }
+
 // The job is read through a reference to the kernel argument: its taps are
 // indexed per thread, which must not copy the argument into registers.
 template <int NKB, bool SHARE_B, bool ALIGNED>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(wave_kernel_waves(ALIGNED), wave_kernel_waves(ALIGNED)))) void
 blur_u8_wave_kernel(WaveJob job) {
     blur_u8_wave_body<NKB, SHARE_B, ALIGNED>(job);
 }
@@ -2012,7 +2015,7 @@ blur_u8_wave_kernel(WaveJob job) {
 // the four waves of a workgroup always belong to one job and share its
 // tables. nwg = total units / 4.
 template <int NKB, bool SHARE_B, bool ALIGNED>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(wave_kernel_waves(ALIGNED), wave_kernel_waves(ALIGNED)))) void
 blur_u8_wave_jobs_kernel(const WaveJob *__restrict__ jobs, int njobs, int nwg) {
     const int chunk = (nwg + 7) >> 3;
     const int wg = (int)(blockIdx.x & 7) * chunk + (int)(blockIdx.x >> 3);
