@@ -774,6 +774,12 @@ iph_status separable_typed(const void *src, void *dst, uint32_t w, uint32_t h, u
         case 9: IPH_SEP(9); break;
         case 11: IPH_SEP(11); break;
         case 13: IPH_SEP(13); break;
+        case 15: IPH_SEP(15); break;
+        case 17: IPH_SEP(17); break;
+        case 19: IPH_SEP(19); break;
+        case 21: IPH_SEP(21); break;
+        case 23: IPH_SEP(23); break;
+        case 25: IPH_SEP(25); break;
         default: IPH_SEP(0); break;
         }
 #undef IPH_SEP

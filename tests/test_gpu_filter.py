@@ -104,7 +104,7 @@ def test_1d_filters_sweep_small_sizes():
                       O.separable_filter(img, k, k[::-1].copy()))
 
 
-@pytest.mark.parametrize("L", [1, 2, 5, 8, 13, 47, 48, 49, 64, 101])
+@pytest.mark.parametrize("L", [1, 2, 5, 8, 13, 15, 17, 21, 25, 47, 48, 49, 64, 101])
 def test_separable_random_lengths(rng, L):
     img = rng.integers(0, 256, (83, 150), dtype=np.uint8)
     hk = rng.uniform(-0.3, 0.6, L).astype(F)
