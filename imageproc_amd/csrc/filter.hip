@@ -126,6 +126,9 @@ constexpr int kFusedMaxTaps = 48;
 constexpr int kSepTW = 64;
 constexpr int kSepTH = 32;
 constexpr int kSepRun = 8;  // outputs per task with compile-time taps
+// output rows per sep_fused_kernel tile: 64 for 21 taps or more (the halo
+// rows then weigh less), else 32
+__host__ __device__ constexpr int sep_tile_h(int lt) { return lt >= 21 ? 64 : kSepTH; }
 
 template <typename K> struct FusedTaps {
     K h[kFusedMaxTaps];
@@ -140,15 +143,16 @@ __global__ __launch_bounds__(256) void sep_fused_kernel(const P *__restrict__ sr
                                                         size_t stride, int Lrt,
                                                         FusedTaps<K> taps) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    constexpr int TH = sep_tile_h(LT);
     const int L = LT > 0 ? LT : Lrt;
-    const int R = kSepTH + L - 1;
+    const int R = TH + L - 1;
     const int C = kSepTW + L - 1;
     P *in_s = reinterpret_cast<P *>(smem_raw);
     P *mid_s = in_s + R * C;
 
     const int half = L / 2;
     const int x0 = blockIdx.x * kSepTW;
-    const int y0 = blockIdx.y * kSepTH;
+    const int y0 = blockIdx.y * TH;
     const P *img = src + (size_t)blockIdx.z * stride;
     P *out = dst + (size_t)blockIdx.z * stride;
 
@@ -176,7 +180,7 @@ __global__ __launch_bounds__(256) void sep_fused_kernel(const P *__restrict__ sr
             }
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < (kSepTH / kSepRun) * kSepTW; i += blockDim.x) {
+        for (int i = threadIdx.x; i < (TH / kSepRun) * kSepTW; i += blockDim.x) {
             const int rb = i / kSepTW, c = i - rb * kSepTW;
             const int r0 = rb * kSepRun, gx = x0 + c;
             const P *col = mid_s + r0 * kSepTW + c;
@@ -208,7 +212,7 @@ __global__ __launch_bounds__(256) void sep_fused_kernel(const P *__restrict__ sr
     }
     __syncthreads();
 
-    for (int i = threadIdx.x; i < kSepTH * kSepTW; i += blockDim.x) {
+    for (int i = threadIdx.x; i < TH * kSepTW; i += blockDim.x) {
         int r = i / kSepTW, c = i - r * kSepTW;
         int gy = y0 + r, gx = x0 + c;
         if (gy >= H || gx >= W) continue;
@@ -762,11 +766,15 @@ iph_status separable_typed(const void *src, void *dst, uint32_t w, uint32_t h, u
         copy_taps(taps.h, hk, len);
         copy_taps(taps.v, vk, len);
         int L = (int)len;
-        size_t lds = sizeof(P) * ((size_t)(kSepTH + L - 1) * (kSepTW + L - 1) +
-                                  (size_t)(kSepTH + L - 1) * kSepTW);
-        dim3 grid((w + kSepTW - 1) / kSepTW, (h + kSepTH - 1) / kSepTH, n);
 #define IPH_SEP(LT)                                                                          \
-    sep_fused_kernel<P, K, LT><<<grid, 256, lds, s>>>(in, out, (int)w, (int)h, stride, L, taps)
+    do {                                                                                     \
+        constexpr int TH = sep_tile_h(LT);                                                   \
+        const size_t lds = sizeof(P) * ((size_t)(TH + L - 1) * (kSepTW + L - 1) +           \
+                                        (size_t)(TH + L - 1) * kSepTW);                     \
+        const dim3 grid((w + kSepTW - 1) / kSepTW, (h + TH - 1) / TH, n);                    \
+        sep_fused_kernel<P, K, LT><<<grid, 256, lds, s>>>(in, out, (int)w, (int)h, stride, L, \
+                                                          taps);                             \
+    } while (0)
         switch (L) {
         case 3: IPH_SEP(3); break;
         case 5: IPH_SEP(5); break;
